@@ -1,0 +1,159 @@
+/*
+ * tribit.h -- C ABI of the MI355X (gfx950) exact triangle-counting library
+ * libtribit_hip.so. Plain C: no C++ types, exceptions or torch types cross it.
+ *
+ * Reference interfaces replaced (sixt99/TriBIT):
+ *   read_mtx                          src/single_gpu/tribit.cu:228-269
+ *   count_triangles (preprocessing)   src/single_gpu/tribit.cu:656-763
+ *   count_triangles (counting kernel) src/single_gpu/tribit.cu:765-802
+ *   init_gpu_kernel5 / 6              src/multi_gpu/cuda/main5_multi.cu:578-587,
+ *                                     src/multi_gpu/cuda/main6_multi.cu:585-593
+ *   allocate_memory (pool warm-up)    src/single_gpu/tribit.cu:825-838
+ *
+ * Conventions.
+ *   - Every entry point returns a tribit_status (0 = success) and never exits
+ *     the process (the reference prints and exit(1)s, tribit.cu:62-65).
+ *   - Edge keys are the reference's packed form: (uint64(min) << 32) | max with
+ *     min < max (tribit.cu:258-267). Key 0xFFFFFFFFFFFFFFFF is a "no edge"
+ *     sentinel and is dropped by preprocessing.
+ *   - Host buffers passed in are borrowed for the call. The library owns all
+ *     device memory; a tribit_graph keeps its count-ready structures resident
+ *     in HBM until tribit_graph_free.
+ *   - Calls are synchronous: they return after the result is on the host.
+ *   - One context per device; use one host thread per context.
+ *   - Counts are exact uint64.
+ */
+#ifndef TRIBIT_H
+#define TRIBIT_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  TRIBIT_OK = 0,
+  TRIBIT_ERR_IO = 1,        /* cannot open or read the input file */
+  TRIBIT_ERR_FORMAT = 2,    /* malformed Matrix Market header or entry */
+  TRIBIT_ERR_SHAPE = 3,     /* non-square matrix (tribit.cu:241-245) */
+  TRIBIT_ERR_RANGE = 4,     /* vertex id outside [1, n] or key not (min<max<n) */
+  TRIBIT_ERR_NOMEM = 5,     /* device allocation failed or exceeds the cap */
+  TRIBIT_ERR_ARG = 6,       /* invalid argument */
+  TRIBIT_ERR_HIP = 7,       /* HIP runtime error */
+  TRIBIT_ERR_NODEVICE = 8,  /* no usable gfx950 device */
+  TRIBIT_ERR_TOO_LARGE = 9  /* graph exceeds a 32-bit layout limit */
+} tribit_status;
+
+typedef struct tribit_ctx tribit_ctx;
+typedef struct tribit_graph tribit_graph;
+
+/* Count-ready graph description (filled by tribit_graph_get_info). */
+typedef struct {
+  uint64_t n;              /* vertices */
+  uint64_t entries;        /* input keys (including duplicates and sentinels) */
+  uint64_t m;              /* undirected simple edges == oriented edges */
+  uint64_t wplus;          /* W+ = sum over oriented u->v of outdeg(v) */
+  uint64_t rows_active;    /* rows with outdeg >= 2 */
+  uint32_t max_outdeg;     /* max oriented out-degree */
+  uint32_t reserved0;
+  uint64_t tiles32;        /* 32-column bit tiles (0 if not built) */
+  uint64_t tiles64;        /* 64-column bit tiles (0 if not built) */
+  double ms_preprocess;    /* device time: keys in HBM -> count-ready (hipEvents) */
+  uint64_t device_bytes;   /* bytes held by the resident structures */
+} tribit_graph_info;
+
+/* Scheduler choice for tribit_count. */
+typedef enum {
+  TRIBIT_SCHED_PLAIN = 0,   /* static degree/density thresholds */
+  TRIBIT_SCHED_LEARNED = 1  /* cost-model driven (falls back to PLAIN without a model) */
+} tribit_sched;
+
+typedef struct {
+  int32_t sched;           /* tribit_sched */
+  int32_t repeats;         /* timed repetitions (>=1); the count of every repeat must agree */
+  int32_t reserved[6];
+} tribit_count_opts;
+
+typedef struct {
+  uint64_t triangles;      /* exact count */
+  double ms_count;         /* scheduler + counting kernels, median over repeats (hipEvents) */
+  double ms_sched;         /* scheduler part of ms_count (median) */
+  double ms_kernel;        /* counting kernels part of ms_count (median) */
+  uint64_t launch_blocks;  /* workgroups launched by the counting kernels */
+  uint64_t tasks;          /* work items the scheduler produced */
+  uint64_t tasks_by_variant[8];
+  int32_t repeats_agree;   /* 1 if every repeat gave the same count */
+  int32_t nvariants;       /* kernel variants (one launch each) */
+  double ms_by_variant[8]; /* device time of each variant's launch (median) */
+} tribit_count_info;
+
+/* Work assigned to each kernel variant by a scheduler, in the units of the
+ * algorithmic byte count 4*(rows+1) + 4*edges + 4*wplus. */
+typedef struct {
+  uint64_t rows[8];        /* rows (outdeg >= 2) */
+  uint64_t edges[8];       /* sum of outdeg(u) over those rows */
+  uint64_t wplus[8];       /* sum of W+(u) over those rows */
+} tribit_variant_work;
+
+/* ---- context ------------------------------------------------------------ */
+/* Selects `device`, creates a stream and sets the default memory pool to keep
+ * freed memory (release threshold = max), as init_gpu_kernel5
+ * (main5_multi.cu:578-587) and allocate_memory (tribit.cu:825-838) do. */
+int tribit_init(int device, tribit_ctx** out);
+int tribit_shutdown(tribit_ctx* ctx);
+/* Test hook: fail device allocations beyond `bytes` in total (0 = no cap). */
+int tribit_set_alloc_cap(tribit_ctx* ctx, uint64_t bytes);
+/* Peak bytes held from the memory pool since tribit_init. */
+int tribit_pool_peak(tribit_ctx* ctx, uint64_t* bytes);
+const char* tribit_status_str(int status);
+/* Build description: library version, rocPRIM and HIP versions, target. */
+const char* tribit_build_info(void);
+/* Name and compute units of the context's device. */
+int tribit_device_info(tribit_ctx* ctx, char* name, size_t name_len, int* cus, double* hbm_peak_gbs);
+
+/* ---- input ----------------------------------------------------------------- */
+/* Matrix Market coordinate reader with the semantics of read_mtx
+ * (tribit.cu:228-269): banner skipped, '%' lines skipped, square check,
+ * 1-based ids, value columns ignored, self-loops dropped, keys packed
+ * (min<<32)|max. Validates ids (reference: undefined behaviour).
+ * *keys is allocated by the library; release with tribit_free_host. */
+int tribit_read_mtx(const char* path, uint64_t** keys, uint64_t* nkeys, uint64_t* n, uint64_t* entries);
+void tribit_free_host(void* p);
+
+/* Generate R-MAT edge keys in HBM (2^scale vertices, ef<<scale samples,
+ * probabilities a,b,c, d = 1-a-b-c). Returned device buffer is owned by the
+ * context and is consumed by tribit_graph_from_device_keys. */
+int tribit_gen_rmat(tribit_ctx* ctx, uint32_t scale, uint32_t ef, double a, double b, double c, uint64_t seed,
+                    uint64_t** d_keys, uint64_t* nkeys);
+/* Erdos-Renyi G(n, M): M samples of uniform vertex pairs. */
+int tribit_gen_er(tribit_ctx* ctx, uint64_t n, uint64_t nsamples, uint64_t seed, uint64_t** d_keys, uint64_t* nkeys);
+
+/* ---- preprocessing: keys -> count-ready graph (tribit.cu:656-763) --------- */
+/* Copies host keys to HBM (untimed, tribit.cu:657-659) then preprocesses. */
+int tribit_graph_from_host_keys(tribit_ctx* ctx, const uint64_t* keys, uint64_t nkeys, uint64_t n,
+                                tribit_graph** out);
+/* Preprocesses keys already in HBM; takes ownership of d_keys (frees it). */
+int tribit_graph_from_device_keys(tribit_ctx* ctx, uint64_t* d_keys, uint64_t nkeys, uint64_t n,
+                                  tribit_graph** out);
+int tribit_graph_get_info(const tribit_graph* g, tribit_graph_info* info);
+/* Copies the oriented CSR to host buffers (row_ptr[n+1], col[m], deg[n]);
+ * any pointer may be NULL. */
+int tribit_graph_download(const tribit_graph* g, uint32_t* row_ptr, uint32_t* col, uint32_t* deg);
+int tribit_graph_free(tribit_graph* g);
+
+/* ---- counting (tribit.cu:765-802) ------------------------------------------ */
+int tribit_count(tribit_ctx* ctx, tribit_graph* g, const tribit_count_opts* opts, tribit_count_info* info);
+/* Per-variant work of the scheduler `sched` on this graph (not timed). */
+int tribit_variant_stats(tribit_ctx* ctx, tribit_graph* g, int sched, tribit_variant_work* out);
+
+/* One-shot path with the shape of count_triangles (tribit.cu:656): host keys
+ * in, exact count out; preprocessing and counting device times reported. */
+int tribit_count_keys(tribit_ctx* ctx, const uint64_t* keys, uint64_t nkeys, uint64_t n,
+                      const tribit_count_opts* opts, uint64_t* triangles, double* ms_preprocess,
+                      tribit_count_info* info);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,191 @@
+"""CPU oracle for exact triangle counting -- TEST INFRASTRUCTURE ONLY.
+
+ctypes binding of oracle/liboracle.so (plain C + OpenMP, built by
+``make -C oracle``). Only tests/, ``__graft_entry__.smoke()`` and the CPU
+baseline leg of bench.py use it, always as the checker; the HIP library never
+calls it. See oracle/oracle.h for what pins each function.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+EXE_PATH = os.path.join(HERE, "tc_oracle")
+SENTINEL = np.uint64(0xFFFFFFFFFFFFFFFF)
+
+_lib = None
+_U64 = ctypes.c_uint64
+_U64P = ctypes.POINTER(ctypes.c_uint64)
+_P = ctypes.c_void_p
+
+
+class _Graph(ctypes.Structure):
+    _fields_ = [("n", _U64), ("m", _U64), ("row_ptr", _U64P), ("col", ctypes.POINTER(ctypes.c_uint32)),
+                ("deg", ctypes.POINTER(ctypes.c_uint32))]
+
+
+class _RefMeta(ctypes.Structure):
+    _fields_ = [(f, _U64) for f in ("m", "w_l", "m_purged", "w_u", "r", "t", "blocks", "triangles")]
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} missing: run `make -C oracle`")
+        L = ctypes.CDLL(LIB_PATH)
+        L.or_read_mtx.argtypes = [ctypes.c_char_p, ctypes.POINTER(_U64P), _U64P, _U64P, _U64P]
+        L.or_gen_rmat.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_double, ctypes.c_double,
+                                  ctypes.c_double, _U64, _P]
+        L.or_gen_er.argtypes = [_U64, _U64, _U64, _P]
+        L.or_dedup.argtypes = [_P, _U64]
+        L.or_dedup.restype = _U64
+        L.or_orient_csr.argtypes = [_P, _U64, _U64, ctypes.POINTER(_Graph)]
+        L.or_graph_free.argtypes = [ctypes.POINTER(_Graph)]
+        L.or_wplus.argtypes = [ctypes.POINTER(_Graph)]
+        L.or_wplus.restype = _U64
+        L.or_tiles.argtypes = [ctypes.POINTER(_Graph), ctypes.c_int, _P, _P, _P]
+        L.or_tiles.restype = _U64
+        L.or_count.argtypes = [ctypes.POINTER(_Graph), ctypes.c_int]
+        L.or_count.restype = _U64
+        L.or_count_rows.argtypes = [ctypes.POINTER(_Graph), _U64, _U64, ctypes.c_int]
+        L.or_count_rows.restype = _U64
+        L.or_count_strided.argtypes = [ctypes.POINTER(_Graph), _U64, _U64, ctypes.c_int]
+        L.or_count_strided.restype = _U64
+        L.or_count_bruteforce.argtypes = [_U64, _P, _U64]
+        L.or_count_bruteforce.restype = _U64
+        L.or_ref_pipeline.argtypes = [_P, _U64, _U64, ctypes.POINTER(_RefMeta)]
+        L.or_mix64.argtypes = [_U64]
+        L.or_mix64.restype = _U64
+        L.or_free.argtypes = [_P]
+        _lib = L
+    return _lib
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data if a.size else None
+
+
+def read_mtx(path: str):
+    L = lib()
+    kp = _U64P()
+    nk, n, ne = _U64(), _U64(), _U64()
+    st = L.or_read_mtx(path.encode(), ctypes.byref(kp), ctypes.byref(nk), ctypes.byref(n), ctypes.byref(ne))
+    if st != 0:
+        return st, None, 0, 0
+    keys = np.ctypeslib.as_array(kp, shape=(nk.value,)).copy() if nk.value else np.zeros(0, np.uint64)
+    L.or_free(ctypes.cast(kp, _P))
+    return 0, keys, n.value, ne.value
+
+
+def gen_rmat(scale: int, ef: int, a=0.57, b=0.19, c=0.19, seed: int = 1) -> np.ndarray:
+    keys = np.empty(ef << scale, np.uint64)
+    assert lib().or_gen_rmat(scale, ef, a, b, c, seed, _ptr(keys)) == 0
+    return keys
+
+
+def gen_er(n: int, nsamples: int, seed: int) -> np.ndarray:
+    keys = np.empty(nsamples, np.uint64)
+    assert lib().or_gen_er(n, nsamples, seed, _ptr(keys)) == 0
+    return keys
+
+
+def dedup(keys: np.ndarray) -> np.ndarray:
+    k = np.array(keys, dtype=np.uint64, copy=True)
+    m = lib().or_dedup(_ptr(k), k.size)
+    return k[:m].copy()
+
+
+class Graph:
+    """Degree-oriented CSR built by the oracle from deduplicated keys."""
+
+    def __init__(self, ukeys: np.ndarray, n: int):
+        self._k = np.ascontiguousarray(ukeys, dtype=np.uint64)
+        self.g = _Graph()
+        st = lib().or_orient_csr(_ptr(self._k), self._k.size, n, ctypes.byref(self.g))
+        if st != 0:
+            raise ValueError(f"or_orient_csr status {st}")
+        self.n, self.m = self.g.n, self.g.m
+
+    @classmethod
+    def from_keys(cls, keys: np.ndarray, n: int) -> "Graph":
+        return cls(dedup(keys), n)
+
+    @classmethod
+    def from_csr(cls, row_ptr: np.ndarray, col: np.ndarray) -> "Graph":
+        """Wraps an existing oriented CSR (arrays are referenced, not copied)."""
+        self = cls.__new__(cls)
+        self._rp = np.ascontiguousarray(row_ptr, dtype=np.uint64)
+        self._col = np.ascontiguousarray(col, dtype=np.uint32)
+        self._borrowed = True
+        self.g = _Graph()
+        self.g.n = self._rp.size - 1
+        self.g.m = self._col.size
+        self.g.row_ptr = self._rp.ctypes.data_as(_U64P)
+        self.g.col = self._col.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
+        self.n, self.m = self.g.n, self.g.m
+        return self
+
+    def count_strided(self, start: int, stride: int, threads: int = 0) -> int:
+        return lib().or_count_strided(ctypes.byref(self.g), start, stride, threads)
+
+    def arrays(self):
+        rp = np.ctypeslib.as_array(self.g.row_ptr, shape=(self.n + 1,)).copy()
+        col = np.ctypeslib.as_array(self.g.col, shape=(self.m,)).copy() if self.m else np.zeros(0, np.uint32)
+        deg = np.ctypeslib.as_array(self.g.deg, shape=(self.n,)).copy() if self.n else np.zeros(0, np.uint32)
+        return rp, col, deg
+
+    def wplus(self) -> int:
+        return lib().or_wplus(ctypes.byref(self.g))
+
+    def count(self, threads: int = 0) -> int:
+        return lib().or_count(ctypes.byref(self.g), threads)
+
+    def count_rows(self, r0: int, r1: int, threads: int = 0) -> int:
+        return lib().or_count_rows(ctypes.byref(self.g), r0, r1, threads)
+
+    def tiles(self, width: int):
+        L = lib()
+        w = L.or_tiles(ctypes.byref(self.g), width, None, None, None)
+        tp = np.zeros(self.n + 1, np.uint64)
+        blk = np.zeros(max(w, 1), np.uint32)
+        mask = np.zeros(max(w, 1), np.uint64)
+        L.or_tiles(ctypes.byref(self.g), width, _ptr(tp), _ptr(blk), _ptr(mask))
+        return tp, blk[:w], mask[:w]
+
+    def __del__(self):
+        if getattr(self, "_borrowed", False):
+            return
+        try:
+            lib().or_graph_free(ctypes.byref(self.g))
+        except Exception:
+            pass
+
+
+def count_keys(keys: np.ndarray, n: int, threads: int = 0) -> int:
+    return Graph.from_keys(keys, n).count(threads)
+
+
+def count_bruteforce(n: int, keys: np.ndarray) -> int:
+    k = np.ascontiguousarray(keys, dtype=np.uint64)
+    return lib().or_count_bruteforce(n, _ptr(k), k.size)
+
+
+def ref_pipeline(ukeys: np.ndarray, n: int) -> dict:
+    k = np.ascontiguousarray(ukeys, dtype=np.uint64)
+    r = _RefMeta()
+    assert lib().or_ref_pipeline(_ptr(k), k.size, n, ctypes.byref(r)) == 0
+    return {f: getattr(r, f) for f, _ in _RefMeta._fields_}
+
+
+def mix64(z: int) -> int:
+    return lib().or_mix64(z)

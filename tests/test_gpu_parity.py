@@ -1,0 +1,215 @@
+"""GPU parity: the HIP path (through the C ABI) against the pinned CPU oracle.
+
+Integer work, so every check is bit-exact: triangle counts, and also the
+count-ready structures themselves (degrees, oriented CSR) and W+.
+Cases: the reference-held wiki-Vote fixture, closed forms, brute-forced tiny
+graphs, the networkx golden set (GPU generators must reproduce the oracle's
+key sets), every scheduler class (including rows split into id-range chunks
+and rows split by wedge count), error paths, and -- at full sizes -- the
+size-independent properties (repeat determinism, relabelling invariance,
+W+ and m against the oracle).
+"""
+import json
+import os
+import re
+import subprocess
+import sys
+from math import comb
+
+import numpy as np
+import pytest
+
+import tribit_amd
+from tests import graphs
+from tests.conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+WIKI = os.path.join(GOLDEN, "wiki-Vote.mtx")
+
+
+def note(msg):
+    print(f"[gpu-case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def gpu_count(ctx, keys, n, sched="plain"):
+    with ctx.graph_from_keys(keys, n) as g:
+        return g.count(sched=sched).triangles
+
+
+def assert_same_structure(g, og):
+    rp, col, deg = g.download()
+    orp, ocol, odeg = og.arrays()
+    assert np.array_equal(rp.astype(np.uint64), orp)
+    assert np.array_equal(col, ocol)
+    assert np.array_equal(deg, odeg)
+    inf = g.info()
+    assert inf["m"] == og.m and inf["wplus"] == og.wplus()
+
+
+def test_wiki_vote(gpu_ctx, oracle_mod):
+    note("wiki-Vote")
+    keys, n, _ = tribit_amd.read_mtx(WIKI)
+    with gpu_ctx.graph_from_keys(keys, n) as g:
+        r = g.count(repeats=3)
+        assert r.triangles == 608389 and r.repeats_agree
+        assert_same_structure(g, oracle_mod.Graph.from_keys(keys, n))
+
+
+def test_closed_forms(gpu_ctx):
+    note("closed forms")
+    for n in (3, 4, 5, 31, 64, 65, 100, 257):
+        keys, t = graphs.complete(n)
+        assert gpu_count(gpu_ctx, keys, n) == t == comb(n, 3), n
+    for k in (4, 5, 100, 1000):
+        keys, t = graphs.wheel(k)
+        assert gpu_count(gpu_ctx, keys, k + 1) == t
+    for k in (1, 7, 500):
+        keys, t = graphs.windmill(k)
+        assert gpu_count(gpu_ctx, keys, 2 * k + 1) == t
+    for n in (3, 4, 1000):
+        keys, t = graphs.cycle(n)
+        assert gpu_count(gpu_ctx, keys, n) == t
+    for parts in ((2, 3, 4), (30, 40, 50), (3, 3, 3, 3)):
+        keys, t, n = graphs.complete_multipartite(parts)
+        assert gpu_count(gpu_ctx, keys, n) == t
+    keys, _, n = graphs.random_bipartite(300, 200, 0.2, 1)
+    assert gpu_count(gpu_ctx, keys, n) == 0
+
+
+def test_bruteforce_tiny(gpu_ctx, oracle_mod):
+    note("brute force tiny")
+    for seed in range(30):
+        rng = np.random.default_rng(100 + seed)
+        n = int(rng.integers(2, 80))
+        keys = graphs.random_graph(n, float(rng.uniform(0.05, 0.95)), seed)
+        assert gpu_count(gpu_ctx, keys, n) == oracle_mod.count_bruteforce(n, keys), seed
+
+
+def test_golden_set(gpu_ctx, oracle_mod):
+    with open(os.path.join(GOLDEN, "golden.json")) as f:
+        gold = json.load(f)
+    for r in gold:
+        note(r["name"])
+        spec = r["spec"]
+        if spec.get("gen") == "rmat":
+            g = gpu_ctx.gen_rmat(spec["scale"], spec["ef"], spec["a"], spec["b"], spec["c"], spec["seed"])
+            okeys = oracle_mod.gen_rmat(spec["scale"], spec["ef"], spec["a"], spec["b"], spec["c"], spec["seed"])
+        elif spec.get("gen") == "er":
+            g = gpu_ctx.gen_er(spec["n"], spec["samples"], spec["seed"])
+            okeys = oracle_mod.gen_er(spec["n"], spec["samples"], spec["seed"])
+        else:
+            keys, n, _ = tribit_amd.read_mtx(os.path.join(GOLDEN, spec["file"]))
+            g = gpu_ctx.graph_from_keys(keys, n)
+            okeys = keys
+        with g:
+            assert g.count().triangles == r["triangles"], r["name"]
+            assert_same_structure(g, oracle_mod.Graph.from_keys(okeys, r["n"]))
+
+
+def test_block_classes_and_chunks(gpu_ctx, oracle_mod):
+    """Rows with outdeg > 256 (block tasks) and > 2048 (id-range chunks)."""
+    note("K_2100 + K_600 (block tasks, 2 chunks)")
+    k1, t1 = graphs.complete(2100)
+    k2, t2 = graphs.complete(600, offset=2100)
+    keys = np.concatenate([k1, k2])
+    with gpu_ctx.graph_from_keys(keys, 2700) as g:
+        r = g.count()
+        assert r.triangles == t1 + t2
+        assert r.tasks_by_variant[3] > 0 and r.tasks_by_variant[2] > 0
+    note("K_4500 (3 chunks)")
+    keys, t = graphs.complete(4500)
+    assert gpu_count(gpu_ctx, keys, 4500) == t
+    note("dense random n=5000 p=0.5")
+    keys = graphs.random_graph(5000, 0.5, 3)
+    with gpu_ctx.graph_from_keys(keys, 5000) as g:
+        assert g.count().triangles == oracle_mod.count_keys(keys, 5000)
+
+
+def test_split_rows_rmat16(gpu_ctx, oracle_mod):
+    note("rmat s16 ef16 seed 4 (wedge-count splits)")
+    with gpu_ctx.gen_rmat(16, 16, seed=4) as g:
+        r = g.count(repeats=2)
+        og = oracle_mod.Graph.from_keys(oracle_mod.gen_rmat(16, 16, seed=4), 1 << 16)
+        assert r.triangles == og.count() and r.repeats_agree
+        assert r.tasks > g.info()["rows_active"]  # some rows were split
+
+
+def test_edge_cases(gpu_ctx):
+    note("edge cases")
+    assert gpu_count(gpu_ctx, np.zeros(0, np.uint64), 10) == 0
+    assert gpu_count(gpu_ctx, np.array([int(graphs.pack(0, 1))], np.uint64), 2) == 0
+    assert gpu_count(gpu_ctx, np.full(7, 0xFFFFFFFFFFFFFFFF, np.uint64), 5) == 0
+    tri = np.array([int(graphs.pack(0, 1)), int(graphs.pack(1, 2)), int(graphs.pack(0, 2))], np.uint64)
+    assert gpu_count(gpu_ctx, np.concatenate([tri, tri, np.full(3, 0xFFFFFFFFFFFFFFFF, np.uint64)]), 3) == 1
+    with pytest.raises(tribit_amd.TribitError) as ei:
+        gpu_ctx.graph_from_keys(np.array([(2 << 32) | 1], np.uint64), 3)  # min > max
+    assert ei.value.status == 4
+    with pytest.raises(tribit_amd.TribitError) as ei:
+        gpu_ctx.graph_from_keys(np.array([(0 << 32) | 3], np.uint64), 3)  # max >= n
+    assert ei.value.status == 4
+
+
+def test_alloc_cap(gpu_ctx):
+    note("allocation cap")
+    keys, _ = graphs.complete(200)
+    gpu_ctx.set_alloc_cap(4096)
+    try:
+        with pytest.raises(tribit_amd.TribitError) as ei:
+            gpu_ctx.graph_from_keys(keys, 200)
+        assert ei.value.status == 5
+    finally:
+        gpu_ctx.set_alloc_cap(0)
+    assert gpu_count(gpu_ctx, keys, 200) == comb(200, 3)
+
+
+def test_rmat20_exact(gpu_ctx, oracle_mod):
+    note("rmat s20 ef16 seed 5 vs oracle")
+    with gpu_ctx.gen_rmat(20, 16, seed=5) as g:
+        tri = g.count().triangles
+        og = oracle_mod.Graph.from_keys(oracle_mod.gen_rmat(20, 16, seed=5), 1 << 20)
+        inf = g.info()
+        assert (inf["m"], inf["wplus"]) == (og.m, og.wplus())
+        assert tri == og.count()
+
+
+def test_rmat22_properties(gpu_ctx, oracle_mod):
+    """Configuration 2 (R-MAT s22 ef16 seed 2): determinism, relabelling invariance, m and W+ vs oracle."""
+    note("rmat s22 ef16 seed 2 properties")
+    with gpu_ctx.gen_rmat(22, 16, seed=2) as g:
+        r = g.count(repeats=3)
+        assert r.repeats_agree
+        inf = g.info()
+    okeys = oracle_mod.gen_rmat(22, 16, seed=2)
+    og = oracle_mod.Graph.from_keys(okeys, 1 << 22)
+    assert (inf["m"], inf["wplus"]) == (og.m, og.wplus())
+    del og
+    perm = graphs.permute(okeys, 1 << 22, 7)
+    del okeys
+    with gpu_ctx.graph_from_keys(perm, 1 << 22) as g2:
+        assert g2.count().triangles == r.triangles
+
+
+def test_cli_wiki_vote_stdout_contract():
+    """`tribit -i wiki-Vote.mtx` output parsed by the reference harness's regexes (run_single_gpu.py:78-93)."""
+    note("cli wiki-Vote")
+    out = subprocess.run([tribit_amd.CLI_PATH, "-i", WIKI], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    fields = {
+        "n_blocks": r"Blocks of threads\s*:\s*(\d+)",
+        "preprocessing_time": r"Preprocessing \(ms\)\s*:\s*([\d.]+)",
+        "kernel_time": r"Kernel \(ms\)\s*:\s*([\d.]+)",
+        "triangles": r"Triangles\s*:\s*(\d+)",
+    }
+    vals = {k: re.search(p, out.stdout).group(1) for k, p in fields.items()}
+    assert vals["triangles"] == "608389"
+    assert out.stdout.count("Triangle Counting Summary") == 1
+    assert "Reading data from " + WIKI in out.stdout
+
+
+def test_cli_generated_config1():
+    note("cli rmat s13 ef12 seed 1")
+    out = subprocess.run([tribit_amd.CLI_PATH, "--rmat", "13,12,0.57,0.19,0.19,1"], capture_output=True, text=True,
+                         timeout=120)
+    assert out.returncode == 0, out.stderr
+    assert re.search(r"Triangles\s*:\s*(\d+)", out.stdout).group(1) == "689356"

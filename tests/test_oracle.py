@@ -1,0 +1,206 @@
+"""Pins the CPU oracle (oracle/) before it is trusted as the checker.
+
+Checks, each chosen so that a plausible slip fails one of them:
+  - the reference-held fixture wiki-Vote.mtx: SNAP's 608,389 triangles, its
+    shape, and the reference pipeline's own metadata (tribit.cu:656-786)
+    derived independently when the reference was surveyed;
+  - closed forms (K_n, wheels, windmills, cycles, multipartite, bipartite);
+  - O(n^3) brute force on tiny random graphs;
+  - networkx golden counts (tests/golden/golden.json);
+  - invariances (relabelling, duplicates, self-loops, reversed direction);
+  - structural invariants of the orientation and of the bit tiles.
+"""
+import json
+import os
+from math import comb
+
+import numpy as np
+import pytest
+
+from tests import graphs
+from tests.conftest import GOLDEN
+
+WIKI = os.path.join(GOLDEN, "wiki-Vote.mtx")
+
+
+def golden():
+    with open(os.path.join(GOLDEN, "golden.json")) as f:
+        return json.load(f)
+
+
+def test_wiki_vote_known_answer(oracle_mod):
+    st, keys, n, ne = oracle_mod.read_mtx(WIKI)
+    assert st == 0
+    assert (n, ne) == (8297, 103689)
+    g = oracle_mod.Graph.from_keys(keys, n)
+    assert g.m == 100762
+    assert g.count() == 608389
+
+
+def test_wiki_vote_reference_metadata(oracle_mod):
+    """Restatement of tribit.cu:656-786 reproduces the surveyed wiki-Vote figures."""
+    st, keys, n, _ = oracle_mod.read_mtx(WIKI)
+    r = oracle_mod.ref_pipeline(oracle_mod.dedup(keys), n)
+    assert r == {"m": 100762, "w_l": 70956, "m_purged": 99318, "w_u": 54880, "r": 2964, "t": 113795,
+                 "blocks": 28449, "triangles": 608389}
+
+
+@pytest.mark.parametrize("n", [3, 4, 5, 10, 31, 64, 100])
+def test_complete_graph(oracle_mod, n):
+    keys, t = graphs.complete(n)
+    assert oracle_mod.count_keys(keys, n) == t == comb(n, 3)
+    assert oracle_mod.ref_pipeline(oracle_mod.dedup(keys), n)["triangles"] == t
+
+
+@pytest.mark.parametrize("k", [3, 4, 5, 17, 100])
+def test_wheel(oracle_mod, k):
+    keys, t = graphs.wheel(k)
+    assert oracle_mod.count_keys(keys, k + 1) == t
+
+
+@pytest.mark.parametrize("k", [1, 2, 7, 50])
+def test_windmill(oracle_mod, k):
+    keys, t = graphs.windmill(k)
+    assert oracle_mod.count_keys(keys, 2 * k + 1) == t
+
+
+@pytest.mark.parametrize("n", [3, 4, 5, 1000])
+def test_cycle(oracle_mod, n):
+    keys, t = graphs.cycle(n)
+    assert oracle_mod.count_keys(keys, n) == t
+
+
+@pytest.mark.parametrize("parts", [(1, 1, 1), (2, 3, 4), (5, 6, 7), (3, 3, 3, 3), (10, 1, 20)])
+def test_complete_multipartite(oracle_mod, parts):
+    keys, t, n = graphs.complete_multipartite(parts)
+    assert oracle_mod.count_keys(keys, n) == t
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_bipartite_has_no_triangles(oracle_mod, seed):
+    keys, t, n = graphs.random_bipartite(40, 60, 0.3, seed)
+    assert oracle_mod.count_keys(keys, n) == 0
+
+
+@pytest.mark.parametrize("seed", range(20))
+def test_bruteforce_tiny(oracle_mod, seed):
+    rng = np.random.default_rng(seed)
+    n = int(rng.integers(2, 60))
+    keys = graphs.random_graph(n, float(rng.uniform(0.05, 0.9)), seed)
+    bf = oracle_mod.count_bruteforce(n, keys)
+    assert oracle_mod.count_keys(keys, n) == bf
+    assert oracle_mod.ref_pipeline(oracle_mod.dedup(keys), n)["triangles"] == bf
+
+
+def test_golden_networkx(oracle_mod):
+    for r in golden():
+        spec = r["spec"]
+        if spec.get("gen") == "rmat":
+            keys = oracle_mod.gen_rmat(spec["scale"], spec["ef"], spec["a"], spec["b"], spec["c"], spec["seed"])
+        elif spec.get("gen") == "er":
+            keys = oracle_mod.gen_er(spec["n"], spec["samples"], spec["seed"])
+        else:
+            st, keys, n, _ = oracle_mod.read_mtx(os.path.join(GOLDEN, spec["file"]))
+            assert st == 0
+        g = oracle_mod.Graph.from_keys(keys, r["n"])
+        assert (g.m, g.wplus(), g.count()) == (r["m"], r["wplus"], r["triangles"]), r["name"]
+
+
+def test_golden_generator_hashes(oracle_mod):
+    import hashlib
+
+    for r in golden():
+        spec = r["spec"]
+        if spec.get("gen") == "rmat":
+            keys = oracle_mod.gen_rmat(spec["scale"], spec["ef"], spec["a"], spec["b"], spec["c"], spec["seed"])
+        elif spec.get("gen") == "er":
+            keys = oracle_mod.gen_er(spec["n"], spec["samples"], spec["seed"])
+        else:
+            continue
+        assert hashlib.sha256(oracle_mod.dedup(keys).tobytes()).hexdigest() == r["keys_sha256"], r["name"]
+
+
+def test_invariances(oracle_mod):
+    keys = oracle_mod.gen_rmat(11, 8, seed=5)
+    n = 1 << 11
+    base = oracle_mod.count_keys(keys, n)
+    assert base > 0
+    # relabelling
+    assert oracle_mod.count_keys(graphs.permute(keys, n, 1), n) == base
+    # duplicated entries and shuffled order
+    rng = np.random.default_rng(0)
+    dup = np.concatenate([keys, keys[rng.permutation(keys.size)[: keys.size // 2]]])
+    assert oracle_mod.count_keys(dup[rng.permutation(dup.size)], n) == base
+    # sentinels (dropped self-loops) anywhere
+    sent = np.concatenate([keys, np.full(100, 0xFFFFFFFFFFFFFFFF, np.uint64)])
+    assert oracle_mod.count_keys(sent[rng.permutation(sent.size)], n) == base
+
+
+def test_orientation_invariants(oracle_mod):
+    keys = oracle_mod.gen_rmat(12, 8, seed=2)
+    n = 1 << 12
+    u = oracle_mod.dedup(keys)
+    g = oracle_mod.Graph(u, n)
+    rp, col, deg = g.arrays()
+    assert rp[0] == 0 and rp[-1] == u.size
+    src = np.repeat(np.arange(n, dtype=np.uint64), np.diff(rp).astype(np.int64))
+    dst = col.astype(np.uint64)
+    # rank order (deg, id) increases along every arc
+    assert np.all((deg[src] < deg[dst]) | ((deg[src] == deg[dst]) & (src < dst)))
+    # every undirected edge exactly once
+    back = graphs.pack(src, dst)
+    assert np.array_equal(np.sort(back), u)
+    # rows sorted ascending
+    for r in range(0, n, 97):
+        row = col[rp[r]:rp[r + 1]]
+        assert np.all(np.diff(row.astype(np.int64)) > 0)
+    # degrees are the undirected degrees
+    a = (u >> np.uint64(32)).astype(np.int64)
+    b = (u & np.uint64(0xFFFFFFFF)).astype(np.int64)
+    assert np.array_equal(deg, np.bincount(np.concatenate([a, b]), minlength=n).astype(np.uint32))
+
+
+@pytest.mark.parametrize("width", [32, 64])
+def test_tiles_decode(oracle_mod, width):
+    keys = oracle_mod.gen_rmat(11, 12, seed=3)
+    n = 1 << 11
+    g = oracle_mod.Graph.from_keys(keys, n)
+    rp, col, _ = g.arrays()
+    tp, blk, mask = g.tiles(width)
+    pc = np.array([bin(int(x)).count("1") for x in mask])
+    assert pc.sum() == g.m
+    for u in range(0, n, 13):
+        dec = []
+        for k in range(int(tp[u]), int(tp[u + 1])):
+            for bit in range(width):
+                if (int(mask[k]) >> bit) & 1:
+                    dec.append(int(blk[k]) * width + bit)
+        assert dec == col[rp[u]:rp[u + 1]].tolist()
+
+
+def test_rmat_generator_distribution(oracle_mod):
+    """Top-level quadrant frequencies follow (a, b, c, d) within 4 sigma."""
+    s, ef = 14, 8
+    keys = oracle_mod.gen_rmat(s, ef, seed=9)
+    assert keys.size == ef << s
+    ok = keys != np.uint64(0xFFFFFFFFFFFFFFFF)
+    lo = (keys[ok] >> np.uint64(32)).astype(np.int64)
+    hi = (keys[ok] & np.uint64(0xFFFFFFFF)).astype(np.int64)
+    half = 1 << (s - 1)
+    # after min/max packing the quadrants (0,1) and (1,0) merge; (0,0): both < half
+    p00 = np.mean((lo < half) & (hi < half))
+    p11 = np.mean((lo >= half) & (hi >= half))
+    N = ok.sum()
+    # self-loops (dropped) only remove same-quadrant pairs; compare loosely
+    assert abs(p00 - 0.57 / (1 - 0.0)) < 0.02 + 4 * np.sqrt(0.25 / N)
+    assert abs(p11 - 0.05) < 0.01 + 4 * np.sqrt(0.25 / N)
+    # determinism
+    assert np.array_equal(keys, oracle_mod.gen_rmat(s, ef, seed=9))
+    assert not np.array_equal(keys, oracle_mod.gen_rmat(s, ef, seed=10))
+
+
+def test_mix64_known_values(oracle_mod):
+    """SplitMix64 reference outputs (seed 0 stream: first outputs of splitmix64 with state 0)."""
+    # splitmix64 with state s returns mix64(s) where mix64 adds the golden gamma first.
+    assert oracle_mod.mix64(0) == 0xE220A8397B1DCDAF
+    assert oracle_mod.mix64(0x9E3779B97F4A7C15) == 0x6E789E6AA1B965F4

@@ -1,0 +1,280 @@
+"""tribit_amd -- ctypes binding of libtribit_hip.so, exact triangle counting on MI355X.
+
+The library (tribit_amd/libtribit_hip.so, built by ``python -m tribit_amd.build``)
+holds every GPU kernel; this module only marshals arguments across the C ABI
+declared in include/tribit.h. There is no CPU fallback: if the library is
+missing, ``load_library`` raises.
+
+Reference surface mirrored (sixt99/TriBIT): ``tribit -i <graph.mtx>``
+(src/single_gpu/tribit.cu:840-866) is the ``tribit`` executable next to this
+file; ``read_mtx`` (tribit.cu:228-269) is :func:`read_mtx`; ``count_triangles``
+(tribit.cu:656-823) is :meth:`Context.graph_from_keys` + :meth:`Graph.count`.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libtribit_hip.so")
+CLI_PATH = os.path.join(_HERE, "tribit")
+HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "tribit.h")
+
+STATUS = {
+    0: "OK", 1: "ERR_IO", 2: "ERR_FORMAT", 3: "ERR_SHAPE", 4: "ERR_RANGE", 5: "ERR_NOMEM",
+    6: "ERR_ARG", 7: "ERR_HIP", 8: "ERR_NODEVICE", 9: "ERR_TOO_LARGE",
+}
+SCHED = {"plain": 0, "learned": 1}
+
+
+class TribitError(RuntimeError):
+    def __init__(self, status: int, what: str):
+        self.status = status
+        self.name = STATUS.get(status, f"status {status}")
+        super().__init__(f"{what}: {self.name}")
+
+
+class GraphInfo(ctypes.Structure):
+    _fields_ = [
+        ("n", ctypes.c_uint64), ("entries", ctypes.c_uint64), ("m", ctypes.c_uint64),
+        ("wplus", ctypes.c_uint64), ("rows_active", ctypes.c_uint64), ("max_outdeg", ctypes.c_uint32),
+        ("reserved0", ctypes.c_uint32), ("tiles32", ctypes.c_uint64), ("tiles64", ctypes.c_uint64),
+        ("ms_preprocess", ctypes.c_double), ("device_bytes", ctypes.c_uint64),
+    ]
+
+
+class CountOpts(ctypes.Structure):
+    _fields_ = [("sched", ctypes.c_int32), ("repeats", ctypes.c_int32), ("reserved", ctypes.c_int32 * 6)]
+
+
+class CountInfo(ctypes.Structure):
+    _fields_ = [
+        ("triangles", ctypes.c_uint64), ("ms_count", ctypes.c_double), ("ms_sched", ctypes.c_double),
+        ("ms_kernel", ctypes.c_double), ("launch_blocks", ctypes.c_uint64), ("tasks", ctypes.c_uint64),
+        ("tasks_by_variant", ctypes.c_uint64 * 8), ("repeats_agree", ctypes.c_int32), ("nvariants", ctypes.c_int32),
+        ("ms_by_variant", ctypes.c_double * 8),
+    ]
+
+
+class VariantStats(ctypes.Structure):
+    _fields_ = [("rows", ctypes.c_uint64 * 8), ("edges", ctypes.c_uint64 * 8), ("wplus", ctypes.c_uint64 * 8)]
+
+
+_lib = None
+_P = ctypes.c_void_p
+_U64 = ctypes.c_uint64
+_U64P = ctypes.POINTER(ctypes.c_uint64)
+
+
+def header_symbols() -> list[str]:
+    """Function names declared in include/tribit.h."""
+    src = open(HEADER_PATH).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(tribit_[a-z0-9_]+)\s*\(", src)))
+
+
+def load_library(path: str | None = None) -> ctypes.CDLL:
+    """Loads libtribit_hip.so (raises if it was not built)."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise RuntimeError(f"{p} is missing: build it with `python -m tribit_amd.build` (no CPU fallback exists)")
+    lib = ctypes.CDLL(p)
+    sig = {
+        "tribit_init": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_P)]),
+        "tribit_shutdown": (ctypes.c_int, [_P]),
+        "tribit_set_alloc_cap": (ctypes.c_int, [_P, _U64]),
+        "tribit_pool_peak": (ctypes.c_int, [_P, _U64P]),
+        "tribit_status_str": (ctypes.c_char_p, [ctypes.c_int]),
+        "tribit_build_info": (ctypes.c_char_p, []),
+        "tribit_device_info": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_int),
+                                              ctypes.POINTER(ctypes.c_double)]),
+        "tribit_read_mtx": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(_U64P), _U64P, _U64P, _U64P]),
+        "tribit_free_host": (None, [_P]),
+        "tribit_gen_rmat": (ctypes.c_int, [_P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_double, ctypes.c_double,
+                                           ctypes.c_double, _U64, ctypes.POINTER(_P), _U64P]),
+        "tribit_gen_er": (ctypes.c_int, [_P, _U64, _U64, _U64, ctypes.POINTER(_P), _U64P]),
+        "tribit_graph_from_host_keys": (ctypes.c_int, [_P, _P, _U64, _U64, ctypes.POINTER(_P)]),
+        "tribit_graph_from_device_keys": (ctypes.c_int, [_P, _P, _U64, _U64, ctypes.POINTER(_P)]),
+        "tribit_graph_get_info": (ctypes.c_int, [_P, ctypes.POINTER(GraphInfo)]),
+        "tribit_graph_download": (ctypes.c_int, [_P, _P, _P, _P]),
+        "tribit_graph_free": (ctypes.c_int, [_P]),
+        "tribit_count": (ctypes.c_int, [_P, _P, ctypes.POINTER(CountOpts), ctypes.POINTER(CountInfo)]),
+        "tribit_variant_stats": (ctypes.c_int, [_P, _P, ctypes.c_int, ctypes.POINTER(VariantStats)]),
+        "tribit_count_keys": (ctypes.c_int, [_P, _P, _U64, _U64, ctypes.POINTER(CountOpts), _U64P,
+                                             ctypes.POINTER(ctypes.c_double), ctypes.POINTER(CountInfo)]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def _check(st: int, what: str) -> None:
+    if st != 0:
+        raise TribitError(st, what)
+
+
+def build_info() -> str:
+    return load_library().tribit_build_info().decode()
+
+
+def read_mtx(path: str):
+    """Matrix Market -> (keys uint64[(min<<32)|max], n, entries), semantics of tribit.cu:228-269."""
+    lib = load_library()
+    kp = _U64P()
+    nk, n, ne = _U64(), _U64(), _U64()
+    _check(lib.tribit_read_mtx(path.encode(), ctypes.byref(kp), ctypes.byref(nk), ctypes.byref(n),
+                               ctypes.byref(ne)), f"read_mtx({path})")
+    try:
+        keys = np.ctypeslib.as_array(kp, shape=(nk.value,)).copy() if nk.value else np.zeros(0, np.uint64)
+    finally:
+        lib.tribit_free_host(ctypes.cast(kp, _P))
+    return keys, n.value, ne.value
+
+
+@dataclass
+class CountResult:
+    triangles: int
+    ms_count: float
+    ms_sched: float
+    ms_kernel: float
+    launch_blocks: int
+    tasks: int
+    tasks_by_variant: list
+    repeats_agree: bool
+    ms_by_variant: list
+
+
+class Graph:
+    """Count-ready graph resident in HBM (oriented CSR + per-row wedge counts)."""
+
+    def __init__(self, ctx: "Context", handle: int):
+        self.ctx = ctx
+        self.h = _P(handle)
+
+    def info(self) -> dict:
+        gi = GraphInfo()
+        _check(self.ctx.lib.tribit_graph_get_info(self.h, ctypes.byref(gi)), "graph_get_info")
+        return {f: getattr(gi, f) for f, _ in GraphInfo._fields_ if not f.startswith("reserved")}
+
+    def download(self):
+        """(row_ptr uint32[n+1], col uint32[m], deg uint32[n]) of the oriented CSR."""
+        inf = self.info()
+        n, m = inf["n"], inf["m"]
+        rp = np.zeros(n + 1, np.uint32)
+        col = np.zeros(max(m, 1), np.uint32)
+        deg = np.zeros(max(n, 1), np.uint32)
+        _check(self.ctx.lib.tribit_graph_download(self.h, rp.ctypes.data, col.ctypes.data, deg.ctypes.data),
+               "graph_download")
+        return rp, col[:m], deg[:n]
+
+    def count(self, sched: str = "plain", repeats: int = 1) -> CountResult:
+        o = CountOpts()
+        o.sched = SCHED[sched]
+        o.repeats = repeats
+        ci = CountInfo()
+        _check(self.ctx.lib.tribit_count(self.ctx.h, self.h, ctypes.byref(o), ctypes.byref(ci)), "count")
+        nv = ci.nvariants
+        return CountResult(ci.triangles, ci.ms_count, ci.ms_sched, ci.ms_kernel, ci.launch_blocks, ci.tasks,
+                           list(ci.tasks_by_variant)[:max(nv, 4)], bool(ci.repeats_agree),
+                           list(ci.ms_by_variant)[:nv])
+
+    def variant_stats(self, sched: str = "plain") -> dict:
+        vs = VariantStats()
+        _check(self.ctx.lib.tribit_variant_stats(self.ctx.h, self.h, SCHED[sched], ctypes.byref(vs)), "variant_stats")
+        return {"rows": list(vs.rows), "edges": list(vs.edges), "wplus": list(vs.wplus)}
+
+    def free(self) -> None:
+        if self.h:
+            self.ctx.lib.tribit_graph_free(self.h)
+            self.h = _P()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.free()
+
+
+class Context:
+    """One device context (stream + pool), as init_gpu_kernel5 (main5_multi.cu:578-587)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load_library()
+        h = _P()
+        _check(self.lib.tribit_init(device, ctypes.byref(h)), f"init(device={device})")
+        self.h = h
+
+    def close(self) -> None:
+        if self.h:
+            self.lib.tribit_shutdown(self.h)
+            self.h = _P()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def device_info(self) -> dict:
+        buf = ctypes.create_string_buffer(256)
+        cus = ctypes.c_int()
+        peak = ctypes.c_double()
+        _check(self.lib.tribit_device_info(self.h, buf, 256, ctypes.byref(cus), ctypes.byref(peak)), "device_info")
+        return {"name": buf.value.decode(), "cus": cus.value, "hbm_peak_gbs": peak.value}
+
+    def set_alloc_cap(self, nbytes: int) -> None:
+        _check(self.lib.tribit_set_alloc_cap(self.h, nbytes), "set_alloc_cap")
+
+    def pool_peak(self) -> int:
+        v = _U64()
+        _check(self.lib.tribit_pool_peak(self.h, ctypes.byref(v)), "pool_peak")
+        return v.value
+
+    def graph_from_keys(self, keys: np.ndarray, n: int) -> Graph:
+        k = np.ascontiguousarray(keys, dtype=np.uint64)
+        g = _P()
+        _check(self.lib.tribit_graph_from_host_keys(self.h, k.ctypes.data if k.size else None, k.size, n,
+                                                    ctypes.byref(g)), "graph_from_host_keys")
+        return Graph(self, g.value)
+
+    def gen_rmat(self, scale: int, ef: int, a=0.57, b=0.19, c=0.19, seed: int = 1) -> Graph:
+        d = _P()
+        nk = _U64()
+        _check(self.lib.tribit_gen_rmat(self.h, scale, ef, a, b, c, seed, ctypes.byref(d), ctypes.byref(nk)),
+               "gen_rmat")
+        g = _P()
+        _check(self.lib.tribit_graph_from_device_keys(self.h, d, nk.value, 1 << scale, ctypes.byref(g)),
+               "graph_from_device_keys")
+        return Graph(self, g.value)
+
+    def gen_er(self, n: int, nsamples: int, seed: int) -> Graph:
+        d = _P()
+        nk = _U64()
+        _check(self.lib.tribit_gen_er(self.h, n, nsamples, seed, ctypes.byref(d), ctypes.byref(nk)), "gen_er")
+        g = _P()
+        _check(self.lib.tribit_graph_from_device_keys(self.h, d, nk.value, n, ctypes.byref(g)),
+               "graph_from_device_keys")
+        return Graph(self, g.value)
+
+    def count_keys(self, keys: np.ndarray, n: int, sched: str = "plain") -> tuple[int, float]:
+        k = np.ascontiguousarray(keys, dtype=np.uint64)
+        o = CountOpts()
+        o.sched = SCHED[sched]
+        o.repeats = 1
+        tri = _U64()
+        msp = ctypes.c_double()
+        ci = CountInfo()
+        _check(self.lib.tribit_count_keys(self.h, k.ctypes.data if k.size else None, k.size, n, ctypes.byref(o),
+                                          ctypes.byref(tri), ctypes.byref(msp), ctypes.byref(ci)), "count_keys")
+        return tri.value, msp.value

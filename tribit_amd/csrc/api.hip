@@ -1,0 +1,304 @@
+// api.hip -- extern "C" entry points of libtribit_hip.so (declared in
+// include/tribit.h). Every entry point catches internal errors and returns a
+// tribit_status; nothing throws across the ABI and nothing exits the process.
+#include <cstdlib>
+#include <cstring>
+#include <new>
+
+#include <rocprim/rocprim_version.hpp>
+
+#include "common.hpp"
+
+namespace {
+
+int guard_status(tribit_ctx* ctx, const tb::Error& e) {
+  if (ctx) ctx->last_error = e.msg;
+  if (getenv("TRIBIT_VERBOSE")) fprintf(stderr, "tribit: %s\n", e.msg.c_str());
+  return e.status;
+}
+
+template <typename F>
+int guarded(tribit_ctx* ctx, F&& f) {
+  try {
+    f();
+    return TRIBIT_OK;
+  } catch (const tb::Error& e) {
+    return guard_status(ctx, e);
+  } catch (const std::bad_alloc&) {
+    return TRIBIT_ERR_NOMEM;
+  } catch (...) {
+    return TRIBIT_ERR_HIP;
+  }
+}
+
+void free_graph_memory(tribit_graph* g) {
+  if (!g || !g->ctx) return;
+  g->ctx->free(g->row_ptr);
+  g->ctx->free(g->col);
+  g->ctx->free(g->deg);
+  g->ctx->free(g->wrow);
+  g->row_ptr = g->col = g->deg = nullptr;
+  g->wrow = nullptr;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* tribit_status_str(int s) {
+  switch (s) {
+    case TRIBIT_OK: return "ok";
+    case TRIBIT_ERR_IO: return "cannot open or read input";
+    case TRIBIT_ERR_FORMAT: return "malformed Matrix Market input";
+    case TRIBIT_ERR_SHAPE: return "matrix is not square";
+    case TRIBIT_ERR_RANGE: return "vertex id out of range";
+    case TRIBIT_ERR_NOMEM: return "device memory exhausted";
+    case TRIBIT_ERR_ARG: return "invalid argument";
+    case TRIBIT_ERR_HIP: return "HIP runtime error";
+    case TRIBIT_ERR_NODEVICE: return "no usable device";
+    case TRIBIT_ERR_TOO_LARGE: return "graph exceeds a 32-bit layout limit";
+    default: return "unknown status";
+  }
+}
+
+const char* tribit_build_info(void) {
+  static char buf[160];
+  snprintf(buf, sizeof(buf), "tribit-mi355x 0.1; rocPRIM %d.%d.%d; HIP %d.%d; gfx950", ROCPRIM_VERSION / 100000,
+           (ROCPRIM_VERSION / 100) % 1000, ROCPRIM_VERSION % 100, HIP_VERSION_MAJOR, HIP_VERSION_MINOR);
+  return buf;
+}
+
+int tribit_init(int device, tribit_ctx** out) {
+  if (!out) return TRIBIT_ERR_ARG;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+    (void)hipGetLastError();
+    return TRIBIT_ERR_NODEVICE;
+  }
+  if (device < 0 || device >= ndev) return TRIBIT_ERR_ARG;
+  tribit_ctx* ctx = new (std::nothrow) tribit_ctx();
+  if (!ctx) return TRIBIT_ERR_NOMEM;
+  const int st = guarded(ctx, [&] {
+    ctx->device = device;
+    TB_HIP(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    TB_HIP(hipGetDeviceProperties(&prop, device));
+    if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+      tb::fail(TRIBIT_ERR_NODEVICE, std::string("library built for gfx950, device is ") + prop.gcnArchName);
+    ctx->cus = prop.multiProcessorCount;
+    ctx->lds_per_block = (int)prop.sharedMemPerBlock;
+    snprintf(ctx->name, sizeof(ctx->name), "%s (%s)", prop.name, prop.gcnArchName);
+    TB_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    TB_HIP(hipDeviceGetDefaultMemPool(&ctx->pool, device));
+    uint64_t thr = UINT64_MAX;  // keep freed blocks in the pool (tribit.cu:829-830)
+    TB_HIP(hipMemPoolSetAttribute(ctx->pool, hipMemPoolAttrReleaseThreshold, &thr));
+    for (auto& e : ctx->ev) TB_HIP(hipEventCreate(&e));
+  });
+  if (st != TRIBIT_OK) {
+    delete ctx;
+    return st;
+  }
+  *out = ctx;
+  return TRIBIT_OK;
+}
+
+int tribit_shutdown(tribit_ctx* ctx) {
+  if (!ctx) return TRIBIT_ERR_ARG;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  for (auto& e : ctx->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return TRIBIT_OK;
+}
+
+int tribit_set_alloc_cap(tribit_ctx* ctx, uint64_t bytes) {
+  if (!ctx) return TRIBIT_ERR_ARG;
+  ctx->cap = bytes;
+  return TRIBIT_OK;
+}
+
+int tribit_pool_peak(tribit_ctx* ctx, uint64_t* bytes) {
+  if (!ctx || !bytes) return TRIBIT_ERR_ARG;
+  *bytes = ctx->peak;
+  return TRIBIT_OK;
+}
+
+int tribit_device_info(tribit_ctx* ctx, char* name, size_t name_len, int* cus, double* hbm_peak_gbs) {
+  if (!ctx) return TRIBIT_ERR_ARG;
+  if (name && name_len) {
+    strncpy(name, ctx->name, name_len - 1);
+    name[name_len - 1] = 0;
+  }
+  if (cus) *cus = ctx->cus;
+  if (hbm_peak_gbs) *hbm_peak_gbs = 8000.0;  // MI355X HBM3E, AMD-published
+  return TRIBIT_OK;
+}
+
+int tribit_read_mtx(const char* path, uint64_t** keys, uint64_t* nkeys, uint64_t* n, uint64_t* entries) {
+  if (!path || !keys || !nkeys || !n) return TRIBIT_ERR_ARG;
+  try {
+    return tb::read_mtx(path, keys, nkeys, n, entries);
+  } catch (const std::bad_alloc&) {
+    return TRIBIT_ERR_NOMEM;
+  }
+}
+
+void tribit_free_host(void* p) { free(p); }
+
+int tribit_gen_rmat(tribit_ctx* ctx, uint32_t scale, uint32_t ef, double a, double b, double c, uint64_t seed,
+                    uint64_t** d_keys, uint64_t* nkeys) {
+  if (!ctx || !d_keys || !nkeys || scale < 1 || scale > 31 || ef < 1) return TRIBIT_ERR_ARG;
+  if (a < 0 || b < 0 || c < 0 || a + b + c > 1.0 + 1e-12) return TRIBIT_ERR_ARG;
+  return guarded(ctx, [&] {
+    TB_HIP(hipSetDevice(ctx->device));
+    const uint64_t ne = (uint64_t)ef << scale;
+    uint64_t* p = ctx->alloc_n<uint64_t>(ne);
+    try {
+      tb::gen_rmat(ctx, scale, ef, a, b, c, seed, p);
+    } catch (...) {
+      ctx->free(p);
+      throw;
+    }
+    *d_keys = p;
+    *nkeys = ne;
+  });
+}
+
+int tribit_gen_er(tribit_ctx* ctx, uint64_t n, uint64_t nsamples, uint64_t seed, uint64_t** d_keys,
+                  uint64_t* nkeys) {
+  if (!ctx || !d_keys || !nkeys || n < 1 || n > 0xFFFFFFFEull) return TRIBIT_ERR_ARG;
+  return guarded(ctx, [&] {
+    TB_HIP(hipSetDevice(ctx->device));
+    uint64_t* p = ctx->alloc_n<uint64_t>(nsamples ? nsamples : 1);
+    try {
+      if (nsamples) tb::gen_er(ctx, n, nsamples, seed, p);
+    } catch (...) {
+      ctx->free(p);
+      throw;
+    }
+    *d_keys = p;
+    *nkeys = nsamples;
+  });
+}
+
+int tribit_graph_from_device_keys(tribit_ctx* ctx, uint64_t* d_keys, uint64_t nkeys, uint64_t n,
+                                  tribit_graph** out) {
+  if (!ctx || !out || (!d_keys && nkeys)) return TRIBIT_ERR_ARG;
+  *out = nullptr;
+  tribit_graph* g = new (std::nothrow) tribit_graph();
+  if (!g) return TRIBIT_ERR_NOMEM;
+  const int st = guarded(ctx, [&] {
+    TB_HIP(hipSetDevice(ctx->device));
+    uint64_t* keys = d_keys;
+    if (!keys) keys = ctx->alloc_n<uint64_t>(1);
+    tb::preprocess(ctx, keys, nkeys, n, g);
+  });
+  if (st != TRIBIT_OK) {
+    free_graph_memory(g);
+    delete g;
+    return st;
+  }
+  *out = g;
+  return TRIBIT_OK;
+}
+
+int tribit_graph_from_host_keys(tribit_ctx* ctx, const uint64_t* keys, uint64_t nkeys, uint64_t n,
+                                tribit_graph** out) {
+  if (!ctx || !out || (!keys && nkeys)) return TRIBIT_ERR_ARG;
+  uint64_t* d = nullptr;
+  const int st = guarded(ctx, [&] {
+    TB_HIP(hipSetDevice(ctx->device));
+    d = ctx->alloc_n<uint64_t>(nkeys ? nkeys : 1);
+    if (nkeys) TB_HIP(hipMemcpyAsync(d, keys, nkeys * 8, hipMemcpyHostToDevice, ctx->stream));
+    ctx->sync();
+  });
+  if (st != TRIBIT_OK) {
+    if (d) ctx->free(d);
+    return st;
+  }
+  return tribit_graph_from_device_keys(ctx, d, nkeys, n, out);
+}
+
+int tribit_graph_get_info(const tribit_graph* g, tribit_graph_info* info) {
+  if (!g || !info) return TRIBIT_ERR_ARG;
+  memset(info, 0, sizeof(*info));
+  info->n = g->n;
+  info->entries = g->entries;
+  info->m = g->m;
+  info->wplus = g->wplus;
+  info->rows_active = g->rows_active;
+  info->max_outdeg = g->max_outdeg;
+  info->ms_preprocess = g->ms_prep;
+  info->device_bytes = g->bytes;
+  return TRIBIT_OK;
+}
+
+int tribit_graph_download(const tribit_graph* g, uint32_t* row_ptr, uint32_t* col, uint32_t* deg) {
+  if (!g) return TRIBIT_ERR_ARG;
+  tribit_ctx* ctx = g->ctx;
+  return guarded(ctx, [&] {
+    TB_HIP(hipSetDevice(ctx->device));
+    if (row_ptr) TB_HIP(hipMemcpyAsync(row_ptr, g->row_ptr, 4 * (g->n + 1), hipMemcpyDeviceToHost, ctx->stream));
+    if (col && g->m) TB_HIP(hipMemcpyAsync(col, g->col, 4 * g->m, hipMemcpyDeviceToHost, ctx->stream));
+    if (deg) TB_HIP(hipMemcpyAsync(deg, g->deg, 4 * g->n, hipMemcpyDeviceToHost, ctx->stream));
+    ctx->sync();
+  });
+}
+
+int tribit_graph_free(tribit_graph* g) {
+  if (!g) return TRIBIT_ERR_ARG;
+  if (g->ctx) {
+    (void)hipSetDevice(g->ctx->device);
+    free_graph_memory(g);
+    (void)hipStreamSynchronize(g->ctx->stream);
+  }
+  delete g;
+  return TRIBIT_OK;
+}
+
+int tribit_count(tribit_ctx* ctx, tribit_graph* g, const tribit_count_opts* opts, tribit_count_info* info) {
+  if (!ctx || !g || g->ctx != ctx) return TRIBIT_ERR_ARG;
+  tribit_count_opts o;
+  memset(&o, 0, sizeof(o));
+  if (opts) o = *opts;
+  if (o.sched != TRIBIT_SCHED_PLAIN && o.sched != TRIBIT_SCHED_LEARNED) return TRIBIT_ERR_ARG;
+  return guarded(ctx, [&] {
+    TB_HIP(hipSetDevice(ctx->device));
+    tribit_count_info local;
+    memset(&local, 0, sizeof(local));
+    tb::count(ctx, g, o, &local);
+    if (info) *info = local;
+  });
+}
+
+int tribit_variant_stats(tribit_ctx* ctx, tribit_graph* g, int sched, tribit_variant_work* out) {
+  if (!ctx || !g || !out || g->ctx != ctx) return TRIBIT_ERR_ARG;
+  return guarded(ctx, [&] {
+    TB_HIP(hipSetDevice(ctx->device));
+    tb::variant_stats(ctx, g, sched, out);
+  });
+}
+
+int tribit_count_keys(tribit_ctx* ctx, const uint64_t* keys, uint64_t nkeys, uint64_t n,
+                      const tribit_count_opts* opts, uint64_t* triangles, double* ms_preprocess,
+                      tribit_count_info* info) {
+  if (!triangles) return TRIBIT_ERR_ARG;
+  tribit_graph* g = nullptr;
+  int st = tribit_graph_from_host_keys(ctx, keys, nkeys, n, &g);
+  if (st != TRIBIT_OK) return st;
+  tribit_count_info local;
+  memset(&local, 0, sizeof(local));
+  st = tribit_count(ctx, g, opts, &local);
+  if (st == TRIBIT_OK) {
+    *triangles = local.triangles;
+    if (ms_preprocess) *ms_preprocess = g->ms_prep;
+    if (info) *info = local;
+  }
+  tribit_graph_free(g);
+  return st;
+}
+
+}  // extern "C"

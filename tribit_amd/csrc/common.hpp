@@ -1,0 +1,152 @@
+// common.hpp -- shared host/device helpers of libtribit_hip (gfx950, wave64).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <unordered_map>
+
+#include "../../include/tribit.h"
+
+namespace tb {
+
+// Internal error carrier: thrown inside the library, converted to a
+// tribit_status at every extern "C" boundary (nothing escapes the ABI).
+struct Error {
+  int status;
+  std::string msg;
+};
+
+[[noreturn]] inline void fail(int status, const std::string& msg) { throw Error{status, msg}; }
+
+inline void hip_check(hipError_t e, const char* what, const char* file, int line) {
+  if (e != hipSuccess) {
+    char buf[512];
+    snprintf(buf, sizeof(buf), "%s failed at %s:%d: %s", what, file, line, hipGetErrorString(e));
+    fail(e == hipErrorOutOfMemory ? TRIBIT_ERR_NOMEM : TRIBIT_ERR_HIP, buf);
+  }
+}
+#define TB_HIP(x) ::tb::hip_check((x), #x, __FILE__, __LINE__)
+
+constexpr uint64_t kSentinel = 0xFFFFFFFFFFFFFFFFull;
+constexpr uint32_t kEmpty = 0xFFFFFFFFu;
+
+inline uint32_t bits_for(uint64_t n) {  // bits needed to hold ids 0..n-1 (>=1)
+  uint32_t b = 1;
+  while (b < 64 && (1ull << b) < n) ++b;
+  return b;
+}
+
+inline unsigned grid_for(uint64_t items, unsigned block) {
+  uint64_t g = (items + block - 1) / block;
+  if (g == 0) g = 1;
+  return (unsigned)g;
+}
+
+}  // namespace tb
+
+// Per-device context: one stream, stream-ordered pool allocations with
+// byte accounting (cap for tests, peak for reporting), timing events.
+struct tribit_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipMemPool_t pool = nullptr;
+  int cus = 0;
+  int lds_per_block = 0;
+  char name[256] = {0};
+  uint64_t cap = 0;    // 0 = unlimited
+  uint64_t held = 0;   // bytes currently allocated through this context
+  uint64_t peak = 0;
+  std::unordered_map<void*, uint64_t> sizes;
+  hipEvent_t ev[8] = {};
+  std::string last_error;
+
+  void* alloc(uint64_t bytes) {
+    if (bytes == 0) bytes = 8;
+    bytes = (bytes + 255) & ~uint64_t(255);
+    if (cap && held + bytes > cap) tb::fail(TRIBIT_ERR_NOMEM, "allocation exceeds the configured cap");
+    void* p = nullptr;
+    hipError_t e = hipMallocAsync(&p, bytes, stream);
+    if (e != hipSuccess || !p) {
+      (void)hipGetLastError();
+      tb::fail(TRIBIT_ERR_NOMEM, std::string("hipMallocAsync: ") + hipGetErrorString(e));
+    }
+    sizes[p] = bytes;
+    held += bytes;
+    if (held > peak) peak = held;
+    return p;
+  }
+  template <typename T>
+  T* alloc_n(uint64_t n) {
+    return static_cast<T*>(alloc(n * sizeof(T)));
+  }
+  void free(void* p) {
+    if (!p) return;
+    auto it = sizes.find(p);
+    if (it != sizes.end()) {
+      held -= it->second;
+      sizes.erase(it);
+    }
+    (void)hipFreeAsync(p, stream);
+  }
+  void sync() { TB_HIP(hipStreamSynchronize(stream)); }
+};
+
+// Scoped device buffer owned by a context (freed on scope exit, also on throw).
+template <typename T>
+struct DevBuf {
+  tribit_ctx* ctx = nullptr;
+  T* p = nullptr;
+  uint64_t n = 0;
+  DevBuf() = default;
+  DevBuf(tribit_ctx* c, uint64_t count) : ctx(c), p(c->alloc_n<T>(count)), n(count) {}
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : ctx(o.ctx), p(o.p), n(o.n) { o.p = nullptr; }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    reset();
+    ctx = o.ctx;
+    p = o.p;
+    n = o.n;
+    o.p = nullptr;
+    return *this;
+  }
+  ~DevBuf() { reset(); }
+  void reset() {
+    if (p && ctx) ctx->free(p);
+    p = nullptr;
+  }
+  T* release() {
+    T* q = p;
+    p = nullptr;
+    return q;
+  }
+};
+
+// Count-ready graph resident in HBM.
+struct tribit_graph {
+  tribit_ctx* ctx = nullptr;
+  uint64_t n = 0, m = 0, entries = 0, wplus = 0, rows_active = 0;
+  uint32_t max_outdeg = 0;
+  uint32_t* row_ptr = nullptr;  // n+1, oriented CSR
+  uint32_t* col = nullptr;      // m, ascending ids within a row
+  uint32_t* deg = nullptr;      // n, undirected degree
+  uint64_t* wrow = nullptr;     // n, W+(u) = sum of outdeg(v) over v in N+(u)
+  double ms_prep = 0.0;
+  uint64_t bytes = 0;
+};
+
+namespace tb {
+// prep.hip
+void preprocess(tribit_ctx* ctx, uint64_t* d_keys, uint64_t nkeys, uint64_t n, tribit_graph* g);
+void gen_rmat(tribit_ctx* ctx, uint32_t scale, uint32_t ef, double a, double b, double c, uint64_t seed,
+              uint64_t* d_keys);
+void gen_er(tribit_ctx* ctx, uint64_t n, uint64_t nsamples, uint64_t seed, uint64_t* d_keys);
+void rmat_thresholds(double a, double b, double c, uint32_t t[3]);
+// count.hip
+void count(tribit_ctx* ctx, tribit_graph* g, const tribit_count_opts& opts, tribit_count_info* info);
+void variant_stats(tribit_ctx* ctx, tribit_graph* g, int sched, tribit_variant_work* out);
+// mtx.cpp
+int read_mtx(const char* path, uint64_t** keys, uint64_t* nkeys, uint64_t* n, uint64_t* entries);
+}  // namespace tb

@@ -1,0 +1,273 @@
+// prep.hip -- on-GPU generators and preprocessing (edge keys in HBM ->
+// degree-oriented int32 CSR + per-row wedge counts), gfx950.
+//
+// Replaces the reference's preprocessing chain, src/single_gpu/tribit.cu:
+// 664-763 (radix sort + unique of packed keys, L/U bit structures, singleton
+// purge, warp schedule). The product here is a degree-oriented CSR instead:
+//   keys (min<<32|max) --sort--> --unique--> m undirected edges
+//   deg[v] (wave-aggregated atomics)
+//   orient {a<b}: a->b iff deg(a) <= deg(b)       (rank = (deg, id))
+//   --sort--> row_ptr[n+1] (u32), col[m] (u32, ascending within a row)
+//   wrow[u] = sum over v in N+(u) of outdeg(v)     (scheduler input)
+// Every launch is a 1-D grid of 256-thread blocks (4 waves).
+#include <cstring>
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
+#include <rocprim/device/device_select.hpp>
+
+#include "common.hpp"
+#include "device.hpp"
+
+namespace tb {
+
+constexpr unsigned kBlock = 256;
+
+// ---------------------------------------------------------------- generators
+
+__global__ void k_gen_rmat(uint64_t* __restrict__ keys, uint64_t ne, uint32_t scale, uint32_t t0, uint32_t t1,
+                           uint32_t t2, uint64_t key) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < ne; e += stride) {
+    uint64_t src = 0, dst = 0, w = 0;
+    for (uint32_t l = 0; l < scale; ++l) {
+      if ((l & 1) == 0) w = rand_word(key, e * 16 + (l >> 1));
+      const uint32_t r = (l & 1) ? (uint32_t)(w >> 32) : (uint32_t)w;
+      const uint64_t sb = (r >= t1) ? 1u : 0u;                       // quadrants (1,0),(1,1)
+      const uint64_t db = (r >= t0 && r < t1) || (r >= t2) ? 1u : 0u;  // quadrants (0,1),(1,1)
+      src |= sb << (scale - 1 - l);
+      dst |= db << (scale - 1 - l);
+    }
+    keys[e] = (src == dst) ? kSentinel : (src < dst ? ((src << 32) | dst) : ((dst << 32) | src));
+  }
+}
+
+__global__ void k_gen_er(uint64_t* __restrict__ keys, uint64_t ne, uint64_t n, uint64_t key) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < ne; e += stride) {
+    const uint64_t w = rand_word(key, e);
+    const uint64_t u = ((w & 0xFFFFFFFFull) * n) >> 32;
+    const uint64_t v = ((w >> 32) * n) >> 32;
+    keys[e] = (u == v) ? kSentinel : (u < v ? ((u << 32) | v) : ((v << 32) | u));
+  }
+}
+
+void rmat_thresholds(double a, double b, double c, uint32_t t[3]) {
+  const double two32 = 4294967296.0;
+  const double s[3] = {a, a + b, a + b + c};
+  for (int i = 0; i < 3; ++i) {
+    double v = s[i] * two32;
+    if (v < 0) v = 0;
+    if (v > 4294967295.0) v = 4294967295.0;
+    t[i] = (uint32_t)v;
+  }
+}
+
+static unsigned gen_grid(tribit_ctx* ctx, uint64_t ne) {
+  uint64_t g = (ne + kBlock - 1) / kBlock;
+  const uint64_t cap = (uint64_t)(ctx->cus > 0 ? ctx->cus : 256) * 64;
+  return (unsigned)(g < 1 ? 1 : (g > cap ? cap : g));
+}
+
+void gen_rmat(tribit_ctx* ctx, uint32_t scale, uint32_t ef, double a, double b, double c, uint64_t seed,
+              uint64_t* d_keys) {
+  uint32_t t[3];
+  rmat_thresholds(a, b, c, t);
+  const uint64_t ne = (uint64_t)ef << scale;
+  k_gen_rmat<<<gen_grid(ctx, ne), kBlock, 0, ctx->stream>>>(d_keys, ne, scale, t[0], t[1], t[2],
+                                                            stream_key(seed, kStreamRmat));
+  TB_HIP(hipGetLastError());
+}
+
+void gen_er(tribit_ctx* ctx, uint64_t n, uint64_t nsamples, uint64_t seed, uint64_t* d_keys) {
+  k_gen_er<<<gen_grid(ctx, nsamples), kBlock, 0, ctx->stream>>>(d_keys, nsamples, n, stream_key(seed, kStreamEr));
+  TB_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------- preprocessing
+
+__global__ void k_fill_u32(uint32_t* __restrict__ p, uint64_t n, uint32_t v) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
+}
+
+// Flags keys that are neither the sentinel nor (min < max < n).
+__global__ void k_validate(const uint64_t* __restrict__ keys, uint64_t nk, uint64_t n, uint32_t* __restrict__ bad) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nk) return;
+  const uint64_t k = keys[i];
+  if (k == kSentinel) return;
+  const uint64_t a = k >> 32, b = k & 0xFFFFFFFFull;
+  if (!(a < b && b < n)) atomicOr(bad, 1u);
+}
+
+// deg[a]++ and deg[b]++ for each undirected edge; keys are sorted, so the
+// 'a' side is aggregated per run inside the wave before the atomic.
+__global__ void k_degree(const uint64_t* __restrict__ keys, uint64_t m, uint32_t* __restrict__ deg) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = i < m;
+  const uint64_t k = valid ? keys[i] : 0;
+  const uint32_t a = valid ? (uint32_t)(k >> 32) : 0xFFFFFFFFu;
+  if (valid) atomicAdd(&deg[(uint32_t)k], 1u);
+  const uint32_t run = wave_run_length(a, valid);
+  if (run) atomicAdd(&deg[a], run);
+}
+
+// Degree orientation: {a<b} becomes a->b iff deg(a) <= deg(b).
+__global__ void k_orient(const uint64_t* __restrict__ keys, uint64_t m, const uint32_t* __restrict__ deg,
+                         uint64_t* __restrict__ okeys) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const uint64_t k = keys[i];
+  const uint32_t a = (uint32_t)(k >> 32), b = (uint32_t)k;
+  okeys[i] = (deg[a] <= deg[b]) ? k : (((uint64_t)b << 32) | a);
+}
+
+// col[i] = dst, odeg[src] += 1 (sorted by src: aggregated per run).
+__global__ void k_csr_split(const uint64_t* __restrict__ okeys, uint64_t m, uint32_t* __restrict__ col,
+                            uint32_t* __restrict__ odeg) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = i < m;
+  const uint64_t k = valid ? okeys[i] : 0;
+  const uint32_t s = valid ? (uint32_t)(k >> 32) : 0xFFFFFFFFu;
+  if (valid) col[i] = (uint32_t)k;
+  const uint32_t run = wave_run_length(s, valid);
+  if (run) atomicAdd(&odeg[s], run);
+}
+
+struct RowStats {
+  unsigned long long wplus;
+  unsigned long long rows_active;
+  unsigned int max_outdeg;
+  unsigned int pad;
+};
+
+// wrow[u] = sum of outdeg(v), v in N+(u); totals reduced per wave.
+__global__ void k_row_stats(const uint32_t* __restrict__ row_ptr, const uint32_t* __restrict__ col, uint64_t n,
+                            uint64_t* __restrict__ wrow, RowStats* __restrict__ st) {
+  const uint64_t u = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t w = 0;
+  uint32_t d = 0;
+  if (u < n) {
+    const uint32_t r0 = row_ptr[u], r1 = row_ptr[u + 1];
+    d = r1 - r0;
+    for (uint32_t e = r0; e < r1; ++e) {
+      const uint32_t v = col[e];
+      w += row_ptr[v + 1] - row_ptr[v];
+    }
+    wrow[u] = w;
+  }
+  const uint64_t ws = wave_sum_u64(w);
+  const uint64_t act = (uint64_t)__popcll(__ballot(d >= 2));
+  const uint32_t dm = wave_max_u32(d);
+  if (lane_id() == 0) {
+    if (ws) atomicAdd(&st->wplus, (unsigned long long)ws);
+    if (act) atomicAdd(&st->rows_active, (unsigned long long)act);
+    if (dm) atomicMax(&st->max_outdeg, dm);
+  }
+}
+
+template <typename F>
+static void with_temp(tribit_ctx* ctx, F&& f) {
+  size_t bytes = 0;
+  TB_HIP(f(nullptr, bytes));
+  DevBuf<uint8_t> tmp(ctx, bytes ? bytes : 8);
+  TB_HIP(f(tmp.p, bytes));
+}
+
+static void sort_keys(tribit_ctx* ctx, const uint64_t* in, uint64_t* out, uint64_t nk, uint32_t end_bit) {
+  with_temp(ctx, [&](void* t, size_t& b) {
+    return rocprim::radix_sort_keys(t, b, in, out, (size_t)nk, 0u, end_bit, ctx->stream);
+  });
+}
+
+void preprocess(tribit_ctx* ctx, uint64_t* d_keys_in, uint64_t nkeys, uint64_t n, tribit_graph* g) {
+  DevBuf<uint64_t> keys;
+  keys.ctx = ctx;
+  keys.p = d_keys_in;  // owned from here on
+  keys.n = nkeys;
+  if (n == 0 || n > 0xFFFFFFFEull) fail(TRIBIT_ERR_TOO_LARGE, "vertex count must be in [1, 2^32-2]");
+  hipStream_t s = ctx->stream;
+  g->ctx = ctx;
+  g->n = n;
+  g->entries = nkeys;
+  TB_HIP(hipEventRecord(ctx->ev[0], s));
+
+  DevBuf<uint32_t> flag(ctx, 4);
+  k_fill_u32<<<1, 64, 0, s>>>(flag.p, 4, 0u);
+  if (nkeys) k_validate<<<grid_for(nkeys, kBlock), kBlock, 0, s>>>(keys.p, nkeys, n, flag.p);
+
+  // 1-2. sort + unique (tribit.cu:666-667). The sentinel sorts last.
+  const uint32_t end_bit = 32 + bits_for(n);
+  uint64_t m = 0;
+  DevBuf<uint64_t> uniq(ctx, nkeys ? nkeys : 1);
+  if (nkeys) {
+    DevBuf<uint64_t> sorted(ctx, nkeys);
+    sort_keys(ctx, keys.p, sorted.p, nkeys, end_bit);
+    keys.reset();
+    DevBuf<uint64_t> ucount(ctx, 1);
+    with_temp(ctx, [&](void* t, size_t& b) {
+      return rocprim::unique(t, b, sorted.p, uniq.p, ucount.p, (size_t)nkeys, rocprim::equal_to<uint64_t>(), s);
+    });
+    uint64_t h[2] = {0, 0};
+    TB_HIP(hipMemcpyAsync(&h[0], ucount.p, 8, hipMemcpyDeviceToHost, s));
+    ctx->sync();
+    if (h[0]) TB_HIP(hipMemcpyAsync(&h[1], uniq.p + h[0] - 1, 8, hipMemcpyDeviceToHost, s));
+    uint32_t bad = 0;
+    TB_HIP(hipMemcpyAsync(&bad, flag.p, 4, hipMemcpyDeviceToHost, s));
+    ctx->sync();
+    if (bad) fail(TRIBIT_ERR_RANGE, "edge key is not (min < max < n)");
+    m = h[0] - ((h[0] && h[1] == kSentinel) ? 1 : 0);
+  } else {
+    keys.reset();
+  }
+  if (m > 0xFFFFFFFFull) fail(TRIBIT_ERR_TOO_LARGE, "more than 2^32-1 edges in one graph");
+  g->m = m;
+
+  // 3. degrees
+  DevBuf<uint32_t> deg(ctx, n);
+  k_fill_u32<<<grid_for(n, kBlock), kBlock, 0, s>>>(deg.p, n, 0u);
+  if (m) k_degree<<<grid_for(m, kBlock), kBlock, 0, s>>>(uniq.p, m, deg.p);
+
+  // 4-5. orient, sort by (src, dst)
+  DevBuf<uint32_t> col(ctx, m ? m : 1);
+  DevBuf<uint32_t> row_ptr(ctx, n + 1);
+  {
+    DevBuf<uint32_t> odeg(ctx, n + 1);
+    k_fill_u32<<<grid_for(n + 1, kBlock), kBlock, 0, s>>>(odeg.p, n + 1, 0u);
+    if (m) {
+      DevBuf<uint64_t> okeys(ctx, m);
+      k_orient<<<grid_for(m, kBlock), kBlock, 0, s>>>(uniq.p, m, deg.p, okeys.p);
+      sort_keys(ctx, okeys.p, uniq.p, m, end_bit);
+      // 6. CSR
+      k_csr_split<<<grid_for(m, kBlock), kBlock, 0, s>>>(uniq.p, m, col.p, odeg.p);
+    }
+    uniq.reset();
+    with_temp(ctx, [&](void* t, size_t& b) {
+      return rocprim::exclusive_scan(t, b, odeg.p, row_ptr.p, 0u, (size_t)(n + 1), rocprim::plus<uint32_t>(), s);
+    });
+  }
+
+  // 7. per-row wedge counts and totals
+  DevBuf<uint64_t> wrow(ctx, n);
+  DevBuf<RowStats> st(ctx, 1);
+  k_fill_u32<<<1, 64, 0, s>>>(reinterpret_cast<uint32_t*>(st.p), sizeof(RowStats) / 4, 0u);
+  k_row_stats<<<grid_for(n, kBlock), kBlock, 0, s>>>(row_ptr.p, col.p, n, wrow.p, st.p);
+  RowStats hst;
+  TB_HIP(hipMemcpyAsync(&hst, st.p, sizeof(hst), hipMemcpyDeviceToHost, s));
+  TB_HIP(hipEventRecord(ctx->ev[1], s));
+  TB_HIP(hipEventSynchronize(ctx->ev[1]));
+  TB_HIP(hipGetLastError());
+  float ms = 0.f;
+  TB_HIP(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
+  g->ms_prep = ms;
+  g->wplus = hst.wplus;
+  g->rows_active = hst.rows_active;
+  g->max_outdeg = hst.max_outdeg;
+  g->row_ptr = row_ptr.release();
+  g->col = col.release();
+  g->deg = deg.release();
+  g->wrow = wrow.release();
+  g->bytes = 4 * (n + 1) + 4 * (m ? m : 1) + 4 * n + 8 * n;
+}
+
+}  // namespace tb

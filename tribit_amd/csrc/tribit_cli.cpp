@@ -1,0 +1,218 @@
+// tribit_cli.cpp -- the `tribit` command: drop-in for the reference's
+// process surface `tribit -i <graph.mtx>` (src/single_gpu/tribit.cu:271-315,
+// 840-866), calling the GPU only through the C ABI in include/tribit.h.
+//
+// Kept from the reference:
+//   - getopt "i:"; no arguments, a missing -i or an unknown option print the
+//     usage and exit EXIT_FAILURE; extra arguments print a warning;
+//   - a warm-up pass (no output) then one measured pass (tribit.cu:857-862);
+//   - one summary block with the labels and spacing of tribit.cu:807-818,
+//     which run_single_gpu.py:78-93 parses with four regexes.
+// Added (long options only, so `-i` stays byte-compatible):
+//   --rmat S,EF,A,B,C,SEED  generate an R-MAT graph in HBM instead of reading
+//   --er N,M,SEED           generate Erdos-Renyi G(n, M) in HBM
+//   --sched plain|learned   scheduler choice
+//   --repeats K             timed counting repetitions (median reported)
+//   --json PATH             write a JSON side record of the measured pass
+// Input or device errors print a message and exit 2 (input) or 3 (device);
+// the reference aborts on an uncaught exception there (tribit.cu:230, 241-245).
+#include <getopt.h>
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+
+#include "../../include/tribit.h"
+
+namespace {
+
+struct Config {
+  std::string input_file;
+  std::string rmat, er, json;
+  int sched = TRIBIT_SCHED_PLAIN;
+  int repeats = 1;
+};
+
+std::string get_usage(const char* prog) {
+  return std::string("Usage: ") + prog +
+         " -i <input_file>\n"
+         "Options:\n"
+         "  -i <input_file>    Path to the input file (required)\n"
+         "  --rmat S,EF,A,B,C,SEED   Generate an R-MAT graph instead of reading a file\n"
+         "  --er N,M,SEED            Generate an Erdos-Renyi G(n,M) graph instead\n"
+         "  --sched plain|learned    Scheduler (default plain)\n"
+         "  --repeats K              Timed counting repetitions (default 1)\n"
+         "  --json PATH              Write a JSON record of the measured pass\n";
+}
+
+[[noreturn]] void usage_exit(const char* prog, const char* msg) {
+  if (msg) fprintf(stderr, "%s", msg);
+  fprintf(stderr, "%s", get_usage(prog).c_str());
+  std::exit(EXIT_FAILURE);
+}
+
+Config program_options(int argc, char** argv) {
+  Config c;
+  if (argc == 1) usage_exit(argv[0], nullptr);
+  static struct option lo[] = {{"rmat", required_argument, nullptr, 1},
+                               {"er", required_argument, nullptr, 2},
+                               {"sched", required_argument, nullptr, 3},
+                               {"repeats", required_argument, nullptr, 4},
+                               {"json", required_argument, nullptr, 5},
+                               {nullptr, 0, nullptr, 0}};
+  int opt;
+  while ((opt = getopt_long(argc, argv, "i:", lo, nullptr)) != -1) {
+    switch (opt) {
+      case 'i': c.input_file = optarg; break;
+      case 1: c.rmat = optarg; break;
+      case 2: c.er = optarg; break;
+      case 3:
+        if (!strcmp(optarg, "plain"))
+          c.sched = TRIBIT_SCHED_PLAIN;
+        else if (!strcmp(optarg, "learned"))
+          c.sched = TRIBIT_SCHED_LEARNED;
+        else
+          usage_exit(argv[0], "Error: --sched must be plain or learned\n");
+        break;
+      case 4:
+        c.repeats = atoi(optarg);
+        if (c.repeats < 1) usage_exit(argv[0], "Error: --repeats must be >= 1\n");
+        break;
+      case 5: c.json = optarg; break;
+      default: usage_exit(argv[0], "Error: unknown option or missing argument\n");
+    }
+  }
+  if (c.input_file.empty() && c.rmat.empty() && c.er.empty())
+    usage_exit(argv[0], "Error: input file is required (-i option)\n");
+  if (optind < argc) fprintf(stderr, "Warning: extra arguments ignored\n");
+  return c;
+}
+
+[[noreturn]] void die(int code, const char* what, int st) {
+  fprintf(stderr, "Error: %s: %s\n", what, tribit_status_str(st));
+  std::exit(code);
+}
+
+struct Input {
+  uint64_t* host_keys = nullptr;
+  uint64_t nkeys = 0, n = 0;
+  // generator spec (keys are produced in HBM on every pass)
+  bool gen_rmat = false, gen_er = false;
+  unsigned scale = 0, ef = 0;
+  double a = 0, b = 0, c = 0;
+  unsigned long long seed = 0, er_n = 0, er_m = 0;
+};
+
+tribit_graph* build_graph(tribit_ctx* ctx, const Input& in) {
+  tribit_graph* g = nullptr;
+  int st;
+  if (in.gen_rmat || in.gen_er) {
+    uint64_t* d = nullptr;
+    uint64_t nk = 0;
+    st = in.gen_rmat ? tribit_gen_rmat(ctx, in.scale, in.ef, in.a, in.b, in.c, in.seed, &d, &nk)
+                     : tribit_gen_er(ctx, in.er_n, in.er_m, in.seed, &d, &nk);
+    if (st) die(3, "generator", st);
+    st = tribit_graph_from_device_keys(ctx, d, nk, in.gen_rmat ? (1ull << in.scale) : in.er_n, &g);
+  } else {
+    st = tribit_graph_from_host_keys(ctx, in.host_keys, in.nkeys, in.n, &g);
+  }
+  if (st) die(3, "preprocessing", st);
+  return g;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  printf("%s\n", tribit_build_info());
+  Config cfg = program_options(argc, argv);
+  int st = TRIBIT_OK;
+  Input in;
+  std::string graph_name;
+  if (!cfg.rmat.empty()) {
+    in.gen_rmat = true;
+    if (sscanf(cfg.rmat.c_str(), "%u,%u,%lf,%lf,%lf,%llu", &in.scale, &in.ef, &in.a, &in.b, &in.c, &in.seed) != 6)
+      usage_exit(argv[0], "Error: --rmat expects S,EF,A,B,C,SEED\n");
+    graph_name = "rmat:" + cfg.rmat;
+  } else if (!cfg.er.empty()) {
+    in.gen_er = true;
+    if (sscanf(cfg.er.c_str(), "%llu,%llu,%llu", &in.er_n, &in.er_m, &in.seed) != 3)
+      usage_exit(argv[0], "Error: --er expects N,M,SEED\n");
+    graph_name = "er:" + cfg.er;
+  } else {
+    printf("Reading data from %s...\n", cfg.input_file.c_str());
+    fflush(stdout);
+    st = tribit_read_mtx(cfg.input_file.c_str(), &in.host_keys, &in.nkeys, &in.n, nullptr);
+    if (st) die(2, cfg.input_file.c_str(), st);
+    graph_name = cfg.input_file;
+  }
+  // the device is opened after the input is read, so input errors need no GPU
+  tribit_ctx* ctx = nullptr;
+  st = tribit_init(0, &ctx);
+  if (st) die(3, "device init", st);
+
+  tribit_count_opts opts;
+  memset(&opts, 0, sizeof(opts));
+  opts.sched = cfg.sched;
+  opts.repeats = 1;
+  // warm-up pass (tribit.cu:858): full pipeline, nothing printed
+  {
+    tribit_graph* g = build_graph(ctx, in);
+    tribit_count_info ci;
+    st = tribit_count(ctx, g, &opts, &ci);
+    if (st) die(3, "count", st);
+    tribit_graph_free(g);
+  }
+  // measured pass (tribit.cu:862)
+  tribit_graph* g = build_graph(ctx, in);
+  opts.repeats = cfg.repeats;
+  tribit_count_info ci;
+  st = tribit_count(ctx, g, &opts, &ci);
+  if (st) die(3, "count", st);
+  tribit_graph_info gi;
+  tribit_graph_get_info(g, &gi);
+
+  printf(
+      "============================================================\n"
+      "                 Triangle Counting Summary\n"
+      "============================================================\n"
+      "Graph              : %s\n"
+      "Blocks of threads  : %llu\n"
+      "Preprocessing (ms) : %.3f\n"
+      "Kernel (ms)        : %.3f\n"
+      "Triangles          : %llu\n"
+      "============================================================\n",
+      graph_name.c_str(), (unsigned long long)ci.launch_blocks, gi.ms_preprocess, ci.ms_count,
+      (unsigned long long)ci.triangles);
+  printf("No hip error detected at %s:%d (%s)\n", __FILE__, __LINE__, __func__);
+  const double gteps = ci.ms_count > 0 ? (double)gi.m / (ci.ms_count * 1e-3) / 1e9 : 0.0;
+  const double balg = 4.0 * (double)(gi.n + 1) + 4.0 * (double)gi.m + 4.0 * (double)gi.wplus;
+  printf("Vertices           : %llu\n", (unsigned long long)gi.n);
+  printf("Edges (undirected) : %llu\n", (unsigned long long)gi.m);
+  printf("Wedge checks (W+)  : %llu\n", (unsigned long long)gi.wplus);
+  printf("Scheduler (ms)     : %.3f\n", ci.ms_sched);
+  printf("GTEPS              : %.3f\n", gteps);
+  printf("Algorithmic GB/s   : %.1f\n", ci.ms_count > 0 ? balg / (ci.ms_count * 1e-3) / 1e9 : 0.0);
+  uint64_t peak = 0;
+  tribit_pool_peak(ctx, &peak);
+  printf("Peak pool (GiB)    : %.3f\n", (double)peak / (1024.0 * 1024.0 * 1024.0));
+  if (!cfg.json.empty()) {
+    FILE* f = fopen(cfg.json.c_str(), "w");
+    if (f) {
+      fprintf(f,
+              "{\"graph\": \"%s\", \"n\": %llu, \"m\": %llu, \"wplus\": %llu, \"triangles\": %llu, "
+              "\"ms_preprocess\": %.4f, \"ms_count\": %.4f, \"ms_sched\": %.4f, \"ms_kernel\": %.4f, "
+              "\"gteps\": %.4f, \"blocks\": %llu, \"tasks\": %llu, \"sched\": \"%s\", \"peak_pool_bytes\": %llu}\n",
+              graph_name.c_str(), (unsigned long long)gi.n, (unsigned long long)gi.m,
+              (unsigned long long)gi.wplus, (unsigned long long)ci.triangles, gi.ms_preprocess, ci.ms_count,
+              ci.ms_sched, ci.ms_kernel, gteps, (unsigned long long)ci.launch_blocks,
+              (unsigned long long)ci.tasks, cfg.sched == TRIBIT_SCHED_LEARNED ? "learned" : "plain",
+              (unsigned long long)peak);
+      fclose(f);
+    }
+  }
+  tribit_graph_free(g);
+  tribit_free_host(in.host_keys);
+  tribit_shutdown(ctx);
+  return 0;
+}
