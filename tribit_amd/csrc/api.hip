@@ -88,7 +88,10 @@ int tribit_init(int device, tribit_ctx** out) {
       tb::fail(TRIBIT_ERR_NODEVICE, std::string("library built for gfx950, device is ") + prop.gcnArchName);
     ctx->cus = prop.multiProcessorCount;
     ctx->lds_per_block = (int)prop.sharedMemPerBlock;
-    snprintf(ctx->name, sizeof(ctx->name), "%s (%s)", prop.name, prop.gcnArchName);
+    char dname[128] = {0};
+    if (hipDeviceGetName(dname, sizeof(dname) - 1, device) != hipSuccess || !dname[0])
+      snprintf(dname, sizeof(dname), "%s", prop.name);
+    snprintf(ctx->name, sizeof(ctx->name), "%s (%s, %d CUs)", dname, prop.gcnArchName, prop.multiProcessorCount);
     TB_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     TB_HIP(hipDeviceGetDefaultMemPool(&ctx->pool, device));
     uint64_t thr = UINT64_MAX;  // keep freed blocks in the pool (tribit.cu:829-830)
