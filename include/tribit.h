@@ -69,10 +69,14 @@ typedef enum {
   TRIBIT_SCHED_LEARNED = 1  /* cost-model driven (falls back to PLAIN without a model) */
 } tribit_sched;
 
+/* tribit_count_opts.flags */
+#define TRIBIT_COUNT_FORCE_LINEAR_PROBE 1 /* test hook: skip the cuckoo sets, use the fallback tables */
+
 typedef struct {
   int32_t sched;           /* tribit_sched */
   int32_t repeats;         /* timed repetitions (>=1); the count of every repeat must agree */
-  int32_t reserved[6];
+  int32_t flags;           /* TRIBIT_COUNT_* bits */
+  int32_t reserved[5];
 } tribit_count_opts;
 
 typedef struct {

@@ -126,6 +126,22 @@ def test_block_classes_and_chunks(gpu_ctx, oracle_mod):
         assert g.count().triangles == oracle_mod.count_keys(keys, 5000)
 
 
+def test_fallback_linear_probe(gpu_ctx, oracle_mod):
+    """The fallback sets (used when every cuckoo build of a task fails) count the same."""
+    note("forced linear-probe fallback")
+    for spec in [(13, 12, 1), (16, 16, 4)]:
+        with gpu_ctx.gen_rmat(spec[0], spec[1], seed=spec[2]) as g:
+            a = g.count().triangles
+            b = g.count(flags=tribit_amd.FORCE_LINEAR_PROBE).triangles
+            assert a == b
+    k1, t1 = graphs.complete(2100)
+    with gpu_ctx.graph_from_keys(k1, 2100) as g:
+        assert g.count(flags=tribit_amd.FORCE_LINEAR_PROBE).triangles == t1
+    keys = graphs.random_graph(3000, 0.3, 5)
+    with gpu_ctx.graph_from_keys(keys, 3000) as g:
+        assert g.count(flags=tribit_amd.FORCE_LINEAR_PROBE).triangles == oracle_mod.count_keys(keys, 3000)
+
+
 def test_rmat16_repeats(gpu_ctx, oracle_mod):
     note("rmat s16 ef16 seed 4")
     with gpu_ctx.gen_rmat(16, 16, seed=4) as g:

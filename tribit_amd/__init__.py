@@ -48,7 +48,11 @@ class GraphInfo(ctypes.Structure):
 
 
 class CountOpts(ctypes.Structure):
-    _fields_ = [("sched", ctypes.c_int32), ("repeats", ctypes.c_int32), ("reserved", ctypes.c_int32 * 6)]
+    _fields_ = [("sched", ctypes.c_int32), ("repeats", ctypes.c_int32), ("flags", ctypes.c_int32),
+                ("reserved", ctypes.c_int32 * 5)]
+
+
+FORCE_LINEAR_PROBE = 1
 
 
 class CountInfo(ctypes.Structure):
@@ -178,10 +182,11 @@ class Graph:
                "graph_download")
         return rp, col[:m], deg[:n]
 
-    def count(self, sched: str = "plain", repeats: int = 1) -> CountResult:
+    def count(self, sched: str = "plain", repeats: int = 1, flags: int = 0) -> CountResult:
         o = CountOpts()
         o.sched = SCHED[sched]
         o.repeats = repeats
+        o.flags = flags
         ci = CountInfo()
         _check(self.ctx.lib.tribit_count(self.ctx.h, self.h, ctypes.byref(o), ctypes.byref(ci)), "count")
         nv = ci.nvariants

@@ -5,17 +5,17 @@
 // (U word, 32-neighbour window), one b1 tensor-core AND+POPC; here the
 // count is the degree-oriented edge iterator
 //     triangles = sum over u, v in N+(u) of |N+(u) & N+(v)|
-// with N+(u) staged once per task in an LDS hash set and every N+(v)
-// streamed from HBM with coalesced int32 loads, one LDS probe per element.
+// with N+(u) staged once per task in an LDS set and every N+(v) streamed
+// from HBM with coalesced int32 loads, one branch-free LDS probe per element.
 //
 // Scheduler (plain): rows with outdeg d >= 2 become tasks; the class of a
-// row picks the kernel variant (LDS table size, wave- or block-level task)
+// row picks the kernel variant (LDS set size, wave- or block-level task)
 // and its wedge count W+(u) = sum of outdeg(v) picks the work split.
-//   class 0: d <= 32      wave task, 128-slot table        (k_count_wave<7>)
-//   class 1: d <= 256     wave task, 1024-slot table       (k_count_wave<10>)
-//   class 2: d <= 2048    block task, 8192-slot table      (k_count_block<13>)
-//   class 3: d >  2048    block task per 2048-wide chunk of N+(u); each N+(v)
-//                         is cut to the chunk's id range by binary search.
+//   class 0: d <= 32      wave task,  cuckoo set 2 x 64 slots
+//   class 1: d <= 256     wave task,  cuckoo set 2 x 512 slots
+//   class 2: d <= 2048    block task, cuckoo set 2 x 4096 slots
+//   class 3: d >  2048    block task per 2048-id chunk of N+(u); each N+(v)
+//                         range is cut to the chunk's id interval.
 #include <cstring>
 #include <algorithm>
 #include <rocprim/device/device_scan.hpp>
@@ -27,36 +27,78 @@
 namespace tb {
 
 constexpr unsigned kBlock = 256;
-constexpr uint32_t kHashMul = 0x9E3779B1u;
 constexpr uint32_t kWaveBatch = 16;      // wave tasks taken per queue atomic
 constexpr uint32_t kChunk = 2048;        // probe-set chunk of class-3 rows
 constexpr uint64_t kWaveCap = 1u << 15;  // target streamed elements per wave task
 constexpr uint64_t kBlockCap = 1u << 18; // target streamed elements per block task
 constexpr int kVariants = 4;             // one kernel launch per row class
 
-// ------------------------------------------------------------ LDS hash set
+// ------------------------------------------------------------ LDS sets
+// N+(u) (or one id-range chunk of it) is held in LDS as a two-table cuckoo
+// set: tables [0,S) and [S,2S), key x lives at h_a(x) or S + h_b(x), so a
+// membership probe is two LDS reads and two compares, with no data-dependent
+// loop (every lane of a wave does the same work). Tables are filled to at
+// most 1/4 of their 2S slots. A build that fails (an insertion chain longer
+// than kCuckooIters) is retried with the next multiplier pair; after
+// kCuckooSeeds failures the task falls back to a linear-probing table in the
+// same 2S slots.
+
+constexpr uint32_t kNoKey = 0xFFFFFFFEu;  // never a vertex id (n <= 2^32-2) and never kEmpty
+constexpr int kCuckooIters = 64;
+constexpr int kCuckooSeeds = 4;
+__device__ __constant__ uint32_t kSeedA[kCuckooSeeds] = {0x9E3779B1u, 0xC2B2AE3Du, 0x165667B1u, 0xFD7046C5u};
+__device__ __constant__ uint32_t kSeedB[kCuckooSeeds] = {0x85EBCA77u, 0x27D4EB2Fu, 0xD3A2646Du, 0xB55A4F09u};
 
 template <int LOGS>
-__device__ __forceinline__ uint32_t hslot(uint32_t w) {
-  return (w * kHashMul) >> (32 - LOGS);
+__device__ __forceinline__ uint32_t hmul(uint32_t x, uint32_t a) {
+  return (x * a) >> (32 - LOGS);
 }
 
 template <int LOGS>
-__device__ __forceinline__ void hash_insert(uint32_t* T, uint32_t w) {
-  uint32_t h = hslot<LOGS>(w);
-  while (atomicCAS(&T[h], kEmpty, w) != kEmpty) h = (h + 1) & ((1u << LOGS) - 1);
-}
-
-template <int LOGS>
-__device__ __forceinline__ uint32_t hash_probe(const uint32_t* T, uint32_t w) {
-  uint32_t h = hslot<LOGS>(w);
-  for (;;) {
-    const uint32_t k = T[h];
-    if (k == w) return 1u;
-    if (k == kEmpty) return 0u;
-    h = (h + 1) & ((1u << LOGS) - 1);
+__device__ __forceinline__ bool cuckoo_insert(uint32_t* T, uint32_t x, uint32_t a, uint32_t b) {
+  constexpr uint32_t S = 1u << LOGS;
+  uint32_t pos = hmul<LOGS>(x, a);
+  for (int it = 0; it < kCuckooIters; ++it) {
+    const uint32_t y = atomicExch(&T[pos], x);
+    if (y == kEmpty) return true;
+    // y sat in one of its two slots; it moves to the other one
+    x = y;
+    const uint32_t p1 = hmul<LOGS>(x, a);
+    pos = (pos == p1) ? S + hmul<LOGS>(x, b) : p1;
   }
+  return false;
 }
+
+template <int LOGS>
+struct CuckooProbe {
+  const uint32_t* T;
+  uint32_t a, b;
+  __device__ __forceinline__ uint32_t operator()(uint32_t x) const {
+    constexpr uint32_t S = 1u << LOGS;
+    return (uint32_t)(T[hmul<LOGS>(x, a)] == x) + (uint32_t)(T[S + hmul<LOGS>(x, b)] == x);
+  }
+};
+
+// Linear probing over all 2S slots (fallback only).
+template <int LOGS2>
+__device__ __forceinline__ void linear_insert(uint32_t* T, uint32_t w) {
+  uint32_t h = hmul<LOGS2>(w, 0x9E3779B1u);
+  while (atomicCAS(&T[h], kEmpty, w) != kEmpty) h = (h + 1) & ((1u << LOGS2) - 1);
+}
+
+template <int LOGS2>
+struct LinearProbe {
+  const uint32_t* T;
+  __device__ __forceinline__ uint32_t operator()(uint32_t w) const {
+    uint32_t h = hmul<LOGS2>(w, 0x9E3779B1u);
+    for (;;) {
+      const uint32_t k = T[h];
+      if (k == w) return 1u;
+      if (k == kEmpty) return 0u;
+      h = (h + 1) & ((1u << LOGS2) - 1);
+    }
+  }
+};
 
 template <int G>
 __host__ __device__ constexpr uint64_t leader_mask() {
@@ -87,56 +129,109 @@ __device__ __forceinline__ uint32_t upper_bound_u32(const uint32_t* a, uint32_t 
   return lo;
 }
 
-// ------------------------------------------------------ wave-level tasks
-// A wave owns one task (u, k0, k1): table of N+(u), then the v = N+(u)[k]
-// for k in [k0,k1) are streamed by 64/G lane groups of G lanes; a group that
-// runs dry takes the next v (ballot rank among the dry groups).
+// --------------------------------------------------------- range streaming
+// The lists to stream, N+(v) for the task's v, are staged in LDS as ranges
+// R[k] = [begin, end) of col. 64/G groups of G lanes each own one range at a
+// time; per step a group covers G*U consecutive elements, each lane issuing
+// U independent loads before probing (U loads in flight per lane). A group
+// whose range is exhausted takes the next index: wave-private counter, or,
+// for block tasks (SHARED), an LDS counter shared by the block's waves.
 
-template <int LOGS, int G>
+template <int G, int U, bool SHARED, typename Probe>
+__device__ __forceinline__ uint32_t stream_ranges(const uint32_t* __restrict__ col, const uint2* R, uint32_t nk,
+                                                  uint32_t* s_next, uint32_t lane, const Probe& probe) {
+  constexpr uint64_t kLeads = leader_mask<G>();
+  const uint32_t g = lane / G, gl = lane % G;
+  const uint64_t below = (g == 0) ? 0ull : ((1ull << (g * G)) - 1);
+  uint32_t base = 0, end = 0, cnt = 0, next = 0;
+  for (;;) {
+    const bool need = base >= end;  // uniform within a group
+    const uint64_t want = __ballot(need && gl == 0);
+    const uint32_t nw = (uint32_t)__popcll(want);
+    uint32_t kb = 0;
+    if (nw) {
+      if (SHARED) {
+        if (lane == 0) kb = atomicAdd(s_next, nw);
+        kb = __shfl(kb, 0);
+      } else {
+        kb = next;
+        next = min(next + nw, nk);
+      }
+      if (want == kLeads && kb >= nk) break;
+    }
+    if (need) {
+      const uint32_t k = kb + (uint32_t)__popcll(want & below);
+      if (k < nk) {
+        const uint2 r = R[k];
+        base = r.x;
+        end = r.y;
+      }
+    }
+    uint32_t x[U];
+#pragma unroll
+    for (int j = 0; j < U; ++j) {
+      const uint32_t p = base + gl + (uint32_t)(j * G);
+      x[j] = (p < end) ? col[p] : kNoKey;
+    }
+#pragma unroll
+    for (int j = 0; j < U; ++j) cnt += probe(x[j]);
+    base += (uint32_t)(G * U);
+  }
+  return cnt;
+}
+
+// ------------------------------------------------------ wave-level tasks
+// A wave owns one task (u, k0, k1): the set N+(u) and the ranges of
+// N+(v), v = N+(u)[k] for k in [k0, k1), both in the wave's LDS slice.
+
+template <int LOGS, int KMAX, int G, int U>
 __global__ __launch_bounds__(256) void k_count_wave(const uint32_t* __restrict__ row_ptr,
                                                     const uint32_t* __restrict__ col,
                                                     const uint2* __restrict__ tasks, uint32_t ntasks,
                                                     uint32_t* __restrict__ queue,
-                                                    unsigned long long* __restrict__ result) {
+                                                    unsigned long long* __restrict__ result, int attempts) {
   constexpr uint32_t S = 1u << LOGS;
-  constexpr uint64_t kLeads = leader_mask<G>();
-  __shared__ uint32_t lds[4 * S];
-  const uint32_t lane = threadIdx.x & 63;
-  uint32_t* T = lds + (threadIdx.x >> 6) * S;
-  const uint32_t g = lane / G, gl = lane % G;
-  const uint64_t below = (g == 0) ? 0ull : ((1ull << (g * G)) - 1);
+  __shared__ uint32_t sT[4][2 * S];
+  __shared__ uint2 sR[4][KMAX];
+  const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  uint32_t* T = sT[wv];
+  uint2* R = sR[wv];
   uint64_t acc = 0;
   for (;;) {
-    uint32_t base = 0;
-    if (lane == 0) base = atomicAdd(queue, kWaveBatch);
-    base = __shfl(base, 0);
-    if (base >= ntasks) break;
-    const uint32_t tend = min(base + kWaveBatch, ntasks);
-    for (uint32_t t = base; t < tend; ++t) {
+    uint32_t tb = 0;
+    if (lane == 0) tb = atomicAdd(queue, kWaveBatch);
+    tb = __shfl(tb, 0);
+    if (tb >= ntasks) break;
+    const uint32_t tend = min(tb + kWaveBatch, ntasks);
+    for (uint32_t t = tb; t < tend; ++t) {
       const uint2 tk = tasks[t];
       const uint32_t u = tk.x, k0 = tk.y & 0xFFFFu, k1 = tk.y >> 16;
-      const uint32_t r0 = row_ptr[u], d = row_ptr[u + 1] - r0;
-      for (uint32_t i = lane; i < S; i += 64) T[i] = kEmpty;
-      wave_lds_sync();
-      for (uint32_t i = lane; i < d; i += 64) hash_insert<LOGS>(T, col[r0 + i]);
-      wave_lds_sync();
-      uint32_t pos = 0, end = 0, cnt = 0, next = k0;
-      for (;;) {
-        const bool need = pos >= end;  // uniform within a group
-        const uint64_t want = __ballot(need && gl == 0);
-        if (want == kLeads && next >= k1) break;
-        if (need) {
-          const uint32_t k = next + (uint32_t)__popcll(want & below);
-          if (k < k1) {
-            const uint32_t v = col[r0 + k];
-            pos = row_ptr[v];
-            end = row_ptr[v + 1];
-          }
-        }
-        next = min(next + (uint32_t)__popcll(want), k1);
-        const uint32_t p = pos + gl;
-        if (p < end) cnt += hash_probe<LOGS>(T, col[p]);
-        pos += G;
+      const uint32_t r0 = row_ptr[u], d = row_ptr[u + 1] - r0, nk = k1 - k0;
+      for (uint32_t k = lane; k < nk; k += 64) {
+        const uint32_t v = col[r0 + k0 + k];
+        R[k] = make_uint2(row_ptr[v], row_ptr[v + 1]);
+      }
+      uint32_t sa = 0, sb = 0;
+      bool ok = false;
+      for (int attempt = 0; attempt < attempts && !ok; ++attempt) {
+        sa = kSeedA[attempt];
+        sb = kSeedB[attempt];
+        for (uint32_t i = lane; i < 2 * S; i += 64) T[i] = kEmpty;
+        wave_lds_sync();
+        bool fail = false;
+        for (uint32_t i = lane; i < d; i += 64) fail |= !cuckoo_insert<LOGS>(T, col[r0 + i], sa, sb);
+        wave_lds_sync();
+        ok = !__any(fail);
+      }
+      uint32_t cnt;
+      if (ok) {
+        cnt = stream_ranges<G, U, false>(col, R, nk, nullptr, lane, CuckooProbe<LOGS>{T, sa, sb});
+      } else {
+        for (uint32_t i = lane; i < 2 * S; i += 64) T[i] = kEmpty;
+        wave_lds_sync();
+        for (uint32_t i = lane; i < d; i += 64) linear_insert<LOGS + 1>(T, col[r0 + i]);
+        wave_lds_sync();
+        cnt = stream_ranges<G, U, false>(col, R, nk, nullptr, lane, LinearProbe<LOGS + 1>{T});
       }
       acc += cnt;
       wave_lds_sync();
@@ -147,24 +242,26 @@ __global__ __launch_bounds__(256) void k_count_wave(const uint32_t* __restrict__
 }
 
 // ----------------------------------------------------- block-level tasks
-// A 256-thread block owns one task (u, k0, k1, chunk c): the probe set is
-// N+(u)[c*kChunk, ...) in one shared table; its 4 waves share the v range
-// through an LDS counter. When the row has more than one chunk, each N+(v)
-// is cut to the chunk's id range [lo, hi] (both lists are sorted).
+// A 512-thread block owns one task (u, k0, k1, chunk c): the probe set is
+// N+(u)[c*kChunk, ...) (at most kChunk ids), the ranges of N+(v) for
+// k in [k0, k1) (at most kChunk); its 8 waves share the ranges through an
+// LDS counter. When the row has more than one chunk, each N+(v) range is cut
+// to the chunk's id interval [lo, hi] while it is staged (both are sorted).
 
-template <int LOGS, int G>
-__global__ __launch_bounds__(256) void k_count_block(const uint32_t* __restrict__ row_ptr,
-                                                     const uint32_t* __restrict__ col,
-                                                     const uint4* __restrict__ tasks, uint32_t ntasks,
-                                                     uint32_t* __restrict__ queue,
-                                                     unsigned long long* __restrict__ result) {
+constexpr int kBlockThreads = 512;
+
+template <int LOGS, int G, int U>
+__global__ __launch_bounds__(kBlockThreads) void k_count_block(const uint32_t* __restrict__ row_ptr,
+                                                               const uint32_t* __restrict__ col,
+                                                               const uint4* __restrict__ tasks, uint32_t ntasks,
+                                                               uint32_t* __restrict__ queue,
+                                                               unsigned long long* __restrict__ result,
+                                                               int attempts) {
   constexpr uint32_t S = 1u << LOGS;
-  constexpr uint64_t kLeads = leader_mask<G>();
-  __shared__ uint32_t T[S];
-  __shared__ uint32_t s_task, s_next;
+  __shared__ uint32_t T[2 * S];
+  __shared__ uint2 R[kChunk];
+  __shared__ uint32_t s_task, s_next, s_fail;
   const uint32_t lane = threadIdx.x & 63;
-  const uint32_t g = lane / G, gl = lane % G;
-  const uint64_t below = (g == 0) ? 0ull : ((1ull << (g * G)) - 1);
   uint64_t acc = 0;
   for (;;) {
     if (threadIdx.x == 0) s_task = atomicAdd(queue, 1u);
@@ -173,42 +270,44 @@ __global__ __launch_bounds__(256) void k_count_block(const uint32_t* __restrict_
     if (t >= ntasks) break;
     const uint4 tk = tasks[t];
     const uint32_t u = tk.x, k0 = tk.y, k1 = tk.z, c = tk.w;
-    const uint32_t r0 = row_ptr[u], d = row_ptr[u + 1] - r0;
+    const uint32_t r0 = row_ptr[u], d = row_ptr[u + 1] - r0, nk = k1 - k0;
     const uint32_t p0 = c * kChunk, p1 = min(p0 + kChunk, d);
     const bool ranged = (p0 > 0) || (p1 < d);
     const uint32_t lo = col[r0 + p0], hi = col[r0 + p1 - 1];
-    for (uint32_t i = threadIdx.x; i < S; i += 256) T[i] = kEmpty;
-    if (threadIdx.x == 0) s_next = k0;
-    __syncthreads();
-    for (uint32_t i = p0 + threadIdx.x; i < p1; i += 256) hash_insert<LOGS>(T, col[r0 + i]);
-    __syncthreads();
-    uint32_t pos = 0, end = 0, cnt = 0;
-    for (;;) {
-      const bool need = pos >= end;
-      const uint64_t want = __ballot(need && gl == 0);
-      const uint32_t nw = (uint32_t)__popcll(want);
-      uint32_t kb = 0;
-      if (nw) {
-        if (lane == 0) kb = atomicAdd(&s_next, nw);
-        kb = __shfl(kb, 0);
-        if (want == kLeads && kb >= k1) break;
+    for (uint32_t k = threadIdx.x; k < nk; k += kBlockThreads) {
+      const uint32_t v = col[r0 + k0 + k];
+      uint32_t b = row_ptr[v], e = row_ptr[v + 1];
+      if (ranged) {
+        b = lower_bound_u32(col, b, e, lo);
+        e = upper_bound_u32(col, b, e, hi);
       }
-      if (need) {
-        const uint32_t k = kb + (uint32_t)__popcll(want & below);
-        if (k < k1) {
-          const uint32_t v = col[r0 + k];
-          pos = row_ptr[v];
-          end = row_ptr[v + 1];
-          if (ranged) {
-            pos = lower_bound_u32(col, pos, end, lo);
-            end = upper_bound_u32(col, pos, end, hi);
-          }
-        }
-      }
-      const uint32_t p = pos + gl;
-      if (p < end) cnt += hash_probe<LOGS>(T, col[p]);
-      pos += G;
+      R[k] = make_uint2(b, e);
     }
+    uint32_t sa = 0, sb = 0;
+    bool ok = false;
+    for (int attempt = 0; attempt < attempts && !ok; ++attempt) {
+      sa = kSeedA[attempt];
+      sb = kSeedB[attempt];
+      for (uint32_t i = threadIdx.x; i < 2 * S; i += kBlockThreads) T[i] = kEmpty;
+      if (threadIdx.x == 0) s_fail = 0;
+      __syncthreads();
+      bool fail = false;
+      for (uint32_t i = p0 + threadIdx.x; i < p1; i += kBlockThreads)
+        fail |= !cuckoo_insert<LOGS>(T, col[r0 + i], sa, sb);
+      if (fail) s_fail = 1;
+      __syncthreads();
+      ok = (s_fail == 0);
+      __syncthreads();
+    }
+    if (!ok) {
+      for (uint32_t i = threadIdx.x; i < 2 * S; i += kBlockThreads) T[i] = kEmpty;
+      __syncthreads();
+      for (uint32_t i = p0 + threadIdx.x; i < p1; i += kBlockThreads) linear_insert<LOGS + 1>(T, col[r0 + i]);
+    }
+    if (threadIdx.x == 0) s_next = 0;
+    __syncthreads();
+    const uint32_t cnt = ok ? stream_ranges<G, U, true>(col, R, nk, &s_next, lane, CuckooProbe<LOGS>{T, sa, sb})
+                            : stream_ranges<G, U, true>(col, R, nk, &s_next, lane, LinearProbe<LOGS + 1>{T});
     acc += cnt;
     __syncthreads();
   }
@@ -229,6 +328,8 @@ __device__ __forceinline__ int row_class(uint32_t d) {
 __device__ __forceinline__ uint32_t row_splits(int cls, uint32_t d, uint64_t w) {
   const uint64_t cap = cls <= 1 ? kWaveCap : kBlockCap;
   uint64_t s = (w + cap - 1) / cap;
+  const uint64_t min_s = (d + kChunk - 1) / kChunk;  // a block task stages at most kChunk ranges
+  if (s < min_s) s = min_s;
   if (s < 1) s = 1;
   if (s > d) s = d;
   return (uint32_t)s;
@@ -295,6 +396,38 @@ struct Uint4Plus {
   }
 };
 
+// Kernel variants (LDS table sizes, lane-group width G, loads in flight U).
+constexpr int kC0Logs = 6, kC0G = 16, kC0U = 8;   // class 0: d <= 32, 2x64 slots
+constexpr int kC1Logs = 9, kC1G = 16, kC1U = 8;   // class 1: d <= 256, 2x512 slots
+constexpr int kC2Logs = 12, kC2G = 16, kC2U = 8;  // classes 2,3: <= 2048 ids, 2x4096 slots
+
+template <int LOGS, int KMAX, int G, int U>
+static void launch_wave(uint32_t ntasks, int cus, hipStream_t s, tribit_graph* g, const uint2* tasks,
+                        uint32_t* queue, unsigned long long* result, int attempts, uint64_t* blocks) {
+  if (!ntasks) return;
+  int per_cu = 0;
+  TB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_count_wave<LOGS, KMAX, G, U>, kBlock, 0));
+  if (per_cu < 1) per_cu = 1;
+  const uint64_t want = (ntasks + 4 * kWaveBatch - 1) / (4 * kWaveBatch);
+  const unsigned grid = (unsigned)std::min<uint64_t>(want, (uint64_t)cus * per_cu);
+  k_count_wave<LOGS, KMAX, G, U><<<grid, kBlock, 0, s>>>(g->row_ptr, g->col, tasks, ntasks, queue, result,
+                                                        attempts);
+  *blocks += grid;
+}
+
+template <int LOGS, int G, int U>
+static void launch_block(uint32_t ntasks, int cus, hipStream_t s, tribit_graph* g, const uint4* tasks,
+                         uint32_t* queue, unsigned long long* result, int attempts, uint64_t* blocks) {
+  if (!ntasks) return;
+  int per_cu = 0;
+  TB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_count_block<LOGS, G, U>, kBlockThreads, 0));
+  if (per_cu < 1) per_cu = 1;
+  const unsigned grid = (unsigned)std::min<uint64_t>(ntasks, (uint64_t)cus * per_cu);
+  k_count_block<LOGS, G, U><<<grid, kBlockThreads, 0, s>>>(g->row_ptr, g->col, tasks, ntasks, queue, result,
+                                                           attempts);
+  *blocks += grid;
+}
+
 static float elapsed(hipEvent_t a, hipEvent_t b) {
   float ms = 0.f;
   TB_HIP(hipEventElapsedTime(&ms, a, b));
@@ -302,8 +435,8 @@ static float elapsed(hipEvent_t a, hipEvent_t b) {
 }
 
 // One scheduled count. Returns the triangle count; fills per-phase times.
-static uint64_t count_once(tribit_ctx* ctx, tribit_graph* g, double* ms_total, double* ms_sched, double* ms_kernel,
-                           double* ms_var, tribit_count_info* info) {
+static uint64_t count_once(tribit_ctx* ctx, tribit_graph* g, int flags, double* ms_total, double* ms_sched,
+                           double* ms_kernel, double* ms_var, tribit_count_info* info) {
   hipStream_t s = ctx->stream;
   const uint64_t n = g->n;
   TB_HIP(hipEventRecord(ctx->ev[2], s));
@@ -337,31 +470,14 @@ static uint64_t count_once(tribit_ctx* ctx, tribit_graph* g, double* ms_total, d
   const int cus = ctx->cus > 0 ? ctx->cus : 256;
   uint64_t blocks = 0;
   // one launch per variant; an event after each gives its device time
-  if (tot.x) {
-    const unsigned grid = (unsigned)std::min<uint64_t>((tot.x + 4 * kWaveBatch - 1) / (4 * kWaveBatch),
-                                                       (uint64_t)cus * 8);
-    k_count_wave<7, 16><<<grid, kBlock, 0, s>>>(g->row_ptr, g->col, wt.p, tot.x, d_queue + 0, d_result);
-    blocks += grid;
-  }
+  const int att = (flags & TRIBIT_COUNT_FORCE_LINEAR_PROBE) ? 0 : kCuckooSeeds;
+  launch_wave<kC0Logs, 32, kC0G, kC0U>(tot.x, cus, s, g, wt.p, d_queue + 0, d_result, att, &blocks);
   TB_HIP(hipEventRecord(ctx->ev[4], s));
-  if (tot.y) {
-    const unsigned grid = (unsigned)std::min<uint64_t>((tot.y + 4 * kWaveBatch - 1) / (4 * kWaveBatch),
-                                                       (uint64_t)cus * 8);
-    k_count_wave<10, 16><<<grid, kBlock, 0, s>>>(g->row_ptr, g->col, wt.p + tot.x, tot.y, d_queue + 1, d_result);
-    blocks += grid;
-  }
+  launch_wave<kC1Logs, 256, kC1G, kC1U>(tot.y, cus, s, g, wt.p + tot.x, d_queue + 1, d_result, att, &blocks);
   TB_HIP(hipEventRecord(ctx->ev[5], s));
-  if (tot.z) {
-    const unsigned grid = (unsigned)std::min<uint64_t>(tot.z, (uint64_t)cus * 4);
-    k_count_block<13, 16><<<grid, kBlock, 0, s>>>(g->row_ptr, g->col, bt.p, tot.z, d_queue + 2, d_result);
-    blocks += grid;
-  }
+  launch_block<kC2Logs, kC2G, kC2U>(tot.z, cus, s, g, bt.p, d_queue + 2, d_result, att, &blocks);
   TB_HIP(hipEventRecord(ctx->ev[6], s));
-  if (tot.w) {
-    const unsigned grid = (unsigned)std::min<uint64_t>(tot.w, (uint64_t)cus * 4);
-    k_count_block<13, 16><<<grid, kBlock, 0, s>>>(g->row_ptr, g->col, bt.p + tot.z, tot.w, d_queue + 3, d_result);
-    blocks += grid;
-  }
+  launch_block<kC2Logs, kC2G, kC2U>(tot.w, cus, s, g, bt.p + tot.z, d_queue + 3, d_result, att, &blocks);
   TB_HIP(hipGetLastError());
   TB_HIP(hipEventRecord(ctx->ev[7], s));
   uint64_t tri = 0;
@@ -399,7 +515,7 @@ void count(tribit_ctx* ctx, tribit_graph* g, const tribit_count_opts& opts, trib
   memset(&local, 0, sizeof(local));
   for (int r = 0; r < reps; ++r) {
     double a, b, c, v[kVariants];
-    const uint64_t tri = count_once(ctx, g, &a, &b, &c, v, &local);
+    const uint64_t tri = count_once(ctx, g, opts.flags, &a, &b, &c, v, &local);
     if (r == 0)
       first = tri;
     else if (tri != first)
