@@ -7,12 +7,15 @@ Workload (default): R-MAT scale 25 (33,554,432 vertices), edge factor 16,
 (a,b,c,d) = (0.57,0.19,0.19,0.05), seed 5 -- generated in HBM by the library,
 symmetrised, deduplicated, self-loops removed, degree-oriented int32 CSR.
 A step is one scheduled exact count of the resident graph (scheduler + counting
-kernels); the count of every step must agree.
+kernels); the count of every step must agree. The selected schedule (default
+"directed": every edge streams the shorter of its two lists) is timed, then
+the "plain" schedule (textbook direction) on the same graph for comparison.
 
 Prints ONE JSON line. value = m / (wall-clock seconds per step) / 1e9 (GTEPS),
 with m the undirected simple edges. roofline: the dominant counting kernel's
 algorithmic bytes 4*(rows+1) + 4*edges + 4*W+ over its own rows, divided by its
-HIP-event device time, against 8 TB/s. cpu_baseline: the OpenMP CPU oracle
+HIP-event device time, against 8 TB/s (4 bytes per owner vertex, per edge and
+per wedge check that launch processes). cpu_baseline: the OpenMP CPU oracle
 (edge iterator, merge intersection) timed on a strided row sample of the same
 CSR, extrapolated to the whole graph by merge work.
 """
@@ -32,7 +35,8 @@ sys.path.insert(0, ROOT)
 
 METRIC = "exact-TC GTEPS on 1 MI355X: learned-sched vs plain; % HBM roofline"
 HBM_PEAK_GBS = 8000.0
-VARIANT_NAMES = ["k_count_wave<7,16>", "k_count_wave<10,16>", "k_count_block<13,16>", "k_count_block<13,16>/chunked"]
+VARIANT_NAMES = ["k_count_wave<6,64,16,8>", "k_count_wave<9,256,16,8>", "k_count_block<12,16,8>",
+                 "k_count_block<12,16,8> (id-range chunks)"]
 
 
 def log(msg: str) -> None:
@@ -90,6 +94,48 @@ def cpu_baseline(g, n: int, m: int, target_s: float) -> dict:
     }
 
 
+def measure(g, sched: str, steps: int, warmup: int):
+    """W untimed counts, then K timed counts (wall clock around all K)."""
+    for i in range(warmup):
+        r = g.count(sched=sched)
+        log(f"{sched} warmup {i}: {r.triangles} triangles, {r.ms_count:.2f} ms (variants {r.ms_by_variant})")
+    results = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        results.append(g.count(sched=sched))
+    wall = time.perf_counter() - t0
+    tri = {r.triangles for r in results}
+    if len(tri) != 1:
+        log(f"{sched}: counts disagree across steps: {tri}")
+        return None
+    nv = len(results[0].ms_by_variant)
+    out = {
+        "triangles": tri.pop(),
+        "ms_per_step": wall / max(steps, 1) * 1e3,
+        "ms_dev": statistics.median(r.ms_count for r in results),
+        "ms_sched": statistics.median(r.ms_sched for r in results),
+        "ms_var": [statistics.median(r.ms_by_variant[v] for r in results) for v in range(nv)],
+        "tasks_by_variant": results[0].tasks_by_variant,
+    }
+    log(f"{sched}: {steps} steps, {out['ms_per_step']:.2f} ms/step wall, {out['ms_dev']:.2f} ms device; "
+        f"variants {[round(x, 3) for x in out['ms_var']]}")
+    return out
+
+
+def copy_calibration() -> int:
+    """A plain device copy of a known size: FETCH_SIZE/WRITE_SIZE of its kernel scale the counters."""
+    import torch
+
+    n = 1 << 28
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4}), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -98,10 +144,15 @@ def main() -> int:
     ap.add_argument("--scale", type=int, default=25)
     ap.add_argument("--ef", type=int, default=16)
     ap.add_argument("--seed", type=int, default=5)
-    ap.add_argument("--sched", choices=["plain", "learned"], default="plain")
+    ap.add_argument("--sched", choices=["plain", "directed", "learned"], default="directed")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain-schedule comparison run")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 1 GiB device copy (reference kernel for HBM byte counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        return copy_calibration()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU path is a design, not a build.",
               file=sys.stderr)
@@ -119,32 +170,25 @@ def main() -> int:
     log(f"graph ready in {time.perf_counter() - t0:.2f} s: n={inf['n']} m={inf['m']} W+={inf['wplus']} "
         f"prep={inf['ms_preprocess']:.1f} ms")
     n, m = inf["n"], inf["m"]
-    work = g.variant_stats(args.sched)
 
-    for i in range(args.warmup):
-        r = g.count(sched=args.sched)
-        log(f"warmup {i}: {r.triangles} triangles, {r.ms_count:.2f} ms (variants {r.ms_by_variant})")
-    tri = None
-    results = []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        r = g.count(sched=args.sched)
-        results.append(r)
-    wall = time.perf_counter() - t0
-    tri = {r.triangles for r in results}
-    if len(tri) != 1:
-        log(f"counts disagree across steps: {tri}")
+    sel = measure(g, args.sched, args.steps, args.warmup)
+    if sel is None:
         return 1
-    tri = tri.pop()
-    ms_per_step = wall / max(args.steps, 1) * 1e3
-    ms_dev = statistics.median(r.ms_count for r in results)
-    nv = len(results[0].ms_by_variant)
-    ms_var = [statistics.median(r.ms_by_variant[v] for r in results) for v in range(nv)]
+    plain = None
+    if args.sched != "plain" and not args.no_plain:
+        plain = measure(g, "plain", args.steps, args.warmup)
+        if plain is None or plain["triangles"] != sel["triangles"]:
+            log("plain and selected schedules disagree")
+            return 1
+    tri = sel["triangles"]
+    ms_per_step = sel["ms_per_step"]
+    ms_dev = sel["ms_dev"]
+    ms_var = sel["ms_var"]
+    work = g.variant_stats(args.sched)
     dom = int(np.argmax(ms_var))
-    b_dom = 4.0 * (work["rows"][dom] + 1) + 4.0 * work["edges"][dom] + 4.0 * work["wplus"][dom]
+    b_dom = 4.0 * work["rows"][dom] + 4.0 * work["edges"][dom] + 4.0 * work["wplus"][dom]
     achieved = b_dom / (ms_var[dom] * 1e-3) / 1e9 if ms_var[dom] > 0 else 0.0
     b_alg = 4.0 * (n + 1) + 4.0 * m + 4.0 * inf["wplus"]
-    log(f"{args.steps} steps: {ms_per_step:.2f} ms/step wall, {ms_dev:.2f} ms device; variants {ms_var}")
 
     out = {
         "metric": METRIC,
@@ -166,13 +210,20 @@ def main() -> int:
         },
         "triangles": tri,
         "ms_count_device": round(ms_dev, 4),
-        "ms_sched_device": round(statistics.median(r.ms_sched for r in results), 4),
+        "ms_sched_device": round(sel["ms_sched"], 4),
         "ms_preprocess_device": round(inf["ms_preprocess"], 3),
         "gteps_device": round(m / (ms_dev * 1e-3) / 1e9, 4),
         "gteps_e2e": round(m / ((ms_dev + inf["ms_preprocess"]) * 1e-3) / 1e9, 4),
-        "b_alg_gbs_whole_count": round(b_alg / (ms_dev * 1e-3) / 1e9, 1),
+        "textbook_b_alg_gbs": round(b_alg / (ms_dev * 1e-3) / 1e9, 1),
         "ms_by_variant": [round(x, 4) for x in ms_var],
-        "tasks_by_variant": results[0].tasks_by_variant,
+        "tasks_by_variant": sel["tasks_by_variant"],
+        "wedge_checks": int(sum(work["wplus"])),
+        "plain": None if plain is None else {
+            "gteps": round(m / (plain["ms_per_step"] * 1e-3) / 1e9, 4),
+            "ms_per_step": round(plain["ms_per_step"], 4),
+            "ms_count_device": round(plain["ms_dev"], 4),
+            "ms_by_variant": [round(x, 4) for x in plain["ms_var"]],
+        },
         "roofline": {
             "bound": "hbm",
             "kernel": VARIANT_NAMES[dom] if dom < len(VARIANT_NAMES) else f"variant{dom}",

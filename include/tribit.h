@@ -65,8 +65,9 @@ typedef struct {
 
 /* Scheduler choice for tribit_count. */
 typedef enum {
-  TRIBIT_SCHED_PLAIN = 0,   /* static degree/density thresholds */
-  TRIBIT_SCHED_LEARNED = 1  /* cost-model driven (falls back to PLAIN without a model) */
+  TRIBIT_SCHED_PLAIN = 0,    /* static degree thresholds; textbook direction: N+(u) is the set, N+(v) streamed */
+  TRIBIT_SCHED_LEARNED = 1,  /* cost-model driven (without a trained model: the DIRECTED heuristic) */
+  TRIBIT_SCHED_DIRECTED = 2  /* static thresholds; every edge streams the shorter of its two lists */
 } tribit_sched;
 
 /* tribit_count_opts.flags */
@@ -93,11 +94,12 @@ typedef struct {
 } tribit_count_info;
 
 /* Work assigned to each kernel variant by a scheduler, in the units of the
- * algorithmic byte count 4*(rows+1) + 4*edges + 4*wplus. */
+ * algorithmic byte count 4*rows + 4*edges + 4*wplus (4 bytes per vertex, per
+ * edge and per wedge check). */
 typedef struct {
-  uint64_t rows[8];        /* rows (outdeg >= 2) */
-  uint64_t edges[8];       /* sum of outdeg(u) over those rows */
-  uint64_t wplus[8];       /* sum of W+(u) over those rows */
+  uint64_t rows[8];        /* owner lists (a vertex and the lists streamed against N+(vertex)) */
+  uint64_t edges[8];       /* edges of those lists */
+  uint64_t wplus[8];       /* streamed elements = wedge checks */
 } tribit_variant_work;
 
 /* ---- context ------------------------------------------------------------ */

@@ -32,9 +32,19 @@ def note(msg):
     sys.stdout.flush()
 
 
-def gpu_count(ctx, keys, n, sched="plain"):
+SCHEDS = ("plain", "directed", "learned")
+
+
+def count_all(g):
+    """Counts with every scheduler; they must agree (each edge is streamed exactly once)."""
+    got = {s: g.count(sched=s).triangles for s in SCHEDS}
+    assert len(set(got.values())) == 1, got
+    return got["plain"]
+
+
+def gpu_count(ctx, keys, n):
     with ctx.graph_from_keys(keys, n) as g:
-        return g.count(sched=sched).triangles
+        return count_all(g)
 
 
 def assert_same_structure(g, og):
@@ -103,7 +113,7 @@ def test_golden_set(gpu_ctx, oracle_mod):
             g = gpu_ctx.graph_from_keys(keys, n)
             okeys = keys
         with g:
-            assert g.count().triangles == r["triangles"], r["name"]
+            assert count_all(g) == r["triangles"], r["name"]
             assert_same_structure(g, oracle_mod.Graph.from_keys(okeys, r["n"]))
 
 
@@ -114,16 +124,17 @@ def test_block_classes_and_chunks(gpu_ctx, oracle_mod):
     k2, t2 = graphs.complete(600, offset=2100)
     keys = np.concatenate([k1, k2])
     with gpu_ctx.graph_from_keys(keys, 2700) as g:
-        r = g.count()
-        assert r.triangles == t1 + t2
-        assert r.tasks_by_variant[3] > 0 and r.tasks_by_variant[2] > 0
+        for sched in SCHEDS:
+            r = g.count(sched=sched)
+            assert r.triangles == t1 + t2
+            assert r.tasks_by_variant[3] > 0 and r.tasks_by_variant[2] > 0
     note("K_4500 (3 chunks)")
     keys, t = graphs.complete(4500)
     assert gpu_count(gpu_ctx, keys, 4500) == t
     note("dense random n=5000 p=0.5")
     keys = graphs.random_graph(5000, 0.5, 3)
     with gpu_ctx.graph_from_keys(keys, 5000) as g:
-        assert g.count().triangles == oracle_mod.count_keys(keys, 5000)
+        assert count_all(g) == oracle_mod.count_keys(keys, 5000)
 
 
 def test_fallback_linear_probe(gpu_ctx, oracle_mod):
@@ -132,22 +143,26 @@ def test_fallback_linear_probe(gpu_ctx, oracle_mod):
     for spec in [(13, 12, 1), (16, 16, 4)]:
         with gpu_ctx.gen_rmat(spec[0], spec[1], seed=spec[2]) as g:
             a = g.count().triangles
-            b = g.count(flags=tribit_amd.FORCE_LINEAR_PROBE).triangles
-            assert a == b
+            for sched in SCHEDS:
+                assert g.count(sched=sched, flags=tribit_amd.FORCE_LINEAR_PROBE).triangles == a
     k1, t1 = graphs.complete(2100)
     with gpu_ctx.graph_from_keys(k1, 2100) as g:
-        assert g.count(flags=tribit_amd.FORCE_LINEAR_PROBE).triangles == t1
+        for sched in SCHEDS:
+            assert g.count(sched=sched, flags=tribit_amd.FORCE_LINEAR_PROBE).triangles == t1
     keys = graphs.random_graph(3000, 0.3, 5)
     with gpu_ctx.graph_from_keys(keys, 3000) as g:
-        assert g.count(flags=tribit_amd.FORCE_LINEAR_PROBE).triangles == oracle_mod.count_keys(keys, 3000)
+        want = oracle_mod.count_keys(keys, 3000)
+        for sched in SCHEDS:
+            assert g.count(sched=sched, flags=tribit_amd.FORCE_LINEAR_PROBE).triangles == want
 
 
 def test_rmat16_repeats(gpu_ctx, oracle_mod):
     note("rmat s16 ef16 seed 4")
     with gpu_ctx.gen_rmat(16, 16, seed=4) as g:
-        r = g.count(repeats=2)
-        og = oracle_mod.Graph.from_keys(oracle_mod.gen_rmat(16, 16, seed=4), 1 << 16)
-        assert r.triangles == og.count() and r.repeats_agree
+        want = oracle_mod.Graph.from_keys(oracle_mod.gen_rmat(16, 16, seed=4), 1 << 16).count()
+        for sched in SCHEDS:
+            r = g.count(sched=sched, repeats=2)
+            assert r.triangles == want and r.repeats_agree
 
 
 def test_edge_cases(gpu_ctx):
@@ -184,6 +199,7 @@ def test_rmat20_exact(gpu_ctx, oracle_mod):
         r = g.count()
         tri = r.triangles
         assert r.tasks > g.info()["rows_active"]  # rows with large W+ were split into several tasks
+        assert count_all(g) == tri
         og = oracle_mod.Graph.from_keys(oracle_mod.gen_rmat(20, 16, seed=5), 1 << 20)
         inf = g.info()
         assert (inf["m"], inf["wplus"]) == (og.m, og.wplus())
@@ -196,6 +212,7 @@ def test_rmat22_properties(gpu_ctx, oracle_mod):
     with gpu_ctx.gen_rmat(22, 16, seed=2) as g:
         r = g.count(repeats=3)
         assert r.repeats_agree
+        assert g.count(sched="directed", repeats=2).triangles == r.triangles
         inf = g.info()
     okeys = oracle_mod.gen_rmat(22, 16, seed=2)
     og = oracle_mod.Graph.from_keys(okeys, 1 << 22)
@@ -204,7 +221,7 @@ def test_rmat22_properties(gpu_ctx, oracle_mod):
     perm = graphs.permute(okeys, 1 << 22, 7)
     del okeys
     with gpu_ctx.graph_from_keys(perm, 1 << 22) as g2:
-        assert g2.count().triangles == r.triangles
+        assert g2.count(sched="directed").triangles == r.triangles
 
 
 def test_cli_wiki_vote_stdout_contract():

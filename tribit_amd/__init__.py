@@ -28,7 +28,7 @@ STATUS = {
     0: "OK", 1: "ERR_IO", 2: "ERR_FORMAT", 3: "ERR_SHAPE", 4: "ERR_RANGE", 5: "ERR_NOMEM",
     6: "ERR_ARG", 7: "ERR_HIP", 8: "ERR_NODEVICE", 9: "ERR_TOO_LARGE",
 }
-SCHED = {"plain": 0, "learned": 1}
+SCHED = {"plain": 0, "learned": 1, "directed": 2}
 
 
 class TribitError(RuntimeError):

@@ -37,8 +37,15 @@ void free_graph_memory(tribit_graph* g) {
   g->ctx->free(g->col);
   g->ctx->free(g->deg);
   g->ctx->free(g->wrow);
-  g->row_ptr = g->col = g->deg = nullptr;
-  g->wrow = nullptr;
+  g->ctx->free(g->rng_all);
+  g->ctx->free(g->rng_dir);
+  g->ctx->free(g->fw_ptr);
+  g->ctx->free(g->rv_ptr);
+  g->ctx->free(g->w_fw);
+  g->ctx->free(g->w_rv);
+  g->row_ptr = g->col = g->deg = g->fw_ptr = g->rv_ptr = nullptr;
+  g->wrow = g->w_fw = g->w_rv = nullptr;
+  g->rng_all = g->rng_dir = nullptr;
 }
 
 }  // namespace
@@ -267,7 +274,9 @@ int tribit_count(tribit_ctx* ctx, tribit_graph* g, const tribit_count_opts* opts
   tribit_count_opts o;
   memset(&o, 0, sizeof(o));
   if (opts) o = *opts;
-  if (o.sched != TRIBIT_SCHED_PLAIN && o.sched != TRIBIT_SCHED_LEARNED) return TRIBIT_ERR_ARG;
+  if (o.sched != TRIBIT_SCHED_PLAIN && o.sched != TRIBIT_SCHED_LEARNED && o.sched != TRIBIT_SCHED_DIRECTED)
+    return TRIBIT_ERR_ARG;
+  if (o.sched == TRIBIT_SCHED_LEARNED) o.sched = TRIBIT_SCHED_DIRECTED;  // no trained model yet
   return guarded(ctx, [&] {
     TB_HIP(hipSetDevice(ctx->device));
     tribit_count_info local;
@@ -279,6 +288,8 @@ int tribit_count(tribit_ctx* ctx, tribit_graph* g, const tribit_count_opts* opts
 
 int tribit_variant_stats(tribit_ctx* ctx, tribit_graph* g, int sched, tribit_variant_work* out) {
   if (!ctx || !g || !out || g->ctx != ctx) return TRIBIT_ERR_ARG;
+  if (sched < TRIBIT_SCHED_PLAIN || sched > TRIBIT_SCHED_DIRECTED) return TRIBIT_ERR_ARG;
+  if (sched == TRIBIT_SCHED_LEARNED) sched = TRIBIT_SCHED_DIRECTED;
   return guarded(ctx, [&] {
     TB_HIP(hipSetDevice(ctx->device));
     tb::variant_stats(ctx, g, sched, out);

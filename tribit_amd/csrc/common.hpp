@@ -133,6 +133,14 @@ struct tribit_graph {
   uint32_t* col = nullptr;      // m, ascending ids within a row
   uint32_t* deg = nullptr;      // n, undirected degree
   uint64_t* wrow = nullptr;     // n, W+(u) = sum of outdeg(v) over v in N+(u)
+  // streamed-range lists (see prep.hip): forward-only and direction-split
+  uint2* rng_all = nullptr;     // m, in CSR order; owner u's ranges at row_ptr[u]
+  uint2* rng_dir = nullptr;     // m: forward part [0, m_fwd), reverse part [m_fwd, m)
+  uint32_t* fw_ptr = nullptr;   // n+1, owner's forward ranges in rng_dir
+  uint32_t* rv_ptr = nullptr;   // n+1, owner's reverse ranges in rng_dir (offsets include m_fwd)
+  uint64_t* w_fw = nullptr;     // n, streamed elements of the owner's forward ranges
+  uint64_t* w_rv = nullptr;     // n, streamed elements of the owner's reverse ranges
+  uint64_t m_fwd = 0;
   double ms_prep = 0.0;
   uint64_t bytes = 0;
 };

@@ -181,13 +181,15 @@ __device__ __forceinline__ uint32_t stream_ranges(const uint32_t* __restrict__ c
 }
 
 // ------------------------------------------------------ wave-level tasks
-// A wave owns one task (u, k0, k1): the set N+(u) and the ranges of
-// N+(v), v = N+(u)[k] for k in [k0, k1), both in the wave's LDS slice.
+// A task (x, rbase, rcount, chunk) pairs the set N+(x) with the lists whose
+// ranges are rng[rbase, rbase + rcount). A wave owns one task: it builds the
+// set in its LDS slice once, then stages the ranges KMAX at a time.
 
 template <int LOGS, int KMAX, int G, int U>
 __global__ __launch_bounds__(256) void k_count_wave(const uint32_t* __restrict__ row_ptr,
                                                     const uint32_t* __restrict__ col,
-                                                    const uint2* __restrict__ tasks, uint32_t ntasks,
+                                                    const uint2* __restrict__ rng,
+                                                    const uint4* __restrict__ tasks, uint32_t ntasks,
                                                     uint32_t* __restrict__ queue,
                                                     unsigned long long* __restrict__ result, int attempts) {
   constexpr uint32_t S = 1u << LOGS;
@@ -204,13 +206,9 @@ __global__ __launch_bounds__(256) void k_count_wave(const uint32_t* __restrict__
     if (tb >= ntasks) break;
     const uint32_t tend = min(tb + kWaveBatch, ntasks);
     for (uint32_t t = tb; t < tend; ++t) {
-      const uint2 tk = tasks[t];
-      const uint32_t u = tk.x, k0 = tk.y & 0xFFFFu, k1 = tk.y >> 16;
-      const uint32_t r0 = row_ptr[u], d = row_ptr[u + 1] - r0, nk = k1 - k0;
-      for (uint32_t k = lane; k < nk; k += 64) {
-        const uint32_t v = col[r0 + k0 + k];
-        R[k] = make_uint2(row_ptr[v], row_ptr[v + 1]);
-      }
+      const uint4 tk = tasks[t];
+      const uint32_t x = tk.x, rbase = tk.y, rcount = tk.z;
+      const uint32_t r0 = row_ptr[x], d = row_ptr[x + 1] - r0;
       uint32_t sa = 0, sb = 0;
       bool ok = false;
       for (int attempt = 0; attempt < attempts && !ok; ++attempt) {
@@ -223,18 +221,21 @@ __global__ __launch_bounds__(256) void k_count_wave(const uint32_t* __restrict__
         wave_lds_sync();
         ok = !__any(fail);
       }
-      uint32_t cnt;
-      if (ok) {
-        cnt = stream_ranges<G, U, false>(col, R, nk, nullptr, lane, CuckooProbe<LOGS>{T, sa, sb});
-      } else {
+      if (!ok) {
         for (uint32_t i = lane; i < 2 * S; i += 64) T[i] = kEmpty;
         wave_lds_sync();
         for (uint32_t i = lane; i < d; i += 64) linear_insert<LOGS + 1>(T, col[r0 + i]);
+      }
+      uint32_t cnt = 0;
+      for (uint32_t c0 = 0; c0 < rcount; c0 += KMAX) {
+        const uint32_t nk = min((uint32_t)KMAX, rcount - c0);
+        for (uint32_t k = lane; k < nk; k += 64) R[k] = rng[rbase + c0 + k];
         wave_lds_sync();
-        cnt = stream_ranges<G, U, false>(col, R, nk, nullptr, lane, LinearProbe<LOGS + 1>{T});
+        cnt += ok ? stream_ranges<G, U, false>(col, R, nk, nullptr, lane, CuckooProbe<LOGS>{T, sa, sb})
+                  : stream_ranges<G, U, false>(col, R, nk, nullptr, lane, LinearProbe<LOGS + 1>{T});
+        wave_lds_sync();
       }
       acc += cnt;
-      wave_lds_sync();
     }
   }
   acc = wave_sum_u64(acc);
@@ -242,17 +243,18 @@ __global__ __launch_bounds__(256) void k_count_wave(const uint32_t* __restrict__
 }
 
 // ----------------------------------------------------- block-level tasks
-// A 512-thread block owns one task (u, k0, k1, chunk c): the probe set is
-// N+(u)[c*kChunk, ...) (at most kChunk ids), the ranges of N+(v) for
-// k in [k0, k1) (at most kChunk); its 8 waves share the ranges through an
-// LDS counter. When the row has more than one chunk, each N+(v) range is cut
-// to the chunk's id interval [lo, hi] while it is staged (both are sorted).
+// A 512-thread block owns one task: the set is the chunk
+// N+(x)[chunk*kChunk, ...) (at most kChunk ids) and the ranges are staged
+// kChunk at a time; its 8 waves share each staged batch through an LDS
+// counter. When x has more than one chunk, every staged range is cut to the
+// chunk's id interval [lo, hi] (lists are sorted by id).
 
 constexpr int kBlockThreads = 512;
 
 template <int LOGS, int G, int U>
 __global__ __launch_bounds__(kBlockThreads) void k_count_block(const uint32_t* __restrict__ row_ptr,
                                                                const uint32_t* __restrict__ col,
+                                                               const uint2* __restrict__ rng,
                                                                const uint4* __restrict__ tasks, uint32_t ntasks,
                                                                uint32_t* __restrict__ queue,
                                                                unsigned long long* __restrict__ result,
@@ -269,20 +271,11 @@ __global__ __launch_bounds__(kBlockThreads) void k_count_block(const uint32_t* _
     const uint32_t t = s_task;
     if (t >= ntasks) break;
     const uint4 tk = tasks[t];
-    const uint32_t u = tk.x, k0 = tk.y, k1 = tk.z, c = tk.w;
-    const uint32_t r0 = row_ptr[u], d = row_ptr[u + 1] - r0, nk = k1 - k0;
+    const uint32_t x = tk.x, rbase = tk.y, rcount = tk.z, c = tk.w;
+    const uint32_t r0 = row_ptr[x], d = row_ptr[x + 1] - r0;
     const uint32_t p0 = c * kChunk, p1 = min(p0 + kChunk, d);
     const bool ranged = (p0 > 0) || (p1 < d);
     const uint32_t lo = col[r0 + p0], hi = col[r0 + p1 - 1];
-    for (uint32_t k = threadIdx.x; k < nk; k += kBlockThreads) {
-      const uint32_t v = col[r0 + k0 + k];
-      uint32_t b = row_ptr[v], e = row_ptr[v + 1];
-      if (ranged) {
-        b = lower_bound_u32(col, b, e, lo);
-        e = upper_bound_u32(col, b, e, hi);
-      }
-      R[k] = make_uint2(b, e);
-    }
     uint32_t sa = 0, sb = 0;
     bool ok = false;
     for (int attempt = 0; attempt < attempts && !ok; ++attempt) {
@@ -304,83 +297,121 @@ __global__ __launch_bounds__(kBlockThreads) void k_count_block(const uint32_t* _
       __syncthreads();
       for (uint32_t i = p0 + threadIdx.x; i < p1; i += kBlockThreads) linear_insert<LOGS + 1>(T, col[r0 + i]);
     }
-    if (threadIdx.x == 0) s_next = 0;
-    __syncthreads();
-    const uint32_t cnt = ok ? stream_ranges<G, U, true>(col, R, nk, &s_next, lane, CuckooProbe<LOGS>{T, sa, sb})
-                            : stream_ranges<G, U, true>(col, R, nk, &s_next, lane, LinearProbe<LOGS + 1>{T});
+    uint32_t cnt = 0;
+    for (uint32_t c0 = 0; c0 < rcount; c0 += kChunk) {
+      const uint32_t nk = min((uint32_t)kChunk, rcount - c0);
+      for (uint32_t k = threadIdx.x; k < nk; k += kBlockThreads) {
+        uint2 r = rng[rbase + c0 + k];
+        if (ranged) {
+          r.x = lower_bound_u32(col, r.x, r.y, lo);
+          r.y = upper_bound_u32(col, r.x, r.y, hi);
+        }
+        R[k] = r;
+      }
+      if (threadIdx.x == 0) s_next = 0;
+      __syncthreads();
+      cnt += ok ? stream_ranges<G, U, true>(col, R, nk, &s_next, lane, CuckooProbe<LOGS>{T, sa, sb})
+                : stream_ranges<G, U, true>(col, R, nk, &s_next, lane, LinearProbe<LOGS + 1>{T});
+      __syncthreads();
+    }
     acc += cnt;
-    __syncthreads();
   }
   acc = wave_sum_u64(acc);
   if (lane == 0 && acc) atomicAdd(result, (unsigned long long)acc);
 }
 
 // ------------------------------------------------------------- scheduler
+// A schedule is a set of owner lists: (x, rbase, rcount, work) where the
+// ranges rng[rbase, rbase+rcount) are streamed against N+(x).
+//   PLAIN:    one forward list per x over rng_all (the textbook edge
+//             iterator: N+(u) is the set, every N+(v) is streamed).
+//   DIRECTED: per x its forward list and its reverse list over rng_dir
+//             (every edge streams the shorter of its two lists).
+// The owner's outdeg picks the class (kernel variant); its work (streamed
+// elements) picks the number of tasks the list is split into.
 
 __device__ __forceinline__ int row_class(uint32_t d) {
-  if (d < 2) return -1;
+  if (d == 0) return -1;
   if (d <= 32) return 0;
   if (d <= 256) return 1;
   if (d <= kChunk) return 2;
   return 3;
 }
 
-__device__ __forceinline__ uint32_t row_splits(int cls, uint32_t d, uint64_t w) {
+__device__ __forceinline__ uint32_t list_splits(int cls, uint32_t count, uint64_t w) {
   const uint64_t cap = cls <= 1 ? kWaveCap : kBlockCap;
   uint64_t s = (w + cap - 1) / cap;
-  const uint64_t min_s = (d + kChunk - 1) / kChunk;  // a block task stages at most kChunk ranges
-  if (s < min_s) s = min_s;
   if (s < 1) s = 1;
-  if (s > d) s = d;
+  if (s > count) s = count;
   return (uint32_t)s;
 }
 
-// Per row: number of tasks in its class slot (uint4 = 4 class counters).
-__global__ void k_sched_count(const uint32_t* __restrict__ row_ptr, const uint64_t* __restrict__ wrow, uint64_t n,
-                              uint4* __restrict__ cnt) {
-  const uint64_t u = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (u > n) return;
-  uint4 c = make_uint4(0, 0, 0, 0);
-  if (u < n) {
-    const uint32_t d = row_ptr[u + 1] - row_ptr[u];
-    const int cls = row_class(d);
-    if (cls >= 0) {
-      uint32_t t = row_splits(cls, d, wrow[u]);
-      if (cls == 3) t *= (d + kChunk - 1) / kChunk;
-      if (cls == 0) c.x = t;
-      else if (cls == 1) c.y = t;
-      else if (cls == 2) c.z = t;
-      else c.w = t;
-    }
-  }
-  cnt[u] = c;
+struct SchedView {
+  const uint32_t* row_ptr;
+  const uint32_t* fw_ptr;  // list 0 offsets (PLAIN: row_ptr)
+  const uint32_t* rv_ptr;  // list 1 offsets (PLAIN: unused)
+  const uint64_t* w_fw;    // list 0 work (PLAIN: wrow)
+  const uint64_t* w_rv;
+  int nlists;              // 1 (PLAIN) or 2 (DIRECTED)
+};
+
+__device__ __forceinline__ void owner_list(const SchedView& v, uint64_t x, int l, uint32_t* base, uint32_t* count,
+                                           uint64_t* work) {
+  const uint32_t* p = (l == 0) ? v.fw_ptr : v.rv_ptr;
+  *base = p[x];
+  *count = p[x + 1] - p[x];
+  *work = (l == 0) ? v.w_fw[x] : v.w_rv[x];
 }
 
-// Writes each row's tasks at its scanned offsets. Wave tasks (classes 0,1)
-// go to wtasks (class 1 after all class-0 tasks), block tasks (2,3) to btasks.
-__global__ void k_sched_fill(const uint32_t* __restrict__ row_ptr, const uint64_t* __restrict__ wrow, uint64_t n,
-                             const uint4* __restrict__ off, uint4 tot, uint2* __restrict__ wtasks,
-                             uint4* __restrict__ btasks) {
-  const uint64_t u = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (u >= n) return;
-  const uint32_t d = row_ptr[u + 1] - row_ptr[u];
+__device__ __forceinline__ bool list_useful(int l, uint32_t d, uint32_t count) {
+  // a forward list needs |N+(x)| >= 2 (its partner v is in N+(x) but not in N+(v))
+  return count > 0 && d >= (l == 0 ? 2u : 1u);
+}
+
+// Per owner: number of tasks in its class slot (uint4 = 4 class counters).
+__global__ void k_sched_count(SchedView sv, uint64_t n, uint4* __restrict__ cnt) {
+  const uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (x > n) return;
+  uint32_t c4[4] = {0, 0, 0, 0};
+  if (x < n) {
+    const uint32_t d = sv.row_ptr[x + 1] - sv.row_ptr[x];
+    const int cls = row_class(d);
+    if (cls >= 0) {
+      const uint32_t nch = (cls == 3) ? (d + kChunk - 1) / kChunk : 1u;
+      for (int l = 0; l < sv.nlists; ++l) {
+        uint32_t b, k;
+        uint64_t w;
+        owner_list(sv, x, l, &b, &k, &w);
+        if (list_useful(l, d, k)) c4[cls] += list_splits(cls, k, w) * nch;
+      }
+    }
+  }
+  cnt[x] = make_uint4(c4[0], c4[1], c4[2], c4[3]);
+}
+
+// Writes each owner's tasks at its scanned offsets: wave tasks (classes 0, 1)
+// into wt (class 1 after all class-0 tasks), block tasks (2, 3) into bt.
+__global__ void k_sched_fill(SchedView sv, uint64_t n, const uint4* __restrict__ off, uint4 tot,
+                             uint4* __restrict__ wt, uint4* __restrict__ bt) {
+  const uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= n) return;
+  const uint32_t d = sv.row_ptr[x + 1] - sv.row_ptr[x];
   const int cls = row_class(d);
   if (cls < 0) return;
-  const uint32_t s = row_splits(cls, d, wrow[u]);
-  const uint4 o = off[u];
-  if (cls <= 1) {
-    const uint32_t b = (cls == 0) ? o.x : tot.x + o.y;
-    for (uint32_t i = 0; i < s; ++i) {
-      const uint32_t k0 = (uint32_t)((uint64_t)d * i / s), k1 = (uint32_t)((uint64_t)d * (i + 1) / s);
-      wtasks[b + i] = make_uint2((uint32_t)u, k0 | (k1 << 16));
-    }
-  } else {
-    const uint32_t nch = (cls == 3) ? (d + kChunk - 1) / kChunk : 1u;
-    const uint32_t b = (cls == 2) ? o.z : tot.z + o.w;
+  const uint4 o = off[x];
+  uint4* out = (cls == 0) ? wt + o.x : (cls == 1) ? wt + tot.x + o.y : (cls == 2) ? bt + o.z : bt + tot.z + o.w;
+  const uint32_t nch = (cls == 3) ? (d + kChunk - 1) / kChunk : 1u;
+  uint32_t j = 0;
+  for (int l = 0; l < sv.nlists; ++l) {
+    uint32_t b, k;
+    uint64_t w;
+    owner_list(sv, x, l, &b, &k, &w);
+    if (!list_useful(l, d, k)) continue;
+    const uint32_t s = list_splits(cls, k, w);
     for (uint32_t c = 0; c < nch; ++c)
       for (uint32_t i = 0; i < s; ++i) {
-        const uint32_t k0 = (uint32_t)((uint64_t)d * i / s), k1 = (uint32_t)((uint64_t)d * (i + 1) / s);
-        btasks[b + c * s + i] = make_uint4((uint32_t)u, k0, k1, c);
+        const uint32_t k0 = (uint32_t)((uint64_t)k * i / s), k1 = (uint32_t)((uint64_t)k * (i + 1) / s);
+        out[j++] = make_uint4((uint32_t)x, b + k0, k1 - k0, c);
       }
   }
 }
@@ -396,34 +427,59 @@ struct Uint4Plus {
   }
 };
 
+static SchedView sched_view(const tribit_graph* g, int sched) {
+  SchedView v;
+  v.row_ptr = g->row_ptr;
+  if (sched == TRIBIT_SCHED_PLAIN) {
+    v.fw_ptr = g->row_ptr;
+    v.rv_ptr = g->row_ptr;
+    v.w_fw = g->wrow;
+    v.w_rv = g->wrow;
+    v.nlists = 1;
+  } else {
+    v.fw_ptr = g->fw_ptr;
+    v.rv_ptr = g->rv_ptr;
+    v.w_fw = g->w_fw;
+    v.w_rv = g->w_rv;
+    v.nlists = 2;
+  }
+  return v;
+}
+
+static const uint2* sched_ranges(const tribit_graph* g, int sched) {
+  return sched == TRIBIT_SCHED_PLAIN ? g->rng_all : g->rng_dir;
+}
+
 // Kernel variants (LDS table sizes, lane-group width G, loads in flight U).
-constexpr int kC0Logs = 6, kC0G = 16, kC0U = 8;   // class 0: d <= 32, 2x64 slots
-constexpr int kC1Logs = 9, kC1G = 16, kC1U = 8;   // class 1: d <= 256, 2x512 slots
-constexpr int kC2Logs = 12, kC2G = 16, kC2U = 8;  // classes 2,3: <= 2048 ids, 2x4096 slots
+constexpr int kC0Logs = 6, kC0K = 64, kC0G = 16, kC0U = 8;    // class 0: d <= 32, 2x64 slots
+constexpr int kC1Logs = 9, kC1K = 256, kC1G = 16, kC1U = 8;   // class 1: d <= 256, 2x512 slots
+constexpr int kC2Logs = 12, kC2G = 16, kC2U = 8;              // classes 2,3: <= 2048 ids, 2x4096 slots
 
 template <int LOGS, int KMAX, int G, int U>
-static void launch_wave(uint32_t ntasks, int cus, hipStream_t s, tribit_graph* g, const uint2* tasks,
-                        uint32_t* queue, unsigned long long* result, int attempts, uint64_t* blocks) {
+static void launch_wave(uint32_t ntasks, int cus, hipStream_t s, tribit_graph* g, const uint2* rng,
+                        const uint4* tasks, uint32_t* queue, unsigned long long* result, int attempts,
+                        uint64_t* blocks) {
   if (!ntasks) return;
   int per_cu = 0;
   TB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_count_wave<LOGS, KMAX, G, U>, kBlock, 0));
   if (per_cu < 1) per_cu = 1;
   const uint64_t want = (ntasks + 4 * kWaveBatch - 1) / (4 * kWaveBatch);
   const unsigned grid = (unsigned)std::min<uint64_t>(want, (uint64_t)cus * per_cu);
-  k_count_wave<LOGS, KMAX, G, U><<<grid, kBlock, 0, s>>>(g->row_ptr, g->col, tasks, ntasks, queue, result,
+  k_count_wave<LOGS, KMAX, G, U><<<grid, kBlock, 0, s>>>(g->row_ptr, g->col, rng, tasks, ntasks, queue, result,
                                                         attempts);
   *blocks += grid;
 }
 
 template <int LOGS, int G, int U>
-static void launch_block(uint32_t ntasks, int cus, hipStream_t s, tribit_graph* g, const uint4* tasks,
-                         uint32_t* queue, unsigned long long* result, int attempts, uint64_t* blocks) {
+static void launch_block(uint32_t ntasks, int cus, hipStream_t s, tribit_graph* g, const uint2* rng,
+                         const uint4* tasks, uint32_t* queue, unsigned long long* result, int attempts,
+                         uint64_t* blocks) {
   if (!ntasks) return;
   int per_cu = 0;
   TB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_count_block<LOGS, G, U>, kBlockThreads, 0));
   if (per_cu < 1) per_cu = 1;
   const unsigned grid = (unsigned)std::min<uint64_t>(ntasks, (uint64_t)cus * per_cu);
-  k_count_block<LOGS, G, U><<<grid, kBlockThreads, 0, s>>>(g->row_ptr, g->col, tasks, ntasks, queue, result,
+  k_count_block<LOGS, G, U><<<grid, kBlockThreads, 0, s>>>(g->row_ptr, g->col, rng, tasks, ntasks, queue, result,
                                                            attempts);
   *blocks += grid;
 }
@@ -435,8 +491,8 @@ static float elapsed(hipEvent_t a, hipEvent_t b) {
 }
 
 // One scheduled count. Returns the triangle count; fills per-phase times.
-static uint64_t count_once(tribit_ctx* ctx, tribit_graph* g, int flags, double* ms_total, double* ms_sched,
-                           double* ms_kernel, double* ms_var, tribit_count_info* info) {
+static uint64_t count_once(tribit_ctx* ctx, tribit_graph* g, int sched, int flags, double* ms_total,
+                           double* ms_sched, double* ms_kernel, double* ms_var, tribit_count_info* info) {
   hipStream_t s = ctx->stream;
   const uint64_t n = g->n;
   TB_HIP(hipEventRecord(ctx->ev[2], s));
@@ -447,7 +503,9 @@ static uint64_t count_once(tribit_ctx* ctx, tribit_graph* g, int flags, double* 
   uint32_t* d_queue = reinterpret_cast<uint32_t*>(scratch.p + 4);
 
   DevBuf<uint4> cnt(ctx, n + 1), off(ctx, n + 1);
-  k_sched_count<<<grid_for(n + 1, kBlock), kBlock, 0, s>>>(g->row_ptr, g->wrow, n, cnt.p);
+  const SchedView sv = sched_view(g, sched);
+  const uint2* rng = sched_ranges(g, sched);
+  k_sched_count<<<grid_for(n + 1, kBlock), kBlock, 0, s>>>(sv, n, cnt.p);
   {
     size_t bytes = 0;
     TB_HIP(rocprim::exclusive_scan(nullptr, bytes, cnt.p, off.p, make_uint4(0, 0, 0, 0), (size_t)(n + 1),
@@ -460,9 +518,9 @@ static uint64_t count_once(tribit_ctx* ctx, tribit_graph* g, int flags, double* 
   TB_HIP(hipMemcpyAsync(&tot, off.p + n, sizeof(uint4), hipMemcpyDeviceToHost, s));
   ctx->sync();
   const uint64_t nw = (uint64_t)tot.x + tot.y, nb = (uint64_t)tot.z + tot.w;
-  DevBuf<uint2> wt(ctx, nw ? nw : 1);
+  DevBuf<uint4> wt(ctx, nw ? nw : 1);
   DevBuf<uint4> bt(ctx, nb ? nb : 1);
-  k_sched_fill<<<grid_for(n, kBlock), kBlock, 0, s>>>(g->row_ptr, g->wrow, n, off.p, tot, wt.p, bt.p);
+  k_sched_fill<<<grid_for(n, kBlock), kBlock, 0, s>>>(sv, n, off.p, tot, wt.p, bt.p);
   cnt.reset();
   off.reset();
   TB_HIP(hipEventRecord(ctx->ev[3], s));
@@ -471,13 +529,13 @@ static uint64_t count_once(tribit_ctx* ctx, tribit_graph* g, int flags, double* 
   uint64_t blocks = 0;
   // one launch per variant; an event after each gives its device time
   const int att = (flags & TRIBIT_COUNT_FORCE_LINEAR_PROBE) ? 0 : kCuckooSeeds;
-  launch_wave<kC0Logs, 32, kC0G, kC0U>(tot.x, cus, s, g, wt.p, d_queue + 0, d_result, att, &blocks);
+  launch_wave<kC0Logs, kC0K, kC0G, kC0U>(tot.x, cus, s, g, rng, wt.p, d_queue + 0, d_result, att, &blocks);
   TB_HIP(hipEventRecord(ctx->ev[4], s));
-  launch_wave<kC1Logs, 256, kC1G, kC1U>(tot.y, cus, s, g, wt.p + tot.x, d_queue + 1, d_result, att, &blocks);
+  launch_wave<kC1Logs, kC1K, kC1G, kC1U>(tot.y, cus, s, g, rng, wt.p + tot.x, d_queue + 1, d_result, att, &blocks);
   TB_HIP(hipEventRecord(ctx->ev[5], s));
-  launch_block<kC2Logs, kC2G, kC2U>(tot.z, cus, s, g, bt.p, d_queue + 2, d_result, att, &blocks);
+  launch_block<kC2Logs, kC2G, kC2U>(tot.z, cus, s, g, rng, bt.p, d_queue + 2, d_result, att, &blocks);
   TB_HIP(hipEventRecord(ctx->ev[6], s));
-  launch_block<kC2Logs, kC2G, kC2U>(tot.w, cus, s, g, bt.p + tot.z, d_queue + 3, d_result, att, &blocks);
+  launch_block<kC2Logs, kC2G, kC2U>(tot.w, cus, s, g, rng, bt.p + tot.z, d_queue + 3, d_result, att, &blocks);
   TB_HIP(hipGetLastError());
   TB_HIP(hipEventRecord(ctx->ev[7], s));
   uint64_t tri = 0;
@@ -515,7 +573,7 @@ void count(tribit_ctx* ctx, tribit_graph* g, const tribit_count_opts& opts, trib
   memset(&local, 0, sizeof(local));
   for (int r = 0; r < reps; ++r) {
     double a, b, c, v[kVariants];
-    const uint64_t tri = count_once(ctx, g, opts.flags, &a, &b, &c, v, &local);
+    const uint64_t tri = count_once(ctx, g, opts.sched, opts.flags, &a, &b, &c, v, &local);
     if (r == 0)
       first = tri;
     else if (tri != first)
@@ -534,20 +592,26 @@ void count(tribit_ctx* ctx, tribit_graph* g, const tribit_count_opts& opts, trib
   if (info) *info = local;
 }
 
-// Per-variant work of the plain schedule: rows, sum outdeg, sum W+.
-__global__ void k_variant_stats(const uint32_t* __restrict__ row_ptr, const uint64_t* __restrict__ wrow, uint64_t n,
-                                unsigned long long* __restrict__ st) {
+// Per-variant work of a schedule: owner lists, partners (edges), streamed
+// elements (wedge checks).
+__global__ void k_variant_stats(SchedView sv, uint64_t n, unsigned long long* __restrict__ st) {
   __shared__ unsigned long long acc[3 * kVariants];
   if (threadIdx.x < 3 * kVariants) acc[threadIdx.x] = 0;
   __syncthreads();
-  const uint64_t u = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (u < n) {
-    const uint32_t d = row_ptr[u + 1] - row_ptr[u];
+  const uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (x < n) {
+    const uint32_t d = sv.row_ptr[x + 1] - sv.row_ptr[x];
     const int c = row_class(d);
     if (c >= 0) {
-      atomicAdd(&acc[c], 1ull);
-      atomicAdd(&acc[kVariants + c], (unsigned long long)d);
-      atomicAdd(&acc[2 * kVariants + c], (unsigned long long)wrow[u]);
+      for (int l = 0; l < sv.nlists; ++l) {
+        uint32_t b, k;
+        uint64_t w;
+        owner_list(sv, x, l, &b, &k, &w);
+        if (!list_useful(l, d, k)) continue;
+        atomicAdd(&acc[c], 1ull);
+        atomicAdd(&acc[kVariants + c], (unsigned long long)k);
+        atomicAdd(&acc[2 * kVariants + c], (unsigned long long)w);
+      }
     }
   }
   __syncthreads();
@@ -555,11 +619,11 @@ __global__ void k_variant_stats(const uint32_t* __restrict__ row_ptr, const uint
 }
 
 void variant_stats(tribit_ctx* ctx, tribit_graph* g, int sched, tribit_variant_work* out) {
-  (void)sched;
   memset(out, 0, sizeof(*out));
   DevBuf<unsigned long long> st(ctx, 3 * kVariants);
   k_zero_u64<<<1, 64, 0, ctx->stream>>>(st.p, 3 * kVariants);
-  if (g->n) k_variant_stats<<<grid_for(g->n, kBlock), kBlock, 0, ctx->stream>>>(g->row_ptr, g->wrow, g->n, st.p);
+  if (g->n)
+    k_variant_stats<<<grid_for(g->n, kBlock), kBlock, 0, ctx->stream>>>(sched_view(g, sched), g->n, st.p);
   unsigned long long h[3 * kVariants];
   TB_HIP(hipMemcpyAsync(h, st.p, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
   ctx->sync();

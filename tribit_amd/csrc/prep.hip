@@ -1,5 +1,5 @@
 // prep.hip -- on-GPU generators and preprocessing (edge keys in HBM ->
-// degree-oriented int32 CSR + per-row wedge counts), gfx950.
+// degree-oriented int32 CSR, per-row wedge counts, streamed-range lists), gfx950.
 //
 // Replaces the reference's preprocessing chain, src/single_gpu/tribit.cu:
 // 664-763 (radix sort + unique of packed keys, L/U bit structures, singleton
@@ -166,6 +166,114 @@ __global__ void k_row_stats(const uint32_t* __restrict__ row_ptr, const uint32_t
   }
 }
 
+// ------------------------------------------------------------ range lists
+// For every oriented edge u->v the count streams one list: N+(v) against the
+// set N+(u) ("forward", owner u) or N+(u) against the set N+(v) ("reverse",
+// owner v). The streamed list is stored as its [begin, end) range in col, so
+// the counting kernels read 8 coalesced bytes per edge instead of gathering
+// row_ptr. Two layouts are built:
+//   rng_all: all edges forward, in CSR order (owner u, row_ptr offsets);
+//   rng_dir: an edge is reverse iff outdeg(u) < outdeg(v) (the shorter list
+//            is streamed): forward edges per owner first (fw_ptr), then the
+//            reverse edges grouped by owner v (rv_ptr, offsets past m_fwd).
+
+__global__ void k_ranges_all(const uint32_t* __restrict__ row_ptr, const uint32_t* __restrict__ col, uint64_t m,
+                             uint2* __restrict__ rng) {
+  const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= m) return;
+  const uint32_t v = col[e];
+  rng[e] = make_uint2(row_ptr[v], row_ptr[v + 1]);
+}
+
+// Per row u: forward-edge count and forward work; reverse-edge count.
+__global__ void k_dir_count(const uint32_t* __restrict__ row_ptr, const uint32_t* __restrict__ col, uint64_t n,
+                            uint32_t* __restrict__ fw_cnt, uint32_t* __restrict__ rv_out, uint64_t* __restrict__ w_fw) {
+  const uint64_t u = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (u > n) return;
+  uint32_t f = 0, r = 0;
+  uint64_t w = 0;
+  if (u < n) {
+    const uint32_t r0 = row_ptr[u], r1 = row_ptr[u + 1], du = r1 - r0;
+    for (uint32_t e = r0; e < r1; ++e) {
+      const uint32_t v = col[e];
+      const uint32_t dv = row_ptr[v + 1] - row_ptr[v];
+      if (du < dv) {
+        ++r;
+      } else {
+        ++f;
+        w += dv;
+      }
+    }
+    w_fw[u] = w;
+  }
+  fw_cnt[u] = f;
+  rv_out[u] = r;
+}
+
+// Writes the forward ranges of row u at fw_ptr[u] and its reverse keys
+// (v << 32 | u) at rv_off[u].
+__global__ void k_dir_fill(const uint32_t* __restrict__ row_ptr, const uint32_t* __restrict__ col, uint64_t n,
+                           const uint32_t* __restrict__ fw_ptr, const uint32_t* __restrict__ rv_off,
+                           uint2* __restrict__ rng, uint64_t* __restrict__ rv_keys) {
+  const uint64_t u = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= n) return;
+  const uint32_t r0 = row_ptr[u], r1 = row_ptr[u + 1], du = r1 - r0;
+  uint32_t f = fw_ptr[u], r = rv_off[u];
+  for (uint32_t e = r0; e < r1; ++e) {
+    const uint32_t v = col[e];
+    const uint32_t b = row_ptr[v], en = row_ptr[v + 1];
+    if (du < en - b)
+      rv_keys[r++] = ((uint64_t)v << 32) | (uint64_t)u;
+    else
+      rng[f++] = make_uint2(b, en);
+  }
+}
+
+// Sorted reverse keys (v << 32 | u): range of N+(u) at rng[base + j];
+// rv_cnt[v] += 1 and w_rv[v] += outdeg(u), aggregated per run of v in a wave.
+__global__ void k_rv_fill(const uint64_t* __restrict__ keys, uint64_t mr, const uint32_t* __restrict__ row_ptr,
+                          uint2* __restrict__ rng, uint32_t* __restrict__ rv_cnt,
+                          unsigned long long* __restrict__ w_rv) {
+  const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = j < mr;
+  const uint64_t k = valid ? keys[j] : 0;
+  const uint32_t v = valid ? (uint32_t)(k >> 32) : 0xFFFFFFFFu;
+  uint64_t du = 0;
+  if (valid) {
+    const uint32_t u = (uint32_t)k;
+    const uint32_t b = row_ptr[u], e = row_ptr[u + 1];
+    rng[j] = make_uint2(b, e);
+    du = e - b;
+  }
+  const uint32_t lane = lane_id();
+  // segmented inclusive sum of du over equal v (keys sorted: equal v are contiguous)
+  uint64_t sum = du;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint64_t t = __shfl_up(sum, o);
+    const uint32_t kv = __shfl_up(v, o);
+    if (lane >= (uint32_t)o && kv == v) sum += t;
+  }
+  const uint32_t nv = __shfl_down(v, 1);
+  const uint64_t vmask = __ballot(valid);
+  const uint64_t above = (lane == 63) ? 0ull : (vmask >> (lane + 1));
+  const bool last_valid = !(above & 1ull);
+  const bool tail = valid && (last_valid || nv != v);
+  const uint32_t run = wave_run_length(v, valid);
+  if (run) atomicAdd(&rv_cnt[v], run);
+  if (tail && sum) atomicAdd(&w_rv[v], (unsigned long long)sum);
+}
+
+__global__ void k_fill_u64(unsigned long long* __restrict__ p, uint64_t n) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = 0ull;
+}
+
+__global__ void k_add_u32(uint32_t* __restrict__ p, uint64_t n, uint32_t v) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] += v;
+}
+
 template <typename F>
 static void with_temp(tribit_ctx* ctx, F&& f) {
   size_t bytes = 0;
@@ -254,6 +362,46 @@ void preprocess(tribit_ctx* ctx, uint64_t* d_keys_in, uint64_t nkeys, uint64_t n
   k_row_stats<<<grid_for(n, kBlock), kBlock, 0, s>>>(row_ptr.p, col.p, n, wrow.p, st.p);
   RowStats hst;
   TB_HIP(hipMemcpyAsync(&hst, st.p, sizeof(hst), hipMemcpyDeviceToHost, s));
+
+  // 8. range lists (forward-only, and direction-split)
+  DevBuf<uint2> rng_all(ctx, m ? m : 1), rng_dir(ctx, m ? m : 1);
+  DevBuf<uint32_t> fw_ptr(ctx, n + 1), rv_ptr(ctx, n + 1);
+  DevBuf<uint64_t> w_fw(ctx, n), w_rv(ctx, n);
+  uint32_t m_fwd = 0;
+  k_fill_u64<<<grid_for(n, kBlock), kBlock, 0, s>>>(reinterpret_cast<unsigned long long*>(w_rv.p), n);
+  if (m) {
+    k_ranges_all<<<grid_for(m, kBlock), kBlock, 0, s>>>(row_ptr.p, col.p, m, rng_all.p);
+    DevBuf<uint32_t> fw_cnt(ctx, n + 1), rv_cnt(ctx, n + 1), rv_off(ctx, n + 1);
+    k_dir_count<<<grid_for(n + 1, kBlock), kBlock, 0, s>>>(row_ptr.p, col.p, n, fw_cnt.p, rv_cnt.p, w_fw.p);
+    with_temp(ctx, [&](void* t, size_t& b) {
+      return rocprim::exclusive_scan(t, b, fw_cnt.p, fw_ptr.p, 0u, (size_t)(n + 1), rocprim::plus<uint32_t>(), s);
+    });
+    with_temp(ctx, [&](void* t, size_t& b) {
+      return rocprim::exclusive_scan(t, b, rv_cnt.p, rv_off.p, 0u, (size_t)(n + 1), rocprim::plus<uint32_t>(), s);
+    });
+    TB_HIP(hipMemcpyAsync(&m_fwd, fw_ptr.p + n, 4, hipMemcpyDeviceToHost, s));
+    ctx->sync();
+    const uint64_t m_rev = m - m_fwd;
+    DevBuf<uint64_t> rv_keys(ctx, m_rev ? m_rev : 1);
+    k_dir_fill<<<grid_for(n, kBlock), kBlock, 0, s>>>(row_ptr.p, col.p, n, fw_ptr.p, rv_off.p, rng_dir.p,
+                                                      rv_keys.p);
+    k_fill_u32<<<grid_for(n + 1, kBlock), kBlock, 0, s>>>(rv_cnt.p, n + 1, 0u);
+    if (m_rev) {
+      DevBuf<uint64_t> rv_sorted(ctx, m_rev);
+      sort_keys(ctx, rv_keys.p, rv_sorted.p, m_rev, end_bit);
+      k_rv_fill<<<grid_for(m_rev, kBlock), kBlock, 0, s>>>(rv_sorted.p, m_rev, row_ptr.p, rng_dir.p + m_fwd,
+                                                           rv_cnt.p,
+                                                           reinterpret_cast<unsigned long long*>(w_rv.p));
+    }
+    with_temp(ctx, [&](void* t, size_t& b) {
+      return rocprim::exclusive_scan(t, b, rv_cnt.p, rv_ptr.p, 0u, (size_t)(n + 1), rocprim::plus<uint32_t>(), s);
+    });
+    k_add_u32<<<grid_for(n + 1, kBlock), kBlock, 0, s>>>(rv_ptr.p, n + 1, m_fwd);
+  } else {
+    k_fill_u32<<<grid_for(n + 1, kBlock), kBlock, 0, s>>>(fw_ptr.p, n + 1, 0u);
+    k_fill_u32<<<grid_for(n + 1, kBlock), kBlock, 0, s>>>(rv_ptr.p, n + 1, 0u);
+    k_fill_u64<<<grid_for(n, kBlock), kBlock, 0, s>>>(reinterpret_cast<unsigned long long*>(w_fw.p), n);
+  }
   TB_HIP(hipEventRecord(ctx->ev[1], s));
   TB_HIP(hipEventSynchronize(ctx->ev[1]));
   TB_HIP(hipGetLastError());
@@ -267,7 +415,14 @@ void preprocess(tribit_ctx* ctx, uint64_t* d_keys_in, uint64_t nkeys, uint64_t n
   g->col = col.release();
   g->deg = deg.release();
   g->wrow = wrow.release();
-  g->bytes = 4 * (n + 1) + 4 * (m ? m : 1) + 4 * n + 8 * n;
+  g->m_fwd = m_fwd;
+  g->rng_all = rng_all.release();
+  g->rng_dir = rng_dir.release();
+  g->fw_ptr = fw_ptr.release();
+  g->rv_ptr = rv_ptr.release();
+  g->w_fw = w_fw.release();
+  g->w_rv = w_rv.release();
+  g->bytes = 4 * (n + 1) * 3 + 4 * (m ? m : 1) + 16 * (m ? m : 1) + 4 * n + 8 * n * 3;
 }
 
 }  // namespace tb

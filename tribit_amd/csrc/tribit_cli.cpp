@@ -11,7 +11,7 @@
 // Added (long options only, so `-i` stays byte-compatible):
 //   --rmat S,EF,A,B,C,SEED  generate an R-MAT graph in HBM instead of reading
 //   --er N,M,SEED           generate Erdos-Renyi G(n, M) in HBM
-//   --sched plain|learned   scheduler choice
+//   --sched plain|directed|learned   scheduler choice (default directed)
 //   --repeats K             timed counting repetitions (median reported)
 //   --json PATH             write a JSON side record of the measured pass
 // Input or device errors print a message and exit 2 (input) or 3 (device);
@@ -30,7 +30,7 @@ namespace {
 struct Config {
   std::string input_file;
   std::string rmat, er, json;
-  int sched = TRIBIT_SCHED_PLAIN;
+  int sched = TRIBIT_SCHED_DIRECTED;
   int repeats = 1;
 };
 
@@ -41,7 +41,7 @@ std::string get_usage(const char* prog) {
          "  -i <input_file>    Path to the input file (required)\n"
          "  --rmat S,EF,A,B,C,SEED   Generate an R-MAT graph instead of reading a file\n"
          "  --er N,M,SEED            Generate an Erdos-Renyi G(n,M) graph instead\n"
-         "  --sched plain|learned    Scheduler (default plain)\n"
+         "  --sched plain|directed|learned  Scheduler (default directed)\n"
          "  --repeats K              Timed counting repetitions (default 1)\n"
          "  --json PATH              Write a JSON record of the measured pass\n";
 }
@@ -72,8 +72,10 @@ Config program_options(int argc, char** argv) {
           c.sched = TRIBIT_SCHED_PLAIN;
         else if (!strcmp(optarg, "learned"))
           c.sched = TRIBIT_SCHED_LEARNED;
+        else if (!strcmp(optarg, "directed"))
+          c.sched = TRIBIT_SCHED_DIRECTED;
         else
-          usage_exit(argv[0], "Error: --sched must be plain or learned\n");
+          usage_exit(argv[0], "Error: --sched must be plain, directed or learned\n");
         break;
       case 4:
         c.repeats = atoi(optarg);
@@ -206,7 +208,8 @@ int main(int argc, char** argv) {
               graph_name.c_str(), (unsigned long long)gi.n, (unsigned long long)gi.m,
               (unsigned long long)gi.wplus, (unsigned long long)ci.triangles, gi.ms_preprocess, ci.ms_count,
               ci.ms_sched, ci.ms_kernel, gteps, (unsigned long long)ci.launch_blocks,
-              (unsigned long long)ci.tasks, cfg.sched == TRIBIT_SCHED_LEARNED ? "learned" : "plain",
+              (unsigned long long)ci.tasks,
+              cfg.sched == TRIBIT_SCHED_LEARNED ? "learned" : (cfg.sched == TRIBIT_SCHED_PLAIN ? "plain" : "directed"),
               (unsigned long long)peak);
       fclose(f);
     }
