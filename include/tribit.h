@@ -72,12 +72,14 @@ typedef enum {
 
 /* tribit_count_opts.flags */
 #define TRIBIT_COUNT_FORCE_LINEAR_PROBE 1 /* test hook: skip the cuckoo sets, use the fallback tables */
+#define TRIBIT_COUNT_FORCE_GLOBAL_LOADS 2 /* test hook: stream with global loads (the path for col >= 4 GiB) */
 
 typedef struct {
   int32_t sched;           /* tribit_sched */
   int32_t repeats;         /* timed repetitions (>=1); the count of every repeat must agree */
   int32_t flags;           /* TRIBIT_COUNT_* bits */
-  int32_t reserved[5];
+  int32_t group_width[4];  /* per kernel variant: lanes per streamed list (4,8,16,32,64; 0 = default) */
+  int32_t reserved0;
 } tribit_count_opts;
 
 typedef struct {
@@ -91,6 +93,8 @@ typedef struct {
   int32_t repeats_agree;   /* 1 if every repeat gave the same count */
   int32_t nvariants;       /* kernel variants (one launch each) */
   double ms_by_variant[8]; /* device time of each variant's launch (median) */
+  uint64_t fallback_tasks; /* tasks whose cuckoo set could not be built (linear-probing fallback) */
+  int32_t group_width[4];  /* lanes per streamed list used by each variant */
 } tribit_count_info;
 
 /* Work assigned to each kernel variant by a scheduler, in the units of the

@@ -156,6 +156,21 @@ def test_fallback_linear_probe(gpu_ctx, oracle_mod):
             assert g.count(sched=sched, flags=tribit_amd.FORCE_LINEAR_PROBE).triangles == want
 
 
+def test_global_load_path(gpu_ctx, oracle_mod):
+    """The streaming path for graphs whose col array exceeds a 32-bit buffer range counts the same."""
+    note("forced global-load streaming")
+    with gpu_ctx.gen_rmat(16, 16, seed=4) as g:
+        a = g.count().triangles
+        for sched in SCHEDS:
+            assert g.count(sched=sched, flags=tribit_amd.FORCE_GLOBAL_LOADS).triangles == a
+            assert g.count(sched=sched, flags=tribit_amd.FORCE_GLOBAL_LOADS | tribit_amd.FORCE_LINEAR_PROBE
+                           ).triangles == a
+    keys, t = graphs.complete(2300)
+    with gpu_ctx.graph_from_keys(keys, 2300) as g:
+        for sched in SCHEDS:
+            assert g.count(sched=sched, flags=tribit_amd.FORCE_GLOBAL_LOADS).triangles == t
+
+
 def test_rmat16_repeats(gpu_ctx, oracle_mod):
     note("rmat s16 ef16 seed 4")
     with gpu_ctx.gen_rmat(16, 16, seed=4) as g:

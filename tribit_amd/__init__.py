@@ -49,10 +49,11 @@ class GraphInfo(ctypes.Structure):
 
 class CountOpts(ctypes.Structure):
     _fields_ = [("sched", ctypes.c_int32), ("repeats", ctypes.c_int32), ("flags", ctypes.c_int32),
-                ("reserved", ctypes.c_int32 * 5)]
+                ("group_width", ctypes.c_int32 * 4), ("reserved0", ctypes.c_int32)]
 
 
 FORCE_LINEAR_PROBE = 1
+FORCE_GLOBAL_LOADS = 2
 
 
 class CountInfo(ctypes.Structure):
@@ -60,7 +61,8 @@ class CountInfo(ctypes.Structure):
         ("triangles", ctypes.c_uint64), ("ms_count", ctypes.c_double), ("ms_sched", ctypes.c_double),
         ("ms_kernel", ctypes.c_double), ("launch_blocks", ctypes.c_uint64), ("tasks", ctypes.c_uint64),
         ("tasks_by_variant", ctypes.c_uint64 * 8), ("repeats_agree", ctypes.c_int32), ("nvariants", ctypes.c_int32),
-        ("ms_by_variant", ctypes.c_double * 8),
+        ("ms_by_variant", ctypes.c_double * 8), ("fallback_tasks", ctypes.c_uint64),
+        ("group_width", ctypes.c_int32 * 4),
     ]
 
 
@@ -157,6 +159,8 @@ class CountResult:
     tasks_by_variant: list
     repeats_agree: bool
     ms_by_variant: list
+    fallback_tasks: int = 0
+    group_width: tuple = ()
 
 
 class Graph:
@@ -182,17 +186,19 @@ class Graph:
                "graph_download")
         return rp, col[:m], deg[:n]
 
-    def count(self, sched: str = "plain", repeats: int = 1, flags: int = 0) -> CountResult:
+    def count(self, sched: str = "plain", repeats: int = 1, flags: int = 0, group_width=None) -> CountResult:
         o = CountOpts()
         o.sched = SCHED[sched]
         o.repeats = repeats
         o.flags = flags
+        for i, w in enumerate(group_width or ()):
+            o.group_width[i] = int(w)
         ci = CountInfo()
         _check(self.ctx.lib.tribit_count(self.ctx.h, self.h, ctypes.byref(o), ctypes.byref(ci)), "count")
         nv = ci.nvariants
         return CountResult(ci.triangles, ci.ms_count, ci.ms_sched, ci.ms_kernel, ci.launch_blocks, ci.tasks,
                            list(ci.tasks_by_variant)[:max(nv, 4)], bool(ci.repeats_agree),
-                           list(ci.ms_by_variant)[:nv])
+                           list(ci.ms_by_variant)[:nv], ci.fallback_tasks, tuple(ci.group_width))
 
     def variant_stats(self, sched: str = "plain") -> dict:
         vs = VariantStats()

@@ -277,6 +277,11 @@ int tribit_count(tribit_ctx* ctx, tribit_graph* g, const tribit_count_opts* opts
   if (o.sched != TRIBIT_SCHED_PLAIN && o.sched != TRIBIT_SCHED_LEARNED && o.sched != TRIBIT_SCHED_DIRECTED)
     return TRIBIT_ERR_ARG;
   if (o.sched == TRIBIT_SCHED_LEARNED) o.sched = TRIBIT_SCHED_DIRECTED;  // no trained model yet
+  for (int i = 0; i < 4; ++i) {
+    const int w = o.group_width[i];
+    if (w != 0 && w != 4 && w != 8 && w != 16 && w != 32 && w != 64) return TRIBIT_ERR_ARG;
+    if (i <= 1 && w == 64) return TRIBIT_ERR_ARG;  // wave variants: at most 32 lanes per list
+  }
   return guarded(ctx, [&] {
     TB_HIP(hipSetDevice(ctx->device));
     tribit_count_info local;

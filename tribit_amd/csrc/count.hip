@@ -5,17 +5,18 @@
 // (U word, 32-neighbour window), one b1 tensor-core AND+POPC; here the
 // count is the degree-oriented edge iterator
 //     triangles = sum over u, v in N+(u) of |N+(u) & N+(v)|
-// with N+(u) staged once per task in an LDS set and every N+(v) streamed
-// from HBM with coalesced int32 loads, one branch-free LDS probe per element.
+// where, for each oriented edge u->v, one of the two lists is staged as an
+// LDS set (the task "owner" x) and the other is streamed from HBM with
+// coalesced int32 loads, one branch-free LDS probe per element. Either
+// direction counts the same triangles; the schedule picks it per edge.
 //
-// Scheduler (plain): rows with outdeg d >= 2 become tasks; the class of a
-// row picks the kernel variant (LDS set size, wave- or block-level task)
-// and its wedge count W+(u) = sum of outdeg(v) picks the work split.
+// The owner's outdeg d picks the kernel variant (class):
 //   class 0: d <= 32      wave task,  cuckoo set 2 x 64 slots
 //   class 1: d <= 256     wave task,  cuckoo set 2 x 512 slots
 //   class 2: d <= 2048    block task, cuckoo set 2 x 4096 slots
-//   class 3: d >  2048    block task per 2048-id chunk of N+(u); each N+(v)
-//                         range is cut to the chunk's id interval.
+//   class 3: d >  2048    block task per 2048-id chunk of N+(x); each
+//                         streamed range is cut to the chunk's id interval.
+// and the owner's streamed work picks how many tasks its lists split into.
 #include <cstring>
 #include <algorithm>
 #include <rocprim/device/device_scan.hpp>
@@ -39,15 +40,25 @@ constexpr int kVariants = 4;             // one kernel launch per row class
 // membership probe is two LDS reads and two compares, with no data-dependent
 // loop (every lane of a wave does the same work). Tables are filled to at
 // most 1/4 of their 2S slots. A build that fails (an insertion chain longer
-// than kCuckooIters) is retried with the next multiplier pair; after
+// than kCuckooIters) is retried with the next multiplier; after
 // kCuckooSeeds failures the task falls back to a linear-probing table in the
 // same 2S slots.
 
 constexpr uint32_t kNoKey = 0xFFFFFFFEu;  // never a vertex id (n <= 2^32-2) and never kEmpty
 constexpr int kCuckooIters = 64;
 constexpr int kCuckooSeeds = 4;
-__device__ __constant__ uint32_t kSeedA[kCuckooSeeds] = {0x9E3779B1u, 0xC2B2AE3Du, 0x165667B1u, 0xFD7046C5u};
-__device__ __constant__ uint32_t kSeedB[kCuckooSeeds] = {0x85EBCA77u, 0x27D4EB2Fu, 0xD3A2646Du, 0xB55A4F09u};
+__device__ __constant__ uint32_t kSeedA[kCuckooSeeds] = {0x9E3779B1u, 0x85EBCA77u, 0xC2B2AE3Du, 0x27D4EB2Fu};
+
+// Both slots of x come from one product h = x * a: table A takes the top
+// LOGS bits, table B the next LOGS bits (one 32-bit multiply per probe).
+template <int LOGS>
+__device__ __forceinline__ uint32_t slot_a(uint32_t h) {
+  return h >> (32 - LOGS);
+}
+template <int LOGS>
+__device__ __forceinline__ uint32_t slot_b(uint32_t h) {
+  return (1u << LOGS) + ((h >> (32 - 2 * LOGS)) & ((1u << LOGS) - 1));
+}
 
 template <int LOGS>
 __device__ __forceinline__ uint32_t hmul(uint32_t x, uint32_t a) {
@@ -55,16 +66,16 @@ __device__ __forceinline__ uint32_t hmul(uint32_t x, uint32_t a) {
 }
 
 template <int LOGS>
-__device__ __forceinline__ bool cuckoo_insert(uint32_t* T, uint32_t x, uint32_t a, uint32_t b) {
-  constexpr uint32_t S = 1u << LOGS;
-  uint32_t pos = hmul<LOGS>(x, a);
+__device__ __forceinline__ bool cuckoo_insert(uint32_t* T, uint32_t x, uint32_t a) {
+  uint32_t pos = slot_a<LOGS>(x * a);
   for (int it = 0; it < kCuckooIters; ++it) {
     const uint32_t y = atomicExch(&T[pos], x);
     if (y == kEmpty) return true;
     // y sat in one of its two slots; it moves to the other one
     x = y;
-    const uint32_t p1 = hmul<LOGS>(x, a);
-    pos = (pos == p1) ? S + hmul<LOGS>(x, b) : p1;
+    const uint32_t h = x * a;
+    const uint32_t pa = slot_a<LOGS>(h);
+    pos = (pos == pa) ? slot_b<LOGS>(h) : pa;
   }
   return false;
 }
@@ -72,10 +83,10 @@ __device__ __forceinline__ bool cuckoo_insert(uint32_t* T, uint32_t x, uint32_t 
 template <int LOGS>
 struct CuckooProbe {
   const uint32_t* T;
-  uint32_t a, b;
+  uint32_t a;
   __device__ __forceinline__ uint32_t operator()(uint32_t x) const {
-    constexpr uint32_t S = 1u << LOGS;
-    return (uint32_t)(T[hmul<LOGS>(x, a)] == x) + (uint32_t)(T[S + hmul<LOGS>(x, b)] == x);
+    const uint32_t h = x * a;
+    return (uint32_t)((T[slot_a<LOGS>(h)] == x) | (T[slot_b<LOGS>(h)] == x));
   }
 };
 
@@ -130,16 +141,32 @@ __device__ __forceinline__ uint32_t upper_bound_u32(const uint32_t* a, uint32_t 
 }
 
 // --------------------------------------------------------- range streaming
-// The lists to stream, N+(v) for the task's v, are staged in LDS as ranges
-// R[k] = [begin, end) of col. 64/G groups of G lanes each own one range at a
-// time; per step a group covers G*U consecutive elements, each lane issuing
-// U independent loads before probing (U loads in flight per lane). A group
-// whose range is exhausted takes the next index: wave-private counter, or,
-// for block tasks (SHARED), an LDS counter shared by the block's waves.
+// The lists to stream are staged in LDS as ranges R[k] = [begin, end) of col.
+// 64/G groups of G lanes each own one range at a time; per step a group
+// covers G*U consecutive elements, each lane issuing U independent loads
+// before probing (U loads in flight per lane). With BUF the loads are raw
+// buffer loads: one 32-bit byte offset per lane, the U loads differ only by
+// an immediate offset, no branches (an offset past the buffer reads 0 and is
+// masked anyway). A group whose range is exhausted takes the next index:
+// wave-private counter, or, for block tasks (SHARED), an LDS counter shared
+// by the block's waves.
 
-template <int G, int U, bool SHARED, typename Probe>
-__device__ __forceinline__ uint32_t stream_ranges(const uint32_t* __restrict__ col, const uint2* R, uint32_t nk,
-                                                  uint32_t* s_next, uint32_t lane, const Probe& probe) {
+struct ColSrc {
+  const uint32_t* col;
+  __amdgpu_buffer_rsrc_t rsrc;  // raw buffer over col (valid when col bytes < 2^32)
+};
+
+__device__ __forceinline__ ColSrc make_col_src(const uint32_t* col, uint64_t m) {
+  ColSrc c;
+  c.col = col;
+  const uint64_t bytes = m * 4 < 0xFFFFFFFFull ? m * 4 : 0xFFFFFFFFull;
+  c.rsrc = __builtin_amdgcn_make_buffer_rsrc(const_cast<uint32_t*>(col), (short)0, (int)(uint32_t)bytes, 0x00020000);
+  return c;
+}
+
+template <int G, int U, bool SHARED, bool BUF, typename Probe>
+__device__ __forceinline__ uint32_t stream_ranges(const ColSrc& src, const uint2* R, uint32_t nk, uint32_t* s_next,
+                                                  uint32_t lane, const Probe& probe) {
   constexpr uint64_t kLeads = leader_mask<G>();
   const uint32_t g = lane / G, gl = lane % G;
   const uint64_t below = (g == 0) ? 0ull : ((1ull << (g * G)) - 1);
@@ -167,11 +194,22 @@ __device__ __forceinline__ uint32_t stream_ranges(const uint32_t* __restrict__ c
         end = r.y;
       }
     }
+    const uint32_t p0 = base + gl;
+    const int rem = (int)(end - p0);  // elements left for this lane's first load (may be <= 0)
     uint32_t x[U];
+    if (BUF) {
+      // an invalid element gets an offset past the buffer: no memory access, reads 0
+      const uint32_t off = p0 * 4u;
 #pragma unroll
-    for (int j = 0; j < U; ++j) {
-      const uint32_t p = base + gl + (uint32_t)(j * G);
-      x[j] = (p < end) ? col[p] : kNoKey;
+      for (int j = 0; j < U; ++j) {
+        const bool ok = j * G < rem;
+        x[j] = __builtin_amdgcn_raw_buffer_load_b32(src.rsrc, (int)(ok ? off + (uint32_t)(j * G * 4) : 0xFFFFFFFFu), 0,
+                                                    0);
+        x[j] = ok ? x[j] : kNoKey;
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < U; ++j) x[j] = (j * G < rem) ? src.col[p0 + j * G] : kNoKey;
     }
 #pragma unroll
     for (int j = 0; j < U; ++j) cnt += probe(x[j]);
@@ -185,20 +223,22 @@ __device__ __forceinline__ uint32_t stream_ranges(const uint32_t* __restrict__ c
 // ranges are rng[rbase, rbase + rcount). A wave owns one task: it builds the
 // set in its LDS slice once, then stages the ranges KMAX at a time.
 
-template <int LOGS, int KMAX, int G, int U>
+template <int LOGS, int KMAX, int G, int U, bool BUF>
 __global__ __launch_bounds__(256) void k_count_wave(const uint32_t* __restrict__ row_ptr,
-                                                    const uint32_t* __restrict__ col,
+                                                    const uint32_t* __restrict__ col, uint64_t m,
                                                     const uint2* __restrict__ rng,
                                                     const uint4* __restrict__ tasks, uint32_t ntasks,
                                                     uint32_t* __restrict__ queue,
                                                     unsigned long long* __restrict__ result, int attempts) {
   constexpr uint32_t S = 1u << LOGS;
+  const ColSrc src = make_col_src(col, m);
   __shared__ uint32_t sT[4][2 * S];
   __shared__ uint2 sR[4][KMAX];
   const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   uint32_t* T = sT[wv];
   uint2* R = sR[wv];
   uint64_t acc = 0;
+  uint32_t fallbacks = 0;
   for (;;) {
     uint32_t tb = 0;
     if (lane == 0) tb = atomicAdd(queue, kWaveBatch);
@@ -209,19 +249,19 @@ __global__ __launch_bounds__(256) void k_count_wave(const uint32_t* __restrict__
       const uint4 tk = tasks[t];
       const uint32_t x = tk.x, rbase = tk.y, rcount = tk.z;
       const uint32_t r0 = row_ptr[x], d = row_ptr[x + 1] - r0;
-      uint32_t sa = 0, sb = 0;
+      uint32_t sa = 0;
       bool ok = false;
       for (int attempt = 0; attempt < attempts && !ok; ++attempt) {
         sa = kSeedA[attempt];
-        sb = kSeedB[attempt];
         for (uint32_t i = lane; i < 2 * S; i += 64) T[i] = kEmpty;
         wave_lds_sync();
         bool fail = false;
-        for (uint32_t i = lane; i < d; i += 64) fail |= !cuckoo_insert<LOGS>(T, col[r0 + i], sa, sb);
+        for (uint32_t i = lane; i < d; i += 64) fail |= !cuckoo_insert<LOGS>(T, col[r0 + i], sa);
         wave_lds_sync();
         ok = !__any(fail);
       }
       if (!ok) {
+        ++fallbacks;
         for (uint32_t i = lane; i < 2 * S; i += 64) T[i] = kEmpty;
         wave_lds_sync();
         for (uint32_t i = lane; i < d; i += 64) linear_insert<LOGS + 1>(T, col[r0 + i]);
@@ -231,8 +271,8 @@ __global__ __launch_bounds__(256) void k_count_wave(const uint32_t* __restrict__
         const uint32_t nk = min((uint32_t)KMAX, rcount - c0);
         for (uint32_t k = lane; k < nk; k += 64) R[k] = rng[rbase + c0 + k];
         wave_lds_sync();
-        cnt += ok ? stream_ranges<G, U, false>(col, R, nk, nullptr, lane, CuckooProbe<LOGS>{T, sa, sb})
-                  : stream_ranges<G, U, false>(col, R, nk, nullptr, lane, LinearProbe<LOGS + 1>{T});
+        cnt += ok ? stream_ranges<G, U, false, BUF>(src, R, nk, nullptr, lane, CuckooProbe<LOGS>{T, sa})
+                  : stream_ranges<G, U, false, BUF>(src, R, nk, nullptr, lane, LinearProbe<LOGS + 1>{T});
         wave_lds_sync();
       }
       acc += cnt;
@@ -240,6 +280,7 @@ __global__ __launch_bounds__(256) void k_count_wave(const uint32_t* __restrict__
   }
   acc = wave_sum_u64(acc);
   if (lane == 0 && acc) atomicAdd(result, (unsigned long long)acc);
+  if (lane == 0 && fallbacks) atomicAdd(result + 1, (unsigned long long)fallbacks);
 }
 
 // ----------------------------------------------------- block-level tasks
@@ -251,9 +292,9 @@ __global__ __launch_bounds__(256) void k_count_wave(const uint32_t* __restrict__
 
 constexpr int kBlockThreads = 512;
 
-template <int LOGS, int G, int U>
+template <int LOGS, int G, int U, bool BUF>
 __global__ __launch_bounds__(kBlockThreads) void k_count_block(const uint32_t* __restrict__ row_ptr,
-                                                               const uint32_t* __restrict__ col,
+                                                               const uint32_t* __restrict__ col, uint64_t m,
                                                                const uint2* __restrict__ rng,
                                                                const uint4* __restrict__ tasks, uint32_t ntasks,
                                                                uint32_t* __restrict__ queue,
@@ -264,7 +305,9 @@ __global__ __launch_bounds__(kBlockThreads) void k_count_block(const uint32_t* _
   __shared__ uint2 R[kChunk];
   __shared__ uint32_t s_task, s_next, s_fail;
   const uint32_t lane = threadIdx.x & 63;
+  const ColSrc src = make_col_src(col, m);
   uint64_t acc = 0;
+  uint32_t fallbacks = 0;
   for (;;) {
     if (threadIdx.x == 0) s_task = atomicAdd(queue, 1u);
     __syncthreads();
@@ -276,23 +319,23 @@ __global__ __launch_bounds__(kBlockThreads) void k_count_block(const uint32_t* _
     const uint32_t p0 = c * kChunk, p1 = min(p0 + kChunk, d);
     const bool ranged = (p0 > 0) || (p1 < d);
     const uint32_t lo = col[r0 + p0], hi = col[r0 + p1 - 1];
-    uint32_t sa = 0, sb = 0;
+    uint32_t sa = 0;
     bool ok = false;
     for (int attempt = 0; attempt < attempts && !ok; ++attempt) {
       sa = kSeedA[attempt];
-      sb = kSeedB[attempt];
       for (uint32_t i = threadIdx.x; i < 2 * S; i += kBlockThreads) T[i] = kEmpty;
       if (threadIdx.x == 0) s_fail = 0;
       __syncthreads();
       bool fail = false;
       for (uint32_t i = p0 + threadIdx.x; i < p1; i += kBlockThreads)
-        fail |= !cuckoo_insert<LOGS>(T, col[r0 + i], sa, sb);
+        fail |= !cuckoo_insert<LOGS>(T, col[r0 + i], sa);
       if (fail) s_fail = 1;
       __syncthreads();
       ok = (s_fail == 0);
       __syncthreads();
     }
     if (!ok) {
+      ++fallbacks;
       for (uint32_t i = threadIdx.x; i < 2 * S; i += kBlockThreads) T[i] = kEmpty;
       __syncthreads();
       for (uint32_t i = p0 + threadIdx.x; i < p1; i += kBlockThreads) linear_insert<LOGS + 1>(T, col[r0 + i]);
@@ -310,14 +353,15 @@ __global__ __launch_bounds__(kBlockThreads) void k_count_block(const uint32_t* _
       }
       if (threadIdx.x == 0) s_next = 0;
       __syncthreads();
-      cnt += ok ? stream_ranges<G, U, true>(col, R, nk, &s_next, lane, CuckooProbe<LOGS>{T, sa, sb})
-                : stream_ranges<G, U, true>(col, R, nk, &s_next, lane, LinearProbe<LOGS + 1>{T});
+      cnt += ok ? stream_ranges<G, U, true, BUF>(src, R, nk, &s_next, lane, CuckooProbe<LOGS>{T, sa})
+                : stream_ranges<G, U, true, BUF>(src, R, nk, &s_next, lane, LinearProbe<LOGS + 1>{T});
       __syncthreads();
     }
     acc += cnt;
   }
   acc = wave_sum_u64(acc);
   if (lane == 0 && acc) atomicAdd(result, (unsigned long long)acc);
+  if (threadIdx.x == 0 && fallbacks) atomicAdd(result + 1, (unsigned long long)fallbacks);
 }
 
 // ------------------------------------------------------------- scheduler
@@ -450,38 +494,97 @@ static const uint2* sched_ranges(const tribit_graph* g, int sched) {
   return sched == TRIBIT_SCHED_PLAIN ? g->rng_all : g->rng_dir;
 }
 
-// Kernel variants (LDS table sizes, lane-group width G, loads in flight U).
-constexpr int kC0Logs = 6, kC0K = 64, kC0G = 16, kC0U = 8;    // class 0: d <= 32, 2x64 slots
-constexpr int kC1Logs = 9, kC1K = 256, kC1G = 16, kC1U = 8;   // class 1: d <= 256, 2x512 slots
-constexpr int kC2Logs = 12, kC2G = 16, kC2U = 8;              // classes 2,3: <= 2048 ids, 2x4096 slots
+// Kernel variants: LDS set size (LOGS), staged ranges per batch (KMAX), lane
+// group width G and loads in flight per lane U. In the DIRECTED schedule a
+// streamed list is never longer than its owner's list, so class 0 streams
+// lists of at most 32 elements and uses narrow groups.
+constexpr int kC0Logs = 6, kC0K = 64;    // class 0: d <= 32, 2x64 slots
+constexpr int kC1Logs = 9, kC1K = 256;   // class 1: d <= 256, 2x512 slots
+constexpr int kC2Logs = 12;              // classes 2,3: <= 2048 ids, 2x4096 slots
+constexpr int kU = 8;
+constexpr int kForceGlobal = 1 << 8;  // launch-time flag folded into `attempts` (stripped before the kernel)
 
-template <int LOGS, int KMAX, int G, int U>
-static void launch_wave(uint32_t ntasks, int cus, hipStream_t s, tribit_graph* g, const uint2* rng,
-                        const uint4* tasks, uint32_t* queue, unsigned long long* result, int attempts,
-                        uint64_t* blocks) {
-  if (!ntasks) return;
+template <int LOGS, int KMAX, int G, int U, bool BUF>
+static void launch_wave_t(uint32_t ntasks, int cus, hipStream_t s, tribit_graph* g, const uint2* rng,
+                          const uint4* tasks, uint32_t* queue, unsigned long long* result, int attempts,
+                          uint64_t* blocks) {
   int per_cu = 0;
-  TB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_count_wave<LOGS, KMAX, G, U>, kBlock, 0));
+  TB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_count_wave<LOGS, KMAX, G, U, BUF>, kBlock, 0));
   if (per_cu < 1) per_cu = 1;
   const uint64_t want = (ntasks + 4 * kWaveBatch - 1) / (4 * kWaveBatch);
   const unsigned grid = (unsigned)std::min<uint64_t>(want, (uint64_t)cus * per_cu);
-  k_count_wave<LOGS, KMAX, G, U><<<grid, kBlock, 0, s>>>(g->row_ptr, g->col, rng, tasks, ntasks, queue, result,
-                                                        attempts);
+  k_count_wave<LOGS, KMAX, G, U, BUF><<<grid, kBlock, 0, s>>>(g->row_ptr, g->col, g->m, rng, tasks, ntasks, queue,
+                                                             result, attempts & (kForceGlobal - 1));
   *blocks += grid;
 }
 
-template <int LOGS, int G, int U>
-static void launch_block(uint32_t ntasks, int cus, hipStream_t s, tribit_graph* g, const uint2* rng,
+template <int LOGS, int KMAX, int G>
+static void launch_wave_g(uint32_t ntasks, int cus, hipStream_t s, tribit_graph* g, const uint2* rng,
+                          const uint4* tasks, uint32_t* queue, unsigned long long* result, int attempts,
+                          uint64_t* blocks) {
+  if (g->m * 4 < 0xFFFFFFFFull && !(attempts & kForceGlobal))
+    launch_wave_t<LOGS, KMAX, G, kU, true>(ntasks, cus, s, g, rng, tasks, queue, result, attempts, blocks);
+  else
+    launch_wave_t<LOGS, KMAX, G, kU, false>(ntasks, cus, s, g, rng, tasks, queue, result, attempts, blocks);
+}
+
+template <int LOGS, int KMAX>
+static void launch_wave(int gw, uint32_t ntasks, int cus, hipStream_t s, tribit_graph* g, const uint2* rng,
+                        const uint4* tasks, uint32_t* queue, unsigned long long* result, int attempts,
+                        uint64_t* blocks) {
+  if (!ntasks) return;
+  switch (gw) {
+    case 4: launch_wave_g<LOGS, KMAX, 4>(ntasks, cus, s, g, rng, tasks, queue, result, attempts, blocks); break;
+    case 8: launch_wave_g<LOGS, KMAX, 8>(ntasks, cus, s, g, rng, tasks, queue, result, attempts, blocks); break;
+    case 32: launch_wave_g<LOGS, KMAX, 32>(ntasks, cus, s, g, rng, tasks, queue, result, attempts, blocks); break;
+    default: launch_wave_g<LOGS, KMAX, 16>(ntasks, cus, s, g, rng, tasks, queue, result, attempts, blocks); break;
+  }
+}
+
+template <int LOGS, int G, int U, bool BUF>
+static void launch_block_t(uint32_t ntasks, int cus, hipStream_t s, tribit_graph* g, const uint2* rng,
+                           const uint4* tasks, uint32_t* queue, unsigned long long* result, int attempts,
+                           uint64_t* blocks) {
+  int per_cu = 0;
+  TB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_count_block<LOGS, G, U, BUF>, kBlockThreads, 0));
+  if (per_cu < 1) per_cu = 1;
+  const unsigned grid = (unsigned)std::min<uint64_t>(ntasks, (uint64_t)cus * per_cu);
+  k_count_block<LOGS, G, U, BUF><<<grid, kBlockThreads, 0, s>>>(g->row_ptr, g->col, g->m, rng, tasks, ntasks, queue,
+                                                               result, attempts & (kForceGlobal - 1));
+  *blocks += grid;
+}
+
+template <int LOGS, int G>
+static void launch_block_g(uint32_t ntasks, int cus, hipStream_t s, tribit_graph* g, const uint2* rng,
+                           const uint4* tasks, uint32_t* queue, unsigned long long* result, int attempts,
+                           uint64_t* blocks) {
+  if (g->m * 4 < 0xFFFFFFFFull && !(attempts & kForceGlobal))
+    launch_block_t<LOGS, G, kU, true>(ntasks, cus, s, g, rng, tasks, queue, result, attempts, blocks);
+  else
+    launch_block_t<LOGS, G, kU, false>(ntasks, cus, s, g, rng, tasks, queue, result, attempts, blocks);
+}
+
+template <int LOGS>
+static void launch_block(int gw, uint32_t ntasks, int cus, hipStream_t s, tribit_graph* g, const uint2* rng,
                          const uint4* tasks, uint32_t* queue, unsigned long long* result, int attempts,
                          uint64_t* blocks) {
   if (!ntasks) return;
-  int per_cu = 0;
-  TB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_count_block<LOGS, G, U>, kBlockThreads, 0));
-  if (per_cu < 1) per_cu = 1;
-  const unsigned grid = (unsigned)std::min<uint64_t>(ntasks, (uint64_t)cus * per_cu);
-  k_count_block<LOGS, G, U><<<grid, kBlockThreads, 0, s>>>(g->row_ptr, g->col, rng, tasks, ntasks, queue, result,
-                                                           attempts);
-  *blocks += grid;
+  switch (gw) {
+    case 4: launch_block_g<LOGS, 4>(ntasks, cus, s, g, rng, tasks, queue, result, attempts, blocks); break;
+    case 8: launch_block_g<LOGS, 8>(ntasks, cus, s, g, rng, tasks, queue, result, attempts, blocks); break;
+    case 32: launch_block_g<LOGS, 32>(ntasks, cus, s, g, rng, tasks, queue, result, attempts, blocks); break;
+    case 64: launch_block_g<LOGS, 64>(ntasks, cus, s, g, rng, tasks, queue, result, attempts, blocks); break;
+    default: launch_block_g<LOGS, 16>(ntasks, cus, s, g, rng, tasks, queue, result, attempts, blocks); break;
+  }
+}
+
+// Default lane-group widths per variant and schedule.
+static void default_widths(int sched, int gw[4]) {
+  if (sched == TRIBIT_SCHED_PLAIN) {
+    gw[0] = 16, gw[1] = 16, gw[2] = 16, gw[3] = 16;
+  } else {
+    gw[0] = 4, gw[1] = 16, gw[2] = 16, gw[3] = 16;
+  }
 }
 
 static float elapsed(hipEvent_t a, hipEvent_t b) {
@@ -491,12 +594,13 @@ static float elapsed(hipEvent_t a, hipEvent_t b) {
 }
 
 // One scheduled count. Returns the triangle count; fills per-phase times.
-static uint64_t count_once(tribit_ctx* ctx, tribit_graph* g, int sched, int flags, double* ms_total,
-                           double* ms_sched, double* ms_kernel, double* ms_var, tribit_count_info* info) {
+static uint64_t count_once(tribit_ctx* ctx, tribit_graph* g, int sched, int flags, const int gw[4],
+                           double* ms_total, double* ms_sched, double* ms_kernel, double* ms_var,
+                           tribit_count_info* info) {
   hipStream_t s = ctx->stream;
   const uint64_t n = g->n;
   TB_HIP(hipEventRecord(ctx->ev[2], s));
-  // scratch: result + 4 queue counters live in one 64-byte block
+  // scratch: [0] count, [1] fallback tasks, [4..7] four u32 queue counters
   DevBuf<unsigned long long> scratch(ctx, 8);
   k_zero_u64<<<1, 64, 0, s>>>(scratch.p, 8);
   unsigned long long* d_result = scratch.p;
@@ -528,18 +632,19 @@ static uint64_t count_once(tribit_ctx* ctx, tribit_graph* g, int sched, int flag
   const int cus = ctx->cus > 0 ? ctx->cus : 256;
   uint64_t blocks = 0;
   // one launch per variant; an event after each gives its device time
-  const int att = (flags & TRIBIT_COUNT_FORCE_LINEAR_PROBE) ? 0 : kCuckooSeeds;
-  launch_wave<kC0Logs, kC0K, kC0G, kC0U>(tot.x, cus, s, g, rng, wt.p, d_queue + 0, d_result, att, &blocks);
+  const int att = ((flags & TRIBIT_COUNT_FORCE_LINEAR_PROBE) ? 0 : kCuckooSeeds) |
+                  ((flags & TRIBIT_COUNT_FORCE_GLOBAL_LOADS) ? kForceGlobal : 0);
+  launch_wave<kC0Logs, kC0K>(gw[0], tot.x, cus, s, g, rng, wt.p, d_queue + 0, d_result, att, &blocks);
   TB_HIP(hipEventRecord(ctx->ev[4], s));
-  launch_wave<kC1Logs, kC1K, kC1G, kC1U>(tot.y, cus, s, g, rng, wt.p + tot.x, d_queue + 1, d_result, att, &blocks);
+  launch_wave<kC1Logs, kC1K>(gw[1], tot.y, cus, s, g, rng, wt.p + tot.x, d_queue + 1, d_result, att, &blocks);
   TB_HIP(hipEventRecord(ctx->ev[5], s));
-  launch_block<kC2Logs, kC2G, kC2U>(tot.z, cus, s, g, rng, bt.p, d_queue + 2, d_result, att, &blocks);
+  launch_block<kC2Logs>(gw[2], tot.z, cus, s, g, rng, bt.p, d_queue + 2, d_result, att, &blocks);
   TB_HIP(hipEventRecord(ctx->ev[6], s));
-  launch_block<kC2Logs, kC2G, kC2U>(tot.w, cus, s, g, rng, bt.p + tot.z, d_queue + 3, d_result, att, &blocks);
+  launch_block<kC2Logs>(gw[3], tot.w, cus, s, g, rng, bt.p + tot.z, d_queue + 3, d_result, att, &blocks);
   TB_HIP(hipGetLastError());
   TB_HIP(hipEventRecord(ctx->ev[7], s));
-  uint64_t tri = 0;
-  TB_HIP(hipMemcpyAsync(&tri, d_result, 8, hipMemcpyDeviceToHost, s));
+  unsigned long long res[2] = {0, 0};
+  TB_HIP(hipMemcpyAsync(res, d_result, sizeof(res), hipMemcpyDeviceToHost, s));
   TB_HIP(hipEventSynchronize(ctx->ev[7]));
   ctx->sync();
   *ms_sched = elapsed(ctx->ev[2], ctx->ev[3]);
@@ -554,8 +659,10 @@ static uint64_t count_once(tribit_ctx* ctx, tribit_graph* g, int sched, int flag
     info->tasks_by_variant[2] = tot.z;
     info->tasks_by_variant[3] = tot.w;
     info->nvariants = kVariants;
+    info->fallback_tasks = res[1];
+    for (int i = 0; i < 4; ++i) info->group_width[i] = gw[i];
   }
-  return tri;
+  return res[0];
 }
 
 static double median(std::vector<double> v) {
@@ -571,9 +678,13 @@ void count(tribit_ctx* ctx, tribit_graph* g, const tribit_count_opts& opts, trib
   bool agree = true;
   tribit_count_info local;
   memset(&local, 0, sizeof(local));
+  int gw[4];
+  default_widths(opts.sched, gw);
+  for (int i = 0; i < 4; ++i)
+    if (opts.group_width[i]) gw[i] = opts.group_width[i];
   for (int r = 0; r < reps; ++r) {
     double a, b, c, v[kVariants];
-    const uint64_t tri = count_once(ctx, g, opts.sched, opts.flags, &a, &b, &c, v, &local);
+    const uint64_t tri = count_once(ctx, g, opts.sched, opts.flags, gw, &a, &b, &c, v, &local);
     if (r == 0)
       first = tri;
     else if (tri != first)
