@@ -49,15 +49,18 @@ constexpr int kCuckooIters = 64;
 constexpr int kCuckooSeeds = 4;
 __device__ __constant__ uint32_t kSeedA[kCuckooSeeds] = {0x9E3779B1u, 0x85EBCA77u, 0xC2B2AE3Du, 0x27D4EB2Fu};
 
-// Both slots of x come from one product h = x * a: table A takes the top
-// LOGS bits, table B the next LOGS bits (one 32-bit multiply per probe).
+// Both slots of x come from one product h = x * a (one 32-bit multiply per
+// probe): table A takes the top LOGS bits of h, table B the next LOGS bits
+// XOR the top ones. The next bits alone depend only on the low bits of x, and
+// R-MAT hubs (ids with few set bits) often share those, for every multiplier;
+// the XOR makes both slots depend on every bit of x.
 template <int LOGS>
 __device__ __forceinline__ uint32_t slot_a(uint32_t h) {
   return h >> (32 - LOGS);
 }
 template <int LOGS>
 __device__ __forceinline__ uint32_t slot_b(uint32_t h) {
-  return (1u << LOGS) + ((h >> (32 - 2 * LOGS)) & ((1u << LOGS) - 1));
+  return (1u << LOGS) + (((h >> (32 - 2 * LOGS)) ^ (h >> (32 - LOGS))) & ((1u << LOGS) - 1));
 }
 
 template <int LOGS>
@@ -579,11 +582,12 @@ static void launch_block(int gw, uint32_t ntasks, int cus, hipStream_t s, tribit
 }
 
 // Default lane-group widths per variant and schedule.
+// (swept on R-MAT scale 22 and 25 over 4..32 / 8..64 lanes per list)
 static void default_widths(int sched, int gw[4]) {
   if (sched == TRIBIT_SCHED_PLAIN) {
-    gw[0] = 16, gw[1] = 16, gw[2] = 16, gw[3] = 16;
+    gw[0] = 32, gw[1] = 16, gw[2] = 32, gw[3] = 16;
   } else {
-    gw[0] = 4, gw[1] = 16, gw[2] = 16, gw[3] = 16;
+    gw[0] = 4, gw[1] = 8, gw[2] = 16, gw[3] = 16;
   }
 }
 
