@@ -94,6 +94,22 @@ def cpu_baseline(g, n: int, m: int, target_s: float) -> dict:
     }
 
 
+def measured_traffic(args, variant: int):
+    """HBM bytes per launch of `variant` from the committed rocprofv3 counter summary of this
+    workload and schedule (FETCH_SIZE x2.0 + WRITE_SIZE x1.0, scaled by a plain device copy),
+    or None when no summary matches."""
+    path = os.path.join(ROOT, "profiles", f"rmat{args.scale}_{args.sched}_counters.json")
+    if args.ef != 16 or args.seed != (5 if args.scale == 25 else -1) or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        summ = json.load(f)
+    names = {0: "class 0", 1: "class 1", 2: "class 2", 3: "class 3"}
+    for launch in summ.get("launches", []):
+        if launch.get("variant") == names.get(variant):
+            return int(launch["traffic_bytes"])
+    return None
+
+
 def measure(g, sched: str, steps: int, warmup: int):
     """W untimed counts, then K timed counts (wall clock around all K)."""
     for i in range(warmup):
@@ -231,7 +247,7 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": measured_traffic(args, dom),
             "algorithmic_bytes": b_dom,
             "ms_per_launch": round(ms_var[dom], 4),
         },
