@@ -139,6 +139,22 @@ int tribit_gen_rmat(tribit_ctx* ctx, uint32_t scale, uint32_t ef, double a, doub
 /* Erdos-Renyi G(n, M): M samples of uniform vertex pairs. */
 int tribit_gen_er(tribit_ctx* ctx, uint64_t n, uint64_t nsamples, uint64_t seed, uint64_t** d_keys, uint64_t* nkeys);
 
+/* Chung-Lu by edge sampling: weights w_i = (i+1)^(-1/(gamma-1)), round(n*avg/2)
+ * samples with both endpoints drawn from the weight distribution. */
+int tribit_gen_chung_lu(tribit_ctx* ctx, uint64_t n, double gamma, double avg_deg, uint64_t seed,
+                        uint64_t** d_keys, uint64_t* nkeys);
+/* Parameters of graph `index` of a Chung-Lu batch drawn from `seed`:
+ * gamma ~ U[2.1, 2.8], n log-uniform in [1024, 262144], avg_deg ~ U[8, 64]. */
+void tribit_chung_lu_batch_params(uint64_t seed, uint64_t index, double* gamma, uint64_t* n, double* avg_deg);
+/* 2-D random geometric graph: n points uniform in the unit square (2^30 grid),
+ * radius sqrt(avg_deg / (pi n)); vertex ids in spatial (cell-major) order. */
+int tribit_gen_rgg(tribit_ctx* ctx, uint64_t n, double avg_deg, uint64_t seed, uint64_t** d_keys, uint64_t* nkeys);
+/* Stochastic block model: `blocks` blocks of `size` consecutive ids, pairs
+ * inside a block kept with probability p_in, plus round(p_out * cross pairs)
+ * uniform cross-block pairs. */
+int tribit_gen_sbm(tribit_ctx* ctx, uint64_t blocks, uint64_t size, double p_in, double p_out, uint64_t seed,
+                   uint64_t** d_keys, uint64_t* nkeys);
+
 /* ---- preprocessing: keys -> count-ready graph (tribit.cu:656-763) --------- */
 /* Copies host keys to HBM (untimed, tribit.cu:657-659) then preprocesses. */
 int tribit_graph_from_host_keys(tribit_ctx* ctx, const uint64_t* keys, uint64_t nkeys, uint64_t n,

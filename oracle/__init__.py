@@ -47,6 +47,17 @@ def lib() -> ctypes.CDLL:
         L.or_gen_rmat.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_double, ctypes.c_double,
                                   ctypes.c_double, _U64, _P]
         L.or_gen_er.argtypes = [_U64, _U64, _U64, _P]
+        L.or_chung_lu_samples.argtypes = [_U64, ctypes.c_double]
+        L.or_chung_lu_samples.restype = _U64
+        L.or_chung_lu_cdf.argtypes = [_U64, ctypes.c_double, _P]
+        L.or_gen_chung_lu.argtypes = [_U64, ctypes.c_double, ctypes.c_double, _U64, _P]
+        L.or_chung_lu_batch_params.argtypes = [_U64, _U64, ctypes.POINTER(ctypes.c_double), _U64P,
+                                               ctypes.POINTER(ctypes.c_double)]
+        L.or_gen_rgg.argtypes = [_U64, ctypes.c_double, _U64, _P, _U64]
+        L.or_gen_rgg.restype = _U64
+        L.or_sbm_slots.argtypes = [_U64, _U64, ctypes.c_double]
+        L.or_sbm_slots.restype = _U64
+        L.or_gen_sbm.argtypes = [_U64, _U64, ctypes.c_double, ctypes.c_double, _U64, _P]
         L.or_dedup.argtypes = [_P, _U64]
         L.or_dedup.restype = _U64
         L.or_orient_csr.argtypes = [_P, _U64, _U64, ctypes.POINTER(_Graph)]
@@ -96,6 +107,38 @@ def gen_rmat(scale: int, ef: int, a=0.57, b=0.19, c=0.19, seed: int = 1) -> np.n
 def gen_er(n: int, nsamples: int, seed: int) -> np.ndarray:
     keys = np.empty(nsamples, np.uint64)
     assert lib().or_gen_er(n, nsamples, seed, _ptr(keys)) == 0
+    return keys
+
+
+def chung_lu_cdf(n: int, gamma: float) -> np.ndarray:
+    cdf = np.empty(n, np.uint64)
+    assert lib().or_chung_lu_cdf(n, gamma, _ptr(cdf)) == 0
+    return cdf
+
+
+def gen_chung_lu(n: int, gamma: float, avg_deg: float, seed: int) -> np.ndarray:
+    keys = np.empty(lib().or_chung_lu_samples(n, avg_deg), np.uint64)
+    assert lib().or_gen_chung_lu(n, gamma, avg_deg, seed, _ptr(keys)) == 0
+    return keys
+
+
+def chung_lu_batch_params(seed: int, index: int):
+    g, d, n = ctypes.c_double(), ctypes.c_double(), _U64()
+    lib().or_chung_lu_batch_params(seed, index, ctypes.byref(g), ctypes.byref(n), ctypes.byref(d))
+    return g.value, n.value, d.value
+
+
+def gen_rgg(n: int, avg_deg: float, seed: int) -> np.ndarray:
+    cap = int(n * avg_deg) + 1024
+    keys = np.empty(cap, np.uint64)
+    got = lib().or_gen_rgg(n, avg_deg, seed, _ptr(keys), cap)
+    assert got != 0xFFFFFFFFFFFFFFFF
+    return keys[:got].copy()
+
+
+def gen_sbm(blocks: int, size: int, p_in: float, p_out: float, seed: int) -> np.ndarray:
+    keys = np.empty(lib().or_sbm_slots(blocks, size, p_out), np.uint64)
+    assert lib().or_gen_sbm(blocks, size, p_in, p_out, seed, _ptr(keys)) == 0
     return keys
 
 

@@ -6,6 +6,7 @@
 #include "oracle.h"
 
 #include <ctype.h>
+#include <math.h>
 #include <errno.h>
 #include <omp.h>
 #include <stdio.h>
@@ -213,6 +214,184 @@ int or_gen_er(uint64_t n, uint64_t nedges, uint64_t seed, uint64_t* keys) {
       keys[e] = OR_SENTINEL;
     else
       keys[e] = u < v ? ((u << 32) | v) : ((v << 32) | u);
+  }
+  return OR_OK;
+}
+
+#define OR_STREAM_CL 0x43484C55ull     /* "CHLU" */
+#define OR_STREAM_CLP 0x43484C50ull    /* "CHLP" */
+#define OR_STREAM_RGG 0x52474732ull    /* "RGG2" */
+#define OR_STREAM_SBMI 0x53424D49ull   /* "SBMI" */
+#define OR_STREAM_SBMO 0x53424D4Full   /* "SBMO" */
+
+static double or_unit(uint64_t w) { return (double)(w >> 11) * (1.0 / 9007199254740992.0); }
+
+uint64_t or_chung_lu_samples(uint64_t n, double avg_deg) { return (uint64_t)((double)n * avg_deg / 2.0 + 0.5); }
+
+int or_chung_lu_cdf(uint64_t n, double gamma, uint64_t* cdf) {
+  if (n < 1 || gamma <= 1.0) return OR_ERR_ARG;
+  const double e = -1.0 / (gamma - 1.0);
+  double* cum = (double*)malloc(n * sizeof(double));
+  if (!cum) return OR_ERR_NOMEM;
+  double acc = 0.0;
+  for (uint64_t i = 0; i < n; ++i) {
+    acc += pow((double)(i + 1), e);
+    cum[i] = acc;
+  }
+  const double scale = 4611686018427387904.0 / acc; /* 2^62 / total */
+  for (uint64_t i = 0; i + 1 < n; ++i) {
+    double v = cum[i] * scale;
+    if (v > 4611686018427387904.0) v = 4611686018427387904.0;
+    cdf[i] = (uint64_t)v;
+  }
+  cdf[n - 1] = 1ull << 62;
+  free(cum);
+  return OR_OK;
+}
+
+static uint64_t cdf_pick(const uint64_t* cdf, uint64_t n, uint64_t r) {
+  uint64_t lo = 0, hi = n; /* first i with cdf[i] > r */
+  while (lo < hi) {
+    uint64_t mid = lo + (hi - lo) / 2;
+    if (cdf[mid] <= r)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo < n ? lo : n - 1;
+}
+
+int or_gen_chung_lu(uint64_t n, double gamma, double avg_deg, uint64_t seed, uint64_t* keys) {
+  if (n < 2 || n > 0xFFFFFFFEull) return OR_ERR_ARG;
+  uint64_t* cdf = (uint64_t*)malloc(n * sizeof(uint64_t));
+  if (!cdf) return OR_ERR_NOMEM;
+  int st = or_chung_lu_cdf(n, gamma, cdf);
+  if (st) {
+    free(cdf);
+    return st;
+  }
+  const uint64_t M = or_chung_lu_samples(n, avg_deg);
+  const uint64_t key = or_stream_key(seed, OR_STREAM_CL);
+#pragma omp parallel for schedule(static)
+  for (int64_t e = 0; e < (int64_t)M; ++e) {
+    uint64_t u = cdf_pick(cdf, n, or_rand(key, 2 * (uint64_t)e) >> 2);
+    uint64_t v = cdf_pick(cdf, n, or_rand(key, 2 * (uint64_t)e + 1) >> 2);
+    keys[e] = (u == v) ? OR_SENTINEL : (u < v ? ((u << 32) | v) : ((v << 32) | u));
+  }
+  free(cdf);
+  return OR_OK;
+}
+
+void or_chung_lu_batch_params(uint64_t seed, uint64_t index, double* gamma, uint64_t* n, double* avg_deg) {
+  const uint64_t key = or_stream_key(seed, OR_STREAM_CLP);
+  const double u0 = or_unit(or_rand(key, 3 * index));
+  const double u1 = or_unit(or_rand(key, 3 * index + 1));
+  const double u2 = or_unit(or_rand(key, 3 * index + 2));
+  *gamma = 2.1 + 0.7 * u0;
+  *n = (uint64_t)(1024.0 * exp2(8.0 * u1) + 0.5);
+  *avg_deg = 8.0 + 56.0 * u2;
+}
+
+uint64_t or_gen_rgg(uint64_t n, double avg_deg, uint64_t seed, uint64_t* keys, uint64_t cap) {
+  const double two30 = 1073741824.0;
+  const uint64_t R = (uint64_t)(sqrt(avg_deg / (3.14159265358979323846 * (double)n)) * two30);
+  if (n < 2 || R < 1) return 0;
+  const uint64_t C = ((1ull << 30) + R - 1) / R; /* cells per side */
+  const uint64_t key = or_stream_key(seed, OR_STREAM_RGG);
+  uint32_t* px = (uint32_t*)malloc(n * 4);
+  uint32_t* py = (uint32_t*)malloc(n * 4);
+  uint64_t* ord = (uint64_t*)malloc(n * 8);
+  for (uint64_t i = 0; i < n; ++i) {
+    uint64_t w = or_rand(key, i);
+    px[i] = (uint32_t)(w & 0x3FFFFFFFull);
+    py[i] = (uint32_t)((w >> 32) & 0x3FFFFFFFull);
+    uint64_t cell = (py[i] / R) * C + (px[i] / R);
+    ord[i] = (cell << 32) | i; /* n < 2^32 */
+  }
+  or_sort_u64(ord, n);
+  /* sorted position = new vertex id; cell_start over sorted order */
+  uint64_t ncell = C * C;
+  uint64_t* cstart = (uint64_t*)calloc(ncell + 1, 8);
+  uint32_t* sx = (uint32_t*)malloc(n * 4);
+  uint32_t* sy = (uint32_t*)malloc(n * 4);
+  for (uint64_t k = 0; k < n; ++k) {
+    uint64_t i = ord[k] & 0xFFFFFFFFull;
+    sx[k] = px[i];
+    sy[k] = py[i];
+    cstart[(ord[k] >> 32) + 1]++;
+  }
+  for (uint64_t c = 0; c < ncell; ++c) cstart[c + 1] += cstart[c];
+  const uint64_t R2 = R * R;
+  uint64_t out = 0;
+  int overflow = 0;
+  for (uint64_t k = 0; k < n && !overflow; ++k) {
+    int64_t cx = (int64_t)(sx[k] / R), cy = (int64_t)(sy[k] / R);
+    for (int64_t dy = -1; dy <= 1; ++dy)
+      for (int64_t dx = -1; dx <= 1; ++dx) {
+        int64_t nx = cx + dx, ny = cy + dy;
+        if (nx < 0 || ny < 0 || nx >= (int64_t)C || ny >= (int64_t)C) continue;
+        uint64_t c = (uint64_t)ny * C + (uint64_t)nx;
+        for (uint64_t j = cstart[c]; j < cstart[c + 1]; ++j) {
+          if (j <= k) continue;
+          uint64_t ddx = sx[k] > sx[j] ? sx[k] - sx[j] : sx[j] - sx[k];
+          uint64_t ddy = sy[k] > sy[j] ? sy[k] - sy[j] : sy[j] - sy[k];
+          if (ddx * ddx + ddy * ddy <= R2) {
+            if (out == cap) {
+              overflow = 1;
+              break;
+            }
+            keys[out++] = (k << 32) | j;
+          }
+        }
+      }
+  }
+  free(px);
+  free(py);
+  free(ord);
+  free(cstart);
+  free(sx);
+  free(sy);
+  return overflow ? UINT64_MAX : out;
+}
+
+static uint64_t or_sbm_out_samples(uint64_t blocks, uint64_t size, double p_out) {
+  const double n = (double)(blocks * size);
+  const double cross = n * (n - 1.0) / 2.0 - (double)blocks * (double)size * ((double)size - 1.0) / 2.0;
+  return (uint64_t)(p_out * cross + 0.5);
+}
+
+uint64_t or_sbm_slots(uint64_t blocks, uint64_t size, double p_out) {
+  return blocks * size * size + or_sbm_out_samples(blocks, size, p_out);
+}
+
+static uint64_t or_prob_threshold(double p) {
+  if (p <= 0.0) return 0;
+  if (p >= 1.0) return UINT64_MAX;
+  return (uint64_t)(p * 18446744073709551616.0);
+}
+
+int or_gen_sbm(uint64_t blocks, uint64_t size, double p_in, double p_out, uint64_t seed, uint64_t* keys) {
+  const uint64_t n = blocks * size;
+  if (blocks < 1 || size < 1 || n > 0xFFFFFFFEull) return OR_ERR_ARG;
+  const uint64_t kin = or_stream_key(seed, OR_STREAM_SBMI), kout = or_stream_key(seed, OR_STREAM_SBMO);
+  const uint64_t tin = or_prob_threshold(p_in);
+  const uint64_t ss = size * size;
+#pragma omp parallel for schedule(static)
+  for (int64_t t = 0; t < (int64_t)(blocks * ss); ++t) {
+    const uint64_t b = (uint64_t)t / ss, r = (uint64_t)t % ss, i = r / size, j = r % size;
+    uint64_t k = OR_SENTINEL;
+    if (i < j && or_rand(kin, (uint64_t)t) < tin) k = ((b * size + i) << 32) | (b * size + j);
+    keys[t] = k;
+  }
+  const uint64_t mo = or_sbm_out_samples(blocks, size, p_out);
+  uint64_t* ko = keys + blocks * ss;
+#pragma omp parallel for schedule(static)
+  for (int64_t e = 0; e < (int64_t)mo; ++e) {
+    const uint64_t w = or_rand(kout, (uint64_t)e);
+    const uint64_t u = ((w & 0xFFFFFFFFull) * n) >> 32, v = ((w >> 32) * n) >> 32;
+    uint64_t k = OR_SENTINEL;
+    if (u / size != v / size) k = u < v ? ((u << 32) | v) : ((v << 32) | u);
+    ko[e] = k;
   }
   return OR_OK;
 }

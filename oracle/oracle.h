@@ -72,6 +72,31 @@ int or_gen_rmat(uint32_t scale, uint32_t ef, double a, double b, double c, uint6
 /* Erdos-Renyi G(n, M): M = n*avg_deg/2 uniform pairs (then dedup). */
 int or_gen_er(uint64_t n, uint64_t nedges, uint64_t seed, uint64_t* keys);
 
+/* Chung-Lu by edge sampling: weights w_i = (i+1)^(-1/(gamma-1)); cdf_i =
+ * floor(sum_{j<=i} w_j / sum w * 2^62) (u64, last = 2^62); M = round(n*avg/2)
+ * samples, each endpoint = first i with cdf_i > (rand >> 2). */
+uint64_t or_chung_lu_samples(uint64_t n, double avg_deg);
+int or_chung_lu_cdf(uint64_t n, double gamma, uint64_t* cdf);
+int or_gen_chung_lu(uint64_t n, double gamma, double avg_deg, uint64_t seed, uint64_t* keys);
+/* Parameters of graph `index` of a Chung-Lu batch drawn from `seed`:
+ * gamma ~ U[2.1, 2.8], n log-uniform in [1024, 262144], avg_deg ~ U[8, 64]. */
+void or_chung_lu_batch_params(uint64_t seed, uint64_t index, double* gamma, uint64_t* n, double* avg_deg);
+/* 2-D random geometric graph: points on a 2^30 x 2^30 integer grid
+ * (x = low 30 bits, y = bits 32..61 of rand(key, i)); edge iff squared
+ * distance <= R^2 with R = floor(sqrt(avg_deg / (pi n)) * 2^30). Vertex ids are
+ * the points' ranks in (cell, index) order, cells of side R row-major.
+ * Writes at most `cap` keys; returns the number written (or UINT64_MAX if cap
+ * is too small). */
+uint64_t or_gen_rgg(uint64_t n, double avg_deg, uint64_t seed, uint64_t* keys, uint64_t cap);
+/* Stochastic block model: `blocks` blocks of `size` consecutive ids; pair
+ * (i<j) in one block kept iff rand(key_in, b*size*size + i*size + j) <
+ * floor(p_in*2^64); plus M_out = round(p_out * cross pairs) samples of uniform
+ * pairs whose endpoints lie in different blocks. keys holds
+ * blocks*size*size + M_out slots (self/duplicate/same-block slots become the
+ * sentinel). */
+uint64_t or_sbm_slots(uint64_t blocks, uint64_t size, double p_out);
+int or_gen_sbm(uint64_t blocks, uint64_t size, double p_in, double p_out, uint64_t seed, uint64_t* keys);
+
 /* ---- preprocessing ------------------------------------------------------ */
 /* Sort ascending + unique + drop OR_SENTINEL, in place. Returns m.
  * Restates radix_sort + unique, tribit.cu:666-667 (CUB SortKeys/Unique). */

@@ -64,6 +64,16 @@ def main() -> None:
     for (nn, mm, seed) in [(5000, 40000, 2), (20000, 200000, 9), (300, 20000, 4)]:
         spec = {"gen": "er", "n": nn, "samples": mm, "seed": seed}
         out.append(record(f"er-n{nn}-m{mm}-seed{seed}", spec, oracle.gen_er(nn, mm, seed), nn))
+    for (nn, gamma, avg, seed) in [(3000, 2.5, 12.0, 7), (20000, 2.1, 30.0, 11), (1024, 2.8, 8.0, 3)]:
+        spec = {"gen": "chung_lu", "n": nn, "gamma": gamma, "avg_deg": avg, "seed": seed}
+        out.append(record(f"chunglu-n{nn}-g{gamma}-d{avg}-seed{seed}", spec,
+                          oracle.gen_chung_lu(nn, gamma, avg, seed), nn))
+    for (nn, avg, seed) in [(20000, 12.0, 3), (5000, 40.0, 8)]:
+        spec = {"gen": "rgg", "n": nn, "avg_deg": avg, "seed": seed}
+        out.append(record(f"rgg-n{nn}-d{avg}-seed{seed}", spec, oracle.gen_rgg(nn, avg, seed), nn))
+    for (nb, bs, pin, pout, seed) in [(20, 64, 0.3, 1e-3, 4), (8, 200, 0.5, 2e-3, 6)]:
+        spec = {"gen": "sbm", "blocks": nb, "size": bs, "p_in": pin, "p_out": pout, "seed": seed}
+        out.append(record(f"sbm-{nb}x{bs}-seed{seed}", spec, oracle.gen_sbm(nb, bs, pin, pout, seed), nb * bs))
     with open(os.path.join(HERE, "golden.json"), "w") as f:
         json.dump(out, f, indent=1)
     for r in out:

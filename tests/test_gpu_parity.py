@@ -20,7 +20,7 @@ import numpy as np
 import pytest
 
 import tribit_amd
-from tests import graphs
+from tests import gens, graphs
 from tests.conftest import GOLDEN
 
 pytestmark = pytest.mark.gpu
@@ -101,17 +101,8 @@ def test_golden_set(gpu_ctx, oracle_mod):
         gold = json.load(f)
     for r in gold:
         note(r["name"])
-        spec = r["spec"]
-        if spec.get("gen") == "rmat":
-            g = gpu_ctx.gen_rmat(spec["scale"], spec["ef"], spec["a"], spec["b"], spec["c"], spec["seed"])
-            okeys = oracle_mod.gen_rmat(spec["scale"], spec["ef"], spec["a"], spec["b"], spec["c"], spec["seed"])
-        elif spec.get("gen") == "er":
-            g = gpu_ctx.gen_er(spec["n"], spec["samples"], spec["seed"])
-            okeys = oracle_mod.gen_er(spec["n"], spec["samples"], spec["seed"])
-        else:
-            keys, n, _ = tribit_amd.read_mtx(os.path.join(GOLDEN, spec["file"]))
-            g = gpu_ctx.graph_from_keys(keys, n)
-            okeys = keys
+        g = gens.gpu_graph(gpu_ctx, r)
+        okeys = gens.oracle_keys(oracle_mod, r)
         with g:
             assert count_all(g) == r["triangles"], r["name"]
             assert_same_structure(g, oracle_mod.Graph.from_keys(okeys, r["n"]))
@@ -219,6 +210,23 @@ def test_rmat20_exact(gpu_ctx, oracle_mod):
         inf = g.info()
         assert (inf["m"], inf["wplus"]) == (og.m, og.wplus())
         assert tri == og.count()
+
+
+@pytest.mark.parametrize("spec", [
+    {"gen": "chung_lu", "n": 262144, "gamma": 2.1, "avg_deg": 64.0, "seed": 13},
+    {"gen": "rgg", "n": 200000, "avg_deg": 24.0, "seed": 21},
+    {"gen": "sbm", "blocks": 100, "size": 512, "p_in": 0.3, "p_out": 1e-4, "seed": 31},
+    {"gen": "er", "n": 1 << 17, "samples": 1 << 21, "seed": 41},
+], ids=lambda s: s["gen"])
+def test_mixed_family_medium(gpu_ctx, oracle_mod, spec):
+    """Scaled-down members of the mixed family (configuration 4) against the oracle: CSR and count."""
+    n = spec["blocks"] * spec["size"] if spec["gen"] == "sbm" else spec["n"]
+    r = {"spec": spec, "n": n}
+    note(f"mixed family {spec}")
+    with gens.gpu_graph(gpu_ctx, r) as g:
+        og = oracle_mod.Graph.from_keys(gens.oracle_keys(oracle_mod, r), n)
+        assert_same_structure(g, og)
+        assert count_all(g) == og.count()
 
 
 def test_rmat22_properties(gpu_ctx, oracle_mod):

@@ -170,3 +170,8 @@ def test_cli_bad_file_fails_before_device(cli_built, tmp_path):
     p = write(tmp_path, "ns.mtx", "%%MM\n3 4 1\n1 2\n")
     r = run_cli("-i", p)
     assert r.returncode == 2 and "not square" in r.stderr
+
+
+def test_chung_lu_batch_params_match_oracle(lib_built, oracle_mod):
+    for i in range(32):
+        assert tribit_amd.chung_lu_batch_params(3, i) == oracle_mod.chung_lu_batch_params(3, i)

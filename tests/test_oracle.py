@@ -17,7 +17,7 @@ from math import comb
 import numpy as np
 import pytest
 
-from tests import graphs
+from tests import gens, graphs
 from tests.conftest import GOLDEN
 
 WIKI = os.path.join(GOLDEN, "wiki-Vote.mtx")
@@ -94,15 +94,7 @@ def test_bruteforce_tiny(oracle_mod, seed):
 
 def test_golden_networkx(oracle_mod):
     for r in golden():
-        spec = r["spec"]
-        if spec.get("gen") == "rmat":
-            keys = oracle_mod.gen_rmat(spec["scale"], spec["ef"], spec["a"], spec["b"], spec["c"], spec["seed"])
-        elif spec.get("gen") == "er":
-            keys = oracle_mod.gen_er(spec["n"], spec["samples"], spec["seed"])
-        else:
-            st, keys, n, _ = oracle_mod.read_mtx(os.path.join(GOLDEN, spec["file"]))
-            assert st == 0
-        g = oracle_mod.Graph.from_keys(keys, r["n"])
+        g = oracle_mod.Graph.from_keys(gens.oracle_keys(oracle_mod, r), r["n"])
         assert (g.m, g.wplus(), g.count()) == (r["m"], r["wplus"], r["triangles"]), r["name"]
 
 
@@ -110,14 +102,47 @@ def test_golden_generator_hashes(oracle_mod):
     import hashlib
 
     for r in golden():
-        spec = r["spec"]
-        if spec.get("gen") == "rmat":
-            keys = oracle_mod.gen_rmat(spec["scale"], spec["ef"], spec["a"], spec["b"], spec["c"], spec["seed"])
-        elif spec.get("gen") == "er":
-            keys = oracle_mod.gen_er(spec["n"], spec["samples"], spec["seed"])
-        else:
+        if "gen" not in r["spec"]:
             continue
+        keys = gens.oracle_keys(oracle_mod, r)
         assert hashlib.sha256(oracle_mod.dedup(keys).tobytes()).hexdigest() == r["keys_sha256"], r["name"]
+
+
+def test_chung_lu_generator(oracle_mod):
+    """CDF is monotone, ends at 2^62, and hub 0 is sampled most often; batch parameters in range."""
+    cdf = oracle_mod.chung_lu_cdf(4096, 2.3)
+    assert np.all(np.diff(cdf.astype(np.float64)) >= 0) and int(cdf[-1]) == 1 << 62
+    keys = oracle_mod.gen_chung_lu(4096, 2.3, 20.0, 1)
+    ok = keys[keys != np.uint64(0xFFFFFFFFFFFFFFFF)]
+    ends = np.concatenate([(ok >> np.uint64(32)), ok & np.uint64(0xFFFFFFFF)]).astype(np.int64)
+    deg = np.bincount(ends, minlength=4096)
+    assert deg[0] == deg.max() and deg[:10].mean() > 5 * deg.mean()
+    for i in range(64):
+        g, n, d = oracle_mod.chung_lu_batch_params(3, i)
+        assert 2.1 <= g < 2.8 and 1024 <= n <= 262144 and 8.0 <= d < 64.0
+
+
+def test_rgg_generator(oracle_mod):
+    """Average degree near the target; ids are spatially ordered (most edges join close ids)."""
+    n, avg = 20000, 12.0
+    keys = oracle_mod.gen_rgg(n, avg, 3)
+    m = oracle_mod.dedup(keys).size
+    assert m == keys.size  # every pair emitted once
+    assert abs(2 * m / n - avg) / avg < 0.1  # boundary effects lower the degree slightly
+    lo = (keys >> np.uint64(32)).astype(np.int64)
+    hi = (keys & np.uint64(0xFFFFFFFF)).astype(np.int64)
+    assert np.median(hi - lo) < n / 50
+
+
+def test_sbm_generator(oracle_mod):
+    nb, bs = 20, 64
+    keys = oracle_mod.dedup(oracle_mod.gen_sbm(nb, bs, 0.3, 1e-3, 4))
+    lo = (keys >> np.uint64(32)).astype(np.int64)
+    hi = (keys & np.uint64(0xFFFFFFFF)).astype(np.int64)
+    inner = (lo // bs) == (hi // bs)
+    pairs_in = nb * bs * (bs - 1) // 2
+    assert abs(inner.sum() / pairs_in - 0.3) < 0.02
+    assert 0 < (~inner).sum() <= round(1e-3 * ((nb * bs) * (nb * bs - 1) / 2 - pairs_in)) + 1
 
 
 def test_invariances(oracle_mod):

@@ -106,6 +106,13 @@ def load_library(path: str | None = None) -> ctypes.CDLL:
         "tribit_gen_rmat": (ctypes.c_int, [_P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_double, ctypes.c_double,
                                            ctypes.c_double, _U64, ctypes.POINTER(_P), _U64P]),
         "tribit_gen_er": (ctypes.c_int, [_P, _U64, _U64, _U64, ctypes.POINTER(_P), _U64P]),
+        "tribit_gen_chung_lu": (ctypes.c_int, [_P, _U64, ctypes.c_double, ctypes.c_double, _U64, ctypes.POINTER(_P),
+                                               _U64P]),
+        "tribit_chung_lu_batch_params": (None, [_U64, _U64, ctypes.POINTER(ctypes.c_double), _U64P,
+                                                ctypes.POINTER(ctypes.c_double)]),
+        "tribit_gen_rgg": (ctypes.c_int, [_P, _U64, ctypes.c_double, _U64, ctypes.POINTER(_P), _U64P]),
+        "tribit_gen_sbm": (ctypes.c_int, [_P, _U64, _U64, ctypes.c_double, ctypes.c_double, _U64, ctypes.POINTER(_P),
+                                          _U64P]),
         "tribit_graph_from_host_keys": (ctypes.c_int, [_P, _P, _U64, _U64, ctypes.POINTER(_P)]),
         "tribit_graph_from_device_keys": (ctypes.c_int, [_P, _P, _U64, _U64, ctypes.POINTER(_P)]),
         "tribit_graph_get_info": (ctypes.c_int, [_P, ctypes.POINTER(GraphInfo)]),
@@ -128,6 +135,13 @@ def load_library(path: str | None = None) -> ctypes.CDLL:
 def _check(st: int, what: str) -> None:
     if st != 0:
         raise TribitError(st, what)
+
+
+def chung_lu_batch_params(seed: int, index: int):
+    """(gamma, n, avg_deg) of graph `index` of a Chung-Lu batch drawn from `seed`."""
+    g, d, n = ctypes.c_double(), ctypes.c_double(), _U64()
+    load_library().tribit_chung_lu_batch_params(seed, index, ctypes.byref(g), ctypes.byref(n), ctypes.byref(d))
+    return g.value, n.value, d.value
 
 
 def build_info() -> str:
@@ -277,6 +291,28 @@ class Context:
         _check(self.lib.tribit_graph_from_device_keys(self.h, d, nk.value, n, ctypes.byref(g)),
                "graph_from_device_keys")
         return Graph(self, g.value)
+
+    def _from_generated(self, st: int, d, nk, n: int, what: str) -> Graph:
+        _check(st, what)
+        g = _P()
+        _check(self.lib.tribit_graph_from_device_keys(self.h, d, nk.value, n, ctypes.byref(g)),
+               "graph_from_device_keys")
+        return Graph(self, g.value)
+
+    def gen_chung_lu(self, n: int, gamma: float, avg_deg: float, seed: int) -> Graph:
+        d, nk = _P(), _U64()
+        st = self.lib.tribit_gen_chung_lu(self.h, n, gamma, avg_deg, seed, ctypes.byref(d), ctypes.byref(nk))
+        return self._from_generated(st, d, nk, n, "gen_chung_lu")
+
+    def gen_rgg(self, n: int, avg_deg: float, seed: int) -> Graph:
+        d, nk = _P(), _U64()
+        st = self.lib.tribit_gen_rgg(self.h, n, avg_deg, seed, ctypes.byref(d), ctypes.byref(nk))
+        return self._from_generated(st, d, nk, n, "gen_rgg")
+
+    def gen_sbm(self, blocks: int, size: int, p_in: float, p_out: float, seed: int) -> Graph:
+        d, nk = _P(), _U64()
+        st = self.lib.tribit_gen_sbm(self.h, blocks, size, p_in, p_out, seed, ctypes.byref(d), ctypes.byref(nk))
+        return self._from_generated(st, d, nk, blocks * size, "gen_sbm")
 
     def count_keys(self, keys: np.ndarray, n: int, sched: str = "plain") -> tuple[int, float]:
         k = np.ascontiguousarray(keys, dtype=np.uint64)

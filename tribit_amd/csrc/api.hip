@@ -194,6 +194,65 @@ int tribit_gen_er(tribit_ctx* ctx, uint64_t n, uint64_t nsamples, uint64_t seed,
   });
 }
 
+int tribit_gen_chung_lu(tribit_ctx* ctx, uint64_t n, double gamma, double avg_deg, uint64_t seed,
+                        uint64_t** d_keys, uint64_t* nkeys) {
+  if (!ctx || !d_keys || !nkeys || n < 2 || n > 0xFFFFFFFEull || !(gamma > 1.0) || !(avg_deg >= 0.0))
+    return TRIBIT_ERR_ARG;
+  return guarded(ctx, [&] {
+    TB_HIP(hipSetDevice(ctx->device));
+    const uint64_t ne = tb::chung_lu_samples(n, avg_deg);
+    uint64_t* p = ctx->alloc_n<uint64_t>(ne ? ne : 1);
+    try {
+      tb::gen_chung_lu(ctx, n, gamma, avg_deg, seed, p);
+    } catch (...) {
+      ctx->free(p);
+      throw;
+    }
+    *d_keys = p;
+    *nkeys = ne;
+  });
+}
+
+void tribit_chung_lu_batch_params(uint64_t seed, uint64_t index, double* gamma, uint64_t* n, double* avg_deg) {
+  double g = 0, d = 0;
+  uint64_t nn = 0;
+  tb::chung_lu_batch_params(seed, index, &g, &nn, &d);
+  if (gamma) *gamma = g;
+  if (n) *n = nn;
+  if (avg_deg) *avg_deg = d;
+}
+
+int tribit_gen_rgg(tribit_ctx* ctx, uint64_t n, double avg_deg, uint64_t seed, uint64_t** d_keys, uint64_t* nkeys) {
+  if (!ctx || !d_keys || !nkeys || n < 2 || n > 0xFFFFFFFEull || !(avg_deg > 0.0)) return TRIBIT_ERR_ARG;
+  return guarded(ctx, [&] {
+    TB_HIP(hipSetDevice(ctx->device));
+    uint64_t ne = 0;
+    uint64_t* p = tb::gen_rgg(ctx, n, avg_deg, seed, &ne);
+    *d_keys = p;
+    *nkeys = ne;
+  });
+}
+
+int tribit_gen_sbm(tribit_ctx* ctx, uint64_t blocks, uint64_t size, double p_in, double p_out, uint64_t seed,
+                   uint64_t** d_keys, uint64_t* nkeys) {
+  if (!ctx || !d_keys || !nkeys || blocks < 1 || size < 1 || blocks * size > 0xFFFFFFFEull ||
+      !(p_in >= 0.0 && p_in <= 1.0) || !(p_out >= 0.0 && p_out <= 1.0))
+    return TRIBIT_ERR_ARG;
+  return guarded(ctx, [&] {
+    TB_HIP(hipSetDevice(ctx->device));
+    const uint64_t slots = tb::sbm_slots(blocks, size, p_out);
+    uint64_t* p = ctx->alloc_n<uint64_t>(slots ? slots : 1);
+    try {
+      tb::gen_sbm(ctx, blocks, size, p_in, p_out, seed, p);
+    } catch (...) {
+      ctx->free(p);
+      throw;
+    }
+    *d_keys = p;
+    *nkeys = slots;
+  });
+}
+
 int tribit_graph_from_device_keys(tribit_ctx* ctx, uint64_t* d_keys, uint64_t nkeys, uint64_t n,
                                   tribit_graph** out) {
   if (!ctx || !out || (!d_keys && nkeys)) return TRIBIT_ERR_ARG;

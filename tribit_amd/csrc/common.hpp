@@ -152,6 +152,15 @@ void gen_rmat(tribit_ctx* ctx, uint32_t scale, uint32_t ef, double a, double b, 
               uint64_t* d_keys);
 void gen_er(tribit_ctx* ctx, uint64_t n, uint64_t nsamples, uint64_t seed, uint64_t* d_keys);
 void rmat_thresholds(double a, double b, double c, uint32_t t[3]);
+// gen.hip
+uint64_t chung_lu_samples(uint64_t n, double avg_deg);
+void chung_lu_cdf(uint64_t n, double gamma, uint64_t* cdf);
+void chung_lu_batch_params(uint64_t seed, uint64_t index, double* gamma, uint64_t* n, double* avg_deg);
+void gen_chung_lu(tribit_ctx* ctx, uint64_t n, double gamma, double avg_deg, uint64_t seed, uint64_t* d_keys);
+uint64_t* gen_rgg(tribit_ctx* ctx, uint64_t n, double avg_deg, uint64_t seed, uint64_t* nkeys);
+uint64_t sbm_slots(uint64_t blocks, uint64_t size, double p_out);
+void gen_sbm(tribit_ctx* ctx, uint64_t blocks, uint64_t size, double p_in, double p_out, uint64_t seed,
+             uint64_t* d_keys);
 // count.hip
 void count(tribit_ctx* ctx, tribit_graph* g, const tribit_count_opts& opts, tribit_count_info* info);
 void variant_stats(tribit_ctx* ctx, tribit_graph* g, int sched, tribit_variant_work* out);
