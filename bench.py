@@ -35,8 +35,10 @@ sys.path.insert(0, ROOT)
 
 METRIC = "exact-TC GTEPS on 1 MI355X: learned-sched vs plain; % HBM roofline"
 HBM_PEAK_GBS = 8000.0
-VARIANT_NAMES = ["k_count_wave<6,64,16,8>", "k_count_wave<9,256,16,8>", "k_count_block<12,16,8>",
-                 "k_count_block<12,16,8> (id-range chunks)"]
+def variant_name(v: int, width: int) -> str:
+    """Kernel instantiation of variant v (class) at lane-group width `width` (32-bit buffer-load path)."""
+    return [f"k_count_wave<6,64,{width},8,true>", f"k_count_wave<9,256,{width},8,true>",
+            f"k_count_block<12,{width},8,true>", f"k_count_block<12,{width},8,true> (id-range chunks)"][v]
 
 
 def log(msg: str) -> None:
@@ -132,6 +134,7 @@ def measure(g, sched: str, steps: int, warmup: int):
         "ms_sched": statistics.median(r.ms_sched for r in results),
         "ms_var": [statistics.median(r.ms_by_variant[v] for r in results) for v in range(nv)],
         "tasks_by_variant": results[0].tasks_by_variant,
+        "group_width": list(results[0].group_width),
     }
     log(f"{sched}: {steps} steps, {out['ms_per_step']:.2f} ms/step wall, {out['ms_dev']:.2f} ms device; "
         f"variants {[round(x, 3) for x in out['ms_var']]}")
@@ -152,6 +155,43 @@ def copy_calibration() -> int:
     return 0
 
 
+MIXED = [
+    ("rmat", "R-MAT scale 20, edge factor 16, seed 7", lambda c: c.gen_rmat(20, 16, 0.57, 0.19, 0.19, 7)),
+    ("er", "Erdos-Renyi n=1,048,576, avg degree 32 (G(n,M), M=16,777,216), seed 7",
+     lambda c: c.gen_er(1 << 20, 1 << 24, 7)),
+    ("rgg", "2-D random geometric n=2,097,152, avg degree 24, seed 7", lambda c: c.gen_rgg(1 << 21, 24.0, 7)),
+    ("sbm", "SBM 1000 blocks x 512, p_in 0.3, p_out 1e-4, seed 7", lambda c: c.gen_sbm(1000, 512, 0.3, 1e-4, 7)),
+]
+
+
+def mixed_family(args) -> int:
+    """One JSON line per member of the mixed family (plain vs selected schedule)."""
+    import tribit_amd
+
+    ctx = tribit_amd.Context(0)
+    for name, desc, make in MIXED:
+        g = make(ctx)
+        inf = g.info()
+        log(f"{name}: n={inf['n']} m={inf['m']} W+={inf['wplus']} prep={inf['ms_preprocess']:.1f} ms")
+        sel = measure(g, args.sched, args.steps, args.warmup)
+        plain = measure(g, "plain", args.steps, args.warmup)
+        if sel is None or plain is None or sel["triangles"] != plain["triangles"]:
+            return 1
+        m = inf["m"]
+        print(json.dumps({
+            "metric": METRIC, "workload": desc, "n": inf["n"], "m": m, "wplus": inf["wplus"],
+            "triangles": sel["triangles"], "sched": args.sched,
+            "gteps": round(m / (sel["ms_per_step"] * 1e-3) / 1e9, 4), "ms_count_device": round(sel["ms_dev"], 4),
+            "ms_by_variant": [round(x, 4) for x in sel["ms_var"]],
+            "plain_gteps": round(m / (plain["ms_per_step"] * 1e-3) / 1e9, 4),
+            "plain_ms_count_device": round(plain["ms_dev"], 4),
+            "ms_preprocess_device": round(inf["ms_preprocess"], 3),
+        }), flush=True)
+        g.free()
+    ctx.close()
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -166,9 +206,14 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (reference kernel for HBM byte counters)")
+    ap.add_argument("--workload", choices=["rmat", "mixed"], default="rmat",
+                    help="rmat: one R-MAT graph (--scale/--ef/--seed); mixed: the four-family set "
+                         "(R-MAT s20, Erdos-Renyi 2^20 avg 32, 2-D RGG 2^21 avg 24, SBM 1000x512), one line each")
     args = ap.parse_args()
     if args.copy_calibration:
         return copy_calibration()
+    if args.workload == "mixed":
+        return mixed_family(args)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU path is a design, not a build.",
               file=sys.stderr)
@@ -242,7 +287,7 @@ def main() -> int:
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": VARIANT_NAMES[dom] if dom < len(VARIANT_NAMES) else f"variant{dom}",
+            "kernel": variant_name(dom, sel["group_width"][dom]),
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
