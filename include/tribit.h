@@ -58,7 +58,7 @@ typedef struct {
   uint32_t max_outdeg;     /* max oriented out-degree */
   uint32_t reserved0;
   uint64_t tiles32;        /* 32-column bit tiles (0 if not built) */
-  uint64_t tiles64;        /* 64-column bit tiles (0 if not built) */
+  uint64_t tiles64;        /* 64-column bit tiles of the oriented rows */
   double ms_preprocess;    /* device time: keys in HBM -> count-ready (hipEvents) */
   uint64_t device_bytes;   /* bytes held by the resident structures */
 } tribit_graph_info;
@@ -73,6 +73,8 @@ typedef enum {
 /* tribit_count_opts.flags */
 #define TRIBIT_COUNT_FORCE_LINEAR_PROBE 1 /* test hook: skip the cuckoo sets, use the fallback tables */
 #define TRIBIT_COUNT_FORCE_GLOBAL_LOADS 2 /* test hook: stream with global loads (the path for col >= 4 GiB) */
+#define TRIBIT_COUNT_NO_TILES 4           /* never use the bit-tile variant */
+#define TRIBIT_COUNT_FORCE_TILES 8        /* use the bit-tile variant for every owner list (test hook) */
 
 typedef struct {
   int32_t sched;           /* tribit_sched */
@@ -114,6 +116,10 @@ int tribit_init(int device, tribit_ctx** out);
 int tribit_shutdown(tribit_ctx* ctx);
 /* Test hook: fail device allocations beyond `bytes` in total (0 = no cap). */
 int tribit_set_alloc_cap(tribit_ctx* ctx, uint64_t bytes);
+/* Context options (tribit_set_option). */
+#define TRIBIT_OPT_RELABEL 1 /* 1 (default): preprocessing renumbers vertices by (degree, id)
+                                rank; 0: input ids kept. Counts are the same either way. */
+int tribit_set_option(tribit_ctx* ctx, int option, int64_t value);
 /* Peak bytes held from the memory pool since tribit_init. */
 int tribit_pool_peak(tribit_ctx* ctx, uint64_t* bytes);
 const char* tribit_status_str(int status);
@@ -164,8 +170,12 @@ int tribit_graph_from_device_keys(tribit_ctx* ctx, uint64_t* d_keys, uint64_t nk
                                   tribit_graph** out);
 int tribit_graph_get_info(const tribit_graph* g, tribit_graph_info* info);
 /* Copies the oriented CSR to host buffers (row_ptr[n+1], col[m], deg[n]);
- * any pointer may be NULL. */
+ * any pointer may be NULL. Ids are internal ids (see tribit_graph_labels). */
 int tribit_graph_download(const tribit_graph* g, uint32_t* row_ptr, uint32_t* col, uint32_t* deg);
+/* Input id of every internal id (input_id[n]; may be NULL) and whether the
+ * graph was relabelled (TRIBIT_OPT_RELABEL). Without relabelling the ids are
+ * the input ids. */
+int tribit_graph_labels(const tribit_graph* g, uint32_t* input_id, int* relabeled);
 int tribit_graph_free(tribit_graph* g);
 
 /* ---- counting (tribit.cu:765-802) ------------------------------------------ */

@@ -61,6 +61,11 @@ def lib() -> ctypes.CDLL:
         L.or_dedup.argtypes = [_P, _U64]
         L.or_dedup.restype = _U64
         L.or_orient_csr.argtypes = [_P, _U64, _U64, ctypes.POINTER(_Graph)]
+        L.or_sort_u64.argtypes = [_P, _U64]
+        L.or_sort_u64.restype = None
+        L.or_degree_labels.argtypes = [_P, _U64, _U64, _P, _P]
+        L.or_relabel_keys.argtypes = [_P, _U64, _P, _P]
+        L.or_relabel_keys.restype = None
         L.or_graph_free.argtypes = [ctypes.POINTER(_Graph)]
         L.or_wplus.argtypes = [ctypes.POINTER(_Graph)]
         L.or_wplus.restype = _U64
@@ -148,6 +153,28 @@ def dedup(keys: np.ndarray) -> np.ndarray:
     return k[:m].copy()
 
 
+def degree_labels(ukeys: np.ndarray, n: int):
+    """(old_of_new, new_of_old): new id = rank of (degree, id)."""
+    k = np.ascontiguousarray(ukeys, dtype=np.uint64)
+    o2 = np.zeros(max(n, 1), np.uint32)
+    n2 = np.zeros(max(n, 1), np.uint32)
+    st = lib().or_degree_labels(_ptr(k), k.size, n, _ptr(o2), _ptr(n2))
+    if st != 0:
+        raise ValueError(f"or_degree_labels status {st}")
+    return o2[:n], n2[:n]
+
+
+def relabel(ukeys: np.ndarray, n: int):
+    """Deduplicated keys renumbered by degree rank (sorted) and old_of_new."""
+    k = np.ascontiguousarray(ukeys, dtype=np.uint64)
+    o2, n2 = degree_labels(k, n)
+    out = np.zeros(max(k.size, 1), np.uint64)
+    lib().or_relabel_keys(_ptr(k), k.size, _ptr(n2), _ptr(out))
+    out = out[:k.size].copy()
+    lib().or_sort_u64(_ptr(out), out.size)
+    return out, o2
+
+
 class Graph:
     """Degree-oriented CSR built by the oracle from deduplicated keys."""
 
@@ -160,8 +187,14 @@ class Graph:
         self.n, self.m = self.g.n, self.g.m
 
     @classmethod
-    def from_keys(cls, keys: np.ndarray, n: int) -> "Graph":
-        return cls(dedup(keys), n)
+    def from_keys(cls, keys: np.ndarray, n: int, relabel: bool = False) -> "Graph":
+        u = dedup(keys)
+        if relabel:
+            u, old_of_new = globals()["relabel"](u, n)
+            self = cls(u, n)
+            self.labels = old_of_new
+            return self
+        return cls(u, n)
 
     @classmethod
     def from_csr(cls, row_ptr: np.ndarray, col: np.ndarray) -> "Graph":

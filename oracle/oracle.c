@@ -523,6 +523,45 @@ int or_orient_csr(const uint64_t* ukeys, uint64_t m, uint64_t n, or_graph* g) {
   return OR_OK;
 }
 
+int or_degree_labels(const uint64_t* ukeys, uint64_t m, uint64_t n, uint32_t* old_of_new, uint32_t* new_of_old) {
+  /* counting sort of the vertices by degree, stable in id: new id = rank of (deg, id) */
+  uint32_t* deg = (uint32_t*)calloc(n ? n : 1, sizeof(uint32_t));
+  uint64_t* start = (uint64_t*)calloc(n + 1, sizeof(uint64_t));
+  if (!deg || !start) {
+    free(deg);
+    free(start);
+    return OR_ERR_NOMEM;
+  }
+  for (uint64_t i = 0; i < m; ++i) {
+    uint64_t a = ukeys[i] >> 32, b = ukeys[i] & 0xFFFFFFFFull;
+    if (a >= n || b >= n || a >= b) {
+      free(deg);
+      free(start);
+      return OR_ERR_RANGE;
+    }
+    deg[a]++;
+    deg[b]++;
+  }
+  for (uint64_t v = 0; v < n; ++v) start[deg[v] + 1]++; /* deg <= n-1 */
+  for (uint64_t d = 0; d < n; ++d) start[d + 1] += start[d];
+  for (uint64_t v = 0; v < n; ++v) {
+    uint64_t r = start[deg[v]]++;
+    old_of_new[r] = (uint32_t)v;
+    if (new_of_old) new_of_old[v] = (uint32_t)r;
+  }
+  free(deg);
+  free(start);
+  return OR_OK;
+}
+
+void or_relabel_keys(const uint64_t* ukeys, uint64_t m, const uint32_t* new_of_old, uint64_t* out) {
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < (int64_t)m; ++i) {
+    uint64_t a = new_of_old[ukeys[i] >> 32], b = new_of_old[ukeys[i] & 0xFFFFFFFFull];
+    out[i] = a < b ? ((a << 32) | b) : ((b << 32) | a);
+  }
+}
+
 uint64_t or_wplus(const or_graph* g) {
   uint64_t w = 0;
 #pragma omp parallel for reduction(+ : w) schedule(dynamic, 4096)

@@ -26,6 +26,8 @@
  *                        reference was surveyed (W_L=70,956, m_purged=99,318,
  *                        W_U=54,880, R=2,964, T=113,795) and by equality of
  *                        its count with or_count.
+ *   or_degree_labels     a permutation; degrees non-decreasing along new ids,
+ *                        ties in input-id order; counts unchanged.
  *   or_tiles             popcount(tile masks) == row lengths; decode of the
  *                        tiles == CSR rows.
  */
@@ -115,6 +117,13 @@ typedef struct {
  * deg(a) <= deg(b) (rank = (deg, id), ties by lower id), else b->a. */
 int or_orient_csr(const uint64_t* ukeys, uint64_t m, uint64_t n, or_graph* g);
 void or_graph_free(or_graph* g);
+
+/* Degree relabelling: new id = rank of (deg, id) (a counting sort by degree,
+ * stable in id). old_of_new[n] and, if not NULL, new_of_old[n]. With these ids
+ * the degree orientation above is the id order (a -> b iff a < b). */
+int or_degree_labels(const uint64_t* ukeys, uint64_t m, uint64_t n, uint32_t* old_of_new, uint32_t* new_of_old);
+/* out[i] = (min<<32)|max of the relabelled pair of ukeys[i] (not re-sorted). */
+void or_relabel_keys(const uint64_t* ukeys, uint64_t m, const uint32_t* new_of_old, uint64_t* out);
 
 /* W+ = sum over oriented edges u->v of outdeg(v). */
 uint64_t or_wplus(const or_graph* g);

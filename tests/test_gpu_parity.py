@@ -47,7 +47,14 @@ def gpu_count(ctx, keys, n):
         return count_all(g)
 
 
-def assert_same_structure(g, og):
+def assert_same_structure(g, keys, n, oracle_mod):
+    """Oriented CSR, degrees and labels equal the oracle's (relabelled like the graph)."""
+    labels, relabeled = g.labels()
+    og = oracle_mod.Graph.from_keys(keys, n, relabel=relabeled)
+    if relabeled:
+        assert np.array_equal(labels, og.labels)
+    else:
+        assert np.array_equal(labels, np.arange(n, dtype=np.uint32))
     rp, col, deg = g.download()
     orp, ocol, odeg = og.arrays()
     assert np.array_equal(rp.astype(np.uint64), orp)
@@ -63,7 +70,24 @@ def test_wiki_vote(gpu_ctx, oracle_mod):
     with gpu_ctx.graph_from_keys(keys, n) as g:
         r = g.count(repeats=3)
         assert r.triangles == 608389 and r.repeats_agree
-        assert_same_structure(g, oracle_mod.Graph.from_keys(keys, n))
+        assert_same_structure(g, keys, n, oracle_mod)
+
+
+def test_relabel_option(gpu_ctx, oracle_mod):
+    """Input ids kept (relabelling off) and degree-rank ids give the same counts; both CSRs equal the oracle's."""
+    note("relabel on/off, rmat s14 ef8 seed 3")
+    keys = oracle_mod.gen_rmat(14, 8, seed=3)
+    n = 1 << 14
+    want = oracle_mod.count_keys(keys, n)
+    for on in (False, True):
+        gpu_ctx.set_relabel(on)
+        try:
+            with gpu_ctx.graph_from_keys(keys, n) as g:
+                assert g.labels()[1] == on
+                assert_same_structure(g, keys, n, oracle_mod)
+                assert count_all(g) == want
+        finally:
+            gpu_ctx.set_relabel(True)
 
 
 def test_closed_forms(gpu_ctx):
@@ -105,7 +129,7 @@ def test_golden_set(gpu_ctx, oracle_mod):
         okeys = gens.oracle_keys(oracle_mod, r)
         with g:
             assert count_all(g) == r["triangles"], r["name"]
-            assert_same_structure(g, oracle_mod.Graph.from_keys(okeys, r["n"]))
+            assert_same_structure(g, okeys, r["n"], oracle_mod)
 
 
 def test_block_classes_and_chunks(gpu_ctx, oracle_mod):
@@ -224,9 +248,9 @@ def test_mixed_family_medium(gpu_ctx, oracle_mod, spec):
     r = {"spec": spec, "n": n}
     note(f"mixed family {spec}")
     with gens.gpu_graph(gpu_ctx, r) as g:
-        og = oracle_mod.Graph.from_keys(gens.oracle_keys(oracle_mod, r), n)
-        assert_same_structure(g, og)
-        assert count_all(g) == og.count()
+        okeys = gens.oracle_keys(oracle_mod, r)
+        assert_same_structure(g, okeys, n, oracle_mod)
+        assert count_all(g) == oracle_mod.count_keys(okeys, n)
 
 
 def test_rmat22_properties(gpu_ctx, oracle_mod):

@@ -185,6 +185,36 @@ def test_orientation_invariants(oracle_mod):
     assert np.array_equal(deg, np.bincount(np.concatenate([a, b]), minlength=n).astype(np.uint32))
 
 
+@pytest.mark.parametrize("seed", [1, 6])
+def test_degree_labels(oracle_mod, seed):
+    """Counting-sort labels equal a numpy lexsort of (degree, id); relabelled
+    orientation is the id order and is isomorphic to the degree orientation."""
+    keys = oracle_mod.gen_rmat(12, 8, seed=seed)
+    n = 1 << 12
+    u = oracle_mod.dedup(keys)
+    a = (u >> np.uint64(32)).astype(np.int64)
+    b = (u & np.uint64(0xFFFFFFFF)).astype(np.int64)
+    deg = np.bincount(np.concatenate([a, b]), minlength=n)
+    old_of_new, new_of_old = oracle_mod.degree_labels(u, n)
+    assert np.array_equal(old_of_new, np.lexsort((np.arange(n), deg)).astype(np.uint32))
+    assert np.array_equal(new_of_old[old_of_new], np.arange(n, dtype=np.uint32))
+    g0 = oracle_mod.Graph(u, n)
+    g1 = oracle_mod.Graph.from_keys(keys, n, relabel=True)
+    assert np.array_equal(g1.labels, old_of_new)
+    rp, col, d1 = g1.arrays()
+    src = np.repeat(np.arange(n, dtype=np.int64), np.diff(rp).astype(np.int64))
+    assert np.all(src < col.astype(np.int64))
+    assert np.all(np.diff(d1.astype(np.int64)) >= 0)
+    assert np.array_equal(d1, deg[old_of_new].astype(np.uint32))
+    assert (g1.m, g1.wplus(), g1.count()) == (g0.m, g0.wplus(), g0.count())
+    # mapping the relabelled arcs back gives the degree orientation's arcs
+    rp0, col0, _ = g0.arrays()
+    src0 = np.repeat(np.arange(n, dtype=np.uint64), np.diff(rp0).astype(np.int64))
+    arcs = (old_of_new[src].astype(np.uint64) << np.uint64(32)) | old_of_new[col].astype(np.uint64)
+    arcs0 = (src0 << np.uint64(32)) | col0.astype(np.uint64)
+    assert np.array_equal(np.sort(arcs), np.sort(arcs0))
+
+
 @pytest.mark.parametrize("width", [32, 64])
 def test_tiles_decode(oracle_mod, width):
     keys = oracle_mod.gen_rmat(11, 12, seed=3)

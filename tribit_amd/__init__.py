@@ -54,6 +54,9 @@ class CountOpts(ctypes.Structure):
 
 FORCE_LINEAR_PROBE = 1
 FORCE_GLOBAL_LOADS = 2
+NO_TILES = 4
+FORCE_TILES = 8
+OPT_RELABEL = 1
 
 
 class CountInfo(ctypes.Structure):
@@ -117,6 +120,8 @@ def load_library(path: str | None = None) -> ctypes.CDLL:
         "tribit_graph_from_device_keys": (ctypes.c_int, [_P, _P, _U64, _U64, ctypes.POINTER(_P)]),
         "tribit_graph_get_info": (ctypes.c_int, [_P, ctypes.POINTER(GraphInfo)]),
         "tribit_graph_download": (ctypes.c_int, [_P, _P, _P, _P]),
+        "tribit_graph_labels": (ctypes.c_int, [_P, _P, ctypes.POINTER(ctypes.c_int)]),
+        "tribit_set_option": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int64]),
         "tribit_graph_free": (ctypes.c_int, [_P]),
         "tribit_count": (ctypes.c_int, [_P, _P, ctypes.POINTER(CountOpts), ctypes.POINTER(CountInfo)]),
         "tribit_variant_stats": (ctypes.c_int, [_P, _P, ctypes.c_int, ctypes.POINTER(VariantStats)]),
@@ -200,6 +205,14 @@ class Graph:
                "graph_download")
         return rp, col[:m], deg[:n]
 
+    def labels(self):
+        """(input id of every internal id uint32[n], relabelled?)."""
+        n = self.info()["n"]
+        ids = np.zeros(max(n, 1), np.uint32)
+        rl = ctypes.c_int()
+        _check(self.ctx.lib.tribit_graph_labels(self.h, ids.ctypes.data, ctypes.byref(rl)), "graph_labels")
+        return ids[:n], bool(rl.value)
+
     def count(self, sched: str = "plain", repeats: int = 1, flags: int = 0, group_width=None) -> CountResult:
         o = CountOpts()
         o.sched = SCHED[sched]
@@ -257,6 +270,10 @@ class Context:
         peak = ctypes.c_double()
         _check(self.lib.tribit_device_info(self.h, buf, 256, ctypes.byref(cus), ctypes.byref(peak)), "device_info")
         return {"name": buf.value.decode(), "cus": cus.value, "hbm_peak_gbs": peak.value}
+
+    def set_relabel(self, on: bool) -> None:
+        """Degree-rank relabelling of graphs built from now on (default on)."""
+        _check(self.lib.tribit_set_option(self.h, OPT_RELABEL, 1 if on else 0), "set_option")
 
     def set_alloc_cap(self, nbytes: int) -> None:
         _check(self.lib.tribit_set_alloc_cap(self.h, nbytes), "set_alloc_cap")

@@ -33,12 +33,17 @@ int guarded(tribit_ctx* ctx, F&& f) {
 
 void free_graph_memory(tribit_graph* g) {
   if (!g || !g->ctx) return;
+  tb::free_tiles(g);
   g->ctx->free(g->row_ptr);
   g->ctx->free(g->col);
   g->ctx->free(g->deg);
+  g->ctx->free(g->labels);
+  g->labels = nullptr;
   g->ctx->free(g->wrow);
   g->ctx->free(g->rng_all);
   g->ctx->free(g->rng_dir);
+  g->ctx->free(g->pid_dir);
+  g->pid_dir = nullptr;
   g->ctx->free(g->fw_ptr);
   g->ctx->free(g->rv_ptr);
   g->ctx->free(g->w_fw);
@@ -128,6 +133,18 @@ int tribit_set_alloc_cap(tribit_ctx* ctx, uint64_t bytes) {
   if (!ctx) return TRIBIT_ERR_ARG;
   ctx->cap = bytes;
   return TRIBIT_OK;
+}
+
+int tribit_set_option(tribit_ctx* ctx, int option, int64_t value) {
+  if (!ctx) return TRIBIT_ERR_ARG;
+  switch (option) {
+    case TRIBIT_OPT_RELABEL:
+      if (value != 0 && value != 1) return TRIBIT_ERR_ARG;
+      ctx->relabel = (int)value;
+      return TRIBIT_OK;
+    default:
+      return TRIBIT_ERR_ARG;
+  }
 }
 
 int tribit_pool_peak(tribit_ctx* ctx, uint64_t* bytes) {
@@ -302,6 +319,7 @@ int tribit_graph_get_info(const tribit_graph* g, tribit_graph_info* info) {
   info->max_outdeg = g->max_outdeg;
   info->ms_preprocess = g->ms_prep;
   info->device_bytes = g->bytes;
+  info->tiles64 = g->tiles64;
   return TRIBIT_OK;
 }
 
@@ -314,6 +332,22 @@ int tribit_graph_download(const tribit_graph* g, uint32_t* row_ptr, uint32_t* co
     if (col && g->m) TB_HIP(hipMemcpyAsync(col, g->col, 4 * g->m, hipMemcpyDeviceToHost, ctx->stream));
     if (deg) TB_HIP(hipMemcpyAsync(deg, g->deg, 4 * g->n, hipMemcpyDeviceToHost, ctx->stream));
     ctx->sync();
+  });
+}
+
+int tribit_graph_labels(const tribit_graph* g, uint32_t* input_id, int* relabeled) {
+  if (!g) return TRIBIT_ERR_ARG;
+  if (relabeled) *relabeled = g->relabeled;
+  if (!input_id) return TRIBIT_OK;
+  tribit_ctx* ctx = g->ctx;
+  return guarded(ctx, [&] {
+    TB_HIP(hipSetDevice(ctx->device));
+    if (g->labels) {
+      TB_HIP(hipMemcpyAsync(input_id, g->labels, 4 * g->n, hipMemcpyDeviceToHost, ctx->stream));
+      ctx->sync();
+    } else {
+      for (uint64_t i = 0; i < g->n; ++i) input_id[i] = (uint32_t)i;
+    }
   });
 }
 

@@ -59,7 +59,8 @@ struct tribit_ctx {
   uint64_t held = 0;   // bytes currently allocated through this context
   uint64_t peak = 0;
   std::unordered_map<void*, uint64_t> sizes;
-  hipEvent_t ev[8] = {};
+  hipEvent_t ev[12] = {};
+  int relabel = 1;     // TRIBIT_OPT_RELABEL: degree-rank relabelling in preprocessing
   std::string last_error;
 
   void* alloc(uint64_t bytes) {
@@ -131,16 +132,30 @@ struct tribit_graph {
   uint32_t max_outdeg = 0;
   uint32_t* row_ptr = nullptr;  // n+1, oriented CSR
   uint32_t* col = nullptr;      // m, ascending ids within a row
-  uint32_t* deg = nullptr;      // n, undirected degree
+  uint32_t* deg = nullptr;      // n, undirected degree (by internal id)
+  uint32_t* labels = nullptr;   // n, input id of each internal id (null: ids kept)
+  int relabeled = 0;
   uint64_t* wrow = nullptr;     // n, W+(u) = sum of outdeg(v) over v in N+(u)
   // streamed-range lists (see prep.hip): forward-only and direction-split
   uint2* rng_all = nullptr;     // m, in CSR order; owner u's ranges at row_ptr[u]
   uint2* rng_dir = nullptr;     // m: forward part [0, m_fwd), reverse part [m_fwd, m)
+  uint32_t* pid_dir = nullptr;  // m: the vertex whose list each rng_dir entry streams
   uint32_t* fw_ptr = nullptr;   // n+1, owner's forward ranges in rng_dir
   uint32_t* rv_ptr = nullptr;   // n+1, owner's reverse ranges in rng_dir (offsets include m_fwd)
   uint64_t* w_fw = nullptr;     // n, streamed elements of the owner's forward ranges
   uint64_t* w_rv = nullptr;     // n, streamed elements of the owner's reverse ranges
   uint64_t m_fwd = 0;
+  // 64-column bit tiles (tiles.hip), built on first use
+  uint64_t tiles64 = 0;          // tile count (computed by preprocessing)
+  uint64_t ntiles = 0;           // tiles built (0 until built)
+  uint32_t* tile_ptr = nullptr;  // n+1
+  uint32_t* tile_blk = nullptr;  // ntiles, column / 64
+  uint64_t* tile_mask = nullptr; // ntiles, bit column % 64
+  uint2* trng_all = nullptr;     // m, tile ranges aligned with rng_all
+  uint2* trng_dir = nullptr;     // m, tile ranges aligned with rng_dir
+  uint64_t* tw_all = nullptr;    // n, tiles streamed per owner list (rng_all layout)
+  uint64_t* tw_fw = nullptr;     // n, (rng_dir forward lists)
+  uint64_t* tw_rv = nullptr;     // n, (rng_dir reverse lists)
   double ms_prep = 0.0;
   uint64_t bytes = 0;
 };
@@ -161,6 +176,10 @@ uint64_t* gen_rgg(tribit_ctx* ctx, uint64_t n, double avg_deg, uint64_t seed, ui
 uint64_t sbm_slots(uint64_t blocks, uint64_t size, double p_out);
 void gen_sbm(tribit_ctx* ctx, uint64_t blocks, uint64_t size, double p_in, double p_out, uint64_t seed,
              uint64_t* d_keys);
+// tiles.hip
+uint64_t count_tiles(tribit_ctx* ctx, tribit_graph* g);
+void build_tiles(tribit_ctx* ctx, tribit_graph* g);
+void free_tiles(tribit_graph* g);
 // count.hip
 void count(tribit_ctx* ctx, tribit_graph* g, const tribit_count_opts& opts, tribit_count_info* info);
 void variant_stats(tribit_ctx* ctx, tribit_graph* g, int sched, tribit_variant_work* out);

@@ -123,6 +123,27 @@ __global__ void k_orient(const uint64_t* __restrict__ keys, uint64_t m, const ui
 }
 
 // col[i] = dst, odeg[src] += 1 (sorted by src: aggregated per run).
+// Degree relabelling: new id = rank of (deg, old id); lab[old] = new.
+__global__ void k_iota_u32(uint32_t* __restrict__ p, uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = (uint32_t)i;
+}
+
+__global__ void k_scatter_labels(const uint32_t* __restrict__ old_of_new, uint64_t n, uint32_t* __restrict__ lab) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) lab[old_of_new[i]] = (uint32_t)i;
+}
+
+// With relabelled ids the degree order is the id order: a -> b iff a < b.
+__global__ void k_orient_relabel(const uint64_t* __restrict__ keys, uint64_t m, const uint32_t* __restrict__ lab,
+                                 uint64_t* __restrict__ okeys) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const uint64_t k = keys[i];
+  const uint32_t a = lab[(uint32_t)(k >> 32)], b = lab[(uint32_t)k];
+  okeys[i] = a < b ? (((uint64_t)a << 32) | b) : (((uint64_t)b << 32) | a);
+}
+
 __global__ void k_csr_split(const uint64_t* __restrict__ okeys, uint64_t m, uint32_t* __restrict__ col,
                             uint32_t* __restrict__ odeg) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -210,11 +231,11 @@ __global__ void k_dir_count(const uint32_t* __restrict__ row_ptr, const uint32_t
   rv_out[u] = r;
 }
 
-// Writes the forward ranges of row u at fw_ptr[u] and its reverse keys
-// (v << 32 | u) at rv_off[u].
+// Writes the forward ranges (and partner ids) of row u at fw_ptr[u] and its
+// reverse keys (v << 32 | u) at rv_off[u].
 __global__ void k_dir_fill(const uint32_t* __restrict__ row_ptr, const uint32_t* __restrict__ col, uint64_t n,
                            const uint32_t* __restrict__ fw_ptr, const uint32_t* __restrict__ rv_off,
-                           uint2* __restrict__ rng, uint64_t* __restrict__ rv_keys) {
+                           uint2* __restrict__ rng, uint32_t* __restrict__ pid, uint64_t* __restrict__ rv_keys) {
   const uint64_t u = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (u >= n) return;
   const uint32_t r0 = row_ptr[u], r1 = row_ptr[u + 1], du = r1 - r0;
@@ -222,17 +243,19 @@ __global__ void k_dir_fill(const uint32_t* __restrict__ row_ptr, const uint32_t*
   for (uint32_t e = r0; e < r1; ++e) {
     const uint32_t v = col[e];
     const uint32_t b = row_ptr[v], en = row_ptr[v + 1];
-    if (du < en - b)
+    if (du < en - b) {
       rv_keys[r++] = ((uint64_t)v << 32) | (uint64_t)u;
-    else
+    } else {
+      pid[f] = v;
       rng[f++] = make_uint2(b, en);
+    }
   }
 }
 
 // Sorted reverse keys (v << 32 | u): range of N+(u) at rng[base + j];
 // rv_cnt[v] += 1 and w_rv[v] += outdeg(u), aggregated per run of v in a wave.
 __global__ void k_rv_fill(const uint64_t* __restrict__ keys, uint64_t mr, const uint32_t* __restrict__ row_ptr,
-                          uint2* __restrict__ rng, uint32_t* __restrict__ rv_cnt,
+                          uint2* __restrict__ rng, uint32_t* __restrict__ pid, uint32_t* __restrict__ rv_cnt,
                           unsigned long long* __restrict__ w_rv) {
   const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = j < mr;
@@ -243,6 +266,7 @@ __global__ void k_rv_fill(const uint64_t* __restrict__ keys, uint64_t mr, const 
     const uint32_t u = (uint32_t)k;
     const uint32_t b = row_ptr[u], e = row_ptr[u + 1];
     rng[j] = make_uint2(b, e);
+    pid[j] = u;
     du = e - b;
   }
   const uint32_t lane = lane_id();
@@ -336,6 +360,28 @@ void preprocess(tribit_ctx* ctx, uint64_t* d_keys_in, uint64_t nkeys, uint64_t n
   k_fill_u32<<<grid_for(n, kBlock), kBlock, 0, s>>>(deg.p, n, 0u);
   if (m) k_degree<<<grid_for(m, kBlock), kBlock, 0, s>>>(uniq.p, m, deg.p);
 
+  // 3b. optional degree relabelling: internal id = rank of (deg, id), so the
+  // orientation below becomes the id order and the out-neighbours of
+  // high-degree vertices are packed into a narrow id range at the top.
+  DevBuf<uint32_t> labels;
+  labels.ctx = ctx;
+  if (ctx->relabel) {
+    DevBuf<uint32_t> ids(ctx, n), deg_sorted(ctx, n), lab(ctx, n);
+    labels = DevBuf<uint32_t>(ctx, n);
+    k_iota_u32<<<grid_for(n, kBlock), kBlock, 0, s>>>(ids.p, n);
+    with_temp(ctx, [&](void* t, size_t& b) {
+      return rocprim::radix_sort_pairs(t, b, deg.p, deg_sorted.p, ids.p, labels.p, (size_t)n, 0u,
+                                       bits_for(n), s);
+    });
+    std::swap(deg.p, deg_sorted.p);  // deg now indexed by internal id
+    k_scatter_labels<<<grid_for(n, kBlock), kBlock, 0, s>>>(labels.p, n, lab.p);
+    if (m) {
+      DevBuf<uint64_t> okeys(ctx, m);
+      k_orient_relabel<<<grid_for(m, kBlock), kBlock, 0, s>>>(uniq.p, m, lab.p, okeys.p);
+      std::swap(uniq.p, okeys.p);
+    }
+  }
+
   // 4-5. orient, sort by (src, dst)
   DevBuf<uint32_t> col(ctx, m ? m : 1);
   DevBuf<uint32_t> row_ptr(ctx, n + 1);
@@ -344,7 +390,8 @@ void preprocess(tribit_ctx* ctx, uint64_t* d_keys_in, uint64_t nkeys, uint64_t n
     k_fill_u32<<<grid_for(n + 1, kBlock), kBlock, 0, s>>>(odeg.p, n + 1, 0u);
     if (m) {
       DevBuf<uint64_t> okeys(ctx, m);
-      k_orient<<<grid_for(m, kBlock), kBlock, 0, s>>>(uniq.p, m, deg.p, okeys.p);
+      if (!ctx->relabel) k_orient<<<grid_for(m, kBlock), kBlock, 0, s>>>(uniq.p, m, deg.p, okeys.p);
+      else std::swap(uniq.p, okeys.p);
       sort_keys(ctx, okeys.p, uniq.p, m, end_bit);
       // 6. CSR
       k_csr_split<<<grid_for(m, kBlock), kBlock, 0, s>>>(uniq.p, m, col.p, odeg.p);
@@ -365,6 +412,7 @@ void preprocess(tribit_ctx* ctx, uint64_t* d_keys_in, uint64_t nkeys, uint64_t n
 
   // 8. range lists (forward-only, and direction-split)
   DevBuf<uint2> rng_all(ctx, m ? m : 1), rng_dir(ctx, m ? m : 1);
+  DevBuf<uint32_t> pid_dir(ctx, m ? m : 1);
   DevBuf<uint32_t> fw_ptr(ctx, n + 1), rv_ptr(ctx, n + 1);
   DevBuf<uint64_t> w_fw(ctx, n), w_rv(ctx, n);
   uint32_t m_fwd = 0;
@@ -384,13 +432,13 @@ void preprocess(tribit_ctx* ctx, uint64_t* d_keys_in, uint64_t nkeys, uint64_t n
     const uint64_t m_rev = m - m_fwd;
     DevBuf<uint64_t> rv_keys(ctx, m_rev ? m_rev : 1);
     k_dir_fill<<<grid_for(n, kBlock), kBlock, 0, s>>>(row_ptr.p, col.p, n, fw_ptr.p, rv_off.p, rng_dir.p,
-                                                      rv_keys.p);
+                                                      pid_dir.p, rv_keys.p);
     k_fill_u32<<<grid_for(n + 1, kBlock), kBlock, 0, s>>>(rv_cnt.p, n + 1, 0u);
     if (m_rev) {
       DevBuf<uint64_t> rv_sorted(ctx, m_rev);
       sort_keys(ctx, rv_keys.p, rv_sorted.p, m_rev, end_bit);
       k_rv_fill<<<grid_for(m_rev, kBlock), kBlock, 0, s>>>(rv_sorted.p, m_rev, row_ptr.p, rng_dir.p + m_fwd,
-                                                           rv_cnt.p,
+                                                           pid_dir.p + m_fwd, rv_cnt.p,
                                                            reinterpret_cast<unsigned long long*>(w_rv.p));
     }
     with_temp(ctx, [&](void* t, size_t& b) {
@@ -402,6 +450,23 @@ void preprocess(tribit_ctx* ctx, uint64_t* d_keys_in, uint64_t nkeys, uint64_t n
     k_fill_u32<<<grid_for(n + 1, kBlock), kBlock, 0, s>>>(rv_ptr.p, n + 1, 0u);
     k_fill_u64<<<grid_for(n, kBlock), kBlock, 0, s>>>(reinterpret_cast<unsigned long long*>(w_fw.p), n);
   }
+  // the graph owns its buffers from here on (freed by tribit_graph_free, also on error)
+  g->row_ptr = row_ptr.release();
+  g->col = col.release();
+  g->deg = deg.release();
+  g->labels = labels.release();
+  g->relabeled = ctx->relabel ? 1 : 0;
+  g->wrow = wrow.release();
+  g->m_fwd = m_fwd;
+  g->rng_all = rng_all.release();
+  g->rng_dir = rng_dir.release();
+  g->pid_dir = pid_dir.release();
+  g->fw_ptr = fw_ptr.release();
+  g->rv_ptr = rv_ptr.release();
+  g->w_fw = w_fw.release();
+  g->w_rv = w_rv.release();
+  g->bytes = 4 * (n + 1) * 3 + 4 * (m ? m : 1) + 20 * (m ? m : 1) + 4 * n + 8 * n * 3 + (g->labels ? 4 * n : 0);
+  g->tiles64 = m ? count_tiles(ctx, g) : 0;
   TB_HIP(hipEventRecord(ctx->ev[1], s));
   TB_HIP(hipEventSynchronize(ctx->ev[1]));
   TB_HIP(hipGetLastError());
@@ -411,18 +476,6 @@ void preprocess(tribit_ctx* ctx, uint64_t* d_keys_in, uint64_t nkeys, uint64_t n
   g->wplus = hst.wplus;
   g->rows_active = hst.rows_active;
   g->max_outdeg = hst.max_outdeg;
-  g->row_ptr = row_ptr.release();
-  g->col = col.release();
-  g->deg = deg.release();
-  g->wrow = wrow.release();
-  g->m_fwd = m_fwd;
-  g->rng_all = rng_all.release();
-  g->rng_dir = rng_dir.release();
-  g->fw_ptr = fw_ptr.release();
-  g->rv_ptr = rv_ptr.release();
-  g->w_fw = w_fw.release();
-  g->w_rv = w_rv.release();
-  g->bytes = 4 * (n + 1) * 3 + 4 * (m ? m : 1) + 16 * (m ? m : 1) + 4 * n + 8 * n * 3;
 }
 
 }  // namespace tb
