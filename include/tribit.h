@@ -47,6 +47,7 @@ typedef enum {
 
 typedef struct tribit_ctx tribit_ctx;
 typedef struct tribit_graph tribit_graph;
+typedef struct tribit_mlp tribit_mlp;
 
 /* Count-ready graph description (filled by tribit_graph_get_info). */
 typedef struct {
@@ -67,7 +68,9 @@ typedef struct {
 typedef enum {
   TRIBIT_SCHED_PLAIN = 0,    /* static degree thresholds; textbook direction: N+(u) is the set, N+(v) streamed */
   TRIBIT_SCHED_LEARNED = 1,  /* cost-model driven (without a trained model: the DIRECTED heuristic) */
-  TRIBIT_SCHED_DIRECTED = 2  /* static thresholds; every edge streams the shorter of its two lists */
+  TRIBIT_SCHED_DIRECTED = 2, /* static thresholds; every edge streams the shorter of its two lists */
+  TRIBIT_SCHED_WEDGE = 3     /* relabelled graphs: every arc u->v streams the part of N+(u) after v
+                                against N+(v) (each out-out wedge of u checked once) */
 } tribit_sched;
 
 /* tribit_count_opts.flags */
@@ -75,6 +78,7 @@ typedef enum {
 #define TRIBIT_COUNT_FORCE_GLOBAL_LOADS 2 /* test hook: stream with global loads (the path for col >= 4 GiB) */
 #define TRIBIT_COUNT_NO_TILES 4           /* never use the bit-tile variant */
 #define TRIBIT_COUNT_FORCE_TILES 8        /* use the bit-tile variant for every owner list (test hook) */
+#define TRIBIT_COUNT_NO_BITMAP 16         /* test hook: block tasks keep hashed sets even for narrow id windows */
 
 typedef struct {
   int32_t sched;           /* tribit_sched */
@@ -96,7 +100,7 @@ typedef struct {
   int32_t nvariants;       /* kernel variants (one launch each) */
   double ms_by_variant[8]; /* device time of each variant's launch (median) */
   uint64_t fallback_tasks; /* tasks whose cuckoo set could not be built (linear-probing fallback) */
-  int32_t group_width[4];  /* lanes per streamed list used by each variant */
+  int32_t group_width[8];  /* lanes per streamed list used by each variant */
 } tribit_count_info;
 
 /* Work assigned to each kernel variant by a scheduler, in the units of the
@@ -183,11 +187,71 @@ int tribit_count(tribit_ctx* ctx, tribit_graph* g, const tribit_count_opts* opts
 /* Per-variant work of the scheduler `sched` on this graph (not timed). */
 int tribit_variant_stats(tribit_ctx* ctx, tribit_graph* g, int sched, tribit_variant_work* out);
 
+/* ---- learned scheduler: row-block buckets, menu, timing logs --------------
+ * Owner lists are grouped into TRIBIT_BUCKETS buckets (outdeg bin x bin of the
+ * mean streamed-range length). A plan maps each bucket to a menu slot
+ * (kernel kind + lane-group width). LEARNED picks, per bucket, the feasible
+ * slot of lowest cost predicted by the cost model (tribit_set_cost_model);
+ * without a model, or for a bucket whose prediction is not finite, it keeps
+ * the density-threshold heuristic (outdeg <= 32 / <= 256 / larger). */
+#define TRIBIT_BUCKETS 800
+#define TRIBIT_MENU_SLOTS 8
+#define TRIBIT_FEATURES 16
+typedef struct {
+  uint64_t lists[TRIBIT_BUCKETS];     /* owner lists in the bucket */
+  uint64_t sum_d[TRIBIT_BUCKETS];     /* sum of owner outdegrees */
+  uint64_t sum_k[TRIBIT_BUCKETS];     /* streamed ranges */
+  uint64_t sum_w[TRIBIT_BUCKETS];     /* streamed elements */
+  uint64_t eligible[TRIBIT_BUCKETS];  /* owners whose ids fit a 2^18-id bitmap window */
+  uint64_t ticks[TRIBIT_MENU_SLOTS][TRIBIT_BUCKETS]; /* summed task durations (wall-clock ticks) */
+  uint8_t slot[TRIBIT_BUCKETS];       /* the plan that ran */
+  double tick_ns;                     /* nanoseconds per tick */
+  double ms_kernel;                   /* device time of the counting launches */
+  uint64_t triangles;
+} tribit_profile;
+/* Counts once with the menu plan `table` (slot per bucket; NULL = heuristic)
+ * over the range layout `sched` (PLAIN, DIRECTED or WEDGE) and logs every
+ * task's duration into out->ticks[slot][bucket]. */
+int tribit_count_profile(tribit_ctx* ctx, tribit_graph* g, int sched, const uint8_t* table, tribit_profile* out);
+/* Feature vector (TRIBIT_FEATURES floats) of (bucket, slot) from a profile:
+ * the inputs the cost model is trained on and scored with. */
+int tribit_profile_features(const tribit_profile* p, int bucket, int slot, float* features);
+/* Menu slot description: kind (0 wave/64-slot set, 1 wave/512-slot set,
+ * 2 block), lane-group width and the largest owner outdeg it accepts. */
+int tribit_menu_slot(int slot, int* kind, int* group_width, uint32_t* max_outdeg);
+/* Largest owner outdeg of a bucket (UINT32_MAX for the last outdeg bin). */
+uint32_t tribit_bucket_max_outdeg(int bucket);
+/* Cost model used by TRIBIT_SCHED_LEARNED (NULL: the heuristic). Not owned;
+ * keep it alive while it is set. */
+int tribit_set_cost_model(tribit_ctx* ctx, tribit_mlp* m);
+/* The slot per bucket LEARNED would use on this graph. */
+int tribit_learned_plan(tribit_ctx* ctx, tribit_graph* g, uint8_t* table);
+
 /* One-shot path with the shape of count_triangles (tribit.cu:656): host keys
  * in, exact count out; preprocessing and counting device times reported. */
 int tribit_count_keys(tribit_ctx* ctx, const uint64_t* keys, uint64_t nkeys, uint64_t n,
                       const tribit_count_opts* opts, uint64_t* triangles, double* ms_preprocess,
                       tribit_count_info* info);
+
+/* ---- cost model of the learned scheduler (no reference counterpart) -------
+ * A 3-layer perceptron, F inputs -> 64 -> 64 -> 1 (ReLU), trained in FP32 by
+ * the library's own HIP kernels (forward, backward, Adam; loss = mean squared
+ * error / 2) and evaluated in BF16. Parameters, flat FP32:
+ * W1[F][64] b1[64] W2[64][64] b2[64] W3[64] b3. Samples are row-major X[n][F]. */
+int tribit_mlp_create(tribit_ctx* ctx, int nfeat, uint64_t seed, tribit_mlp** out);
+int tribit_mlp_free(tribit_mlp* m);
+/* Number of parameters (floats) of the model. */
+uint64_t tribit_mlp_num_params(const tribit_mlp* m);
+int tribit_mlp_get_params(tribit_mlp* m, float* params);
+/* Replaces the parameters (and resets the optimiser state). */
+int tribit_mlp_set_params(tribit_mlp* m, const float* params);
+/* Loss and gradient (FP32, device kernels) of one batch X[B][F], t[B]. */
+int tribit_mlp_grad(tribit_mlp* m, const float* X, const float* t, uint32_t B, float* grad, double* loss);
+/* Mini-batch Adam over n samples; *loss = mean loss of the last epoch. */
+int tribit_mlp_train(tribit_mlp* m, const float* X, const float* t, uint64_t n, int epochs, int batch, float lr,
+                     uint64_t seed, double* loss);
+/* BF16 inference: out[n]. */
+int tribit_mlp_infer(tribit_mlp* m, const float* X, uint64_t n, float* out);
 
 #ifdef __cplusplus
 }

@@ -35,9 +35,14 @@ def note(msg):
 SCHEDS = ("plain", "directed", "learned")
 
 
+def scheds_of(g):
+    """Every schedule the graph supports (the wedge schedule needs relabelled ids)."""
+    return SCHEDS + (("wedge",) if g.labels()[1] else ())
+
+
 def count_all(g):
-    """Counts with every scheduler; they must agree (each edge is streamed exactly once)."""
-    got = {s: g.count(sched=s).triangles for s in SCHEDS}
+    """Counts with every scheduler; they must agree (each triangle is found exactly once)."""
+    got = {s: g.count(sched=s).triangles for s in scheds_of(g)}
     assert len(set(got.values())) == 1, got
     return got["plain"]
 
@@ -86,6 +91,9 @@ def test_relabel_option(gpu_ctx, oracle_mod):
                 assert g.labels()[1] == on
                 assert_same_structure(g, keys, n, oracle_mod)
                 assert count_all(g) == want
+                if not on:
+                    with pytest.raises(tribit_amd.TribitError):
+                        g.count(sched="wedge")
         finally:
             gpu_ctx.set_relabel(True)
 
@@ -139,10 +147,13 @@ def test_block_classes_and_chunks(gpu_ctx, oracle_mod):
     k2, t2 = graphs.complete(600, offset=2100)
     keys = np.concatenate([k1, k2])
     with gpu_ctx.graph_from_keys(keys, 2700) as g:
-        for sched in SCHEDS:
+        for sched in scheds_of(g):
             r = g.count(sched=sched)
             assert r.triangles == t1 + t2
-            assert r.tasks_by_variant[3] > 0 and r.tasks_by_variant[2] > 0
+            if sched == "learned":  # menu slots 5..7 are the block kernels
+                assert sum(r.tasks_by_variant[5:8]) > 0
+            else:
+                assert r.tasks_by_variant[3] > 0 and r.tasks_by_variant[2] > 0
     note("K_4500 (3 chunks)")
     keys, t = graphs.complete(4500)
     assert gpu_count(gpu_ctx, keys, 4500) == t
@@ -152,22 +163,51 @@ def test_block_classes_and_chunks(gpu_ctx, oracle_mod):
         assert count_all(g) == oracle_mod.count_keys(keys, 5000)
 
 
+def test_bitmap_and_hashed_sets(gpu_ctx, oracle_mod):
+    """Block tasks whose owner ids fit a 2^18-id window use LDS bitmaps, wider ones cuckoo sets:
+    both, and the hashed sets forced everywhere, give the same counts."""
+    note("bitmap vs hashed owner sets")
+    wide = graphs.complete(300)[0]  # ids 0, 2000, ... (window 598,000 ids)
+    wa = (wide >> np.uint64(32)) * np.uint64(2000)
+    wb = (wide & np.uint64(0xFFFFFFFF)) * np.uint64(2000)
+    narrow_a = (wide >> np.uint64(32)) * np.uint64(500) + np.uint64(1)  # ids 1, 501, ... (149,501 ids)
+    narrow_b = (wide & np.uint64(0xFFFFFFFF)) * np.uint64(500) + np.uint64(1)
+    keys = np.concatenate([graphs.pack(wa, wb), graphs.pack(narrow_a, narrow_b)])
+    n = 600000
+    want = 2 * comb(300, 3)
+    for on in (False, True):
+        gpu_ctx.set_relabel(on)
+        try:
+            with gpu_ctx.graph_from_keys(keys, n) as g:
+                for sched in scheds_of(g):
+                    for fl in (0, tribit_amd.NO_BITMAP, tribit_amd.FORCE_LINEAR_PROBE):
+                        r = g.count(sched=sched, flags=fl)
+                        assert r.triangles == want, (on, sched, fl)
+                        assert (sum(r.tasks_by_variant[5:8]) if sched == "learned" else r.tasks_by_variant[2]) > 0
+        finally:
+            gpu_ctx.set_relabel(True)
+    with gpu_ctx.gen_rmat(20, 16, seed=6) as g:
+        a = g.count(sched="wedge").triangles
+        for sched in scheds_of(g):
+            assert g.count(sched=sched, flags=tribit_amd.NO_BITMAP).triangles == a
+
+
 def test_fallback_linear_probe(gpu_ctx, oracle_mod):
     """The fallback sets (used when every cuckoo build of a task fails) count the same."""
     note("forced linear-probe fallback")
     for spec in [(13, 12, 1), (16, 16, 4)]:
         with gpu_ctx.gen_rmat(spec[0], spec[1], seed=spec[2]) as g:
             a = g.count().triangles
-            for sched in SCHEDS:
+            for sched in scheds_of(g):
                 assert g.count(sched=sched, flags=tribit_amd.FORCE_LINEAR_PROBE).triangles == a
     k1, t1 = graphs.complete(2100)
     with gpu_ctx.graph_from_keys(k1, 2100) as g:
-        for sched in SCHEDS:
+        for sched in scheds_of(g):
             assert g.count(sched=sched, flags=tribit_amd.FORCE_LINEAR_PROBE).triangles == t1
     keys = graphs.random_graph(3000, 0.3, 5)
     with gpu_ctx.graph_from_keys(keys, 3000) as g:
         want = oracle_mod.count_keys(keys, 3000)
-        for sched in SCHEDS:
+        for sched in scheds_of(g):
             assert g.count(sched=sched, flags=tribit_amd.FORCE_LINEAR_PROBE).triangles == want
 
 
@@ -176,13 +216,13 @@ def test_global_load_path(gpu_ctx, oracle_mod):
     note("forced global-load streaming")
     with gpu_ctx.gen_rmat(16, 16, seed=4) as g:
         a = g.count().triangles
-        for sched in SCHEDS:
+        for sched in scheds_of(g):
             assert g.count(sched=sched, flags=tribit_amd.FORCE_GLOBAL_LOADS).triangles == a
             assert g.count(sched=sched, flags=tribit_amd.FORCE_GLOBAL_LOADS | tribit_amd.FORCE_LINEAR_PROBE
                            ).triangles == a
     keys, t = graphs.complete(2300)
     with gpu_ctx.graph_from_keys(keys, 2300) as g:
-        for sched in SCHEDS:
+        for sched in scheds_of(g):
             assert g.count(sched=sched, flags=tribit_amd.FORCE_GLOBAL_LOADS).triangles == t
 
 
@@ -190,7 +230,7 @@ def test_rmat16_repeats(gpu_ctx, oracle_mod):
     note("rmat s16 ef16 seed 4")
     with gpu_ctx.gen_rmat(16, 16, seed=4) as g:
         want = oracle_mod.Graph.from_keys(oracle_mod.gen_rmat(16, 16, seed=4), 1 << 16).count()
-        for sched in SCHEDS:
+        for sched in scheds_of(g):
             r = g.count(sched=sched, repeats=2)
             assert r.triangles == want and r.repeats_agree
 
@@ -294,3 +334,41 @@ def test_cli_generated_config1():
                          timeout=120)
     assert out.returncode == 0, out.stderr
     assert re.search(r"Triangles\s*:\s*(\d+)", out.stdout).group(1) == "689356"
+
+
+def feasible_slots(bucket):
+    top = tribit_amd.bucket_max_outdeg(bucket)
+    return [s for s, (_, _, dmax) in enumerate(tribit_amd.menu()) if top <= dmax]
+
+
+def test_plans_profiles_and_learned_tables(gpu_ctx, oracle_mod):
+    """Every feasible menu plan (round-robin over the slots), the heuristic, an
+    untrained cost model's plan and the timing-log run give the exact count."""
+    note("menu plans / profiles / learned tables, rmat s16 and sbm")
+    cases = [(gpu_ctx.gen_rmat(16, 16, seed=4), oracle_mod.count_keys(oracle_mod.gen_rmat(16, 16, seed=4), 1 << 16)),
+             (None, None)]
+    sbm = {"spec": {"gen": "sbm", "blocks": 20, "size": 300, "p_in": 0.4, "p_out": 1e-3, "seed": 8}, "n": 6000}
+    cases[1] = (gens.gpu_graph(gpu_ctx, sbm), oracle_mod.count_keys(gens.oracle_keys(oracle_mod, sbm), 6000))
+    for g, want in cases:
+        with g:
+            assert g.count(sched="wedge").triangles == want
+            p = g.profile("wedge")
+            assert p.triangles == want and p.tick_ns > 0
+            used = [b for b in range(tribit_amd.BUCKETS) if p.lists[b]]
+            assert used and sum(p.ticks[p.slot[b]][b] for b in used) > 0
+            for r in range(3):
+                table = np.array([feasible_slots(b)[(b + r) % len(feasible_slots(b))]
+                                  for b in range(tribit_amd.BUCKETS)], np.uint8)
+                for layout in ("wedge", "directed", "plain"):
+                    assert g.profile(layout, table).triangles == want, (r, layout)
+            heur = g.learned_plan()
+            assert g.count(sched="learned").triangles == want
+            with gpu_ctx.mlp(seed=3) as m:
+                gpu_ctx.set_cost_model(m)
+                try:
+                    t = g.learned_plan()
+                    assert all(t[b] in feasible_slots(b) for b in range(tribit_amd.BUCKETS))
+                    assert g.count(sched="learned").triangles == want
+                finally:
+                    gpu_ctx.set_cost_model(None)
+            assert np.array_equal(g.learned_plan(), heur)

@@ -28,7 +28,7 @@ STATUS = {
     0: "OK", 1: "ERR_IO", 2: "ERR_FORMAT", 3: "ERR_SHAPE", 4: "ERR_RANGE", 5: "ERR_NOMEM",
     6: "ERR_ARG", 7: "ERR_HIP", 8: "ERR_NODEVICE", 9: "ERR_TOO_LARGE",
 }
-SCHED = {"plain": 0, "learned": 1, "directed": 2}
+SCHED = {"plain": 0, "learned": 1, "directed": 2, "wedge": 3}
 
 
 class TribitError(RuntimeError):
@@ -56,6 +56,7 @@ FORCE_LINEAR_PROBE = 1
 FORCE_GLOBAL_LOADS = 2
 NO_TILES = 4
 FORCE_TILES = 8
+NO_BITMAP = 16
 OPT_RELABEL = 1
 
 
@@ -65,7 +66,22 @@ class CountInfo(ctypes.Structure):
         ("ms_kernel", ctypes.c_double), ("launch_blocks", ctypes.c_uint64), ("tasks", ctypes.c_uint64),
         ("tasks_by_variant", ctypes.c_uint64 * 8), ("repeats_agree", ctypes.c_int32), ("nvariants", ctypes.c_int32),
         ("ms_by_variant", ctypes.c_double * 8), ("fallback_tasks", ctypes.c_uint64),
-        ("group_width", ctypes.c_int32 * 4),
+        ("group_width", ctypes.c_int32 * 8),
+    ]
+
+
+BUCKETS = 800
+MENU_SLOTS = 8
+FEATURES = 16
+
+
+class Profile(ctypes.Structure):
+    _fields_ = [
+        ("lists", ctypes.c_uint64 * BUCKETS), ("sum_d", ctypes.c_uint64 * BUCKETS),
+        ("sum_k", ctypes.c_uint64 * BUCKETS), ("sum_w", ctypes.c_uint64 * BUCKETS),
+        ("eligible", ctypes.c_uint64 * BUCKETS), ("ticks", (ctypes.c_uint64 * BUCKETS) * MENU_SLOTS),
+        ("slot", ctypes.c_uint8 * BUCKETS), ("tick_ns", ctypes.c_double), ("ms_kernel", ctypes.c_double),
+        ("triangles", ctypes.c_uint64),
     ]
 
 
@@ -124,6 +140,22 @@ def load_library(path: str | None = None) -> ctypes.CDLL:
         "tribit_set_option": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int64]),
         "tribit_graph_free": (ctypes.c_int, [_P]),
         "tribit_count": (ctypes.c_int, [_P, _P, ctypes.POINTER(CountOpts), ctypes.POINTER(CountInfo)]),
+        "tribit_count_profile": (ctypes.c_int, [_P, _P, ctypes.c_int, _P, ctypes.POINTER(Profile)]),
+        "tribit_profile_features": (ctypes.c_int, [ctypes.POINTER(Profile), ctypes.c_int, ctypes.c_int, _P]),
+        "tribit_menu_slot": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                                            ctypes.POINTER(ctypes.c_uint32)]),
+        "tribit_bucket_max_outdeg": (ctypes.c_uint32, [ctypes.c_int]),
+        "tribit_set_cost_model": (ctypes.c_int, [_P, _P]),
+        "tribit_learned_plan": (ctypes.c_int, [_P, _P, _P]),
+        "tribit_mlp_create": (ctypes.c_int, [_P, ctypes.c_int, _U64, ctypes.POINTER(_P)]),
+        "tribit_mlp_free": (ctypes.c_int, [_P]),
+        "tribit_mlp_num_params": (_U64, [_P]),
+        "tribit_mlp_get_params": (ctypes.c_int, [_P, _P]),
+        "tribit_mlp_set_params": (ctypes.c_int, [_P, _P]),
+        "tribit_mlp_grad": (ctypes.c_int, [_P, _P, _P, ctypes.c_uint32, _P, ctypes.POINTER(ctypes.c_double)]),
+        "tribit_mlp_train": (ctypes.c_int, [_P, _P, _P, _U64, ctypes.c_int, ctypes.c_int, ctypes.c_float, _U64,
+                                            ctypes.POINTER(ctypes.c_double)]),
+        "tribit_mlp_infer": (ctypes.c_int, [_P, _P, _U64, _P]),
         "tribit_variant_stats": (ctypes.c_int, [_P, _P, ctypes.c_int, ctypes.POINTER(VariantStats)]),
         "tribit_count_keys": (ctypes.c_int, [_P, _P, _U64, _U64, ctypes.POINTER(CountOpts), _U64P,
                                              ctypes.POINTER(ctypes.c_double), ctypes.POINTER(CountInfo)]),
@@ -147,6 +179,26 @@ def chung_lu_batch_params(seed: int, index: int):
     g, d, n = ctypes.c_double(), ctypes.c_double(), _U64()
     load_library().tribit_chung_lu_batch_params(seed, index, ctypes.byref(g), ctypes.byref(n), ctypes.byref(d))
     return g.value, n.value, d.value
+
+
+def profile_features(p: Profile, bucket: int, slot: int) -> np.ndarray:
+    f = np.zeros(FEATURES, np.float32)
+    _check(load_library().tribit_profile_features(ctypes.byref(p), bucket, slot, f.ctypes.data), "profile_features")
+    return f
+
+
+def menu():
+    """[(kind, group_width, max_outdeg)] of the learned scheduler's menu slots."""
+    out = []
+    for s in range(MENU_SLOTS):
+        k, g, d = ctypes.c_int(), ctypes.c_int(), ctypes.c_uint32()
+        _check(load_library().tribit_menu_slot(s, ctypes.byref(k), ctypes.byref(g), ctypes.byref(d)), "menu_slot")
+        out.append((k.value, g.value, d.value))
+    return out
+
+
+def bucket_max_outdeg(bucket: int) -> int:
+    return int(load_library().tribit_bucket_max_outdeg(bucket))
 
 
 def build_info() -> str:
@@ -182,6 +234,65 @@ class CountResult:
     group_width: tuple = ()
 
 
+class Mlp:
+    """The learned scheduler's cost model (F -> 64 -> 64 -> 1), trained by HIP kernels."""
+
+    def __init__(self, ctx: "Context", nfeat: int, seed: int = 1):
+        self.ctx = ctx
+        self.nfeat = nfeat
+        h = _P()
+        _check(ctx.lib.tribit_mlp_create(ctx.h, nfeat, seed, ctypes.byref(h)), "mlp_create")
+        self.h = h
+
+    def num_params(self) -> int:
+        return int(self.ctx.lib.tribit_mlp_num_params(self.h))
+
+    def params(self) -> np.ndarray:
+        p = np.zeros(self.num_params(), np.float32)
+        _check(self.ctx.lib.tribit_mlp_get_params(self.h, p.ctypes.data), "mlp_get_params")
+        return p
+
+    def set_params(self, p: np.ndarray) -> None:
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == self.num_params()
+        _check(self.ctx.lib.tribit_mlp_set_params(self.h, p.ctypes.data), "mlp_set_params")
+
+    def grad(self, X: np.ndarray, t: np.ndarray):
+        X = np.ascontiguousarray(X, np.float32)
+        t = np.ascontiguousarray(t, np.float32)
+        g = np.zeros(self.num_params(), np.float32)
+        loss = ctypes.c_double()
+        _check(self.ctx.lib.tribit_mlp_grad(self.h, X.ctypes.data, t.ctypes.data, t.size, g.ctypes.data,
+                                            ctypes.byref(loss)), "mlp_grad")
+        return loss.value, g
+
+    def train(self, X: np.ndarray, t: np.ndarray, epochs: int = 10, batch: int = 256, lr: float = 1e-3,
+              seed: int = 1) -> float:
+        X = np.ascontiguousarray(X, np.float32)
+        t = np.ascontiguousarray(t, np.float32)
+        loss = ctypes.c_double()
+        _check(self.ctx.lib.tribit_mlp_train(self.h, X.ctypes.data, t.ctypes.data, t.size, epochs, batch, lr, seed,
+                                             ctypes.byref(loss)), "mlp_train")
+        return loss.value
+
+    def infer(self, X: np.ndarray) -> np.ndarray:
+        X = np.ascontiguousarray(X, np.float32)
+        out = np.zeros(max(X.shape[0], 1), np.float32)
+        _check(self.ctx.lib.tribit_mlp_infer(self.h, X.ctypes.data, X.shape[0], out.ctypes.data), "mlp_infer")
+        return out[:X.shape[0]]
+
+    def free(self) -> None:
+        if self.h:
+            self.ctx.lib.tribit_mlp_free(self.h)
+            self.h = _P()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.free()
+
+
 class Graph:
     """Count-ready graph resident in HBM (oriented CSR + per-row wedge counts)."""
 
@@ -204,6 +315,23 @@ class Graph:
         _check(self.ctx.lib.tribit_graph_download(self.h, rp.ctypes.data, col.ctypes.data, deg.ctypes.data),
                "graph_download")
         return rp, col[:m], deg[:n]
+
+    def profile(self, sched: str = "wedge", table=None) -> Profile:
+        """One count under a menu plan (slot per bucket; None = heuristic) with per-task timing logs."""
+        p = Profile()
+        tp = None
+        if table is not None:
+            t = np.ascontiguousarray(table, np.uint8)
+            assert t.size == BUCKETS
+            tp = t.ctypes.data
+        _check(self.ctx.lib.tribit_count_profile(self.ctx.h, self.h, SCHED[sched], tp, ctypes.byref(p)),
+               "count_profile")
+        return p
+
+    def learned_plan(self) -> np.ndarray:
+        t = np.zeros(BUCKETS, np.uint8)
+        _check(self.ctx.lib.tribit_learned_plan(self.ctx.h, self.h, t.ctypes.data), "learned_plan")
+        return t
 
     def labels(self):
         """(input id of every internal id uint32[n], relabelled?)."""
@@ -270,6 +398,13 @@ class Context:
         peak = ctypes.c_double()
         _check(self.lib.tribit_device_info(self.h, buf, 256, ctypes.byref(cus), ctypes.byref(peak)), "device_info")
         return {"name": buf.value.decode(), "cus": cus.value, "hbm_peak_gbs": peak.value}
+
+    def mlp(self, nfeat: int = FEATURES, seed: int = 1) -> Mlp:
+        return Mlp(self, nfeat, seed)
+
+    def set_cost_model(self, m) -> None:
+        """Model used by sched="learned" (None: the density-threshold heuristic)."""
+        _check(self.lib.tribit_set_cost_model(self.h, m.h if m is not None else None), "set_cost_model")
 
     def set_relabel(self, on: bool) -> None:
         """Degree-rank relabelling of graphs built from now on (default on)."""

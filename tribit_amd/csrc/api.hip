@@ -39,6 +39,12 @@ void free_graph_memory(tribit_graph* g) {
   g->ctx->free(g->deg);
   g->ctx->free(g->labels);
   g->labels = nullptr;
+  g->ctx->free(g->rng_sfx);
+  g->ctx->free(g->sfx_ptr);
+  g->ctx->free(g->w_sfx);
+  g->rng_sfx = nullptr;
+  g->sfx_ptr = nullptr;
+  g->w_sfx = nullptr;
   g->ctx->free(g->wrow);
   g->ctx->free(g->rng_all);
   g->ctx->free(g->rng_dir);
@@ -367,9 +373,14 @@ int tribit_count(tribit_ctx* ctx, tribit_graph* g, const tribit_count_opts* opts
   tribit_count_opts o;
   memset(&o, 0, sizeof(o));
   if (opts) o = *opts;
-  if (o.sched != TRIBIT_SCHED_PLAIN && o.sched != TRIBIT_SCHED_LEARNED && o.sched != TRIBIT_SCHED_DIRECTED)
+  if (o.sched != TRIBIT_SCHED_PLAIN && o.sched != TRIBIT_SCHED_LEARNED && o.sched != TRIBIT_SCHED_DIRECTED &&
+      o.sched != TRIBIT_SCHED_WEDGE)
     return TRIBIT_ERR_ARG;
-  if (o.sched == TRIBIT_SCHED_LEARNED) o.sched = TRIBIT_SCHED_DIRECTED;  // no trained model yet
+  if (o.sched == TRIBIT_SCHED_WEDGE && !g->rng_sfx) {
+    ctx->last_error = "the wedge schedule needs a relabelled graph (TRIBIT_OPT_RELABEL)";
+    return TRIBIT_ERR_ARG;
+  }
+
   for (int i = 0; i < 4; ++i) {
     const int w = o.group_width[i];
     if (w != 0 && w != 4 && w != 8 && w != 16 && w != 32 && w != 64) return TRIBIT_ERR_ARG;
@@ -386,8 +397,8 @@ int tribit_count(tribit_ctx* ctx, tribit_graph* g, const tribit_count_opts* opts
 
 int tribit_variant_stats(tribit_ctx* ctx, tribit_graph* g, int sched, tribit_variant_work* out) {
   if (!ctx || !g || !out || g->ctx != ctx) return TRIBIT_ERR_ARG;
-  if (sched < TRIBIT_SCHED_PLAIN || sched > TRIBIT_SCHED_DIRECTED) return TRIBIT_ERR_ARG;
-  if (sched == TRIBIT_SCHED_LEARNED) sched = TRIBIT_SCHED_DIRECTED;
+  if (sched < TRIBIT_SCHED_PLAIN || sched > TRIBIT_SCHED_WEDGE) return TRIBIT_ERR_ARG;
+  if (sched == TRIBIT_SCHED_WEDGE && !g->rng_sfx) return TRIBIT_ERR_ARG;
   return guarded(ctx, [&] {
     TB_HIP(hipSetDevice(ctx->device));
     tb::variant_stats(ctx, g, sched, out);
@@ -411,6 +422,108 @@ int tribit_count_keys(tribit_ctx* ctx, const uint64_t* keys, uint64_t nkeys, uin
   }
   tribit_graph_free(g);
   return st;
+}
+
+int tribit_mlp_create(tribit_ctx* ctx, int nfeat, uint64_t seed, tribit_mlp** out) {
+  if (!ctx || !out) return TRIBIT_ERR_ARG;
+  *out = nullptr;
+  return guarded(ctx, [&] {
+    TB_HIP(hipSetDevice(ctx->device));
+    *out = tb::mlp_create(ctx, nfeat, seed);
+  });
+}
+
+int tribit_mlp_free(tribit_mlp* m) {
+  if (!m) return TRIBIT_ERR_ARG;
+  tribit_ctx* c = tb::mlp_context(m);
+  if (c && c->model == m) c->model = nullptr;
+  tb::mlp_free(m);
+  return TRIBIT_OK;
+}
+
+uint64_t tribit_mlp_num_params(const tribit_mlp* m) { return m ? (uint64_t)tb::mlp_num_params(m) : 0; }
+
+static tribit_ctx* mlp_ctx(tribit_mlp* m) {
+  tribit_ctx* c = tb::mlp_context(m);
+  (void)hipSetDevice(c->device);
+  return c;
+}
+
+int tribit_mlp_get_params(tribit_mlp* m, float* params) {
+  if (!m || !params) return TRIBIT_ERR_ARG;
+  return guarded(mlp_ctx(m), [&] { tb::mlp_get(m, params); });
+}
+
+int tribit_mlp_set_params(tribit_mlp* m, const float* params) {
+  if (!m || !params) return TRIBIT_ERR_ARG;
+  return guarded(mlp_ctx(m), [&] { tb::mlp_set(m, params); });
+}
+
+int tribit_mlp_grad(tribit_mlp* m, const float* X, const float* t, uint32_t B, float* grad, double* loss) {
+  if (!m || !X || !t || !grad || B == 0) return TRIBIT_ERR_ARG;
+  return guarded(mlp_ctx(m), [&] {
+    const double l = tb::mlp_grad(m, X, t, (int)B, grad);
+    if (loss) *loss = l;
+  });
+}
+
+int tribit_mlp_train(tribit_mlp* m, const float* X, const float* t, uint64_t n, int epochs, int batch, float lr,
+                     uint64_t seed, double* loss) {
+  if (!m || !X || !t || n == 0 || epochs < 0 || batch < 1) return TRIBIT_ERR_ARG;
+  return guarded(mlp_ctx(m), [&] {
+    const double l = tb::mlp_train(m, X, t, n, epochs, batch, lr, seed);
+    if (loss) *loss = l;
+  });
+}
+
+int tribit_mlp_infer(tribit_mlp* m, const float* X, uint64_t n, float* out) {
+  if (!m || (n && (!X || !out))) return TRIBIT_ERR_ARG;
+  return guarded(mlp_ctx(m), [&] {
+    if (n) tb::mlp_infer(m, X, n, out);
+  });
+}
+
+int tribit_count_profile(tribit_ctx* ctx, tribit_graph* g, int sched, const uint8_t* table, tribit_profile* out) {
+  if (!ctx || !g || g->ctx != ctx || !out) return TRIBIT_ERR_ARG;
+  if (sched != TRIBIT_SCHED_PLAIN && sched != TRIBIT_SCHED_DIRECTED && sched != TRIBIT_SCHED_WEDGE)
+    return TRIBIT_ERR_ARG;
+  if (sched == TRIBIT_SCHED_WEDGE && !g->rng_sfx) return TRIBIT_ERR_ARG;
+  return guarded(ctx, [&] {
+    TB_HIP(hipSetDevice(ctx->device));
+    tb::count_profile(ctx, g, sched, table, out);
+  });
+}
+
+int tribit_profile_features(const tribit_profile* p, int bucket, int slot, float* features) {
+  if (!p || !features || bucket < 0 || bucket >= TRIBIT_BUCKETS || slot < 0 || slot >= TRIBIT_MENU_SLOTS)
+    return TRIBIT_ERR_ARG;
+  tb::profile_features(p, bucket, slot, features);
+  return TRIBIT_OK;
+}
+
+int tribit_menu_slot(int slot, int* kind, int* group_width, uint32_t* max_outdeg) {
+  if (slot < 0 || slot >= TRIBIT_MENU_SLOTS) return TRIBIT_ERR_ARG;
+  tb::menu_slot(slot, kind, group_width, max_outdeg);
+  return TRIBIT_OK;
+}
+
+uint32_t tribit_bucket_max_outdeg(int bucket) {
+  return (bucket < 0 || bucket >= TRIBIT_BUCKETS) ? 0 : tb::bucket_max_outdeg(bucket);
+}
+
+int tribit_set_cost_model(tribit_ctx* ctx, tribit_mlp* m) {
+  if (!ctx) return TRIBIT_ERR_ARG;
+  if (m && (tb::mlp_context(m) != ctx || tb::mlp_num_features(m) != TRIBIT_FEATURES)) return TRIBIT_ERR_ARG;
+  ctx->model = m;
+  return TRIBIT_OK;
+}
+
+int tribit_learned_plan(tribit_ctx* ctx, tribit_graph* g, uint8_t* table) {
+  if (!ctx || !g || g->ctx != ctx || !table) return TRIBIT_ERR_ARG;
+  return guarded(ctx, [&] {
+    TB_HIP(hipSetDevice(ctx->device));
+    tb::learned_plan(ctx, g, table);
+  });
 }
 
 }  // extern "C"

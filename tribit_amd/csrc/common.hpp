@@ -61,6 +61,7 @@ struct tribit_ctx {
   std::unordered_map<void*, uint64_t> sizes;
   hipEvent_t ev[12] = {};
   int relabel = 1;     // TRIBIT_OPT_RELABEL: degree-rank relabelling in preprocessing
+  tribit_mlp* model = nullptr;  // cost model of the learned scheduler (not owned)
   std::string last_error;
 
   void* alloc(uint64_t bytes) {
@@ -145,6 +146,11 @@ struct tribit_graph {
   uint64_t* w_fw = nullptr;     // n, streamed elements of the owner's forward ranges
   uint64_t* w_rv = nullptr;     // n, streamed elements of the owner's reverse ranges
   uint64_t m_fwd = 0;
+  // wedge ranges (relabelled graphs only): arcs u->v grouped by v, each
+  // streaming the suffix of N+(u) after v
+  uint2* rng_sfx = nullptr;     // m
+  uint32_t* sfx_ptr = nullptr;  // n+1, owner v's ranges in rng_sfx
+  uint64_t* w_sfx = nullptr;    // n, streamed elements of v's ranges
   // 64-column bit tiles (tiles.hip), built on first use
   uint64_t tiles64 = 0;          // tile count (computed by preprocessing)
   uint64_t ntiles = 0;           // tiles built (0 until built)
@@ -176,13 +182,32 @@ uint64_t* gen_rgg(tribit_ctx* ctx, uint64_t n, double avg_deg, uint64_t seed, ui
 uint64_t sbm_slots(uint64_t blocks, uint64_t size, double p_out);
 void gen_sbm(tribit_ctx* ctx, uint64_t blocks, uint64_t size, double p_in, double p_out, uint64_t seed,
              uint64_t* d_keys);
+// count.hip
+void count(tribit_ctx* ctx, tribit_graph* g, const tribit_count_opts& opts, tribit_count_info* info);
+void variant_stats(tribit_ctx* ctx, tribit_graph* g, int sched, tribit_variant_work* out);
+void count_profile(tribit_ctx* ctx, tribit_graph* g, int layout, const uint8_t* table, tribit_profile* out);
+void profile_features(const tribit_profile* p, int bucket, int slot, float* f);
+void learned_plan(tribit_ctx* ctx, tribit_graph* g, uint8_t* table);
+void menu_slot(int slot, int* kind, int* g, uint32_t* max_d);
+uint32_t bucket_max_outdeg(int bucket);
+// mlp.hip
+tribit_mlp* mlp_create(tribit_ctx* ctx, int nfeat, uint64_t seed);
+void mlp_free(tribit_mlp* m);
+size_t mlp_num_params(const tribit_mlp* m);
+tribit_ctx* mlp_context(const tribit_mlp* m);
+int mlp_num_features(const tribit_mlp* m);
+void mlp_get(tribit_mlp* m, float* h);
+void mlp_set(tribit_mlp* m, const float* h);
+double mlp_grad(tribit_mlp* m, const float* hX, const float* ht, int B, float* hgrad);
+double mlp_train(tribit_mlp* m, const float* hX, const float* ht, uint64_t n, int epochs, int batch, float lr,
+                 uint64_t seed);
+void mlp_infer(tribit_mlp* m, const float* hX, uint64_t n, float* hout);
+void mlp_infer_dev(tribit_mlp* m, const float* dX, uint64_t n, float* dout);
 // tiles.hip
 uint64_t count_tiles(tribit_ctx* ctx, tribit_graph* g);
 void build_tiles(tribit_ctx* ctx, tribit_graph* g);
 void free_tiles(tribit_graph* g);
 // count.hip
-void count(tribit_ctx* ctx, tribit_graph* g, const tribit_count_opts& opts, tribit_count_info* info);
-void variant_stats(tribit_ctx* ctx, tribit_graph* g, int sched, tribit_variant_work* out);
 // mtx.cpp
 int read_mtx(const char* path, uint64_t** keys, uint64_t* nkeys, uint64_t* n, uint64_t* entries);
 }  // namespace tb

@@ -32,7 +32,8 @@ constexpr uint32_t kWaveBatch = 16;      // wave tasks taken per queue atomic
 constexpr uint32_t kChunk = 2048;        // probe-set chunk of class-3 rows
 constexpr uint64_t kWaveCap = 1u << 15;  // target streamed elements per wave task
 constexpr uint64_t kBlockCap = 1u << 18; // target streamed elements per block task
-constexpr int kVariants = 4;             // one kernel launch per row class
+constexpr uint32_t kBucketShift = 21;   // task.w = chunk | bucket << 21
+constexpr uint32_t kChunkMask = (1u << kBucketShift) - 1;
 
 // ------------------------------------------------------------ LDS sets
 // N+(u) (or one id-range chunk of it) is held in LDS as a two-table cuckoo
@@ -45,6 +46,7 @@ constexpr int kVariants = 4;             // one kernel launch per row class
 // same 2S slots.
 
 constexpr uint32_t kNoKey = 0xFFFFFFFEu;  // never a vertex id (n <= 2^32-2) and never kEmpty
+constexpr int kNoBitmapBit = 1 << 9;       // kernel mode bit: block tasks never use bitmap sets
 constexpr int kCuckooIters = 64;
 constexpr int kCuckooSeeds = 4;
 __device__ __constant__ uint32_t kSeedA[kCuckooSeeds] = {0x9E3779B1u, 0x85EBCA77u, 0xC2B2AE3Du, 0x27D4EB2Fu};
@@ -111,6 +113,21 @@ struct LinearProbe {
       if (k == kEmpty) return 0u;
       h = (h + 1) & ((1u << LOGS2) - 1);
     }
+  }
+};
+
+// Bitmap over the id window [base, base + 32 * words): one LDS read per
+// probe. Block tasks use it instead of the cuckoo set when the owner's ids
+// fit the set's 2S words (2^18 ids for S = 4096), as they do for almost all
+// owners once ids follow the degree rank. kNoKey and ids below base wrap to
+// indexes past the window and miss.
+template <int LOGW>
+struct BitmapProbe {
+  const uint32_t* T;
+  uint32_t base;
+  __device__ __forceinline__ uint32_t operator()(uint32_t x) const {
+    const uint32_t i = x - base;
+    return i < (32u << LOGW) ? (T[i >> 5] >> (i & 31)) & 1u : 0u;
   }
 };
 
@@ -232,7 +249,8 @@ __global__ __launch_bounds__(256) void k_count_wave(const uint32_t* __restrict__
                                                     const uint2* __restrict__ rng,
                                                     const uint4* __restrict__ tasks, uint32_t ntasks,
                                                     uint32_t* __restrict__ queue,
-                                                    unsigned long long* __restrict__ result, int attempts) {
+                                                    unsigned long long* __restrict__ result, int attempts,
+                                                    unsigned long long* __restrict__ tlog) {
   constexpr uint32_t S = 1u << LOGS;
   const ColSrc src = make_col_src(col, m);
   __shared__ uint32_t sT[4][2 * S];
@@ -251,6 +269,7 @@ __global__ __launch_bounds__(256) void k_count_wave(const uint32_t* __restrict__
     for (uint32_t t = tb; t < tend; ++t) {
       const uint4 tk = tasks[t];
       const uint32_t x = tk.x, rbase = tk.y, rcount = tk.z;
+      const uint64_t t0 = tlog ? (uint64_t)wall_clock64() : 0;
       const uint32_t r0 = row_ptr[x], d = row_ptr[x + 1] - r0;
       uint32_t sa = 0;
       bool ok = false;
@@ -279,6 +298,7 @@ __global__ __launch_bounds__(256) void k_count_wave(const uint32_t* __restrict__
         wave_lds_sync();
       }
       acc += cnt;
+      if (tlog && lane == 0) atomicAdd(&tlog[tk.w >> kBucketShift], (unsigned long long)(wall_clock64() - t0));
     }
   }
   acc = wave_sum_u64(acc);
@@ -302,8 +322,11 @@ __global__ __launch_bounds__(kBlockThreads) void k_count_block(const uint32_t* _
                                                                const uint4* __restrict__ tasks, uint32_t ntasks,
                                                                uint32_t* __restrict__ queue,
                                                                unsigned long long* __restrict__ result,
-                                                               int attempts) {
+                                                               int mode, unsigned long long* __restrict__ tlog) {
   constexpr uint32_t S = 1u << LOGS;
+  constexpr uint32_t kSpan = 64u * S;  // ids covered by the 2S-word bitmap
+  const int attempts = mode & 0xFF;
+  const bool bitmaps = !(mode & kNoBitmapBit) && attempts > 0;
   __shared__ uint32_t T[2 * S];
   __shared__ uint2 R[kChunk];
   __shared__ uint32_t s_task, s_next, s_fail;
@@ -317,13 +340,26 @@ __global__ __launch_bounds__(kBlockThreads) void k_count_block(const uint32_t* _
     const uint32_t t = s_task;
     if (t >= ntasks) break;
     const uint4 tk = tasks[t];
-    const uint32_t x = tk.x, rbase = tk.y, rcount = tk.z, c = tk.w;
+    const uint32_t x = tk.x, rbase = tk.y, rcount = tk.z, c = tk.w & kChunkMask;
+    const uint64_t t0 = tlog ? (uint64_t)wall_clock64() : 0;
     const uint32_t r0 = row_ptr[x], d = row_ptr[x + 1] - r0;
     const uint32_t p0 = c * kChunk, p1 = min(p0 + kChunk, d);
     const bool ranged = (p0 > 0) || (p1 < d);
     const uint32_t lo = col[r0 + p0], hi = col[r0 + p1 - 1];
+    const uint32_t base = lo & ~63u;
+    const bool bitmap = bitmaps && (hi - base) < kSpan && hi < 0xFFF00000u;
     uint32_t sa = 0;
     bool ok = false;
+    if (bitmap) {
+      for (uint32_t i = threadIdx.x; i < 2 * S; i += kBlockThreads) T[i] = 0u;
+      __syncthreads();
+      for (uint32_t i = p0 + threadIdx.x; i < p1; i += kBlockThreads) {
+        const uint32_t y = col[r0 + i] - base;
+        atomicOr(&T[y >> 5], 1u << (y & 31));
+      }
+      __syncthreads();
+      ok = true;
+    }
     for (int attempt = 0; attempt < attempts && !ok; ++attempt) {
       sa = kSeedA[attempt];
       for (uint32_t i = threadIdx.x; i < 2 * S; i += kBlockThreads) T[i] = kEmpty;
@@ -356,16 +392,31 @@ __global__ __launch_bounds__(kBlockThreads) void k_count_block(const uint32_t* _
       }
       if (threadIdx.x == 0) s_next = 0;
       __syncthreads();
-      cnt += ok ? stream_ranges<G, U, true, BUF>(src, R, nk, &s_next, lane, CuckooProbe<LOGS>{T, sa})
-                : stream_ranges<G, U, true, BUF>(src, R, nk, &s_next, lane, LinearProbe<LOGS + 1>{T});
+      if (bitmap)
+        cnt += stream_ranges<G, U, true, BUF>(src, R, nk, &s_next, lane, BitmapProbe<LOGS + 1>{T, base});
+      else if (ok)
+        cnt += stream_ranges<G, U, true, BUF>(src, R, nk, &s_next, lane, CuckooProbe<LOGS>{T, sa});
+      else
+        cnt += stream_ranges<G, U, true, BUF>(src, R, nk, &s_next, lane, LinearProbe<LOGS + 1>{T});
       __syncthreads();
     }
     acc += cnt;
+    if (tlog && threadIdx.x == 0) atomicAdd(&tlog[tk.w >> kBucketShift], (unsigned long long)(wall_clock64() - t0));
   }
   acc = wave_sum_u64(acc);
   if (lane == 0 && acc) atomicAdd(result, (unsigned long long)acc);
   if (threadIdx.x == 0 && fallbacks) atomicAdd(result + 1, (unsigned long long)fallbacks);
 }
+
+// Kernel variants: LDS set size (LOGS), staged ranges per batch (KMAX), lane
+// group width G and loads in flight per lane U. In the DIRECTED schedule a
+// streamed list is never longer than its owner's list, so the 64-slot sets
+// stream lists of at most 32 elements and use narrow groups.
+constexpr int kC0Logs = 6, kC0K = 64;    // d <= 32, 2x64 slots
+constexpr int kC1Logs = 9, kC1K = 256;   // d <= 256, 2x512 slots
+constexpr int kC2Logs = 12;              // block tasks: <= 2048 ids, 2x4096 slots
+constexpr int kU = 8;
+constexpr int kForceGlobal = 1 << 8;  // launch-time flag folded into the mode word (stripped before the kernel)
 
 // ------------------------------------------------------------- scheduler
 // A schedule is a set of owner lists: (x, rbase, rcount, work) where the
@@ -374,32 +425,85 @@ __global__ __launch_bounds__(kBlockThreads) void k_count_block(const uint32_t* _
 //             iterator: N+(u) is the set, every N+(v) is streamed).
 //   DIRECTED: per x its forward list and its reverse list over rng_dir
 //             (every edge streams the shorter of its two lists).
-// The owner's outdeg picks the class (kernel variant); its work (streamed
-// elements) picks the number of tasks the list is split into.
+//   WEDGE:    per x one list over rng_sfx: for every arc u -> x the part of
+//             N+(u) after x (relabelled graphs, where N+(x) holds only ids > x).
+// Lists are grouped into buckets ("row-blocks" of one shape): the owner's
+// outdeg bin (bin edges include 32, 256 and 2048, the set capacities) times
+// the bin of the mean streamed-range length. A plan maps every bucket to a
+// slot = (kernel kind, lane-group width); one launch per slot. The static
+// plans map by outdeg alone (the density-threshold heuristic); the learned
+// plan picks, per bucket, the menu slot of lowest predicted cost.
 
-__device__ __forceinline__ int row_class(uint32_t d) {
-  if (d == 0) return -1;
-  if (d <= 32) return 0;
-  if (d <= 256) return 1;
-  if (d <= kChunk) return 2;
-  return 3;
+constexpr int kDBins = 25;
+constexpr int kLBins = 32;
+constexpr int kBuckets = TRIBIT_BUCKETS;
+constexpr int kSlots = TRIBIT_MENU_SLOTS;
+constexpr int kFeatures = TRIBIT_FEATURES;
+static_assert(kBuckets == kDBins * kLBins, "bucket grid");
+enum SlotKind { kKindWave64 = 0, kKindWave512 = 1, kKindBlock = 2 };
+
+struct Plan {
+  int nslots;
+  int kind[kSlots];
+  int g[kSlots];
+  uint8_t table[kBuckets];  // bucket -> slot
+};
+
+struct BucketStats {
+  uint64_t lists[kBuckets], sum_d[kBuckets], sum_k[kBuckets], sum_w[kBuckets], eligible[kBuckets];
+};
+
+__host__ __device__ inline uint32_t d_bin_top(int b) {
+  constexpr uint32_t top[kDBins - 1] = {1,   2,   3,   4,   6,   8,   12,   16,   24,   32,   48,   64,
+                                        96,  128, 192, 256, 384, 512, 768, 1024, 1536, 2048, 4096, 8192};
+  return b < kDBins - 1 ? top[b] : 0xFFFFFFFFu;
 }
 
-__device__ __forceinline__ uint32_t list_splits(int cls, uint32_t count, uint64_t w) {
-  const uint64_t cap = cls <= 1 ? kWaveCap : kBlockCap;
-  uint64_t s = (w + cap - 1) / cap;
-  if (s < 1) s = 1;
-  if (s > count) s = count;
-  return (uint32_t)s;
+__host__ __device__ inline int d_bin(uint32_t d) {
+  int i = 0;
+  while (i < kDBins - 1 && d > d_bin_top(i)) ++i;
+  return i;
+}
+
+// floor(2 * log2(1 + mean range length)), capped
+__host__ __device__ inline int l_bin(uint64_t w, uint32_t k) {
+  uint64_t x = 1 + (k ? w / k : 0);
+  if (x > (1ull << 20)) x = 1ull << 20;
+  const int f = 63 - __builtin_clzll(x);
+  const int b = 2 * f + (x * x >= (1ull << (2 * f + 1)) ? 1 : 0);
+  return b < kLBins ? b : kLBins - 1;
+}
+
+__host__ __device__ inline int bucket_of(uint32_t d, uint64_t w, uint32_t k) { return d_bin(d) * kLBins + l_bin(w, k); }
+
+__host__ __device__ inline uint32_t kind_max_d(int kind) {
+  return kind == kKindWave64 ? 32u : kind == kKindWave512 ? 256u : 0xFFFFFFFFu;
 }
 
 struct SchedView {
   const uint32_t* row_ptr;
-  const uint32_t* fw_ptr;  // list 0 offsets (PLAIN: row_ptr)
-  const uint32_t* rv_ptr;  // list 1 offsets (PLAIN: unused)
-  const uint64_t* w_fw;    // list 0 work (PLAIN: wrow)
+  const uint32_t* fw_ptr;  // list 0 offsets (PLAIN: row_ptr, WEDGE: sfx_ptr)
+  const uint32_t* rv_ptr;  // list 1 offsets (PLAIN, WEDGE: unused)
+  const uint64_t* w_fw;    // list 0 work (PLAIN: wrow, WEDGE: w_sfx)
   const uint64_t* w_rv;
-  int nlists;              // 1 (PLAIN) or 2 (DIRECTED)
+  int nlists;              // 1 (PLAIN, WEDGE) or 2 (DIRECTED)
+  uint32_t min_d0;         // smallest owner outdeg for which list 0 can hold a triangle
+};
+
+struct PlanDev {
+  int kind[kSlots];
+  const uint8_t* table;  // bucket -> slot
+};
+
+struct Cnt8 {
+  uint32_t v[kSlots];
+};
+struct Cnt8Plus {
+  __host__ __device__ Cnt8 operator()(const Cnt8& a, const Cnt8& b) const {
+    Cnt8 r;
+    for (int i = 0; i < kSlots; ++i) r.v[i] = a.v[i] + b.v[i];
+    return r;
+  }
 };
 
 __device__ __forceinline__ void owner_list(const SchedView& v, uint64_t x, int l, uint32_t* base, uint32_t* count,
@@ -410,57 +514,99 @@ __device__ __forceinline__ void owner_list(const SchedView& v, uint64_t x, int l
   *work = (l == 0) ? v.w_fw[x] : v.w_rv[x];
 }
 
-__device__ __forceinline__ bool list_useful(int l, uint32_t d, uint32_t count) {
+__device__ __forceinline__ bool list_useful(const SchedView& v, int l, uint32_t d, uint32_t count) {
   // a forward list needs |N+(x)| >= 2 (its partner v is in N+(x) but not in N+(v))
-  return count > 0 && d >= (l == 0 ? 2u : 1u);
+  return count > 0 && d >= (l == 0 ? v.min_d0 : 1u);
 }
 
-// Per owner: number of tasks in its class slot (uint4 = 4 class counters).
-__global__ void k_sched_count(SchedView sv, uint64_t n, uint4* __restrict__ cnt) {
+__device__ __forceinline__ uint32_t list_splits(int kind, uint32_t count, uint64_t w) {
+  const uint64_t cap = kind == kKindBlock ? kBlockCap : kWaveCap;
+  uint64_t s = (w + cap - 1) / cap;
+  if (s < 1) s = 1;
+  if (s > count) s = count;
+  return (uint32_t)s;
+}
+
+__device__ __forceinline__ uint32_t list_chunks(int kind, uint32_t d) {
+  return (kind == kKindBlock && d > kChunk) ? (d + kChunk - 1) / kChunk : 1u;
+}
+
+// Per owner: tasks per slot.
+__global__ void k_sched_count(SchedView sv, PlanDev pd, uint64_t n, Cnt8* __restrict__ cnt) {
   const uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (x > n) return;
-  uint32_t c4[4] = {0, 0, 0, 0};
+  Cnt8 c;
+  for (int i = 0; i < kSlots; ++i) c.v[i] = 0;
   if (x < n) {
     const uint32_t d = sv.row_ptr[x + 1] - sv.row_ptr[x];
-    const int cls = row_class(d);
-    if (cls >= 0) {
-      const uint32_t nch = (cls == 3) ? (d + kChunk - 1) / kChunk : 1u;
-      for (int l = 0; l < sv.nlists; ++l) {
-        uint32_t b, k;
-        uint64_t w;
-        owner_list(sv, x, l, &b, &k, &w);
-        if (list_useful(l, d, k)) c4[cls] += list_splits(cls, k, w) * nch;
-      }
+    for (int l = 0; l < sv.nlists; ++l) {
+      uint32_t b, k;
+      uint64_t w;
+      owner_list(sv, x, l, &b, &k, &w);
+      if (!list_useful(sv, l, d, k)) continue;
+      const int s = pd.table[bucket_of(d, w, k)];
+      const int kind = pd.kind[s];
+      c.v[s] += list_splits(kind, k, w) * list_chunks(kind, d);
     }
   }
-  cnt[x] = make_uint4(c4[0], c4[1], c4[2], c4[3]);
+  cnt[x] = c;
 }
 
-// Writes each owner's tasks at its scanned offsets: wave tasks (classes 0, 1)
-// into wt (class 1 after all class-0 tasks), block tasks (2, 3) into bt.
-__global__ void k_sched_fill(SchedView sv, uint64_t n, const uint4* __restrict__ off, uint4 tot,
-                             uint4* __restrict__ wt, uint4* __restrict__ bt) {
+// Writes each owner's tasks at its scanned offsets inside its slots' regions.
+__global__ void k_sched_fill(SchedView sv, PlanDev pd, uint64_t n, const Cnt8* __restrict__ off, Cnt8 base,
+                             uint4* __restrict__ tasks) {
   const uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (x >= n) return;
   const uint32_t d = sv.row_ptr[x + 1] - sv.row_ptr[x];
-  const int cls = row_class(d);
-  if (cls < 0) return;
-  const uint4 o = off[x];
-  uint4* out = (cls == 0) ? wt + o.x : (cls == 1) ? wt + tot.x + o.y : (cls == 2) ? bt + o.z : bt + tot.z + o.w;
-  const uint32_t nch = (cls == 3) ? (d + kChunk - 1) / kChunk : 1u;
-  uint32_t j = 0;
+  const Cnt8 o = off[x];
+  uint32_t j[kSlots];
+  for (int i = 0; i < kSlots; ++i) j[i] = 0;
   for (int l = 0; l < sv.nlists; ++l) {
     uint32_t b, k;
     uint64_t w;
     owner_list(sv, x, l, &b, &k, &w);
-    if (!list_useful(l, d, k)) continue;
-    const uint32_t s = list_splits(cls, k, w);
+    if (!list_useful(sv, l, d, k)) continue;
+    const uint32_t bucket = (uint32_t)bucket_of(d, w, k);
+    const int s = pd.table[bucket];
+    const int kind = pd.kind[s];
+    const uint32_t sp = list_splits(kind, k, w), nch = list_chunks(kind, d);
+    uint4* out = tasks + base.v[s] + o.v[s];
     for (uint32_t c = 0; c < nch; ++c)
-      for (uint32_t i = 0; i < s; ++i) {
-        const uint32_t k0 = (uint32_t)((uint64_t)k * i / s), k1 = (uint32_t)((uint64_t)k * (i + 1) / s);
-        out[j++] = make_uint4((uint32_t)x, b + k0, k1 - k0, c);
+      for (uint32_t i = 0; i < sp; ++i) {
+        const uint32_t k0 = (uint32_t)((uint64_t)k * i / sp), k1 = (uint32_t)((uint64_t)k * (i + 1) / sp);
+        out[j[s]++] = make_uint4((uint32_t)x, b + k0, k1 - k0, c | (bucket << kBucketShift));
       }
   }
+}
+
+// Per bucket: lists, sum of outdeg, ranges, streamed elements, owners whose
+// set fits a block bitmap window. Privatised in LDS per block.
+constexpr int kStat = 5;
+__global__ __launch_bounds__(256) void k_bucket_stats(SchedView sv, const uint32_t* __restrict__ col, uint64_t n,
+                                                      unsigned long long* __restrict__ st) {
+  __shared__ unsigned long long acc[kStat * kBuckets];
+  for (int i = threadIdx.x; i < kStat * kBuckets; i += blockDim.x) acc[i] = 0ull;
+  __syncthreads();
+  const uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (x < n) {
+    const uint32_t r0 = sv.row_ptr[x], d = sv.row_ptr[x + 1] - r0;
+    for (int l = 0; l < sv.nlists; ++l) {
+      uint32_t b, k;
+      uint64_t w;
+      owner_list(sv, x, l, &b, &k, &w);
+      if (!list_useful(sv, l, d, k)) continue;
+      const int bk = bucket_of(d, w, k);
+      const uint32_t span = col[r0 + d - 1] - (col[r0] & ~63u);
+      atomicAdd(&acc[bk], 1ull);
+      atomicAdd(&acc[kBuckets + bk], (unsigned long long)d);
+      atomicAdd(&acc[2 * kBuckets + bk], (unsigned long long)k);
+      atomicAdd(&acc[3 * kBuckets + bk], (unsigned long long)w);
+      if (span < (64u << kC2Logs)) atomicAdd(&acc[4 * kBuckets + bk], 1ull);
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < kStat * kBuckets; i += blockDim.x)
+    if (acc[i]) atomicAdd(&st[i], acc[i]);
 }
 
 __global__ void k_zero_u64(unsigned long long* p, uint32_t n) {
@@ -468,21 +614,23 @@ __global__ void k_zero_u64(unsigned long long* p, uint32_t n) {
   if (i < n) p[i] = 0ull;
 }
 
-struct Uint4Plus {
-  __host__ __device__ uint4 operator()(const uint4& a, const uint4& b) const {
-    return make_uint4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w);
-  }
-};
-
 static SchedView sched_view(const tribit_graph* g, int sched) {
   SchedView v;
   v.row_ptr = g->row_ptr;
+  v.min_d0 = 2;
   if (sched == TRIBIT_SCHED_PLAIN) {
     v.fw_ptr = g->row_ptr;
     v.rv_ptr = g->row_ptr;
     v.w_fw = g->wrow;
     v.w_rv = g->wrow;
     v.nlists = 1;
+  } else if (sched == TRIBIT_SCHED_WEDGE) {
+    v.fw_ptr = g->sfx_ptr;
+    v.rv_ptr = g->sfx_ptr;
+    v.w_fw = g->w_sfx;
+    v.w_rv = g->w_sfx;
+    v.nlists = 1;
+    v.min_d0 = 1;
   } else {
     v.fw_ptr = g->fw_ptr;
     v.rv_ptr = g->rv_ptr;
@@ -494,90 +642,56 @@ static SchedView sched_view(const tribit_graph* g, int sched) {
 }
 
 static const uint2* sched_ranges(const tribit_graph* g, int sched) {
-  return sched == TRIBIT_SCHED_PLAIN ? g->rng_all : g->rng_dir;
+  return sched == TRIBIT_SCHED_PLAIN ? g->rng_all : sched == TRIBIT_SCHED_WEDGE ? g->rng_sfx : g->rng_dir;
 }
 
-// Kernel variants: LDS set size (LOGS), staged ranges per batch (KMAX), lane
-// group width G and loads in flight per lane U. In the DIRECTED schedule a
-// streamed list is never longer than its owner's list, so class 0 streams
-// lists of at most 32 elements and uses narrow groups.
-constexpr int kC0Logs = 6, kC0K = 64;    // class 0: d <= 32, 2x64 slots
-constexpr int kC1Logs = 9, kC1K = 256;   // class 1: d <= 256, 2x512 slots
-constexpr int kC2Logs = 12;              // classes 2,3: <= 2048 ids, 2x4096 slots
-constexpr int kU = 8;
-constexpr int kForceGlobal = 1 << 8;  // launch-time flag folded into `attempts` (stripped before the kernel)
+// ------------------------------------------------------------------ plans
+// The learned scheduler's menu: (kind, lane-group width) per slot.
+static const int kMenuKind[kSlots] = {kKindWave64,  kKindWave64, kKindWave512, kKindWave512,
+                                      kKindWave512, kKindBlock,  kKindBlock,   kKindBlock};
+static const int kMenuG[kSlots] = {4, 16, 4, 8, 16, 8, 16, 32};
 
-template <int LOGS, int KMAX, int G, int U, bool BUF>
-static void launch_wave_t(uint32_t ntasks, int cus, hipStream_t s, tribit_graph* g, const uint2* rng,
-                          const uint4* tasks, uint32_t* queue, unsigned long long* result, int attempts,
-                          uint64_t* blocks) {
-  int per_cu = 0;
-  TB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_count_wave<LOGS, KMAX, G, U, BUF>, kBlock, 0));
-  if (per_cu < 1) per_cu = 1;
-  const uint64_t want = (ntasks + 4 * kWaveBatch - 1) / (4 * kWaveBatch);
-  const unsigned grid = (unsigned)std::min<uint64_t>(want, (uint64_t)cus * per_cu);
-  k_count_wave<LOGS, KMAX, G, U, BUF><<<grid, kBlock, 0, s>>>(g->row_ptr, g->col, g->m, rng, tasks, ntasks, queue,
-                                                             result, attempts & (kForceGlobal - 1));
-  *blocks += grid;
+static int kind_of_dbin(int db) {
+  const uint32_t top = d_bin_top(db);
+  return top <= 32 ? kKindWave64 : top <= 256 ? kKindWave512 : kKindBlock;
 }
 
-template <int LOGS, int KMAX, int G>
-static void launch_wave_g(uint32_t ntasks, int cus, hipStream_t s, tribit_graph* g, const uint2* rng,
-                          const uint4* tasks, uint32_t* queue, unsigned long long* result, int attempts,
-                          uint64_t* blocks) {
-  if (g->m * 4 < 0xFFFFFFFFull && !(attempts & kForceGlobal))
-    launch_wave_t<LOGS, KMAX, G, kU, true>(ntasks, cus, s, g, rng, tasks, queue, result, attempts, blocks);
-  else
-    launch_wave_t<LOGS, KMAX, G, kU, false>(ntasks, cus, s, g, rng, tasks, queue, result, attempts, blocks);
+static bool slot_feasible(int slot, int bucket) {
+  return d_bin_top(bucket / kLBins) <= kind_max_d(kMenuKind[slot]);
 }
 
-template <int LOGS, int KMAX>
-static void launch_wave(int gw, uint32_t ntasks, int cus, hipStream_t s, tribit_graph* g, const uint2* rng,
-                        const uint4* tasks, uint32_t* queue, unsigned long long* result, int attempts,
-                        uint64_t* blocks) {
-  if (!ntasks) return;
-  switch (gw) {
-    case 4: launch_wave_g<LOGS, KMAX, 4>(ntasks, cus, s, g, rng, tasks, queue, result, attempts, blocks); break;
-    case 8: launch_wave_g<LOGS, KMAX, 8>(ntasks, cus, s, g, rng, tasks, queue, result, attempts, blocks); break;
-    case 32: launch_wave_g<LOGS, KMAX, 32>(ntasks, cus, s, g, rng, tasks, queue, result, attempts, blocks); break;
-    default: launch_wave_g<LOGS, KMAX, 16>(ntasks, cus, s, g, rng, tasks, queue, result, attempts, blocks); break;
+// Static plan of a fixed schedule: four slots by outdeg (<= 32, <= 256,
+// <= 2048, > 2048) with the given widths.
+static Plan static_plan(const int gw[4]) {
+  Plan p;
+  p.nslots = 4;
+  const int kinds[4] = {kKindWave64, kKindWave512, kKindBlock, kKindBlock};
+  for (int i = 0; i < 4; ++i) {
+    p.kind[i] = kinds[i];
+    p.g[i] = gw[i];
   }
+  for (int b = 0; b < kBuckets; ++b) {
+    const uint32_t top = d_bin_top(b / kLBins);
+    p.table[b] = (uint8_t)(top <= 32 ? 0 : top <= 256 ? 1 : top <= kChunk ? 2 : 3);
+  }
+  return p;
 }
 
-template <int LOGS, int G, int U, bool BUF>
-static void launch_block_t(uint32_t ntasks, int cus, hipStream_t s, tribit_graph* g, const uint2* rng,
-                           const uint4* tasks, uint32_t* queue, unsigned long long* result, int attempts,
-                           uint64_t* blocks) {
-  int per_cu = 0;
-  TB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_count_block<LOGS, G, U, BUF>, kBlockThreads, 0));
-  if (per_cu < 1) per_cu = 1;
-  const unsigned grid = (unsigned)std::min<uint64_t>(ntasks, (uint64_t)cus * per_cu);
-  k_count_block<LOGS, G, U, BUF><<<grid, kBlockThreads, 0, s>>>(g->row_ptr, g->col, g->m, rng, tasks, ntasks, queue,
-                                                               result, attempts & (kForceGlobal - 1));
-  *blocks += grid;
+static Plan menu_plan() {
+  Plan p;
+  p.nslots = kSlots;
+  for (int i = 0; i < kSlots; ++i) {
+    p.kind[i] = kMenuKind[i];
+    p.g[i] = kMenuG[i];
+  }
+  return p;
 }
 
-template <int LOGS, int G>
-static void launch_block_g(uint32_t ntasks, int cus, hipStream_t s, tribit_graph* g, const uint2* rng,
-                           const uint4* tasks, uint32_t* queue, unsigned long long* result, int attempts,
-                           uint64_t* blocks) {
-  if (g->m * 4 < 0xFFFFFFFFull && !(attempts & kForceGlobal))
-    launch_block_t<LOGS, G, kU, true>(ntasks, cus, s, g, rng, tasks, queue, result, attempts, blocks);
-  else
-    launch_block_t<LOGS, G, kU, false>(ntasks, cus, s, g, rng, tasks, queue, result, attempts, blocks);
-}
-
-template <int LOGS>
-static void launch_block(int gw, uint32_t ntasks, int cus, hipStream_t s, tribit_graph* g, const uint2* rng,
-                         const uint4* tasks, uint32_t* queue, unsigned long long* result, int attempts,
-                         uint64_t* blocks) {
-  if (!ntasks) return;
-  switch (gw) {
-    case 4: launch_block_g<LOGS, 4>(ntasks, cus, s, g, rng, tasks, queue, result, attempts, blocks); break;
-    case 8: launch_block_g<LOGS, 8>(ntasks, cus, s, g, rng, tasks, queue, result, attempts, blocks); break;
-    case 32: launch_block_g<LOGS, 32>(ntasks, cus, s, g, rng, tasks, queue, result, attempts, blocks); break;
-    case 64: launch_block_g<LOGS, 64>(ntasks, cus, s, g, rng, tasks, queue, result, attempts, blocks); break;
-    default: launch_block_g<LOGS, 16>(ntasks, cus, s, g, rng, tasks, queue, result, attempts, blocks); break;
+// The density-threshold heuristic over the menu (fallback of LEARNED).
+static void heuristic_table(uint8_t* table) {
+  for (int b = 0; b < kBuckets; ++b) {
+    const int kind = kind_of_dbin(b / kLBins);
+    table[b] = (uint8_t)(kind == kKindWave64 ? 0 : kind == kKindWave512 ? 3 : 6);
   }
 }
 
@@ -591,80 +705,287 @@ static void default_widths(int sched, int gw[4]) {
   }
 }
 
+void bucket_stats(tribit_ctx* ctx, tribit_graph* g, int sched, BucketStats* out) {
+  memset(out, 0, sizeof(*out));
+  DevBuf<unsigned long long> st(ctx, kStat * kBuckets);
+  k_zero_u64<<<grid_for(kStat * kBuckets, kBlock), kBlock, 0, ctx->stream>>>(st.p, kStat * kBuckets);
+  if (g->n && g->m)
+    k_bucket_stats<<<grid_for(g->n, 256), 256, 0, ctx->stream>>>(sched_view(g, sched), g->col, g->n, st.p);
+  std::vector<unsigned long long> h(kStat * kBuckets);
+  TB_HIP(hipMemcpyAsync(h.data(), st.p, 8 * h.size(), hipMemcpyDeviceToHost, ctx->stream));
+  ctx->sync();
+  for (int b = 0; b < kBuckets; ++b) {
+    out->lists[b] = h[b];
+    out->sum_d[b] = h[kBuckets + b];
+    out->sum_k[b] = h[2 * kBuckets + b];
+    out->sum_w[b] = h[3 * kBuckets + b];
+    out->eligible[b] = h[4 * kBuckets + b];
+  }
+}
+
+// Feature vector of (bucket, menu slot); the same function builds training
+// samples and scores candidates.
+void features(const BucketStats& st, int b, int slot, float* f) {
+  const double L = (double)(st.lists[b] ? st.lists[b] : 1);
+  const double K = (double)(st.sum_k[b] ? st.sum_k[b] : 1);
+  f[0] = (float)std::log2(1.0 + st.sum_d[b] / L);
+  f[1] = (float)std::log2(1.0 + st.sum_k[b] / L);
+  f[2] = (float)std::log2(1.0 + st.sum_w[b] / L);
+  f[3] = (float)std::log2(1.0 + st.sum_w[b] / K);
+  f[4] = (float)(st.eligible[b] / L);
+  f[5] = (float)(std::log2(1.0 + L) / 16.0);
+  const int kind = kMenuKind[slot], g = kMenuG[slot];
+  f[6] = kind == kKindWave64;
+  f[7] = kind == kKindWave512;
+  f[8] = kind == kKindBlock;
+  f[9] = (float)(std::log2((double)g) / 6.0);
+  f[10] = g == 4;
+  f[11] = g == 8;
+  f[12] = g == 16;
+  f[13] = g == 32;
+  f[14] = (float)(std::log2(1.0 + (st.sum_w[b] + st.sum_k[b]) / L) / 32.0);
+  f[15] = 1.0f;
+}
+
+double bucket_work(const BucketStats& st, int b) { return (double)st.sum_w[b] + (double)st.sum_k[b] + 1.0; }
+
+static void stats_of_profile(const tribit_profile* p, BucketStats* st) {
+  memcpy(st->lists, p->lists, sizeof(st->lists));
+  memcpy(st->sum_d, p->sum_d, sizeof(st->sum_d));
+  memcpy(st->sum_k, p->sum_k, sizeof(st->sum_k));
+  memcpy(st->sum_w, p->sum_w, sizeof(st->sum_w));
+  memcpy(st->eligible, p->eligible, sizeof(st->eligible));
+}
+
+void profile_features(const tribit_profile* p, int bucket, int slot, float* f) {
+  BucketStats st;
+  stats_of_profile(p, &st);
+  features(st, bucket, slot, f);
+}
+
+void menu_slot(int slot, int* kind, int* g, uint32_t* max_d) {
+  if (kind) *kind = kMenuKind[slot];
+  if (g) *g = kMenuG[slot];
+  if (max_d) *max_d = kind_max_d(kMenuKind[slot]);
+}
+
+uint32_t bucket_max_outdeg(int bucket) { return d_bin_top(bucket / kLBins); }
+
+// LEARNED: per bucket with lists, the feasible menu slot of lowest predicted
+// cost 2^model(features) * work. Without a model, or when a prediction is not
+// finite, the heuristic's slot.
+static void learned_table(tribit_ctx* ctx, const BucketStats& st, uint8_t* table) {
+  heuristic_table(table);
+  if (!ctx->model) return;
+  std::vector<float> X;
+  std::vector<std::pair<int, int>> ids;
+  for (int b = 0; b < kBuckets; ++b) {
+    if (!st.lists[b]) continue;
+    for (int s = 0; s < kSlots; ++s) {
+      if (!slot_feasible(s, b)) continue;
+      float f[kFeatures];
+      features(st, b, s, f);
+      X.insert(X.end(), f, f + kFeatures);
+      ids.push_back({b, s});
+    }
+  }
+  if (ids.empty()) return;
+  std::vector<float> y(ids.size());
+  mlp_infer(ctx->model, X.data(), ids.size(), y.data());
+  std::vector<double> best(kBuckets, INFINITY);
+  for (size_t i = 0; i < ids.size(); ++i) {
+    const int b = ids[i].first, s = ids[i].second;
+    if (!std::isfinite(y[i])) {
+      best[b] = -INFINITY;  // keep the heuristic for this bucket
+      continue;
+    }
+    const double c = std::exp2((double)y[i]) * bucket_work(st, b);
+    if (best[b] != -INFINITY && c < best[b]) {
+      best[b] = c;
+      table[b] = (uint8_t)s;
+    }
+  }
+  for (int b = 0; b < kBuckets; ++b)
+    if (best[b] == -INFINITY) {
+      uint8_t h[kBuckets];
+      heuristic_table(h);
+      table[b] = h[b];
+    }
+}
+
+// ----------------------------------------------------------------- launches
+template <int LOGS, int KMAX, int G, int U, bool BUF>
+static void launch_wave_t(uint32_t ntasks, int cus, hipStream_t s, tribit_graph* g, const uint2* rng,
+                          const uint4* tasks, uint32_t* queue, unsigned long long* result, int attempts,
+                          unsigned long long* tlog, uint64_t* blocks) {
+  int per_cu = 0;
+  TB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_count_wave<LOGS, KMAX, G, U, BUF>, kBlock, 0));
+  if (per_cu < 1) per_cu = 1;
+  const uint64_t want = (ntasks + 4 * kWaveBatch - 1) / (4 * kWaveBatch);
+  const unsigned grid = (unsigned)std::min<uint64_t>(want, (uint64_t)cus * per_cu);
+  k_count_wave<LOGS, KMAX, G, U, BUF><<<grid, kBlock, 0, s>>>(g->row_ptr, g->col, g->m, rng, tasks, ntasks, queue,
+                                                             result, attempts & 0xFF, tlog);
+  *blocks += grid;
+}
+
+template <int LOGS, int KMAX, int G>
+static void launch_wave_g(uint32_t ntasks, int cus, hipStream_t s, tribit_graph* g, const uint2* rng,
+                          const uint4* tasks, uint32_t* queue, unsigned long long* result, int attempts,
+                          unsigned long long* tlog, uint64_t* blocks) {
+  if (g->m * 4 < 0xFFFFFFFFull && !(attempts & kForceGlobal))
+    launch_wave_t<LOGS, KMAX, G, kU, true>(ntasks, cus, s, g, rng, tasks, queue, result, attempts, tlog, blocks);
+  else
+    launch_wave_t<LOGS, KMAX, G, kU, false>(ntasks, cus, s, g, rng, tasks, queue, result, attempts, tlog, blocks);
+}
+
+template <int LOGS, int KMAX>
+static void launch_wave(int gw, uint32_t ntasks, int cus, hipStream_t s, tribit_graph* g, const uint2* rng,
+                        const uint4* tasks, uint32_t* queue, unsigned long long* result, int attempts,
+                        unsigned long long* tlog, uint64_t* blocks) {
+  if (!ntasks) return;
+  switch (gw) {
+    case 4: launch_wave_g<LOGS, KMAX, 4>(ntasks, cus, s, g, rng, tasks, queue, result, attempts, tlog, blocks); break;
+    case 8: launch_wave_g<LOGS, KMAX, 8>(ntasks, cus, s, g, rng, tasks, queue, result, attempts, tlog, blocks); break;
+    case 32:
+      launch_wave_g<LOGS, KMAX, 32>(ntasks, cus, s, g, rng, tasks, queue, result, attempts, tlog, blocks);
+      break;
+    default:
+      launch_wave_g<LOGS, KMAX, 16>(ntasks, cus, s, g, rng, tasks, queue, result, attempts, tlog, blocks);
+      break;
+  }
+}
+
+template <int LOGS, int G, int U, bool BUF>
+static void launch_block_t(uint32_t ntasks, int cus, hipStream_t s, tribit_graph* g, const uint2* rng,
+                           const uint4* tasks, uint32_t* queue, unsigned long long* result, int attempts,
+                           unsigned long long* tlog, uint64_t* blocks) {
+  int per_cu = 0;
+  TB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_count_block<LOGS, G, U, BUF>, kBlockThreads, 0));
+  if (per_cu < 1) per_cu = 1;
+  const unsigned grid = (unsigned)std::min<uint64_t>(ntasks, (uint64_t)cus * per_cu);
+  k_count_block<LOGS, G, U, BUF><<<grid, kBlockThreads, 0, s>>>(g->row_ptr, g->col, g->m, rng, tasks, ntasks, queue,
+                                                               result, attempts & ~kForceGlobal, tlog);
+  *blocks += grid;
+}
+
+template <int LOGS, int G>
+static void launch_block_g(uint32_t ntasks, int cus, hipStream_t s, tribit_graph* g, const uint2* rng,
+                           const uint4* tasks, uint32_t* queue, unsigned long long* result, int attempts,
+                           unsigned long long* tlog, uint64_t* blocks) {
+  if (g->m * 4 < 0xFFFFFFFFull && !(attempts & kForceGlobal))
+    launch_block_t<LOGS, G, kU, true>(ntasks, cus, s, g, rng, tasks, queue, result, attempts, tlog, blocks);
+  else
+    launch_block_t<LOGS, G, kU, false>(ntasks, cus, s, g, rng, tasks, queue, result, attempts, tlog, blocks);
+}
+
+template <int LOGS>
+static void launch_block(int gw, uint32_t ntasks, int cus, hipStream_t s, tribit_graph* g, const uint2* rng,
+                         const uint4* tasks, uint32_t* queue, unsigned long long* result, int attempts,
+                         unsigned long long* tlog, uint64_t* blocks) {
+  if (!ntasks) return;
+  switch (gw) {
+    case 4: launch_block_g<LOGS, 4>(ntasks, cus, s, g, rng, tasks, queue, result, attempts, tlog, blocks); break;
+    case 8: launch_block_g<LOGS, 8>(ntasks, cus, s, g, rng, tasks, queue, result, attempts, tlog, blocks); break;
+    case 32: launch_block_g<LOGS, 32>(ntasks, cus, s, g, rng, tasks, queue, result, attempts, tlog, blocks); break;
+    case 64: launch_block_g<LOGS, 64>(ntasks, cus, s, g, rng, tasks, queue, result, attempts, tlog, blocks); break;
+    default: launch_block_g<LOGS, 16>(ntasks, cus, s, g, rng, tasks, queue, result, attempts, tlog, blocks); break;
+  }
+}
+
 static float elapsed(hipEvent_t a, hipEvent_t b) {
   float ms = 0.f;
   TB_HIP(hipEventElapsedTime(&ms, a, b));
   return ms;
 }
 
-// One scheduled count. Returns the triangle count; fills per-phase times.
-static uint64_t count_once(tribit_ctx* ctx, tribit_graph* g, int sched, int flags, const int gw[4],
+// One scheduled count under `plan`. Returns the triangle count; fills
+// per-phase times. With `tlog` (kSlots x kBuckets, zeroed by the caller) every
+// task adds its wall-clock duration to tlog[slot][bucket].
+static uint64_t count_once(tribit_ctx* ctx, tribit_graph* g, int sched, int flags, Plan& plan, bool learned,
                            double* ms_total, double* ms_sched, double* ms_kernel, double* ms_var,
-                           tribit_count_info* info) {
+                           tribit_count_info* info, unsigned long long* tlog) {
   hipStream_t s = ctx->stream;
   const uint64_t n = g->n;
   TB_HIP(hipEventRecord(ctx->ev[2], s));
-  // scratch: [0] count, [1] fallback tasks, [4..7] four u32 queue counters
+  if (learned) {
+    // the model's choice is part of the scheduler's time
+    BucketStats st;
+    bucket_stats(ctx, g, sched, &st);
+    learned_table(ctx, st, plan.table);
+  }
+  // scratch: [0] count, [1] fallback tasks, [4..7] eight u32 queue counters
   DevBuf<unsigned long long> scratch(ctx, 8);
   k_zero_u64<<<1, 64, 0, s>>>(scratch.p, 8);
   unsigned long long* d_result = scratch.p;
   uint32_t* d_queue = reinterpret_cast<uint32_t*>(scratch.p + 4);
+  DevBuf<uint8_t> dtable(ctx, kBuckets);
+  TB_HIP(hipMemcpyAsync(dtable.p, plan.table, kBuckets, hipMemcpyHostToDevice, s));
+  PlanDev pd;
+  for (int i = 0; i < kSlots; ++i) pd.kind[i] = i < plan.nslots ? plan.kind[i] : kKindBlock;
+  pd.table = dtable.p;
 
-  DevBuf<uint4> cnt(ctx, n + 1), off(ctx, n + 1);
+  DevBuf<Cnt8> cnt(ctx, n + 1), off(ctx, n + 1);
   const SchedView sv = sched_view(g, sched);
   const uint2* rng = sched_ranges(g, sched);
-  k_sched_count<<<grid_for(n + 1, kBlock), kBlock, 0, s>>>(sv, n, cnt.p);
+  k_sched_count<<<grid_for(n + 1, kBlock), kBlock, 0, s>>>(sv, pd, n, cnt.p);
   {
+    Cnt8 zero;
+    for (int i = 0; i < kSlots; ++i) zero.v[i] = 0;
     size_t bytes = 0;
-    TB_HIP(rocprim::exclusive_scan(nullptr, bytes, cnt.p, off.p, make_uint4(0, 0, 0, 0), (size_t)(n + 1),
-                                   Uint4Plus(), s));
+    TB_HIP(rocprim::exclusive_scan(nullptr, bytes, cnt.p, off.p, zero, (size_t)(n + 1), Cnt8Plus(), s));
     DevBuf<uint8_t> tmp(ctx, bytes ? bytes : 8);
-    TB_HIP(rocprim::exclusive_scan(tmp.p, bytes, cnt.p, off.p, make_uint4(0, 0, 0, 0), (size_t)(n + 1), Uint4Plus(),
-                                   s));
+    TB_HIP(rocprim::exclusive_scan(tmp.p, bytes, cnt.p, off.p, zero, (size_t)(n + 1), Cnt8Plus(), s));
   }
-  uint4 tot;  // off[n] = totals (cnt[n] is zero)
-  TB_HIP(hipMemcpyAsync(&tot, off.p + n, sizeof(uint4), hipMemcpyDeviceToHost, s));
+  Cnt8 tot;  // off[n] = totals (cnt[n] is zero)
+  TB_HIP(hipMemcpyAsync(&tot, off.p + n, sizeof(Cnt8), hipMemcpyDeviceToHost, s));
   ctx->sync();
-  const uint64_t nw = (uint64_t)tot.x + tot.y, nb = (uint64_t)tot.z + tot.w;
-  DevBuf<uint4> wt(ctx, nw ? nw : 1);
-  DevBuf<uint4> bt(ctx, nb ? nb : 1);
-  k_sched_fill<<<grid_for(n, kBlock), kBlock, 0, s>>>(sv, n, off.p, tot, wt.p, bt.p);
+  Cnt8 base;
+  uint64_t ntask = 0;
+  for (int i = 0; i < kSlots; ++i) {
+    base.v[i] = (uint32_t)ntask;
+    ntask += tot.v[i];
+  }
+  if (ntask > 0xFFFFFFFFull) fail(TRIBIT_ERR_TOO_LARGE, "more than 2^32-1 tasks");
+  DevBuf<uint4> tasks(ctx, ntask ? ntask : 1);
+  k_sched_fill<<<grid_for(n, kBlock), kBlock, 0, s>>>(sv, pd, n, off.p, base, tasks.p);
   cnt.reset();
   off.reset();
   TB_HIP(hipEventRecord(ctx->ev[3], s));
 
   const int cus = ctx->cus > 0 ? ctx->cus : 256;
   uint64_t blocks = 0;
-  // one launch per variant; an event after each gives its device time
+  // one launch per slot; an event after each gives its device time
   const int att = ((flags & TRIBIT_COUNT_FORCE_LINEAR_PROBE) ? 0 : kCuckooSeeds) |
-                  ((flags & TRIBIT_COUNT_FORCE_GLOBAL_LOADS) ? kForceGlobal : 0);
-  launch_wave<kC0Logs, kC0K>(gw[0], tot.x, cus, s, g, rng, wt.p, d_queue + 0, d_result, att, &blocks);
-  TB_HIP(hipEventRecord(ctx->ev[4], s));
-  launch_wave<kC1Logs, kC1K>(gw[1], tot.y, cus, s, g, rng, wt.p + tot.x, d_queue + 1, d_result, att, &blocks);
-  TB_HIP(hipEventRecord(ctx->ev[5], s));
-  launch_block<kC2Logs>(gw[2], tot.z, cus, s, g, rng, bt.p, d_queue + 2, d_result, att, &blocks);
-  TB_HIP(hipEventRecord(ctx->ev[6], s));
-  launch_block<kC2Logs>(gw[3], tot.w, cus, s, g, rng, bt.p + tot.z, d_queue + 3, d_result, att, &blocks);
+                  ((flags & TRIBIT_COUNT_FORCE_GLOBAL_LOADS) ? kForceGlobal : 0) |
+                  ((flags & TRIBIT_COUNT_NO_BITMAP) ? kNoBitmapBit : 0);
+  for (int i = 0; i < plan.nslots; ++i) {
+    const uint4* tk = tasks.p + base.v[i];
+    unsigned long long* tl = tlog ? tlog + (size_t)i * kBuckets : nullptr;
+    if (plan.kind[i] == kKindWave64)
+      launch_wave<kC0Logs, kC0K>(plan.g[i], tot.v[i], cus, s, g, rng, tk, d_queue + i, d_result, att, tl, &blocks);
+    else if (plan.kind[i] == kKindWave512)
+      launch_wave<kC1Logs, kC1K>(plan.g[i], tot.v[i], cus, s, g, rng, tk, d_queue + i, d_result, att, tl, &blocks);
+    else
+      launch_block<kC2Logs>(plan.g[i], tot.v[i], cus, s, g, rng, tk, d_queue + i, d_result, att, tl, &blocks);
+    TB_HIP(hipEventRecord(ctx->ev[4 + i], s));
+  }
   TB_HIP(hipGetLastError());
-  TB_HIP(hipEventRecord(ctx->ev[7], s));
+  hipEvent_t last = ctx->ev[3 + plan.nslots];
   unsigned long long res[2] = {0, 0};
   TB_HIP(hipMemcpyAsync(res, d_result, sizeof(res), hipMemcpyDeviceToHost, s));
-  TB_HIP(hipEventSynchronize(ctx->ev[7]));
+  TB_HIP(hipEventSynchronize(last));
   ctx->sync();
   *ms_sched = elapsed(ctx->ev[2], ctx->ev[3]);
-  *ms_kernel = elapsed(ctx->ev[3], ctx->ev[7]);
-  *ms_total = elapsed(ctx->ev[2], ctx->ev[7]);
-  for (int v = 0; v < kVariants; ++v) ms_var[v] = elapsed(ctx->ev[3 + v], ctx->ev[4 + v]);
+  *ms_kernel = elapsed(ctx->ev[3], last);
+  *ms_total = elapsed(ctx->ev[2], last);
+  for (int v = 0; v < plan.nslots; ++v) ms_var[v] = elapsed(ctx->ev[3 + v], ctx->ev[4 + v]);
   if (info) {
     info->launch_blocks = blocks;
-    info->tasks = nw + nb;
-    info->tasks_by_variant[0] = tot.x;
-    info->tasks_by_variant[1] = tot.y;
-    info->tasks_by_variant[2] = tot.z;
-    info->tasks_by_variant[3] = tot.w;
-    info->nvariants = kVariants;
+    info->tasks = ntask;
+    for (int i = 0; i < 8; ++i) info->tasks_by_variant[i] = i < plan.nslots ? tot.v[i] : 0;
+    info->nvariants = plan.nslots;
     info->fallback_tasks = res[1];
-    for (int i = 0; i < 4; ++i) info->group_width[i] = gw[i];
+    for (int i = 0; i < 8; ++i) info->group_width[i] = i < plan.nslots ? plan.g[i] : 0;
   }
   return res[0];
 }
@@ -675,20 +996,37 @@ static double median(std::vector<double> v) {
   return k ? ((k & 1) ? v[k / 2] : 0.5 * (v[k / 2 - 1] + v[k / 2])) : 0.0;
 }
 
+// The plan a schedule runs with: static widths for PLAIN/DIRECTED/WEDGE
+// (overridable per class), the menu for LEARNED (table filled at count time).
+static Plan plan_for(int sched, const int32_t* override_gw, int* layout) {
+  if (sched == TRIBIT_SCHED_LEARNED) {
+    *layout = -1;  // resolved by the caller
+    Plan p = menu_plan();
+    heuristic_table(p.table);
+    return p;
+  }
+  *layout = sched;
+  int gw[4];
+  default_widths(sched, gw);
+  for (int i = 0; i < 4; ++i)
+    if (override_gw && override_gw[i]) gw[i] = override_gw[i];
+  return static_plan(gw);
+}
+
 void count(tribit_ctx* ctx, tribit_graph* g, const tribit_count_opts& opts, tribit_count_info* info) {
   const int reps = opts.repeats > 0 ? opts.repeats : 1;
-  std::vector<double> tt, ts, tk, tv[kVariants];
+  std::vector<double> tt, ts, tk, tv[kSlots];
   uint64_t first = 0;
   bool agree = true;
   tribit_count_info local;
   memset(&local, 0, sizeof(local));
-  int gw[4];
-  default_widths(opts.sched, gw);
-  for (int i = 0; i < 4; ++i)
-    if (opts.group_width[i]) gw[i] = opts.group_width[i];
+  int layout = 0;
+  Plan plan = plan_for(opts.sched, opts.group_width, &layout);
+  const bool learned = opts.sched == TRIBIT_SCHED_LEARNED;
+  if (learned) layout = g->rng_sfx ? TRIBIT_SCHED_WEDGE : TRIBIT_SCHED_DIRECTED;
   for (int r = 0; r < reps; ++r) {
-    double a, b, c, v[kVariants];
-    const uint64_t tri = count_once(ctx, g, opts.sched, opts.flags, gw, &a, &b, &c, v, &local);
+    double a, b, c, v[kSlots] = {0};
+    const uint64_t tri = count_once(ctx, g, layout, opts.flags, plan, learned, &a, &b, &c, v, &local, nullptr);
     if (r == 0)
       first = tri;
     else if (tri != first)
@@ -696,56 +1034,75 @@ void count(tribit_ctx* ctx, tribit_graph* g, const tribit_count_opts& opts, trib
     tt.push_back(a);
     ts.push_back(b);
     tk.push_back(c);
-    for (int i = 0; i < kVariants; ++i) tv[i].push_back(v[i]);
+    for (int i = 0; i < kSlots; ++i) tv[i].push_back(v[i]);
   }
   local.triangles = first;
   local.ms_count = median(tt);
   local.ms_sched = median(ts);
   local.ms_kernel = median(tk);
-  for (int i = 0; i < kVariants; ++i) local.ms_by_variant[i] = median(tv[i]);
+  for (int i = 0; i < kSlots; ++i) local.ms_by_variant[i] = i < plan.nslots ? median(tv[i]) : 0.0;
   local.repeats_agree = agree ? 1 : 0;
   if (info) *info = local;
 }
 
-// Per-variant work of a schedule: owner lists, partners (edges), streamed
-// elements (wedge checks).
-__global__ void k_variant_stats(SchedView sv, uint64_t n, unsigned long long* __restrict__ st) {
-  __shared__ unsigned long long acc[3 * kVariants];
-  if (threadIdx.x < 3 * kVariants) acc[threadIdx.x] = 0;
-  __syncthreads();
-  const uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (x < n) {
-    const uint32_t d = sv.row_ptr[x + 1] - sv.row_ptr[x];
-    const int c = row_class(d);
-    if (c >= 0) {
-      for (int l = 0; l < sv.nlists; ++l) {
-        uint32_t b, k;
-        uint64_t w;
-        owner_list(sv, x, l, &b, &k, &w);
-        if (!list_useful(l, d, k)) continue;
-        atomicAdd(&acc[c], 1ull);
-        atomicAdd(&acc[kVariants + c], (unsigned long long)k);
-        atomicAdd(&acc[2 * kVariants + c], (unsigned long long)w);
-      }
+// Timed count under a menu plan given per bucket (NULL: the heuristic), with
+// task durations logged per (slot, bucket).
+void count_profile(tribit_ctx* ctx, tribit_graph* g, int layout, const uint8_t* table, tribit_profile* out) {
+  memset(out, 0, sizeof(*out));
+  Plan plan = menu_plan();
+  if (table) {
+    for (int b = 0; b < kBuckets; ++b) {
+      if (table[b] >= kSlots || !slot_feasible(table[b], b)) fail(TRIBIT_ERR_ARG, "plan slot not feasible for bucket");
+      plan.table[b] = table[b];
     }
+  } else {
+    heuristic_table(plan.table);
   }
-  __syncthreads();
-  if (threadIdx.x < 3 * kVariants && acc[threadIdx.x]) atomicAdd(&st[threadIdx.x], acc[threadIdx.x]);
+  BucketStats st;
+  bucket_stats(ctx, g, layout, &st);
+  DevBuf<unsigned long long> tl(ctx, (size_t)kSlots * kBuckets);
+  k_zero_u64<<<grid_for(kSlots * kBuckets, kBlock), kBlock, 0, ctx->stream>>>(tl.p, kSlots * kBuckets);
+  double a, b, c, v[kSlots] = {0};
+  out->triangles = count_once(ctx, g, layout, 0, plan, false, &a, &b, &c, v, nullptr, tl.p);
+  out->ms_kernel = c;
+  std::vector<unsigned long long> h((size_t)kSlots * kBuckets);
+  TB_HIP(hipMemcpyAsync(h.data(), tl.p, 8 * h.size(), hipMemcpyDeviceToHost, ctx->stream));
+  ctx->sync();
+  for (int s2 = 0; s2 < kSlots; ++s2)
+    for (int bk = 0; bk < kBuckets; ++bk) out->ticks[s2][bk] = h[(size_t)s2 * kBuckets + bk];
+  for (int bk = 0; bk < kBuckets; ++bk) {
+    out->lists[bk] = st.lists[bk];
+    out->sum_d[bk] = st.sum_d[bk];
+    out->sum_k[bk] = st.sum_k[bk];
+    out->sum_w[bk] = st.sum_w[bk];
+    out->eligible[bk] = st.eligible[bk];
+    out->slot[bk] = plan.table[bk];
+  }
+  int khz = 0;
+  TB_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, ctx->device));
+  out->tick_ns = khz > 0 ? 1e6 / khz : 10.0;
 }
 
+void learned_plan(tribit_ctx* ctx, tribit_graph* g, uint8_t* table) {
+  BucketStats st;
+  bucket_stats(ctx, g, g->rng_sfx ? TRIBIT_SCHED_WEDGE : TRIBIT_SCHED_DIRECTED, &st);
+  learned_table(ctx, st, table);
+}
+
+// Per-slot work of a schedule's plan (LEARNED: the table the model picks).
 void variant_stats(tribit_ctx* ctx, tribit_graph* g, int sched, tribit_variant_work* out) {
   memset(out, 0, sizeof(*out));
-  DevBuf<unsigned long long> st(ctx, 3 * kVariants);
-  k_zero_u64<<<1, 64, 0, ctx->stream>>>(st.p, 3 * kVariants);
-  if (g->n)
-    k_variant_stats<<<grid_for(g->n, kBlock), kBlock, 0, ctx->stream>>>(sched_view(g, sched), g->n, st.p);
-  unsigned long long h[3 * kVariants];
-  TB_HIP(hipMemcpyAsync(h, st.p, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
-  ctx->sync();
-  for (int i = 0; i < kVariants; ++i) {
-    out->rows[i] = h[i];
-    out->edges[i] = h[kVariants + i];
-    out->wplus[i] = h[2 * kVariants + i];
+  int layout = 0;
+  Plan plan = plan_for(sched, nullptr, &layout);
+  if (sched == TRIBIT_SCHED_LEARNED) layout = g->rng_sfx ? TRIBIT_SCHED_WEDGE : TRIBIT_SCHED_DIRECTED;
+  BucketStats st;
+  bucket_stats(ctx, g, layout, &st);
+  if (sched == TRIBIT_SCHED_LEARNED) learned_table(ctx, st, plan.table);
+  for (int b = 0; b < kBuckets; ++b) {
+    const int s = plan.table[b];
+    out->rows[s] += st.lists[b];
+    out->edges[s] += st.sum_k[b];
+    out->wplus[s] += st.sum_w[b];
   }
 }
 

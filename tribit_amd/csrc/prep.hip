@@ -288,6 +288,46 @@ __global__ void k_rv_fill(const uint64_t* __restrict__ keys, uint64_t mr, const 
   if (tail && sum) atomicAdd(&w_rv[v], (unsigned long long)sum);
 }
 
+// Row end of every arc e = (u -> v): rend[e] = row_ptr[u + 1] (okeys sorted by (u, v)).
+__global__ void k_arc_row_end(const uint64_t* __restrict__ okeys, uint64_t m, const uint32_t* __restrict__ row_ptr,
+                              uint32_t* __restrict__ rend) {
+  const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < m) rend[e] = row_ptr[(uint32_t)(okeys[e] >> 32) + 1];
+}
+
+// Arcs e = (u -> v) sorted by head v: the suffix of N+(u) after v,
+// [e + 1, rend[e]), is streamed against N+(v). With ids in degree-rank order
+// N+(v) holds only ids > v, so the elements of N+(u) up to v can never match.
+// sfx_cnt[v] += 1, w_sfx[v] += suffix length, aggregated per run of v in a wave.
+__global__ void k_sfx_fill(const uint32_t* __restrict__ head, const uint32_t* __restrict__ arc, uint64_t m,
+                           const uint32_t* __restrict__ rend, uint2* __restrict__ rng,
+                           uint32_t* __restrict__ sfx_cnt, unsigned long long* __restrict__ w_sfx) {
+  const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = j < m;
+  const uint32_t v = valid ? head[j] : 0xFFFFFFFFu;
+  uint64_t len = 0;
+  if (valid) {
+    const uint32_t e = arc[j], end = rend[e];
+    rng[j] = make_uint2(e + 1, end);
+    len = end - e - 1;
+  }
+  const uint32_t lane = lane_id();
+  uint64_t sum = len;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint64_t t = __shfl_up(sum, o);
+    const uint32_t kv = __shfl_up(v, o);
+    if (lane >= (uint32_t)o && kv == v) sum += t;
+  }
+  const uint32_t nv = __shfl_down(v, 1);
+  const uint64_t vmask = __ballot(valid);
+  const uint64_t above = (lane == 63) ? 0ull : (vmask >> (lane + 1));
+  const bool tail = valid && (!(above & 1ull) || nv != v);
+  const uint32_t run = wave_run_length(v, valid);
+  if (run) atomicAdd(&sfx_cnt[v], run);
+  if (tail && sum) atomicAdd(&w_sfx[v], (unsigned long long)sum);
+}
+
 __global__ void k_fill_u64(unsigned long long* __restrict__ p, uint64_t n) {
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = 0ull;
@@ -396,11 +436,17 @@ void preprocess(tribit_ctx* ctx, uint64_t* d_keys_in, uint64_t nkeys, uint64_t n
       // 6. CSR
       k_csr_split<<<grid_for(m, kBlock), kBlock, 0, s>>>(uniq.p, m, col.p, odeg.p);
     }
-    uniq.reset();
     with_temp(ctx, [&](void* t, size_t& b) {
       return rocprim::exclusive_scan(t, b, odeg.p, row_ptr.p, 0u, (size_t)(n + 1), rocprim::plus<uint32_t>(), s);
     });
   }
+  DevBuf<uint32_t> rend;  // row end of every arc (wedge ranges of relabelled graphs)
+  rend.ctx = ctx;
+  if (ctx->relabel && m) {
+    rend = DevBuf<uint32_t>(ctx, m);
+    k_arc_row_end<<<grid_for(m, kBlock), kBlock, 0, s>>>(uniq.p, m, row_ptr.p, rend.p);
+  }
+  uniq.reset();
 
   // 7. per-row wedge counts and totals
   DevBuf<uint64_t> wrow(ctx, n);
@@ -450,6 +496,34 @@ void preprocess(tribit_ctx* ctx, uint64_t* d_keys_in, uint64_t nkeys, uint64_t n
     k_fill_u32<<<grid_for(n + 1, kBlock), kBlock, 0, s>>>(rv_ptr.p, n + 1, 0u);
     k_fill_u64<<<grid_for(n, kBlock), kBlock, 0, s>>>(reinterpret_cast<unsigned long long*>(w_fw.p), n);
   }
+  // 9. wedge ranges (relabelled graphs): in-arcs of every v, each with its suffix range
+  DevBuf<uint2> rng_sfx;
+  DevBuf<uint32_t> sfx_ptr;
+  DevBuf<uint64_t> w_sfx;
+  rng_sfx.ctx = sfx_ptr.ctx = w_sfx.ctx = ctx;
+  if (ctx->relabel) {
+    rng_sfx = DevBuf<uint2>(ctx, m ? m : 1);
+    sfx_ptr = DevBuf<uint32_t>(ctx, n + 1);
+    w_sfx = DevBuf<uint64_t>(ctx, n);
+    k_fill_u64<<<grid_for(n, kBlock), kBlock, 0, s>>>(reinterpret_cast<unsigned long long*>(w_sfx.p), n);
+    DevBuf<uint32_t> cnt(ctx, n + 1);
+    k_fill_u32<<<grid_for(n + 1, kBlock), kBlock, 0, s>>>(cnt.p, n + 1, 0u);
+    if (m) {
+      DevBuf<uint32_t> arcs(ctx, m), head(ctx, m), arc_sorted(ctx, m);
+      k_iota_u32<<<grid_for(m, kBlock), kBlock, 0, s>>>(arcs.p, m);
+      with_temp(ctx, [&](void* t, size_t& b) {
+        return rocprim::radix_sort_pairs(t, b, col.p, head.p, arcs.p, arc_sorted.p, (size_t)m, 0u, bits_for(n), s);
+      });
+      arcs.reset();
+      k_sfx_fill<<<grid_for(m, kBlock), kBlock, 0, s>>>(head.p, arc_sorted.p, m, rend.p, rng_sfx.p, cnt.p,
+                                                        reinterpret_cast<unsigned long long*>(w_sfx.p));
+    }
+    with_temp(ctx, [&](void* t, size_t& b) {
+      return rocprim::exclusive_scan(t, b, cnt.p, sfx_ptr.p, 0u, (size_t)(n + 1), rocprim::plus<uint32_t>(), s);
+    });
+  }
+  rend.reset();
+
   // the graph owns its buffers from here on (freed by tribit_graph_free, also on error)
   g->row_ptr = row_ptr.release();
   g->col = col.release();
@@ -465,7 +539,11 @@ void preprocess(tribit_ctx* ctx, uint64_t* d_keys_in, uint64_t nkeys, uint64_t n
   g->rv_ptr = rv_ptr.release();
   g->w_fw = w_fw.release();
   g->w_rv = w_rv.release();
-  g->bytes = 4 * (n + 1) * 3 + 4 * (m ? m : 1) + 20 * (m ? m : 1) + 4 * n + 8 * n * 3 + (g->labels ? 4 * n : 0);
+  g->rng_sfx = rng_sfx.release();
+  g->sfx_ptr = sfx_ptr.release();
+  g->w_sfx = w_sfx.release();
+  g->bytes = 4 * (n + 1) * 3 + 4 * (m ? m : 1) + 20 * (m ? m : 1) + 4 * n + 8 * n * 3 + (g->labels ? 4 * n : 0) +
+             (g->rng_sfx ? 8 * (m ? m : 1) + 4 * (n + 1) + 8 * n : 0);
   g->tiles64 = m ? count_tiles(ctx, g) : 0;
   TB_HIP(hipEventRecord(ctx->ev[1], s));
   TB_HIP(hipEventSynchronize(ctx->ev[1]));

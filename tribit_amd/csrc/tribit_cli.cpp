@@ -11,7 +11,7 @@
 // Added (long options only, so `-i` stays byte-compatible):
 //   --rmat S,EF,A,B,C,SEED  generate an R-MAT graph in HBM instead of reading
 //   --er N,M,SEED           generate Erdos-Renyi G(n, M) in HBM
-//   --sched plain|directed|learned   scheduler choice (default directed)
+//   --sched plain|directed|wedge|learned   scheduler choice (default learned)
 //   --repeats K             timed counting repetitions (median reported)
 //   --json PATH             write a JSON side record of the measured pass
 // Input or device errors print a message and exit 2 (input) or 3 (device);
@@ -30,7 +30,7 @@ namespace {
 struct Config {
   std::string input_file;
   std::string rmat, er, json;
-  int sched = TRIBIT_SCHED_DIRECTED;
+  int sched = TRIBIT_SCHED_LEARNED;
   int repeats = 1;
 };
 
@@ -41,7 +41,7 @@ std::string get_usage(const char* prog) {
          "  -i <input_file>    Path to the input file (required)\n"
          "  --rmat S,EF,A,B,C,SEED   Generate an R-MAT graph instead of reading a file\n"
          "  --er N,M,SEED            Generate an Erdos-Renyi G(n,M) graph instead\n"
-         "  --sched plain|directed|learned  Scheduler (default directed)\n"
+         "  --sched plain|directed|wedge|learned  Scheduler (default learned)\n"
          "  --repeats K              Timed counting repetitions (default 1)\n"
          "  --json PATH              Write a JSON record of the measured pass\n";
 }
@@ -74,8 +74,10 @@ Config program_options(int argc, char** argv) {
           c.sched = TRIBIT_SCHED_LEARNED;
         else if (!strcmp(optarg, "directed"))
           c.sched = TRIBIT_SCHED_DIRECTED;
+        else if (!strcmp(optarg, "wedge"))
+          c.sched = TRIBIT_SCHED_WEDGE;
         else
-          usage_exit(argv[0], "Error: --sched must be plain, directed or learned\n");
+          usage_exit(argv[0], "Error: --sched must be plain, directed, wedge or learned\n");
         break;
       case 4:
         c.repeats = atoi(optarg);
@@ -124,6 +126,15 @@ tribit_graph* build_graph(tribit_ctx* ctx, const Input& in) {
 }
 
 }  // namespace
+
+static const char* sched_name(int s) {
+  switch (s) {
+    case TRIBIT_SCHED_PLAIN: return "plain";
+    case TRIBIT_SCHED_DIRECTED: return "directed";
+    case TRIBIT_SCHED_WEDGE: return "wedge";
+    default: return "learned";
+  }
+}
 
 int main(int argc, char** argv) {
   printf("%s\n", tribit_build_info());
@@ -209,7 +220,7 @@ int main(int argc, char** argv) {
               (unsigned long long)gi.wplus, (unsigned long long)ci.triangles, gi.ms_preprocess, ci.ms_count,
               ci.ms_sched, ci.ms_kernel, gteps, (unsigned long long)ci.launch_blocks,
               (unsigned long long)ci.tasks,
-              cfg.sched == TRIBIT_SCHED_LEARNED ? "learned" : (cfg.sched == TRIBIT_SCHED_PLAIN ? "plain" : "directed"),
+              sched_name(cfg.sched),
               (unsigned long long)peak);
       fclose(f);
     }
