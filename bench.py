@@ -1,15 +1,17 @@
 """Benchmark: exact triangle counting GTEPS on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--scale 25] [--ef 16] [--seed 5]
-                    [--sched plain|learned] [--no-cpu-baseline]
+                    [--sched learned|wedge|directed|plain] [--no-cpu-baseline]
 
 Workload (default): R-MAT scale 25 (33,554,432 vertices), edge factor 16,
 (a,b,c,d) = (0.57,0.19,0.19,0.05), seed 5 -- generated in HBM by the library,
-symmetrised, deduplicated, self-loops removed, degree-oriented int32 CSR.
-A step is one scheduled exact count of the resident graph (scheduler + counting
+symmetrised, deduplicated, self-loops removed, vertices renumbered by degree
+rank, degree-oriented int32 CSR. A step is one scheduled exact count of the
+resident graph (scheduler, including the cost model's choice, + counting
 kernels); the count of every step must agree. The selected schedule (default
-"directed": every edge streams the shorter of its two lists) is timed, then
-the "plain" schedule (textbook direction) on the same graph for comparison.
+"learned": wedge ranges, per-bucket kernel variant picked by the trained cost
+model in tribit_amd/cost_model.f32) is timed, then the "plain" schedule
+(textbook edge iterator with the static outdeg thresholds) on the same graph.
 
 Prints ONE JSON line. value = m / (wall-clock seconds per step) / 1e9 (GTEPS),
 with m the undirected simple edges. roofline: the dominant counting kernel's
@@ -35,10 +37,15 @@ sys.path.insert(0, ROOT)
 
 METRIC = "exact-TC GTEPS on 1 MI355X: learned-sched vs plain; % HBM roofline"
 HBM_PEAK_GBS = 8000.0
-def variant_name(v: int, width: int) -> str:
-    """Kernel instantiation of variant v (class) at lane-group width `width` (32-bit buffer-load path)."""
-    return [f"k_count_wave<6,64,{width},8,true>", f"k_count_wave<9,256,{width},8,true>",
-            f"k_count_block<12,{width},8,true>", f"k_count_block<12,{width},8,true> (id-range chunks)"][v]
+def variant_name(sched: str, v: int, width: int) -> str:
+    """Kernel instantiation of launch slot v at lane-group width `width` (32-bit buffer-load path)."""
+    names = [f"k_count_wave<6,64,{width},8,true>", f"k_count_wave<9,256,{width},8,true>",
+             f"k_count_block<12,{width},8,true>"]
+    if sched == "learned":
+        import tribit_amd
+
+        return names[tribit_amd.menu()[v][0]]
+    return (names + [f"k_count_block<12,{width},8,true> (id-range chunks)"])[v]
 
 
 def log(msg: str) -> None:
@@ -105,9 +112,8 @@ def measured_traffic(args, variant: int):
         return None
     with open(path) as f:
         summ = json.load(f)
-    names = {0: "class 0", 1: "class 1", 2: "class 2", 3: "class 3"}
     for launch in summ.get("launches", []):
-        if launch.get("variant") == names.get(variant):
+        if launch.get("slot") == variant:
             return int(launch["traffic_bytes"])
     return None
 
@@ -169,6 +175,8 @@ def mixed_family(args) -> int:
     import tribit_amd
 
     ctx = tribit_amd.Context(0)
+    if args.sched == "learned":
+        log(f"cost model loaded: {ctx.load_cost_model()}")
     for name, desc, make in MIXED:
         g = make(ctx)
         inf = g.info()
@@ -200,7 +208,7 @@ def main() -> int:
     ap.add_argument("--scale", type=int, default=25)
     ap.add_argument("--ef", type=int, default=16)
     ap.add_argument("--seed", type=int, default=5)
-    ap.add_argument("--sched", choices=["plain", "directed", "learned"], default="directed")
+    ap.add_argument("--sched", choices=["plain", "directed", "wedge", "learned"], default="learned")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-schedule comparison run")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -223,6 +231,9 @@ def main() -> int:
 
     tribit_amd.load_library()
     ctx = tribit_amd.Context(0)
+    model = ctx.load_cost_model() if args.sched == "learned" else False
+    if args.sched == "learned":
+        log(f"cost model loaded: {model}")
     dev = ctx.device_info()
     log(f"device {dev}; generating R-MAT s{args.scale} ef{args.ef} seed {args.seed}")
     t0 = time.perf_counter()
@@ -266,8 +277,10 @@ def main() -> int:
         "data": "synthetic (R-MAT generated in HBM by the library's counter-based generator)",
         "config": {
             "workload": f"R-MAT scale {args.scale}, edge factor {args.ef}, (a,b,c,d)=(0.57,0.19,0.19,0.05), "
-                        f"seed {args.seed}; symmetrised, deduplicated, self-loops removed; degree-oriented int32 CSR",
+                        f"seed {args.seed}; symmetrised, deduplicated, self-loops removed; degree-rank ids; "
+                        f"degree-oriented int32 CSR",
             "n": n, "m": m, "wplus": inf["wplus"], "sched": args.sched, "parallelism": "single GPU",
+            "cost_model": "tribit_amd/cost_model.f32" if model else None,
         },
         "triangles": tri,
         "ms_count_device": round(ms_dev, 4),
@@ -285,9 +298,10 @@ def main() -> int:
             "ms_count_device": round(plain["ms_dev"], 4),
             "ms_by_variant": [round(x, 4) for x in plain["ms_var"]],
         },
+        "speedup_vs_plain": None if plain is None else round(plain["ms_per_step"] / ms_per_step, 3),
         "roofline": {
             "bound": "hbm",
-            "kernel": variant_name(dom, sel["group_width"][dom]),
+            "kernel": variant_name(args.sched, dom, sel["group_width"][dom]),
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",

@@ -76,8 +76,9 @@ typedef enum {
 /* tribit_count_opts.flags */
 #define TRIBIT_COUNT_FORCE_LINEAR_PROBE 1 /* test hook: skip the cuckoo sets, use the fallback tables */
 #define TRIBIT_COUNT_FORCE_GLOBAL_LOADS 2 /* test hook: stream with global loads (the path for col >= 4 GiB) */
-#define TRIBIT_COUNT_NO_TILES 4           /* never use the bit-tile variant */
-#define TRIBIT_COUNT_FORCE_TILES 8        /* use the bit-tile variant for every owner list (test hook) */
+#define TRIBIT_COUNT_NO_TILES 4           /* never stream bit tiles (block launches stream elements) */
+#define TRIBIT_COUNT_FORCE_TILES 8        /* every block launch streams 64-column bit tiles for owners whose
+                                             set fits the LDS bitmap (test hook) */
 #define TRIBIT_COUNT_NO_BITMAP 16         /* test hook: block tasks keep hashed sets even for narrow id windows */
 
 typedef struct {
@@ -203,6 +204,7 @@ typedef struct {
   uint64_t sum_k[TRIBIT_BUCKETS];     /* streamed ranges */
   uint64_t sum_w[TRIBIT_BUCKETS];     /* streamed elements */
   uint64_t eligible[TRIBIT_BUCKETS];  /* owners whose ids fit a 2^18-id bitmap window */
+  uint64_t sum_t[TRIBIT_BUCKETS];     /* 64-column tiles of the owners' rows */
   uint64_t ticks[TRIBIT_MENU_SLOTS][TRIBIT_BUCKETS]; /* summed task durations (wall-clock ticks) */
   uint8_t slot[TRIBIT_BUCKETS];       /* the plan that ran */
   double tick_ns;                     /* nanoseconds per tick */

@@ -180,7 +180,7 @@ def test_bitmap_and_hashed_sets(gpu_ctx, oracle_mod):
         try:
             with gpu_ctx.graph_from_keys(keys, n) as g:
                 for sched in scheds_of(g):
-                    for fl in (0, tribit_amd.NO_BITMAP, tribit_amd.FORCE_LINEAR_PROBE):
+                    for fl in (0, tribit_amd.NO_BITMAP, tribit_amd.FORCE_LINEAR_PROBE, tribit_amd.FORCE_TILES):
                         r = g.count(sched=sched, flags=fl)
                         assert r.triangles == want, (on, sched, fl)
                         assert (sum(r.tasks_by_variant[5:8]) if sched == "learned" else r.tasks_by_variant[2]) > 0
@@ -372,3 +372,37 @@ def test_plans_profiles_and_learned_tables(gpu_ctx, oracle_mod):
                 finally:
                     gpu_ctx.set_cost_model(None)
             assert np.array_equal(g.learned_plan(), heur)
+
+
+@pytest.mark.parametrize("spec", [
+    {"gen": "sbm", "blocks": 40, "size": 512, "p_in": 0.5, "p_out": 1e-3, "seed": 5},
+    {"gen": "rmat", "scale": 18, "ef": 16, "a": 0.57, "b": 0.19, "c": 0.19, "seed": 3},
+    {"gen": "chung_lu", "n": 100000, "gamma": 2.1, "avg_deg": 48.0, "seed": 6},
+], ids=lambda s: s["gen"])
+def test_bit_tiles(gpu_ctx, oracle_mod, spec):
+    """64-column bit tiles (AND + popcount against the LDS bitmap) count the same under every schedule,
+    relabelled or not, and through the menu's tile slot."""
+    n = spec["blocks"] * spec["size"] if spec["gen"] == "sbm" else (1 << spec["scale"] if spec["gen"] == "rmat"
+                                                                    else spec["n"])
+    r = {"spec": spec, "n": n}
+    note(f"bit tiles {spec}")
+    want = oracle_mod.count_keys(gens.oracle_keys(oracle_mod, r), n)
+    for on in (True, False):
+        gpu_ctx.set_relabel(on)
+        try:
+            with gens.gpu_graph(gpu_ctx, r) as g:
+                for sched in scheds_of(g):
+                    assert g.count(sched=sched, flags=tribit_amd.FORCE_TILES).triangles == want, (on, sched)
+                if on:
+                    tile_table = np.array([7 if tribit_amd.bucket_max_outdeg(b) > 256 else 0 if
+                                           tribit_amd.bucket_max_outdeg(b) <= 32 else 2
+                                           for b in range(tribit_amd.BUCKETS)], np.uint8)
+                    assert tribit_amd.menu()[7][0] == 3
+                    assert g.profile("wedge", tile_table).triangles == want
+                    assert g.profile("directed", tile_table).triangles == want
+        finally:
+            gpu_ctx.set_relabel(True)
+    keys, t = graphs.complete(2300)
+    with gpu_ctx.graph_from_keys(keys, 2300) as g:
+        for sched in scheds_of(g):
+            assert g.count(sched=sched, flags=tribit_amd.FORCE_TILES).triangles == t

@@ -70,6 +70,7 @@ class CountInfo(ctypes.Structure):
     ]
 
 
+COST_MODEL_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cost_model.f32")
 BUCKETS = 800
 MENU_SLOTS = 8
 FEATURES = 16
@@ -79,7 +80,8 @@ class Profile(ctypes.Structure):
     _fields_ = [
         ("lists", ctypes.c_uint64 * BUCKETS), ("sum_d", ctypes.c_uint64 * BUCKETS),
         ("sum_k", ctypes.c_uint64 * BUCKETS), ("sum_w", ctypes.c_uint64 * BUCKETS),
-        ("eligible", ctypes.c_uint64 * BUCKETS), ("ticks", (ctypes.c_uint64 * BUCKETS) * MENU_SLOTS),
+        ("eligible", ctypes.c_uint64 * BUCKETS), ("sum_t", ctypes.c_uint64 * BUCKETS),
+        ("ticks", (ctypes.c_uint64 * BUCKETS) * MENU_SLOTS),
         ("slot", ctypes.c_uint8 * BUCKETS), ("tick_ns", ctypes.c_double), ("ms_kernel", ctypes.c_double),
         ("triangles", ctypes.c_uint64),
     ]
@@ -383,6 +385,11 @@ class Context:
 
     def close(self) -> None:
         if self.h:
+            own = getattr(self, "_owned_model", None)
+            if own is not None:
+                self.set_cost_model(None)
+                own.free()
+                self._owned_model = None
             self.lib.tribit_shutdown(self.h)
             self.h = _P()
 
@@ -405,6 +412,26 @@ class Context:
     def set_cost_model(self, m) -> None:
         """Model used by sched="learned" (None: the density-threshold heuristic)."""
         _check(self.lib.tribit_set_cost_model(self.h, m.h if m is not None else None), "set_cost_model")
+        self._model = m
+
+    def load_cost_model(self, path: str | None = None) -> bool:
+        """Loads trained parameters (default: the package's cost_model.f32) and sets the model.
+        Returns False when the file does not exist (sched="learned" then keeps the heuristic)."""
+        path = path or COST_MODEL_PATH
+        if not os.path.exists(path):
+            return False
+        p = np.fromfile(path, dtype="<f4")
+        m = self.mlp(FEATURES, seed=1)
+        if p.size != m.num_params():
+            m.free()
+            raise ValueError(f"{path}: {p.size} parameters, model has {m.num_params()}")
+        m.set_params(p)
+        self.set_cost_model(m)
+        old = getattr(self, "_owned_model", None)
+        if old is not None:
+            old.free()
+        self._owned_model = m
+        return True
 
     def set_relabel(self, on: bool) -> None:
         """Degree-rank relabelling of graphs built from now on (default on)."""

@@ -157,11 +157,10 @@ struct tribit_graph {
   uint32_t* tile_ptr = nullptr;  // n+1
   uint32_t* tile_blk = nullptr;  // ntiles, column / 64
   uint64_t* tile_mask = nullptr; // ntiles, bit column % 64
+  uint32_t* tile_idx = nullptr;  // m, tile holding each element
   uint2* trng_all = nullptr;     // m, tile ranges aligned with rng_all
   uint2* trng_dir = nullptr;     // m, tile ranges aligned with rng_dir
-  uint64_t* tw_all = nullptr;    // n, tiles streamed per owner list (rng_all layout)
-  uint64_t* tw_fw = nullptr;     // n, (rng_dir forward lists)
-  uint64_t* tw_rv = nullptr;     // n, (rng_dir reverse lists)
+  uint2* trng_sfx = nullptr;     // m, tile ranges aligned with rng_sfx
   double ms_prep = 0.0;
   uint64_t bytes = 0;
 };
@@ -205,7 +204,7 @@ void mlp_infer(tribit_mlp* m, const float* hX, uint64_t n, float* hout);
 void mlp_infer_dev(tribit_mlp* m, const float* dX, uint64_t n, float* dout);
 // tiles.hip
 uint64_t count_tiles(tribit_ctx* ctx, tribit_graph* g);
-void build_tiles(tribit_ctx* ctx, tribit_graph* g);
+void build_tiles(tribit_ctx* ctx, tribit_graph* g, int layout);
 void free_tiles(tribit_graph* g);
 // count.hip
 // mtx.cpp

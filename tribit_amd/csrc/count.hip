@@ -238,6 +238,63 @@ __device__ __forceinline__ uint32_t stream_ranges(const ColSrc& src, const uint2
   return cnt;
 }
 
+// Tile streaming for block tasks whose set is a bitmap: each item is a tile
+// (blk, mask) of a streamed list; its matches are popcount(mask & the
+// bitmap's 64-bit word for columns [64 blk, 64 blk + 64)), one LDS read per
+// 64 columns. Same range hand-out as stream_ranges (block-shared counter).
+struct TileSrc {
+  const uint32_t* blk;
+  const unsigned long long* mask;
+  const uint2* trng;
+};
+
+template <int G, int U>
+__device__ __forceinline__ uint32_t stream_tiles(const TileSrc& ts, const uint2* R, uint32_t nk, uint32_t* s_next,
+                                                 uint32_t lane, const uint32_t* T, uint32_t base, uint32_t span) {
+  constexpr uint64_t kLeads = leader_mask<G>();
+  const uint32_t g = lane / G, gl = lane % G;
+  const uint64_t below = (g == 0) ? 0ull : ((1ull << (g * G)) - 1);
+  const unsigned long long* W = reinterpret_cast<const unsigned long long*>(T);
+  uint32_t base_r = 0, end = 0, cnt = 0;
+  for (;;) {
+    const bool need = base_r >= end;
+    const uint64_t want = __ballot(need && gl == 0);
+    const uint32_t nw = (uint32_t)__popcll(want);
+    uint32_t kb = 0;
+    if (nw) {
+      if (lane == 0) kb = atomicAdd(s_next, nw);
+      kb = __shfl(kb, 0);
+      if (want == kLeads && kb >= nk) break;
+    }
+    if (need) {
+      const uint32_t k = kb + (uint32_t)__popcll(want & below);
+      if (k < nk) {
+        const uint2 r = R[k];
+        base_r = r.x;
+        end = r.y;
+      }
+    }
+    const uint32_t p0 = base_r + gl;
+    const int rem = (int)(end - p0);
+    uint32_t b[U];
+    unsigned long long mk[U];
+#pragma unroll
+    for (int j = 0; j < U; ++j) {
+      const bool ok = j * G < rem;
+      b[j] = ok ? ts.blk[p0 + j * G] : 0xFFFFFFFFu;
+      mk[j] = ok ? ts.mask[p0 + j * G] : 0ull;
+    }
+#pragma unroll
+    for (int j = 0; j < U; ++j) {
+      const uint32_t i = b[j] * 64u - base;
+      const unsigned long long w = i < span ? W[i >> 6] : 0ull;
+      cnt += (uint32_t)__popcll(mk[j] & w);
+    }
+    base_r += (uint32_t)(G * U);
+  }
+  return cnt;
+}
+
 // ------------------------------------------------------ wave-level tasks
 // A task (x, rbase, rcount, chunk) pairs the set N+(x) with the lists whose
 // ranges are rng[rbase, rbase + rcount). A wave owns one task: it builds the
@@ -315,10 +372,10 @@ __global__ __launch_bounds__(256) void k_count_wave(const uint32_t* __restrict__
 
 constexpr int kBlockThreads = 512;
 
-template <int LOGS, int G, int U, bool BUF>
+template <int LOGS, int G, int U, bool BUF, bool TILES>
 __global__ __launch_bounds__(kBlockThreads) void k_count_block(const uint32_t* __restrict__ row_ptr,
                                                                const uint32_t* __restrict__ col, uint64_t m,
-                                                               const uint2* __restrict__ rng,
+                                                               const uint2* __restrict__ rng, TileSrc ts,
                                                                const uint4* __restrict__ tasks, uint32_t ntasks,
                                                                uint32_t* __restrict__ queue,
                                                                unsigned long long* __restrict__ result,
@@ -380,11 +437,13 @@ __global__ __launch_bounds__(kBlockThreads) void k_count_block(const uint32_t* _
       for (uint32_t i = p0 + threadIdx.x; i < p1; i += kBlockThreads) linear_insert<LOGS + 1>(T, col[r0 + i]);
     }
     uint32_t cnt = 0;
+    const bool tiles = TILES && bitmap;  // block-uniform
+    const uint2* rs = tiles ? ts.trng : rng;
     for (uint32_t c0 = 0; c0 < rcount; c0 += kChunk) {
       const uint32_t nk = min((uint32_t)kChunk, rcount - c0);
       for (uint32_t k = threadIdx.x; k < nk; k += kBlockThreads) {
-        uint2 r = rng[rbase + c0 + k];
-        if (ranged) {
+        uint2 r = rs[rbase + c0 + k];
+        if (ranged && !tiles) {  // tiles outside the chunk's window miss in the bitmap
           r.x = lower_bound_u32(col, r.x, r.y, lo);
           r.y = upper_bound_u32(col, r.x, r.y, hi);
         }
@@ -392,7 +451,9 @@ __global__ __launch_bounds__(kBlockThreads) void k_count_block(const uint32_t* _
       }
       if (threadIdx.x == 0) s_next = 0;
       __syncthreads();
-      if (bitmap)
+      if (tiles)
+        cnt += stream_tiles<G, U>(ts, R, nk, &s_next, lane, T, base, kSpan);
+      else if (bitmap)
         cnt += stream_ranges<G, U, true, BUF>(src, R, nk, &s_next, lane, BitmapProbe<LOGS + 1>{T, base});
       else if (ok)
         cnt += stream_ranges<G, U, true, BUF>(src, R, nk, &s_next, lane, CuckooProbe<LOGS>{T, sa});
@@ -440,7 +501,7 @@ constexpr int kBuckets = TRIBIT_BUCKETS;
 constexpr int kSlots = TRIBIT_MENU_SLOTS;
 constexpr int kFeatures = TRIBIT_FEATURES;
 static_assert(kBuckets == kDBins * kLBins, "bucket grid");
-enum SlotKind { kKindWave64 = 0, kKindWave512 = 1, kKindBlock = 2 };
+enum SlotKind { kKindWave64 = 0, kKindWave512 = 1, kKindBlock = 2, kKindTile = 3 };
 
 struct Plan {
   int nslots;
@@ -450,7 +511,7 @@ struct Plan {
 };
 
 struct BucketStats {
-  uint64_t lists[kBuckets], sum_d[kBuckets], sum_k[kBuckets], sum_w[kBuckets], eligible[kBuckets];
+  uint64_t lists[kBuckets], sum_d[kBuckets], sum_k[kBuckets], sum_w[kBuckets], eligible[kBuckets], sum_t[kBuckets];
 };
 
 __host__ __device__ inline uint32_t d_bin_top(int b) {
@@ -520,7 +581,7 @@ __device__ __forceinline__ bool list_useful(const SchedView& v, int l, uint32_t 
 }
 
 __device__ __forceinline__ uint32_t list_splits(int kind, uint32_t count, uint64_t w) {
-  const uint64_t cap = kind == kKindBlock ? kBlockCap : kWaveCap;
+  const uint64_t cap = kind >= kKindBlock ? kBlockCap : kWaveCap;
   uint64_t s = (w + cap - 1) / cap;
   if (s < 1) s = 1;
   if (s > count) s = count;
@@ -528,7 +589,7 @@ __device__ __forceinline__ uint32_t list_splits(int kind, uint32_t count, uint64
 }
 
 __device__ __forceinline__ uint32_t list_chunks(int kind, uint32_t d) {
-  return (kind == kKindBlock && d > kChunk) ? (d + kChunk - 1) / kChunk : 1u;
+  return (kind >= kKindBlock && d > kChunk) ? (d + kChunk - 1) / kChunk : 1u;
 }
 
 // Per owner: tasks per slot.
@@ -580,8 +641,9 @@ __global__ void k_sched_fill(SchedView sv, PlanDev pd, uint64_t n, const Cnt8* _
 }
 
 // Per bucket: lists, sum of outdeg, ranges, streamed elements, owners whose
-// set fits a block bitmap window. Privatised in LDS per block.
-constexpr int kStat = 5;
+// set fits a block bitmap window, 64-column tiles of the owners' rows.
+// Privatised in LDS per block.
+constexpr int kStat = 6;
 __global__ __launch_bounds__(256) void k_bucket_stats(SchedView sv, const uint32_t* __restrict__ col, uint64_t n,
                                                       unsigned long long* __restrict__ st) {
   __shared__ unsigned long long acc[kStat * kBuckets];
@@ -597,6 +659,13 @@ __global__ __launch_bounds__(256) void k_bucket_stats(SchedView sv, const uint32
       if (!list_useful(sv, l, d, k)) continue;
       const int bk = bucket_of(d, w, k);
       const uint32_t span = col[r0 + d - 1] - (col[r0] & ~63u);
+      uint32_t nt = 0, prev = 0xFFFFFFFFu;
+      for (uint32_t e = r0; e < r0 + d; ++e) {
+        const uint32_t tb = col[e] >> 6;
+        nt += tb != prev;
+        prev = tb;
+      }
+      atomicAdd(&acc[5 * kBuckets + bk], (unsigned long long)nt);
       atomicAdd(&acc[bk], 1ull);
       atomicAdd(&acc[kBuckets + bk], (unsigned long long)d);
       atomicAdd(&acc[2 * kBuckets + bk], (unsigned long long)k);
@@ -647,9 +716,9 @@ static const uint2* sched_ranges(const tribit_graph* g, int sched) {
 
 // ------------------------------------------------------------------ plans
 // The learned scheduler's menu: (kind, lane-group width) per slot.
-static const int kMenuKind[kSlots] = {kKindWave64,  kKindWave64, kKindWave512, kKindWave512,
-                                      kKindWave512, kKindBlock,  kKindBlock,   kKindBlock};
-static const int kMenuG[kSlots] = {4, 16, 4, 8, 16, 8, 16, 32};
+static const int kMenuKind[kSlots] = {kKindWave64, kKindWave64, kKindWave512, kKindWave512,
+                                      kKindBlock,  kKindBlock,  kKindBlock,   kKindTile};
+static const int kMenuG[kSlots] = {4, 16, 8, 16, 8, 16, 32, 8};
 
 static int kind_of_dbin(int db) {
   const uint32_t top = d_bin_top(db);
@@ -691,7 +760,7 @@ static Plan menu_plan() {
 static void heuristic_table(uint8_t* table) {
   for (int b = 0; b < kBuckets; ++b) {
     const int kind = kind_of_dbin(b / kLBins);
-    table[b] = (uint8_t)(kind == kKindWave64 ? 0 : kind == kKindWave512 ? 3 : 6);
+    table[b] = (uint8_t)(kind == kKindWave64 ? 0 : kind == kKindWave512 ? 2 : 5);
   }
 }
 
@@ -720,6 +789,7 @@ void bucket_stats(tribit_ctx* ctx, tribit_graph* g, int sched, BucketStats* out)
     out->sum_k[b] = h[2 * kBuckets + b];
     out->sum_w[b] = h[3 * kBuckets + b];
     out->eligible[b] = h[4 * kBuckets + b];
+    out->sum_t[b] = h[5 * kBuckets + b];
   }
 }
 
@@ -733,18 +803,18 @@ void features(const BucketStats& st, int b, int slot, float* f) {
   f[2] = (float)std::log2(1.0 + st.sum_w[b] / L);
   f[3] = (float)std::log2(1.0 + st.sum_w[b] / K);
   f[4] = (float)(st.eligible[b] / L);
-  f[5] = (float)(std::log2(1.0 + L) / 16.0);
+  f[5] = (float)std::log2((1.0 + st.sum_d[b]) / (1.0 + st.sum_t[b]));  // owner-row elements per 64-column tile
   const int kind = kMenuKind[slot], g = kMenuG[slot];
   f[6] = kind == kKindWave64;
   f[7] = kind == kKindWave512;
-  f[8] = kind == kKindBlock;
+  f[8] = kind >= kKindBlock;
   f[9] = (float)(std::log2((double)g) / 6.0);
   f[10] = g == 4;
   f[11] = g == 8;
   f[12] = g == 16;
   f[13] = g == 32;
   f[14] = (float)(std::log2(1.0 + (st.sum_w[b] + st.sum_k[b]) / L) / 32.0);
-  f[15] = 1.0f;
+  f[15] = kind == kKindTile;
 }
 
 double bucket_work(const BucketStats& st, int b) { return (double)st.sum_w[b] + (double)st.sum_k[b] + 1.0; }
@@ -755,6 +825,7 @@ static void stats_of_profile(const tribit_profile* p, BucketStats* st) {
   memcpy(st->sum_k, p->sum_k, sizeof(st->sum_k));
   memcpy(st->sum_w, p->sum_w, sizeof(st->sum_w));
   memcpy(st->eligible, p->eligible, sizeof(st->eligible));
+  memcpy(st->sum_t, p->sum_t, sizeof(st->sum_t));
 }
 
 void profile_features(const tribit_profile* p, int bucket, int slot, float* f) {
@@ -774,15 +845,15 @@ uint32_t bucket_max_outdeg(int bucket) { return d_bin_top(bucket / kLBins); }
 // LEARNED: per bucket with lists, the feasible menu slot of lowest predicted
 // cost 2^model(features) * work. Without a model, or when a prediction is not
 // finite, the heuristic's slot.
-static void learned_table(tribit_ctx* ctx, const BucketStats& st, uint8_t* table) {
+static void learned_table(tribit_ctx* ctx, const BucketStats& st, int layout, int flags, uint8_t* table) {
   heuristic_table(table);
-  if (!ctx->model) return;
+  if (!ctx->model || layout != TRIBIT_SCHED_WEDGE) return;  // the model is trained on wedge-schedule timings
   std::vector<float> X;
   std::vector<std::pair<int, int>> ids;
   for (int b = 0; b < kBuckets; ++b) {
     if (!st.lists[b]) continue;
     for (int s = 0; s < kSlots; ++s) {
-      if (!slot_feasible(s, b)) continue;
+      if (!slot_feasible(s, b) || ((flags & TRIBIT_COUNT_NO_TILES) && kMenuKind[s] == kKindTile)) continue;
       float f[kFeatures];
       features(st, b, s, f);
       X.insert(X.end(), f, f + kFeatures);
@@ -855,41 +926,53 @@ static void launch_wave(int gw, uint32_t ntasks, int cus, hipStream_t s, tribit_
   }
 }
 
-template <int LOGS, int G, int U, bool BUF>
+template <int LOGS, int G, int U, bool BUF, bool TILES>
 static void launch_block_t(uint32_t ntasks, int cus, hipStream_t s, tribit_graph* g, const uint2* rng,
-                           const uint4* tasks, uint32_t* queue, unsigned long long* result, int attempts,
-                           unsigned long long* tlog, uint64_t* blocks) {
+                           const TileSrc& ts, const uint4* tasks, uint32_t* queue, unsigned long long* result,
+                           int attempts, unsigned long long* tlog, uint64_t* blocks) {
   int per_cu = 0;
-  TB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_count_block<LOGS, G, U, BUF>, kBlockThreads, 0));
+  TB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_count_block<LOGS, G, U, BUF, TILES>, kBlockThreads,
+                                                      0));
   if (per_cu < 1) per_cu = 1;
   const unsigned grid = (unsigned)std::min<uint64_t>(ntasks, (uint64_t)cus * per_cu);
-  k_count_block<LOGS, G, U, BUF><<<grid, kBlockThreads, 0, s>>>(g->row_ptr, g->col, g->m, rng, tasks, ntasks, queue,
-                                                               result, attempts & ~kForceGlobal, tlog);
+  k_count_block<LOGS, G, U, BUF, TILES><<<grid, kBlockThreads, 0, s>>>(
+      g->row_ptr, g->col, g->m, rng, ts, tasks, ntasks, queue, result, attempts & ~kForceGlobal, tlog);
   *blocks += grid;
 }
 
 template <int LOGS, int G>
 static void launch_block_g(uint32_t ntasks, int cus, hipStream_t s, tribit_graph* g, const uint2* rng,
-                           const uint4* tasks, uint32_t* queue, unsigned long long* result, int attempts,
-                           unsigned long long* tlog, uint64_t* blocks) {
-  if (g->m * 4 < 0xFFFFFFFFull && !(attempts & kForceGlobal))
-    launch_block_t<LOGS, G, kU, true>(ntasks, cus, s, g, rng, tasks, queue, result, attempts, tlog, blocks);
-  else
-    launch_block_t<LOGS, G, kU, false>(ntasks, cus, s, g, rng, tasks, queue, result, attempts, tlog, blocks);
+                           const TileSrc& ts, const uint4* tasks, uint32_t* queue, unsigned long long* result,
+                           int attempts, unsigned long long* tlog, uint64_t* blocks) {
+  const bool buf = g->m * 4 < 0xFFFFFFFFull && !(attempts & kForceGlobal);
+  if (ts.trng) {
+    if (buf)
+      launch_block_t<LOGS, G, kU, true, true>(ntasks, cus, s, g, rng, ts, tasks, queue, result, attempts, tlog, blocks);
+    else
+      launch_block_t<LOGS, G, kU, false, true>(ntasks, cus, s, g, rng, ts, tasks, queue, result, attempts, tlog,
+                                               blocks);
+  } else if (buf) {
+    launch_block_t<LOGS, G, kU, true, false>(ntasks, cus, s, g, rng, ts, tasks, queue, result, attempts, tlog, blocks);
+  } else {
+    launch_block_t<LOGS, G, kU, false, false>(ntasks, cus, s, g, rng, ts, tasks, queue, result, attempts, tlog,
+                                              blocks);
+  }
 }
 
 template <int LOGS>
 static void launch_block(int gw, uint32_t ntasks, int cus, hipStream_t s, tribit_graph* g, const uint2* rng,
-                         const uint4* tasks, uint32_t* queue, unsigned long long* result, int attempts,
-                         unsigned long long* tlog, uint64_t* blocks) {
+                         const TileSrc& ts, const uint4* tasks, uint32_t* queue, unsigned long long* result,
+                         int attempts, unsigned long long* tlog, uint64_t* blocks) {
   if (!ntasks) return;
+#define TB_BLK(GW) launch_block_g<LOGS, GW>(ntasks, cus, s, g, rng, ts, tasks, queue, result, attempts, tlog, blocks)
   switch (gw) {
-    case 4: launch_block_g<LOGS, 4>(ntasks, cus, s, g, rng, tasks, queue, result, attempts, tlog, blocks); break;
-    case 8: launch_block_g<LOGS, 8>(ntasks, cus, s, g, rng, tasks, queue, result, attempts, tlog, blocks); break;
-    case 32: launch_block_g<LOGS, 32>(ntasks, cus, s, g, rng, tasks, queue, result, attempts, tlog, blocks); break;
-    case 64: launch_block_g<LOGS, 64>(ntasks, cus, s, g, rng, tasks, queue, result, attempts, tlog, blocks); break;
-    default: launch_block_g<LOGS, 16>(ntasks, cus, s, g, rng, tasks, queue, result, attempts, tlog, blocks); break;
+    case 4: TB_BLK(4); break;
+    case 8: TB_BLK(8); break;
+    case 32: TB_BLK(32); break;
+    case 64: TB_BLK(64); break;
+    default: TB_BLK(16); break;
   }
+#undef TB_BLK
 }
 
 static float elapsed(hipEvent_t a, hipEvent_t b) {
@@ -911,13 +994,27 @@ static uint64_t count_once(tribit_ctx* ctx, tribit_graph* g, int sched, int flag
     // the model's choice is part of the scheduler's time
     BucketStats st;
     bucket_stats(ctx, g, sched, &st);
-    learned_table(ctx, st, plan.table);
+    learned_table(ctx, st, sched, flags, plan.table);
   }
   // scratch: [0] count, [1] fallback tasks, [4..7] eight u32 queue counters
   DevBuf<unsigned long long> scratch(ctx, 8);
   k_zero_u64<<<1, 64, 0, s>>>(scratch.p, 8);
   unsigned long long* d_result = scratch.p;
   uint32_t* d_queue = reinterpret_cast<uint32_t*>(scratch.p + 4);
+  // test hooks / opt-outs: every block launch streams tiles, or none does
+  for (int i = 0; i < plan.nslots; ++i) {
+    if ((flags & TRIBIT_COUNT_FORCE_TILES) && plan.kind[i] == kKindBlock) plan.kind[i] = kKindTile;
+    if ((flags & TRIBIT_COUNT_NO_TILES) && plan.kind[i] == kKindTile) plan.kind[i] = kKindBlock;
+  }
+  TileSrc ts{nullptr, nullptr, nullptr};
+  bool any_tiles = false;
+  for (int i = 0; i < plan.nslots; ++i) any_tiles |= plan.kind[i] == kKindTile;
+  if (any_tiles && g->m) {
+    build_tiles(ctx, g, sched);  // once per graph and layout
+    ts.blk = g->tile_blk;
+    ts.mask = reinterpret_cast<const unsigned long long*>(g->tile_mask);
+    ts.trng = sched == TRIBIT_SCHED_PLAIN ? g->trng_all : sched == TRIBIT_SCHED_WEDGE ? g->trng_sfx : g->trng_dir;
+  }
   DevBuf<uint8_t> dtable(ctx, kBuckets);
   TB_HIP(hipMemcpyAsync(dtable.p, plan.table, kBuckets, hipMemcpyHostToDevice, s));
   PlanDev pd;
@@ -965,8 +1062,11 @@ static uint64_t count_once(tribit_ctx* ctx, tribit_graph* g, int sched, int flag
       launch_wave<kC0Logs, kC0K>(plan.g[i], tot.v[i], cus, s, g, rng, tk, d_queue + i, d_result, att, tl, &blocks);
     else if (plan.kind[i] == kKindWave512)
       launch_wave<kC1Logs, kC1K>(plan.g[i], tot.v[i], cus, s, g, rng, tk, d_queue + i, d_result, att, tl, &blocks);
+    else if (plan.kind[i] == kKindTile && ts.trng)
+      launch_block<kC2Logs>(plan.g[i], tot.v[i], cus, s, g, rng, ts, tk, d_queue + i, d_result, att, tl, &blocks);
     else
-      launch_block<kC2Logs>(plan.g[i], tot.v[i], cus, s, g, rng, tk, d_queue + i, d_result, att, tl, &blocks);
+      launch_block<kC2Logs>(plan.g[i], tot.v[i], cus, s, g, rng, TileSrc{nullptr, nullptr, nullptr}, tk,
+                            d_queue + i, d_result, att, tl, &blocks);
     TB_HIP(hipEventRecord(ctx->ev[4 + i], s));
   }
   TB_HIP(hipGetLastError());
@@ -1076,6 +1176,7 @@ void count_profile(tribit_ctx* ctx, tribit_graph* g, int layout, const uint8_t* 
     out->sum_k[bk] = st.sum_k[bk];
     out->sum_w[bk] = st.sum_w[bk];
     out->eligible[bk] = st.eligible[bk];
+    out->sum_t[bk] = st.sum_t[bk];
     out->slot[bk] = plan.table[bk];
   }
   int khz = 0;
@@ -1085,8 +1186,9 @@ void count_profile(tribit_ctx* ctx, tribit_graph* g, int layout, const uint8_t* 
 
 void learned_plan(tribit_ctx* ctx, tribit_graph* g, uint8_t* table) {
   BucketStats st;
-  bucket_stats(ctx, g, g->rng_sfx ? TRIBIT_SCHED_WEDGE : TRIBIT_SCHED_DIRECTED, &st);
-  learned_table(ctx, st, table);
+  const int layout = g->rng_sfx ? TRIBIT_SCHED_WEDGE : TRIBIT_SCHED_DIRECTED;
+  bucket_stats(ctx, g, layout, &st);
+  learned_table(ctx, st, layout, 0, table);
 }
 
 // Per-slot work of a schedule's plan (LEARNED: the table the model picks).
@@ -1097,7 +1199,7 @@ void variant_stats(tribit_ctx* ctx, tribit_graph* g, int sched, tribit_variant_w
   if (sched == TRIBIT_SCHED_LEARNED) layout = g->rng_sfx ? TRIBIT_SCHED_WEDGE : TRIBIT_SCHED_DIRECTED;
   BucketStats st;
   bucket_stats(ctx, g, layout, &st);
-  if (sched == TRIBIT_SCHED_LEARNED) learned_table(ctx, st, plan.table);
+  if (sched == TRIBIT_SCHED_LEARNED) learned_table(ctx, st, layout, 0, plan.table);
   for (int b = 0; b < kBuckets; ++b) {
     const int s = plan.table[b];
     out->rows[s] += st.lists[b];

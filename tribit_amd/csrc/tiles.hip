@@ -7,10 +7,8 @@
 // AND + popcount.
 //
 // Built from the oriented CSR: per-row tile counts, a scan (tile_ptr), a
-// per-row fill, then the tile ranges of every streamed list in both range
-// layouts (trng_all in CSR order, trng_dir in rng_dir order) and, per owner
-// list, the total tiles streamed (the density heuristic compares it with
-// the elements streamed).
+// per-row fill that also records the tile of every element, then, per range
+// layout on first use, the tile range of every streamed element range.
 #include <cstring>
 #include <rocprim/device/device_reduce.hpp>
 #include <rocprim/device/device_scan.hpp>
@@ -41,7 +39,7 @@ __global__ void k_tile_count(const uint32_t* __restrict__ row_ptr, const uint32_
 
 __global__ void k_tile_fill(const uint32_t* __restrict__ row_ptr, const uint32_t* __restrict__ col, uint64_t n,
                             const uint32_t* __restrict__ tile_ptr, uint32_t* __restrict__ blk,
-                            unsigned long long* __restrict__ mask) {
+                            unsigned long long* __restrict__ mask, uint32_t* __restrict__ tidx) {
   const uint64_t u = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (u >= n) return;
   uint32_t t = tile_ptr[u], prev = 0xFFFFFFFFu;
@@ -58,6 +56,7 @@ __global__ void k_tile_fill(const uint32_t* __restrict__ row_ptr, const uint32_t
       acc = 0;
     }
     acc |= 1ull << (c & 63);
+    tidx[e] = t;  // tile holding element e
   }
   if (prev != 0xFFFFFFFFu) {
     blk[t] = prev;
@@ -65,27 +64,17 @@ __global__ void k_tile_fill(const uint32_t* __restrict__ row_ptr, const uint32_t
   }
 }
 
-// Tile range of the list each range entry streams: partner ids from `pid`
-// (or, when pid is null, the CSR column itself).
-__global__ void k_tile_ranges(const uint32_t* __restrict__ tile_ptr, const uint32_t* __restrict__ pid, uint64_t m,
+// Tile range of an element range [b, e) of one row: the tiles holding its
+// first and last element. Bits of the first and last tile outside [b, e)
+// belong to the same row; the schedules that cut ranges (the wedge suffix
+// after the owner's id, id-range chunks) cut at ids the owner's set does not
+// hold, so those bits never match.
+__global__ void k_tile_ranges(const uint2* __restrict__ rng, uint64_t m, const uint32_t* __restrict__ tidx,
                               uint2* __restrict__ trng) {
   const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= m) return;
-  const uint32_t p = pid[k];
-  trng[k] = make_uint2(tile_ptr[p], tile_ptr[p + 1]);
-}
-
-// Per owner list [ptr[x], ptr[x+1]) of a range layout: tiles streamed.
-__global__ void k_tile_work(const uint32_t* __restrict__ ptr, const uint2* __restrict__ trng, uint64_t n,
-                            uint64_t* __restrict__ tw) {
-  const uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (x >= n) return;
-  uint64_t w = 0;
-  for (uint32_t k = ptr[x]; k < ptr[x + 1]; ++k) {
-    const uint2 r = trng[k];
-    w += r.y - r.x;
-  }
-  tw[x] = w;
+  const uint2 r = rng[k];
+  trng[k] = r.x < r.y ? make_uint2(tidx[r.x], tidx[r.y - 1] + 1) : make_uint2(0u, 0u);
 }
 
 // Total tiles of the graph (cheap: one pass over col), for graph info and the
@@ -106,49 +95,50 @@ uint64_t count_tiles(tribit_ctx* ctx, tribit_graph* g) {
   return h;
 }
 
-void build_tiles(tribit_ctx* ctx, tribit_graph* g) {
-  if (g->tile_ptr) return;
+// Builds the tiles (once) and the tile ranges of range layout `layout`
+// (PLAIN: rng_all, DIRECTED: rng_dir, WEDGE: rng_sfx) on first use.
+void build_tiles(tribit_ctx* ctx, tribit_graph* g, int layout) {
   hipStream_t s = ctx->stream;
   const uint64_t n = g->n, m = g->m;
-  DevBuf<uint32_t> tile_ptr(ctx, n + 1);
-  {
-    DevBuf<uint32_t> tcnt(ctx, n + 1);
-    k_tile_count<<<grid_for(n + 1, kTBlock), kTBlock, 0, s>>>(g->row_ptr, g->col, n, tcnt.p);
-    size_t tb = 0;
-    TB_HIP(rocprim::exclusive_scan(nullptr, tb, tcnt.p, tile_ptr.p, 0u, (size_t)(n + 1), rocprim::plus<uint32_t>(),
-                                   s));
-    DevBuf<uint8_t> tmp(ctx, tb ? tb : 8);
-    TB_HIP(rocprim::exclusive_scan(tmp.p, tb, tcnt.p, tile_ptr.p, 0u, (size_t)(n + 1), rocprim::plus<uint32_t>(),
-                                   s));
+  if (!g->tile_ptr) {
+    DevBuf<uint32_t> tile_ptr(ctx, n + 1);
+    {
+      DevBuf<uint32_t> tcnt(ctx, n + 1);
+      k_tile_count<<<grid_for(n + 1, kTBlock), kTBlock, 0, s>>>(g->row_ptr, g->col, n, tcnt.p);
+      size_t tb = 0;
+      TB_HIP(rocprim::exclusive_scan(nullptr, tb, tcnt.p, tile_ptr.p, 0u, (size_t)(n + 1),
+                                     rocprim::plus<uint32_t>(), s));
+      DevBuf<uint8_t> tmp(ctx, tb ? tb : 8);
+      TB_HIP(rocprim::exclusive_scan(tmp.p, tb, tcnt.p, tile_ptr.p, 0u, (size_t)(n + 1),
+                                     rocprim::plus<uint32_t>(), s));
+    }
+    uint32_t W = 0;
+    TB_HIP(hipMemcpyAsync(&W, tile_ptr.p + n, 4, hipMemcpyDeviceToHost, s));
+    ctx->sync();
+    DevBuf<uint32_t> blk(ctx, W ? W : 1), tidx(ctx, m ? m : 1);
+    DevBuf<uint64_t> mask(ctx, W ? W : 1);
+    k_tile_fill<<<grid_for(n, kTBlock), kTBlock, 0, s>>>(g->row_ptr, g->col, n, tile_ptr.p, blk.p,
+                                                       reinterpret_cast<unsigned long long*>(mask.p), tidx.p);
+    TB_HIP(hipGetLastError());
+    ctx->sync();
+    g->ntiles = W;
+    g->tile_ptr = tile_ptr.release();
+    g->tile_blk = blk.release();
+    g->tile_mask = mask.release();
+    g->tile_idx = tidx.release();
+    g->bytes += 4 * (n + 1) + 12ull * W + 4 * m;
   }
-  uint32_t W = 0;
-  TB_HIP(hipMemcpyAsync(&W, tile_ptr.p + n, 4, hipMemcpyDeviceToHost, s));
-  ctx->sync();
-  DevBuf<uint32_t> blk(ctx, W ? W : 1);
-  DevBuf<uint64_t> mask(ctx, W ? W : 1);
-  k_tile_fill<<<grid_for(n, kTBlock), kTBlock, 0, s>>>(g->row_ptr, g->col, n, tile_ptr.p, blk.p,
-                                                     reinterpret_cast<unsigned long long*>(mask.p));
-  DevBuf<uint2> trng_all(ctx, m ? m : 1), trng_dir(ctx, m ? m : 1);
-  DevBuf<uint64_t> tw_all(ctx, n), tw_fw(ctx, n), tw_rv(ctx, n);
-  if (m) {
-    k_tile_ranges<<<grid_for(m, kTBlock), kTBlock, 0, s>>>(tile_ptr.p, g->col, m, trng_all.p);
-    k_tile_ranges<<<grid_for(m, kTBlock), kTBlock, 0, s>>>(tile_ptr.p, g->pid_dir, m, trng_dir.p);
-  }
-  k_tile_work<<<grid_for(n, kTBlock), kTBlock, 0, s>>>(g->row_ptr, trng_all.p, n, tw_all.p);
-  k_tile_work<<<grid_for(n, kTBlock), kTBlock, 0, s>>>(g->fw_ptr, trng_dir.p, n, tw_fw.p);
-  k_tile_work<<<grid_for(n, kTBlock), kTBlock, 0, s>>>(g->rv_ptr, trng_dir.p, n, tw_rv.p);
+  uint2** dst = layout == TRIBIT_SCHED_PLAIN ? &g->trng_all : layout == TRIBIT_SCHED_WEDGE ? &g->trng_sfx
+                                                                                             : &g->trng_dir;
+  const uint2* src = layout == TRIBIT_SCHED_PLAIN ? g->rng_all : layout == TRIBIT_SCHED_WEDGE ? g->rng_sfx
+                                                                                               : g->rng_dir;
+  if (*dst || !src) return;
+  DevBuf<uint2> tr(ctx, m ? m : 1);
+  if (m) k_tile_ranges<<<grid_for(m, kTBlock), kTBlock, 0, s>>>(src, m, g->tile_idx, tr.p);
   TB_HIP(hipGetLastError());
   ctx->sync();
-  g->ntiles = W;
-  g->tile_ptr = tile_ptr.release();
-  g->tile_blk = blk.release();
-  g->tile_mask = mask.release();
-  g->trng_all = trng_all.release();
-  g->trng_dir = trng_dir.release();
-  g->tw_all = tw_all.release();
-  g->tw_fw = tw_fw.release();
-  g->tw_rv = tw_rv.release();
-  g->bytes += 4 * (n + 1) + 12ull * W + 16 * m + 24 * n;
+  *dst = tr.release();
+  g->bytes += 8 * (m ? m : 1);
 }
 
 void free_tiles(tribit_graph* g) {
@@ -156,14 +146,14 @@ void free_tiles(tribit_graph* g) {
   c->free(g->tile_ptr);
   c->free(g->tile_blk);
   c->free(g->tile_mask);
+  c->free(g->tile_idx);
   c->free(g->trng_all);
   c->free(g->trng_dir);
-  c->free(g->tw_all);
-  c->free(g->tw_fw);
-  c->free(g->tw_rv);
-  g->tile_ptr = g->tile_blk = nullptr;
-  g->tile_mask = g->tw_all = g->tw_fw = g->tw_rv = nullptr;
-  g->trng_all = g->trng_dir = nullptr;
+  c->free(g->trng_sfx);
+  g->tile_ptr = g->tile_blk = g->tile_idx = nullptr;
+  g->tile_mask = nullptr;
+  g->trng_all = g->trng_dir = g->trng_sfx = nullptr;
+  g->ntiles = 0;
 }
 
 }  // namespace tb

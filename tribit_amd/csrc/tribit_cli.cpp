@@ -14,6 +14,8 @@
 //   --sched plain|directed|wedge|learned   scheduler choice (default learned)
 //   --repeats K             timed counting repetitions (median reported)
 //   --json PATH             write a JSON side record of the measured pass
+//   --model PATH            cost model for --sched learned (default: the
+//                           cost_model.f32 next to the executable, if present)
 // Input or device errors print a message and exit 2 (input) or 3 (device);
 // the reference aborts on an uncaught exception there (tribit.cu:230, 241-245).
 #include <getopt.h>
@@ -22,6 +24,9 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <vector>
+
+#include <unistd.h>
 
 #include "../../include/tribit.h"
 
@@ -29,7 +34,7 @@ namespace {
 
 struct Config {
   std::string input_file;
-  std::string rmat, er, json;
+  std::string rmat, er, json, model;
   int sched = TRIBIT_SCHED_LEARNED;
   int repeats = 1;
 };
@@ -43,7 +48,8 @@ std::string get_usage(const char* prog) {
          "  --er N,M,SEED            Generate an Erdos-Renyi G(n,M) graph instead\n"
          "  --sched plain|directed|wedge|learned  Scheduler (default learned)\n"
          "  --repeats K              Timed counting repetitions (default 1)\n"
-         "  --json PATH              Write a JSON record of the measured pass\n";
+         "  --json PATH              Write a JSON record of the measured pass\n"
+         "  --model PATH             Cost model of the learned scheduler (float32 parameters)\n";
 }
 
 [[noreturn]] void usage_exit(const char* prog, const char* msg) {
@@ -60,6 +66,7 @@ Config program_options(int argc, char** argv) {
                                {"sched", required_argument, nullptr, 3},
                                {"repeats", required_argument, nullptr, 4},
                                {"json", required_argument, nullptr, 5},
+                               {"model", required_argument, nullptr, 6},
                                {nullptr, 0, nullptr, 0}};
   int opt;
   while ((opt = getopt_long(argc, argv, "i:", lo, nullptr)) != -1) {
@@ -84,6 +91,7 @@ Config program_options(int argc, char** argv) {
         if (c.repeats < 1) usage_exit(argv[0], "Error: --repeats must be >= 1\n");
         break;
       case 5: c.json = optarg; break;
+      case 6: c.model = optarg; break;
       default: usage_exit(argv[0], "Error: unknown option or missing argument\n");
     }
   }
@@ -125,6 +133,37 @@ tribit_graph* build_graph(tribit_ctx* ctx, const Input& in) {
   return g;
 }
 
+// Loads float32 parameters of the cost model and hands it to the context.
+// Returns the model (freed by the caller) or nullptr when no file is given or
+// found (the learned scheduler then keeps its density-threshold heuristic).
+tribit_mlp* load_model(tribit_ctx* ctx, std::string path, bool required) {
+  if (path.empty()) {
+    char exe[4096];
+    const ssize_t k = readlink("/proc/self/exe", exe, sizeof(exe) - 1);
+    if (k <= 0) return nullptr;
+    exe[k] = 0;
+    std::string dir(exe);
+    dir = dir.substr(0, dir.find_last_of('/'));
+    path = dir + "/cost_model.f32";
+  }
+  FILE* f = fopen(path.c_str(), "rb");
+  if (!f) {
+    if (required) die(2, path.c_str(), TRIBIT_ERR_IO);
+    return nullptr;
+  }
+  tribit_mlp* m = nullptr;
+  int st = tribit_mlp_create(ctx, TRIBIT_FEATURES, 1, &m);
+  if (st) die(3, "cost model", st);
+  std::vector<float> p(tribit_mlp_num_params(m));
+  const size_t got = fread(p.data(), sizeof(float), p.size() + 1, f);
+  fclose(f);
+  if (got != p.size()) die(2, (path + " (parameter count)").c_str(), TRIBIT_ERR_FORMAT);
+  st = tribit_mlp_set_params(m, p.data());
+  if (!st) st = tribit_set_cost_model(ctx, m);
+  if (st) die(3, "cost model", st);
+  return m;
+}
+
 }  // namespace
 
 static const char* sched_name(int s) {
@@ -163,6 +202,7 @@ int main(int argc, char** argv) {
   tribit_ctx* ctx = nullptr;
   st = tribit_init(0, &ctx);
   if (st) die(3, "device init", st);
+  tribit_mlp* model = cfg.sched == TRIBIT_SCHED_LEARNED ? load_model(ctx, cfg.model, !cfg.model.empty()) : nullptr;
 
   tribit_count_opts opts;
   memset(&opts, 0, sizeof(opts));
@@ -204,6 +244,8 @@ int main(int argc, char** argv) {
   printf("Edges (undirected) : %llu\n", (unsigned long long)gi.m);
   printf("Wedge checks (W+)  : %llu\n", (unsigned long long)gi.wplus);
   printf("Scheduler (ms)     : %.3f\n", ci.ms_sched);
+  printf("Schedule           : %s%s\n", sched_name(cfg.sched),
+         cfg.sched == TRIBIT_SCHED_LEARNED ? (model ? " (cost model)" : " (heuristic, no cost model)") : "");
   printf("GTEPS              : %.3f\n", gteps);
   printf("Algorithmic GB/s   : %.1f\n", ci.ms_count > 0 ? balg / (ci.ms_count * 1e-3) / 1e9 : 0.0);
   uint64_t peak = 0;
@@ -227,6 +269,7 @@ int main(int argc, char** argv) {
   }
   tribit_graph_free(g);
   tribit_free_host(in.host_keys);
+  if (model) tribit_mlp_free(model);
   tribit_shutdown(ctx);
   return 0;
 }
