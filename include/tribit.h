@@ -48,6 +48,7 @@ typedef enum {
 typedef struct tribit_ctx tribit_ctx;
 typedef struct tribit_graph tribit_graph;
 typedef struct tribit_mlp tribit_mlp;
+typedef struct tribit_batch tribit_batch;
 
 /* Count-ready graph description (filled by tribit_graph_get_info). */
 typedef struct {
@@ -124,6 +125,7 @@ int tribit_set_alloc_cap(tribit_ctx* ctx, uint64_t bytes);
 /* Context options (tribit_set_option). */
 #define TRIBIT_OPT_RELABEL 1 /* 1 (default): preprocessing renumbers vertices by (degree, id)
                                 rank; 0: input ids kept. Counts are the same either way. */
+#define TRIBIT_OPT_BATCH_PART_KEYS 2 /* input keys per batch part (default 2^30) */
 int tribit_set_option(tribit_ctx* ctx, int option, int64_t value);
 /* Peak bytes held from the memory pool since tribit_init. */
 int tribit_pool_peak(tribit_ctx* ctx, uint64_t* bytes);
@@ -234,6 +236,48 @@ int tribit_learned_plan(tribit_ctx* ctx, tribit_graph* g, uint8_t* table);
 int tribit_count_keys(tribit_ctx* ctx, const uint64_t* keys, uint64_t nkeys, uint64_t n,
                       const tribit_count_opts* opts, uint64_t* triangles, double* ms_preprocess,
                       tribit_count_info* info);
+
+/* ---- batches: many graphs in one counting launch (no reference counterpart)
+ * Graphs are packed into parts (disjoint unions, vertex ids offset, each part
+ * under 2^30 input keys), preprocessed once per part; the counting launches
+ * of all parts are captured into one hipGraph and replayed, and every task
+ * adds its count to its owner's graph. */
+typedef struct {
+  double ms_count;         /* replay of the captured counting launches (hipEvents), median of repeats */
+  double ms_sched;         /* building the task lists (first count of a configuration only) */
+  double ms_capture;       /* stream capture + instantiation (first count of a configuration only) */
+  uint64_t triangles;      /* sum over the batch */
+  uint64_t tasks;
+  uint32_t parts;
+  uint32_t launches;       /* kernel launches in the captured graph */
+  int32_t repeats_agree;
+  int32_t nslots;
+  int32_t group_width[8];  /* lanes per streamed list per slot */
+  int32_t waves[8];        /* workgroups per CU per slot (0: occupancy limit) */
+} tribit_batch_count_info;
+/* Graph j has keys[key_offsets[j] .. key_offsets[j+1]) ((min<<32)|max of local
+ * ids in [0, nverts[j]); duplicates and UINT64_MAX sentinels allowed). */
+int tribit_batch_from_host(tribit_ctx* ctx, const uint64_t* keys, const uint64_t* key_offsets, const uint64_t* nverts,
+                           uint32_t ngraphs, tribit_batch** out);
+/* Graphs first .. first+ngraphs-1 of a Chung-Lu batch: graph i has the
+ * parameters of tribit_chung_lu_batch_params(seed, i) and generator seed
+ * (seed << 32) + i. */
+int tribit_batch_gen_chung_lu(tribit_ctx* ctx, uint64_t seed, uint64_t first, uint32_t ngraphs, tribit_batch** out);
+/* Sizes: graphs, parts, total vertices and edges, device preprocessing ms. */
+int tribit_batch_info(const tribit_batch* b, uint32_t* ngraphs, uint32_t* nparts, uint64_t* total_n,
+                      uint64_t* total_m, double* ms_preprocess);
+/* Per graph vertices and simple undirected edges (either may be NULL). */
+int tribit_batch_graph_sizes(const tribit_batch* b, uint64_t* n, uint64_t* m);
+/* Exact per-graph counts (counts[ngraphs]); opts->sched PLAIN, DIRECTED, WEDGE
+ * or LEARNED, opts->repeats replays of the captured launches. */
+int tribit_batch_count(tribit_ctx* ctx, tribit_batch* b, const tribit_count_opts* opts, uint64_t* counts,
+                       tribit_batch_count_info* info);
+/* Autotuner: coordinate search over lane-group width and workgroups per CU
+ * of every launch slot, timing replays of the captured graph; the best
+ * configuration stays with the batch for later counts and is returned. */
+int tribit_batch_autotune(tribit_ctx* ctx, tribit_batch* b, const tribit_count_opts* opts,
+                          tribit_batch_count_info* best);
+int tribit_batch_free(tribit_batch* b);
 
 /* ---- cost model of the learned scheduler (no reference counterpart) -------
  * A 3-layer perceptron, F inputs -> 64 -> 64 -> 1 (ReLU), trained in FP32 by

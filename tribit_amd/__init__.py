@@ -58,6 +58,7 @@ NO_TILES = 4
 FORCE_TILES = 8
 NO_BITMAP = 16
 OPT_RELABEL = 1
+OPT_BATCH_PART_KEYS = 2
 
 
 class CountInfo(ctypes.Structure):
@@ -71,6 +72,15 @@ class CountInfo(ctypes.Structure):
 
 
 COST_MODEL_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cost_model.f32")
+class BatchCountInfo(ctypes.Structure):
+    _fields_ = [
+        ("ms_count", ctypes.c_double), ("ms_sched", ctypes.c_double), ("ms_capture", ctypes.c_double),
+        ("triangles", ctypes.c_uint64), ("tasks", ctypes.c_uint64), ("parts", ctypes.c_uint32),
+        ("launches", ctypes.c_uint32), ("repeats_agree", ctypes.c_int32), ("nslots", ctypes.c_int32),
+        ("group_width", ctypes.c_int32 * 8), ("waves", ctypes.c_int32 * 8),
+    ]
+
+
 BUCKETS = 800
 MENU_SLOTS = 8
 FEATURES = 16
@@ -149,6 +159,16 @@ def load_library(path: str | None = None) -> ctypes.CDLL:
         "tribit_bucket_max_outdeg": (ctypes.c_uint32, [ctypes.c_int]),
         "tribit_set_cost_model": (ctypes.c_int, [_P, _P]),
         "tribit_learned_plan": (ctypes.c_int, [_P, _P, _P]),
+        "tribit_batch_from_host": (ctypes.c_int, [_P, _P, _P, _P, ctypes.c_uint32, ctypes.POINTER(_P)]),
+        "tribit_batch_gen_chung_lu": (ctypes.c_int, [_P, _U64, _U64, ctypes.c_uint32, ctypes.POINTER(_P)]),
+        "tribit_batch_info": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32),
+                                             _U64P, _U64P, ctypes.POINTER(ctypes.c_double)]),
+        "tribit_batch_graph_sizes": (ctypes.c_int, [_P, _P, _P]),
+        "tribit_batch_count": (ctypes.c_int, [_P, _P, ctypes.POINTER(CountOpts), _P,
+                                              ctypes.POINTER(BatchCountInfo)]),
+        "tribit_batch_autotune": (ctypes.c_int, [_P, _P, ctypes.POINTER(CountOpts),
+                                                 ctypes.POINTER(BatchCountInfo)]),
+        "tribit_batch_free": (ctypes.c_int, [_P]),
         "tribit_mlp_create": (ctypes.c_int, [_P, ctypes.c_int, _U64, ctypes.POINTER(_P)]),
         "tribit_mlp_free": (ctypes.c_int, [_P]),
         "tribit_mlp_num_params": (_U64, [_P]),
@@ -295,6 +315,72 @@ class Mlp:
         self.free()
 
 
+def _info_dict(ci) -> dict:
+    d = {f: getattr(ci, f) for f, _ in ci._fields_}
+    n = ci.nslots
+    d["group_width"] = list(ci.group_width)[:n]
+    d["waves"] = list(ci.waves)[:n]
+    d["repeats_agree"] = bool(ci.repeats_agree)
+    return d
+
+
+class Batch:
+    """Many graphs resident in HBM, counted by one captured hipGraph launch."""
+
+    def __init__(self, ctx: "Context", handle):
+        self.ctx = ctx
+        self.h = handle
+
+    def info(self) -> dict:
+        ng, npart = ctypes.c_uint32(), ctypes.c_uint32()
+        tn, tm, ms = _U64(), _U64(), ctypes.c_double()
+        _check(self.ctx.lib.tribit_batch_info(self.h, ctypes.byref(ng), ctypes.byref(npart), ctypes.byref(tn),
+                                              ctypes.byref(tm), ctypes.byref(ms)), "batch_info")
+        return {"graphs": ng.value, "parts": npart.value, "n": tn.value, "m": tm.value, "ms_preprocess": ms.value}
+
+    def sizes(self):
+        ng = self.info()["graphs"]
+        n = np.zeros(ng, np.uint64)
+        m = np.zeros(ng, np.uint64)
+        _check(self.ctx.lib.tribit_batch_graph_sizes(self.h, n.ctypes.data, m.ctypes.data), "batch_graph_sizes")
+        return n, m
+
+    def _opts(self, sched, repeats, flags):
+        o = CountOpts()
+        o.sched = SCHED[sched]
+        o.repeats = repeats
+        o.flags = flags
+        return o
+
+    def count(self, sched: str = "learned", repeats: int = 1, flags: int = 0):
+        """(per-graph counts uint64[graphs], info dict)."""
+        ng = self.info()["graphs"]
+        out = np.zeros(ng, np.uint64)
+        o = self._opts(sched, repeats, flags)
+        ci = BatchCountInfo()
+        _check(self.ctx.lib.tribit_batch_count(self.ctx.h, self.h, ctypes.byref(o), out.ctypes.data,
+                                               ctypes.byref(ci)), "batch_count")
+        return out, _info_dict(ci)
+
+    def autotune(self, sched: str = "learned", repeats: int = 3, flags: int = 0) -> dict:
+        o = self._opts(sched, repeats, flags)
+        ci = BatchCountInfo()
+        _check(self.ctx.lib.tribit_batch_autotune(self.ctx.h, self.h, ctypes.byref(o), ctypes.byref(ci)),
+               "batch_autotune")
+        return _info_dict(ci)
+
+    def free(self) -> None:
+        if self.h:
+            self.ctx.lib.tribit_batch_free(self.h)
+            self.h = _P()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.free()
+
+
 class Graph:
     """Count-ready graph resident in HBM (oriented CSR + per-row wedge counts)."""
 
@@ -432,6 +518,27 @@ class Context:
             old.free()
         self._owned_model = m
         return True
+
+    def batch_from_keys(self, keys_list, n_list) -> Batch:
+        """Batch of graphs given as host key arrays ((min<<32)|max, local ids) and vertex counts."""
+        sizes = np.array([np.asarray(k).size for k in keys_list], np.uint64)
+        off = np.zeros(len(keys_list) + 1, np.uint64)
+        off[1:] = np.cumsum(sizes)
+        keys = np.ascontiguousarray(np.concatenate([np.asarray(k, np.uint64) for k in keys_list])
+                                    if len(keys_list) else np.zeros(0, np.uint64))
+        nv = np.ascontiguousarray(n_list, np.uint64)
+        h = _P()
+        _check(self.lib.tribit_batch_from_host(self.h, keys.ctypes.data if keys.size else None, off.ctypes.data,
+                                               nv.ctypes.data, len(keys_list), ctypes.byref(h)), "batch_from_host")
+        return Batch(self, h)
+
+    def batch_gen_chung_lu(self, seed: int, first: int, count: int) -> Batch:
+        h = _P()
+        _check(self.lib.tribit_batch_gen_chung_lu(self.h, seed, first, count, ctypes.byref(h)), "batch_gen_chung_lu")
+        return Batch(self, h)
+
+    def set_batch_part_keys(self, nkeys: int) -> None:
+        _check(self.lib.tribit_set_option(self.h, OPT_BATCH_PART_KEYS, nkeys), "set_option")
 
     def set_relabel(self, on: bool) -> None:
         """Degree-rank relabelling of graphs built from now on (default on)."""

@@ -31,34 +31,6 @@ int guarded(tribit_ctx* ctx, F&& f) {
   }
 }
 
-void free_graph_memory(tribit_graph* g) {
-  if (!g || !g->ctx) return;
-  tb::free_tiles(g);
-  g->ctx->free(g->row_ptr);
-  g->ctx->free(g->col);
-  g->ctx->free(g->deg);
-  g->ctx->free(g->labels);
-  g->labels = nullptr;
-  g->ctx->free(g->rng_sfx);
-  g->ctx->free(g->sfx_ptr);
-  g->ctx->free(g->w_sfx);
-  g->rng_sfx = nullptr;
-  g->sfx_ptr = nullptr;
-  g->w_sfx = nullptr;
-  g->ctx->free(g->wrow);
-  g->ctx->free(g->rng_all);
-  g->ctx->free(g->rng_dir);
-  g->ctx->free(g->pid_dir);
-  g->pid_dir = nullptr;
-  g->ctx->free(g->fw_ptr);
-  g->ctx->free(g->rv_ptr);
-  g->ctx->free(g->w_fw);
-  g->ctx->free(g->w_rv);
-  g->row_ptr = g->col = g->deg = g->fw_ptr = g->rv_ptr = nullptr;
-  g->wrow = g->w_fw = g->w_rv = nullptr;
-  g->rng_all = g->rng_dir = nullptr;
-}
-
 }  // namespace
 
 extern "C" {
@@ -147,6 +119,10 @@ int tribit_set_option(tribit_ctx* ctx, int option, int64_t value) {
     case TRIBIT_OPT_RELABEL:
       if (value != 0 && value != 1) return TRIBIT_ERR_ARG;
       ctx->relabel = (int)value;
+      return TRIBIT_OK;
+    case TRIBIT_OPT_BATCH_PART_KEYS:
+      if (value < 1) return TRIBIT_ERR_ARG;
+      ctx->part_keys = (uint64_t)value;
       return TRIBIT_OK;
     default:
       return TRIBIT_ERR_ARG;
@@ -289,7 +265,7 @@ int tribit_graph_from_device_keys(tribit_ctx* ctx, uint64_t* d_keys, uint64_t nk
     tb::preprocess(ctx, keys, nkeys, n, g);
   });
   if (st != TRIBIT_OK) {
-    free_graph_memory(g);
+    tb::free_graph_memory(g);
     delete g;
     return st;
   }
@@ -361,7 +337,7 @@ int tribit_graph_free(tribit_graph* g) {
   if (!g) return TRIBIT_ERR_ARG;
   if (g->ctx) {
     (void)hipSetDevice(g->ctx->device);
-    free_graph_memory(g);
+    tb::free_graph_memory(g);
     (void)hipStreamSynchronize(g->ctx->stream);
   }
   delete g;
@@ -524,6 +500,93 @@ int tribit_learned_plan(tribit_ctx* ctx, tribit_graph* g, uint8_t* table) {
     TB_HIP(hipSetDevice(ctx->device));
     tb::learned_plan(ctx, g, table);
   });
+}
+
+int tribit_batch_from_host(tribit_ctx* ctx, const uint64_t* keys, const uint64_t* key_offsets, const uint64_t* nverts,
+                           uint32_t ngraphs, tribit_batch** out) {
+  if (!ctx || !out || !key_offsets || !nverts || ngraphs == 0 || (!keys && key_offsets[ngraphs] > key_offsets[0]))
+    return TRIBIT_ERR_ARG;
+  *out = nullptr;
+  return guarded(ctx, [&] {
+    TB_HIP(hipSetDevice(ctx->device));
+    *out = tb::batch_from_host(ctx, keys, key_offsets, nverts, ngraphs);
+  });
+}
+
+int tribit_batch_gen_chung_lu(tribit_ctx* ctx, uint64_t seed, uint64_t first, uint32_t ngraphs, tribit_batch** out) {
+  if (!ctx || !out || ngraphs == 0) return TRIBIT_ERR_ARG;
+  *out = nullptr;
+  return guarded(ctx, [&] {
+    TB_HIP(hipSetDevice(ctx->device));
+    *out = tb::batch_gen_chung_lu(ctx, seed, first, ngraphs);
+  });
+}
+
+int tribit_batch_info(const tribit_batch* b, uint32_t* ngraphs, uint32_t* nparts, uint64_t* total_n,
+                      uint64_t* total_m, double* ms_preprocess) {
+  if (!b) return TRIBIT_ERR_ARG;
+  uint64_t tn = 0, tm = 0;
+  for (uint32_t j = 0; j < b->ngraphs; ++j) {
+    tn += b->n[j];
+    tm += b->m[j];
+  }
+  if (ngraphs) *ngraphs = b->ngraphs;
+  if (nparts) *nparts = (uint32_t)b->parts.size();
+  if (total_n) *total_n = tn;
+  if (total_m) *total_m = tm;
+  if (ms_preprocess) *ms_preprocess = b->ms_prep;
+  return TRIBIT_OK;
+}
+
+int tribit_batch_graph_sizes(const tribit_batch* b, uint64_t* n, uint64_t* m) {
+  if (!b) return TRIBIT_ERR_ARG;
+  for (uint32_t j = 0; j < b->ngraphs; ++j) {
+    if (n) n[j] = b->n[j];
+    if (m) m[j] = b->m[j];
+  }
+  return TRIBIT_OK;
+}
+
+static int batch_opts(const tribit_count_opts* opts, const tribit_batch* b, tribit_count_opts* o) {
+  memset(o, 0, sizeof(*o));
+  o->sched = TRIBIT_SCHED_LEARNED;
+  if (opts) *o = *opts;
+  if (o->sched < TRIBIT_SCHED_PLAIN || o->sched > TRIBIT_SCHED_WEDGE) return TRIBIT_ERR_ARG;
+  if (o->sched == TRIBIT_SCHED_WEDGE)
+    for (const auto& p : b->parts)
+      if (!p.g->rng_sfx) return TRIBIT_ERR_ARG;
+  return TRIBIT_OK;
+}
+
+int tribit_batch_count(tribit_ctx* ctx, tribit_batch* b, const tribit_count_opts* opts, uint64_t* counts,
+                       tribit_batch_count_info* info) {
+  if (!ctx || !b || b->ctx != ctx || !counts) return TRIBIT_ERR_ARG;
+  tribit_count_opts o;
+  const int st = batch_opts(opts, b, &o);
+  if (st) return st;
+  return guarded(ctx, [&] {
+    TB_HIP(hipSetDevice(ctx->device));
+    tb::batch_count(ctx, b, o, counts, info);
+  });
+}
+
+int tribit_batch_autotune(tribit_ctx* ctx, tribit_batch* b, const tribit_count_opts* opts,
+                          tribit_batch_count_info* best) {
+  if (!ctx || !b || b->ctx != ctx) return TRIBIT_ERR_ARG;
+  tribit_count_opts o;
+  const int st = batch_opts(opts, b, &o);
+  if (st) return st;
+  return guarded(ctx, [&] {
+    TB_HIP(hipSetDevice(ctx->device));
+    tb::batch_autotune(ctx, b, o, best);
+  });
+}
+
+int tribit_batch_free(tribit_batch* b) {
+  if (!b) return TRIBIT_ERR_ARG;
+  if (b->ctx) (void)hipSetDevice(b->ctx->device);
+  tb::batch_free(b);
+  return TRIBIT_OK;
 }
 
 }  // extern "C"

@@ -6,6 +6,7 @@
 #include <cstdio>
 #include <string>
 #include <unordered_map>
+#include <vector>
 
 #include "../../include/tribit.h"
 
@@ -62,6 +63,7 @@ struct tribit_ctx {
   hipEvent_t ev[12] = {};
   int relabel = 1;     // TRIBIT_OPT_RELABEL: degree-rank relabelling in preprocessing
   tribit_mlp* model = nullptr;  // cost model of the learned scheduler (not owned)
+  uint64_t part_keys = 1ull << 30;  // TRIBIT_OPT_BATCH_PART_KEYS: input keys per batch part
   std::string last_error;
 
   void* alloc(uint64_t bytes) {
@@ -166,8 +168,41 @@ struct tribit_graph {
 };
 
 namespace tb {
+struct Schedule;
+void schedule_free(Schedule* s);
+}  // namespace tb
+
+// Many graphs counted together: each part is the disjoint union of a run of
+// graphs (vertex ids offset), preprocessed as one graph; every vertex carries
+// its graph index, so each task's count goes to the owner's graph.
+struct tribit_batch {
+  struct Part {
+    tribit_graph* g = nullptr;
+    uint32_t first = 0, count = 0;  // graphs [first, first + count)
+    uint32_t* gid = nullptr;        // device: graph index of each internal vertex
+    tb::Schedule* S = nullptr;      // tasks (built on the first count)
+  };
+  tribit_ctx* ctx = nullptr;
+  uint32_t ngraphs = 0;
+  std::vector<Part> parts;
+  std::vector<uint64_t> n, m;             // per graph: vertices, simple undirected edges
+  unsigned long long* gcount = nullptr;   // device: per-graph triangle counts
+  unsigned long long* scratch = nullptr;  // device: 8 words per part (count, fallbacks, queues)
+  hipGraph_t graph = nullptr;             // captured counting launches of all parts
+  hipGraphExec_t exec = nullptr;
+  int sched = -1, flags = 0;              // what the schedules / capture were built for
+  int gw[8] = {0};                        // tuned lane-group width per slot (0: the plan's)
+  int waves[8] = {0};                     // tuned workgroups per CU per slot (0: occupancy limit)
+  uint32_t launches = 0;                  // kernels in the captured graph
+  uint64_t tasks = 0;
+  double ms_prep = 0.0;
+};
+
+namespace tb {
 // prep.hip
 void preprocess(tribit_ctx* ctx, uint64_t* d_keys, uint64_t nkeys, uint64_t n, tribit_graph* g);
+void free_graph_memory(tribit_graph* g);
+void free_graph(tribit_graph* g);
 void gen_rmat(tribit_ctx* ctx, uint32_t scale, uint32_t ef, double a, double b, double c, uint64_t seed,
               uint64_t* d_keys);
 void gen_er(tribit_ctx* ctx, uint64_t n, uint64_t nsamples, uint64_t seed, uint64_t* d_keys);
@@ -189,6 +224,15 @@ void profile_features(const tribit_profile* p, int bucket, int slot, float* f);
 void learned_plan(tribit_ctx* ctx, tribit_graph* g, uint8_t* table);
 void menu_slot(int slot, int* kind, int* g, uint32_t* max_d);
 uint32_t bucket_max_outdeg(int bucket);
+void batch_count(tribit_ctx* ctx, tribit_batch* b, const tribit_count_opts& opts, uint64_t* counts,
+                 tribit_batch_count_info* info);
+void batch_autotune(tribit_ctx* ctx, tribit_batch* b, const tribit_count_opts& opts, tribit_batch_count_info* info);
+void batch_drop_exec(tribit_batch* b);
+// batch.hip
+tribit_batch* batch_from_host(tribit_ctx* ctx, const uint64_t* keys, const uint64_t* key_off, const uint64_t* nverts,
+                              uint32_t ngraphs);
+tribit_batch* batch_gen_chung_lu(tribit_ctx* ctx, uint64_t seed, uint64_t first, uint32_t ngraphs);
+void batch_free(tribit_batch* b);
 // mlp.hip
 tribit_mlp* mlp_create(tribit_ctx* ctx, int nfeat, uint64_t seed);
 void mlp_free(tribit_mlp* m);

@@ -19,6 +19,7 @@
 // and the owner's streamed work picks how many tasks its lists split into.
 #include <cstring>
 #include <algorithm>
+#include <chrono>
 #include <rocprim/device/device_scan.hpp>
 #include <vector>
 
@@ -307,7 +308,9 @@ __global__ __launch_bounds__(256) void k_count_wave(const uint32_t* __restrict__
                                                     const uint4* __restrict__ tasks, uint32_t ntasks,
                                                     uint32_t* __restrict__ queue,
                                                     unsigned long long* __restrict__ result, int attempts,
-                                                    unsigned long long* __restrict__ tlog) {
+                                                    unsigned long long* __restrict__ tlog,
+                                                    const uint32_t* __restrict__ gid,
+                                                    unsigned long long* __restrict__ gcount) {
   constexpr uint32_t S = 1u << LOGS;
   const ColSrc src = make_col_src(col, m);
   __shared__ uint32_t sT[4][2 * S];
@@ -355,6 +358,10 @@ __global__ __launch_bounds__(256) void k_count_wave(const uint32_t* __restrict__
         wave_lds_sync();
       }
       acc += cnt;
+      if (gcount) {  // per-graph counts of a batch: the owner's graph
+        const uint64_t tc = wave_sum_u64(cnt);
+        if (lane == 0 && tc) atomicAdd(&gcount[gid[x]], (unsigned long long)tc);
+      }
       if (tlog && lane == 0) atomicAdd(&tlog[tk.w >> kBucketShift], (unsigned long long)(wall_clock64() - t0));
     }
   }
@@ -379,7 +386,9 @@ __global__ __launch_bounds__(kBlockThreads) void k_count_block(const uint32_t* _
                                                                const uint4* __restrict__ tasks, uint32_t ntasks,
                                                                uint32_t* __restrict__ queue,
                                                                unsigned long long* __restrict__ result,
-                                                               int mode, unsigned long long* __restrict__ tlog) {
+                                                               int mode, unsigned long long* __restrict__ tlog,
+                                                               const uint32_t* __restrict__ gid,
+                                                               unsigned long long* __restrict__ gcount) {
   constexpr uint32_t S = 1u << LOGS;
   constexpr uint32_t kSpan = 64u * S;  // ids covered by the 2S-word bitmap
   const int attempts = mode & 0xFF;
@@ -387,6 +396,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_count_block(const uint32_t* _
   __shared__ uint32_t T[2 * S];
   __shared__ uint2 R[kChunk];
   __shared__ uint32_t s_task, s_next, s_fail;
+  __shared__ unsigned long long s_part[kBlockThreads / 64];
   const uint32_t lane = threadIdx.x & 63;
   const ColSrc src = make_col_src(col, m);
   uint64_t acc = 0;
@@ -462,6 +472,16 @@ __global__ __launch_bounds__(kBlockThreads) void k_count_block(const uint32_t* _
       __syncthreads();
     }
     acc += cnt;
+    if (gcount) {  // per-graph counts of a batch: the owner's graph
+      const uint64_t tc = wave_sum_u64(cnt);
+      if (lane == 0) s_part[threadIdx.x >> 6] = tc;
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (int w = 0; w < kBlockThreads / 64; ++w) t += s_part[w];
+        if (t) atomicAdd(&gcount[gid[x]], t);
+      }
+    }
     if (tlog && threadIdx.x == 0) atomicAdd(&tlog[tk.w >> kBucketShift], (unsigned long long)(wall_clock64() - t0));
   }
   acc = wave_sum_u64(acc);
@@ -885,94 +905,97 @@ static void learned_table(tribit_ctx* ctx, const BucketStats& st, int layout, in
 }
 
 // ----------------------------------------------------------------- launches
+// Per-count launch arguments (no allocation or host sync between them, so a
+// whole count can be captured into a hipGraph).
+struct LaunchArgs {
+  int att = 0;                           // mode word (attempts | flags)
+  unsigned long long* result = nullptr;  // [0] triangles, [1] fallback tasks
+  uint32_t* queue = nullptr;             // kSlots task counters
+  unsigned long long* tlog = nullptr;    // kSlots x kBuckets task durations, or null
+  const uint32_t* gid = nullptr;         // graph of each vertex (batches), or null
+  unsigned long long* gcount = nullptr;  // per-graph counts (batches), or null
+  int waves[kSlots] = {0};               // workgroups per CU cap per slot (0: occupancy limit)
+};
+
 template <int LOGS, int KMAX, int G, int U, bool BUF>
-static void launch_wave_t(uint32_t ntasks, int cus, hipStream_t s, tribit_graph* g, const uint2* rng,
-                          const uint4* tasks, uint32_t* queue, unsigned long long* result, int attempts,
-                          unsigned long long* tlog, uint64_t* blocks) {
+static void launch_wave_t(const LaunchArgs& a, int slot, uint32_t ntasks, int cus, hipStream_t s, tribit_graph* g,
+                          const uint2* rng, const uint4* tasks, uint64_t* blocks) {
   int per_cu = 0;
   TB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_count_wave<LOGS, KMAX, G, U, BUF>, kBlock, 0));
+  if (a.waves[slot] > 0 && per_cu > a.waves[slot]) per_cu = a.waves[slot];
   if (per_cu < 1) per_cu = 1;
   const uint64_t want = (ntasks + 4 * kWaveBatch - 1) / (4 * kWaveBatch);
   const unsigned grid = (unsigned)std::min<uint64_t>(want, (uint64_t)cus * per_cu);
-  k_count_wave<LOGS, KMAX, G, U, BUF><<<grid, kBlock, 0, s>>>(g->row_ptr, g->col, g->m, rng, tasks, ntasks, queue,
-                                                             result, attempts & 0xFF, tlog);
+  k_count_wave<LOGS, KMAX, G, U, BUF><<<grid, kBlock, 0, s>>>(g->row_ptr, g->col, g->m, rng, tasks, ntasks,
+                                                             a.queue + slot, a.result, a.att & 0xFF,
+                                                             a.tlog ? a.tlog + (size_t)slot * kBuckets : nullptr,
+                                                             a.gid, a.gcount);
   *blocks += grid;
 }
 
 template <int LOGS, int KMAX, int G>
-static void launch_wave_g(uint32_t ntasks, int cus, hipStream_t s, tribit_graph* g, const uint2* rng,
-                          const uint4* tasks, uint32_t* queue, unsigned long long* result, int attempts,
-                          unsigned long long* tlog, uint64_t* blocks) {
-  if (g->m * 4 < 0xFFFFFFFFull && !(attempts & kForceGlobal))
-    launch_wave_t<LOGS, KMAX, G, kU, true>(ntasks, cus, s, g, rng, tasks, queue, result, attempts, tlog, blocks);
+static void launch_wave_g(const LaunchArgs& a, int slot, uint32_t ntasks, int cus, hipStream_t s, tribit_graph* g,
+                          const uint2* rng, const uint4* tasks, uint64_t* blocks) {
+  if (g->m * 4 < 0xFFFFFFFFull && !(a.att & kForceGlobal))
+    launch_wave_t<LOGS, KMAX, G, kU, true>(a, slot, ntasks, cus, s, g, rng, tasks, blocks);
   else
-    launch_wave_t<LOGS, KMAX, G, kU, false>(ntasks, cus, s, g, rng, tasks, queue, result, attempts, tlog, blocks);
+    launch_wave_t<LOGS, KMAX, G, kU, false>(a, slot, ntasks, cus, s, g, rng, tasks, blocks);
 }
 
 template <int LOGS, int KMAX>
-static void launch_wave(int gw, uint32_t ntasks, int cus, hipStream_t s, tribit_graph* g, const uint2* rng,
-                        const uint4* tasks, uint32_t* queue, unsigned long long* result, int attempts,
-                        unsigned long long* tlog, uint64_t* blocks) {
+static void launch_wave(const LaunchArgs& a, int slot, int gw, uint32_t ntasks, int cus, hipStream_t s,
+                        tribit_graph* g, const uint2* rng, const uint4* tasks, uint64_t* blocks) {
   if (!ntasks) return;
   switch (gw) {
-    case 4: launch_wave_g<LOGS, KMAX, 4>(ntasks, cus, s, g, rng, tasks, queue, result, attempts, tlog, blocks); break;
-    case 8: launch_wave_g<LOGS, KMAX, 8>(ntasks, cus, s, g, rng, tasks, queue, result, attempts, tlog, blocks); break;
-    case 32:
-      launch_wave_g<LOGS, KMAX, 32>(ntasks, cus, s, g, rng, tasks, queue, result, attempts, tlog, blocks);
-      break;
-    default:
-      launch_wave_g<LOGS, KMAX, 16>(ntasks, cus, s, g, rng, tasks, queue, result, attempts, tlog, blocks);
-      break;
+    case 4: launch_wave_g<LOGS, KMAX, 4>(a, slot, ntasks, cus, s, g, rng, tasks, blocks); break;
+    case 8: launch_wave_g<LOGS, KMAX, 8>(a, slot, ntasks, cus, s, g, rng, tasks, blocks); break;
+    case 32: launch_wave_g<LOGS, KMAX, 32>(a, slot, ntasks, cus, s, g, rng, tasks, blocks); break;
+    default: launch_wave_g<LOGS, KMAX, 16>(a, slot, ntasks, cus, s, g, rng, tasks, blocks); break;
   }
 }
 
 template <int LOGS, int G, int U, bool BUF, bool TILES>
-static void launch_block_t(uint32_t ntasks, int cus, hipStream_t s, tribit_graph* g, const uint2* rng,
-                           const TileSrc& ts, const uint4* tasks, uint32_t* queue, unsigned long long* result,
-                           int attempts, unsigned long long* tlog, uint64_t* blocks) {
+static void launch_block_t(const LaunchArgs& a, int slot, uint32_t ntasks, int cus, hipStream_t s, tribit_graph* g,
+                           const uint2* rng, const TileSrc& ts, const uint4* tasks, uint64_t* blocks) {
   int per_cu = 0;
   TB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_count_block<LOGS, G, U, BUF, TILES>, kBlockThreads,
                                                       0));
+  if (a.waves[slot] > 0 && per_cu > a.waves[slot]) per_cu = a.waves[slot];
   if (per_cu < 1) per_cu = 1;
   const unsigned grid = (unsigned)std::min<uint64_t>(ntasks, (uint64_t)cus * per_cu);
   k_count_block<LOGS, G, U, BUF, TILES><<<grid, kBlockThreads, 0, s>>>(
-      g->row_ptr, g->col, g->m, rng, ts, tasks, ntasks, queue, result, attempts & ~kForceGlobal, tlog);
+      g->row_ptr, g->col, g->m, rng, ts, tasks, ntasks, a.queue + slot, a.result, a.att & ~kForceGlobal,
+      a.tlog ? a.tlog + (size_t)slot * kBuckets : nullptr, a.gid, a.gcount);
   *blocks += grid;
 }
 
 template <int LOGS, int G>
-static void launch_block_g(uint32_t ntasks, int cus, hipStream_t s, tribit_graph* g, const uint2* rng,
-                           const TileSrc& ts, const uint4* tasks, uint32_t* queue, unsigned long long* result,
-                           int attempts, unsigned long long* tlog, uint64_t* blocks) {
-  const bool buf = g->m * 4 < 0xFFFFFFFFull && !(attempts & kForceGlobal);
+static void launch_block_g(const LaunchArgs& a, int slot, uint32_t ntasks, int cus, hipStream_t s, tribit_graph* g,
+                           const uint2* rng, const TileSrc& ts, const uint4* tasks, uint64_t* blocks) {
+  const bool buf = g->m * 4 < 0xFFFFFFFFull && !(a.att & kForceGlobal);
   if (ts.trng) {
     if (buf)
-      launch_block_t<LOGS, G, kU, true, true>(ntasks, cus, s, g, rng, ts, tasks, queue, result, attempts, tlog, blocks);
+      launch_block_t<LOGS, G, kU, true, true>(a, slot, ntasks, cus, s, g, rng, ts, tasks, blocks);
     else
-      launch_block_t<LOGS, G, kU, false, true>(ntasks, cus, s, g, rng, ts, tasks, queue, result, attempts, tlog,
-                                               blocks);
+      launch_block_t<LOGS, G, kU, false, true>(a, slot, ntasks, cus, s, g, rng, ts, tasks, blocks);
   } else if (buf) {
-    launch_block_t<LOGS, G, kU, true, false>(ntasks, cus, s, g, rng, ts, tasks, queue, result, attempts, tlog, blocks);
+    launch_block_t<LOGS, G, kU, true, false>(a, slot, ntasks, cus, s, g, rng, ts, tasks, blocks);
   } else {
-    launch_block_t<LOGS, G, kU, false, false>(ntasks, cus, s, g, rng, ts, tasks, queue, result, attempts, tlog,
-                                              blocks);
+    launch_block_t<LOGS, G, kU, false, false>(a, slot, ntasks, cus, s, g, rng, ts, tasks, blocks);
   }
 }
 
 template <int LOGS>
-static void launch_block(int gw, uint32_t ntasks, int cus, hipStream_t s, tribit_graph* g, const uint2* rng,
-                         const TileSrc& ts, const uint4* tasks, uint32_t* queue, unsigned long long* result,
-                         int attempts, unsigned long long* tlog, uint64_t* blocks) {
+static void launch_block(const LaunchArgs& a, int slot, int gw, uint32_t ntasks, int cus, hipStream_t s,
+                         tribit_graph* g, const uint2* rng, const TileSrc& ts, const uint4* tasks, uint64_t* blocks) {
   if (!ntasks) return;
-#define TB_BLK(GW) launch_block_g<LOGS, GW>(ntasks, cus, s, g, rng, ts, tasks, queue, result, attempts, tlog, blocks)
   switch (gw) {
-    case 4: TB_BLK(4); break;
-    case 8: TB_BLK(8); break;
-    case 32: TB_BLK(32); break;
-    case 64: TB_BLK(64); break;
-    default: TB_BLK(16); break;
+    case 4: launch_block_g<LOGS, 4>(a, slot, ntasks, cus, s, g, rng, ts, tasks, blocks); break;
+    case 8: launch_block_g<LOGS, 8>(a, slot, ntasks, cus, s, g, rng, ts, tasks, blocks); break;
+    case 32: launch_block_g<LOGS, 32>(a, slot, ntasks, cus, s, g, rng, ts, tasks, blocks); break;
+    case 64: launch_block_g<LOGS, 64>(a, slot, ntasks, cus, s, g, rng, ts, tasks, blocks); break;
+    default: launch_block_g<LOGS, 16>(a, slot, ntasks, cus, s, g, rng, ts, tasks, blocks); break;
   }
-#undef TB_BLK
 }
 
 static float elapsed(hipEvent_t a, hipEvent_t b) {
@@ -981,39 +1004,50 @@ static float elapsed(hipEvent_t a, hipEvent_t b) {
   return ms;
 }
 
-// One scheduled count under `plan`. Returns the triangle count; fills
-// per-phase times. With `tlog` (kSlots x kBuckets, zeroed by the caller) every
-// task adds its wall-clock duration to tlog[slot][bucket].
-static uint64_t count_once(tribit_ctx* ctx, tribit_graph* g, int sched, int flags, Plan& plan, bool learned,
-                           double* ms_total, double* ms_sched, double* ms_kernel, double* ms_var,
-                           tribit_count_info* info, unsigned long long* tlog) {
+static int mode_word(int flags) {
+  return ((flags & TRIBIT_COUNT_FORCE_LINEAR_PROBE) ? 0 : kCuckooSeeds) |
+         ((flags & TRIBIT_COUNT_FORCE_GLOBAL_LOADS) ? kForceGlobal : 0) |
+         ((flags & TRIBIT_COUNT_NO_BITMAP) ? kNoBitmapBit : 0);
+}
+
+// Tasks of one count under a plan, ready to launch.
+struct Schedule {
+  Plan plan;
+  int gdef[kSlots] = {0};  // the plan's own widths (a batch's tuned widths override them)
+  int layout = 0;
+  const uint2* rng = nullptr;
+  TileSrc ts{nullptr, nullptr, nullptr};
+  DevBuf<uint4> tasks;
+  Cnt8 tot{}, base{};
+  uint64_t ntask = 0;
+};
+
+// Resolves the plan (learned table, tile opt-in/out) and builds the task
+// lists (one host sync for the per-slot totals).
+void make_schedule(tribit_ctx* ctx, tribit_graph* g, int layout, int flags, bool learned, Schedule& S) {
   hipStream_t s = ctx->stream;
   const uint64_t n = g->n;
-  TB_HIP(hipEventRecord(ctx->ev[2], s));
+  Plan& plan = S.plan;
+  S.layout = layout;
   if (learned) {
     // the model's choice is part of the scheduler's time
     BucketStats st;
-    bucket_stats(ctx, g, sched, &st);
-    learned_table(ctx, st, sched, flags, plan.table);
+    bucket_stats(ctx, g, layout, &st);
+    learned_table(ctx, st, layout, flags, plan.table);
   }
-  // scratch: [0] count, [1] fallback tasks, [4..7] eight u32 queue counters
-  DevBuf<unsigned long long> scratch(ctx, 8);
-  k_zero_u64<<<1, 64, 0, s>>>(scratch.p, 8);
-  unsigned long long* d_result = scratch.p;
-  uint32_t* d_queue = reinterpret_cast<uint32_t*>(scratch.p + 4);
   // test hooks / opt-outs: every block launch streams tiles, or none does
   for (int i = 0; i < plan.nslots; ++i) {
     if ((flags & TRIBIT_COUNT_FORCE_TILES) && plan.kind[i] == kKindBlock) plan.kind[i] = kKindTile;
     if ((flags & TRIBIT_COUNT_NO_TILES) && plan.kind[i] == kKindTile) plan.kind[i] = kKindBlock;
   }
-  TileSrc ts{nullptr, nullptr, nullptr};
   bool any_tiles = false;
   for (int i = 0; i < plan.nslots; ++i) any_tiles |= plan.kind[i] == kKindTile;
   if (any_tiles && g->m) {
-    build_tiles(ctx, g, sched);  // once per graph and layout
-    ts.blk = g->tile_blk;
-    ts.mask = reinterpret_cast<const unsigned long long*>(g->tile_mask);
-    ts.trng = sched == TRIBIT_SCHED_PLAIN ? g->trng_all : sched == TRIBIT_SCHED_WEDGE ? g->trng_sfx : g->trng_dir;
+    build_tiles(ctx, g, layout);  // once per graph and layout
+    S.ts.blk = g->tile_blk;
+    S.ts.mask = reinterpret_cast<const unsigned long long*>(g->tile_mask);
+    S.ts.trng = layout == TRIBIT_SCHED_PLAIN ? g->trng_all : layout == TRIBIT_SCHED_WEDGE ? g->trng_sfx
+                                                                                          : g->trng_dir;
   }
   DevBuf<uint8_t> dtable(ctx, kBuckets);
   TB_HIP(hipMemcpyAsync(dtable.p, plan.table, kBuckets, hipMemcpyHostToDevice, s));
@@ -1022,8 +1056,8 @@ static uint64_t count_once(tribit_ctx* ctx, tribit_graph* g, int sched, int flag
   pd.table = dtable.p;
 
   DevBuf<Cnt8> cnt(ctx, n + 1), off(ctx, n + 1);
-  const SchedView sv = sched_view(g, sched);
-  const uint2* rng = sched_ranges(g, sched);
+  const SchedView sv = sched_view(g, layout);
+  S.rng = sched_ranges(g, layout);
   k_sched_count<<<grid_for(n + 1, kBlock), kBlock, 0, s>>>(sv, pd, n, cnt.p);
   {
     Cnt8 zero;
@@ -1033,46 +1067,70 @@ static uint64_t count_once(tribit_ctx* ctx, tribit_graph* g, int sched, int flag
     DevBuf<uint8_t> tmp(ctx, bytes ? bytes : 8);
     TB_HIP(rocprim::exclusive_scan(tmp.p, bytes, cnt.p, off.p, zero, (size_t)(n + 1), Cnt8Plus(), s));
   }
-  Cnt8 tot;  // off[n] = totals (cnt[n] is zero)
-  TB_HIP(hipMemcpyAsync(&tot, off.p + n, sizeof(Cnt8), hipMemcpyDeviceToHost, s));
+  // off[n] = totals (cnt[n] is zero)
+  TB_HIP(hipMemcpyAsync(&S.tot, off.p + n, sizeof(Cnt8), hipMemcpyDeviceToHost, s));
   ctx->sync();
-  Cnt8 base;
-  uint64_t ntask = 0;
+  S.ntask = 0;
   for (int i = 0; i < kSlots; ++i) {
-    base.v[i] = (uint32_t)ntask;
-    ntask += tot.v[i];
+    S.base.v[i] = (uint32_t)S.ntask;
+    S.ntask += S.tot.v[i];
   }
-  if (ntask > 0xFFFFFFFFull) fail(TRIBIT_ERR_TOO_LARGE, "more than 2^32-1 tasks");
-  DevBuf<uint4> tasks(ctx, ntask ? ntask : 1);
-  k_sched_fill<<<grid_for(n, kBlock), kBlock, 0, s>>>(sv, pd, n, off.p, base, tasks.p);
-  cnt.reset();
-  off.reset();
-  TB_HIP(hipEventRecord(ctx->ev[3], s));
+  if (S.ntask > 0xFFFFFFFFull) fail(TRIBIT_ERR_TOO_LARGE, "more than 2^32-1 tasks");
+  S.tasks = DevBuf<uint4>(ctx, S.ntask ? S.ntask : 1);
+  k_sched_fill<<<grid_for(n, kBlock), kBlock, 0, s>>>(sv, pd, n, off.p, S.base, S.tasks.p);
+  TB_HIP(hipGetLastError());
+}
 
+// The counting launches of a schedule (capturable: no allocation, no sync).
+// The caller zeroes a.result and a.queue. With `ev`, an event is recorded
+// after each slot's launch (ev[i] for slot i).
+void launch_schedule(tribit_ctx* ctx, tribit_graph* g, const Schedule& S, const LaunchArgs& a, hipEvent_t* ev,
+                     uint64_t* blocks) {
+  hipStream_t s = ctx->stream;
   const int cus = ctx->cus > 0 ? ctx->cus : 256;
-  uint64_t blocks = 0;
-  // one launch per slot; an event after each gives its device time
-  const int att = ((flags & TRIBIT_COUNT_FORCE_LINEAR_PROBE) ? 0 : kCuckooSeeds) |
-                  ((flags & TRIBIT_COUNT_FORCE_GLOBAL_LOADS) ? kForceGlobal : 0) |
-                  ((flags & TRIBIT_COUNT_NO_BITMAP) ? kNoBitmapBit : 0);
+  const Plan& plan = S.plan;
   for (int i = 0; i < plan.nslots; ++i) {
-    const uint4* tk = tasks.p + base.v[i];
-    unsigned long long* tl = tlog ? tlog + (size_t)i * kBuckets : nullptr;
+    const uint4* tk = S.tasks.p + S.base.v[i];
+    const uint32_t nt = S.tot.v[i];
     if (plan.kind[i] == kKindWave64)
-      launch_wave<kC0Logs, kC0K>(plan.g[i], tot.v[i], cus, s, g, rng, tk, d_queue + i, d_result, att, tl, &blocks);
+      launch_wave<kC0Logs, kC0K>(a, i, plan.g[i], nt, cus, s, g, S.rng, tk, blocks);
     else if (plan.kind[i] == kKindWave512)
-      launch_wave<kC1Logs, kC1K>(plan.g[i], tot.v[i], cus, s, g, rng, tk, d_queue + i, d_result, att, tl, &blocks);
-    else if (plan.kind[i] == kKindTile && ts.trng)
-      launch_block<kC2Logs>(plan.g[i], tot.v[i], cus, s, g, rng, ts, tk, d_queue + i, d_result, att, tl, &blocks);
+      launch_wave<kC1Logs, kC1K>(a, i, plan.g[i], nt, cus, s, g, S.rng, tk, blocks);
+    else if (plan.kind[i] == kKindTile && S.ts.trng)
+      launch_block<kC2Logs>(a, i, plan.g[i], nt, cus, s, g, S.rng, S.ts, tk, blocks);
     else
-      launch_block<kC2Logs>(plan.g[i], tot.v[i], cus, s, g, rng, TileSrc{nullptr, nullptr, nullptr}, tk,
-                            d_queue + i, d_result, att, tl, &blocks);
-    TB_HIP(hipEventRecord(ctx->ev[4 + i], s));
+      launch_block<kC2Logs>(a, i, plan.g[i], nt, cus, s, g, S.rng, TileSrc{nullptr, nullptr, nullptr}, tk, blocks);
+    if (ev) TB_HIP(hipEventRecord(ev[i], s));
   }
   TB_HIP(hipGetLastError());
+}
+
+// One scheduled count under `plan`. Returns the triangle count; fills
+// per-phase times. With `tlog` (kSlots x kBuckets, zeroed by the caller) every
+// task adds its wall-clock duration to tlog[slot][bucket].
+static uint64_t count_once(tribit_ctx* ctx, tribit_graph* g, int sched, int flags, Plan& plan, bool learned,
+                           double* ms_total, double* ms_sched, double* ms_kernel, double* ms_var,
+                           tribit_count_info* info, unsigned long long* tlog) {
+  hipStream_t s = ctx->stream;
+  TB_HIP(hipEventRecord(ctx->ev[2], s));
+  Schedule S;
+  S.plan = plan;
+  make_schedule(ctx, g, sched, flags, learned, S);
+  plan = S.plan;
+  // scratch: [0] count, [1] fallback tasks, [4..7] eight u32 queue counters
+  DevBuf<unsigned long long> scratch(ctx, 8);
+  k_zero_u64<<<1, 64, 0, s>>>(scratch.p, 8);
+  LaunchArgs a;
+  a.att = mode_word(flags);
+  a.result = scratch.p;
+  a.queue = reinterpret_cast<uint32_t*>(scratch.p + 4);
+  a.tlog = tlog;
+  TB_HIP(hipEventRecord(ctx->ev[3], s));
+  uint64_t blocks = 0;
+  launch_schedule(ctx, g, S, a, ctx->ev + 4, &blocks);
   hipEvent_t last = ctx->ev[3 + plan.nslots];
   unsigned long long res[2] = {0, 0};
-  TB_HIP(hipMemcpyAsync(res, d_result, sizeof(res), hipMemcpyDeviceToHost, s));
+  TB_HIP(hipMemcpyAsync(res, a.result, sizeof(res), hipMemcpyDeviceToHost, s));
   TB_HIP(hipEventSynchronize(last));
   ctx->sync();
   *ms_sched = elapsed(ctx->ev[2], ctx->ev[3]);
@@ -1081,8 +1139,8 @@ static uint64_t count_once(tribit_ctx* ctx, tribit_graph* g, int sched, int flag
   for (int v = 0; v < plan.nslots; ++v) ms_var[v] = elapsed(ctx->ev[3 + v], ctx->ev[4 + v]);
   if (info) {
     info->launch_blocks = blocks;
-    info->tasks = ntask;
-    for (int i = 0; i < 8; ++i) info->tasks_by_variant[i] = i < plan.nslots ? tot.v[i] : 0;
+    info->tasks = S.ntask;
+    for (int i = 0; i < 8; ++i) info->tasks_by_variant[i] = i < plan.nslots ? S.tot.v[i] : 0;
     info->nvariants = plan.nslots;
     info->fallback_tasks = res[1];
     for (int i = 0; i < 8; ++i) info->group_width[i] = i < plan.nslots ? plan.g[i] : 0;
@@ -1206,6 +1264,209 @@ void variant_stats(tribit_ctx* ctx, tribit_graph* g, int sched, tribit_variant_w
     out->edges[s] += st.sum_k[b];
     out->wplus[s] += st.sum_w[b];
   }
+}
+
+// ------------------------------------------------------------------ batches
+void schedule_free(Schedule* s) { delete s; }
+
+void batch_drop_exec(tribit_batch* b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+}
+
+// Task lists of every part for (sched, flags); returns the device ms spent.
+static double batch_schedules(tribit_ctx* ctx, tribit_batch* b, int sched, int flags) {
+  bool ready = b->sched == sched && b->flags == flags;
+  for (auto& p : b->parts) ready = ready && p.S;
+  if (ready) return 0.0;
+  batch_drop_exec(b);
+  TB_HIP(hipEventRecord(ctx->ev[0], ctx->stream));
+  for (auto& p : b->parts) {
+    schedule_free(p.S);
+    p.S = nullptr;
+    int layout = 0;
+    Plan plan = plan_for(sched, nullptr, &layout);
+    const bool learned = sched == TRIBIT_SCHED_LEARNED;
+    if (learned) layout = p.g->rng_sfx ? TRIBIT_SCHED_WEDGE : TRIBIT_SCHED_DIRECTED;
+    if (layout == TRIBIT_SCHED_WEDGE && !p.g->rng_sfx) fail(TRIBIT_ERR_ARG, "the wedge schedule needs relabelled graphs");
+    p.S = new Schedule;
+    p.S->plan = plan;
+    make_schedule(ctx, p.g, layout, flags, learned, *p.S);
+    for (int i = 0; i < kSlots; ++i) p.S->gdef[i] = p.S->plan.g[i];
+  }
+  TB_HIP(hipEventRecord(ctx->ev[1], ctx->stream));
+  TB_HIP(hipEventSynchronize(ctx->ev[1]));
+  b->sched = sched;
+  b->flags = flags;
+  return elapsed(ctx->ev[0], ctx->ev[1]);
+}
+
+// Captures the zeroing of the counters and the counting launches of every
+// part into one hipGraph (counters are cleared by a kernel, not a memset
+// node). Returns the host ms of capture + instantiation.
+static double batch_capture(tribit_ctx* ctx, tribit_batch* b, int att, uint32_t* launches, uint64_t* tasks) {
+  batch_drop_exec(b);
+  hipStream_t s = ctx->stream;
+  TB_HIP(hipStreamSynchronize(s));
+  const auto t0 = std::chrono::steady_clock::now();
+  uint64_t blocks = 0;
+  uint32_t nl = 2;
+  uint64_t nt = 0;
+  TB_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  try {
+    k_zero_u64<<<grid_for(b->ngraphs ? b->ngraphs : 1, kBlock), kBlock, 0, s>>>(b->gcount, b->ngraphs);
+    const uint32_t nw = (uint32_t)(8 * b->parts.size());
+    k_zero_u64<<<grid_for(nw ? nw : 1, kBlock), kBlock, 0, s>>>(b->scratch, nw);
+    for (size_t i = 0; i < b->parts.size(); ++i) {
+      auto& p = b->parts[i];
+      for (int k = 0; k < kSlots; ++k) p.S->plan.g[k] = b->gw[k] ? b->gw[k] : p.S->gdef[k];
+      LaunchArgs a;
+      a.att = att;
+      a.result = b->scratch + 8 * i;
+      a.queue = reinterpret_cast<uint32_t*>(b->scratch + 8 * i + 4);
+      a.gid = p.gid;
+      a.gcount = b->gcount;
+      for (int k = 0; k < kSlots; ++k) a.waves[k] = b->waves[k];
+      launch_schedule(ctx, p.g, *p.S, a, nullptr, &blocks);
+      for (int k = 0; k < p.S->plan.nslots; ++k) nl += p.S->tot.v[k] ? 1 : 0;
+      nt += p.S->ntask;
+    }
+  } catch (...) {
+    hipGraph_t tmp = nullptr;
+    (void)hipStreamEndCapture(s, &tmp);
+    if (tmp) (void)hipGraphDestroy(tmp);
+    throw;
+  }
+  TB_HIP(hipStreamEndCapture(s, &b->graph));
+  TB_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  if (launches) *launches = nl;
+  if (tasks) *tasks = nt;
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// Replays the captured graph `reps` times; per-graph counts of the last
+// replay into `counts`; returns the median device ms; *agree = every replay
+// gave the same counts.
+static double batch_replay(tribit_ctx* ctx, tribit_batch* b, int reps, std::vector<unsigned long long>& counts,
+                           bool* agree) {
+  hipStream_t s = ctx->stream;
+  std::vector<double> t;
+  std::vector<unsigned long long> cur(b->ngraphs);
+  *agree = true;
+  for (int r = 0; r < reps; ++r) {
+    TB_HIP(hipEventRecord(ctx->ev[2], s));
+    TB_HIP(hipGraphLaunch(b->exec, s));
+    TB_HIP(hipEventRecord(ctx->ev[3], s));
+    if (b->ngraphs) TB_HIP(hipMemcpyAsync(cur.data(), b->gcount, 8 * b->ngraphs, hipMemcpyDeviceToHost, s));
+    TB_HIP(hipEventSynchronize(ctx->ev[3]));
+    ctx->sync();
+    t.push_back(elapsed(ctx->ev[2], ctx->ev[3]));
+    if (r == 0)
+      counts = cur;
+    else if (cur != counts)
+      *agree = false;
+  }
+  return median(t);
+}
+
+static void batch_fill_info(tribit_batch* b, double ms, double ms_sched, double ms_cap, uint32_t launches,
+                            uint64_t tasks, bool agree, const std::vector<unsigned long long>& counts,
+                            tribit_batch_count_info* info) {
+  if (!info) return;
+  memset(info, 0, sizeof(*info));
+  info->ms_count = ms;
+  info->ms_sched = ms_sched;
+  info->ms_capture = ms_cap;
+  uint64_t tri = 0;
+  for (auto c : counts) tri += c;
+  info->triangles = tri;
+  info->tasks = tasks;
+  info->parts = (uint32_t)b->parts.size();
+  info->launches = launches;
+  info->repeats_agree = agree ? 1 : 0;
+  if (!b->parts.empty() && b->parts[0].S) {
+    const Plan& pl = b->parts[0].S->plan;
+    info->nslots = pl.nslots;
+    for (int i = 0; i < pl.nslots && i < 8; ++i) {
+      info->group_width[i] = pl.g[i];
+      info->waves[i] = b->waves[i];
+    }
+  }
+}
+
+void batch_count(tribit_ctx* ctx, tribit_batch* b, const tribit_count_opts& opts, uint64_t* counts,
+                 tribit_batch_count_info* info) {
+  const int reps = opts.repeats > 0 ? opts.repeats : 1;
+  const double ms_sched = batch_schedules(ctx, b, opts.sched, opts.flags);
+  double ms_cap = 0.0;
+  if (!b->exec) ms_cap = batch_capture(ctx, b, mode_word(opts.flags), &b->launches, &b->tasks);
+  const uint32_t launches = b->launches;
+  const uint64_t tasks = b->tasks;
+  std::vector<unsigned long long> c;
+  bool agree = true;
+  const double ms = batch_replay(ctx, b, reps, c, &agree);
+  for (uint32_t j = 0; j < b->ngraphs; ++j) counts[j] = c[j];
+  batch_fill_info(b, ms, ms_sched, ms_cap, launches, tasks, agree, c, info);
+}
+
+// Coordinate search: for every slot with tasks, the lane-group width, then
+// the workgroups per CU, each kept when a replay of the recaptured graph is
+// faster. Every configuration must give the same per-graph counts.
+void batch_autotune(tribit_ctx* ctx, tribit_batch* b, const tribit_count_opts& opts, tribit_batch_count_info* info) {
+  const double ms_sched = batch_schedules(ctx, b, opts.sched, opts.flags);
+  const int att = mode_word(opts.flags);
+  const int reps = opts.repeats > 0 ? opts.repeats : 3;
+  std::vector<unsigned long long> ref, c;
+  bool agree = true;
+  uint32_t launches = 0;
+  uint64_t tasks = 0;
+  auto trial = [&]() {
+    batch_capture(ctx, b, att, &launches, &tasks);
+    const double ms = batch_replay(ctx, b, reps, c, &agree);
+    if (!agree || (!ref.empty() && c != ref)) fail(TRIBIT_ERR_HIP, "batch counts differ between configurations");
+    if (ref.empty()) ref = c;
+    return ms;
+  };
+  double best = trial();
+  const Plan& pl = b->parts.empty() ? Plan{} : b->parts[0].S->plan;
+  const int nslots = b->parts.empty() ? 0 : pl.nslots;
+  for (int i = 0; i < nslots; ++i) {
+    uint64_t nt = 0;
+    for (auto& p : b->parts) nt += p.S->tot.v[i];
+    if (!nt) continue;
+    const bool wave = pl.kind[i] == kKindWave64 || pl.kind[i] == kKindWave512;
+    const int gws[4] = {wave ? 4 : 8, wave ? 8 : 16, wave ? 16 : 32, wave ? 32 : 64};
+    const int g0 = b->gw[i] ? b->gw[i] : b->parts[0].S->gdef[i];
+    int bestg = g0;
+    for (int gw : gws) {
+      if (gw == g0) continue;
+      b->gw[i] = gw;
+      const double ms = trial();
+      if (ms < best) {
+        best = ms;
+        bestg = gw;
+      }
+    }
+    b->gw[i] = bestg;
+    const int w0 = b->waves[i];
+    int bestw = w0;
+    for (int wv : {1, 2, 4, 0}) {
+      if (wv == w0) continue;
+      b->waves[i] = wv;
+      const double ms = trial();
+      if (ms < best) {
+        best = ms;
+        bestw = wv;
+      }
+    }
+    b->waves[i] = bestw;
+  }
+  const double ms_cap = batch_capture(ctx, b, att, &launches, &tasks);
+  const double ms = batch_replay(ctx, b, reps, c, &agree);
+  if (!agree || (!ref.empty() && c != ref)) fail(TRIBIT_ERR_HIP, "batch counts differ between configurations");
+  batch_fill_info(b, ms, ms_sched, ms_cap, launches, tasks, agree, c, info);
 }
 
 }  // namespace tb

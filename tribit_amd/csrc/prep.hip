@@ -556,4 +556,40 @@ void preprocess(tribit_ctx* ctx, uint64_t* d_keys_in, uint64_t nkeys, uint64_t n
   g->max_outdeg = hst.max_outdeg;
 }
 
+// Releases every device buffer of a graph (the struct stays).
+void free_graph_memory(tribit_graph* g) {
+  if (!g || !g->ctx) return;
+  free_tiles(g);
+  g->ctx->free(g->row_ptr);
+  g->ctx->free(g->col);
+  g->ctx->free(g->deg);
+  g->ctx->free(g->labels);
+  g->labels = nullptr;
+  g->ctx->free(g->rng_sfx);
+  g->ctx->free(g->sfx_ptr);
+  g->ctx->free(g->w_sfx);
+  g->rng_sfx = nullptr;
+  g->sfx_ptr = nullptr;
+  g->w_sfx = nullptr;
+  g->ctx->free(g->wrow);
+  g->ctx->free(g->rng_all);
+  g->ctx->free(g->rng_dir);
+  g->ctx->free(g->pid_dir);
+  g->pid_dir = nullptr;
+  g->ctx->free(g->fw_ptr);
+  g->ctx->free(g->rv_ptr);
+  g->ctx->free(g->w_fw);
+  g->ctx->free(g->w_rv);
+  g->row_ptr = g->col = g->deg = g->fw_ptr = g->rv_ptr = nullptr;
+  g->wrow = g->w_fw = g->w_rv = nullptr;
+  g->rng_all = g->rng_dir = nullptr;
+}
+
+
+void free_graph(tribit_graph* g) {
+  if (!g) return;
+  free_graph_memory(g);
+  delete g;
+}
+
 }  // namespace tb
