@@ -87,7 +87,8 @@ typedef struct {
   int32_t repeats;         /* timed repetitions (>=1); the count of every repeat must agree */
   int32_t flags;           /* TRIBIT_COUNT_* bits */
   int32_t group_width[4];  /* per kernel variant: lanes per streamed list (4,8,16,32,64; 0 = default) */
-  int32_t reserved0;
+  int32_t dense_core;      /* relabelled graphs: vertices of the top-id core counted on the matrix cores;
+                              0 = chosen by the cost estimate for LEARNED, none otherwise; -1 = never */
 } tribit_count_opts;
 
 typedef struct {
@@ -103,6 +104,8 @@ typedef struct {
   double ms_by_variant[8]; /* device time of each variant's launch (median) */
   uint64_t fallback_tasks; /* tasks whose cuckoo set could not be built (linear-probing fallback) */
   int32_t group_width[8];  /* lanes per streamed list used by each variant */
+  uint64_t dense_core;     /* core vertices counted by the int8 MFMA kernel (0: none) */
+  double ms_dense;         /* device time of the core launch (median) */
 } tribit_count_info;
 
 /* Work assigned to each kernel variant by a scheduler, in the units of the

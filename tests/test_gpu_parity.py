@@ -406,3 +406,33 @@ def test_bit_tiles(gpu_ctx, oracle_mod, spec):
     with gpu_ctx.graph_from_keys(keys, 2300) as g:
         for sched in scheds_of(g):
             assert g.count(sched=sched, flags=tribit_amd.FORCE_TILES).triangles == t
+
+
+def test_dense_core_mfma(gpu_ctx, oracle_mod):
+    """Triangles of the top-id core counted by the int8 MFMA kernel plus the rest streamed: exact for
+    every core size (partial tiles, the whole graph) and every schedule."""
+    note("dense core: K_300, rmat s16, sbm, rmat s20 auto")
+    keys, t = graphs.complete(300)
+    with gpu_ctx.graph_from_keys(keys, 300) as g:
+        for core in (1, 64, 128, 200, 256, 300):
+            for sched in scheds_of(g):
+                r = g.count(sched=sched, dense_core=core)
+                assert r.triangles == t, (core, sched)
+                assert r.dense_core == core
+    keys = oracle_mod.gen_rmat(16, 16, seed=4)
+    want = oracle_mod.count_keys(keys, 1 << 16)
+    with gpu_ctx.graph_from_keys(keys, 1 << 16) as g:
+        for core in (128, 1000, 4096, 1 << 16):
+            for sched in scheds_of(g):
+                assert g.count(sched=sched, dense_core=core).triangles == want, (core, sched)
+        assert g.count(sched="wedge", dense_core=4096, flags=tribit_amd.FORCE_TILES).triangles == want
+    sbm = {"spec": {"gen": "sbm", "blocks": 16, "size": 256, "p_in": 0.5, "p_out": 2e-3, "seed": 9}, "n": 4096}
+    want = oracle_mod.count_keys(gens.oracle_keys(oracle_mod, sbm), 4096)
+    with gens.gpu_graph(gpu_ctx, sbm) as g:
+        for core in (512, 2048, 4096):
+            assert g.count(sched="learned", dense_core=core).triangles == want
+    with gpu_ctx.gen_rmat(20, 16, seed=6) as g:
+        a = g.count(sched="wedge", dense_core=-1).triangles
+        r = g.count(sched="learned")
+        assert r.triangles == a
+        assert g.count(sched="learned", dense_core=8192).triangles == a

@@ -49,7 +49,7 @@ class GraphInfo(ctypes.Structure):
 
 class CountOpts(ctypes.Structure):
     _fields_ = [("sched", ctypes.c_int32), ("repeats", ctypes.c_int32), ("flags", ctypes.c_int32),
-                ("group_width", ctypes.c_int32 * 4), ("reserved0", ctypes.c_int32)]
+                ("group_width", ctypes.c_int32 * 4), ("dense_core", ctypes.c_int32)]
 
 
 FORCE_LINEAR_PROBE = 1
@@ -67,11 +67,13 @@ class CountInfo(ctypes.Structure):
         ("ms_kernel", ctypes.c_double), ("launch_blocks", ctypes.c_uint64), ("tasks", ctypes.c_uint64),
         ("tasks_by_variant", ctypes.c_uint64 * 8), ("repeats_agree", ctypes.c_int32), ("nvariants", ctypes.c_int32),
         ("ms_by_variant", ctypes.c_double * 8), ("fallback_tasks", ctypes.c_uint64),
-        ("group_width", ctypes.c_int32 * 8),
+        ("group_width", ctypes.c_int32 * 8), ("dense_core", ctypes.c_uint64), ("ms_dense", ctypes.c_double),
     ]
 
 
 COST_MODEL_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cost_model.f32")
+
+
 class BatchCountInfo(ctypes.Structure):
     _fields_ = [
         ("ms_count", ctypes.c_double), ("ms_sched", ctypes.c_double), ("ms_capture", ctypes.c_double),
@@ -254,6 +256,8 @@ class CountResult:
     ms_by_variant: list
     fallback_tasks: int = 0
     group_width: tuple = ()
+    dense_core: int = 0
+    ms_dense: float = 0.0
 
 
 class Mlp:
@@ -429,11 +433,13 @@ class Graph:
         _check(self.ctx.lib.tribit_graph_labels(self.h, ids.ctypes.data, ctypes.byref(rl)), "graph_labels")
         return ids[:n], bool(rl.value)
 
-    def count(self, sched: str = "plain", repeats: int = 1, flags: int = 0, group_width=None) -> CountResult:
+    def count(self, sched: str = "plain", repeats: int = 1, flags: int = 0, group_width=None,
+              dense_core: int = 0) -> CountResult:
         o = CountOpts()
         o.sched = SCHED[sched]
         o.repeats = repeats
         o.flags = flags
+        o.dense_core = dense_core
         for i, w in enumerate(group_width or ()):
             o.group_width[i] = int(w)
         ci = CountInfo()
@@ -441,7 +447,8 @@ class Graph:
         nv = ci.nvariants
         return CountResult(ci.triangles, ci.ms_count, ci.ms_sched, ci.ms_kernel, ci.launch_blocks, ci.tasks,
                            list(ci.tasks_by_variant)[:max(nv, 4)], bool(ci.repeats_agree),
-                           list(ci.ms_by_variant)[:nv], ci.fallback_tasks, tuple(ci.group_width))
+                           list(ci.ms_by_variant)[:nv], ci.fallback_tasks, tuple(ci.group_width)[:nv],
+                           ci.dense_core, ci.ms_dense)
 
     def variant_stats(self, sched: str = "plain") -> dict:
         vs = VariantStats()

@@ -21,7 +21,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libtribit_hip.so")
 CLI = os.path.join(HERE, "tribit")
 
-HIP_SOURCES = ["prep.hip", "gen.hip", "tiles.hip", "count.hip", "batch.hip", "mlp.hip", "api.hip"]
+HIP_SOURCES = ["prep.hip", "gen.hip", "tiles.hip", "count.hip", "dense.hip", "batch.hip", "mlp.hip", "api.hip"]
 HOST_SOURCES = ["mtx.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -60,7 +60,7 @@ struct tribit_ctx {
   uint64_t held = 0;   // bytes currently allocated through this context
   uint64_t peak = 0;
   std::unordered_map<void*, uint64_t> sizes;
-  hipEvent_t ev[12] = {};
+  hipEvent_t ev[16] = {};
   int relabel = 1;     // TRIBIT_OPT_RELABEL: degree-rank relabelling in preprocessing
   tribit_mlp* model = nullptr;  // cost model of the learned scheduler (not owned)
   uint64_t part_keys = 1ull << 30;  // TRIBIT_OPT_BATCH_PART_KEYS: input keys per batch part
@@ -163,6 +163,9 @@ struct tribit_graph {
   uint2* trng_all = nullptr;     // m, tile ranges aligned with rng_all
   uint2* trng_dir = nullptr;     // m, tile ranges aligned with rng_dir
   uint2* trng_sfx = nullptr;     // m, tile ranges aligned with rng_sfx
+  // dense core (dense.hip): int8 adjacency of the top core_c ids, padded to core_kp
+  int8_t* core_A = nullptr;
+  uint64_t core_c = 0, core_kp = 0;
   double ms_prep = 0.0;
   uint64_t bytes = 0;
 };
@@ -246,6 +249,12 @@ double mlp_train(tribit_mlp* m, const float* hX, const float* ht, uint64_t n, in
                  uint64_t seed);
 void mlp_infer(tribit_mlp* m, const float* hX, uint64_t n, float* hout);
 void mlp_infer_dev(tribit_mlp* m, const float* dX, uint64_t n, float* dout);
+// dense.hip
+void core_prepare(tribit_ctx* ctx, tribit_graph* g, uint64_t c);
+void core_release(tribit_graph* g);
+void core_launch(tribit_ctx* ctx, const tribit_graph* g, unsigned long long* result);
+uint64_t core_choose(tribit_ctx* ctx, const tribit_graph* g);
+double core_macs(uint64_t kp);
 // tiles.hip
 uint64_t count_tiles(tribit_ctx* ctx, tribit_graph* g);
 void build_tiles(tribit_ctx* ctx, tribit_graph* g, int layout);

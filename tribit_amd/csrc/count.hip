@@ -569,6 +569,12 @@ struct SchedView {
   const uint64_t* w_rv;
   int nlists;              // 1 (PLAIN, WEDGE) or 2 (DIRECTED)
   uint32_t min_d0;         // smallest owner outdeg for which list 0 can hold a triangle
+  int layout;              // TRIBIT_SCHED_PLAIN / DIRECTED / WEDGE
+  // dense core [core_d0, n) (counted on the matrix cores): entries whose
+  // smallest vertex is in the core are dropped; core_d0 = UINT32_MAX: none
+  const uint2* rng;
+  uint32_t core_d0;
+  uint32_t core_cut;       // entries sorted by source: kept iff rng.x < core_cut
 };
 
 struct PlanDev {
@@ -593,6 +599,24 @@ __device__ __forceinline__ void owner_list(const SchedView& v, uint64_t x, int l
   *base = p[x];
   *count = p[x + 1] - p[x];
   *work = (l == 0) ? v.w_fw[x] : v.w_rv[x];
+  if (v.core_d0 == 0xFFFFFFFFu || x < v.core_d0 || *count == 0) return;
+  // owner in the core: lists whose triangles all have their smallest vertex
+  // in the core go to the core kernel; source-sorted lists keep their prefix
+  // of sources below the core
+  uint32_t keep = 0;
+  if (v.layout == TRIBIT_SCHED_WEDGE || (v.layout == TRIBIT_SCHED_DIRECTED && l == 1)) {
+    uint32_t lo = 0, hi = *count;
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (v.rng[*base + mid].x < v.core_cut)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    keep = lo;
+  }
+  *work = keep ? (uint64_t)((double)*work * keep / *count) : 0;
+  *count = keep;
 }
 
 __device__ __forceinline__ bool list_useful(const SchedView& v, int l, uint32_t d, uint32_t count) {
@@ -703,10 +727,15 @@ __global__ void k_zero_u64(unsigned long long* p, uint32_t n) {
   if (i < n) p[i] = 0ull;
 }
 
-static SchedView sched_view(const tribit_graph* g, int sched) {
+static SchedView sched_view(const tribit_graph* g, int sched, uint32_t core_d0 = 0xFFFFFFFFu,
+                            uint32_t core_cut = 0) {
   SchedView v;
   v.row_ptr = g->row_ptr;
   v.min_d0 = 2;
+  v.layout = sched;
+  v.rng = sched == TRIBIT_SCHED_PLAIN ? g->rng_all : sched == TRIBIT_SCHED_WEDGE ? g->rng_sfx : g->rng_dir;
+  v.core_d0 = core_d0;
+  v.core_cut = core_cut;
   if (sched == TRIBIT_SCHED_PLAIN) {
     v.fw_ptr = g->row_ptr;
     v.rv_ptr = g->row_ptr;
@@ -794,12 +823,14 @@ static void default_widths(int sched, int gw[4]) {
   }
 }
 
-void bucket_stats(tribit_ctx* ctx, tribit_graph* g, int sched, BucketStats* out) {
+void bucket_stats(tribit_ctx* ctx, tribit_graph* g, int sched, BucketStats* out, uint32_t core_d0 = 0xFFFFFFFFu,
+                  uint32_t core_cut = 0) {
   memset(out, 0, sizeof(*out));
   DevBuf<unsigned long long> st(ctx, kStat * kBuckets);
   k_zero_u64<<<grid_for(kStat * kBuckets, kBlock), kBlock, 0, ctx->stream>>>(st.p, kStat * kBuckets);
   if (g->n && g->m)
-    k_bucket_stats<<<grid_for(g->n, 256), 256, 0, ctx->stream>>>(sched_view(g, sched), g->col, g->n, st.p);
+    k_bucket_stats<<<grid_for(g->n, 256), 256, 0, ctx->stream>>>(sched_view(g, sched, core_d0, core_cut), g->col,
+                                                                 g->n, st.p);
   std::vector<unsigned long long> h(kStat * kBuckets);
   TB_HIP(hipMemcpyAsync(h.data(), st.p, 8 * h.size(), hipMemcpyDeviceToHost, ctx->stream));
   ctx->sync();
@@ -1014,6 +1045,7 @@ static int mode_word(int flags) {
 struct Schedule {
   Plan plan;
   int gdef[kSlots] = {0};  // the plan's own widths (a batch's tuned widths override them)
+  uint64_t core_c = 0;     // dense-core vertices counted on the matrix cores (0: none)
   int layout = 0;
   const uint2* rng = nullptr;
   TileSrc ts{nullptr, nullptr, nullptr};
@@ -1024,15 +1056,28 @@ struct Schedule {
 
 // Resolves the plan (learned table, tile opt-in/out) and builds the task
 // lists (one host sync for the per-slot totals).
-void make_schedule(tribit_ctx* ctx, tribit_graph* g, int layout, int flags, bool learned, Schedule& S) {
+void make_schedule(tribit_ctx* ctx, tribit_graph* g, int layout, int flags, bool learned, Schedule& S,
+                   uint64_t core_c) {
   hipStream_t s = ctx->stream;
   const uint64_t n = g->n;
   Plan& plan = S.plan;
   S.layout = layout;
+  uint32_t core_d0 = 0xFFFFFFFFu, core_cut = 0;
+  if (core_c > g->n || !g->relabeled) core_c = 0;
+  S.core_c = core_c;
+  if (core_c) {
+    core_prepare(ctx, g, core_c);
+    core_d0 = (uint32_t)(n - core_c);
+    uint32_t e0 = 0;
+    TB_HIP(hipMemcpyAsync(&e0, g->row_ptr + core_d0, 4, hipMemcpyDeviceToHost, s));
+    ctx->sync();
+    // wedge entries start one past their arc; reverse entries at their row start
+    core_cut = layout == TRIBIT_SCHED_WEDGE ? e0 + 1 : e0;
+  }
   if (learned) {
     // the model's choice is part of the scheduler's time
     BucketStats st;
-    bucket_stats(ctx, g, layout, &st);
+    bucket_stats(ctx, g, layout, &st, core_d0, core_cut);
     learned_table(ctx, st, layout, flags, plan.table);
   }
   // test hooks / opt-outs: every block launch streams tiles, or none does
@@ -1056,7 +1101,7 @@ void make_schedule(tribit_ctx* ctx, tribit_graph* g, int layout, int flags, bool
   pd.table = dtable.p;
 
   DevBuf<Cnt8> cnt(ctx, n + 1), off(ctx, n + 1);
-  const SchedView sv = sched_view(g, layout);
+  const SchedView sv = sched_view(g, layout, core_d0, core_cut);
   S.rng = sched_ranges(g, layout);
   k_sched_count<<<grid_for(n + 1, kBlock), kBlock, 0, s>>>(sv, pd, n, cnt.p);
   {
@@ -1102,6 +1147,10 @@ void launch_schedule(tribit_ctx* ctx, tribit_graph* g, const Schedule& S, const 
       launch_block<kC2Logs>(a, i, plan.g[i], nt, cus, s, g, S.rng, TileSrc{nullptr, nullptr, nullptr}, tk, blocks);
     if (ev) TB_HIP(hipEventRecord(ev[i], s));
   }
+  if (S.core_c) {
+    core_launch(ctx, g, a.result);
+    if (ev) TB_HIP(hipEventRecord(ev[plan.nslots], s));
+  }
   TB_HIP(hipGetLastError());
 }
 
@@ -1110,12 +1159,13 @@ void launch_schedule(tribit_ctx* ctx, tribit_graph* g, const Schedule& S, const 
 // task adds its wall-clock duration to tlog[slot][bucket].
 static uint64_t count_once(tribit_ctx* ctx, tribit_graph* g, int sched, int flags, Plan& plan, bool learned,
                            double* ms_total, double* ms_sched, double* ms_kernel, double* ms_var,
-                           tribit_count_info* info, unsigned long long* tlog) {
+                           tribit_count_info* info, unsigned long long* tlog, uint64_t core_c = 0,
+                           double* ms_core = nullptr) {
   hipStream_t s = ctx->stream;
   TB_HIP(hipEventRecord(ctx->ev[2], s));
   Schedule S;
   S.plan = plan;
-  make_schedule(ctx, g, sched, flags, learned, S);
+  make_schedule(ctx, g, sched, flags, learned, S, core_c);
   plan = S.plan;
   // scratch: [0] count, [1] fallback tasks, [4..7] eight u32 queue counters
   DevBuf<unsigned long long> scratch(ctx, 8);
@@ -1128,7 +1178,7 @@ static uint64_t count_once(tribit_ctx* ctx, tribit_graph* g, int sched, int flag
   TB_HIP(hipEventRecord(ctx->ev[3], s));
   uint64_t blocks = 0;
   launch_schedule(ctx, g, S, a, ctx->ev + 4, &blocks);
-  hipEvent_t last = ctx->ev[3 + plan.nslots];
+  hipEvent_t last = ctx->ev[3 + plan.nslots + (S.core_c ? 1 : 0)];
   unsigned long long res[2] = {0, 0};
   TB_HIP(hipMemcpyAsync(res, a.result, sizeof(res), hipMemcpyDeviceToHost, s));
   TB_HIP(hipEventSynchronize(last));
@@ -1137,6 +1187,7 @@ static uint64_t count_once(tribit_ctx* ctx, tribit_graph* g, int sched, int flag
   *ms_kernel = elapsed(ctx->ev[3], last);
   *ms_total = elapsed(ctx->ev[2], last);
   for (int v = 0; v < plan.nslots; ++v) ms_var[v] = elapsed(ctx->ev[3 + v], ctx->ev[4 + v]);
+  if (ms_core) *ms_core = S.core_c ? elapsed(ctx->ev[3 + plan.nslots], last) : 0.0;
   if (info) {
     info->launch_blocks = blocks;
     info->tasks = S.ntask;
@@ -1144,6 +1195,7 @@ static uint64_t count_once(tribit_ctx* ctx, tribit_graph* g, int sched, int flag
     info->nvariants = plan.nslots;
     info->fallback_tasks = res[1];
     for (int i = 0; i < 8; ++i) info->group_width[i] = i < plan.nslots ? plan.g[i] : 0;
+    info->dense_core = S.core_c;
   }
   return res[0];
 }
@@ -1182,9 +1234,20 @@ void count(tribit_ctx* ctx, tribit_graph* g, const tribit_count_opts& opts, trib
   Plan plan = plan_for(opts.sched, opts.group_width, &layout);
   const bool learned = opts.sched == TRIBIT_SCHED_LEARNED;
   if (learned) layout = g->rng_sfx ? TRIBIT_SCHED_WEDGE : TRIBIT_SCHED_DIRECTED;
+  // dense core: LEARNED picks its size by the cost estimate, the fixed
+  // schedules only use one when asked for (opts.dense_core > 0)
+  uint64_t core_c = 0;
+  if (opts.dense_core > 0)
+    core_c = std::min<uint64_t>((uint64_t)opts.dense_core, g->n);
+  else if (opts.dense_core == 0 && learned)
+    core_c = core_choose(ctx, g);
+  if (!g->relabeled) core_c = 0;
+  std::vector<double> tcore;
   for (int r = 0; r < reps; ++r) {
-    double a, b, c, v[kSlots] = {0};
-    const uint64_t tri = count_once(ctx, g, layout, opts.flags, plan, learned, &a, &b, &c, v, &local, nullptr);
+    double a, b, c, v[kSlots] = {0}, mc = 0.0;
+    const uint64_t tri =
+        count_once(ctx, g, layout, opts.flags, plan, learned, &a, &b, &c, v, &local, nullptr, core_c, &mc);
+    tcore.push_back(mc);
     if (r == 0)
       first = tri;
     else if (tri != first)
@@ -1200,6 +1263,7 @@ void count(tribit_ctx* ctx, tribit_graph* g, const tribit_count_opts& opts, trib
   local.ms_kernel = median(tk);
   for (int i = 0; i < kSlots; ++i) local.ms_by_variant[i] = i < plan.nslots ? median(tv[i]) : 0.0;
   local.repeats_agree = agree ? 1 : 0;
+  local.ms_dense = median(tcore);
   if (info) *info = local;
 }
 
@@ -1293,7 +1357,7 @@ static double batch_schedules(tribit_ctx* ctx, tribit_batch* b, int sched, int f
     if (layout == TRIBIT_SCHED_WEDGE && !p.g->rng_sfx) fail(TRIBIT_ERR_ARG, "the wedge schedule needs relabelled graphs");
     p.S = new Schedule;
     p.S->plan = plan;
-    make_schedule(ctx, p.g, layout, flags, learned, *p.S);
+    make_schedule(ctx, p.g, layout, flags, learned, *p.S, 0);  // per-graph attribution: no dense core
     for (int i = 0; i < kSlots; ++i) p.S->gdef[i] = p.S->plan.g[i];
   }
   TB_HIP(hipEventRecord(ctx->ev[1], ctx->stream));

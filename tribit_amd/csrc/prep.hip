@@ -560,6 +560,7 @@ void preprocess(tribit_ctx* ctx, uint64_t* d_keys_in, uint64_t nkeys, uint64_t n
 void free_graph_memory(tribit_graph* g) {
   if (!g || !g->ctx) return;
   free_tiles(g);
+  core_release(g);
   g->ctx->free(g->row_ptr);
   g->ctx->free(g->col);
   g->ctx->free(g->deg);
