@@ -68,7 +68,9 @@ typedef struct {
 /* Scheduler choice for tribit_count. */
 typedef enum {
   TRIBIT_SCHED_PLAIN = 0,    /* static degree thresholds; textbook direction: N+(u) is the set, N+(v) streamed */
-  TRIBIT_SCHED_LEARNED = 1,  /* cost-model driven (without a trained model: the DIRECTED heuristic) */
+  TRIBIT_SCHED_LEARNED = 1,  /* per-bucket kernel variant from the cost model, dense core by cost estimate;
+                                the first count of a graph times the model's plan against the heuristic plans
+                                and keeps the fastest (see tribit_count_info.candidate) */
   TRIBIT_SCHED_DIRECTED = 2, /* static thresholds; every edge streams the shorter of its two lists */
   TRIBIT_SCHED_WEDGE = 3     /* relabelled graphs: every arc u->v streams the part of N+(u) after v
                                 against N+(v) (each out-out wedge of u checked once) */
@@ -81,12 +83,15 @@ typedef enum {
 #define TRIBIT_COUNT_FORCE_TILES 8        /* every block launch streams 64-column bit tiles for owners whose
                                              set fits the LDS bitmap (test hook) */
 #define TRIBIT_COUNT_NO_BITMAP 16         /* test hook: block tasks keep hashed sets even for narrow id windows */
+#define TRIBIT_COUNT_NARROW 32            /* wedge ranges whose ids share their high 16 bits are streamed as
+                                             16-bit halves (dword pairs); off by default: slower on R-MAT */
 
 typedef struct {
   int32_t sched;           /* tribit_sched */
   int32_t repeats;         /* timed repetitions (>=1); the count of every repeat must agree */
   int32_t flags;           /* TRIBIT_COUNT_* bits */
-  int32_t group_width[4];  /* per kernel variant: lanes per streamed list (4,8,16,32,64; 0 = default) */
+  int32_t group_width[4];  /* per kernel variant: lanes per streamed list (wave variants 1..32, block
+                              variants 4..64, powers of two; 0 = default) */
   int32_t dense_core;      /* relabelled graphs: vertices of the top-id core counted on the matrix cores;
                               0 = chosen by the cost estimate for LEARNED, none otherwise; -1 = never */
 } tribit_count_opts;
@@ -106,6 +111,10 @@ typedef struct {
   int32_t group_width[8];  /* lanes per streamed list used by each variant */
   uint64_t dense_core;     /* core vertices counted by the int8 MFMA kernel (0: none) */
   double ms_dense;         /* device time of the core launch (median) */
+  int32_t candidate;       /* LEARNED: plan that ran (0 cost model, 1 heuristic, 2 heuristic on the
+                              directed layout, 3 textbook plain); -1 for the fixed schedules */
+  int32_t layout;          /* range layout that ran (TRIBIT_SCHED_PLAIN / DIRECTED / WEDGE) */
+  double ms_judge;         /* LEARNED: device time of the candidate trials this call ran (first call) */
 } tribit_count_info;
 
 /* Work assigned to each kernel variant by a scheduler, in the units of the

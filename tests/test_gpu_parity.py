@@ -190,6 +190,9 @@ def test_bitmap_and_hashed_sets(gpu_ctx, oracle_mod):
         a = g.count(sched="wedge").triangles
         for sched in scheds_of(g):
             assert g.count(sched=sched, flags=tribit_amd.NO_BITMAP).triangles == a
+            # 16-bit halves of narrow wedge ranges vs full ids
+            assert g.count(sched=sched, flags=tribit_amd.NARROW).triangles == a
+            assert g.count(sched=sched, flags=tribit_amd.NARROW | tribit_amd.NO_BITMAP).triangles == a
 
 
 def test_fallback_linear_probe(gpu_ctx, oracle_mod):
@@ -395,7 +398,7 @@ def test_bit_tiles(gpu_ctx, oracle_mod, spec):
                     assert g.count(sched=sched, flags=tribit_amd.FORCE_TILES).triangles == want, (on, sched)
                 if on:
                     tile_table = np.array([7 if tribit_amd.bucket_max_outdeg(b) > 256 else 0 if
-                                           tribit_amd.bucket_max_outdeg(b) <= 32 else 2
+                                           tribit_amd.bucket_max_outdeg(b) <= 32 else 3
                                            for b in range(tribit_amd.BUCKETS)], np.uint8)
                     assert tribit_amd.menu()[7][0] == 3
                     assert g.profile("wedge", tile_table).triangles == want

@@ -57,6 +57,7 @@ FORCE_GLOBAL_LOADS = 2
 NO_TILES = 4
 FORCE_TILES = 8
 NO_BITMAP = 16
+NARROW = 32
 OPT_RELABEL = 1
 OPT_BATCH_PART_KEYS = 2
 
@@ -68,6 +69,7 @@ class CountInfo(ctypes.Structure):
         ("tasks_by_variant", ctypes.c_uint64 * 8), ("repeats_agree", ctypes.c_int32), ("nvariants", ctypes.c_int32),
         ("ms_by_variant", ctypes.c_double * 8), ("fallback_tasks", ctypes.c_uint64),
         ("group_width", ctypes.c_int32 * 8), ("dense_core", ctypes.c_uint64), ("ms_dense", ctypes.c_double),
+        ("candidate", ctypes.c_int32), ("layout", ctypes.c_int32), ("ms_judge", ctypes.c_double),
     ]
 
 
@@ -258,6 +260,9 @@ class CountResult:
     group_width: tuple = ()
     dense_core: int = 0
     ms_dense: float = 0.0
+    candidate: int = -1
+    layout: int = 0
+    ms_judge: float = 0.0
 
 
 class Mlp:
@@ -448,7 +453,7 @@ class Graph:
         return CountResult(ci.triangles, ci.ms_count, ci.ms_sched, ci.ms_kernel, ci.launch_blocks, ci.tasks,
                            list(ci.tasks_by_variant)[:max(nv, 4)], bool(ci.repeats_agree),
                            list(ci.ms_by_variant)[:nv], ci.fallback_tasks, tuple(ci.group_width)[:nv],
-                           ci.dense_core, ci.ms_dense)
+                           ci.dense_core, ci.ms_dense, ci.candidate, ci.layout, ci.ms_judge)
 
     def variant_stats(self, sched: str = "plain") -> dict:
         vs = VariantStats()

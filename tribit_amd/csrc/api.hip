@@ -359,8 +359,9 @@ int tribit_count(tribit_ctx* ctx, tribit_graph* g, const tribit_count_opts* opts
 
   for (int i = 0; i < 4; ++i) {
     const int w = o.group_width[i];
-    if (w != 0 && w != 4 && w != 8 && w != 16 && w != 32 && w != 64) return TRIBIT_ERR_ARG;
+    if (w != 0 && w != 1 && w != 2 && w != 4 && w != 8 && w != 16 && w != 32 && w != 64) return TRIBIT_ERR_ARG;
     if (i <= 1 && w == 64) return TRIBIT_ERR_ARG;  // wave variants: at most 32 lanes per list
+    if (i >= 2 && w != 0 && w < 4) return TRIBIT_ERR_ARG;  // block variants: at least 4 lanes per list
   }
   return guarded(ctx, [&] {
     TB_HIP(hipSetDevice(ctx->device));
@@ -412,7 +413,10 @@ int tribit_mlp_create(tribit_ctx* ctx, int nfeat, uint64_t seed, tribit_mlp** ou
 int tribit_mlp_free(tribit_mlp* m) {
   if (!m) return TRIBIT_ERR_ARG;
   tribit_ctx* c = tb::mlp_context(m);
-  if (c && c->model == m) c->model = nullptr;
+  if (c && c->model == m) {
+    c->model = nullptr;
+    ++c->model_version;
+  }
   tb::mlp_free(m);
   return TRIBIT_OK;
 }
@@ -491,6 +495,7 @@ int tribit_set_cost_model(tribit_ctx* ctx, tribit_mlp* m) {
   if (!ctx) return TRIBIT_ERR_ARG;
   if (m && (tb::mlp_context(m) != ctx || tb::mlp_num_features(m) != TRIBIT_FEATURES)) return TRIBIT_ERR_ARG;
   ctx->model = m;
+  ++ctx->model_version;
   return TRIBIT_OK;
 }
 

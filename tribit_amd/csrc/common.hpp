@@ -63,6 +63,7 @@ struct tribit_ctx {
   hipEvent_t ev[16] = {};
   int relabel = 1;     // TRIBIT_OPT_RELABEL: degree-rank relabelling in preprocessing
   tribit_mlp* model = nullptr;  // cost model of the learned scheduler (not owned)
+  uint64_t model_version = 1;   // bumped whenever the model is set (judged choices go stale)
   uint64_t part_keys = 1ull << 30;  // TRIBIT_OPT_BATCH_PART_KEYS: input keys per batch part
   std::string last_error;
 
@@ -128,6 +129,10 @@ struct DevBuf {
   }
 };
 
+namespace tb {
+struct BucketCache;
+}
+
 // Count-ready graph resident in HBM.
 struct tribit_graph {
   tribit_ctx* ctx = nullptr;
@@ -153,6 +158,8 @@ struct tribit_graph {
   uint2* rng_sfx = nullptr;     // m
   uint32_t* sfx_ptr = nullptr;  // n+1, owner v's ranges in rng_sfx
   uint64_t* w_sfx = nullptr;    // n, streamed elements of v's ranges
+  uint32_t* colx = nullptr;     // [col | col16] for the narrow-range stream (built on first use, m < 2^30)
+  uint16_t* col16 = nullptr;    // inside colx: low 16 bits of every id (+ padding)
   // 64-column bit tiles (tiles.hip), built on first use
   uint64_t tiles64 = 0;          // tile count (computed by preprocessing)
   uint64_t ntiles = 0;           // tiles built (0 until built)
@@ -166,6 +173,13 @@ struct tribit_graph {
   // dense core (dense.hip): int8 adjacency of the top core_c ids, padded to core_kp
   int8_t* core_A = nullptr;
   uint64_t core_c = 0, core_kp = 0;
+  tb::BucketCache* bcache = nullptr;  // scheduler statistics (count.hip)
+  // the learned schedule's judged candidate (count.hip), valid for the model
+  // version, flags and core request it was judged under
+  int judge_choice = -1;
+  uint64_t judge_model = 0;
+  int judge_flags = 0, judge_core = 0;
+  double judge_ms[4] = {0, 0, 0, 0};
   double ms_prep = 0.0;
   uint64_t bytes = 0;
 };
@@ -205,6 +219,8 @@ namespace tb {
 // prep.hip
 void preprocess(tribit_ctx* ctx, uint64_t* d_keys, uint64_t nkeys, uint64_t n, tribit_graph* g);
 void free_graph_memory(tribit_graph* g);
+void bucket_cache_free(tribit_graph* g);
+void build_col16(tribit_ctx* ctx, tribit_graph* g);
 void free_graph(tribit_graph* g);
 void gen_rmat(tribit_ctx* ctx, uint32_t scale, uint32_t ef, double a, double b, double c, uint64_t seed,
               uint64_t* d_keys);

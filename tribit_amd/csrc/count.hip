@@ -47,7 +47,13 @@ constexpr uint32_t kChunkMask = (1u << kBucketShift) - 1;
 // same 2S slots.
 
 constexpr uint32_t kNoKey = 0xFFFFFFFEu;  // never a vertex id (n <= 2^32-2) and never kEmpty
+// Range entries (begin, end | flag): bit 31 of the end marks a range whose
+// ids share their high 16 bits (wedge ranges of graphs with m < 2^30); the
+// block kernel may stream such a range from the 16-bit low halves (col16).
+constexpr uint32_t kEndMask = 0x7FFFFFFFu;
+constexpr uint32_t kNarrowFlag = 0x80000000u;
 constexpr int kNoBitmapBit = 1 << 9;       // kernel mode bit: block tasks never use bitmap sets
+constexpr int kNarrowBit = 1 << 10;        // launch-time flag: narrow ranges streamed as 16-bit halves
 constexpr int kCuckooIters = 64;
 constexpr int kCuckooSeeds = 4;
 __device__ __constant__ uint32_t kSeedA[kCuckooSeeds] = {0x9E3779B1u, 0x85EBCA77u, 0xC2B2AE3Du, 0x27D4EB2Fu};
@@ -174,24 +180,34 @@ __device__ __forceinline__ uint32_t upper_bound_u32(const uint32_t* a, uint32_t 
 
 struct ColSrc {
   const uint32_t* col;
-  __amdgpu_buffer_rsrc_t rsrc;  // raw buffer over col (valid when col bytes < 2^32)
+  __amdgpu_buffer_rsrc_t rsrc;  // raw buffer over col (and, with col16, over col16 right after it)
+  uint32_t o16;                 // byte offset of col16 in that buffer (4 m)
 };
 
-__device__ __forceinline__ ColSrc make_col_src(const uint32_t* col, uint64_t m) {
+// col16, when built, lives in the same allocation right after col (the low
+// 16 bits of every id), so one buffer resource covers both.
+__device__ __forceinline__ ColSrc make_col_src(const uint32_t* col, uint64_t m, bool with16 = false) {
   ColSrc c;
   c.col = col;
-  const uint64_t bytes = m * 4 < 0xFFFFFFFFull ? m * 4 : 0xFFFFFFFFull;
+  const uint64_t want = with16 ? m * 4 + ((m + 3) / 2) * 4 : m * 4;
+  const uint64_t bytes = want < 0xFFFFFFFFull ? want : 0xFFFFFFFFull;
   c.rsrc = __builtin_amdgcn_make_buffer_rsrc(const_cast<uint32_t*>(col), (short)0, (int)(uint32_t)bytes, 0x00020000);
+  c.o16 = (uint32_t)(m * 4);
   return c;
 }
 
-template <int G, int U, bool SHARED, bool BUF, typename Probe>
+// N16: a range flagged narrow is streamed from col16 as dword pairs: each
+// load returns two consecutive ids' low halves (the high half of the range,
+// staged in Rh, is or-ed back), so a narrow group covers 2 G U ids per step
+// and every lane still issues one 32-bit load per slot.
+template <int G, int U, bool SHARED, bool BUF, typename Probe, bool N16 = false>
 __device__ __forceinline__ uint32_t stream_ranges(const ColSrc& src, const uint2* R, uint32_t nk, uint32_t* s_next,
-                                                  uint32_t lane, const Probe& probe) {
+                                                  uint32_t lane, const Probe& probe, const uint16_t* Rh = nullptr) {
   constexpr uint64_t kLeads = leader_mask<G>();
   const uint32_t g = lane / G, gl = lane % G;
   const uint64_t below = (g == 0) ? 0ull : ((1ull << (g * G)) - 1);
-  uint32_t base = 0, end = 0, cnt = 0, next = 0;
+  uint32_t base = 0, end = 0, cnt = 0, next = 0, hi = 0, lo = 0;
+  bool f16 = false;
   for (;;) {
     const bool need = base >= end;  // uniform within a group
     const uint64_t want = __ballot(need && gl == 0);
@@ -212,8 +228,37 @@ __device__ __forceinline__ uint32_t stream_ranges(const ColSrc& src, const uint2
       if (k < nk) {
         const uint2 r = R[k];
         base = r.x;
-        end = r.y;
+        end = r.y & kEndMask;
+        if (N16) {
+          f16 = (r.y & kNarrowFlag) != 0;
+          hi = (uint32_t)Rh[k] << 16;
+          lo = base;
+          if (f16) base &= ~1u;  // pairs start at even ids
+        }
       }
+    }
+    if (N16 && BUF) {
+      // one dword per slot: an id (wide ranges) or two low halves (narrow ranges)
+      const uint32_t e0 = base + (f16 ? 2 * gl : gl);
+      const uint32_t step = f16 ? 2 * G : G;
+      const uint32_t ob = f16 ? src.o16 + 2 * e0 : 4 * e0, ostep = f16 ? 4 * G : 4 * G;
+      uint32_t w[U];
+#pragma unroll
+      for (int j = 0; j < U; ++j) {  // all U loads in flight before the first probe
+        const uint32_t e = e0 + j * step;
+        const bool any = (e >= lo && e < end) || (f16 && e + 1 < end);
+        w[j] = __builtin_amdgcn_raw_buffer_load_b32(src.rsrc, (int)(any ? ob + j * ostep : 0xFFFFFFFFu), 0, 0);
+      }
+#pragma unroll
+      for (int j = 0; j < U; ++j) {
+        const uint32_t e = e0 + j * step;
+        const bool ok0 = e >= lo && e < end, ok1 = f16 && e + 1 < end;
+        const uint32_t k0 = ok0 ? (f16 ? (hi | (w[j] & 0xFFFFu)) : w[j]) : kNoKey;
+        const uint32_t k1 = ok1 ? (hi | (w[j] >> 16)) : kNoKey;
+        cnt += probe(k0) + probe(k1);
+      }
+      base += (f16 ? 2u : 1u) * (uint32_t)(G * U);
+      continue;
     }
     const uint32_t p0 = base + gl;
     const int rem = (int)(end - p0);  // elements left for this lane's first load (may be <= 0)
@@ -272,7 +317,7 @@ __device__ __forceinline__ uint32_t stream_tiles(const TileSrc& ts, const uint2*
       if (k < nk) {
         const uint2 r = R[k];
         base_r = r.x;
-        end = r.y;
+        end = r.y;  // tile ranges carry no flag
       }
     }
     const uint32_t p0 = base_r + gl;
@@ -379,9 +424,10 @@ __global__ __launch_bounds__(256) void k_count_wave(const uint32_t* __restrict__
 
 constexpr int kBlockThreads = 512;
 
-template <int LOGS, int G, int U, bool BUF, bool TILES>
+template <int LOGS, int G, int U, bool BUF, bool TILES, bool N16>
 __global__ __launch_bounds__(kBlockThreads) void k_count_block(const uint32_t* __restrict__ row_ptr,
                                                                const uint32_t* __restrict__ col, uint64_t m,
+                                                               const uint16_t* __restrict__ col16,
                                                                const uint2* __restrict__ rng, TileSrc ts,
                                                                const uint4* __restrict__ tasks, uint32_t ntasks,
                                                                uint32_t* __restrict__ queue,
@@ -395,10 +441,11 @@ __global__ __launch_bounds__(kBlockThreads) void k_count_block(const uint32_t* _
   const bool bitmaps = !(mode & kNoBitmapBit) && attempts > 0;
   __shared__ uint32_t T[2 * S];
   __shared__ uint2 R[kChunk];
+  __shared__ uint16_t Rh[N16 ? kChunk : 1];  // high halves of narrow ranges
   __shared__ uint32_t s_task, s_next, s_fail;
   __shared__ unsigned long long s_part[kBlockThreads / 64];
   const uint32_t lane = threadIdx.x & 63;
-  const ColSrc src = make_col_src(col, m);
+  const ColSrc src = make_col_src(col, m, N16);
   uint64_t acc = 0;
   uint32_t fallbacks = 0;
   for (;;) {
@@ -453,9 +500,15 @@ __global__ __launch_bounds__(kBlockThreads) void k_count_block(const uint32_t* _
       const uint32_t nk = min((uint32_t)kChunk, rcount - c0);
       for (uint32_t k = threadIdx.x; k < nk; k += kBlockThreads) {
         uint2 r = rs[rbase + c0 + k];
-        if (ranged && !tiles) {  // tiles outside the chunk's window miss in the bitmap
-          r.x = lower_bound_u32(col, r.x, r.y, lo);
-          r.y = upper_bound_u32(col, r.x, r.y, hi);
+        if (!tiles) {
+          const uint32_t fl = r.y & kNarrowFlag;
+          uint32_t e = r.y & kEndMask;
+          if (N16 && fl) Rh[k] = (uint16_t)(col[r.x] >> 16);  // flagged ranges are non-empty
+          if (ranged) {  // tiles outside the chunk's window miss in the bitmap
+            r.x = lower_bound_u32(col, r.x, e, lo);
+            e = upper_bound_u32(col, r.x, e, hi);
+          }
+          r.y = e | (N16 ? fl : 0u);
         }
         R[k] = r;
       }
@@ -464,11 +517,14 @@ __global__ __launch_bounds__(kBlockThreads) void k_count_block(const uint32_t* _
       if (tiles)
         cnt += stream_tiles<G, U>(ts, R, nk, &s_next, lane, T, base, kSpan);
       else if (bitmap)
-        cnt += stream_ranges<G, U, true, BUF>(src, R, nk, &s_next, lane, BitmapProbe<LOGS + 1>{T, base});
+        cnt += stream_ranges<G, U, true, BUF, BitmapProbe<LOGS + 1>, N16>(src, R, nk, &s_next, lane,
+                                                                          BitmapProbe<LOGS + 1>{T, base}, Rh);
       else if (ok)
-        cnt += stream_ranges<G, U, true, BUF>(src, R, nk, &s_next, lane, CuckooProbe<LOGS>{T, sa});
+        cnt += stream_ranges<G, U, true, BUF, CuckooProbe<LOGS>, N16>(src, R, nk, &s_next, lane,
+                                                                      CuckooProbe<LOGS>{T, sa}, Rh);
       else
-        cnt += stream_ranges<G, U, true, BUF>(src, R, nk, &s_next, lane, LinearProbe<LOGS + 1>{T});
+        cnt += stream_ranges<G, U, true, BUF, LinearProbe<LOGS + 1>, N16>(src, R, nk, &s_next, lane,
+                                                                          LinearProbe<LOGS + 1>{T}, Rh);
       __syncthreads();
     }
     acc += cnt;
@@ -765,9 +821,9 @@ static const uint2* sched_ranges(const tribit_graph* g, int sched) {
 
 // ------------------------------------------------------------------ plans
 // The learned scheduler's menu: (kind, lane-group width) per slot.
-static const int kMenuKind[kSlots] = {kKindWave64, kKindWave64, kKindWave512, kKindWave512,
-                                      kKindBlock,  kKindBlock,  kKindBlock,   kKindTile};
-static const int kMenuG[kSlots] = {4, 16, 8, 16, 8, 16, 32, 8};
+static const int kMenuKind[kSlots] = {kKindWave64,  kKindWave64, kKindWave512, kKindWave512,
+                                      kKindWave512, kKindBlock,  kKindBlock,   kKindTile};
+static const int kMenuG[kSlots] = {1, 4, 2, 4, 8, 8, 16, 8};
 
 static int kind_of_dbin(int db) {
   const uint32_t top = d_bin_top(db);
@@ -809,22 +865,43 @@ static Plan menu_plan() {
 static void heuristic_table(uint8_t* table) {
   for (int b = 0; b < kBuckets; ++b) {
     const int kind = kind_of_dbin(b / kLBins);
-    table[b] = (uint8_t)(kind == kKindWave64 ? 0 : kind == kKindWave512 ? 2 : 5);
+    table[b] = (uint8_t)(kind == kKindWave64 ? 0 : kind == kKindWave512 ? 3 : 6);
   }
 }
 
-// Default lane-group widths per variant and schedule.
-// (swept on R-MAT scale 22 and 25 over 4..32 / 8..64 lanes per list)
+// Default lane-group widths per variant and schedule (swept on R-MAT scale
+// 22 and 25 over 1..32 lanes per list for wave tasks, 8..64 for block tasks).
+// Wedge lists of small owners are short: one lane per list keeps 64 lists in
+// flight per wave.
 static void default_widths(int sched, int gw[4]) {
   if (sched == TRIBIT_SCHED_PLAIN) {
     gw[0] = 32, gw[1] = 16, gw[2] = 32, gw[3] = 16;
+  } else if (sched == TRIBIT_SCHED_WEDGE) {
+    gw[0] = 1, gw[1] = 4, gw[2] = 16, gw[3] = 8;
   } else {
-    gw[0] = 4, gw[1] = 8, gw[2] = 16, gw[3] = 16;
+    gw[0] = 1, gw[1] = 8, gw[2] = 16, gw[3] = 16;
   }
+}
+
+// Bucket statistics depend only on the graph, the range layout and the core:
+// computed once per graph and kept (the last combination asked for).
+struct BucketCache {
+  int layout;
+  uint32_t core_d0;
+  BucketStats st;
+};
+
+void bucket_cache_free(tribit_graph* g) {
+  delete g->bcache;
+  g->bcache = nullptr;
 }
 
 void bucket_stats(tribit_ctx* ctx, tribit_graph* g, int sched, BucketStats* out, uint32_t core_d0 = 0xFFFFFFFFu,
                   uint32_t core_cut = 0) {
+  if (g->bcache && g->bcache->layout == sched && g->bcache->core_d0 == core_d0) {
+    *out = g->bcache->st;
+    return;
+  }
   memset(out, 0, sizeof(*out));
   DevBuf<unsigned long long> st(ctx, kStat * kBuckets);
   k_zero_u64<<<grid_for(kStat * kBuckets, kBlock), kBlock, 0, ctx->stream>>>(st.p, kStat * kBuckets);
@@ -842,6 +919,10 @@ void bucket_stats(tribit_ctx* ctx, tribit_graph* g, int sched, BucketStats* out,
     out->eligible[b] = h[4 * kBuckets + b];
     out->sum_t[b] = h[5 * kBuckets + b];
   }
+  if (!g->bcache) g->bcache = new BucketCache;
+  g->bcache->layout = sched;
+  g->bcache->core_d0 = core_d0;
+  g->bcache->st = *out;
 }
 
 // Feature vector of (bucket, menu slot); the same function builds training
@@ -978,6 +1059,8 @@ static void launch_wave(const LaunchArgs& a, int slot, int gw, uint32_t ntasks, 
                         tribit_graph* g, const uint2* rng, const uint4* tasks, uint64_t* blocks) {
   if (!ntasks) return;
   switch (gw) {
+    case 1: launch_wave_g<LOGS, KMAX, 1>(a, slot, ntasks, cus, s, g, rng, tasks, blocks); break;
+    case 2: launch_wave_g<LOGS, KMAX, 2>(a, slot, ntasks, cus, s, g, rng, tasks, blocks); break;
     case 4: launch_wave_g<LOGS, KMAX, 4>(a, slot, ntasks, cus, s, g, rng, tasks, blocks); break;
     case 8: launch_wave_g<LOGS, KMAX, 8>(a, slot, ntasks, cus, s, g, rng, tasks, blocks); break;
     case 32: launch_wave_g<LOGS, KMAX, 32>(a, slot, ntasks, cus, s, g, rng, tasks, blocks); break;
@@ -985,17 +1068,18 @@ static void launch_wave(const LaunchArgs& a, int slot, int gw, uint32_t ntasks, 
   }
 }
 
-template <int LOGS, int G, int U, bool BUF, bool TILES>
+template <int LOGS, int G, int U, bool BUF, bool TILES, bool N16>
 static void launch_block_t(const LaunchArgs& a, int slot, uint32_t ntasks, int cus, hipStream_t s, tribit_graph* g,
                            const uint2* rng, const TileSrc& ts, const uint4* tasks, uint64_t* blocks) {
   int per_cu = 0;
-  TB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_count_block<LOGS, G, U, BUF, TILES>, kBlockThreads,
-                                                      0));
+  TB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_count_block<LOGS, G, U, BUF, TILES, N16>,
+                                                      kBlockThreads, 0));
   if (a.waves[slot] > 0 && per_cu > a.waves[slot]) per_cu = a.waves[slot];
   if (per_cu < 1) per_cu = 1;
   const unsigned grid = (unsigned)std::min<uint64_t>(ntasks, (uint64_t)cus * per_cu);
-  k_count_block<LOGS, G, U, BUF, TILES><<<grid, kBlockThreads, 0, s>>>(
-      g->row_ptr, g->col, g->m, rng, ts, tasks, ntasks, a.queue + slot, a.result, a.att & ~kForceGlobal,
+  k_count_block<LOGS, G, U, BUF, TILES, N16><<<grid, kBlockThreads, 0, s>>>(
+      g->row_ptr, N16 ? g->colx : g->col, g->m, g->col16, rng, ts, tasks, ntasks, a.queue + slot, a.result,
+      a.att & ~kForceGlobal,
       a.tlog ? a.tlog + (size_t)slot * kBuckets : nullptr, a.gid, a.gcount);
   *blocks += grid;
 }
@@ -1004,15 +1088,22 @@ template <int LOGS, int G>
 static void launch_block_g(const LaunchArgs& a, int slot, uint32_t ntasks, int cus, hipStream_t s, tribit_graph* g,
                            const uint2* rng, const TileSrc& ts, const uint4* tasks, uint64_t* blocks) {
   const bool buf = g->m * 4 < 0xFFFFFFFFull && !(a.att & kForceGlobal);
+  // narrow ranges exist only in the wedge layout; opt-in (measured slower than
+  // 32-bit ids on R-MAT: the block kernel is not bound by HBM bytes alone)
+  const bool n16 = buf && g->colx && rng == g->rng_sfx && (a.att & kNarrowBit);
   if (ts.trng) {
-    if (buf)
-      launch_block_t<LOGS, G, kU, true, true>(a, slot, ntasks, cus, s, g, rng, ts, tasks, blocks);
+    if (n16)
+      launch_block_t<LOGS, G, kU, true, true, true>(a, slot, ntasks, cus, s, g, rng, ts, tasks, blocks);
+    else if (buf)
+      launch_block_t<LOGS, G, kU, true, true, false>(a, slot, ntasks, cus, s, g, rng, ts, tasks, blocks);
     else
-      launch_block_t<LOGS, G, kU, false, true>(a, slot, ntasks, cus, s, g, rng, ts, tasks, blocks);
+      launch_block_t<LOGS, G, kU, false, true, false>(a, slot, ntasks, cus, s, g, rng, ts, tasks, blocks);
+  } else if (n16) {
+    launch_block_t<LOGS, G, kU, true, false, true>(a, slot, ntasks, cus, s, g, rng, ts, tasks, blocks);
   } else if (buf) {
-    launch_block_t<LOGS, G, kU, true, false>(a, slot, ntasks, cus, s, g, rng, ts, tasks, blocks);
+    launch_block_t<LOGS, G, kU, true, false, false>(a, slot, ntasks, cus, s, g, rng, ts, tasks, blocks);
   } else {
-    launch_block_t<LOGS, G, kU, false, false>(a, slot, ntasks, cus, s, g, rng, ts, tasks, blocks);
+    launch_block_t<LOGS, G, kU, false, false, false>(a, slot, ntasks, cus, s, g, rng, ts, tasks, blocks);
   }
 }
 
@@ -1038,7 +1129,7 @@ static float elapsed(hipEvent_t a, hipEvent_t b) {
 static int mode_word(int flags) {
   return ((flags & TRIBIT_COUNT_FORCE_LINEAR_PROBE) ? 0 : kCuckooSeeds) |
          ((flags & TRIBIT_COUNT_FORCE_GLOBAL_LOADS) ? kForceGlobal : 0) |
-         ((flags & TRIBIT_COUNT_NO_BITMAP) ? kNoBitmapBit : 0);
+         ((flags & TRIBIT_COUNT_NO_BITMAP) ? kNoBitmapBit : 0) | ((flags & TRIBIT_COUNT_NARROW) ? kNarrowBit : 0);
 }
 
 // Tasks of one count under a plan, ready to launch.
@@ -1085,6 +1176,7 @@ void make_schedule(tribit_ctx* ctx, tribit_graph* g, int layout, int flags, bool
     if ((flags & TRIBIT_COUNT_FORCE_TILES) && plan.kind[i] == kKindBlock) plan.kind[i] = kKindTile;
     if ((flags & TRIBIT_COUNT_NO_TILES) && plan.kind[i] == kKindTile) plan.kind[i] = kKindBlock;
   }
+  if ((flags & TRIBIT_COUNT_NARROW) && layout == TRIBIT_SCHED_WEDGE) build_col16(ctx, g);
   bool any_tiles = false;
   for (int i = 0; i < plan.nslots; ++i) any_tiles |= plan.kind[i] == kKindTile;
   if (any_tiles && g->m) {
@@ -1223,6 +1315,49 @@ static Plan plan_for(int sched, const int32_t* override_gw, int* layout) {
   return static_plan(gw);
 }
 
+// LEARNED runs one of these candidates; the first count of a graph (or after
+// the model changes) times each once and keeps the fastest, so the heuristic
+// plans judge the model's plan by measurement:
+//   0: the cost model's plan on the preferred layout (+ dense core)
+//   1: the density-threshold heuristic on the preferred layout (+ dense core)
+//   2: the heuristic on the DIRECTED layout (+ dense core), for relabelled graphs
+//   3: the textbook PLAIN plan
+// (preferred layout: WEDGE for relabelled graphs, else DIRECTED).
+enum { kCandModel = 0, kCandHeur = 1, kCandDirected = 2, kCandPlain = 3, kCands = 4 };
+
+struct CandidateRun {
+  int layout;
+  Plan plan;
+  bool model;
+  uint64_t core_c;
+};
+
+static bool candidate(tribit_ctx* ctx, tribit_graph* g, const tribit_count_opts& opts, int c, CandidateRun* out) {
+  const int pref = g->rng_sfx ? TRIBIT_SCHED_WEDGE : TRIBIT_SCHED_DIRECTED;
+  uint64_t core = 0;
+  if (opts.dense_core > 0)
+    core = std::min<uint64_t>((uint64_t)opts.dense_core, g->n);
+  else if (opts.dense_core == 0 && g->relabeled)
+    core = core_choose(ctx, g);
+  int dummy = 0;
+  switch (c) {
+    case kCandModel:
+      if (!ctx->model || pref != TRIBIT_SCHED_WEDGE) return false;  // the model is trained on wedge timings
+      *out = {pref, plan_for(TRIBIT_SCHED_LEARNED, nullptr, &dummy), true, core};
+      return true;
+    case kCandHeur:
+      *out = {pref, plan_for(TRIBIT_SCHED_LEARNED, nullptr, &dummy), false, core};
+      return true;
+    case kCandDirected:
+      if (pref != TRIBIT_SCHED_WEDGE) return false;  // already candidate 1
+      *out = {TRIBIT_SCHED_DIRECTED, plan_for(TRIBIT_SCHED_LEARNED, nullptr, &dummy), false, core};
+      return true;
+    default:
+      *out = {TRIBIT_SCHED_PLAIN, plan_for(TRIBIT_SCHED_PLAIN, nullptr, &dummy), false, 0};
+      return true;
+  }
+}
+
 void count(tribit_ctx* ctx, tribit_graph* g, const tribit_count_opts& opts, tribit_count_info* info) {
   const int reps = opts.repeats > 0 ? opts.repeats : 1;
   std::vector<double> tt, ts, tk, tv[kSlots];
@@ -1230,23 +1365,60 @@ void count(tribit_ctx* ctx, tribit_graph* g, const tribit_count_opts& opts, trib
   bool agree = true;
   tribit_count_info local;
   memset(&local, 0, sizeof(local));
-  int layout = 0;
-  Plan plan = plan_for(opts.sched, opts.group_width, &layout);
   const bool learned = opts.sched == TRIBIT_SCHED_LEARNED;
-  if (learned) layout = g->rng_sfx ? TRIBIT_SCHED_WEDGE : TRIBIT_SCHED_DIRECTED;
-  // dense core: LEARNED picks its size by the cost estimate, the fixed
-  // schedules only use one when asked for (opts.dense_core > 0)
-  uint64_t core_c = 0;
-  if (opts.dense_core > 0)
-    core_c = std::min<uint64_t>((uint64_t)opts.dense_core, g->n);
-  else if (opts.dense_core == 0 && learned)
-    core_c = core_choose(ctx, g);
-  if (!g->relabeled) core_c = 0;
+  CandidateRun run;
+  int chosen = -1;
+  double ms_judge = 0.0;
+  if (learned) {
+    const bool stale = g->judge_choice < 0 || g->judge_model != ctx->model_version ||
+                       g->judge_flags != opts.flags || g->judge_core != opts.dense_core;
+    if (stale) {
+      double best = INFINITY;
+      uint64_t ref = 0;
+      bool have = false;
+      for (int c = 0; c < kCands; ++c) {
+        CandidateRun cr;
+        if (!candidate(ctx, g, opts, c, &cr)) continue;
+        // twice: the first run also builds what the candidate caches (core
+        // matrix, bucket statistics, tiles); the second is the one compared
+        double a = 0.0, b, k, v[kSlots] = {0};
+        for (int rep = 0; rep < 2; ++rep) {
+          Plan pl = cr.plan;
+          const uint64_t tri = count_once(ctx, g, cr.layout, opts.flags, pl, cr.model, &a, &b, &k, v, nullptr,
+                                          nullptr, cr.core_c, nullptr);
+          if (have && tri != ref) fail(TRIBIT_ERR_HIP, "learned candidates disagree");
+          ref = tri;
+          have = true;
+          ms_judge += a;
+        }
+        g->judge_ms[c] = a;
+        if (a < best) {
+          best = a;
+          chosen = c;
+        }
+      }
+      g->judge_choice = chosen;
+      g->judge_model = ctx->model_version;
+      g->judge_flags = opts.flags;
+      g->judge_core = opts.dense_core;
+    }
+    chosen = g->judge_choice;
+    candidate(ctx, g, opts, chosen, &run);
+  } else {
+    int layout = 0;
+    run.plan = plan_for(opts.sched, opts.group_width, &layout);
+    run.layout = layout;
+    run.model = false;
+    // fixed schedules use a dense core only when asked for (opts.dense_core > 0)
+    run.core_c = (opts.dense_core > 0 && g->relabeled) ? std::min<uint64_t>((uint64_t)opts.dense_core, g->n) : 0;
+  }
   std::vector<double> tcore;
   for (int r = 0; r < reps; ++r) {
     double a, b, c, v[kSlots] = {0}, mc = 0.0;
-    const uint64_t tri =
-        count_once(ctx, g, layout, opts.flags, plan, learned, &a, &b, &c, v, &local, nullptr, core_c, &mc);
+    Plan plan = run.plan;
+    const uint64_t tri = count_once(ctx, g, run.layout, opts.flags, plan, run.model, &a, &b, &c, v, &local, nullptr,
+                                    run.core_c, &mc);
+    if (r == reps - 1) run.plan = plan;
     tcore.push_back(mc);
     if (r == 0)
       first = tri;
@@ -1261,9 +1433,12 @@ void count(tribit_ctx* ctx, tribit_graph* g, const tribit_count_opts& opts, trib
   local.ms_count = median(tt);
   local.ms_sched = median(ts);
   local.ms_kernel = median(tk);
-  for (int i = 0; i < kSlots; ++i) local.ms_by_variant[i] = i < plan.nslots ? median(tv[i]) : 0.0;
+  for (int i = 0; i < kSlots; ++i) local.ms_by_variant[i] = i < run.plan.nslots ? median(tv[i]) : 0.0;
   local.repeats_agree = agree ? 1 : 0;
   local.ms_dense = median(tcore);
+  local.candidate = learned ? chosen : -1;
+  local.layout = run.layout;
+  local.ms_judge = ms_judge;
   if (info) *info = local;
 }
 
@@ -1314,14 +1489,36 @@ void learned_plan(tribit_ctx* ctx, tribit_graph* g, uint8_t* table) {
 }
 
 // Per-slot work of a schedule's plan (LEARNED: the table the model picks).
+// Per-slot work of a schedule's plan. LEARNED: the judged candidate (the
+// heuristic before the first count), streamed work outside its dense core.
 void variant_stats(tribit_ctx* ctx, tribit_graph* g, int sched, tribit_variant_work* out) {
   memset(out, 0, sizeof(*out));
   int layout = 0;
   Plan plan = plan_for(sched, nullptr, &layout);
-  if (sched == TRIBIT_SCHED_LEARNED) layout = g->rng_sfx ? TRIBIT_SCHED_WEDGE : TRIBIT_SCHED_DIRECTED;
   BucketStats st;
-  bucket_stats(ctx, g, layout, &st);
-  if (sched == TRIBIT_SCHED_LEARNED) learned_table(ctx, st, layout, 0, plan.table);
+  if (sched == TRIBIT_SCHED_LEARNED) {
+    tribit_count_opts o;
+    memset(&o, 0, sizeof(o));
+    o.sched = TRIBIT_SCHED_LEARNED;
+    o.flags = g->judge_choice >= 0 ? g->judge_flags : 0;
+    o.dense_core = g->judge_choice >= 0 ? g->judge_core : 0;
+    CandidateRun cr;
+    candidate(ctx, g, o, g->judge_choice >= 0 ? g->judge_choice : kCandHeur, &cr);
+    layout = cr.layout;
+    plan = cr.plan;
+    uint32_t d0 = 0xFFFFFFFFu, cut = 0;
+    if (cr.core_c) {
+      d0 = (uint32_t)(g->n - cr.core_c);
+      uint32_t e0 = 0;
+      TB_HIP(hipMemcpyAsync(&e0, g->row_ptr + d0, 4, hipMemcpyDeviceToHost, ctx->stream));
+      ctx->sync();
+      cut = layout == TRIBIT_SCHED_WEDGE ? e0 + 1 : e0;
+    }
+    bucket_stats(ctx, g, layout, &st, d0, cut);
+    if (cr.model) learned_table(ctx, st, layout, 0, plan.table);
+  } else {
+    bucket_stats(ctx, g, layout, &st);
+  }
   for (int b = 0; b < kBuckets; ++b) {
     const int s = plan.table[b];
     out->rows[s] += st.lists[b];

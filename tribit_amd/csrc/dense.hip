@@ -22,7 +22,7 @@
 namespace tb {
 
 constexpr int kCoreTile = 128;  // output tile side
-constexpr int kCoreK = 64;      // K bytes per LDS stage
+constexpr int kCoreK = 128;     // K bytes per LDS stage
 constexpr int kCorePad = 16;    // LDS row padding (bytes)
 constexpr int kCoreThreads = 256;
 
@@ -69,16 +69,30 @@ __global__ __launch_bounds__(kCoreThreads) void k_core_count(const int8_t* __res
     for (int j = 0; j < 2; ++j)
 #pragma unroll
       for (int v = 0; v < 16; ++v) acc[i][j][v] = 0;
-  const uint32_t lr = tid >> 1, lh = (tid & 1) * 32;  // staging: row, 32-byte half
-  for (uint32_t k0 = J * kCoreTile; k0 < kp; k0 += kCoreK) {
-    const uint4* ga = reinterpret_cast<const uint4*>(A + (rowA + lr) * kp + k0 + lh);
-    const uint4* gb = reinterpret_cast<const uint4*>(A + (rowB + lr) * kp + k0 + lh);
-    const uint4 a0 = ga[0], a1 = ga[1], b0 = gb[0], b1 = gb[1];
-    *reinterpret_cast<uint4*>(&sA[lr][lh]) = a0;
-    *reinterpret_cast<uint4*>(&sA[lr][lh + 16]) = a1;
-    *reinterpret_cast<uint4*>(&sB[lr][lh]) = b0;
-    *reinterpret_cast<uint4*>(&sB[lr][lh + 16]) = b1;
+  // staging: thread tid moves 64 bytes of row tid / 2 of each operand per
+  // stage; the next stage's loads are in flight while the MFMAs run
+  const uint32_t lr = tid >> 1, lh = (tid & 1) * 64;
+  const uint32_t k_begin = J * kCoreTile;
+  const uint4* ga = reinterpret_cast<const uint4*>(A + (rowA + lr) * kp + k_begin + lh);
+  const uint4* gb = reinterpret_cast<const uint4*>(A + (rowB + lr) * kp + k_begin + lh);
+  uint4 pa0 = ga[0], pa1 = ga[1], pa2 = ga[2], pa3 = ga[3];
+  uint4 pb0 = gb[0], pb1 = gb[1], pb2 = gb[2], pb3 = gb[3];
+  for (uint32_t k0 = k_begin; k0 < kp; k0 += kCoreK) {
+    *reinterpret_cast<uint4*>(&sA[lr][lh]) = pa0;
+    *reinterpret_cast<uint4*>(&sA[lr][lh + 16]) = pa1;
+    *reinterpret_cast<uint4*>(&sA[lr][lh + 32]) = pa2;
+    *reinterpret_cast<uint4*>(&sA[lr][lh + 48]) = pa3;
+    *reinterpret_cast<uint4*>(&sB[lr][lh]) = pb0;
+    *reinterpret_cast<uint4*>(&sB[lr][lh + 16]) = pb1;
+    *reinterpret_cast<uint4*>(&sB[lr][lh + 32]) = pb2;
+    *reinterpret_cast<uint4*>(&sB[lr][lh + 48]) = pb3;
     __syncthreads();
+    if (k0 + kCoreK < kp) {  // next stage in flight during the MFMAs
+      ga += kCoreK / 16;
+      gb += kCoreK / 16;
+      pa0 = ga[0], pa1 = ga[1], pa2 = ga[2], pa3 = ga[3];
+      pb0 = gb[0], pb1 = gb[1], pb2 = gb[2], pb3 = gb[3];
+    }
 #pragma unroll
     for (int kk = 0; kk < kCoreK; kk += 32) {
       const int ko = kk + (lane >> 5) * 16;  // the same k slice for both operands
@@ -164,15 +178,14 @@ double core_macs(uint64_t kp) {
   return macs;
 }
 
-// Bytes the core launch stages (both operands of every tile, all K stages).
-double core_stage_bytes(uint64_t kp) { return core_macs(kp) * 2.0 / kCoreTile; }
+static double kCoreMacPerMs = 5.5e11;
 
 // Core size chosen for a relabelled graph: the candidate (4096 .. 65536
 // ids, at most n) with the largest estimated saving
-//     streamed wedges of the core / wedge rate  -  (MACs / MFMA rate + staged bytes / HBM rate)
-// if that saving exceeds 0.5 ms; 0 = no core. The rates are measured
-// figures of this build on MI355X (streaming kernels ~2e12 wedge checks/s,
-// core kernel ~1.2e15 int8 MAC/s and ~6 TB/s staging).
+//     streamed wedges of the core / wedge rate  -  MACs / MAC rate
+// if that saving exceeds 0.5 ms; 0 = no core. The rates are the kernels'
+// measured figures on MI355X (streaming kernels ~2e12 wedge checks/s, the
+// core kernel ~5.5e14 int8 MAC/s including its staging).
 uint64_t core_choose(tribit_ctx* ctx, const tribit_graph* g) {
   if (!g->relabeled || g->n < 4096 || !g->m) return 0;
   const uint64_t top = std::min<uint64_t>(g->n, 65536);
@@ -189,7 +202,7 @@ uint64_t core_choose(tribit_ctx* ctx, const tribit_graph* g) {
     }
     const uint64_t kp = (c + kCoreTile - 1) / kCoreTile * kCoreTile;
     const double sparse_ms = wedges / 2.0e9;
-    const double dense_ms = core_macs(kp) / 1.2e12 + core_stage_bytes(kp) / 6.0e9;
+    const double dense_ms = core_macs(kp) / kCoreMacPerMs;
     if (sparse_ms - dense_ms > best_gain) {
       best_gain = sparse_ms - dense_ms;
       best = c;

@@ -328,6 +328,20 @@ __global__ void k_sfx_fill(const uint32_t* __restrict__ head, const uint32_t* __
   if (tail && sum) atomicAdd(&w_sfx[v], (unsigned long long)sum);
 }
 
+// Low 16 bits of every id, and the narrow flag (bit 31 of the end) on every
+// non-empty range whose ids share their high 16 bits.
+__global__ void k_col16(const uint32_t* __restrict__ col, uint64_t m, uint16_t* __restrict__ col16) {
+  const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < m) col16[e] = (uint16_t)(col[e] & 0xFFFFu);
+}
+
+__global__ void k_flag_narrow(uint2* __restrict__ rng, uint64_t m, const uint32_t* __restrict__ col) {
+  const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= m) return;
+  const uint2 r = rng[k];
+  if (r.x < r.y && (col[r.x] >> 16) == (col[r.y - 1] >> 16)) rng[k].y = r.y | 0x80000000u;
+}
+
 __global__ void k_fill_u64(unsigned long long* __restrict__ p, uint64_t n) {
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = 0ull;
@@ -523,6 +537,10 @@ void preprocess(tribit_ctx* ctx, uint64_t* d_keys_in, uint64_t nkeys, uint64_t n
     });
   }
   rend.reset();
+  // 10. narrow wedge ranges (ids sharing their high 16 bits) are flagged; the
+  // 16-bit copy of col they can be streamed from is built on first use
+  if (ctx->relabel && m && m < (1ull << 30))
+    k_flag_narrow<<<grid_for(m, kBlock), kBlock, 0, s>>>(rng_sfx.p, m, col.p);
 
   // the graph owns its buffers from here on (freed by tribit_graph_free, also on error)
   g->row_ptr = row_ptr.release();
@@ -540,6 +558,7 @@ void preprocess(tribit_ctx* ctx, uint64_t* d_keys_in, uint64_t nkeys, uint64_t n
   g->w_fw = w_fw.release();
   g->w_rv = w_rv.release();
   g->rng_sfx = rng_sfx.release();
+
   g->sfx_ptr = sfx_ptr.release();
   g->w_sfx = w_sfx.release();
   g->bytes = 4 * (n + 1) * 3 + 4 * (m ? m : 1) + 20 * (m ? m : 1) + 4 * n + 8 * n * 3 + (g->labels ? 4 * n : 0) +
@@ -556,17 +575,38 @@ void preprocess(tribit_ctx* ctx, uint64_t* d_keys_in, uint64_t nkeys, uint64_t n
   g->max_outdeg = hst.max_outdeg;
 }
 
+// Combined copy [col | col16] for the narrow-range stream (built on first
+// use): col16 holds the low 16 bits of every id, padded to whole dwords so a
+// pair read at the last even position stays inside the buffer.
+void build_col16(tribit_ctx* ctx, tribit_graph* g) {
+  if (g->colx || !g->m || g->m >= (1ull << 30)) return;
+  const uint64_t m = g->m;
+  hipStream_t s = ctx->stream;
+  DevBuf<uint32_t> x(ctx, m + (m + 3) / 2);
+  TB_HIP(hipMemcpyAsync(x.p, g->col, 4 * m, hipMemcpyDeviceToDevice, s));
+  k_fill_u32<<<1, 64, 0, s>>>(x.p + m + (m + 3) / 2 - 1, 1, 0u);
+  k_col16<<<grid_for(m, kBlock), kBlock, 0, s>>>(g->col, m, reinterpret_cast<uint16_t*>(x.p + m));
+  TB_HIP(hipGetLastError());
+  g->colx = x.release();
+  g->col16 = reinterpret_cast<uint16_t*>(g->colx + m);
+  g->bytes += 6 * m;
+}
+
 // Releases every device buffer of a graph (the struct stays).
 void free_graph_memory(tribit_graph* g) {
   if (!g || !g->ctx) return;
   free_tiles(g);
   core_release(g);
+  bucket_cache_free(g);
   g->ctx->free(g->row_ptr);
   g->ctx->free(g->col);
   g->ctx->free(g->deg);
   g->ctx->free(g->labels);
   g->labels = nullptr;
   g->ctx->free(g->rng_sfx);
+  g->ctx->free(g->colx);
+  g->colx = nullptr;
+  g->col16 = nullptr;
   g->ctx->free(g->sfx_ptr);
   g->ctx->free(g->w_sfx);
   g->rng_sfx = nullptr;

@@ -74,7 +74,8 @@ __global__ void k_tile_ranges(const uint2* __restrict__ rng, uint64_t m, const u
   const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= m) return;
   const uint2 r = rng[k];
-  trng[k] = r.x < r.y ? make_uint2(tidx[r.x], tidx[r.y - 1] + 1) : make_uint2(0u, 0u);
+  const uint32_t e = r.y & 0x7FFFFFFFu;  // bit 31: narrow-range flag of wedge ranges
+  trng[k] = r.x < e ? make_uint2(tidx[r.x], tidx[e - 1] + 1) : make_uint2(0u, 0u);
 }
 
 // Total tiles of the graph (cheap: one pass over col), for graph info and the
