@@ -37,15 +37,10 @@ sys.path.insert(0, ROOT)
 
 METRIC = "exact-TC GTEPS on 1 MI355X: learned-sched vs plain; % HBM roofline"
 HBM_PEAK_GBS = 8000.0
-def variant_name(sched: str, v: int, width: int) -> str:
-    """Kernel instantiation of launch slot v at lane-group width `width` (32-bit buffer-load path)."""
-    names = [f"k_count_wave<6,64,{width},8,true>", f"k_count_wave<9,256,{width},8,true>",
-             f"k_count_block<12,{width},8,true>"]
-    if sched == "learned":
-        import tribit_amd
-
-        return names[tribit_amd.menu()[v][0]]
-    return (names + [f"k_count_block<12,{width},8,true> (id-range chunks)"])[v]
+def variant_name(kind: int, width: int) -> str:
+    """Kernel instantiation of a launch of `kind` at lane-group width `width` (32-bit buffer-load path)."""
+    return [f"k_count_wave<6,64,{width},8,true>", f"k_count_wave<9,256,{width},8,true>",
+            f"k_count_block<12,{width},8,true,false,false>", f"k_count_block<12,{width},8,true,true,false>"][kind]
 
 
 def log(msg: str) -> None:
@@ -122,7 +117,9 @@ def measure(g, sched: str, steps: int, warmup: int):
     """W untimed counts, then K timed counts (wall clock around all K)."""
     for i in range(warmup):
         r = g.count(sched=sched)
-        log(f"{sched} warmup {i}: {r.triangles} triangles, {r.ms_count:.2f} ms (variants {r.ms_by_variant})")
+        log(f"{sched} warmup {i}: {r.triangles} triangles, {r.ms_count:.2f} ms (variants {r.ms_by_variant})"
+            + (f"; judged candidates {[round(t, 2) for t in r.ms_candidates]} -> {r.candidate}"
+               if sched == "learned" else ""))
     results = []
     t0 = time.perf_counter()
     for _ in range(steps):
@@ -141,6 +138,11 @@ def measure(g, sched: str, steps: int, warmup: int):
         "ms_var": [statistics.median(r.ms_by_variant[v] for r in results) for v in range(nv)],
         "tasks_by_variant": results[0].tasks_by_variant,
         "group_width": list(results[0].group_width),
+        "slot_kind": list(results[0].slot_kind),
+        "candidate": results[0].candidate,
+        "layout": results[0].layout,
+        "dense_core": results[0].dense_core,
+        "ms_dense": statistics.median(r.ms_dense for r in results),
     }
     log(f"{sched}: {steps} steps, {out['ms_per_step']:.2f} ms/step wall, {out['ms_dev']:.2f} ms device; "
         f"variants {[round(x, 3) for x in out['ms_var']]}")
@@ -291,6 +293,12 @@ def main() -> int:
         "textbook_b_alg_gbs": round(b_alg / (ms_dev * 1e-3) / 1e9, 1),
         "ms_by_variant": [round(x, 4) for x in ms_var],
         "tasks_by_variant": sel["tasks_by_variant"],
+        "learned": None if args.sched != "learned" else {
+            "candidate": ["cost-model plan", "heuristic plan", "heuristic plan, directed layout",
+                          "textbook plain plan"][sel["candidate"]],
+            "layout": {0: "plain", 2: "directed", 3: "wedge"}[sel["layout"]],
+            "dense_core_vertices": sel["dense_core"], "ms_dense_core": round(sel["ms_dense"], 4),
+        },
         "wedge_checks": int(sum(work["wplus"])),
         "plain": None if plain is None else {
             "gteps": round(m / (plain["ms_per_step"] * 1e-3) / 1e9, 4),
@@ -301,7 +309,7 @@ def main() -> int:
         "speedup_vs_plain": None if plain is None else round(plain["ms_per_step"] / ms_per_step, 3),
         "roofline": {
             "bound": "hbm",
-            "kernel": variant_name(args.sched, dom, sel["group_width"][dom]),
+            "kernel": variant_name(sel["slot_kind"][dom], sel["group_width"][dom]),
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",

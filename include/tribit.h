@@ -115,6 +115,9 @@ typedef struct {
                               directed layout, 3 textbook plain); -1 for the fixed schedules */
   int32_t layout;          /* range layout that ran (TRIBIT_SCHED_PLAIN / DIRECTED / WEDGE) */
   double ms_judge;         /* LEARNED: device time of the candidate trials this call ran (first call) */
+  double ms_candidates[4]; /* LEARNED: judged time of each candidate (0 = not a candidate for this graph) */
+  int32_t slot_kind[8];    /* kernel of each launch: 0 wave/64-slot set, 1 wave/512-slot set, 2 block, 3 block
+                              streaming bit tiles */
 } tribit_count_info;
 
 /* Work assigned to each kernel variant by a scheduler, in the units of the

@@ -70,6 +70,7 @@ class CountInfo(ctypes.Structure):
         ("ms_by_variant", ctypes.c_double * 8), ("fallback_tasks", ctypes.c_uint64),
         ("group_width", ctypes.c_int32 * 8), ("dense_core", ctypes.c_uint64), ("ms_dense", ctypes.c_double),
         ("candidate", ctypes.c_int32), ("layout", ctypes.c_int32), ("ms_judge", ctypes.c_double),
+        ("ms_candidates", ctypes.c_double * 4), ("slot_kind", ctypes.c_int32 * 8),
     ]
 
 
@@ -263,6 +264,8 @@ class CountResult:
     candidate: int = -1
     layout: int = 0
     ms_judge: float = 0.0
+    ms_candidates: tuple = ()
+    slot_kind: tuple = ()
 
 
 class Mlp:
@@ -453,7 +456,8 @@ class Graph:
         return CountResult(ci.triangles, ci.ms_count, ci.ms_sched, ci.ms_kernel, ci.launch_blocks, ci.tasks,
                            list(ci.tasks_by_variant)[:max(nv, 4)], bool(ci.repeats_agree),
                            list(ci.ms_by_variant)[:nv], ci.fallback_tasks, tuple(ci.group_width)[:nv],
-                           ci.dense_core, ci.ms_dense, ci.candidate, ci.layout, ci.ms_judge)
+                           ci.dense_core, ci.ms_dense, ci.candidate, ci.layout, ci.ms_judge,
+                           tuple(ci.ms_candidates), tuple(ci.slot_kind)[:nv])
 
     def variant_stats(self, sched: str = "plain") -> dict:
         vs = VariantStats()

@@ -1287,6 +1287,7 @@ static uint64_t count_once(tribit_ctx* ctx, tribit_graph* g, int sched, int flag
     info->nvariants = plan.nslots;
     info->fallback_tasks = res[1];
     for (int i = 0; i < 8; ++i) info->group_width[i] = i < plan.nslots ? plan.g[i] : 0;
+    for (int i = 0; i < 8; ++i) info->slot_kind[i] = i < plan.nslots ? plan.kind[i] : 0;
     info->dense_core = S.core_c;
   }
   return res[0];
@@ -1373,6 +1374,7 @@ void count(tribit_ctx* ctx, tribit_graph* g, const tribit_count_opts& opts, trib
     const bool stale = g->judge_choice < 0 || g->judge_model != ctx->model_version ||
                        g->judge_flags != opts.flags || g->judge_core != opts.dense_core;
     if (stale) {
+      for (int c = 0; c < kCands; ++c) g->judge_ms[c] = 0.0;
       double best = INFINITY;
       uint64_t ref = 0;
       bool have = false;
@@ -1439,6 +1441,8 @@ void count(tribit_ctx* ctx, tribit_graph* g, const tribit_count_opts& opts, trib
   local.candidate = learned ? chosen : -1;
   local.layout = run.layout;
   local.ms_judge = ms_judge;
+  if (learned)
+    for (int c = 0; c < kCands; ++c) local.ms_candidates[c] = g->judge_ms[c];
   if (info) *info = local;
 }
 
