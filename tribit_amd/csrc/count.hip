@@ -96,9 +96,22 @@ template <int LOGS>
 struct CuckooProbe {
   const uint32_t* T;
   uint32_t a;
-  __device__ __forceinline__ uint32_t operator()(uint32_t x) const {
+  __device__ __forceinline__ uint32_t operator()(uint32_t x, uint32_t = 0) const {
     const uint32_t h = x * a;
     return (uint32_t)((T[slot_a<LOGS>(h)] == x) | (T[slot_b<LOGS>(h)] == x));
+  }
+};
+
+// Several cuckoo sets of 2S slots side by side (multi-task waves): the
+// selector picks the set of the range being streamed.
+template <int LOGS>
+struct MultiCuckooProbe {
+  const uint32_t* T;
+  uint32_t a;
+  __device__ __forceinline__ uint32_t operator()(uint32_t x, uint32_t sel) const {
+    const uint32_t* t = T + (sel << (LOGS + 1));
+    const uint32_t h = x * a;
+    return (uint32_t)((t[slot_a<LOGS>(h)] == x) | (t[slot_b<LOGS>(h)] == x));
   }
 };
 
@@ -112,10 +125,11 @@ __device__ __forceinline__ void linear_insert(uint32_t* T, uint32_t w) {
 template <int LOGS2>
 struct LinearProbe {
   const uint32_t* T;
-  __device__ __forceinline__ uint32_t operator()(uint32_t w) const {
+  __device__ __forceinline__ uint32_t operator()(uint32_t w, uint32_t sel = 0) const {
+    const uint32_t* t = T + (sel << LOGS2);  // sets of 2^LOGS2 slots side by side (selector 0: one set)
     uint32_t h = hmul<LOGS2>(w, 0x9E3779B1u);
     for (;;) {
-      const uint32_t k = T[h];
+      const uint32_t k = t[h];
       if (k == w) return 1u;
       if (k == kEmpty) return 0u;
       h = (h + 1) & ((1u << LOGS2) - 1);
@@ -132,7 +146,7 @@ template <int LOGW>
 struct BitmapProbe {
   const uint32_t* T;
   uint32_t base;
-  __device__ __forceinline__ uint32_t operator()(uint32_t x) const {
+  __device__ __forceinline__ uint32_t operator()(uint32_t x, uint32_t = 0) const {
     const uint32_t i = x - base;
     return i < (32u << LOGW) ? (T[i >> 5] >> (i & 31)) & 1u : 0u;
   }
@@ -202,11 +216,12 @@ __device__ __forceinline__ ColSrc make_col_src(const uint32_t* col, uint64_t m, 
 // and every lane still issues one 32-bit load per slot.
 template <int G, int U, bool SHARED, bool BUF, typename Probe, bool N16 = false>
 __device__ __forceinline__ uint32_t stream_ranges(const ColSrc& src, const uint2* R, uint32_t nk, uint32_t* s_next,
-                                                  uint32_t lane, const Probe& probe, const uint16_t* Rh = nullptr) {
+                                                  uint32_t lane, const Probe& probe, const uint16_t* Rh = nullptr,
+                                                  const uint8_t* Rsel = nullptr) {
   constexpr uint64_t kLeads = leader_mask<G>();
   const uint32_t g = lane / G, gl = lane % G;
   const uint64_t below = (g == 0) ? 0ull : ((1ull << (g * G)) - 1);
-  uint32_t base = 0, end = 0, cnt = 0, next = 0, hi = 0, lo = 0;
+  uint32_t base = 0, end = 0, cnt = 0, next = 0, hi = 0, lo = 0, sel = 0;
   bool f16 = false;
   for (;;) {
     const bool need = base >= end;  // uniform within a group
@@ -229,6 +244,7 @@ __device__ __forceinline__ uint32_t stream_ranges(const ColSrc& src, const uint2
         const uint2 r = R[k];
         base = r.x;
         end = r.y & kEndMask;
+        if (Rsel) sel = Rsel[k];
         if (N16) {
           f16 = (r.y & kNarrowFlag) != 0;
           hi = (uint32_t)Rh[k] << 16;
@@ -255,7 +271,7 @@ __device__ __forceinline__ uint32_t stream_ranges(const ColSrc& src, const uint2
         const bool ok0 = e >= lo && e < end, ok1 = f16 && e + 1 < end;
         const uint32_t k0 = ok0 ? (f16 ? (hi | (w[j] & 0xFFFFu)) : w[j]) : kNoKey;
         const uint32_t k1 = ok1 ? (hi | (w[j] >> 16)) : kNoKey;
-        cnt += probe(k0) + probe(k1);
+        cnt += probe(k0, sel) + probe(k1, sel);
       }
       base += (f16 ? 2u : 1u) * (uint32_t)(G * U);
       continue;
@@ -278,7 +294,7 @@ __device__ __forceinline__ uint32_t stream_ranges(const ColSrc& src, const uint2
       for (int j = 0; j < U; ++j) x[j] = (j * G < rem) ? src.col[p0 + j * G] : kNoKey;
     }
 #pragma unroll
-    for (int j = 0; j < U; ++j) cnt += probe(x[j]);
+    for (int j = 0; j < U; ++j) cnt += probe(x[j], sel);
     base += (uint32_t)(G * U);
   }
   return cnt;
@@ -408,6 +424,130 @@ __global__ __launch_bounds__(256) void k_count_wave(const uint32_t* __restrict__
         if (lane == 0 && tc) atomicAdd(&gcount[gid[x]], (unsigned long long)tc);
       }
       if (tlog && lane == 0) atomicAdd(&tlog[tk.w >> kBucketShift], (unsigned long long)(wall_clock64() - t0));
+    }
+  }
+  acc = wave_sum_u64(acc);
+  if (lane == 0 && acc) atomicAdd(result, (unsigned long long)acc);
+  if (lane == 0 && fallbacks) atomicAdd(result + 1, (unsigned long long)fallbacks);
+}
+
+// Multi-task waves for owners with tiny sets: a wave takes MT tasks at once,
+// builds their MT sets (2S slots each) side by side in its LDS slice, stages
+// the concatenation of their range lists with the set of each range, and
+// streams them together, so a wave keeps up to 64 short lists in flight
+// instead of the few of one small owner. Batches (per-graph counts) use the
+// one-task kernel; timing logs split a batch's time evenly over its tasks.
+template <int LOGS, int KMAX, int MT, int G, int U, bool BUF>
+__global__ __launch_bounds__(256) void k_count_wave_multi(const uint32_t* __restrict__ row_ptr,
+                                                          const uint32_t* __restrict__ col, uint64_t m,
+                                                          const uint2* __restrict__ rng,
+                                                          const uint4* __restrict__ tasks, uint32_t ntasks,
+                                                          uint32_t* __restrict__ queue,
+                                                          unsigned long long* __restrict__ result, int attempts,
+                                                          unsigned long long* __restrict__ tlog) {
+  constexpr uint32_t S = 1u << LOGS;
+  static_assert(MT <= 16 && (MT & (MT - 1)) == 0, "MT: power of two up to 16");
+  const ColSrc src = make_col_src(col, m);
+  __shared__ uint32_t sT[4][MT * 2 * S];
+  __shared__ uint2 sR[4][KMAX];
+  __shared__ uint8_t sSel[4][KMAX];
+  __shared__ uint32_t sInfo[4][4][MT];  // per task: outdeg prefix, row start, range prefix, range base
+  const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  uint32_t* T = sT[wv];
+  uint2* R = sR[wv];
+  uint8_t* Sel = sSel[wv];
+  uint32_t* DP = sInfo[wv][0];
+  uint32_t* R0 = sInfo[wv][1];
+  uint32_t* KP = sInfo[wv][2];
+  uint32_t* RB = sInfo[wv][3];
+  uint64_t acc = 0;
+  uint32_t fallbacks = 0;
+  for (;;) {
+    uint32_t tb = 0;
+    if (lane == 0) tb = atomicAdd(queue, kWaveBatch);
+    tb = __shfl(tb, 0);
+    if (tb >= ntasks) break;
+    const uint32_t tend = min(tb + kWaveBatch, ntasks);
+    for (uint32_t t0 = tb; t0 < tend; t0 += MT) {
+      const uint32_t nt = min((uint32_t)MT, tend - t0);
+      const uint64_t c0t = tlog ? (uint64_t)wall_clock64() : 0;
+      // lane j < nt holds task j: owner row, outdeg, ranges
+      uint4 tk = make_uint4(0u, 0u, 0u, 0u);
+      uint32_t r0 = 0, d = 0;
+      if (lane < nt) {
+        tk = tasks[t0 + lane];
+        r0 = row_ptr[tk.x];
+        d = row_ptr[tk.x + 1] - r0;
+      }
+      // exclusive prefixes of outdeg and range counts over the nt tasks
+      uint32_t dp = d, kp = tk.z;
+#pragma unroll
+      for (int o = 1; o < MT; o <<= 1) {
+        const uint32_t a = __shfl_up(dp, o), b = __shfl_up(kp, o);
+        if (lane >= (uint32_t)o) {
+          dp += a;
+          kp += b;
+        }
+      }
+      const uint32_t D = __shfl(dp, MT - 1), K = __shfl(kp, MT - 1);
+      dp -= d;
+      kp -= tk.z;
+      if (lane < (uint32_t)MT) {  // tasks >= nt get empty slots past every index
+        DP[lane] = lane < nt ? dp : 0xFFFFFFFFu;
+        R0[lane] = r0;
+        KP[lane] = lane < nt ? kp : 0xFFFFFFFFu;
+        RB[lane] = tk.y;
+      }
+      wave_lds_sync();
+      uint32_t sa = 0;
+      bool ok = false;
+      for (int attempt = 0; attempt < attempts && !ok; ++attempt) {
+        sa = kSeedA[attempt];
+        for (uint32_t i = lane; i < nt * 2 * S; i += 64) T[i] = kEmpty;
+        wave_lds_sync();
+        bool fail = false;
+        for (uint32_t f = lane; f < D; f += 64) {
+          uint32_t j = 0;
+#pragma unroll
+          for (int q = 1; q < MT; ++q) j += f >= DP[q] ? 1u : 0u;  // DP ascending
+          fail |= !cuckoo_insert<LOGS>(T + j * 2 * S, col[R0[j] + f - DP[j]], sa);
+        }
+        wave_lds_sync();
+        ok = !__any(fail);
+      }
+      if (!ok) {
+        ++fallbacks;
+        for (uint32_t i = lane; i < nt * 2 * S; i += 64) T[i] = kEmpty;
+        wave_lds_sync();
+        for (uint32_t f = lane; f < D; f += 64) {
+          uint32_t j = 0;
+#pragma unroll
+          for (int q = 1; q < MT; ++q) j += f >= DP[q] ? 1u : 0u;
+          linear_insert<LOGS + 1>(T + j * 2 * S, col[R0[j] + f - DP[j]]);
+        }
+      }
+      uint32_t cnt = 0;
+      for (uint32_t c0 = 0; c0 < K; c0 += KMAX) {
+        const uint32_t nk = min((uint32_t)KMAX, K - c0);
+        for (uint32_t k = lane; k < nk; k += 64) {
+          const uint32_t f = c0 + k;
+          uint32_t j = 0;
+#pragma unroll
+          for (int q = 1; q < MT; ++q) j += f >= KP[q] ? 1u : 0u;
+          R[k] = rng[RB[j] + f - KP[j]];
+          Sel[k] = (uint8_t)j;
+        }
+        wave_lds_sync();
+        cnt += ok ? stream_ranges<G, U, false, BUF>(src, R, nk, nullptr, lane, MultiCuckooProbe<LOGS>{T, sa},
+                                                     nullptr, Sel)
+                  : stream_ranges<G, U, false, BUF>(src, R, nk, nullptr, lane, LinearProbe<LOGS + 1>{T}, nullptr,
+                                                     Sel);
+        wave_lds_sync();
+      }
+      acc += cnt;
+      if (tlog && lane < nt)
+        atomicAdd(&tlog[tk.w >> kBucketShift], (unsigned long long)((wall_clock64() - c0t) / nt));
+      wave_lds_sync();  // the next batch rewrites the task table and sets
     }
   }
   acc = wave_sum_u64(acc);
@@ -577,7 +717,7 @@ constexpr int kBuckets = TRIBIT_BUCKETS;
 constexpr int kSlots = TRIBIT_MENU_SLOTS;
 constexpr int kFeatures = TRIBIT_FEATURES;
 static_assert(kBuckets == kDBins * kLBins, "bucket grid");
-enum SlotKind { kKindWave64 = 0, kKindWave512 = 1, kKindBlock = 2, kKindTile = 3 };
+enum SlotKind { kKindWave64 = 0, kKindWave512 = 1, kKindBlock = 2, kKindTile = 3, kKindWaveMulti = 4 };
 
 struct Plan {
   int nslots;
@@ -614,7 +754,7 @@ __host__ __device__ inline int l_bin(uint64_t w, uint32_t k) {
 __host__ __device__ inline int bucket_of(uint32_t d, uint64_t w, uint32_t k) { return d_bin(d) * kLBins + l_bin(w, k); }
 
 __host__ __device__ inline uint32_t kind_max_d(int kind) {
-  return kind == kKindWave64 ? 32u : kind == kKindWave512 ? 256u : 0xFFFFFFFFu;
+  return (kind == kKindWave64 || kind == kKindWaveMulti) ? 32u : kind == kKindWave512 ? 256u : 0xFFFFFFFFu;
 }
 
 struct SchedView {
@@ -821,9 +961,9 @@ static const uint2* sched_ranges(const tribit_graph* g, int sched) {
 
 // ------------------------------------------------------------------ plans
 // The learned scheduler's menu: (kind, lane-group width) per slot.
-static const int kMenuKind[kSlots] = {kKindWave64,  kKindWave64, kKindWave512, kKindWave512,
-                                      kKindWave512, kKindBlock,  kKindBlock,   kKindTile};
-static const int kMenuG[kSlots] = {1, 4, 2, 4, 8, 8, 16, 8};
+static const int kMenuKind[kSlots] = {kKindWave64,  kKindWaveMulti, kKindWave512, kKindWave512,
+                                      kKindWave512, kKindBlock,     kKindBlock,   kKindTile};
+static const int kMenuG[kSlots] = {1, 1, 2, 4, 8, 8, 16, 8};
 
 static int kind_of_dbin(int db) {
   const uint32_t top = d_bin_top(db);
@@ -1068,6 +1208,47 @@ static void launch_wave(const LaunchArgs& a, int slot, int gw, uint32_t ntasks, 
   }
 }
 
+constexpr int kMultiTasks = 8;  // tasks per multi-task wave batch
+
+template <int G, bool BUF>
+static void launch_multi_t(const LaunchArgs& a, int slot, uint32_t ntasks, int cus, hipStream_t s, tribit_graph* g,
+                           const uint2* rng, const uint4* tasks, uint64_t* blocks) {
+  auto k = k_count_wave_multi<kC0Logs, kC0K, kMultiTasks, G, kU, BUF>;
+  int per_cu = 0;
+  TB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, kBlock, 0));
+  if (a.waves[slot] > 0 && per_cu > a.waves[slot]) per_cu = a.waves[slot];
+  if (per_cu < 1) per_cu = 1;
+  const uint64_t want = (ntasks + 4 * kWaveBatch - 1) / (4 * kWaveBatch);
+  const unsigned grid = (unsigned)std::min<uint64_t>(want, (uint64_t)cus * per_cu);
+  k<<<grid, kBlock, 0, s>>>(g->row_ptr, g->col, g->m, rng, tasks, ntasks, a.queue + slot, a.result, a.att & 0xFF,
+                            a.tlog ? a.tlog + (size_t)slot * kBuckets : nullptr);
+  *blocks += grid;
+}
+
+static void launch_multi(const LaunchArgs& a, int slot, int gw, uint32_t ntasks, int cus, hipStream_t s,
+                         tribit_graph* g, const uint2* rng, const uint4* tasks, uint64_t* blocks) {
+  if (!ntasks) return;
+  if (a.gcount) {  // per-graph counts: the one-task kernel attributes every task
+    launch_wave<kC0Logs, kC0K>(a, slot, gw, ntasks, cus, s, g, rng, tasks, blocks);
+    return;
+  }
+  const bool buf = g->m * 4 < 0xFFFFFFFFull && !(a.att & kForceGlobal);
+  switch (gw) {
+    case 2:
+      buf ? launch_multi_t<2, true>(a, slot, ntasks, cus, s, g, rng, tasks, blocks)
+          : launch_multi_t<2, false>(a, slot, ntasks, cus, s, g, rng, tasks, blocks);
+      break;
+    case 4:
+      buf ? launch_multi_t<4, true>(a, slot, ntasks, cus, s, g, rng, tasks, blocks)
+          : launch_multi_t<4, false>(a, slot, ntasks, cus, s, g, rng, tasks, blocks);
+      break;
+    default:
+      buf ? launch_multi_t<1, true>(a, slot, ntasks, cus, s, g, rng, tasks, blocks)
+          : launch_multi_t<1, false>(a, slot, ntasks, cus, s, g, rng, tasks, blocks);
+      break;
+  }
+}
+
 template <int LOGS, int G, int U, bool BUF, bool TILES, bool N16>
 static void launch_block_t(const LaunchArgs& a, int slot, uint32_t ntasks, int cus, hipStream_t s, tribit_graph* g,
                            const uint2* rng, const TileSrc& ts, const uint4* tasks, uint64_t* blocks) {
@@ -1231,6 +1412,8 @@ void launch_schedule(tribit_ctx* ctx, tribit_graph* g, const Schedule& S, const 
     const uint32_t nt = S.tot.v[i];
     if (plan.kind[i] == kKindWave64)
       launch_wave<kC0Logs, kC0K>(a, i, plan.g[i], nt, cus, s, g, S.rng, tk, blocks);
+    else if (plan.kind[i] == kKindWaveMulti)
+      launch_multi(a, i, plan.g[i], nt, cus, s, g, S.rng, tk, blocks);
     else if (plan.kind[i] == kKindWave512)
       launch_wave<kC1Logs, kC1K>(a, i, plan.g[i], nt, cus, s, g, S.rng, tk, blocks);
     else if (plan.kind[i] == kKindTile && S.ts.trng)

@@ -33,7 +33,7 @@ sys.path.insert(0, os.path.dirname(HERE))
 import tribit_amd as tb  # noqa: E402
 
 DEFAULT_OUT = os.path.join(HERE, "cost_model.f32")
-THREADS = {0: 64, 1: 64, 2: 512, 3: 512}  # menu kind -> threads of one task
+THREADS = {0: 64, 1: 64, 2: 512, 3: 512, 4: 64}  # menu kind -> threads of one task (multi-task waves log a share)
 
 TRAIN_GRAPHS = [
     ("rmat", dict(scale=18, ef=16, seed=11)),
