@@ -294,6 +294,35 @@ int tribit_batch_autotune(tribit_ctx* ctx, tribit_batch* b, const tribit_count_o
                           tribit_batch_count_info* best);
 int tribit_batch_free(tribit_batch* b);
 
+/* ---- one rank of the reference's multi-GPU decomposition (main.rs:97-220) ---
+ * Keys are packed (min<<32)|max over global ids < num_nodes; chunk i holds
+ * ptr[i+1] - ptr[i] keys (ptr cumulative, num_ptrs + 1 entries); the caller
+ * owns the host chunks, the library frees its device memory before return.
+ * Diagonal task (main5_multi.cu:589-728): the chunks hold every edge whose
+ * larger id is in partition p; returns the triangles whose two larger
+ * vertices are in p (the triangles of those edges).
+ * Off-diagonal task (main6_multi.cu:595-751): cols hold the edges whose larger
+ * id is in partition pc, idxs those whose larger id is in later partitions S;
+ * returns the triangles a < b < c with b in pc and c in S (computed as
+ * triangles(cols + idxs) - triangles(cols) - triangles(idxs)). */
+int tribit_count_partial_diagonal(tribit_ctx* ctx, const uint64_t* const* chunks, const uint64_t* nnz_ptr,
+                                  uint32_t num_ptrs, uint64_t num_nodes, uint64_t* triangles);
+int tribit_count_partial_off_diagonal(tribit_ctx* ctx, const uint64_t* const* cols, const uint64_t* nnz_cols_ptr,
+                                      uint32_t num_ptrs_cols, const uint64_t* const* idxs,
+                                      const uint64_t* nnz_idxs_ptr, uint32_t num_ptrs_idxs, uint64_t num_nodes,
+                                      uint64_t* triangles);
+/* The reference-named symbols, same signatures (main5_multi.cu:578-589,
+ * main6_multi.cu:585-598; Rust declarations main.rs:15-36): init selects the
+ * device for the calling thread; a count returns UINT64_MAX on error (the
+ * reference exits) with the status in tribit_partial_last_status(). */
+void init_gpu_kernel5(int gpu_id);
+void init_gpu_kernel6(int gpu_id);
+uint64_t count_partial_triangles_diagonal(uint64_t** chunks, uint64_t* nnz_ptr, uint32_t num_ptrs, uint64_t num_nodes);
+uint64_t count_partial_triangles_off_diagonal(uint64_t** cols, uint64_t* nnz_cols_ptr, uint32_t num_ptrs_cols,
+                                              uint64_t** idxs, uint64_t* nnz_idxs_ptr, uint32_t num_ptrs_idxs,
+                                              uint64_t num_nodes);
+int tribit_partial_last_status(void);
+
 /* ---- cost model of the learned scheduler (no reference counterpart) -------
  * A 3-layer perceptron, F inputs -> 64 -> 64 -> 1 (ReLU), trained in FP32 by
  * the library's own HIP kernels (forward, backward, Adam; loss = mean squared

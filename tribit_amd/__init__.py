@@ -116,7 +116,8 @@ def header_symbols() -> list[str]:
     """Function names declared in include/tribit.h."""
     src = open(HEADER_PATH).read()
     src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
-    return sorted(set(re.findall(r"\b(tribit_[a-z0-9_]+)\s*\(", src)))
+    return sorted(set(re.findall(r"\b(tribit_[a-z0-9_]+|init_gpu_kernel[56]|count_partial_triangles_[a-z_]+)\s*\(",
+                                 src)))
 
 
 def load_library(path: str | None = None) -> ctypes.CDLL:
@@ -174,6 +175,14 @@ def load_library(path: str | None = None) -> ctypes.CDLL:
         "tribit_batch_autotune": (ctypes.c_int, [_P, _P, ctypes.POINTER(CountOpts),
                                                  ctypes.POINTER(BatchCountInfo)]),
         "tribit_batch_free": (ctypes.c_int, [_P]),
+        "tribit_count_partial_diagonal": (ctypes.c_int, [_P, _P, _P, ctypes.c_uint32, _U64, _U64P]),
+        "tribit_count_partial_off_diagonal": (ctypes.c_int, [_P, _P, _P, ctypes.c_uint32, _P, _P, ctypes.c_uint32,
+                                                             _U64, _U64P]),
+        "init_gpu_kernel5": (None, [ctypes.c_int]),
+        "init_gpu_kernel6": (None, [ctypes.c_int]),
+        "count_partial_triangles_diagonal": (_U64, [_P, _P, ctypes.c_uint32, _U64]),
+        "count_partial_triangles_off_diagonal": (_U64, [_P, _P, ctypes.c_uint32, _P, _P, ctypes.c_uint32, _U64]),
+        "tribit_partial_last_status": (ctypes.c_int, []),
         "tribit_mlp_create": (ctypes.c_int, [_P, ctypes.c_int, _U64, ctypes.POINTER(_P)]),
         "tribit_mlp_free": (ctypes.c_int, [_P]),
         "tribit_mlp_num_params": (_U64, [_P]),
@@ -206,6 +215,34 @@ def chung_lu_batch_params(seed: int, index: int):
     g, d, n = ctypes.c_double(), ctypes.c_double(), _U64()
     load_library().tribit_chung_lu_batch_params(seed, index, ctypes.byref(g), ctypes.byref(n), ctypes.byref(d))
     return g.value, n.value, d.value
+
+
+class Chunks:
+    """Host key chunks in the reference's layout: uint64_t** plus a cumulative uint64_t* (num + 1)."""
+
+    def __init__(self, arrays):
+        self.arrays = [np.ascontiguousarray(a, np.uint64) for a in arrays]
+        self.ptrs = (ctypes.c_void_p * max(len(self.arrays), 1))(*[a.ctypes.data for a in self.arrays])
+        self.cum = np.zeros(len(self.arrays) + 1, np.uint64)
+        self.cum[1:] = np.cumsum([a.size for a in self.arrays])
+        self.num = len(self.arrays)
+
+
+def count_partial_diagonal(ctx: "Context", chunks, num_nodes: int) -> int:
+    c = Chunks(chunks)
+    out = _U64()
+    _check(ctx.lib.tribit_count_partial_diagonal(ctx.h, ctypes.addressof(c.ptrs), c.cum.ctypes.data, c.num,
+                                                 num_nodes, ctypes.byref(out)), "count_partial_diagonal")
+    return out.value
+
+
+def count_partial_off_diagonal(ctx: "Context", cols, idxs, num_nodes: int) -> int:
+    c, x = Chunks(cols), Chunks(idxs)
+    out = _U64()
+    _check(ctx.lib.tribit_count_partial_off_diagonal(ctx.h, ctypes.addressof(c.ptrs), c.cum.ctypes.data, c.num,
+                                                     ctypes.addressof(x.ptrs), x.cum.ctypes.data, x.num, num_nodes,
+                                                     ctypes.byref(out)), "count_partial_off_diagonal")
+    return out.value
 
 
 def profile_features(p: Profile, bucket: int, slot: int) -> np.ndarray:

@@ -3,6 +3,7 @@
 // tribit_status; nothing throws across the ABI and nothing exits the process.
 #include <cstdlib>
 #include <cstring>
+#include <vector>
 #include <new>
 
 #include <rocprim/rocprim_version.hpp>
@@ -593,5 +594,122 @@ int tribit_batch_free(tribit_batch* b) {
   tb::batch_free(b);
   return TRIBIT_OK;
 }
+
+// ---- one rank of the reference's multi-GPU decomposition ------------------
+namespace {
+
+// Host keys of `num` chunks (chunk i: ptr[i+1] - ptr[i] keys) in one vector.
+std::vector<uint64_t> gather_chunks(const uint64_t* const* chunks, const uint64_t* ptr, uint32_t num) {
+  std::vector<uint64_t> v;
+  v.reserve(ptr[num] - ptr[0]);
+  for (uint32_t i = 0; i < num; ++i) {
+    const uint64_t k = ptr[i + 1] - ptr[i];
+    if (k) v.insert(v.end(), chunks[i], chunks[i] + k);
+  }
+  return v;
+}
+
+uint64_t count_vector(tribit_ctx* ctx, const std::vector<uint64_t>& keys, uint64_t n) {
+  tribit_graph* g = new tribit_graph;
+  try {
+    uint64_t* d = ctx->alloc_n<uint64_t>(keys.empty() ? 1 : keys.size());
+    if (!keys.empty())
+      TB_HIP(hipMemcpyAsync(d, keys.data(), 8 * keys.size(), hipMemcpyHostToDevice, ctx->stream));
+    tb::preprocess(ctx, d, keys.size(), n, g);
+    tribit_count_opts o;
+    memset(&o, 0, sizeof(o));
+    o.sched = TRIBIT_SCHED_LEARNED;
+    o.repeats = 1;
+    tribit_count_info ci;
+    tb::count(ctx, g, o, &ci);
+    tb::free_graph(g);
+    return ci.triangles;
+  } catch (...) {
+    tb::free_graph(g);
+    throw;
+  }
+}
+
+bool chunks_ok(const uint64_t* const* chunks, const uint64_t* ptr, uint32_t num) {
+  if (!ptr) return false;
+  for (uint32_t i = 0; i < num; ++i)
+    if (ptr[i + 1] < ptr[i] || (ptr[i + 1] > ptr[i] && !chunks)) return false;
+  for (uint32_t i = 0; i < num; ++i)
+    if (ptr[i + 1] > ptr[i] && !chunks[i]) return false;
+  return true;
+}
+
+thread_local tribit_ctx* t_rank_ctx = nullptr;
+thread_local int t_rank_status = TRIBIT_OK;
+
+}  // namespace
+
+int tribit_count_partial_diagonal(tribit_ctx* ctx, const uint64_t* const* chunks, const uint64_t* nnz_ptr,
+                                  uint32_t num_ptrs, uint64_t num_nodes, uint64_t* triangles) {
+  if (!ctx || !triangles || !chunks_ok(chunks, nnz_ptr, num_ptrs) || num_nodes == 0) return TRIBIT_ERR_ARG;
+  return guarded(ctx, [&] {
+    TB_HIP(hipSetDevice(ctx->device));
+    *triangles = count_vector(ctx, gather_chunks(chunks, nnz_ptr, num_ptrs), num_nodes);
+  });
+}
+
+int tribit_count_partial_off_diagonal(tribit_ctx* ctx, const uint64_t* const* cols, const uint64_t* nnz_cols_ptr,
+                                      uint32_t num_ptrs_cols, const uint64_t* const* idxs,
+                                      const uint64_t* nnz_idxs_ptr, uint32_t num_ptrs_idxs, uint64_t num_nodes,
+                                      uint64_t* triangles) {
+  if (!ctx || !triangles || !chunks_ok(cols, nnz_cols_ptr, num_ptrs_cols) ||
+      !chunks_ok(idxs, nnz_idxs_ptr, num_ptrs_idxs) || num_nodes == 0)
+    return TRIBIT_ERR_ARG;
+  return guarded(ctx, [&] {
+    TB_HIP(hipSetDevice(ctx->device));
+    // every triangle a < b < c of cols + idxs has (b, c) and (a, c) in one set:
+    // both in cols (c in the column partition), both in idxs with (a, b) in
+    // idxs (b beyond the column partition) or (a, b) in cols (the wanted ones)
+    const std::vector<uint64_t> c = gather_chunks(cols, nnz_cols_ptr, num_ptrs_cols);
+    const std::vector<uint64_t> x = gather_chunks(idxs, nnz_idxs_ptr, num_ptrs_idxs);
+    std::vector<uint64_t> u(c);
+    u.insert(u.end(), x.begin(), x.end());
+    const uint64_t tu = count_vector(ctx, u, num_nodes);
+    const uint64_t tc = count_vector(ctx, c, num_nodes);
+    const uint64_t tx = count_vector(ctx, x, num_nodes);
+    if (tc + tx > tu) tb::fail(TRIBIT_ERR_ARG, "column and index sets overlap");
+    *triangles = tu - tc - tx;
+  });
+}
+
+// Reference-named entry points (same signatures as main5_multi.cu:578-589 and
+// main6_multi.cu:585-598): a thread-local context per calling thread; errors
+// give UINT64_MAX and tribit_partial_last_status() (the reference exits).
+void init_gpu_kernel5(int gpu_id) {
+  if (t_rank_ctx) {
+    tribit_shutdown(t_rank_ctx);
+    t_rank_ctx = nullptr;
+  }
+  t_rank_status = tribit_init(gpu_id, &t_rank_ctx);
+}
+
+void init_gpu_kernel6(int gpu_id) { init_gpu_kernel5(gpu_id); }
+
+uint64_t count_partial_triangles_diagonal(uint64_t** chunks, uint64_t* nnz_ptr, uint32_t num_ptrs,
+                                          uint64_t num_nodes) {
+  if (!t_rank_ctx) init_gpu_kernel5(0);
+  uint64_t t = 0;
+  t_rank_status = t_rank_ctx ? tribit_count_partial_diagonal(t_rank_ctx, chunks, nnz_ptr, num_ptrs, num_nodes, &t)
+                             : t_rank_status;
+  return t_rank_status == TRIBIT_OK ? t : UINT64_MAX;
+}
+
+uint64_t count_partial_triangles_off_diagonal(uint64_t** cols, uint64_t* nnz_cols_ptr, uint32_t num_ptrs_cols,
+                                              uint64_t** idxs, uint64_t* nnz_idxs_ptr, uint32_t num_ptrs_idxs,
+                                              uint64_t num_nodes) {
+  if (!t_rank_ctx) init_gpu_kernel6(0);
+  uint64_t t = 0;
+  t_rank_status = t_rank_ctx ? tribit_count_partial_off_diagonal(t_rank_ctx, cols, nnz_cols_ptr, num_ptrs_cols, idxs,
+                                                                 nnz_idxs_ptr, num_ptrs_idxs, num_nodes, &t)
+                             : t_rank_status;
+  return t_rank_status == TRIBIT_OK ? t : UINT64_MAX;
+}
+
+int tribit_partial_last_status(void) { return t_rank_status; }
 
 }  // extern "C"
