@@ -29,10 +29,15 @@
 namespace tb {
 
 constexpr unsigned kBlock = 256;
+// Every kernel takes its tasks from the END of its list. Lists hold owners in
+// id order, and with degree-rank ids the highest ids are the hubs, whose
+// tasks are the longest: taken first, they overlap with the many short
+// tasks instead of forming the launch's tail.
 constexpr uint32_t kWaveBatch = 16;      // wave tasks taken per queue atomic
 constexpr uint32_t kChunk = 2048;        // probe-set chunk of class-3 rows
 constexpr uint64_t kWaveCap = 1u << 15;  // target streamed elements per wave task
 constexpr uint64_t kBlockCap = 1u << 18; // target streamed elements per block task
+constexpr uint32_t kWaveListCap = 1024;   // most lists per wave task
 constexpr uint32_t kBucketShift = 21;   // task.w = chunk | bucket << 21
 constexpr uint32_t kChunkMask = (1u << kBucketShift) - 1;
 
@@ -388,7 +393,7 @@ __global__ __launch_bounds__(256) void k_count_wave(const uint32_t* __restrict__
     if (tb >= ntasks) break;
     const uint32_t tend = min(tb + kWaveBatch, ntasks);
     for (uint32_t t = tb; t < tend; ++t) {
-      const uint4 tk = tasks[t];
+      const uint4 tk = tasks[ntasks - 1 - t];
       const uint32_t x = tk.x, rbase = tk.y, rcount = tk.z;
       const uint64_t t0 = tlog ? (uint64_t)wall_clock64() : 0;
       const uint32_t r0 = row_ptr[x], d = row_ptr[x + 1] - r0;
@@ -475,7 +480,7 @@ __global__ __launch_bounds__(256) void k_count_wave_multi(const uint32_t* __rest
       uint4 tk = make_uint4(0u, 0u, 0u, 0u);
       uint32_t r0 = 0, d = 0;
       if (lane < nt) {
-        tk = tasks[t0 + lane];
+        tk = tasks[ntasks - 1 - (t0 + lane)];
         r0 = row_ptr[tk.x];
         d = row_ptr[tk.x + 1] - r0;
       }
@@ -593,7 +598,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_count_block(const uint32_t* _
     __syncthreads();
     const uint32_t t = s_task;
     if (t >= ntasks) break;
-    const uint4 tk = tasks[t];
+    const uint4 tk = tasks[ntasks - 1 - t];
     const uint32_t x = tk.x, rbase = tk.y, rcount = tk.z, c = tk.w & kChunkMask;
     const uint64_t t0 = tlog ? (uint64_t)wall_clock64() : 0;
     const uint32_t r0 = row_ptr[x], d = row_ptr[x + 1] - r0;
@@ -717,7 +722,15 @@ constexpr int kBuckets = TRIBIT_BUCKETS;
 constexpr int kSlots = TRIBIT_MENU_SLOTS;
 constexpr int kFeatures = TRIBIT_FEATURES;
 static_assert(kBuckets == kDBins * kLBins, "bucket grid");
-enum SlotKind { kKindWave64 = 0, kKindWave512 = 1, kKindBlock = 2, kKindTile = 3, kKindWaveMulti = 4 };
+enum SlotKind {
+  kKindWave64 = 0,
+  kKindWave512 = 1,
+  kKindBlock = 2,
+  kKindTile = 3,
+  kKindWaveMulti = 4
+};
+
+__host__ __device__ inline bool kind_is_block(int kind) { return kind == kKindBlock || kind == kKindTile; }
 
 struct Plan {
   int nslots;
@@ -821,15 +834,21 @@ __device__ __forceinline__ bool list_useful(const SchedView& v, int l, uint32_t 
 }
 
 __device__ __forceinline__ uint32_t list_splits(int kind, uint32_t count, uint64_t w) {
-  const uint64_t cap = kind >= kKindBlock ? kBlockCap : kWaveCap;
+  const uint64_t cap = kind_is_block(kind) ? kBlockCap : kWaveCap;
   uint64_t s = (w + cap - 1) / cap;
+  // wave tasks stage 64 lists per round trip: hubs with many short lists
+  // (the top ids have tiny out-lists and huge in-degree) split by list count
+  if (!kind_is_block(kind)) {
+    const uint64_t sl = (count + kWaveListCap - 1) / kWaveListCap;
+    if (sl > s) s = sl;
+  }
   if (s < 1) s = 1;
   if (s > count) s = count;
   return (uint32_t)s;
 }
 
 __device__ __forceinline__ uint32_t list_chunks(int kind, uint32_t d) {
-  return (kind >= kKindBlock && d > kChunk) ? (d + kChunk - 1) / kChunk : 1u;
+  return (kind_is_block(kind) && d > kChunk) ? (d + kChunk - 1) / kChunk : 1u;
 }
 
 // Per owner: tasks per slot.
@@ -1079,7 +1098,7 @@ void features(const BucketStats& st, int b, int slot, float* f) {
   const int kind = kMenuKind[slot], g = kMenuG[slot];
   f[6] = kind == kKindWave64;
   f[7] = kind == kKindWave512;
-  f[8] = kind >= kKindBlock;
+  f[8] = kind_is_block(kind);
   f[9] = (float)(std::log2((double)g) / 6.0);
   f[10] = g == 4;
   f[11] = g == 8;
