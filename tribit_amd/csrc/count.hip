@@ -99,6 +99,7 @@ __device__ __forceinline__ bool cuckoo_insert(uint32_t* T, uint32_t x, uint32_t 
 
 template <int LOGS>
 struct CuckooProbe {
+  static constexpr bool kZeroSafe = false;  // id 0 may be in the set
   const uint32_t* T;
   uint32_t a;
   __device__ __forceinline__ uint32_t operator()(uint32_t x, uint32_t = 0) const {
@@ -111,6 +112,7 @@ struct CuckooProbe {
 // selector picks the set of the range being streamed.
 template <int LOGS>
 struct MultiCuckooProbe {
+  static constexpr bool kZeroSafe = false;
   const uint32_t* T;
   uint32_t a;
   __device__ __forceinline__ uint32_t operator()(uint32_t x, uint32_t sel) const {
@@ -129,6 +131,7 @@ __device__ __forceinline__ void linear_insert(uint32_t* T, uint32_t w) {
 
 template <int LOGS2>
 struct LinearProbe {
+  static constexpr bool kZeroSafe = false;
   const uint32_t* T;
   __device__ __forceinline__ uint32_t operator()(uint32_t w, uint32_t sel = 0) const {
     const uint32_t* t = T + (sel << LOGS2);  // sets of 2^LOGS2 slots side by side (selector 0: one set)
@@ -146,14 +149,20 @@ struct LinearProbe {
 // probe. Block tasks use it instead of the cuckoo set when the owner's ids
 // fit the set's 2S words (2^18 ids for S = 4096), as they do for almost all
 // owners once ids follow the degree rank. kNoKey and ids below base wrap to
-// indexes past the window and miss.
+// indexes past the window; every such index is clamped to the zero word that
+// follows the window, so a probe is one unconditional LDS read (a branch
+// around the read would serialize the probes of a step). Block tasks use the
+// bitmap only for sets without id 0, so the 0 that a masked load returns
+// misses as well (kZeroSafe).
 template <int LOGW>
 struct BitmapProbe {
-  const uint32_t* T;
+  static constexpr bool kZeroSafe = true;
+  const uint32_t* T;  // 2^LOGW words, then a zero word
   uint32_t base;
   __device__ __forceinline__ uint32_t operator()(uint32_t x, uint32_t = 0) const {
     const uint32_t i = x - base;
-    return i < (32u << LOGW) ? (T[i >> 5] >> (i & 31)) & 1u : 0u;
+    const uint32_t w = min(i >> 5, 1u << LOGW);
+    return __builtin_amdgcn_ubfe(T[w], i, 1u);  // the bit field offset is i mod 32
   }
 };
 
@@ -197,6 +206,13 @@ __device__ __forceinline__ uint32_t upper_bound_u32(const uint32_t* a, uint32_t 
 // wave-private counter, or, for block tasks (SHARED), an LDS counter shared
 // by the block's waves.
 
+// Raw buffer loads address col with 32-bit byte offsets: the buffer path is
+// taken when the streamed array's bytes stay at or below kBufMax, and a
+// masked slot uses offset kBufOob (plus at most 4 KiB of immediate offset),
+// past every such buffer.
+constexpr uint64_t kBufMax = 0xFFFFE000ull;
+constexpr uint32_t kBufOob = 0xFFFFE000u;
+
 struct ColSrc {
   const uint32_t* col;
   __amdgpu_buffer_rsrc_t rsrc;  // raw buffer over col (and, with col16, over col16 right after it)
@@ -209,7 +225,7 @@ __device__ __forceinline__ ColSrc make_col_src(const uint32_t* col, uint64_t m, 
   ColSrc c;
   c.col = col;
   const uint64_t want = with16 ? m * 4 + ((m + 3) / 2) * 4 : m * 4;
-  const uint64_t bytes = want < 0xFFFFFFFFull ? want : 0xFFFFFFFFull;
+  const uint64_t bytes = want < kBufMax ? want : kBufMax;
   c.rsrc = __builtin_amdgcn_make_buffer_rsrc(const_cast<uint32_t*>(col), (short)0, (int)(uint32_t)bytes, 0x00020000);
   c.o16 = (uint32_t)(m * 4);
   return c;
@@ -285,15 +301,18 @@ __device__ __forceinline__ uint32_t stream_ranges(const ColSrc& src, const uint2
     const int rem = (int)(end - p0);  // elements left for this lane's first load (may be <= 0)
     uint32_t x[U];
     if (BUF) {
-      // an invalid element gets an offset past the buffer: no memory access, reads 0
+      // an invalid element gets an offset past the buffer (kBufOob plus the
+      // slot's constant offset stays below 2^32): no memory access, reads 0.
+      // The slot's offset j G 4 goes in the instruction's scalar offset, so a
+      // slot costs one compare and one select.
       const uint32_t off = p0 * 4u;
 #pragma unroll
       for (int j = 0; j < U; ++j) {
         const bool ok = j * G < rem;
-        x[j] = __builtin_amdgcn_raw_buffer_load_b32(src.rsrc, (int)(ok ? off + (uint32_t)(j * G * 4) : 0xFFFFFFFFu), 0,
-                                                    0);
-        x[j] = ok ? x[j] : kNoKey;
+        x[j] = __builtin_amdgcn_raw_buffer_load_b32(src.rsrc, (int)(ok ? off : kBufOob), j * G * 4, 0);
+        if (!Probe::kZeroSafe) x[j] = ok ? x[j] : kNoKey;
       }
+      __builtin_amdgcn_sched_barrier(0);  // all U loads issue before the first probe waits
     } else {
 #pragma unroll
       for (int j = 0; j < U; ++j) x[j] = (j * G < rem) ? src.col[p0 + j * G] : kNoKey;
@@ -353,8 +372,8 @@ __device__ __forceinline__ uint32_t stream_tiles(const TileSrc& ts, const uint2*
     }
 #pragma unroll
     for (int j = 0; j < U; ++j) {
-      const uint32_t i = b[j] * 64u - base;
-      const unsigned long long w = i < span ? W[i >> 6] : 0ull;
+      const uint32_t i = b[j] * 64u - base;  // words past the window: the zero word after it
+      const unsigned long long w = W[min(i >> 6, span >> 6)];
       cnt += (uint32_t)__popcll(mk[j] & w);
     }
     base_r += (uint32_t)(G * U);
@@ -584,7 +603,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_count_block(const uint32_t* _
   constexpr uint32_t kSpan = 64u * S;  // ids covered by the 2S-word bitmap
   const int attempts = mode & 0xFF;
   const bool bitmaps = !(mode & kNoBitmapBit) && attempts > 0;
-  __shared__ uint32_t T[2 * S];
+  __shared__ __attribute__((aligned(8))) uint32_t T[2 * S + 2];  // bitmap mode: 2S words, then a zero 64-bit word
   __shared__ uint2 R[kChunk];
   __shared__ uint16_t Rh[N16 ? kChunk : 1];  // high halves of narrow ranges
   __shared__ uint32_t s_task, s_next, s_fail;
@@ -606,11 +625,11 @@ __global__ __launch_bounds__(kBlockThreads) void k_count_block(const uint32_t* _
     const bool ranged = (p0 > 0) || (p1 < d);
     const uint32_t lo = col[r0 + p0], hi = col[r0 + p1 - 1];
     const uint32_t base = lo & ~63u;
-    const bool bitmap = bitmaps && (hi - base) < kSpan && hi < 0xFFF00000u;
+    const bool bitmap = bitmaps && (hi - base) < kSpan && hi < 0xFFF00000u && lo != 0;
     uint32_t sa = 0;
     bool ok = false;
     if (bitmap) {
-      for (uint32_t i = threadIdx.x; i < 2 * S; i += kBlockThreads) T[i] = 0u;
+      for (uint32_t i = threadIdx.x; i < 2 * S + 2; i += kBlockThreads) T[i] = 0u;
       __syncthreads();
       for (uint32_t i = p0 + threadIdx.x; i < p1; i += kBlockThreads) {
         const uint32_t y = col[r0 + i] - base;
@@ -1207,7 +1226,7 @@ static void launch_wave_t(const LaunchArgs& a, int slot, uint32_t ntasks, int cu
 template <int LOGS, int KMAX, int G>
 static void launch_wave_g(const LaunchArgs& a, int slot, uint32_t ntasks, int cus, hipStream_t s, tribit_graph* g,
                           const uint2* rng, const uint4* tasks, uint64_t* blocks) {
-  if (g->m * 4 < 0xFFFFFFFFull && !(a.att & kForceGlobal))
+  if (g->m * 4 <= kBufMax && !(a.att & kForceGlobal))
     launch_wave_t<LOGS, KMAX, G, kU, true>(a, slot, ntasks, cus, s, g, rng, tasks, blocks);
   else
     launch_wave_t<LOGS, KMAX, G, kU, false>(a, slot, ntasks, cus, s, g, rng, tasks, blocks);
@@ -1251,7 +1270,7 @@ static void launch_multi(const LaunchArgs& a, int slot, int gw, uint32_t ntasks,
     launch_wave<kC0Logs, kC0K>(a, slot, gw, ntasks, cus, s, g, rng, tasks, blocks);
     return;
   }
-  const bool buf = g->m * 4 < 0xFFFFFFFFull && !(a.att & kForceGlobal);
+  const bool buf = g->m * 4 <= kBufMax && !(a.att & kForceGlobal);
   switch (gw) {
     case 2:
       buf ? launch_multi_t<2, true>(a, slot, ntasks, cus, s, g, rng, tasks, blocks)
@@ -1287,10 +1306,11 @@ static void launch_block_t(const LaunchArgs& a, int slot, uint32_t ntasks, int c
 template <int LOGS, int G>
 static void launch_block_g(const LaunchArgs& a, int slot, uint32_t ntasks, int cus, hipStream_t s, tribit_graph* g,
                            const uint2* rng, const TileSrc& ts, const uint4* tasks, uint64_t* blocks) {
-  const bool buf = g->m * 4 < 0xFFFFFFFFull && !(a.att & kForceGlobal);
+  const bool buf = g->m * 4 <= kBufMax && !(a.att & kForceGlobal);
   // narrow ranges exist only in the wedge layout; opt-in (measured slower than
   // 32-bit ids on R-MAT: the block kernel is not bound by HBM bytes alone)
-  const bool n16 = buf && g->colx && rng == g->rng_sfx && (a.att & kNarrowBit);
+  const bool n16 = buf && g->colx && rng == g->rng_sfx && (a.att & kNarrowBit) &&
+                   g->m * 4 + (g->m + 3) / 2 * 4 <= kBufMax;  // [col | col16] in one buffer
   if (ts.trng) {
     if (n16)
       launch_block_t<LOGS, G, kU, true, true, true>(a, slot, ntasks, cus, s, g, rng, ts, tasks, blocks);
