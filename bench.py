@@ -15,9 +15,11 @@ model in tribit_amd/cost_model.f32) is timed, then the "plain" schedule
 
 Prints ONE JSON line. value = m / (wall-clock seconds per step) / 1e9 (GTEPS),
 with m the undirected simple edges. roofline: the dominant counting kernel's
-algorithmic bytes 4*(rows+1) + 4*edges + 4*W+ over its own rows, divided by its
-HIP-event device time, against 8 TB/s (4 bytes per owner vertex, per edge and
-per wedge check that launch processes). cpu_baseline: the OpenMP CPU oracle
+algorithmic bytes over its own rows, divided by its HIP-event device time,
+against 8 TB/s: 4 bytes per owner vertex and per streamed range (edge), and per
+wedge check 4 bytes (a 32-bit id) or 2 bytes (an id of a narrow wedge range,
+read as its 16-bit low half; each such range also reads one byte, its high
+half relative to its owner's). cpu_baseline: the OpenMP CPU oracle
 (edge iterator, merge intersection) timed on a strided row sample of the same
 CSR, extrapolated to the whole graph by merge work.
 """
@@ -37,10 +39,12 @@ sys.path.insert(0, ROOT)
 
 METRIC = "exact-TC GTEPS on 1 MI355X: learned-sched vs plain; % HBM roofline"
 HBM_PEAK_GBS = 8000.0
-def variant_name(kind: int, width: int) -> str:
-    """Kernel instantiation of a launch of `kind` at lane-group width `width` (32-bit buffer-load path)."""
+def variant_name(kind: int, width: int, narrow: bool = False) -> str:
+    """Kernel instantiation of a launch of `kind` at lane-group width `width` (buffer-load path)."""
+    n16 = "true" if narrow else "false"
     return [f"k_count_wave<6,64,{width},8,true>", f"k_count_wave<9,256,{width},8,true>",
-            f"k_count_block<12,{width},8,true,false,false>", f"k_count_block<12,{width},8,true,true,false>"][kind]
+            f"k_count_block<12,{width},8,true,false,{n16}>", f"k_count_block<12,{width},8,true,true,{n16}>",
+            f"k_count_wave_multi<6,64,8,{width},8,true>"][kind]
 
 
 def log(msg: str) -> None:
@@ -301,7 +305,9 @@ def main() -> int:
     ms_var = sel["ms_var"]
     work = g.variant_stats(args.sched)
     dom = int(np.argmax(ms_var))
-    b_dom = 4.0 * work["rows"][dom] + 4.0 * work["edges"][dom] + 4.0 * work["wplus"][dom]
+    nar = work["narrow"][dom]
+    b_dom = (4.0 * work["rows"][dom] + 4.0 * work["edges"][dom] + 4.0 * (work["wplus"][dom] - nar) + 2.0 * nar +
+             1.0 * work["narrow_ranges"][dom])
     achieved = b_dom / (ms_var[dom] * 1e-3) / 1e9 if ms_var[dom] > 0 else 0.0
     b_alg = 4.0 * (n + 1) + 4.0 * m + 4.0 * inf["wplus"]
 
@@ -350,7 +356,7 @@ def main() -> int:
         "speedup_vs_plain": None if plain is None else round(plain["ms_per_step"] / ms_per_step, 3),
         "roofline": {
             "bound": "hbm",
-            "kernel": variant_name(sel["slot_kind"][dom], sel["group_width"][dom]),
+            "kernel": variant_name(sel["slot_kind"][dom], sel["group_width"][dom], nar > 0),
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",

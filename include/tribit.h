@@ -83,8 +83,9 @@ typedef enum {
 #define TRIBIT_COUNT_FORCE_TILES 8        /* every block launch streams 64-column bit tiles for owners whose
                                              set fits the LDS bitmap (test hook) */
 #define TRIBIT_COUNT_NO_BITMAP 16         /* test hook: block tasks keep hashed sets even for narrow id windows */
-#define TRIBIT_COUNT_NARROW 32            /* wedge ranges whose ids share their high 16 bits are streamed as
-                                             16-bit halves (dword pairs); off by default: slower on R-MAT */
+#define TRIBIT_COUNT_WIDE_IDS 32          /* test hook: wedge ranges whose ids share their high 16 bits are
+                                             streamed as 32-bit ids too (by default block tasks stream them
+                                             from the 16-bit low halves, two ids per load) */
 
 typedef struct {
   int32_t sched;           /* tribit_sched */
@@ -127,6 +128,10 @@ typedef struct {
   uint64_t rows[8];        /* owner lists (a vertex and the lists streamed against N+(vertex)) */
   uint64_t edges[8];       /* edges of those lists */
   uint64_t wplus[8];       /* streamed elements = wedge checks */
+  uint64_t narrow[8];      /* of those, elements read as 16-bit halves (2 bytes each: narrow wedge ranges
+                              streamed by block tasks) */
+  uint64_t narrow_ranges[8]; /* ranges streamed that way (each also reads one byte: its ids' high half
+                                relative to its owner's) */
 } tribit_variant_work;
 
 /* ---- context ------------------------------------------------------------ */

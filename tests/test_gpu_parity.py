@@ -187,12 +187,42 @@ def test_bitmap_and_hashed_sets(gpu_ctx, oracle_mod):
         finally:
             gpu_ctx.set_relabel(True)
     with gpu_ctx.gen_rmat(20, 16, seed=6) as g:
-        a = g.count(sched="wedge").triangles
+        a = g.count(sched="plain").triangles  # the plain layout has no narrow ranges
         for sched in scheds_of(g):
             assert g.count(sched=sched, flags=tribit_amd.NO_BITMAP).triangles == a
-            # 16-bit halves of narrow wedge ranges vs full ids
-            assert g.count(sched=sched, flags=tribit_amd.NARROW).triangles == a
-            assert g.count(sched=sched, flags=tribit_amd.NARROW | tribit_amd.NO_BITMAP).triangles == a
+            # narrow wedge ranges as full ids instead of 16-bit halves
+            assert g.count(sched=sched, flags=tribit_amd.WIDE_IDS).triangles == a
+            assert g.count(sched=sched, flags=tribit_amd.WIDE_IDS | tribit_amd.NO_BITMAP).triangles == a
+
+
+def test_narrow_ranges(gpu_ctx):
+    """Wedge ranges streamed as 16-bit halves (two ids per load): ranges inside one 64K-id window, ranges
+    crossing windows, odd starts and ends, owners cut into 2048-id chunks, hashed and bitmap sets."""
+    note("narrow wedge ranges")
+    k = 2500  # owners of outdeg > 2048: chunked sets, cut ranges
+    a, b = np.triu_indices(k, 1)
+    for spread, shift in ((3, 1), (97, 5), (26, 60000)):  # one window; crossing windows; straddling 2^16
+        ids = np.arange(k, dtype=np.uint64) * np.uint64(spread) + np.uint64(shift)
+        keys = graphs.pack(ids[a], ids[b])
+        n = int(ids[-1]) + 7
+        want = comb(k, 3)
+        with gpu_ctx.graph_from_keys(keys, n) as g:
+            for sched in ("wedge", "learned"):
+                for fl in (0, tribit_amd.NO_BITMAP, tribit_amd.FORCE_LINEAR_PROBE, tribit_amd.WIDE_IDS):
+                    for gw in (None, (1, 4, 4, 4), (2, 8, 64, 32)):
+                        r = g.count(sched=sched, flags=fl, group_width=gw if sched == "wedge" else None)
+                        assert r.triangles == want, (spread, sched, fl, gw)
+    rng = np.random.default_rng(11)
+    for n, m in ((70000, 900000), (300000, 2000000)):
+        u = rng.integers(0, n, m, dtype=np.uint64)
+        v = (u + rng.integers(1, 3000, m, dtype=np.uint64)) % np.uint64(n)
+        keys = graphs.pack(np.minimum(u, v), np.maximum(u, v))
+        keys = keys[(keys >> np.uint64(32)) != (keys & np.uint64(0xFFFFFFFF))]
+        with gpu_ctx.graph_from_keys(keys, n) as g:
+            want = g.count(sched="plain").triangles
+            for fl in (0, tribit_amd.NO_BITMAP, tribit_amd.WIDE_IDS):
+                for gw in (None, (1, 2, 4, 4), (1, 4, 32, 64)):
+                    assert g.count(sched="wedge", flags=fl, group_width=gw).triangles == want, (n, fl, gw)
 
 
 def test_fallback_linear_probe(gpu_ctx, oracle_mod):

@@ -57,7 +57,7 @@ FORCE_GLOBAL_LOADS = 2
 NO_TILES = 4
 FORCE_TILES = 8
 NO_BITMAP = 16
-NARROW = 32
+WIDE_IDS = 32
 OPT_RELABEL = 1
 OPT_BATCH_PART_KEYS = 2
 
@@ -103,7 +103,8 @@ class Profile(ctypes.Structure):
 
 
 class VariantStats(ctypes.Structure):
-    _fields_ = [("rows", ctypes.c_uint64 * 8), ("edges", ctypes.c_uint64 * 8), ("wplus", ctypes.c_uint64 * 8)]
+    _fields_ = [("rows", ctypes.c_uint64 * 8), ("edges", ctypes.c_uint64 * 8), ("wplus", ctypes.c_uint64 * 8),
+                ("narrow", ctypes.c_uint64 * 8), ("narrow_ranges", ctypes.c_uint64 * 8)]
 
 
 _lib = None
@@ -499,7 +500,8 @@ class Graph:
     def variant_stats(self, sched: str = "plain") -> dict:
         vs = VariantStats()
         _check(self.ctx.lib.tribit_variant_stats(self.ctx.h, self.h, SCHED[sched], ctypes.byref(vs)), "variant_stats")
-        return {"rows": list(vs.rows), "edges": list(vs.edges), "wplus": list(vs.wplus)}
+        return {"rows": list(vs.rows), "edges": list(vs.edges), "wplus": list(vs.wplus),
+                "narrow": list(vs.narrow), "narrow_ranges": list(vs.narrow_ranges)}
 
     def free(self) -> None:
         if self.h:

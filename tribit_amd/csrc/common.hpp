@@ -158,8 +158,8 @@ struct tribit_graph {
   uint2* rng_sfx = nullptr;     // m
   uint32_t* sfx_ptr = nullptr;  // n+1, owner v's ranges in rng_sfx
   uint64_t* w_sfx = nullptr;    // n, streamed elements of v's ranges
-  uint32_t* colx = nullptr;     // [col | col16] for the narrow-range stream (built on first use, m < 2^30)
-  uint16_t* col16 = nullptr;    // inside colx: low 16 bits of every id (+ padding)
+  uint16_t* col16 = nullptr;    // low 16 bits of every id, for narrow wedge ranges (relabelled, m < 2^30)
+  uint8_t* sfx_dh = nullptr;    // per wedge range: its ids' high half minus its owner's (255: read col)
   // 64-column bit tiles (tiles.hip), built on first use
   uint64_t tiles64 = 0;          // tile count (computed by preprocessing)
   uint64_t ntiles = 0;           // tiles built (0 until built)
@@ -220,7 +220,6 @@ namespace tb {
 void preprocess(tribit_ctx* ctx, uint64_t* d_keys, uint64_t nkeys, uint64_t n, tribit_graph* g);
 void free_graph_memory(tribit_graph* g);
 void bucket_cache_free(tribit_graph* g);
-void build_col16(tribit_ctx* ctx, tribit_graph* g);
 void free_graph(tribit_graph* g);
 void gen_rmat(tribit_ctx* ctx, uint32_t scale, uint32_t ef, double a, double b, double c, uint64_t seed,
               uint64_t* d_keys);

@@ -58,7 +58,7 @@ constexpr uint32_t kNoKey = 0xFFFFFFFEu;  // never a vertex id (n <= 2^32-2) and
 constexpr uint32_t kEndMask = 0x7FFFFFFFu;
 constexpr uint32_t kNarrowFlag = 0x80000000u;
 constexpr int kNoBitmapBit = 1 << 9;       // kernel mode bit: block tasks never use bitmap sets
-constexpr int kNarrowBit = 1 << 10;        // launch-time flag: narrow ranges streamed as 16-bit halves
+constexpr int kWideBit = 1 << 10;          // launch-time flag: narrow ranges streamed as 32-bit ids too
 constexpr int kCuckooIters = 64;
 constexpr int kCuckooSeeds = 4;
 __device__ __constant__ uint32_t kSeedA[kCuckooSeeds] = {0x9E3779B1u, 0x85EBCA77u, 0xC2B2AE3Du, 0x27D4EB2Fu};
@@ -100,6 +100,7 @@ __device__ __forceinline__ bool cuckoo_insert(uint32_t* T, uint32_t x, uint32_t 
 template <int LOGS>
 struct CuckooProbe {
   static constexpr bool kZeroSafe = false;  // id 0 may be in the set
+  static constexpr bool kRelative = false;
   const uint32_t* T;
   uint32_t a;
   __device__ __forceinline__ uint32_t operator()(uint32_t x, uint32_t = 0) const {
@@ -113,6 +114,7 @@ struct CuckooProbe {
 template <int LOGS>
 struct MultiCuckooProbe {
   static constexpr bool kZeroSafe = false;
+  static constexpr bool kRelative = false;
   const uint32_t* T;
   uint32_t a;
   __device__ __forceinline__ uint32_t operator()(uint32_t x, uint32_t sel) const {
@@ -132,6 +134,7 @@ __device__ __forceinline__ void linear_insert(uint32_t* T, uint32_t w) {
 template <int LOGS2>
 struct LinearProbe {
   static constexpr bool kZeroSafe = false;
+  static constexpr bool kRelative = false;
   const uint32_t* T;
   __device__ __forceinline__ uint32_t operator()(uint32_t w, uint32_t sel = 0) const {
     const uint32_t* t = T + (sel << LOGS2);  // sets of 2^LOGS2 slots side by side (selector 0: one set)
@@ -157,13 +160,14 @@ struct LinearProbe {
 template <int LOGW>
 struct BitmapProbe {
   static constexpr bool kZeroSafe = true;
+  static constexpr bool kRelative = true;  // also probes by window index i = x - base (rel)
   const uint32_t* T;  // 2^LOGW words, then a zero word
   uint32_t base;
-  __device__ __forceinline__ uint32_t operator()(uint32_t x, uint32_t = 0) const {
-    const uint32_t i = x - base;
+  __device__ __forceinline__ uint32_t rel(uint32_t i) const {
     const uint32_t w = min(i >> 5, 1u << LOGW);
     return __builtin_amdgcn_ubfe(T[w], i, 1u);  // the bit field offset is i mod 32
   }
+  __device__ __forceinline__ uint32_t operator()(uint32_t x, uint32_t = 0) const { return rel(x - base); }
 };
 
 template <int G>
@@ -215,35 +219,29 @@ constexpr uint32_t kBufOob = 0xFFFFE000u;
 
 struct ColSrc {
   const uint32_t* col;
-  __amdgpu_buffer_rsrc_t rsrc;  // raw buffer over col (and, with col16, over col16 right after it)
-  uint32_t o16;                 // byte offset of col16 in that buffer (4 m)
+  __amdgpu_buffer_rsrc_t rsrc;    // raw buffer over col
+  __amdgpu_buffer_rsrc_t rsrc16;  // raw buffer over col16 (narrow ranges), when given
 };
 
-// col16, when built, lives in the same allocation right after col (the low
-// 16 bits of every id), so one buffer resource covers both.
-__device__ __forceinline__ ColSrc make_col_src(const uint32_t* col, uint64_t m, bool with16 = false) {
+__device__ __forceinline__ ColSrc make_col_src(const uint32_t* col, uint64_t m, const uint16_t* col16 = nullptr) {
   ColSrc c;
   c.col = col;
-  const uint64_t want = with16 ? m * 4 + ((m + 3) / 2) * 4 : m * 4;
-  const uint64_t bytes = want < kBufMax ? want : kBufMax;
+  const uint64_t bytes = m * 4 < kBufMax ? m * 4 : kBufMax;
   c.rsrc = __builtin_amdgcn_make_buffer_rsrc(const_cast<uint32_t*>(col), (short)0, (int)(uint32_t)bytes, 0x00020000);
-  c.o16 = (uint32_t)(m * 4);
+  const uint64_t bytes16 = col16 ? 4 * ((m + 1) / 2 + 1) : 0;  // col16 is padded to whole dwords plus one
+  c.rsrc16 = __builtin_amdgcn_make_buffer_rsrc(const_cast<uint16_t*>(col16 ? col16 : reinterpret_cast<const uint16_t*>(col)),
+                                               (short)0, (int)(uint32_t)(bytes16 < kBufMax ? bytes16 : kBufMax),
+                                               0x00020000);
   return c;
 }
 
-// N16: a range flagged narrow is streamed from col16 as dword pairs: each
-// load returns two consecutive ids' low halves (the high half of the range,
-// staged in Rh, is or-ed back), so a narrow group covers 2 G U ids per step
-// and every lane still issues one 32-bit load per slot.
-template <int G, int U, bool SHARED, bool BUF, typename Probe, bool N16 = false>
+template <int G, int U, bool SHARED, bool BUF, typename Probe>
 __device__ __forceinline__ uint32_t stream_ranges(const ColSrc& src, const uint2* R, uint32_t nk, uint32_t* s_next,
-                                                  uint32_t lane, const Probe& probe, const uint16_t* Rh = nullptr,
-                                                  const uint8_t* Rsel = nullptr) {
+                                                  uint32_t lane, const Probe& probe, const uint8_t* Rsel = nullptr) {
   constexpr uint64_t kLeads = leader_mask<G>();
   const uint32_t g = lane / G, gl = lane % G;
   const uint64_t below = (g == 0) ? 0ull : ((1ull << (g * G)) - 1);
-  uint32_t base = 0, end = 0, cnt = 0, next = 0, hi = 0, lo = 0, sel = 0;
-  bool f16 = false;
+  uint32_t base = 0, end = 0, cnt = 0, next = 0, sel = 0;
   for (;;) {
     const bool need = base >= end;  // uniform within a group
     const uint64_t want = __ballot(need && gl == 0);
@@ -266,36 +264,7 @@ __device__ __forceinline__ uint32_t stream_ranges(const ColSrc& src, const uint2
         base = r.x;
         end = r.y & kEndMask;
         if (Rsel) sel = Rsel[k];
-        if (N16) {
-          f16 = (r.y & kNarrowFlag) != 0;
-          hi = (uint32_t)Rh[k] << 16;
-          lo = base;
-          if (f16) base &= ~1u;  // pairs start at even ids
-        }
       }
-    }
-    if (N16 && BUF) {
-      // one dword per slot: an id (wide ranges) or two low halves (narrow ranges)
-      const uint32_t e0 = base + (f16 ? 2 * gl : gl);
-      const uint32_t step = f16 ? 2 * G : G;
-      const uint32_t ob = f16 ? src.o16 + 2 * e0 : 4 * e0, ostep = f16 ? 4 * G : 4 * G;
-      uint32_t w[U];
-#pragma unroll
-      for (int j = 0; j < U; ++j) {  // all U loads in flight before the first probe
-        const uint32_t e = e0 + j * step;
-        const bool any = (e >= lo && e < end) || (f16 && e + 1 < end);
-        w[j] = __builtin_amdgcn_raw_buffer_load_b32(src.rsrc, (int)(any ? ob + j * ostep : 0xFFFFFFFFu), 0, 0);
-      }
-#pragma unroll
-      for (int j = 0; j < U; ++j) {
-        const uint32_t e = e0 + j * step;
-        const bool ok0 = e >= lo && e < end, ok1 = f16 && e + 1 < end;
-        const uint32_t k0 = ok0 ? (f16 ? (hi | (w[j] & 0xFFFFu)) : w[j]) : kNoKey;
-        const uint32_t k1 = ok1 ? (hi | (w[j] >> 16)) : kNoKey;
-        cnt += probe(k0, sel) + probe(k1, sel);
-      }
-      base += (f16 ? 2u : 1u) * (uint32_t)(G * U);
-      continue;
     }
     const uint32_t p0 = base + gl;
     const int rem = (int)(end - p0);  // elements left for this lane's first load (may be <= 0)
@@ -320,6 +289,80 @@ __device__ __forceinline__ uint32_t stream_ranges(const ColSrc& src, const uint2
 #pragma unroll
     for (int j = 0; j < U; ++j) cnt += probe(x[j], sel);
     base += (uint32_t)(G * U);
+  }
+  return cnt;
+}
+
+// Narrow ranges (ids sharing their high 16 bits, flagged by the range
+// builder) streamed from col16, the low halves of all ids: one 32-bit load
+// returns two consecutive ids, the range's high half (staged in Rh) is or-ed
+// back. A slot of lane gl covers dword q = qb + gl + j G, positions 2q and
+// 2q + 1; the range [b, e) may start at an odd position (the first half of
+// its first dword lies outside it) and end at one. Block tasks only (a
+// block-shared counter hands out the ranges).
+template <int G, int U, typename Probe>
+__device__ __forceinline__ uint32_t stream_narrow(const ColSrc& src, const uint2* R, const uint16_t* Rh, uint32_t nk,
+                                                  uint32_t* s_next, uint32_t lane, const Probe& probe) {
+  constexpr uint64_t kLeads = leader_mask<G>();
+  const uint32_t g = lane / G, gl = lane % G;
+  const uint64_t below = (g == 0) ? 0ull : ((1ull << (g * G)) - 1);
+  uint32_t qb = 0, b = 0, e = 0, hi = 0, cnt = 0;
+  for (;;) {
+    const bool need = 2 * qb >= e;  // uniform within a group
+    const uint64_t want = __ballot(need && gl == 0);
+    const uint32_t nw = (uint32_t)__popcll(want);
+    uint32_t kb = 0;
+    if (nw) {
+      if (lane == 0) kb = atomicAdd(s_next, nw);
+      kb = __shfl(kb, 0);
+      if (want == kLeads && kb >= nk) break;
+    }
+    if (need) {
+      const uint32_t k = kb + (uint32_t)__popcll(want & below);
+      if (k < nk) {
+        const uint2 r = R[k];
+        b = r.x;
+        e = r.y;
+        qb = b >> 1;
+        hi = (uint32_t)Rh[k] << 16;
+      }
+    }
+    const uint32_t q0 = qb + gl;
+    const int rem = (int)(e - 2 * q0);  // positions left from this lane's first dword (may be <= 0)
+    const bool skip0 = 2 * q0 < b;      // first dword of a range starting at an odd position
+    const uint32_t off = 4 * q0;
+    uint32_t w[U];
+#pragma unroll
+    for (int j = 0; j < U; ++j)
+      w[j] = __builtin_amdgcn_raw_buffer_load_b32(src.rsrc16, (int)(2 * j * G < rem ? off : kBufOob), j * G * 4, 0);
+    __builtin_amdgcn_sched_barrier(0);
+    if constexpr (Probe::kRelative) {
+      // window indexes hi + half - base: one add per id; a masked slot or
+      // half gets an offset that puts it past the window
+      const uint32_t dh = hi - probe.base;
+#pragma unroll
+      for (int j = 0; j < U; ++j) {
+        const uint32_t d = 2 * j * G < rem ? dh : 0xFFFF0000u;
+        uint32_t i0 = (w[j] & 0xFFFFu) + d;
+        uint32_t i1 = (w[j] >> 16) + d;
+        if (j == 0) i0 = skip0 ? 0xFFFFFFFFu : i0;
+        i1 = 2 * j * G + 1 < rem ? i1 : 0xFFFFFFFFu;
+        cnt += probe.rel(i0) + probe.rel(i1);
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < U; ++j) {
+        const bool ok = 2 * j * G < rem;  // a masked slot read 0: both halves become 0 (or kNoKey)
+        const uint32_t h = ok ? hi : 0u;
+        uint32_t x0 = (w[j] & 0xFFFFu) | h;
+        uint32_t x1 = (w[j] >> 16) | h;
+        if (j == 0) x0 = skip0 ? kNoKey : x0;
+        x1 = 2 * j * G + 1 < rem ? x1 : kNoKey;
+        if (!Probe::kZeroSafe) x0 = ok ? x0 : kNoKey;
+        cnt += probe(x0) + probe(x1);
+      }
+    }
+    qb += (uint32_t)(G * U);
   }
   return cnt;
 }
@@ -562,10 +605,8 @@ __global__ __launch_bounds__(256) void k_count_wave_multi(const uint32_t* __rest
           Sel[k] = (uint8_t)j;
         }
         wave_lds_sync();
-        cnt += ok ? stream_ranges<G, U, false, BUF>(src, R, nk, nullptr, lane, MultiCuckooProbe<LOGS>{T, sa},
-                                                     nullptr, Sel)
-                  : stream_ranges<G, U, false, BUF>(src, R, nk, nullptr, lane, LinearProbe<LOGS + 1>{T}, nullptr,
-                                                     Sel);
+        cnt += ok ? stream_ranges<G, U, false, BUF>(src, R, nk, nullptr, lane, MultiCuckooProbe<LOGS>{T, sa}, Sel)
+                  : stream_ranges<G, U, false, BUF>(src, R, nk, nullptr, lane, LinearProbe<LOGS + 1>{T}, Sel);
         wave_lds_sync();
       }
       acc += cnt;
@@ -592,6 +633,7 @@ template <int LOGS, int G, int U, bool BUF, bool TILES, bool N16>
 __global__ __launch_bounds__(kBlockThreads) void k_count_block(const uint32_t* __restrict__ row_ptr,
                                                                const uint32_t* __restrict__ col, uint64_t m,
                                                                const uint16_t* __restrict__ col16,
+                                                               const uint8_t* __restrict__ dh,
                                                                const uint2* __restrict__ rng, TileSrc ts,
                                                                const uint4* __restrict__ tasks, uint32_t ntasks,
                                                                uint32_t* __restrict__ queue,
@@ -606,10 +648,10 @@ __global__ __launch_bounds__(kBlockThreads) void k_count_block(const uint32_t* _
   __shared__ __attribute__((aligned(8))) uint32_t T[2 * S + 2];  // bitmap mode: 2S words, then a zero 64-bit word
   __shared__ uint2 R[kChunk];
   __shared__ uint16_t Rh[N16 ? kChunk : 1];  // high halves of narrow ranges
-  __shared__ uint32_t s_task, s_next, s_fail;
+  __shared__ uint32_t s_task, s_next, s_fail, s_next16, s_n16, s_nwide;
   __shared__ unsigned long long s_part[kBlockThreads / 64];
   const uint32_t lane = threadIdx.x & 63;
-  const ColSrc src = make_col_src(col, m, N16);
+  const ColSrc src = make_col_src(col, m, N16 ? col16 : nullptr);
   uint64_t acc = 0;
   uint32_t fallbacks = 0;
   for (;;) {
@@ -662,33 +704,51 @@ __global__ __launch_bounds__(kBlockThreads) void k_count_block(const uint32_t* _
     const uint2* rs = tiles ? ts.trng : rng;
     for (uint32_t c0 = 0; c0 < rcount; c0 += kChunk) {
       const uint32_t nk = min((uint32_t)kChunk, rcount - c0);
+      if (N16) {  // narrow ranges are staged from the front, the others from the back
+        if (threadIdx.x == 0) s_n16 = s_nwide = 0;
+        __syncthreads();
+      }
       for (uint32_t k = threadIdx.x; k < nk; k += kBlockThreads) {
         uint2 r = rs[rbase + c0 + k];
+        uint32_t at = k;
         if (!tiles) {
-          const uint32_t fl = r.y & kNarrowFlag;
+          const bool narrow = (r.y & kNarrowFlag) != 0;
           uint32_t e = r.y & kEndMask;
-          if (N16 && fl) Rh[k] = (uint16_t)(col[r.x] >> 16);  // flagged ranges are non-empty
           if (ranged) {  // tiles outside the chunk's window miss in the bitmap
             r.x = lower_bound_u32(col, r.x, e, lo);
             e = upper_bound_u32(col, r.x, e, hi);
           }
-          r.y = e | (N16 ? fl : 0u);
+          r.y = e;
+          if (N16) {
+            if (narrow && r.x < e) {
+              at = atomicAdd(&s_n16, 1u);
+              const uint32_t d8 = dh[rbase + c0 + k];  // high half relative to the owner's: no read of col
+              Rh[at] = (uint16_t)(d8 < 255u ? (x >> 16) + d8 : col[r.x] >> 16);
+            } else {
+              at = nk - 1 - atomicAdd(&s_nwide, 1u);
+            }
+          }
         }
-        R[k] = r;
+        R[at] = r;
       }
-      if (threadIdx.x == 0) s_next = 0;
+      if (threadIdx.x == 0) s_next = s_next16 = 0;
       __syncthreads();
-      if (tiles)
+      const uint32_t n16 = N16 ? s_n16 : 0u;  // R[0, n16) narrow, R[n16, nk) the others
+      if (tiles) {
         cnt += stream_tiles<G, U>(ts, R, nk, &s_next, lane, T, base, kSpan);
-      else if (bitmap)
-        cnt += stream_ranges<G, U, true, BUF, BitmapProbe<LOGS + 1>, N16>(src, R, nk, &s_next, lane,
-                                                                          BitmapProbe<LOGS + 1>{T, base}, Rh);
-      else if (ok)
-        cnt += stream_ranges<G, U, true, BUF, CuckooProbe<LOGS>, N16>(src, R, nk, &s_next, lane,
-                                                                      CuckooProbe<LOGS>{T, sa}, Rh);
-      else
-        cnt += stream_ranges<G, U, true, BUF, LinearProbe<LOGS + 1>, N16>(src, R, nk, &s_next, lane,
-                                                                          LinearProbe<LOGS + 1>{T}, Rh);
+      } else if (bitmap) {
+        const BitmapProbe<LOGS + 1> pr{T, base};
+        if (N16 && n16) cnt += stream_narrow<G, U>(src, R, Rh, n16, &s_next16, lane, pr);
+        cnt += stream_ranges<G, U, true, BUF>(src, R + n16, nk - n16, &s_next, lane, pr);
+      } else if (ok) {
+        const CuckooProbe<LOGS> pr{T, sa};
+        if (N16 && n16) cnt += stream_narrow<G, U>(src, R, Rh, n16, &s_next16, lane, pr);
+        cnt += stream_ranges<G, U, true, BUF>(src, R + n16, nk - n16, &s_next, lane, pr);
+      } else {
+        const LinearProbe<LOGS + 1> pr{T};
+        if (N16 && n16) cnt += stream_narrow<G, U>(src, R, Rh, n16, &s_next16, lane, pr);
+        cnt += stream_ranges<G, U, true, BUF>(src, R + n16, nk - n16, &s_next, lane, pr);
+      }
       __syncthreads();
     }
     acc += cnt;
@@ -956,6 +1016,39 @@ __global__ __launch_bounds__(256) void k_bucket_stats(SchedView sv, const uint32
     if (acc[i]) atomicAdd(&st[i], acc[i]);
 }
 
+// Per slot of a wedge plan: elements and ranges of narrow wedge ranges (the
+// lists' kept ranges whose end carries the narrow flag).
+__global__ __launch_bounds__(256) void k_narrow_stats(SchedView sv, uint64_t n, const uint8_t* __restrict__ table,
+                                                      unsigned long long* __restrict__ out) {
+  __shared__ unsigned long long acc[2 * kSlots];
+  if (threadIdx.x < 2 * kSlots) acc[threadIdx.x] = 0ull;
+  __syncthreads();
+  const uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (x < n) {
+    const uint32_t d = sv.row_ptr[x + 1] - sv.row_ptr[x];
+    uint32_t b, k;
+    uint64_t w;
+    owner_list(sv, x, 0, &b, &k, &w);
+    if (list_useful(sv, 0, d, k)) {
+      const int slot = table[bucket_of(d, w, k)];
+      unsigned long long ne = 0, nr = 0;
+      for (uint32_t i = b; i < b + k; ++i) {
+        const uint2 r = sv.rng[i];
+        if (r.y & kNarrowFlag) {
+          ne += (r.y & kEndMask) - r.x;
+          ++nr;
+        }
+      }
+      if (nr) {
+        atomicAdd(&acc[slot], ne);
+        atomicAdd(&acc[kSlots + slot], nr);
+      }
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < 2 * kSlots && acc[threadIdx.x]) atomicAdd(&out[threadIdx.x], acc[threadIdx.x]);
+}
+
 __global__ void k_zero_u64(unsigned long long* p, uint32_t n) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0ull;
@@ -1039,11 +1132,14 @@ static Plan menu_plan() {
   return p;
 }
 
-// The density-threshold heuristic over the menu (fallback of LEARNED).
-static void heuristic_table(uint8_t* table) {
+// The density-threshold heuristic over the menu (fallback of LEARNED): block
+// tasks stream 8 lanes per list on the wedge layout (narrow ranges: two ids
+// per load), 16 on the others.
+static void heuristic_table(uint8_t* table, int layout) {
+  const uint8_t blk = layout == TRIBIT_SCHED_WEDGE ? 5 : 6;
   for (int b = 0; b < kBuckets; ++b) {
     const int kind = kind_of_dbin(b / kLBins);
-    table[b] = (uint8_t)(kind == kKindWave64 ? 0 : kind == kKindWave512 ? 3 : 6);
+    table[b] = kind == kKindWave64 ? 0 : kind == kKindWave512 ? 3 : blk;
   }
 }
 
@@ -1055,7 +1151,7 @@ static void default_widths(int sched, int gw[4]) {
   if (sched == TRIBIT_SCHED_PLAIN) {
     gw[0] = 32, gw[1] = 16, gw[2] = 32, gw[3] = 16;
   } else if (sched == TRIBIT_SCHED_WEDGE) {
-    gw[0] = 1, gw[1] = 4, gw[2] = 16, gw[3] = 8;
+    gw[0] = 1, gw[1] = 4, gw[2] = 8, gw[3] = 8;
   } else {
     gw[0] = 1, gw[1] = 8, gw[2] = 16, gw[3] = 16;
   }
@@ -1156,7 +1252,7 @@ uint32_t bucket_max_outdeg(int bucket) { return d_bin_top(bucket / kLBins); }
 // cost 2^model(features) * work. Without a model, or when a prediction is not
 // finite, the heuristic's slot.
 static void learned_table(tribit_ctx* ctx, const BucketStats& st, int layout, int flags, uint8_t* table) {
-  heuristic_table(table);
+  heuristic_table(table, layout);
   if (!ctx->model || layout != TRIBIT_SCHED_WEDGE) return;  // the model is trained on wedge-schedule timings
   std::vector<float> X;
   std::vector<std::pair<int, int>> ids;
@@ -1189,7 +1285,7 @@ static void learned_table(tribit_ctx* ctx, const BucketStats& st, int layout, in
   for (int b = 0; b < kBuckets; ++b)
     if (best[b] == -INFINITY) {
       uint8_t h[kBuckets];
-      heuristic_table(h);
+      heuristic_table(h, layout);
       table[b] = h[b];
     }
 }
@@ -1297,7 +1393,7 @@ static void launch_block_t(const LaunchArgs& a, int slot, uint32_t ntasks, int c
   if (per_cu < 1) per_cu = 1;
   const unsigned grid = (unsigned)std::min<uint64_t>(ntasks, (uint64_t)cus * per_cu);
   k_count_block<LOGS, G, U, BUF, TILES, N16><<<grid, kBlockThreads, 0, s>>>(
-      g->row_ptr, N16 ? g->colx : g->col, g->m, g->col16, rng, ts, tasks, ntasks, a.queue + slot, a.result,
+      g->row_ptr, g->col, g->m, g->col16, g->sfx_dh, rng, ts, tasks, ntasks, a.queue + slot, a.result,
       a.att & ~kForceGlobal,
       a.tlog ? a.tlog + (size_t)slot * kBuckets : nullptr, a.gid, a.gcount);
   *blocks += grid;
@@ -1307,10 +1403,8 @@ template <int LOGS, int G>
 static void launch_block_g(const LaunchArgs& a, int slot, uint32_t ntasks, int cus, hipStream_t s, tribit_graph* g,
                            const uint2* rng, const TileSrc& ts, const uint4* tasks, uint64_t* blocks) {
   const bool buf = g->m * 4 <= kBufMax && !(a.att & kForceGlobal);
-  // narrow ranges exist only in the wedge layout; opt-in (measured slower than
-  // 32-bit ids on R-MAT: the block kernel is not bound by HBM bytes alone)
-  const bool n16 = buf && g->colx && rng == g->rng_sfx && (a.att & kNarrowBit) &&
-                   g->m * 4 + (g->m + 3) / 2 * 4 <= kBufMax;  // [col | col16] in one buffer
+  // narrow ranges exist only in the wedge layout (col16 is built with it)
+  const bool n16 = buf && g->col16 && g->sfx_dh && rng == g->rng_sfx && !(a.att & kWideBit);
   if (ts.trng) {
     if (n16)
       launch_block_t<LOGS, G, kU, true, true, true>(a, slot, ntasks, cus, s, g, rng, ts, tasks, blocks);
@@ -1349,7 +1443,7 @@ static float elapsed(hipEvent_t a, hipEvent_t b) {
 static int mode_word(int flags) {
   return ((flags & TRIBIT_COUNT_FORCE_LINEAR_PROBE) ? 0 : kCuckooSeeds) |
          ((flags & TRIBIT_COUNT_FORCE_GLOBAL_LOADS) ? kForceGlobal : 0) |
-         ((flags & TRIBIT_COUNT_NO_BITMAP) ? kNoBitmapBit : 0) | ((flags & TRIBIT_COUNT_NARROW) ? kNarrowBit : 0);
+         ((flags & TRIBIT_COUNT_NO_BITMAP) ? kNoBitmapBit : 0) | ((flags & TRIBIT_COUNT_WIDE_IDS) ? kWideBit : 0);
 }
 
 // Tasks of one count under a plan, ready to launch.
@@ -1396,7 +1490,6 @@ void make_schedule(tribit_ctx* ctx, tribit_graph* g, int layout, int flags, bool
     if ((flags & TRIBIT_COUNT_FORCE_TILES) && plan.kind[i] == kKindBlock) plan.kind[i] = kKindTile;
     if ((flags & TRIBIT_COUNT_NO_TILES) && plan.kind[i] == kKindTile) plan.kind[i] = kKindBlock;
   }
-  if ((flags & TRIBIT_COUNT_NARROW) && layout == TRIBIT_SCHED_WEDGE) build_col16(ctx, g);
   bool any_tiles = false;
   for (int i = 0; i < plan.nslots; ++i) any_tiles |= plan.kind[i] == kKindTile;
   if (any_tiles && g->m) {
@@ -1527,7 +1620,7 @@ static Plan plan_for(int sched, const int32_t* override_gw, int* layout) {
   if (sched == TRIBIT_SCHED_LEARNED) {
     *layout = -1;  // resolved by the caller
     Plan p = menu_plan();
-    heuristic_table(p.table);
+    heuristic_table(p.table, TRIBIT_SCHED_WEDGE);  // re-filled for the layout that runs
     return p;
   }
   *layout = sched;
@@ -1570,10 +1663,12 @@ static bool candidate(tribit_ctx* ctx, tribit_graph* g, const tribit_count_opts&
       return true;
     case kCandHeur:
       *out = {pref, plan_for(TRIBIT_SCHED_LEARNED, nullptr, &dummy), false, core};
+      heuristic_table(out->plan.table, pref);
       return true;
     case kCandDirected:
       if (pref != TRIBIT_SCHED_WEDGE) return false;  // already candidate 1
       *out = {TRIBIT_SCHED_DIRECTED, plan_for(TRIBIT_SCHED_LEARNED, nullptr, &dummy), false, core};
+      heuristic_table(out->plan.table, TRIBIT_SCHED_DIRECTED);
       return true;
     default:
       *out = {TRIBIT_SCHED_PLAIN, plan_for(TRIBIT_SCHED_PLAIN, nullptr, &dummy), false, 0};
@@ -1679,7 +1774,7 @@ void count_profile(tribit_ctx* ctx, tribit_graph* g, int layout, const uint8_t* 
       plan.table[b] = table[b];
     }
   } else {
-    heuristic_table(plan.table);
+    heuristic_table(plan.table, layout);
   }
   BucketStats st;
   bucket_stats(ctx, g, layout, &st);
@@ -1719,9 +1814,10 @@ void learned_plan(tribit_ctx* ctx, tribit_graph* g, uint8_t* table) {
 // heuristic before the first count), streamed work outside its dense core.
 void variant_stats(tribit_ctx* ctx, tribit_graph* g, int sched, tribit_variant_work* out) {
   memset(out, 0, sizeof(*out));
-  int layout = 0;
+  int layout = 0, flags = 0;
   Plan plan = plan_for(sched, nullptr, &layout);
   BucketStats st;
+  uint32_t d0 = 0xFFFFFFFFu, cut = 0;
   if (sched == TRIBIT_SCHED_LEARNED) {
     tribit_count_opts o;
     memset(&o, 0, sizeof(o));
@@ -1732,7 +1828,6 @@ void variant_stats(tribit_ctx* ctx, tribit_graph* g, int sched, tribit_variant_w
     candidate(ctx, g, o, g->judge_choice >= 0 ? g->judge_choice : kCandHeur, &cr);
     layout = cr.layout;
     plan = cr.plan;
-    uint32_t d0 = 0xFFFFFFFFu, cut = 0;
     if (cr.core_c) {
       d0 = (uint32_t)(g->n - cr.core_c);
       uint32_t e0 = 0;
@@ -1742,6 +1837,7 @@ void variant_stats(tribit_ctx* ctx, tribit_graph* g, int sched, tribit_variant_w
     }
     bucket_stats(ctx, g, layout, &st, d0, cut);
     if (cr.model) learned_table(ctx, st, layout, 0, plan.table);
+    flags = o.flags;
   } else {
     bucket_stats(ctx, g, layout, &st);
   }
@@ -1750,6 +1846,23 @@ void variant_stats(tribit_ctx* ctx, tribit_graph* g, int sched, tribit_variant_w
     out->rows[s] += st.lists[b];
     out->edges[s] += st.sum_k[b];
     out->wplus[s] += st.sum_w[b];
+  }
+  // block launches stream narrow wedge ranges as 16-bit halves
+  if (layout == TRIBIT_SCHED_WEDGE && g->col16 && g->m * 4 <= kBufMax && !(flags & TRIBIT_COUNT_WIDE_IDS) &&
+      !(flags & TRIBIT_COUNT_FORCE_GLOBAL_LOADS) && g->n) {
+    DevBuf<uint8_t> dt(ctx, kBuckets);
+    DevBuf<unsigned long long> ns(ctx, 2 * kSlots);
+    TB_HIP(hipMemcpyAsync(dt.p, plan.table, kBuckets, hipMemcpyHostToDevice, ctx->stream));
+    k_zero_u64<<<1, 64, 0, ctx->stream>>>(ns.p, 2 * kSlots);
+    k_narrow_stats<<<grid_for(g->n, 256), 256, 0, ctx->stream>>>(sched_view(g, layout, d0, cut), g->n, dt.p, ns.p);
+    unsigned long long h[2 * kSlots];
+    TB_HIP(hipMemcpyAsync(h, ns.p, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+    ctx->sync();
+    for (int s = 0; s < plan.nslots; ++s)
+      if (plan.kind[s] == kKindBlock) {
+        out->narrow[s] = h[s];
+        out->narrow_ranges[s] = h[kSlots + s];
+      }
   }
 }
 

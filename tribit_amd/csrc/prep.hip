@@ -335,11 +335,21 @@ __global__ void k_col16(const uint32_t* __restrict__ col, uint64_t m, uint16_t* 
   if (e < m) col16[e] = (uint16_t)(col[e] & 0xFFFFu);
 }
 
-__global__ void k_flag_narrow(uint2* __restrict__ rng, uint64_t m, const uint32_t* __restrict__ col) {
+// Wedge range k = [b, e) (the part of N+(u) after v, v = col[b - 1]) is
+// narrow when its ids share their high 16 bits H; dh[k] = H - (v >> 16)
+// (255: not representable, the kernel reads col[b] instead).
+__global__ void k_flag_narrow(uint2* __restrict__ rng, uint64_t m, const uint32_t* __restrict__ col,
+                              uint8_t* __restrict__ dh) {
   const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= m) return;
   const uint2 r = rng[k];
-  if (r.x < r.y && (col[r.x] >> 16) == (col[r.y - 1] >> 16)) rng[k].y = r.y | 0x80000000u;
+  uint32_t d = 0;
+  if (r.x < r.y && (col[r.x] >> 16) == (col[r.y - 1] >> 16)) {
+    rng[k].y = r.y | 0x80000000u;
+    d = (col[r.x] >> 16) - (col[r.x - 1] >> 16);  // ids after v: H >= v's high half
+    d = d < 255u ? d : 255u;
+  }
+  dh[k] = (uint8_t)d;
 }
 
 __global__ void k_fill_u64(unsigned long long* __restrict__ p, uint64_t n) {
@@ -537,10 +547,19 @@ void preprocess(tribit_ctx* ctx, uint64_t* d_keys_in, uint64_t nkeys, uint64_t n
     });
   }
   rend.reset();
-  // 10. narrow wedge ranges (ids sharing their high 16 bits) are flagged; the
-  // 16-bit copy of col they can be streamed from is built on first use
-  if (ctx->relabel && m && m < (1ull << 30))
-    k_flag_narrow<<<grid_for(m, kBlock), kBlock, 0, s>>>(rng_sfx.p, m, col.p);
+  // 10. narrow wedge ranges (ids sharing their high 16 bits) are flagged and
+  // col16, the low 16 bits of every id, is built for streaming them (padded
+  // to whole dwords plus one)
+  DevBuf<uint32_t> col16;
+  DevBuf<uint8_t> sfx_dh;
+  if (ctx->relabel && m && m < (1ull << 30)) {
+    sfx_dh = DevBuf<uint8_t>(ctx, m);
+    k_flag_narrow<<<grid_for(m, kBlock), kBlock, 0, s>>>(rng_sfx.p, m, col.p, sfx_dh.p);
+    const uint64_t words = (m + 1) / 2 + 1;
+    col16 = DevBuf<uint32_t>(ctx, words);
+    k_fill_u32<<<1, 64, 0, s>>>(col16.p + words - 2, 2, 0u);
+    k_col16<<<grid_for(m, kBlock), kBlock, 0, s>>>(col.p, m, reinterpret_cast<uint16_t*>(col16.p));
+  }
 
   // the graph owns its buffers from here on (freed by tribit_graph_free, also on error)
   g->row_ptr = row_ptr.release();
@@ -558,11 +577,13 @@ void preprocess(tribit_ctx* ctx, uint64_t* d_keys_in, uint64_t nkeys, uint64_t n
   g->w_fw = w_fw.release();
   g->w_rv = w_rv.release();
   g->rng_sfx = rng_sfx.release();
+  g->col16 = reinterpret_cast<uint16_t*>(col16.release());
+  g->sfx_dh = sfx_dh.release();
 
   g->sfx_ptr = sfx_ptr.release();
   g->w_sfx = w_sfx.release();
   g->bytes = 4 * (n + 1) * 3 + 4 * (m ? m : 1) + 20 * (m ? m : 1) + 4 * n + 8 * n * 3 + (g->labels ? 4 * n : 0) +
-             (g->rng_sfx ? 8 * (m ? m : 1) + 4 * (n + 1) + 8 * n : 0);
+             (g->rng_sfx ? 8 * (m ? m : 1) + 4 * (n + 1) + 8 * n : 0) + (g->col16 ? 4 * ((m + 1) / 2 + 1) + m : 0);
   g->tiles64 = m ? count_tiles(ctx, g) : 0;
   TB_HIP(hipEventRecord(ctx->ev[1], s));
   TB_HIP(hipEventSynchronize(ctx->ev[1]));
@@ -573,23 +594,6 @@ void preprocess(tribit_ctx* ctx, uint64_t* d_keys_in, uint64_t nkeys, uint64_t n
   g->wplus = hst.wplus;
   g->rows_active = hst.rows_active;
   g->max_outdeg = hst.max_outdeg;
-}
-
-// Combined copy [col | col16] for the narrow-range stream (built on first
-// use): col16 holds the low 16 bits of every id, padded to whole dwords so a
-// pair read at the last even position stays inside the buffer.
-void build_col16(tribit_ctx* ctx, tribit_graph* g) {
-  if (g->colx || !g->m || g->m >= (1ull << 30)) return;
-  const uint64_t m = g->m;
-  hipStream_t s = ctx->stream;
-  DevBuf<uint32_t> x(ctx, m + (m + 3) / 2);
-  TB_HIP(hipMemcpyAsync(x.p, g->col, 4 * m, hipMemcpyDeviceToDevice, s));
-  k_fill_u32<<<1, 64, 0, s>>>(x.p + m + (m + 3) / 2 - 1, 1, 0u);
-  k_col16<<<grid_for(m, kBlock), kBlock, 0, s>>>(g->col, m, reinterpret_cast<uint16_t*>(x.p + m));
-  TB_HIP(hipGetLastError());
-  g->colx = x.release();
-  g->col16 = reinterpret_cast<uint16_t*>(g->colx + m);
-  g->bytes += 6 * m;
 }
 
 // Releases every device buffer of a graph (the struct stays).
@@ -604,9 +608,10 @@ void free_graph_memory(tribit_graph* g) {
   g->ctx->free(g->labels);
   g->labels = nullptr;
   g->ctx->free(g->rng_sfx);
-  g->ctx->free(g->colx);
-  g->colx = nullptr;
+  g->ctx->free(g->col16);
   g->col16 = nullptr;
+  g->ctx->free(g->sfx_dh);
+  g->sfx_dh = nullptr;
   g->ctx->free(g->sfx_ptr);
   g->ctx->free(g->w_sfx);
   g->rng_sfx = nullptr;
