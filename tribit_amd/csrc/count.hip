@@ -20,6 +20,8 @@
 #include <cstring>
 #include <algorithm>
 #include <chrono>
+#include <rocprim/block/block_reduce.hpp>
+#include <rocprim/block/block_scan.hpp>
 #include <rocprim/device/device_scan.hpp>
 #include <vector>
 
@@ -931,33 +933,63 @@ __device__ __forceinline__ uint32_t list_chunks(int kind, uint32_t d) {
 }
 
 // Per owner: tasks per slot.
-__global__ void k_sched_count(SchedView sv, PlanDev pd, uint64_t n, Cnt8* __restrict__ cnt) {
-  const uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (x > n) return;
+// Tasks of owner x per slot.
+__device__ __forceinline__ Cnt8 owner_counts(const SchedView& sv, const PlanDev& pd, uint64_t x) {
   Cnt8 c;
   for (int i = 0; i < kSlots; ++i) c.v[i] = 0;
-  if (x < n) {
-    const uint32_t d = sv.row_ptr[x + 1] - sv.row_ptr[x];
-    for (int l = 0; l < sv.nlists; ++l) {
-      uint32_t b, k;
-      uint64_t w;
-      owner_list(sv, x, l, &b, &k, &w);
-      if (!list_useful(sv, l, d, k)) continue;
-      const int s = pd.table[bucket_of(d, w, k)];
-      const int kind = pd.kind[s];
-      c.v[s] += list_splits(kind, k, w) * list_chunks(kind, d);
-    }
+  const uint32_t d = sv.row_ptr[x + 1] - sv.row_ptr[x];
+  for (int l = 0; l < sv.nlists; ++l) {
+    uint32_t b, k;
+    uint64_t w;
+    owner_list(sv, x, l, &b, &k, &w);
+    if (!list_useful(sv, l, d, k)) continue;
+    const int s = pd.table[bucket_of(d, w, k)];
+    const int kind = pd.kind[s];
+    c.v[s] += list_splits(kind, k, w) * list_chunks(kind, d);
   }
-  cnt[x] = c;
+  return c;
 }
 
-// Writes each owner's tasks at its scanned offsets inside its slots' regions.
-__global__ void k_sched_fill(SchedView sv, PlanDev pd, uint64_t n, const Cnt8* __restrict__ off, Cnt8 base,
-                             uint4* __restrict__ tasks) {
-  const uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// Task lists in owner order per slot, in two passes over blocks of
+// kSchedBlock owners: per block the per-slot task counts (scanned over the
+// blocks on the device), then each block scans its owners' counts in LDS and
+// writes their tasks at its block's offsets. Only per-block sums go through
+// HBM.
+constexpr unsigned kSchedBlock = 256;
+
+__global__ __launch_bounds__(kSchedBlock) void k_sched_count(SchedView sv, PlanDev pd, uint64_t n,
+                                                             Cnt8* __restrict__ blocksum) {
+  using Reduce = rocprim::block_reduce<Cnt8, kSchedBlock>;
+  __shared__ typename Reduce::storage_type tmp;
+  const uint64_t x = (uint64_t)blockIdx.x * kSchedBlock + threadIdx.x;
+  Cnt8 c;
+  if (x < n) {
+    c = owner_counts(sv, pd, x);
+  } else {
+    for (int i = 0; i < kSlots; ++i) c.v[i] = 0;
+  }
+  Cnt8 t;
+  Reduce().reduce(c, t, tmp, Cnt8Plus());
+  if (threadIdx.x == 0) blocksum[blockIdx.x] = t;
+}
+
+// Writes each owner's tasks at its block's offset plus its prefix in the block.
+__global__ __launch_bounds__(kSchedBlock) void k_sched_fill(SchedView sv, PlanDev pd, uint64_t n,
+                                                            const Cnt8* __restrict__ blockoff, Cnt8 base,
+                                                            uint4* __restrict__ tasks) {
+  using Scan = rocprim::block_scan<Cnt8, kSchedBlock>;
+  __shared__ typename Scan::storage_type tmp;
+  const uint64_t x = (uint64_t)blockIdx.x * kSchedBlock + threadIdx.x;
+  Cnt8 c;
+  if (x < n) {
+    c = owner_counts(sv, pd, x);
+  } else {
+    for (int i = 0; i < kSlots; ++i) c.v[i] = 0;
+  }
+  Cnt8 o;
+  Scan().exclusive_scan(c, o, blockoff[blockIdx.x], tmp, Cnt8Plus());
   if (x >= n) return;
   const uint32_t d = sv.row_ptr[x + 1] - sv.row_ptr[x];
-  const Cnt8 o = off[x];
   uint32_t j[kSlots];
   for (int i = 0; i < kSlots; ++i) j[i] = 0;
   for (int l = 0; l < sv.nlists; ++l) {
@@ -1505,20 +1537,22 @@ void make_schedule(tribit_ctx* ctx, tribit_graph* g, int layout, int flags, bool
   for (int i = 0; i < kSlots; ++i) pd.kind[i] = i < plan.nslots ? plan.kind[i] : kKindBlock;
   pd.table = dtable.p;
 
-  DevBuf<Cnt8> cnt(ctx, n + 1), off(ctx, n + 1);
+  // per-block task counts (block nb covers no owner: its sum is zero), scanned
+  const uint64_t nb = (n + kSchedBlock - 1) / kSchedBlock;
+  DevBuf<Cnt8> cnt(ctx, nb + 1), off(ctx, nb + 1);
   const SchedView sv = sched_view(g, layout, core_d0, core_cut);
   S.rng = sched_ranges(g, layout);
-  k_sched_count<<<grid_for(n + 1, kBlock), kBlock, 0, s>>>(sv, pd, n, cnt.p);
+  k_sched_count<<<(unsigned)(nb + 1), kSchedBlock, 0, s>>>(sv, pd, n, cnt.p);
   {
     Cnt8 zero;
     for (int i = 0; i < kSlots; ++i) zero.v[i] = 0;
     size_t bytes = 0;
-    TB_HIP(rocprim::exclusive_scan(nullptr, bytes, cnt.p, off.p, zero, (size_t)(n + 1), Cnt8Plus(), s));
+    TB_HIP(rocprim::exclusive_scan(nullptr, bytes, cnt.p, off.p, zero, (size_t)(nb + 1), Cnt8Plus(), s));
     DevBuf<uint8_t> tmp(ctx, bytes ? bytes : 8);
-    TB_HIP(rocprim::exclusive_scan(tmp.p, bytes, cnt.p, off.p, zero, (size_t)(n + 1), Cnt8Plus(), s));
+    TB_HIP(rocprim::exclusive_scan(tmp.p, bytes, cnt.p, off.p, zero, (size_t)(nb + 1), Cnt8Plus(), s));
   }
-  // off[n] = totals (cnt[n] is zero)
-  TB_HIP(hipMemcpyAsync(&S.tot, off.p + n, sizeof(Cnt8), hipMemcpyDeviceToHost, s));
+  // off[nb] = totals
+  TB_HIP(hipMemcpyAsync(&S.tot, off.p + nb, sizeof(Cnt8), hipMemcpyDeviceToHost, s));
   ctx->sync();
   S.ntask = 0;
   for (int i = 0; i < kSlots; ++i) {
@@ -1527,7 +1561,7 @@ void make_schedule(tribit_ctx* ctx, tribit_graph* g, int layout, int flags, bool
   }
   if (S.ntask > 0xFFFFFFFFull) fail(TRIBIT_ERR_TOO_LARGE, "more than 2^32-1 tasks");
   S.tasks = DevBuf<uint4>(ctx, S.ntask ? S.ntask : 1);
-  k_sched_fill<<<grid_for(n, kBlock), kBlock, 0, s>>>(sv, pd, n, off.p, S.base, S.tasks.p);
+  if (nb) k_sched_fill<<<(unsigned)nb, kSchedBlock, 0, s>>>(sv, pd, n, off.p, S.base, S.tasks.p);
   TB_HIP(hipGetLastError());
 }
 
