@@ -232,6 +232,7 @@ typedef struct {
   double tick_ns;                     /* nanoseconds per tick */
   double ms_kernel;                   /* device time of the counting launches */
   uint64_t triangles;
+  double ms_slot[TRIBIT_MENU_SLOTS];  /* device time of each slot's launch */
 } tribit_profile;
 /* Counts once with the menu plan `table` (slot per bucket; NULL = heuristic)
  * over the range layout `sched` (PLAIN, DIRECTED or WEDGE) and logs every

@@ -98,7 +98,7 @@ class Profile(ctypes.Structure):
         ("eligible", ctypes.c_uint64 * BUCKETS), ("sum_t", ctypes.c_uint64 * BUCKETS),
         ("ticks", (ctypes.c_uint64 * BUCKETS) * MENU_SLOTS),
         ("slot", ctypes.c_uint8 * BUCKETS), ("tick_ns", ctypes.c_double), ("ms_kernel", ctypes.c_double),
-        ("triangles", ctypes.c_uint64),
+        ("triangles", ctypes.c_uint64), ("ms_slot", ctypes.c_double * MENU_SLOTS),
     ]
 
 

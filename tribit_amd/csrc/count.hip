@@ -438,7 +438,7 @@ __global__ __launch_bounds__(256) void k_count_wave(const uint32_t* __restrict__
                                                     const uint4* __restrict__ tasks, uint32_t ntasks,
                                                     uint32_t* __restrict__ queue,
                                                     unsigned long long* __restrict__ result, int attempts,
-                                                    unsigned long long* __restrict__ tlog,
+                                                    uint32_t* __restrict__ tdur,
                                                     const uint32_t* __restrict__ gid,
                                                     unsigned long long* __restrict__ gcount) {
   constexpr uint32_t S = 1u << LOGS;
@@ -459,7 +459,7 @@ __global__ __launch_bounds__(256) void k_count_wave(const uint32_t* __restrict__
     for (uint32_t t = tb; t < tend; ++t) {
       const uint4 tk = tasks[ntasks - 1 - t];
       const uint32_t x = tk.x, rbase = tk.y, rcount = tk.z;
-      const uint64_t t0 = tlog ? (uint64_t)wall_clock64() : 0;
+      const uint64_t t0 = tdur ? (uint64_t)wall_clock64() : 0;
       const uint32_t r0 = row_ptr[x], d = row_ptr[x + 1] - r0;
       uint32_t sa = 0;
       bool ok = false;
@@ -492,7 +492,7 @@ __global__ __launch_bounds__(256) void k_count_wave(const uint32_t* __restrict__
         const uint64_t tc = wave_sum_u64(cnt);
         if (lane == 0 && tc) atomicAdd(&gcount[gid[x]], (unsigned long long)tc);
       }
-      if (tlog && lane == 0) atomicAdd(&tlog[tk.w >> kBucketShift], (unsigned long long)(wall_clock64() - t0));
+      if (tdur && lane == 0) tdur[ntasks - 1 - t] = (uint32_t)(wall_clock64() - t0);
     }
   }
   acc = wave_sum_u64(acc);
@@ -513,7 +513,7 @@ __global__ __launch_bounds__(256) void k_count_wave_multi(const uint32_t* __rest
                                                           const uint4* __restrict__ tasks, uint32_t ntasks,
                                                           uint32_t* __restrict__ queue,
                                                           unsigned long long* __restrict__ result, int attempts,
-                                                          unsigned long long* __restrict__ tlog) {
+                                                          uint32_t* __restrict__ tdur) {
   constexpr uint32_t S = 1u << LOGS;
   static_assert(MT <= 16 && (MT & (MT - 1)) == 0, "MT: power of two up to 16");
   const ColSrc src = make_col_src(col, m);
@@ -539,7 +539,7 @@ __global__ __launch_bounds__(256) void k_count_wave_multi(const uint32_t* __rest
     const uint32_t tend = min(tb + kWaveBatch, ntasks);
     for (uint32_t t0 = tb; t0 < tend; t0 += MT) {
       const uint32_t nt = min((uint32_t)MT, tend - t0);
-      const uint64_t c0t = tlog ? (uint64_t)wall_clock64() : 0;
+      const uint64_t c0t = tdur ? (uint64_t)wall_clock64() : 0;
       // lane j < nt holds task j: owner row, outdeg, ranges
       uint4 tk = make_uint4(0u, 0u, 0u, 0u);
       uint32_t r0 = 0, d = 0;
@@ -612,8 +612,7 @@ __global__ __launch_bounds__(256) void k_count_wave_multi(const uint32_t* __rest
         wave_lds_sync();
       }
       acc += cnt;
-      if (tlog && lane < nt)
-        atomicAdd(&tlog[tk.w >> kBucketShift], (unsigned long long)((wall_clock64() - c0t) / nt));
+      if (tdur && lane < nt) tdur[ntasks - 1 - (t0 + lane)] = (uint32_t)((wall_clock64() - c0t) / nt);
       wave_lds_sync();  // the next batch rewrites the task table and sets
     }
   }
@@ -640,7 +639,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_count_block(const uint32_t* _
                                                                const uint4* __restrict__ tasks, uint32_t ntasks,
                                                                uint32_t* __restrict__ queue,
                                                                unsigned long long* __restrict__ result,
-                                                               int mode, unsigned long long* __restrict__ tlog,
+                                                               int mode, uint32_t* __restrict__ tdur,
                                                                const uint32_t* __restrict__ gid,
                                                                unsigned long long* __restrict__ gcount) {
   constexpr uint32_t S = 1u << LOGS;
@@ -663,7 +662,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_count_block(const uint32_t* _
     if (t >= ntasks) break;
     const uint4 tk = tasks[ntasks - 1 - t];
     const uint32_t x = tk.x, rbase = tk.y, rcount = tk.z, c = tk.w & kChunkMask;
-    const uint64_t t0 = tlog ? (uint64_t)wall_clock64() : 0;
+    const uint64_t t0 = tdur ? (uint64_t)wall_clock64() : 0;
     const uint32_t r0 = row_ptr[x], d = row_ptr[x + 1] - r0;
     const uint32_t p0 = c * kChunk, p1 = min(p0 + kChunk, d);
     const bool ranged = (p0 > 0) || (p1 < d);
@@ -764,7 +763,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_count_block(const uint32_t* _
         if (t) atomicAdd(&gcount[gid[x]], t);
       }
     }
-    if (tlog && threadIdx.x == 0) atomicAdd(&tlog[tk.w >> kBucketShift], (unsigned long long)(wall_clock64() - t0));
+    if (tdur && threadIdx.x == 0) tdur[ntasks - 1 - t] = (uint32_t)(wall_clock64() - t0);
   }
   acc = wave_sum_u64(acc);
   if (lane == 0 && acc) atomicAdd(result, (unsigned long long)acc);
@@ -1081,6 +1080,21 @@ __global__ __launch_bounds__(256) void k_narrow_stats(SchedView sv, uint64_t n, 
   if (threadIdx.x < 2 * kSlots && acc[threadIdx.x]) atomicAdd(&out[threadIdx.x], acc[threadIdx.x]);
 }
 
+// Timing logs: per (bucket) sum of the durations of a slot's tasks, from the
+// per-task durations the counting kernels wrote (a block histogram in LDS, so
+// the counting launches themselves do no atomics on shared counters).
+__global__ __launch_bounds__(256) void k_tdur_reduce(const uint4* __restrict__ tasks, const uint32_t* __restrict__ tdur,
+                                                     uint32_t ntasks, unsigned long long* __restrict__ tlog) {
+  __shared__ unsigned long long h[kBuckets];
+  for (int i = threadIdx.x; i < kBuckets; i += blockDim.x) h[i] = 0ull;
+  __syncthreads();
+  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < ntasks; t += gridDim.x * blockDim.x)
+    atomicAdd(&h[tasks[t].w >> kBucketShift], (unsigned long long)tdur[t]);
+  __syncthreads();
+  for (int i = threadIdx.x; i < kBuckets; i += blockDim.x)
+    if (h[i]) atomicAdd(&tlog[i], h[i]);
+}
+
 __global__ void k_zero_u64(unsigned long long* p, uint32_t n) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0ull;
@@ -1280,9 +1294,15 @@ void menu_slot(int slot, int* kind, int* g, uint32_t* max_d) {
 
 uint32_t bucket_max_outdeg(int bucket) { return d_bin_top(bucket / kLBins); }
 
-// LEARNED: per bucket with lists, the feasible menu slot of lowest predicted
-// cost 2^model(features) * work. Without a model, or when a prediction is not
-// finite, the heuristic's slot.
+// LEARNED: one menu slot per outdeg class of the heuristic (<= 32, <= 256,
+// larger), the slot feasible for every bucket of the class with the lowest
+// predicted class cost: the sum over the class's buckets of
+// 2^model(features) * work. One launch per class keeps the per-bucket costs,
+// which were measured as shares of whole launches, additive: a plan that
+// scatters a class's buckets over several launches pays launch tails and
+// contention that no per-bucket figure holds. Without a model, or when a
+// class has no slot with finite predictions for all its buckets, the
+// heuristic's slot.
 static void learned_table(tribit_ctx* ctx, const BucketStats& st, int layout, int flags, uint8_t* table) {
   heuristic_table(table, layout);
   if (!ctx->model || layout != TRIBIT_SCHED_WEDGE) return;  // the model is trained on wedge-schedule timings
@@ -1301,25 +1321,29 @@ static void learned_table(tribit_ctx* ctx, const BucketStats& st, int layout, in
   if (ids.empty()) return;
   std::vector<float> y(ids.size());
   mlp_infer(ctx->model, X.data(), ids.size(), y.data());
-  std::vector<double> best(kBuckets, INFINITY);
-  for (size_t i = 0; i < ids.size(); ++i) {
-    const int b = ids[i].first, s = ids[i].second;
-    if (!std::isfinite(y[i])) {
-      best[b] = -INFINITY;  // keep the heuristic for this bucket
-      continue;
-    }
-    const double c = std::exp2((double)y[i]) * bucket_work(st, b);
-    if (best[b] != -INFINITY && c < best[b]) {
-      best[b] = c;
-      table[b] = (uint8_t)s;
-    }
-  }
+  constexpr int kClasses = 3;
+  auto class_of = [](int b) {
+    const uint32_t top = d_bin_top(b / kLBins);
+    return top <= 32 ? 0 : top <= 256 ? 1 : 2;
+  };
+  double cost[kClasses][kSlots] = {};
+  int covered[kClasses][kSlots] = {}, need[kClasses] = {};
   for (int b = 0; b < kBuckets; ++b)
-    if (best[b] == -INFINITY) {
-      uint8_t h[kBuckets];
-      heuristic_table(h, layout);
-      table[b] = h[b];
-    }
+    if (st.lists[b]) ++need[class_of(b)];
+  for (size_t i = 0; i < ids.size(); ++i) {
+    const int b = ids[i].first, s = ids[i].second, c = class_of(b);
+    if (!std::isfinite(y[i])) continue;
+    cost[c][s] += std::exp2((double)y[i]) * bucket_work(st, b);
+    ++covered[c][s];
+  }
+  for (int c = 0; c < kClasses; ++c) {
+    int best = -1;
+    for (int s = 0; s < kSlots; ++s)
+      if (need[c] && covered[c][s] == need[c] && (best < 0 || cost[c][s] < cost[c][best])) best = s;
+    if (best < 0) continue;
+    for (int b = 0; b < kBuckets; ++b)
+      if (class_of(b) == c) table[b] = (uint8_t)best;
+  }
 }
 
 // ----------------------------------------------------------------- launches
@@ -1329,7 +1353,8 @@ struct LaunchArgs {
   int att = 0;                           // mode word (attempts | flags)
   unsigned long long* result = nullptr;  // [0] triangles, [1] fallback tasks
   uint32_t* queue = nullptr;             // kSlots task counters
-  unsigned long long* tlog = nullptr;    // kSlots x kBuckets task durations, or null
+  uint32_t* tdur = nullptr;              // per-task durations, indexed like the schedule's tasks, or null
+  const uint4* task0 = nullptr;          // the schedule's first task (tdur's origin)
   const uint32_t* gid = nullptr;         // graph of each vertex (batches), or null
   unsigned long long* gcount = nullptr;  // per-graph counts (batches), or null
   int waves[kSlots] = {0};               // workgroups per CU cap per slot (0: occupancy limit)
@@ -1346,7 +1371,7 @@ static void launch_wave_t(const LaunchArgs& a, int slot, uint32_t ntasks, int cu
   const unsigned grid = (unsigned)std::min<uint64_t>(want, (uint64_t)cus * per_cu);
   k_count_wave<LOGS, KMAX, G, U, BUF><<<grid, kBlock, 0, s>>>(g->row_ptr, g->col, g->m, rng, tasks, ntasks,
                                                              a.queue + slot, a.result, a.att & 0xFF,
-                                                             a.tlog ? a.tlog + (size_t)slot * kBuckets : nullptr,
+                                                             a.tdur ? a.tdur + (tasks - a.task0) : nullptr,
                                                              a.gid, a.gcount);
   *blocks += grid;
 }
@@ -1387,7 +1412,7 @@ static void launch_multi_t(const LaunchArgs& a, int slot, uint32_t ntasks, int c
   const uint64_t want = (ntasks + 4 * kWaveBatch - 1) / (4 * kWaveBatch);
   const unsigned grid = (unsigned)std::min<uint64_t>(want, (uint64_t)cus * per_cu);
   k<<<grid, kBlock, 0, s>>>(g->row_ptr, g->col, g->m, rng, tasks, ntasks, a.queue + slot, a.result, a.att & 0xFF,
-                            a.tlog ? a.tlog + (size_t)slot * kBuckets : nullptr);
+                            a.tdur ? a.tdur + (tasks - a.task0) : nullptr);
   *blocks += grid;
 }
 
@@ -1427,7 +1452,7 @@ static void launch_block_t(const LaunchArgs& a, int slot, uint32_t ntasks, int c
   k_count_block<LOGS, G, U, BUF, TILES, N16><<<grid, kBlockThreads, 0, s>>>(
       g->row_ptr, g->col, g->m, g->col16, g->sfx_dh, rng, ts, tasks, ntasks, a.queue + slot, a.result,
       a.att & ~kForceGlobal,
-      a.tlog ? a.tlog + (size_t)slot * kBuckets : nullptr, a.gid, a.gcount);
+      a.tdur ? a.tdur + (tasks - a.task0) : nullptr, a.gid, a.gcount);
   *blocks += grid;
 }
 
@@ -1597,7 +1622,8 @@ void launch_schedule(tribit_ctx* ctx, tribit_graph* g, const Schedule& S, const 
 
 // One scheduled count under `plan`. Returns the triangle count; fills
 // per-phase times. With `tlog` (kSlots x kBuckets, zeroed by the caller) every
-// task adds its wall-clock duration to tlog[slot][bucket].
+// task's wall-clock duration is added to tlog[slot][bucket] (recorded per task
+// during the launches, summed after them).
 static uint64_t count_once(tribit_ctx* ctx, tribit_graph* g, int sched, int flags, Plan& plan, bool learned,
                            double* ms_total, double* ms_sched, double* ms_kernel, double* ms_var,
                            tribit_count_info* info, unsigned long long* tlog, uint64_t core_c = 0,
@@ -1615,11 +1641,21 @@ static uint64_t count_once(tribit_ctx* ctx, tribit_graph* g, int sched, int flag
   a.att = mode_word(flags);
   a.result = scratch.p;
   a.queue = reinterpret_cast<uint32_t*>(scratch.p + 4);
-  a.tlog = tlog;
+  DevBuf<uint32_t> tdur;
+  if (tlog) {
+    tdur = DevBuf<uint32_t>(ctx, S.ntask ? S.ntask : 1);
+    a.tdur = tdur.p;
+    a.task0 = S.tasks.p;
+  }
   TB_HIP(hipEventRecord(ctx->ev[3], s));
   uint64_t blocks = 0;
   launch_schedule(ctx, g, S, a, ctx->ev + 4, &blocks);
   hipEvent_t last = ctx->ev[3 + plan.nslots + (S.core_c ? 1 : 0)];
+  if (tlog)
+    for (int v = 0; v < plan.nslots; ++v)
+      if (S.tot.v[v])
+        k_tdur_reduce<<<(unsigned)std::min<uint64_t>((S.tot.v[v] + 255) / 256, 1024), 256, 0, s>>>(
+            S.tasks.p + S.base.v[v], tdur.p + S.base.v[v], S.tot.v[v], tlog + (size_t)v * kBuckets);
   unsigned long long res[2] = {0, 0};
   TB_HIP(hipMemcpyAsync(res, a.result, sizeof(res), hipMemcpyDeviceToHost, s));
   TB_HIP(hipEventSynchronize(last));
@@ -1817,6 +1853,7 @@ void count_profile(tribit_ctx* ctx, tribit_graph* g, int layout, const uint8_t* 
   double a, b, c, v[kSlots] = {0};
   out->triangles = count_once(ctx, g, layout, 0, plan, false, &a, &b, &c, v, nullptr, tl.p);
   out->ms_kernel = c;
+  for (int s2 = 0; s2 < kSlots; ++s2) out->ms_slot[s2] = v[s2];
   std::vector<unsigned long long> h((size_t)kSlots * kBuckets);
   TB_HIP(hipMemcpyAsync(h.data(), tl.p, 8 * h.size(), hipMemcpyDeviceToHost, ctx->stream));
   ctx->sync();

@@ -5,13 +5,17 @@
 1. Builds a set of training graphs on the device (R-MAT at several scales and
    seeds, Chung-Lu, 2-D random geometric, stochastic block model, Erdos-Renyi;
    the R-MAT scale-25 member uses seed 1, not the benchmark's seed 5).
-2. Counts each graph under exploration plans: in round r every bucket runs on
-   its r-th feasible menu slot (round-robin), so each (bucket, slot) pair is
-   timed; the kernels log every task's wall-clock duration per (slot, bucket).
+2. Counts each graph under exploration plans: in round r every owner-outdeg
+   class (<= 32, <= 256, larger) runs on one of its feasible menu slots, the
+   rounds cycling through them, so each (bucket, slot) pair is timed in plans
+   of the shape the scheduler chooses from; the kernels log every task's
+   wall-clock duration, summed per (slot, bucket).
 3. One sample per timed (bucket, slot): the library's feature vector and the
-   target log2(thread-nanoseconds per unit of work), thread-nanoseconds =
-   task duration x threads of the task (64 for wave tasks, 512 for block
-   tasks), work = streamed elements + streamed ranges.
+   target log2(nanoseconds of the slot's launch per unit of work), where the
+   bucket's share of its launch's device time is the share of the launch's
+   summed task durations its tasks took, and work = streamed elements +
+   streamed ranges. Launch shares carry what per-task durations alone miss:
+   the kernel's occupancy and the contention of the tasks running together.
 4. Trains the 3-layer perceptron with the library's HIP kernels (FP32, Adam)
    on all graphs but one held out, reports the held-out fit, then retrains on
    all graphs and writes the parameters (float32, little endian) and a JSON
@@ -33,7 +37,6 @@ sys.path.insert(0, os.path.dirname(HERE))
 import tribit_amd as tb  # noqa: E402
 
 DEFAULT_OUT = os.path.join(HERE, "cost_model.f32")
-THREADS = {0: 64, 1: 64, 2: 512, 3: 512, 4: 64}  # menu kind -> threads of one task (multi-task waves log a share)
 
 TRAIN_GRAPHS = [
     ("rmat", dict(scale=18, ef=16, seed=11)),
@@ -71,25 +74,45 @@ def feasible(bucket: int, menu) -> list:
     return [s for s, (_, _, dmax) in enumerate(menu) if top <= dmax]
 
 
+def outdeg_class(bucket: int) -> int:
+    """The heuristic's class of a bucket: owner outdeg <= 32, <= 256, larger."""
+    top = tb.bucket_max_outdeg(bucket)
+    return 0 if top <= 32 else 1 if top <= 256 else 2
+
+
 def exploration_tables(menu, rounds: int):
-    feas = [feasible(b, menu) for b in range(tb.BUCKETS)]
-    return [np.array([f[(b + r) % len(f)] for b, f in enumerate(feas)], np.uint8) for r in range(rounds)]
+    """Round r runs every outdeg class on one slot, its r-th feasible one (the
+    classes' slot lists rotated against each other), the shape of the plans
+    the scheduler picks from: one launch per class, classes sharing a launch
+    when they pick the same slot."""
+    cls = [outdeg_class(b) for b in range(tb.BUCKETS)]
+    feas = []
+    for c in range(3):
+        bs = [b for b in range(tb.BUCKETS) if cls[b] == c]
+        feas.append([s for s in range(len(menu)) if all(tb.bucket_max_outdeg(b) <= menu[s][2] for b in bs)])
+    tables = []
+    for r in range(rounds):
+        pick = [feas[c][(r + c) % len(feas[c])] for c in range(3)]
+        tables.append(np.array([pick[cls[b]] for b in range(tb.BUCKETS)], np.uint8))
+    tables.append(np.array([heuristic_slot(b) for b in range(tb.BUCKETS)], np.uint8))  # and the heuristic plan
+    return tables
 
 
 def samples_of_profile(p: tb.Profile, menu, min_lists: int = 1):
     """(X, y, work, bucket, slot) for every (bucket, slot) timed in profile p."""
     X, y, w, bk, sl = [], [], [], [], []
+    tsum = [sum(float(p.ticks[s][b]) for b in range(tb.BUCKETS)) for s in range(tb.MENU_SLOTS)]
     for b in range(tb.BUCKETS):
         if p.lists[b] < min_lists:
             continue
         s = p.slot[b]
         ticks = p.ticks[s][b]
-        if ticks == 0:
+        if ticks == 0 or tsum[s] <= 0 or p.ms_slot[s] <= 0:
             continue
         work = float(p.sum_w[b]) + float(p.sum_k[b]) + 1.0
-        tns = ticks * p.tick_ns * THREADS[menu[s][0]]
+        ns = p.ms_slot[s] * 1e6 * ticks / tsum[s]  # the bucket's share of its launch
         X.append(tb.profile_features(p, b, s))
-        y.append(np.log2(tns / work))
+        y.append(np.log2(ns / work))
         w.append(work)
         bk.append(b)
         sl.append(s)
@@ -137,9 +160,9 @@ def train(ctx, X, y, epochs, seed):
 
 
 def heuristic_slot(bucket: int) -> int:
-    """The density-threshold heuristic's menu slot (outdeg <= 32 / <= 256 / larger)."""
+    """The density-threshold heuristic's menu slot on the wedge layout (outdeg <= 32 / <= 256 / larger)."""
     top = tb.bucket_max_outdeg(bucket)
-    return 0 if top <= 32 else 3 if top <= 256 else 6
+    return 0 if top <= 32 else 3 if top <= 256 else 5
 
 
 def choice_quality(pred, d):
