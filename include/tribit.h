@@ -118,7 +118,7 @@ typedef struct {
   double ms_judge;         /* LEARNED: device time of the candidate trials this call ran (first call) */
   double ms_candidates[4]; /* LEARNED: judged time of each candidate (0 = not a candidate for this graph) */
   int32_t slot_kind[8];    /* kernel of each launch: 0 wave/64-slot set, 1 wave/512-slot set, 2 block, 3 block
-                              streaming bit tiles */
+                              streaming bit tiles, 4 multi-task wave (eight 64-slot sets) */
 } tribit_count_info;
 
 /* Work assigned to each kernel variant by a scheduler, in the units of the

@@ -219,7 +219,7 @@ def main() -> None:
             "held_out": data[hold][0], "held_out_r2": float(r2), "held_out_choice_ratio": float(ratio),
             "held_out_heuristic_ratio": float(hratio),
             "final_loss": float(loss), "epochs": a.epochs, "seed": a.seed, "rounds": a.rounds,
-            "repeats": a.repeats, "target": "log2(task thread-ns per (streamed element + range))"}
+            "repeats": a.repeats, "target": "log2(ns of launch time per (streamed element + range)), the bucket's share of its launch"}
     with open(os.path.splitext(a.out)[0] + ".json", "w") as f:
         json.dump(meta, f, indent=1)
     log(f"wrote {a.out} ({params.size} floats), final loss {loss:.4f}")
