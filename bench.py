@@ -99,7 +99,22 @@ def cpu_baseline(g, n: int, m: int, target_s: float) -> dict:
         "sample": f"rows u = 0 mod {stride} ({int(np.count_nonzero(d[::stride] >= 2))} active rows, "
                   f"{sample_work:.3g} of {total_work:.3g} merge steps) counted in {elapsed:.2f} s, "
                   f"extrapolated by merge work to {t_full:.1f} s for the whole graph",
+        "cpu_model": cpu_model(),
     }
+
+
+def cpu_model() -> str:
+    """The host CPU's model name (first processor in /proc/cpuinfo) and its logical CPUs."""
+    name = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    name = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return f"{name} ({os.cpu_count()} logical CPUs visible)"
 
 
 def measured_traffic(args, variant: int):

@@ -361,6 +361,25 @@ def test_cli_wiki_vote_stdout_contract():
     assert "Reading data from " + WIKI in out.stdout
 
 
+def test_harness_csv_wiki_vote(tmp_path):
+    """The CSV harness over a folder holding wiki-Vote.mtx, with the real binary: the reference
+    harness's columns, the exact count and the pool high-water mark in bytes."""
+    from tribit_amd import harness
+    note("harness csv wiki-Vote")
+    data = tmp_path / "graphs"
+    data.mkdir()
+    os.symlink(WIKI, data / "wiki-Vote.mtx")
+    out = tmp_path / "results.csv"
+    assert harness.main(["--exe_path", tribit_amd.CLI_PATH, "--data_path", str(data), "--out_path", str(out),
+                         "--get_memory_consumption"]) == 0
+    lines = open(out).read().splitlines()
+    assert lines[0] == harness.HEADER and len(lines) == 2
+    row = lines[1].split(",")
+    assert row[:5] == ["tribit", "wiki-Vote.mtx", "8297", "8297", "103689"]
+    assert row[8] == "608389" and int(row[5]) > 0 and float(row[7]) > 0
+    assert int(row[9]) >= 8 * 103689  # the pool held at least the edge keys
+
+
 def test_cli_generated_config1():
     note("cli rmat s13 ef12 seed 1")
     out = subprocess.run([tribit_amd.CLI_PATH, "--rmat", "13,12,0.57,0.19,0.19,1"], capture_output=True, text=True,
