@@ -160,6 +160,7 @@ struct tribit_graph {
   uint64_t* w_sfx = nullptr;    // n, streamed elements of v's ranges
   uint16_t* col16 = nullptr;    // low 16 bits of every id, for narrow wedge ranges (relabelled, m < 2^30)
   uint8_t* sfx_dh = nullptr;    // per wedge range: its ids' high half minus its owner's (255: read col)
+  uint16_t* sfx_split = nullptr; // per wedge range crossing one 64K-id boundary: offset of its first id past it
   // 64-column bit tiles (tiles.hip), built on first use
   uint64_t tiles64 = 0;          // tile count (computed by preprocessing)
   uint64_t ntiles = 0;           // tiles built (0 until built)
@@ -237,6 +238,12 @@ void gen_sbm(tribit_ctx* ctx, uint64_t blocks, uint64_t size, double p_in, doubl
 // count.hip
 void count(tribit_ctx* ctx, tribit_graph* g, const tribit_count_opts& opts, tribit_count_info* info);
 void variant_stats(tribit_ctx* ctx, tribit_graph* g, int sched, tribit_variant_work* out);
+
+// Mask of a range end: wedge range ends carry flags in bits 31 (narrow) and
+// 30 (two 64K-id windows) when m < 2^30; larger graphs carry none.
+__host__ __device__ inline uint32_t range_end_mask(uint64_t m) {
+  return m < (1ull << 30) ? 0x3FFFFFFFu : 0xFFFFFFFFu;
+}
 void count_profile(tribit_ctx* ctx, tribit_graph* g, int layout, const uint8_t* table, tribit_profile* out);
 void profile_features(const tribit_profile* p, int bucket, int slot, float* f);
 void learned_plan(tribit_ctx* ctx, tribit_graph* g, uint8_t* table);

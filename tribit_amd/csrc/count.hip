@@ -54,11 +54,13 @@ constexpr uint32_t kChunkMask = (1u << kBucketShift) - 1;
 // same 2S slots.
 
 constexpr uint32_t kNoKey = 0xFFFFFFFEu;  // never a vertex id (n <= 2^32-2) and never kEmpty
-// Range entries (begin, end | flag): bit 31 of the end marks a range whose
-// ids share their high 16 bits (wedge ranges of graphs with m < 2^30); the
-// block kernel may stream such a range from the 16-bit low halves (col16).
-constexpr uint32_t kEndMask = 0x7FFFFFFFu;
+// Range entries (begin, end | flags): for wedge ranges of graphs with
+// m < 2^30, bit 31 of the end marks a narrow range, whose ids share their high
+// 16 bits or (bit 30 too) cross one 64K-id boundary, at a position kept in
+// sfx_split; block tasks stream narrow ranges from the 16-bit low halves
+// (col16). range_end_mask(m) strips the flags.
 constexpr uint32_t kNarrowFlag = 0x80000000u;
+constexpr uint32_t kTwoWinFlag = 0x40000000u;
 constexpr int kNoBitmapBit = 1 << 9;       // kernel mode bit: block tasks never use bitmap sets
 constexpr int kWideBit = 1 << 10;          // launch-time flag: narrow ranges streamed as 32-bit ids too
 constexpr int kCuckooIters = 64;
@@ -223,6 +225,7 @@ struct ColSrc {
   const uint32_t* col;
   __amdgpu_buffer_rsrc_t rsrc;    // raw buffer over col
   __amdgpu_buffer_rsrc_t rsrc16;  // raw buffer over col16 (narrow ranges), when given
+  uint32_t emask;                 // range_end_mask(m)
 };
 
 __device__ __forceinline__ ColSrc make_col_src(const uint32_t* col, uint64_t m, const uint16_t* col16 = nullptr) {
@@ -234,6 +237,7 @@ __device__ __forceinline__ ColSrc make_col_src(const uint32_t* col, uint64_t m, 
   c.rsrc16 = __builtin_amdgcn_make_buffer_rsrc(const_cast<uint16_t*>(col16 ? col16 : reinterpret_cast<const uint16_t*>(col)),
                                                (short)0, (int)(uint32_t)(bytes16 < kBufMax ? bytes16 : kBufMax),
                                                0x00020000);
+  c.emask = range_end_mask(m);
   return c;
 }
 
@@ -264,7 +268,7 @@ __device__ __forceinline__ uint32_t stream_ranges(const ColSrc& src, const uint2
       if (k < nk) {
         const uint2 r = R[k];
         base = r.x;
-        end = r.y & kEndMask;
+        end = r.y & src.emask;
         if (Rsel) sel = Rsel[k];
       }
     }
@@ -361,6 +365,78 @@ __device__ __forceinline__ uint32_t stream_narrow(const ColSrc& src, const uint2
         if (j == 0) x0 = skip0 ? kNoKey : x0;
         x1 = 2 * j * G + 1 < rem ? x1 : kNoKey;
         if (!Probe::kZeroSafe) x0 = ok ? x0 : kNoKey;
+        cnt += probe(x0) + probe(x1);
+      }
+    }
+    qb += (uint32_t)(G * U);
+  }
+  return cnt;
+}
+
+// Narrow ranges crossing one 64K-id boundary: staged as (begin, split << 16 |
+// length), ids before begin + split have high half hi, the others hi + 1.
+// Same slots as stream_narrow; each half picks its high half by comparing its
+// position with the split (t = split position - the lane's first position).
+template <int G, int U, typename Probe>
+__device__ __forceinline__ uint32_t stream_narrow2(const ColSrc& src, const uint2* R, const uint16_t* Rh, uint32_t nk,
+                                                   uint32_t* s_next, uint32_t lane, const Probe& probe) {
+  constexpr uint64_t kLeads = leader_mask<G>();
+  const uint32_t g = lane / G, gl = lane % G;
+  const uint64_t below = (g == 0) ? 0ull : ((1ull << (g * G)) - 1);
+  uint32_t qb = 0, b = 0, e = 0, p = 0, hi = 0, cnt = 0;
+  for (;;) {
+    const bool need = 2 * qb >= e;
+    const uint64_t want = __ballot(need && gl == 0);
+    const uint32_t nw = (uint32_t)__popcll(want);
+    uint32_t kb = 0;
+    if (nw) {
+      if (lane == 0) kb = atomicAdd(s_next, nw);
+      kb = __shfl(kb, 0);
+      if (want == kLeads && kb >= nk) break;
+    }
+    if (need) {
+      const uint32_t k = kb + (uint32_t)__popcll(want & below);
+      if (k < nk) {
+        const uint2 r = R[k];
+        b = r.x;
+        e = b + (r.y & 0xFFFFu);
+        p = b + (r.y >> 16);
+        qb = b >> 1;
+        hi = (uint32_t)Rh[k] << 16;
+      }
+    }
+    const uint32_t q0 = qb + gl;
+    const int rem = (int)(e - 2 * q0);
+    const int t = (int)(p - 2 * q0);  // slot j's first half is past the split iff 2 j G >= t
+    const bool skip0 = 2 * q0 < b;
+    const uint32_t off = 4 * q0;
+    uint32_t w[U];
+#pragma unroll
+    for (int j = 0; j < U; ++j)
+      w[j] = __builtin_amdgcn_raw_buffer_load_b32(src.rsrc16, (int)(2 * j * G < rem ? off : kBufOob), j * G * 4, 0);
+    __builtin_amdgcn_sched_barrier(0);
+    if constexpr (Probe::kRelative) {
+      const uint32_t dh = hi - probe.base, dh2 = dh + 0x10000u;
+#pragma unroll
+      for (int j = 0; j < U; ++j) {
+        const bool ok = 2 * j * G < rem;
+        const uint32_t d0 = ok ? (2 * j * G < t ? dh : dh2) : 0xFFFF0000u;
+        const uint32_t d1 = 2 * j * G + 1 < t ? dh : dh2;
+        uint32_t i0 = (w[j] & 0xFFFFu) + d0;
+        uint32_t i1 = (w[j] >> 16) + d1;
+        if (j == 0) i0 = skip0 ? 0xFFFFFFFFu : i0;
+        i1 = 2 * j * G + 1 < rem ? i1 : 0xFFFFFFFFu;
+        cnt += probe.rel(i0) + probe.rel(i1);
+      }
+    } else {
+      const uint32_t hi2 = hi + 0x10000u;
+#pragma unroll
+      for (int j = 0; j < U; ++j) {
+        const bool ok = 2 * j * G < rem;
+        uint32_t x0 = (w[j] & 0xFFFFu) | (2 * j * G < t ? hi : hi2);
+        uint32_t x1 = (w[j] >> 16) | (2 * j * G + 1 < t ? hi : hi2);
+        x0 = ok && !(j == 0 && skip0) ? x0 : kNoKey;
+        x1 = 2 * j * G + 1 < rem ? x1 : kNoKey;
         cnt += probe(x0) + probe(x1);
       }
     }
@@ -635,6 +711,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_count_block(const uint32_t* _
                                                                const uint32_t* __restrict__ col, uint64_t m,
                                                                const uint16_t* __restrict__ col16,
                                                                const uint8_t* __restrict__ dh,
+                                                               const uint16_t* __restrict__ split,
                                                                const uint2* __restrict__ rng, TileSrc ts,
                                                                const uint4* __restrict__ tasks, uint32_t ntasks,
                                                                uint32_t* __restrict__ queue,
@@ -649,7 +726,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_count_block(const uint32_t* _
   __shared__ __attribute__((aligned(8))) uint32_t T[2 * S + 2];  // bitmap mode: 2S words, then a zero 64-bit word
   __shared__ uint2 R[kChunk];
   __shared__ uint16_t Rh[N16 ? kChunk : 1];  // high halves of narrow ranges
-  __shared__ uint32_t s_task, s_next, s_fail, s_next16, s_n16, s_nwide;
+  __shared__ uint32_t s_task, s_next, s_fail, s_next16, s_next2, s_cat[3];
   __shared__ unsigned long long s_part[kBlockThreads / 64];
   const uint32_t lane = threadIdx.x & 63;
   const ColSrc src = make_col_src(col, m, N16 ? col16 : nullptr);
@@ -705,50 +782,89 @@ __global__ __launch_bounds__(kBlockThreads) void k_count_block(const uint32_t* _
     const uint2* rs = tiles ? ts.trng : rng;
     for (uint32_t c0 = 0; c0 < rcount; c0 += kChunk) {
       const uint32_t nk = min((uint32_t)kChunk, rcount - c0);
-      if (N16) {  // narrow ranges are staged from the front, the others from the back
-        if (threadIdx.x == 0) s_n16 = s_nwide = 0;
+      uint32_t n1 = 0, n2 = 0;  // with N16: R[0, n1) narrow, R[n1, n1 + n2) narrow across two windows, then wide
+      if (N16 && !tiles) {
+        constexpr int kPer = kChunk / kBlockThreads;
+        if (threadIdx.x < 3) s_cat[threadIdx.x] = 0;
         __syncthreads();
-      }
-      for (uint32_t k = threadIdx.x; k < nk; k += kBlockThreads) {
-        uint2 r = rs[rbase + c0 + k];
-        uint32_t at = k;
-        if (!tiles) {
-          const bool narrow = (r.y & kNarrowFlag) != 0;
-          uint32_t e = r.y & kEndMask;
-          if (ranged) {  // tiles outside the chunk's window miss in the bitmap
-            r.x = lower_bound_u32(col, r.x, e, lo);
-            e = upper_bound_u32(col, r.x, e, hi);
-          }
-          r.y = e;
-          if (N16) {
-            if (narrow && r.x < e) {
-              at = atomicAdd(&s_n16, 1u);
-              const uint32_t d8 = dh[rbase + c0 + k];  // high half relative to the owner's: no read of col
-              Rh[at] = (uint16_t)(d8 < 255u ? (x >> 16) + d8 : col[r.x] >> 16);
-            } else {
-              at = nk - 1 - atomicAdd(&s_nwide, 1u);
+        uint2 rv[kPer];
+        uint32_t cv[kPer], iv[kPer], hv[kPer];
+#pragma unroll
+        for (int i = 0; i < kPer; ++i) {
+          const uint32_t k = threadIdx.x + i * kBlockThreads;
+          cv[i] = 3u;
+          if (k < nk) {
+            uint2 r = rng[rbase + c0 + k];
+            const uint32_t fl = r.y & ~src.emask, b0 = r.x;
+            uint32_t e = r.y & src.emask;
+            if (ranged) {
+              r.x = lower_bound_u32(col, r.x, e, lo);
+              e = upper_bound_u32(col, r.x, e, hi);
             }
+            uint32_t c = 0, h = 0;
+            if ((fl & kNarrowFlag) && r.x < e) {
+              const uint32_t d8 = dh[rbase + c0 + k];  // high half relative to the owner's: no read of col
+              h = d8 < 255u ? (x >> 16) + d8 : col[b0] >> 16;
+              c = 1;
+              if (fl & kTwoWinFlag) {
+                const uint32_t sp = b0 + split[rbase + c0 + k];  // first position with high half h + 1
+                if (r.x >= sp)
+                  ++h;  // the cut range lies past the boundary
+                else if (e > sp)
+                  c = 2;
+              }
+            }
+            rv[i] = c == 2 ? make_uint2(r.x, ((b0 + split[rbase + c0 + k] - r.x) << 16) | (e - r.x))
+                           : make_uint2(r.x, e);
+            cv[i] = c;
+            hv[i] = h;
+            iv[i] = atomicAdd(&s_cat[c], 1u);
           }
         }
-        R[at] = r;
+        __syncthreads();
+        n1 = s_cat[1];
+        n2 = s_cat[2];
+#pragma unroll
+        for (int i = 0; i < kPer; ++i) {
+          if (cv[i] > 2u) continue;
+          const uint32_t at = (cv[i] == 1u ? 0u : cv[i] == 2u ? n1 : n1 + n2) + iv[i];
+          R[at] = rv[i];
+          if (cv[i]) Rh[at] = (uint16_t)hv[i];
+        }
+      } else {
+        for (uint32_t k = threadIdx.x; k < nk; k += kBlockThreads) {
+          uint2 r = rs[rbase + c0 + k];
+          if (!tiles) {
+            uint32_t e = r.y & src.emask;
+            if (ranged) {  // tiles outside the chunk's window miss in the bitmap
+              r.x = lower_bound_u32(col, r.x, e, lo);
+              e = upper_bound_u32(col, r.x, e, hi);
+            }
+            r.y = e;
+          }
+          R[k] = r;
+        }
       }
-      if (threadIdx.x == 0) s_next = s_next16 = 0;
+      if (threadIdx.x == 0) s_next = s_next16 = s_next2 = 0;
       __syncthreads();
-      const uint32_t n16 = N16 ? s_n16 : 0u;  // R[0, n16) narrow, R[n16, nk) the others
+      const uint32_t nn = n1 + n2;
       if (tiles) {
         cnt += stream_tiles<G, U>(ts, R, nk, &s_next, lane, T, base, kSpan);
       } else if (bitmap) {
         const BitmapProbe<LOGS + 1> pr{T, base};
-        if (N16 && n16) cnt += stream_narrow<G, U>(src, R, Rh, n16, &s_next16, lane, pr);
-        cnt += stream_ranges<G, U, true, BUF>(src, R + n16, nk - n16, &s_next, lane, pr);
+        if (N16 && n1) cnt += stream_narrow<G, U>(src, R, Rh, n1, &s_next16, lane, pr);
+        if (N16 && n2) cnt += stream_narrow2<G, U>(src, R + n1, Rh + n1, n2, &s_next2, lane, pr);
+        cnt += stream_ranges<G, U, true, BUF>(src, R + nn, nk - nn, &s_next, lane, pr);
       } else if (ok) {
         const CuckooProbe<LOGS> pr{T, sa};
-        if (N16 && n16) cnt += stream_narrow<G, U>(src, R, Rh, n16, &s_next16, lane, pr);
-        cnt += stream_ranges<G, U, true, BUF>(src, R + n16, nk - n16, &s_next, lane, pr);
+        if (N16 && n1) cnt += stream_narrow<G, U>(src, R, Rh, n1, &s_next16, lane, pr);
+        if (N16 && n2) cnt += stream_narrow2<G, U>(src, R + n1, Rh + n1, n2, &s_next2, lane, pr);
+        cnt += stream_ranges<G, U, true, BUF>(src, R + nn, nk - nn, &s_next, lane, pr);
       } else {
         const LinearProbe<LOGS + 1> pr{T};
-        if (N16 && n16) cnt += stream_narrow<G, U>(src, R, Rh, n16, &s_next16, lane, pr);
-        cnt += stream_ranges<G, U, true, BUF>(src, R + n16, nk - n16, &s_next, lane, pr);
+        if (N16 && n1) cnt += stream_narrow<G, U>(src, R, Rh, n1, &s_next16, lane, pr);
+        if (N16 && n2) cnt += stream_narrow2<G, U>(src, R + n1, Rh + n1, n2, &s_next2, lane, pr);
+        cnt += stream_ranges<G, U, true, BUF>(src, R + nn, nk - nn, &s_next, lane, pr);
       }
       __syncthreads();
     }
@@ -1066,7 +1182,7 @@ __global__ __launch_bounds__(256) void k_narrow_stats(SchedView sv, uint64_t n, 
       for (uint32_t i = b; i < b + k; ++i) {
         const uint2 r = sv.rng[i];
         if (r.y & kNarrowFlag) {
-          ne += (r.y & kEndMask) - r.x;
+          ne += (r.y & 0x3FFFFFFFu) - r.x;  // flags exist only when m < 2^30
           ++nr;
         }
       }
@@ -1450,7 +1566,7 @@ static void launch_block_t(const LaunchArgs& a, int slot, uint32_t ntasks, int c
   if (per_cu < 1) per_cu = 1;
   const unsigned grid = (unsigned)std::min<uint64_t>(ntasks, (uint64_t)cus * per_cu);
   k_count_block<LOGS, G, U, BUF, TILES, N16><<<grid, kBlockThreads, 0, s>>>(
-      g->row_ptr, g->col, g->m, g->col16, g->sfx_dh, rng, ts, tasks, ntasks, a.queue + slot, a.result,
+      g->row_ptr, g->col, g->m, g->col16, g->sfx_dh, g->sfx_split, rng, ts, tasks, ntasks, a.queue + slot, a.result,
       a.att & ~kForceGlobal,
       a.tdur ? a.tdur + (tasks - a.task0) : nullptr, a.gid, a.gcount);
   *blocks += grid;
@@ -1461,7 +1577,7 @@ static void launch_block_g(const LaunchArgs& a, int slot, uint32_t ntasks, int c
                            const uint2* rng, const TileSrc& ts, const uint4* tasks, uint64_t* blocks) {
   const bool buf = g->m * 4 <= kBufMax && !(a.att & kForceGlobal);
   // narrow ranges exist only in the wedge layout (col16 is built with it)
-  const bool n16 = buf && g->col16 && g->sfx_dh && rng == g->rng_sfx && !(a.att & kWideBit);
+  const bool n16 = buf && g->col16 && g->sfx_dh && g->sfx_split && rng == g->rng_sfx && !(a.att & kWideBit);
   if (ts.trng) {
     if (n16)
       launch_block_t<LOGS, G, kU, true, true, true>(a, slot, ntasks, cus, s, g, rng, ts, tasks, blocks);

@@ -336,20 +336,38 @@ __global__ void k_col16(const uint32_t* __restrict__ col, uint64_t m, uint16_t* 
 }
 
 // Wedge range k = [b, e) (the part of N+(u) after v, v = col[b - 1]) is
-// narrow when its ids share their high 16 bits H; dh[k] = H - (v >> 16)
-// (255: not representable, the kernel reads col[b] instead).
+// narrow when its ids share their high 16 bits H (end bit 31), or, shorter
+// than 2^16 ids, when they have high half H up to position b + split[k] and
+// H + 1 from there (end bits 31 and 30). dh[k] = H - (v >> 16) (255: not
+// representable, the kernel reads col[b] instead).
 __global__ void k_flag_narrow(uint2* __restrict__ rng, uint64_t m, const uint32_t* __restrict__ col,
-                              uint8_t* __restrict__ dh) {
+                              uint8_t* __restrict__ dh, uint16_t* __restrict__ split) {
   const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= m) return;
   const uint2 r = rng[k];
-  uint32_t d = 0;
-  if (r.x < r.y && (col[r.x] >> 16) == (col[r.y - 1] >> 16)) {
-    rng[k].y = r.y | 0x80000000u;
-    d = (col[r.x] >> 16) - (col[r.x - 1] >> 16);  // ids after v: H >= v's high half
-    d = d < 255u ? d : 255u;
+  uint32_t d = 0, sp = 0;
+  if (r.x < r.y) {
+    const uint32_t f = col[r.x] >> 16, l = col[r.y - 1] >> 16;
+    const bool two = l == f + 1 && r.y - r.x < 65536u;
+    if (f == l || two) {
+      if (two) {  // first position with high half l: col[lo] has f, col[hi] has l
+        uint32_t lo = r.x, hi = r.y - 1;
+        while (lo + 1 < hi) {
+          const uint32_t mid = lo + ((hi - lo) >> 1);
+          if ((col[mid] >> 16) == f)
+            lo = mid;
+          else
+            hi = mid;
+        }
+        sp = hi - r.x;
+      }
+      rng[k].y = r.y | 0x80000000u | (two ? 0x40000000u : 0u);
+      d = f - (col[r.x - 1] >> 16);  // ids after v: H >= v's high half
+      d = d < 255u ? d : 255u;
+    }
   }
   dh[k] = (uint8_t)d;
+  split[k] = (uint16_t)sp;
 }
 
 __global__ void k_fill_u64(unsigned long long* __restrict__ p, uint64_t n) {
@@ -552,9 +570,11 @@ void preprocess(tribit_ctx* ctx, uint64_t* d_keys_in, uint64_t nkeys, uint64_t n
   // to whole dwords plus one)
   DevBuf<uint32_t> col16;
   DevBuf<uint8_t> sfx_dh;
+  DevBuf<uint16_t> sfx_split;
   if (ctx->relabel && m && m < (1ull << 30)) {
     sfx_dh = DevBuf<uint8_t>(ctx, m);
-    k_flag_narrow<<<grid_for(m, kBlock), kBlock, 0, s>>>(rng_sfx.p, m, col.p, sfx_dh.p);
+    sfx_split = DevBuf<uint16_t>(ctx, m);
+    k_flag_narrow<<<grid_for(m, kBlock), kBlock, 0, s>>>(rng_sfx.p, m, col.p, sfx_dh.p, sfx_split.p);
     const uint64_t words = (m + 1) / 2 + 1;
     col16 = DevBuf<uint32_t>(ctx, words);
     k_fill_u32<<<1, 64, 0, s>>>(col16.p + words - 2, 2, 0u);
@@ -579,11 +599,12 @@ void preprocess(tribit_ctx* ctx, uint64_t* d_keys_in, uint64_t nkeys, uint64_t n
   g->rng_sfx = rng_sfx.release();
   g->col16 = reinterpret_cast<uint16_t*>(col16.release());
   g->sfx_dh = sfx_dh.release();
+  g->sfx_split = sfx_split.release();
 
   g->sfx_ptr = sfx_ptr.release();
   g->w_sfx = w_sfx.release();
   g->bytes = 4 * (n + 1) * 3 + 4 * (m ? m : 1) + 20 * (m ? m : 1) + 4 * n + 8 * n * 3 + (g->labels ? 4 * n : 0) +
-             (g->rng_sfx ? 8 * (m ? m : 1) + 4 * (n + 1) + 8 * n : 0) + (g->col16 ? 4 * ((m + 1) / 2 + 1) + m : 0);
+             (g->rng_sfx ? 8 * (m ? m : 1) + 4 * (n + 1) + 8 * n : 0) + (g->col16 ? 4 * ((m + 1) / 2 + 1) + 3 * m : 0);
   g->tiles64 = m ? count_tiles(ctx, g) : 0;
   TB_HIP(hipEventRecord(ctx->ev[1], s));
   TB_HIP(hipEventSynchronize(ctx->ev[1]));
@@ -612,6 +633,8 @@ void free_graph_memory(tribit_graph* g) {
   g->col16 = nullptr;
   g->ctx->free(g->sfx_dh);
   g->sfx_dh = nullptr;
+  g->ctx->free(g->sfx_split);
+  g->sfx_split = nullptr;
   g->ctx->free(g->sfx_ptr);
   g->ctx->free(g->w_sfx);
   g->rng_sfx = nullptr;

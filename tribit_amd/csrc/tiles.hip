@@ -74,7 +74,7 @@ __global__ void k_tile_ranges(const uint2* __restrict__ rng, uint64_t m, const u
   const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= m) return;
   const uint2 r = rng[k];
-  const uint32_t e = r.y & 0x7FFFFFFFu;  // bit 31: narrow-range flag of wedge ranges
+  const uint32_t e = r.y & range_end_mask(m);  // bits 31 and 30: narrow-range flags of wedge ranges
   trng[k] = r.x < e ? make_uint2(tidx[r.x], tidx[e - 1] + 1) : make_uint2(0u, 0u);
 }
 
