@@ -953,7 +953,8 @@ __host__ __device__ inline int d_bin(uint32_t d) {
 
 // floor(2 * log2(1 + mean range length)), capped
 __host__ __device__ inline int l_bin(uint64_t w, uint32_t k) {
-  uint64_t x = 1 + (k ? w / k : 0);
+  // 32-bit division when it fits (a 64-bit one is a long software sequence on the GPU)
+  uint64_t x = 1 + (k ? (w <= 0xFFFFFFFFull ? (uint64_t)((uint32_t)w / k) : w / k) : 0);
   if (x > (1ull << 20)) x = 1ull << 20;
   const int f = 63 - __builtin_clzll(x);
   const int b = 2 * f + (x * x >= (1ull << (2 * f + 1)) ? 1 : 0);
@@ -1116,11 +1117,12 @@ __global__ __launch_bounds__(kSchedBlock) void k_sched_fill(SchedView sv, PlanDe
     const int s = pd.table[bucket];
     const int kind = pd.kind[s];
     const uint32_t sp = list_splits(kind, k, w), nch = list_chunks(kind, d);
+    const uint32_t q = k / sp, r = k % sp;  // split i: q lists, one more for i < r
     uint4* out = tasks + base.v[s] + o.v[s];
     for (uint32_t c = 0; c < nch; ++c)
       for (uint32_t i = 0; i < sp; ++i) {
-        const uint32_t k0 = (uint32_t)((uint64_t)k * i / sp), k1 = (uint32_t)((uint64_t)k * (i + 1) / sp);
-        out[j[s]++] = make_uint4((uint32_t)x, b + k0, k1 - k0, c | (bucket << kBucketShift));
+        const uint32_t k0 = i * q + min(i, r);
+        out[j[s]++] = make_uint4((uint32_t)x, b + k0, q + (i < r ? 1u : 0u), c | (bucket << kBucketShift));
       }
   }
 }
