@@ -19,7 +19,8 @@ algorithmic bytes over its own rows, divided by its HIP-event device time,
 against 8 TB/s: 4 bytes per owner vertex and per streamed range (edge), and per
 wedge check 4 bytes (a 32-bit id) or 2 bytes (an id of a narrow wedge range,
 read as its 16-bit low half; each such range also reads one byte, its high
-half relative to its owner's). cpu_baseline: the OpenMP CPU oracle
+half relative to its owner's, and a range across a 64K-id boundary two more,
+the position of the crossing). cpu_baseline: the OpenMP CPU oracle
 (edge iterator, merge intersection) timed on a strided row sample of the same
 CSR, extrapolated to the whole graph by merge work.
 """
@@ -322,7 +323,7 @@ def main() -> int:
     dom = int(np.argmax(ms_var))
     nar = work["narrow"][dom]
     b_dom = (4.0 * work["rows"][dom] + 4.0 * work["edges"][dom] + 4.0 * (work["wplus"][dom] - nar) + 2.0 * nar +
-             1.0 * work["narrow_ranges"][dom])
+             1.0 * work["narrow_ranges"][dom] + 2.0 * work["narrow2_ranges"][dom])
     achieved = b_dom / (ms_var[dom] * 1e-3) / 1e9 if ms_var[dom] > 0 else 0.0
     b_alg = 4.0 * (n + 1) + 4.0 * m + 4.0 * inf["wplus"]
 

@@ -132,6 +132,8 @@ typedef struct {
                               streamed by block tasks) */
   uint64_t narrow_ranges[8]; /* ranges streamed that way (each also reads one byte: its ids' high half
                                 relative to its owner's) */
+  uint64_t narrow2_ranges[8]; /* of those, ranges across one 64K-id boundary (two more bytes each: the
+                                 position of the crossing) */
 } tribit_variant_work;
 
 /* ---- context ------------------------------------------------------------ */

@@ -104,7 +104,8 @@ class Profile(ctypes.Structure):
 
 class VariantStats(ctypes.Structure):
     _fields_ = [("rows", ctypes.c_uint64 * 8), ("edges", ctypes.c_uint64 * 8), ("wplus", ctypes.c_uint64 * 8),
-                ("narrow", ctypes.c_uint64 * 8), ("narrow_ranges", ctypes.c_uint64 * 8)]
+                ("narrow", ctypes.c_uint64 * 8), ("narrow_ranges", ctypes.c_uint64 * 8),
+                ("narrow2_ranges", ctypes.c_uint64 * 8)]
 
 
 _lib = None
@@ -501,7 +502,8 @@ class Graph:
         vs = VariantStats()
         _check(self.ctx.lib.tribit_variant_stats(self.ctx.h, self.h, SCHED[sched], ctypes.byref(vs)), "variant_stats")
         return {"rows": list(vs.rows), "edges": list(vs.edges), "wplus": list(vs.wplus),
-                "narrow": list(vs.narrow), "narrow_ranges": list(vs.narrow_ranges)}
+                "narrow": list(vs.narrow), "narrow_ranges": list(vs.narrow_ranges),
+                "narrow2_ranges": list(vs.narrow2_ranges)}
 
     def free(self) -> None:
         if self.h:

@@ -1165,12 +1165,12 @@ __global__ __launch_bounds__(256) void k_bucket_stats(SchedView sv, const uint32
     if (acc[i]) atomicAdd(&st[i], acc[i]);
 }
 
-// Per slot of a wedge plan: elements and ranges of narrow wedge ranges (the
-// lists' kept ranges whose end carries the narrow flag).
+// Per slot of a wedge plan: elements, ranges and two-window ranges of narrow
+// wedge ranges (the lists' kept ranges whose end carries the narrow flag).
 __global__ __launch_bounds__(256) void k_narrow_stats(SchedView sv, uint64_t n, const uint8_t* __restrict__ table,
                                                       unsigned long long* __restrict__ out) {
-  __shared__ unsigned long long acc[2 * kSlots];
-  if (threadIdx.x < 2 * kSlots) acc[threadIdx.x] = 0ull;
+  __shared__ unsigned long long acc[3 * kSlots];
+  if (threadIdx.x < 3 * kSlots) acc[threadIdx.x] = 0ull;
   __syncthreads();
   const uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (x < n) {
@@ -1180,22 +1180,24 @@ __global__ __launch_bounds__(256) void k_narrow_stats(SchedView sv, uint64_t n, 
     owner_list(sv, x, 0, &b, &k, &w);
     if (list_useful(sv, 0, d, k)) {
       const int slot = table[bucket_of(d, w, k)];
-      unsigned long long ne = 0, nr = 0;
+      unsigned long long ne = 0, nr = 0, n2 = 0;
       for (uint32_t i = b; i < b + k; ++i) {
         const uint2 r = sv.rng[i];
         if (r.y & kNarrowFlag) {
           ne += (r.y & 0x3FFFFFFFu) - r.x;  // flags exist only when m < 2^30
           ++nr;
+          n2 += (r.y & kTwoWinFlag) ? 1u : 0u;
         }
       }
       if (nr) {
         atomicAdd(&acc[slot], ne);
         atomicAdd(&acc[kSlots + slot], nr);
+        if (n2) atomicAdd(&acc[2 * kSlots + slot], n2);
       }
     }
   }
   __syncthreads();
-  if (threadIdx.x < 2 * kSlots && acc[threadIdx.x]) atomicAdd(&out[threadIdx.x], acc[threadIdx.x]);
+  if (threadIdx.x < 3 * kSlots && acc[threadIdx.x]) atomicAdd(&out[threadIdx.x], acc[threadIdx.x]);
 }
 
 // Timing logs: per (bucket) sum of the durations of a slot's tasks, from the
@@ -2040,17 +2042,18 @@ void variant_stats(tribit_ctx* ctx, tribit_graph* g, int sched, tribit_variant_w
   if (layout == TRIBIT_SCHED_WEDGE && g->col16 && g->m * 4 <= kBufMax && !(flags & TRIBIT_COUNT_WIDE_IDS) &&
       !(flags & TRIBIT_COUNT_FORCE_GLOBAL_LOADS) && g->n) {
     DevBuf<uint8_t> dt(ctx, kBuckets);
-    DevBuf<unsigned long long> ns(ctx, 2 * kSlots);
+    DevBuf<unsigned long long> ns(ctx, 3 * kSlots);
     TB_HIP(hipMemcpyAsync(dt.p, plan.table, kBuckets, hipMemcpyHostToDevice, ctx->stream));
-    k_zero_u64<<<1, 64, 0, ctx->stream>>>(ns.p, 2 * kSlots);
+    k_zero_u64<<<1, 64, 0, ctx->stream>>>(ns.p, 3 * kSlots);
     k_narrow_stats<<<grid_for(g->n, 256), 256, 0, ctx->stream>>>(sched_view(g, layout, d0, cut), g->n, dt.p, ns.p);
-    unsigned long long h[2 * kSlots];
+    unsigned long long h[3 * kSlots];
     TB_HIP(hipMemcpyAsync(h, ns.p, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
     ctx->sync();
     for (int s = 0; s < plan.nslots; ++s)
       if (plan.kind[s] == kKindBlock) {
         out->narrow[s] = h[s];
         out->narrow_ranges[s] = h[kSlots + s];
+        out->narrow2_ranges[s] = h[2 * kSlots + s];
       }
   }
 }
