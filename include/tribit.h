@@ -164,6 +164,14 @@ int tribit_device_info(tribit_ctx* ctx, char* name, size_t name_len, int* cus, d
  * (min<<32)|max. Validates ids (reference: undefined behaviour).
  * *keys is allocated by the library; release with tribit_free_host. */
 int tribit_read_mtx(const char* path, uint64_t** keys, uint64_t* nkeys, uint64_t* n, uint64_t* entries);
+/* BVGraph (WebGraph) reader with the semantics of the reference's multi-GPU
+ * ingest (src/multi_gpu/src/main.rs:88, :150-161: BvGraphSeq over
+ * <basename>.properties + <basename>.graph, every arc (u, v), u != v, as
+ * (min<<32)|max, both directions of a symmetric graph kept). Sequential
+ * decode of the published format (big-endian bit order; gamma, delta, unary
+ * and zeta_k codes per the compressionflags). *keys is allocated by the
+ * library; release with tribit_free_host. */
+int tribit_read_webgraph(const char* basename, uint64_t** keys, uint64_t* nkeys, uint64_t* n);
 void tribit_free_host(void* p);
 
 /* Generate R-MAT edge keys in HBM (2^scale vertices, ef<<scale samples,

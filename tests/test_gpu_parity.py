@@ -380,6 +380,34 @@ def test_harness_csv_wiki_vote(tmp_path):
     assert int(row[9]) >= 8 * 103689  # the pool held at least the edge keys
 
 
+def test_webgraph_input(tmp_path, gpu_ctx, oracle_mod):
+    """A BVGraph file (written by the test encoder, both directions of every edge, as the
+    reference's symmetric WebGraph inputs) read by tribit_read_webgraph and by `tribit --webgraph`:
+    the count equals the oracle's on the same edges."""
+    from tests import webgraph_enc
+    note("webgraph input rmat s12")
+    keys = oracle_mod.gen_rmat(12, 8, seed=9)
+    n = 1 << 12
+    g0 = oracle_mod.Graph.from_keys(keys, n)
+    want = g0.count()
+    u = (keys >> np.uint64(32)).astype(np.int64)
+    v = (keys & np.uint64(0xFFFFFFFF)).astype(np.int64)
+    succ = [set() for _ in range(n)]
+    for a, b in zip(u.tolist(), v.tolist()):
+        if a != b:
+            succ[a].add(b)
+            succ[b].add(a)
+    base = str(tmp_path / "rmat12")
+    webgraph_enc.write(base, [sorted(s) for s in succ])
+    wkeys, wn = tribit_amd.read_webgraph(base)
+    assert wn == n
+    with gpu_ctx.graph_from_keys(wkeys, wn) as g:
+        assert g.count(sched="learned").triangles == want
+    out = subprocess.run([tribit_amd.CLI_PATH, "--webgraph", base], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    assert re.search(r"Triangles\s*:\s*(\d+)", out.stdout).group(1) == str(want)
+
+
 def test_cli_generated_config1():
     note("cli rmat s13 ef12 seed 1")
     out = subprocess.run([tribit_amd.CLI_PATH, "--rmat", "13,12,0.57,0.19,0.19,1"], capture_output=True, text=True,

@@ -141,6 +141,7 @@ def load_library(path: str | None = None) -> ctypes.CDLL:
         "tribit_device_info": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_int),
                                               ctypes.POINTER(ctypes.c_double)]),
         "tribit_read_mtx": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(_U64P), _U64P, _U64P, _U64P]),
+        "tribit_read_webgraph": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(_U64P), _U64P, _U64P]),
         "tribit_free_host": (None, [_P]),
         "tribit_gen_rmat": (ctypes.c_int, [_P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_double, ctypes.c_double,
                                            ctypes.c_double, _U64, ctypes.POINTER(_P), _U64P]),
@@ -283,6 +284,21 @@ def read_mtx(path: str):
     finally:
         lib.tribit_free_host(ctypes.cast(kp, _P))
     return keys, n.value, ne.value
+
+
+def read_webgraph(basename: str):
+    """BVGraph <basename>.properties/.graph -> (keys uint64[(min<<32)|max] per arc, n), the reference's
+    multi-GPU ingest (src/multi_gpu/src/main.rs:88, :150-161)."""
+    lib = load_library()
+    kp = _U64P()
+    nk, n = _U64(), _U64()
+    _check(lib.tribit_read_webgraph(basename.encode(), ctypes.byref(kp), ctypes.byref(nk), ctypes.byref(n)),
+           f"read_webgraph({basename})")
+    try:
+        keys = np.ctypeslib.as_array(kp, shape=(nk.value,)).copy() if nk.value else np.zeros(0, np.uint64)
+    finally:
+        lib.tribit_free_host(ctypes.cast(kp, _P))
+    return keys, n.value
 
 
 @dataclass

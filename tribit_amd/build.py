@@ -22,7 +22,7 @@ LIB = os.path.join(HERE, "libtribit_hip.so")
 CLI = os.path.join(HERE, "tribit")
 
 HIP_SOURCES = ["prep.hip", "gen.hip", "tiles.hip", "count.hip", "dense.hip", "batch.hip", "mlp.hip", "api.hip"]
-HOST_SOURCES = ["mtx.cpp"]
+HOST_SOURCES = ["mtx.cpp", "webgraph.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = [

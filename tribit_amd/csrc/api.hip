@@ -156,6 +156,15 @@ int tribit_read_mtx(const char* path, uint64_t** keys, uint64_t* nkeys, uint64_t
   }
 }
 
+int tribit_read_webgraph(const char* basename, uint64_t** keys, uint64_t* nkeys, uint64_t* n) {
+  if (!basename || !keys || !nkeys || !n) return TRIBIT_ERR_ARG;
+  try {
+    return tb::read_webgraph(basename, keys, nkeys, n);
+  } catch (const std::bad_alloc&) {
+    return TRIBIT_ERR_NOMEM;
+  }
+}
+
 void tribit_free_host(void* p) { free(p); }
 
 int tribit_gen_rmat(tribit_ctx* ctx, uint32_t scale, uint32_t ef, double a, double b, double c, uint64_t seed,

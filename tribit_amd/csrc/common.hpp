@@ -284,4 +284,6 @@ void free_tiles(tribit_graph* g);
 // count.hip
 // mtx.cpp
 int read_mtx(const char* path, uint64_t** keys, uint64_t* nkeys, uint64_t* n, uint64_t* entries);
+// webgraph.cpp
+int read_webgraph(const char* basename, uint64_t** keys, uint64_t* nkeys, uint64_t* n);
 }  // namespace tb

@@ -11,6 +11,8 @@
 // Added (long options only, so `-i` stays byte-compatible):
 //   --rmat S,EF,A,B,C,SEED  generate an R-MAT graph in HBM instead of reading
 //   --er N,M,SEED           generate Erdos-Renyi G(n, M) in HBM
+//   --webgraph BASENAME     read a BVGraph (WebGraph) graph, the input format of
+//                           the reference's multi-GPU path, instead of -i
 //   --sched plain|directed|wedge|learned   scheduler choice (default learned)
 //   --repeats K             timed counting repetitions (median reported)
 //   --json PATH             write a JSON side record of the measured pass
@@ -34,7 +36,7 @@ namespace {
 
 struct Config {
   std::string input_file;
-  std::string rmat, er, json, model;
+  std::string rmat, er, json, model, webgraph;
   int sched = TRIBIT_SCHED_LEARNED;
   int repeats = 1;
 };
@@ -46,6 +48,7 @@ std::string get_usage(const char* prog) {
          "  -i <input_file>    Path to the input file (required)\n"
          "  --rmat S,EF,A,B,C,SEED   Generate an R-MAT graph instead of reading a file\n"
          "  --er N,M,SEED            Generate an Erdos-Renyi G(n,M) graph instead\n"
+         "  --webgraph BASENAME      Read a BVGraph (BASENAME.properties, BASENAME.graph) instead\n"
          "  --sched plain|directed|wedge|learned  Scheduler (default learned)\n"
          "  --repeats K              Timed counting repetitions (default 1)\n"
          "  --json PATH              Write a JSON record of the measured pass\n"
@@ -67,6 +70,7 @@ Config program_options(int argc, char** argv) {
                                {"repeats", required_argument, nullptr, 4},
                                {"json", required_argument, nullptr, 5},
                                {"model", required_argument, nullptr, 6},
+                               {"webgraph", required_argument, nullptr, 7},
                                {nullptr, 0, nullptr, 0}};
   int opt;
   while ((opt = getopt_long(argc, argv, "i:", lo, nullptr)) != -1) {
@@ -92,10 +96,11 @@ Config program_options(int argc, char** argv) {
         break;
       case 5: c.json = optarg; break;
       case 6: c.model = optarg; break;
+      case 7: c.webgraph = optarg; break;
       default: usage_exit(argv[0], "Error: unknown option or missing argument\n");
     }
   }
-  if (c.input_file.empty() && c.rmat.empty() && c.er.empty())
+  if (c.input_file.empty() && c.rmat.empty() && c.er.empty() && c.webgraph.empty())
     usage_exit(argv[0], "Error: input file is required (-i option)\n");
   if (optind < argc) fprintf(stderr, "Warning: extra arguments ignored\n");
   return c;
@@ -191,6 +196,12 @@ int main(int argc, char** argv) {
     if (sscanf(cfg.er.c_str(), "%llu,%llu,%llu", &in.er_n, &in.er_m, &in.seed) != 3)
       usage_exit(argv[0], "Error: --er expects N,M,SEED\n");
     graph_name = "er:" + cfg.er;
+  } else if (!cfg.webgraph.empty()) {
+    printf("Reading data from %s (BVGraph)...\n", cfg.webgraph.c_str());
+    fflush(stdout);
+    st = tribit_read_webgraph(cfg.webgraph.c_str(), &in.host_keys, &in.nkeys, &in.n);
+    if (st) die(2, cfg.webgraph.c_str(), st);
+    graph_name = cfg.webgraph;
   } else {
     printf("Reading data from %s...\n", cfg.input_file.c_str());
     fflush(stdout);
