@@ -20,8 +20,6 @@
 #include <cstring>
 #include <algorithm>
 #include <chrono>
-#include <rocprim/block/block_reduce.hpp>
-#include <rocprim/block/block_scan.hpp>
 #include <rocprim/device/device_scan.hpp>
 #include <vector>
 
@@ -1066,48 +1064,39 @@ __device__ __forceinline__ Cnt8 owner_counts(const SchedView& sv, const PlanDev&
   return c;
 }
 
-// Task lists in owner order per slot, in two passes over blocks of
-// kSchedBlock owners: per block the per-slot task counts (scanned over the
-// blocks on the device), then each block scans its owners' counts in LDS and
-// writes their tasks at its block's offsets. Only per-block sums go through
-// HBM.
+// Task lists in owner order per slot (up to the order inside a block), in
+// two passes over blocks of kSchedBlock owners: per block the per-slot task
+// counts (LDS counters; scanned over the blocks on the device), then each
+// owner takes its place in its block's region by an LDS atomic. Only
+// per-block sums go through HBM.
 constexpr unsigned kSchedBlock = 256;
 
 __global__ __launch_bounds__(kSchedBlock) void k_sched_count(SchedView sv, PlanDev pd, uint64_t n,
                                                              Cnt8* __restrict__ blocksum) {
-  using Reduce = rocprim::block_reduce<Cnt8, kSchedBlock>;
-  __shared__ typename Reduce::storage_type tmp;
+  __shared__ uint32_t sc[kSlots];
+  if (threadIdx.x < kSlots) sc[threadIdx.x] = 0;
+  __syncthreads();
   const uint64_t x = (uint64_t)blockIdx.x * kSchedBlock + threadIdx.x;
-  Cnt8 c;
   if (x < n) {
-    c = owner_counts(sv, pd, x);
-  } else {
-    for (int i = 0; i < kSlots; ++i) c.v[i] = 0;
+    const Cnt8 c = owner_counts(sv, pd, x);
+    for (int i = 0; i < kSlots; ++i)
+      if (c.v[i]) atomicAdd(&sc[i], c.v[i]);
   }
-  Cnt8 t;
-  Reduce().reduce(c, t, tmp, Cnt8Plus());
-  if (threadIdx.x == 0) blocksum[blockIdx.x] = t;
+  __syncthreads();
+  if (threadIdx.x < kSlots) blocksum[blockIdx.x].v[threadIdx.x] = sc[threadIdx.x];
 }
 
-// Writes each owner's tasks at its block's offset plus its prefix in the block.
+// Writes each owner's tasks in its block's region of each slot; inside a
+// block (kSchedBlock consecutive owners) the order is that of LDS atomics.
 __global__ __launch_bounds__(kSchedBlock) void k_sched_fill(SchedView sv, PlanDev pd, uint64_t n,
                                                             const Cnt8* __restrict__ blockoff, Cnt8 base,
                                                             uint4* __restrict__ tasks) {
-  using Scan = rocprim::block_scan<Cnt8, kSchedBlock>;
-  __shared__ typename Scan::storage_type tmp;
+  __shared__ uint32_t sc[kSlots];
+  if (threadIdx.x < kSlots) sc[threadIdx.x] = 0;
+  __syncthreads();
   const uint64_t x = (uint64_t)blockIdx.x * kSchedBlock + threadIdx.x;
-  Cnt8 c;
-  if (x < n) {
-    c = owner_counts(sv, pd, x);
-  } else {
-    for (int i = 0; i < kSlots; ++i) c.v[i] = 0;
-  }
-  Cnt8 o;
-  Scan().exclusive_scan(c, o, blockoff[blockIdx.x], tmp, Cnt8Plus());
   if (x >= n) return;
   const uint32_t d = sv.row_ptr[x + 1] - sv.row_ptr[x];
-  uint32_t j[kSlots];
-  for (int i = 0; i < kSlots; ++i) j[i] = 0;
   for (int l = 0; l < sv.nlists; ++l) {
     uint32_t b, k;
     uint64_t w;
@@ -1118,11 +1107,12 @@ __global__ __launch_bounds__(kSchedBlock) void k_sched_fill(SchedView sv, PlanDe
     const int kind = pd.kind[s];
     const uint32_t sp = list_splits(kind, k, w), nch = list_chunks(kind, d);
     const uint32_t q = k / sp, r = k % sp;  // split i: q lists, one more for i < r
-    uint4* out = tasks + base.v[s] + o.v[s];
+    uint4* out = tasks + base.v[s] + blockoff[blockIdx.x].v[s] + atomicAdd(&sc[s], sp * nch);
+    uint32_t j = 0;
     for (uint32_t c = 0; c < nch; ++c)
       for (uint32_t i = 0; i < sp; ++i) {
         const uint32_t k0 = i * q + min(i, r);
-        out[j[s]++] = make_uint4((uint32_t)x, b + k0, q + (i < r ? 1u : 0u), c | (bucket << kBucketShift));
+        out[j++] = make_uint4((uint32_t)x, b + k0, q + (i < r ? 1u : 0u), c | (bucket << kBucketShift));
       }
   }
 }
