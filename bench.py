@@ -1,7 +1,7 @@
 """Benchmark: exact triangle counting GTEPS on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--scale 25] [--ef 16] [--seed 5]
-                    [--sched learned|wedge|directed|plain] [--no-cpu-baseline]
+                    [--sched learned|wedge|directed|plain] [--no-cpu-baseline] [--cpu-full]
 
 Workload (default): R-MAT scale 25 (33,554,432 vertices), edge factor 16,
 (a,b,c,d) = (0.57,0.19,0.19,0.05), seed 5 -- generated in HBM by the library,
@@ -22,7 +22,9 @@ read as its 16-bit low half; each such range also reads one byte, its high
 half relative to its owner's, and a range across a 64K-id boundary two more,
 the position of the crossing). cpu_baseline: the OpenMP CPU oracle
 (edge iterator, merge intersection) timed on a strided row sample of the same
-CSR, extrapolated to the whole graph by merge work.
+CSR, extrapolated to the whole graph by merge work. When that sample is every
+row (small graphs, or --cpu-full), the oracle's count is compared with the
+GPU's (cpu_baseline.count_match).
 """
 from __future__ import annotations
 
@@ -52,8 +54,9 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(g, n: int, m: int, target_s: float) -> dict:
-    """Times the CPU oracle on a strided sample of rows of the same oriented CSR."""
+def cpu_baseline(g, n: int, m: int, target_s: float, triangles: int, full: bool = False) -> dict:
+    """Times the CPU oracle on a strided sample of rows of the same oriented CSR. When the sample is
+    every row (small graphs, or full=True), the oracle's count is compared with the GPU's."""
     import oracle
 
     if not os.path.exists(oracle.LIB_PATH):
@@ -75,11 +78,11 @@ def cpu_baseline(g, n: int, m: int, target_s: float) -> dict:
     total_work = float(work.sum())
     og = oracle.Graph.from_csr(rp, col)
     # grow the sample (smaller stride) until it takes about target_s
-    stride = 1 << 16
+    stride = 1 if full else 1 << 16
     while True:
         sample_work = float(work[::stride].sum())
         t0 = time.perf_counter()
-        og.count_strided(0, stride, threads)
+        sample_count = og.count_strided(0, stride, threads)
         elapsed = time.perf_counter() - t0
         if elapsed >= target_s / 3 or stride == 1:
             break
@@ -92,7 +95,7 @@ def cpu_baseline(g, n: int, m: int, target_s: float) -> dict:
             new *= 2
         stride = max(1, min(new, stride // 2))
     t_full = elapsed * total_work / max(sample_work, 1.0)
-    return {
+    out = {
         "value": round(m / t_full / 1e9, 6),
         "unit": "GTEPS",
         "cores": threads,
@@ -102,6 +105,10 @@ def cpu_baseline(g, n: int, m: int, target_s: float) -> dict:
                   f"extrapolated by merge work to {t_full:.1f} s for the whole graph",
         "cpu_model": cpu_model(),
     }
+    if stride == 1:
+        out["oracle_triangles"] = int(sample_count)
+        out["count_match"] = int(sample_count) == int(triangles)
+    return out
 
 
 def cpu_model() -> str:
@@ -271,6 +278,8 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-schedule comparison run")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-full", action="store_true",
+                    help="run the CPU oracle over every row and compare its count with the GPU's")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (reference kernel for HBM byte counters)")
     ap.add_argument("--batch-graphs", type=int, default=4096, help="graphs of the batch workload")
@@ -386,10 +395,13 @@ def main() -> int:
     }
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle on a row sample)")
-        out["cpu_baseline"] = cpu_baseline(g, n, m, args.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(g, n, m, args.cpu_seconds, tri, args.cpu_full)
     g.free()
     ctx.close()
     print(json.dumps(out), flush=True)
+    if out["cpu_baseline"] and out["cpu_baseline"].get("count_match") is False:
+        log(f"GPU count {tri} differs from the CPU oracle's {out['cpu_baseline']['oracle_triangles']}")
+        return 1
     return 0
 
 

@@ -153,7 +153,7 @@ tribit_batch* new_batch(tribit_ctx* ctx, uint32_t ngraphs) {
 
 void finish_batch(tribit_ctx* ctx, tribit_batch* b) {
   b->gcount = ctx->alloc_n<unsigned long long>(b->ngraphs ? b->ngraphs : 1);
-  b->scratch = ctx->alloc_n<unsigned long long>(8 * (b->parts.size() ? b->parts.size() : 1));
+  b->scratch = ctx->alloc_n<unsigned long long>(tb::kCountScratch * (b->parts.size() ? b->parts.size() : 1));
 }
 
 }  // namespace

@@ -131,7 +131,15 @@ struct DevBuf {
 
 namespace tb {
 struct BucketCache;
-}
+
+// Device counters of one count: u64 [0] triangles, [1] fallback tasks; from
+// u64 index kQueueOff, the task queues: per menu slot, kQueues sub-queues of
+// one u32 each, kQStride u32 apart so that no two share a cache line.
+constexpr uint32_t kQueues = 8;
+constexpr uint32_t kQStride = 64;
+constexpr size_t kQueueOff = 32;
+constexpr size_t kCountScratch = kQueueOff + (size_t)TRIBIT_MENU_SLOTS * kQueues * kQStride / 2;
+}  // namespace tb
 
 // Count-ready graph resident in HBM.
 struct tribit_graph {
@@ -174,6 +182,8 @@ struct tribit_graph {
   // dense core (dense.hip): int8 adjacency of the top core_c ids, padded to core_kp
   int8_t* core_A = nullptr;
   uint64_t core_c = 0, core_kp = 0;
+  uint32_t core_e0 = 0;    // row_ptr[n - core_c]: first arc of the core's rows
+  int64_t core_pick = -1;  // the learned schedule's core size from core_choose (-1: not chosen yet)
   tb::BucketCache* bcache = nullptr;  // scheduler statistics (count.hip)
   // the learned schedule's judged candidate (count.hip), valid for the model
   // version, flags and core request it was judged under

@@ -33,7 +33,8 @@ constexpr unsigned kBlock = 256;
 // id order, and with degree-rank ids the highest ids are the hubs, whose
 // tasks are the longest: taken first, they overlap with the many short
 // tasks instead of forming the launch's tail.
-constexpr uint32_t kWaveBatch = 16;      // wave tasks taken per queue atomic
+constexpr uint32_t kWaveBatch = 16;      // most wave tasks taken per queue atomic
+constexpr uint32_t kClaimsPerWave = 8;   // queue claims per wave of a wave launch (when tasks allow)
 constexpr uint32_t kChunk = 2048;        // probe-set chunk of class-3 rows
 constexpr uint64_t kWaveCap = 1u << 15;  // target streamed elements per wave task
 constexpr uint64_t kBlockCap = 1u << 18; // target streamed elements per block task
@@ -505,6 +506,54 @@ __device__ __forceinline__ uint32_t stream_tiles(const TileSrc& ts, const uint2*
 // ranges are rng[rbase, rbase + rcount). A wave owns one task: it builds the
 // set in its LDS slice once, then stages the ranges KMAX at a time.
 
+// Hands out the tasks of a wave launch to its waves in units of `unit`
+// consecutive tasks. The units are split over kQueues sub-queues (unit k goes
+// to sub-queue k % kQueues, in order: each sub-queue also starts with its
+// heaviest) and claimed `claim` units at a time; a block starts on sub-queue
+// blockIdx.x % kQueues and moves on to the next one when that one runs dry.
+// Claims on one address serialise (about 13 ns each on MI355X): with one
+// queue, a launch of ~100K tasks spent most of its time claiming. Every value
+// is wave-uniform.
+struct TaskFeed {
+  uint32_t* queue;
+  uint32_t ntasks, unit, claim;
+  uint32_t q, left;      // current sub-queue, sub-queues not yet found empty
+  uint32_t c, cend, nq;  // claimed units [c, cend) of sub-queue q, its unit count
+
+  __device__ TaskFeed(uint32_t* qs, uint32_t nt, uint32_t unit_, uint32_t claim_)
+      : queue(qs), ntasks(nt), unit(unit_), claim(claim_), q(blockIdx.x % kQueues), left(kQueues), c(0),
+        cend(0), nq(units_of(blockIdx.x % kQueues)) {}
+
+  __device__ uint32_t units_of(uint32_t sq) const {
+    const uint32_t nu = (ntasks + unit - 1) / unit;
+    return nu > sq ? (nu - sq + kQueues - 1) / kQueues : 0u;
+  }
+
+  // The next unit [tb, tend) of this wave; false when the launch has none left.
+  __device__ bool get(uint32_t lane, uint32_t& tb, uint32_t& tend) {
+    for (;;) {
+      if (c < cend) {
+        tb = (c * kQueues + q) * unit;
+        tend = min(tb + unit, ntasks);
+        ++c;
+        return true;
+      }
+      if (!left) return false;
+      uint32_t v = 0;
+      if (lane == 0) v = atomicAdd(queue + q * kQStride, claim);
+      v = __shfl(v, 0);
+      if (v < nq) {
+        c = v;
+        cend = min(v + claim, nq);
+      } else {
+        --left;
+        q = (q + 1) % kQueues;
+        nq = units_of(q);
+      }
+    }
+  }
+};
+
 template <int LOGS, int KMAX, int G, int U, bool BUF>
 __global__ __launch_bounds__(256) void k_count_wave(const uint32_t* __restrict__ row_ptr,
                                                     const uint32_t* __restrict__ col, uint64_t m,
@@ -514,7 +563,7 @@ __global__ __launch_bounds__(256) void k_count_wave(const uint32_t* __restrict__
                                                     unsigned long long* __restrict__ result, int attempts,
                                                     uint32_t* __restrict__ tdur,
                                                     const uint32_t* __restrict__ gid,
-                                                    unsigned long long* __restrict__ gcount) {
+                                                    unsigned long long* __restrict__ gcount, uint32_t wb) {
   constexpr uint32_t S = 1u << LOGS;
   const ColSrc src = make_col_src(col, m);
   __shared__ uint32_t sT[4][2 * S];
@@ -524,12 +573,9 @@ __global__ __launch_bounds__(256) void k_count_wave(const uint32_t* __restrict__
   uint2* R = sR[wv];
   uint64_t acc = 0;
   uint32_t fallbacks = 0;
-  for (;;) {
-    uint32_t tb = 0;
-    if (lane == 0) tb = atomicAdd(queue, kWaveBatch);
-    tb = __shfl(tb, 0);
-    if (tb >= ntasks) break;
-    const uint32_t tend = min(tb + kWaveBatch, ntasks);
+  TaskFeed feed(queue, ntasks, 1, wb);
+  uint32_t tb, tend;
+  while (feed.get(lane, tb, tend)) {
     for (uint32_t t = tb; t < tend; ++t) {
       const uint4 tk = tasks[ntasks - 1 - t];
       const uint32_t x = tk.x, rbase = tk.y, rcount = tk.z;
@@ -587,7 +633,7 @@ __global__ __launch_bounds__(256) void k_count_wave_multi(const uint32_t* __rest
                                                           const uint4* __restrict__ tasks, uint32_t ntasks,
                                                           uint32_t* __restrict__ queue,
                                                           unsigned long long* __restrict__ result, int attempts,
-                                                          uint32_t* __restrict__ tdur) {
+                                                          uint32_t* __restrict__ tdur, uint32_t wb) {
   constexpr uint32_t S = 1u << LOGS;
   static_assert(MT <= 16 && (MT & (MT - 1)) == 0, "MT: power of two up to 16");
   const ColSrc src = make_col_src(col, m);
@@ -605,12 +651,9 @@ __global__ __launch_bounds__(256) void k_count_wave_multi(const uint32_t* __rest
   uint32_t* RB = sInfo[wv][3];
   uint64_t acc = 0;
   uint32_t fallbacks = 0;
-  for (;;) {
-    uint32_t tb = 0;
-    if (lane == 0) tb = atomicAdd(queue, kWaveBatch);
-    tb = __shfl(tb, 0);
-    if (tb >= ntasks) break;
-    const uint32_t tend = min(tb + kWaveBatch, ntasks);
+  TaskFeed feed(queue, ntasks, MT, wb / MT);
+  uint32_t tb, tend;
+  while (feed.get(lane, tb, tend)) {
     for (uint32_t t0 = tb; t0 < tend; t0 += MT) {
       const uint32_t nt = min((uint32_t)MT, tend - t0);
       const uint64_t c0t = tdur ? (uint64_t)wall_clock64() : 0;
@@ -1462,13 +1505,25 @@ static void learned_table(tribit_ctx* ctx, const BucketStats& st, int layout, in
 struct LaunchArgs {
   int att = 0;                           // mode word (attempts | flags)
   unsigned long long* result = nullptr;  // [0] triangles, [1] fallback tasks
-  uint32_t* queue = nullptr;             // kSlots task counters
+  uint32_t* queue = nullptr;             // per slot kQueues task counters, kQStride apart
   uint32_t* tdur = nullptr;              // per-task durations, indexed like the schedule's tasks, or null
   const uint4* task0 = nullptr;          // the schedule's first task (tdur's origin)
   const uint32_t* gid = nullptr;         // graph of each vertex (batches), or null
   unsigned long long* gcount = nullptr;  // per-graph counts (batches), or null
   int waves[kSlots] = {0};               // workgroups per CU cap per slot (0: occupancy limit)
 };
+
+// Tasks per claim of a wave launch of `ntasks` tasks on `waves` waves: a
+// multiple of `unit` (the tasks a wave takes at once), at most kWaveBatch,
+// and small enough that a wave makes about kClaimsPerWave claims. A claim
+// takes consecutive tasks of its sub-queue, which at the list's end are often
+// the chunks of one hub's range list, so it stays small against a wave's share.
+static uint32_t claim_size(uint32_t ntasks, uint64_t waves, uint32_t unit) {
+  uint64_t b = ntasks / (waves * kClaimsPerWave);
+  if (b > kWaveBatch) b = kWaveBatch;
+  b = b / unit * unit;
+  return (uint32_t)(b < unit ? unit : b);
+}
 
 template <int LOGS, int KMAX, int G, int U, bool BUF>
 static void launch_wave_t(const LaunchArgs& a, int slot, uint32_t ntasks, int cus, hipStream_t s, tribit_graph* g,
@@ -1477,12 +1532,17 @@ static void launch_wave_t(const LaunchArgs& a, int slot, uint32_t ntasks, int cu
   TB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_count_wave<LOGS, KMAX, G, U, BUF>, kBlock, 0));
   if (a.waves[slot] > 0 && per_cu > a.waves[slot]) per_cu = a.waves[slot];
   if (per_cu < 1) per_cu = 1;
-  const uint64_t want = (ntasks + 4 * kWaveBatch - 1) / (4 * kWaveBatch);
+  // waves for about kWaveBatch tasks each (4 for the 512-slot sets, whose
+  // short launches on small graphs gained from more waves; the 64-slot
+  // kernel's did not)
+  const uint64_t tpw = LOGS >= kC1Logs ? 4 : kWaveBatch;
+  const uint64_t want = (ntasks + 4 * tpw - 1) / (4 * tpw);
   const unsigned grid = (unsigned)std::min<uint64_t>(want, (uint64_t)cus * per_cu);
+  const uint32_t wb = claim_size(ntasks, 4ull * grid, 1);
   k_count_wave<LOGS, KMAX, G, U, BUF><<<grid, kBlock, 0, s>>>(g->row_ptr, g->col, g->m, rng, tasks, ntasks,
-                                                             a.queue + slot, a.result, a.att & 0xFF,
+                                                             a.queue + (size_t)slot * kQueues * kQStride, a.result, a.att & 0xFF,
                                                              a.tdur ? a.tdur + (tasks - a.task0) : nullptr,
-                                                             a.gid, a.gcount);
+                                                             a.gid, a.gcount, wb);
   *blocks += grid;
 }
 
@@ -1521,8 +1581,9 @@ static void launch_multi_t(const LaunchArgs& a, int slot, uint32_t ntasks, int c
   if (per_cu < 1) per_cu = 1;
   const uint64_t want = (ntasks + 4 * kWaveBatch - 1) / (4 * kWaveBatch);
   const unsigned grid = (unsigned)std::min<uint64_t>(want, (uint64_t)cus * per_cu);
-  k<<<grid, kBlock, 0, s>>>(g->row_ptr, g->col, g->m, rng, tasks, ntasks, a.queue + slot, a.result, a.att & 0xFF,
-                            a.tdur ? a.tdur + (tasks - a.task0) : nullptr);
+  const uint32_t wb = claim_size(ntasks, 4ull * grid, kMultiTasks);
+  k<<<grid, kBlock, 0, s>>>(g->row_ptr, g->col, g->m, rng, tasks, ntasks, a.queue + (size_t)slot * kQueues * kQStride, a.result, a.att & 0xFF,
+                            a.tdur ? a.tdur + (tasks - a.task0) : nullptr, wb);
   *blocks += grid;
 }
 
@@ -1560,7 +1621,7 @@ static void launch_block_t(const LaunchArgs& a, int slot, uint32_t ntasks, int c
   if (per_cu < 1) per_cu = 1;
   const unsigned grid = (unsigned)std::min<uint64_t>(ntasks, (uint64_t)cus * per_cu);
   k_count_block<LOGS, G, U, BUF, TILES, N16><<<grid, kBlockThreads, 0, s>>>(
-      g->row_ptr, g->col, g->m, g->col16, g->sfx_dh, g->sfx_split, rng, ts, tasks, ntasks, a.queue + slot, a.result,
+      g->row_ptr, g->col, g->m, g->col16, g->sfx_dh, g->sfx_split, rng, ts, tasks, ntasks, a.queue + (size_t)slot * kQueues * kQStride, a.result,
       a.att & ~kForceGlobal,
       a.tdur ? a.tdur + (tasks - a.task0) : nullptr, a.gid, a.gcount);
   *blocks += grid;
@@ -1640,9 +1701,7 @@ void make_schedule(tribit_ctx* ctx, tribit_graph* g, int layout, int flags, bool
   if (core_c) {
     core_prepare(ctx, g, core_c);
     core_d0 = (uint32_t)(n - core_c);
-    uint32_t e0 = 0;
-    TB_HIP(hipMemcpyAsync(&e0, g->row_ptr + core_d0, 4, hipMemcpyDeviceToHost, s));
-    ctx->sync();
+    const uint32_t e0 = g->core_e0;
     // wedge entries start one past their arc; reverse entries at their row start
     core_cut = layout == TRIBIT_SCHED_WEDGE ? e0 + 1 : e0;
   }
@@ -1744,13 +1803,12 @@ static uint64_t count_once(tribit_ctx* ctx, tribit_graph* g, int sched, int flag
   S.plan = plan;
   make_schedule(ctx, g, sched, flags, learned, S, core_c);
   plan = S.plan;
-  // scratch: [0] count, [1] fallback tasks, [4..7] eight u32 queue counters
-  DevBuf<unsigned long long> scratch(ctx, 8);
-  k_zero_u64<<<1, 64, 0, s>>>(scratch.p, 8);
+  DevBuf<unsigned long long> scratch(ctx, kCountScratch);
+  k_zero_u64<<<grid_for(kCountScratch, kBlock), kBlock, 0, s>>>(scratch.p, kCountScratch);
   LaunchArgs a;
   a.att = mode_word(flags);
   a.result = scratch.p;
-  a.queue = reinterpret_cast<uint32_t*>(scratch.p + 4);
+  a.queue = reinterpret_cast<uint32_t*>(scratch.p + kQueueOff);
   DevBuf<uint32_t> tdur;
   if (tlog) {
     tdur = DevBuf<uint32_t>(ctx, S.ntask ? S.ntask : 1);
@@ -1833,8 +1891,10 @@ static bool candidate(tribit_ctx* ctx, tribit_graph* g, const tribit_count_opts&
   uint64_t core = 0;
   if (opts.dense_core > 0)
     core = std::min<uint64_t>((uint64_t)opts.dense_core, g->n);
-  else if (opts.dense_core == 0 && g->relabeled)
-    core = core_choose(ctx, g);
+  else if (opts.dense_core == 0 && g->relabeled) {
+    if (g->core_pick < 0) g->core_pick = (int64_t)core_choose(ctx, g);  // the graph does not change
+    core = (uint64_t)g->core_pick;
+  }
   int dummy = 0;
   switch (c) {
     case kCandModel:
@@ -2099,15 +2159,15 @@ static double batch_capture(tribit_ctx* ctx, tribit_batch* b, int att, uint32_t*
   TB_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
   try {
     k_zero_u64<<<grid_for(b->ngraphs ? b->ngraphs : 1, kBlock), kBlock, 0, s>>>(b->gcount, b->ngraphs);
-    const uint32_t nw = (uint32_t)(8 * b->parts.size());
+    const uint32_t nw = (uint32_t)(kCountScratch * b->parts.size());
     k_zero_u64<<<grid_for(nw ? nw : 1, kBlock), kBlock, 0, s>>>(b->scratch, nw);
     for (size_t i = 0; i < b->parts.size(); ++i) {
       auto& p = b->parts[i];
       for (int k = 0; k < kSlots; ++k) p.S->plan.g[k] = b->gw[k] ? b->gw[k] : p.S->gdef[k];
       LaunchArgs a;
       a.att = att;
-      a.result = b->scratch + 8 * i;
-      a.queue = reinterpret_cast<uint32_t*>(b->scratch + 8 * i + 4);
+      a.result = b->scratch + kCountScratch * i;
+      a.queue = reinterpret_cast<uint32_t*>(b->scratch + kCountScratch * i + kQueueOff);
       a.gid = p.gid;
       a.gcount = b->gcount;
       for (int k = 0; k < kSlots; ++k) a.waves[k] = b->waves[k];

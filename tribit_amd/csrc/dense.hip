@@ -145,9 +145,13 @@ void core_prepare(tribit_ctx* ctx, tribit_graph* g, uint64_t c) {
   k_core_zero<<<2048, 256, 0, s>>>(reinterpret_cast<uint4*>(A.p), kp * kp / 16);
   k_core_fill<<<grid_for(c, 256), 256, 0, s>>>(g->row_ptr, g->col, g->n, d0, (uint32_t)kp, A.p);
   TB_HIP(hipGetLastError());
+  uint32_t e0 = 0;
+  TB_HIP(hipMemcpyAsync(&e0, g->row_ptr + d0, 4, hipMemcpyDeviceToHost, s));
+  ctx->sync();
   g->core_A = A.release();
   g->core_c = c;
   g->core_kp = kp;
+  g->core_e0 = e0;
 }
 
 void core_release(tribit_graph* g) {
