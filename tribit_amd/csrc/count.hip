@@ -974,6 +974,7 @@ struct Plan {
   int kind[kSlots];
   int g[kSlots];
   uint8_t table[kBuckets];  // bucket -> slot
+  bool menu = false;        // the learned schedule's menu (its launches may change kernel by task count)
 };
 
 struct BucketStats {
@@ -1323,6 +1324,7 @@ static Plan static_plan(const int gw[4]) {
 
 static Plan menu_plan() {
   Plan p;
+  p.menu = true;
   p.nslots = kSlots;
   for (int i = 0; i < kSlots; ++i) {
     p.kind[i] = kMenuKind[i];
@@ -1570,6 +1572,7 @@ static void launch_wave(const LaunchArgs& a, int slot, int gw, uint32_t ntasks, 
 }
 
 constexpr int kMultiTasks = 8;  // tasks per multi-task wave batch
+constexpr uint32_t kMultiMinTasksPerCU = 4096;  // menu wave-64 launches switch to multi-task waves above this
 
 template <int G, bool BUF>
 static void launch_multi_t(const LaunchArgs& a, int slot, uint32_t ntasks, int cus, hipStream_t s, tribit_graph* g,
@@ -1757,6 +1760,16 @@ void make_schedule(tribit_ctx* ctx, tribit_graph* g, int layout, int flags, bool
   S.tasks = DevBuf<uint4>(ctx, S.ntask ? S.ntask : 1);
   if (nb) k_sched_fill<<<(unsigned)nb, kSchedBlock, 0, s>>>(sv, pd, n, off.p, S.base, S.tasks.p);
   TB_HIP(hipGetLastError());
+  // menu plans: a one-task wave-64 launch with thousands of tasks per CU runs
+  // as multi-task waves (same tasks, eight per wave at once); with fewer, the
+  // one-task kernel was faster (R-MAT s25: 1.40 -> 1.03 ms at 2.1M tasks;
+  // s22, 0.39M tasks: 0.37 -> 0.57 ms)
+  const int cus = ctx->cus > 0 ? ctx->cus : 256;
+  if (plan.menu)
+    for (int i = 0; i < plan.nslots; ++i)
+      if (plan.kind[i] == kKindWave64 && S.tot.v[i] >= (uint64_t)kMultiMinTasksPerCU * cus &&
+          (plan.g[i] == 1 || plan.g[i] == 2 || plan.g[i] == 4))
+        plan.kind[i] = kKindWaveMulti;
 }
 
 // The counting launches of a schedule (capturable: no allocation, no sync).
