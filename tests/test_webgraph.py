@@ -148,3 +148,48 @@ def test_errors(tmp_path):
     with pytest.raises(tribit_amd.TribitError) as e:
         tribit_amd.read_webgraph(base)
     assert e.value.status == 2
+
+
+def test_malformed_values(tmp_path):
+    # values that no well-formed file holds are format errors, not out-of-range ids or overflows
+    base = str(tmp_path / "g")
+    props = {"nodes": 3, "arcs": 2, "windowsize": 0, "minintervallength": 0, "zetak": 3,
+             "compressionflags": "RESIDUALS_GAMMA"}
+    w = enc.BitWriter()
+    w.gamma(2)              # node 0: outdegree 2
+    w.gamma(2)              # first residual: int2nat(1) -> id 1
+    w.gamma((1 << 61) - 1)  # a gap far beyond the node count
+    w.gamma(0)
+    w.gamma(0)
+    with open(base + ".graph", "wb") as f:
+        f.write(w.to_bytes())
+    with open(base + ".properties", "w") as f:
+        f.write("".join(f"{k}={v}\n" for k, v in props.items()))
+    with pytest.raises(tribit_amd.TribitError) as e:
+        tribit_amd.read_webgraph(base)
+    assert e.value.status == 2
+    # a reference to a node before the first one (node 0, r = 1)
+    props.update({"windowsize": 1, "compressionflags": ""})
+    w = enc.BitWriter()
+    w.gamma(1)
+    w.unary(1)
+    w.gamma(0)
+    w.gamma(0)
+    with open(base + ".graph", "wb") as f:
+        f.write(w.to_bytes())
+    with open(base + ".properties", "w") as f:
+        f.write("".join(f"{k}={v}\n" for k, v in props.items()))
+    with pytest.raises(tribit_amd.TribitError) as e:
+        tribit_amd.read_webgraph(base)
+    assert e.value.status == 2
+    # an outdegree above the node count
+    props.update({"windowsize": 0})
+    w = enc.BitWriter()
+    w.gamma(4)
+    with open(base + ".graph", "wb") as f:
+        f.write(w.to_bytes())
+    with open(base + ".properties", "w") as f:
+        f.write("".join(f"{k}={v}\n" for k, v in props.items()))
+    with pytest.raises(tribit_amd.TribitError) as e:
+        tribit_amd.read_webgraph(base)
+    assert e.value.status == 2

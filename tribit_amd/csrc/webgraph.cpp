@@ -291,7 +291,7 @@ int read_webgraph(const char* basename, uint64_t** keys_out, uint64_t* nkeys_out
         bool copy = true;
         for (uint64_t i = 0; i < nb && st == TRIBIT_OK; ++i) {
           const uint64_t len = in.code(pr.blocks, pr.zetak) + (i ? 1 : 0);
-          if (at + len > ref.size()) {
+          if (in.over || len > ref.size() || at + len > ref.size()) {
             st = TRIBIT_ERR_FORMAT;
             break;
           }
@@ -311,13 +311,19 @@ int read_webgraph(const char* basename, uint64_t** keys_out, uint64_t* nkeys_out
         const uint64_t ni = in.code(pr.intervals, pr.zetak);
         int64_t prev = 0;  // one past the previous interval's end
         for (uint64_t i = 0; i < ni; ++i) {
+          // every decoded value is checked against n before it enters a sum
+          const uint64_t g = in.code(pr.intervals, pr.zetak);
+          if (g > 2 * n) {
+            st = TRIBIT_ERR_FORMAT;
+            break;
+          }
           int64_t left;
           if (i == 0)
-            left = (int64_t)x + nat2int(in.code(pr.intervals, pr.zetak));
+            left = (int64_t)x + nat2int(g);
           else
-            left = prev + 1 + (int64_t)in.code(pr.intervals, pr.zetak);  // intervals are at least one id apart
+            left = prev + 1 + (int64_t)g;  // intervals are at least one id apart
           const uint64_t len = in.code(pr.intervals, pr.zetak) + (uint64_t)pr.min_interval;
-          if (left < 0 || (uint64_t)left + len > n || len > left_over) {
+          if (in.over || left < 0 || len > n || (uint64_t)left + len > n || len > left_over) {
             st = TRIBIT_ERR_FORMAT;
             break;
           }
@@ -329,12 +335,10 @@ int read_webgraph(const char* basename, uint64_t** keys_out, uint64_t* nkeys_out
       if (st != TRIBIT_OK) break;
       int64_t prev = 0;  // the previous residual
       for (uint64_t i = 0; i < left_over; ++i) {
-        int64_t v;
-        if (i == 0)
-          v = (int64_t)x + nat2int(in.code(pr.residuals, pr.zetak));
-        else
-          v = prev + 1 + (int64_t)in.code(pr.residuals, pr.zetak);
-        if (v < 0 || (uint64_t)v >= n) {
+        const uint64_t g = in.code(pr.residuals, pr.zetak);
+        int64_t v = -1;
+        if (g <= 2 * n) v = i == 0 ? (int64_t)x + nat2int(g) : prev + 1 + (int64_t)g;
+        if (in.over || v < 0 || (uint64_t)v >= n) {
           st = TRIBIT_ERR_FORMAT;
           break;
         }
