@@ -90,10 +90,14 @@ int or_read_mtx(const char* path, uint64_t** keys_out, uint64_t* nkeys_out, uint
   uint64_t nk = 0, ne = 0;
   int st = OR_OK;
   /* tribit.cu:252-268: entries "r c [values...]", 1-based, self-loops dropped,
-   * packed as (min<<32)|max. The loop ends at the first line that does not
-   * start with two integers (as `file >> r >> c` fails there). */
+   * packed as (min<<32)|max. Blank lines are skipped; the loop ends at the
+   * first other line that does not start with two integers (as
+   * `file >> r >> c` skips whitespace and fails there). */
   while ((len = getline(&line, &cap, f)) >= 0) {
     const char* p = line;
+    const char* q = line;
+    while (*q == ' ' || *q == '\t' || *q == '\r') ++q;
+    if (*q == '\n' || *q == '\0') continue; /* blank line: `file >> r >> c` skips whitespace */
     uint64_t r, c;
     if (!parse_u64(&p, &r) || !parse_u64(&p, &c)) break;
     if (r < 1 || c < 1 || r > nr || c > nr) {

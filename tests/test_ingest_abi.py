@@ -55,6 +55,8 @@ CASES = {
     "crlf_blank": ("%%MatrixMarket matrix coordinate pattern general\r\n%c\r\n\r\n3 3 2\r\n1 2\r\n3 2\r\n", 0, 3, 2,
                    [graphs.pack(0, 1), graphs.pack(1, 2)]),
     "stop_at_percent": ("%%MM\n3 3 3\n1 2\n% trailing comment ends the read\n2 3\n", 0, 3, 1, [graphs.pack(0, 1)]),
+    "blank_between_entries": ("%%MM\n3 3 3\n1 2\n\n  \t\r\n2 3\nx 1\n3 1\n", 0, 3, 2,
+                              [graphs.pack(0, 1), graphs.pack(1, 2)]),
     "values_multi": ("%%MM complex\n3 3 2\n1 2 1.0 2.0\n3 1 -4 5\n", 0, 3, 2, [graphs.pack(0, 1), graphs.pack(0, 2)]),
     "nonsquare": ("%%MM\n3 4 1\n1 2\n", 3, None, None, None),
     "id_zero": ("%%MM\n3 3 1\n0 2\n", 4, None, None, None),

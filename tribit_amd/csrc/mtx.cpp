@@ -7,10 +7,12 @@
 //   - rows != cols is an error (tribit.cu:241-245);
 //   - each entry line is "r c [values...]", 1-based; value columns ignored;
 //   - self-loops are dropped; every other entry becomes (min<<32)|max;
-//   - reading ends at the first line that does not start with two integers
-//     (the reference's `file >> r >> c` loop stops there too).
+//   - blank lines between entries are skipped and reading ends at the first
+//     other line that does not start with two integers (the reference's
+//     `file >> r >> c` loop skips whitespace and stops there too).
 // Divergences, on malformed input only: blank lines before the size line
-// are skipped; an id outside [1, n] is TRIBIT_ERR_RANGE (reference: UB).
+// are skipped; an id outside [1, n] is TRIBIT_ERR_RANGE (reference: UB); an
+// entry is one line (the reference would take r and c from two lines).
 //
 // The file is memory-mapped and the entry region is parsed by several host
 // threads, each on a line-aligned slice (the reference parses with one
@@ -67,6 +69,11 @@ void parse_slice(const char* p, const char* e, uint64_t n, Slice* out) {
   while (p < e) {
     const char* le = line_end(p, e);
     const char* q = p;
+    while (q < le && is_sp(*q)) ++q;
+    if (q == le) {  // blank line: `file >> r >> c` skips it
+      p = le + 1;
+      continue;
+    }
     uint64_t r, c;
     if (!parse_u64(q, le, r) || !parse_u64(q, le, c)) {
       out->stop = p;
