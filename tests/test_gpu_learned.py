@@ -47,6 +47,21 @@ def test_learned_exact_and_judged(model_ctx, oracle_mod, spec):
         assert again.triangles == want and again.candidate == first.candidate and again.ms_judge == 0
 
 
+def test_learned_without_judge(model_ctx, oracle_mod):
+    """NO_JUDGE runs the heuristic candidate at once (the partial-count entry points use it for graphs
+    counted once); a later plain LEARNED count of the same graph still judges."""
+    spec = HELD_OUT[0]
+    note(f"learned without judge {spec}")
+    r = {"spec": spec, "n": n_of(spec)}
+    want = oracle_mod.count_keys(gens.oracle_keys(oracle_mod, r), r["n"])
+    with gens.gpu_graph(model_ctx, r) as g:
+        quick = g.count(sched="learned", flags=tribit_amd.NO_JUDGE)
+        assert quick.triangles == want
+        assert quick.candidate == 1 and quick.ms_judge == 0 and not any(quick.ms_candidates)
+        judged = g.count(sched="learned")
+        assert judged.triangles == want and judged.ms_judge > 0
+
+
 def test_trained_model_gain(model_ctx):
     """Learned schedule with the trained model vs without it (same judge), held-out R-MAT scale 21."""
     note("learned gain over no model, rmat s21 seed 78")

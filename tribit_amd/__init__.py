@@ -58,6 +58,7 @@ NO_TILES = 4
 FORCE_TILES = 8
 NO_BITMAP = 16
 WIDE_IDS = 32
+NO_JUDGE = 64
 OPT_RELABEL = 1
 OPT_BATCH_PART_KEYS = 2
 

@@ -5,6 +5,7 @@
 #include <cstring>
 #include <vector>
 #include <new>
+#include <stdexcept>
 
 #include <rocprim/rocprim_version.hpp>
 
@@ -153,6 +154,8 @@ int tribit_read_mtx(const char* path, uint64_t** keys, uint64_t* nkeys, uint64_t
     return tb::read_mtx(path, keys, nkeys, n, entries);
   } catch (const std::bad_alloc&) {
     return TRIBIT_ERR_NOMEM;
+  } catch (...) {  // e.g. a parse thread that could not start
+    return TRIBIT_ERR_IO;
   }
 }
 
@@ -162,6 +165,10 @@ int tribit_read_webgraph(const char* basename, uint64_t** keys, uint64_t* nkeys,
     return tb::read_webgraph(basename, keys, nkeys, n);
   } catch (const std::bad_alloc&) {
     return TRIBIT_ERR_NOMEM;
+  } catch (const std::length_error&) {
+    return TRIBIT_ERR_NOMEM;
+  } catch (...) {
+    return TRIBIT_ERR_IO;
   }
 }
 
@@ -628,6 +635,7 @@ uint64_t count_vector(tribit_ctx* ctx, const std::vector<uint64_t>& keys, uint64
     tribit_count_opts o;
     memset(&o, 0, sizeof(o));
     o.sched = TRIBIT_SCHED_LEARNED;
+    o.flags = TRIBIT_COUNT_NO_JUDGE;  // a graph counted once: no candidate timing
     o.repeats = 1;
     tribit_count_info ci;
     tb::count(ctx, g, o, &ci);

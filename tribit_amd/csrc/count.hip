@@ -1940,7 +1940,10 @@ void count(tribit_ctx* ctx, tribit_graph* g, const tribit_count_opts& opts, trib
   CandidateRun run;
   int chosen = -1;
   double ms_judge = 0.0;
-  if (learned) {
+  if (learned && (opts.flags & TRIBIT_COUNT_NO_JUDGE)) {
+    chosen = kCandHeur;
+    candidate(ctx, g, opts, chosen, &run);
+  } else if (learned) {
     const bool stale = g->judge_choice < 0 || g->judge_model != ctx->model_version ||
                        g->judge_flags != opts.flags || g->judge_core != opts.dense_core;
     if (stale) {
@@ -2011,7 +2014,7 @@ void count(tribit_ctx* ctx, tribit_graph* g, const tribit_count_opts& opts, trib
   local.candidate = learned ? chosen : -1;
   local.layout = run.layout;
   local.ms_judge = ms_judge;
-  if (learned)
+  if (learned && !(opts.flags & TRIBIT_COUNT_NO_JUDGE))
     for (int c = 0; c < kCands; ++c) local.ms_candidates[c] = g->judge_ms[c];
   if (info) *info = local;
 }
