@@ -100,14 +100,43 @@ __global__ void k_validate(const uint64_t* __restrict__ keys, uint64_t nk, uint6
   if (!(a < b && b < n)) atomicOr(bad, 1u);
 }
 
-// deg[a]++ and deg[b]++ for each undirected edge; keys are sorted, so the
-// 'a' side is aggregated per run inside the wave before the atomic.
-__global__ void k_degree(const uint64_t* __restrict__ keys, uint64_t m, uint32_t* __restrict__ deg) {
+// pos[i] = 1 iff sorted key i is the first of its value and not the sentinel.
+__global__ void k_head_flags(const uint64_t* __restrict__ k, uint64_t nk, uint32_t* __restrict__ pos) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nk) return;
+  const uint64_t x = k[i];
+  pos[i] = (x != kSentinel && (i == 0 || k[i - 1] != x)) ? 1u : 0u;
+}
+
+// out[pos[i]] = k[i] for the heads (pos = exclusive scan of the head flags).
+__global__ void k_scatter_heads(const uint64_t* __restrict__ k, uint64_t nk, const uint32_t* __restrict__ pos,
+                                uint64_t* __restrict__ out) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nk) return;
+  if (pos[i + 1] != pos[i]) out[pos[i]] = k[i];
+}
+
+// Degrees of the sorted unique keys. Consecutive keys repeat their smaller
+// endpoint, so a wave adds each run of it with one atomic; the larger
+// endpoints are written to `hi`, sorted, and added the same way by
+// k_add_runs (one atomic per key took 37 ms for the 523M keys of R-MAT
+// scale 25).
+__global__ void k_degree_lo(const uint64_t* __restrict__ keys, uint64_t m, uint32_t* __restrict__ deg,
+                            uint32_t* __restrict__ hi) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = i < m;
   const uint64_t k = valid ? keys[i] : 0;
   const uint32_t a = valid ? (uint32_t)(k >> 32) : 0xFFFFFFFFu;
-  if (valid) atomicAdd(&deg[(uint32_t)k], 1u);
+  if (valid) hi[i] = (uint32_t)k;
+  const uint32_t run = wave_run_length(a, valid);
+  if (run) atomicAdd(&deg[a], run);
+}
+
+// deg[v] += the number of v in the sorted ids[0, m), one atomic per run per wave.
+__global__ void k_add_runs(const uint32_t* __restrict__ ids, uint64_t m, uint32_t* __restrict__ deg) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = i < m;
+  const uint32_t a = valid ? ids[i] : 0xFFFFFFFFu;
   const uint32_t run = wave_run_length(a, valid);
   if (run) atomicAdd(&deg[a], run);
 }
@@ -418,19 +447,38 @@ void preprocess(tribit_ctx* ctx, uint64_t* d_keys_in, uint64_t nkeys, uint64_t n
     DevBuf<uint64_t> sorted(ctx, nkeys);
     sort_keys(ctx, keys.p, sorted.p, nkeys, end_bit);
     keys.reset();
-    DevBuf<uint64_t> ucount(ctx, 1);
-    with_temp(ctx, [&](void* t, size_t& b) {
-      return rocprim::unique(t, b, sorted.p, uniq.p, ucount.p, (size_t)nkeys, rocprim::equal_to<uint64_t>(), s);
-    });
-    uint64_t h[2] = {0, 0};
-    TB_HIP(hipMemcpyAsync(&h[0], ucount.p, 8, hipMemcpyDeviceToHost, s));
-    ctx->sync();
-    if (h[0]) TB_HIP(hipMemcpyAsync(&h[1], uniq.p + h[0] - 1, 8, hipMemcpyDeviceToHost, s));
+    // unique: head flags, their exclusive scan (the last entry is the count),
+    // scatter of the heads; the sentinel is never a head. 32-bit positions
+    // (rocprim::unique took 16 ms for the 537M keys of R-MAT scale 25; more
+    // than 2^32 - 2 keys still go through it).
     uint32_t bad = 0;
-    TB_HIP(hipMemcpyAsync(&bad, flag.p, 4, hipMemcpyDeviceToHost, s));
-    ctx->sync();
+    if (nkeys <= 0xFFFFFFFEull) {
+      DevBuf<uint32_t> head(ctx, nkeys + 1), pos(ctx, nkeys + 1);
+      k_head_flags<<<grid_for(nkeys, kBlock), kBlock, 0, s>>>(sorted.p, nkeys, head.p);
+      k_fill_u32<<<1, 64, 0, s>>>(head.p + nkeys, 1, 0u);
+      with_temp(ctx, [&](void* t, size_t& b) {
+        return rocprim::exclusive_scan(t, b, head.p, pos.p, 0u, (size_t)(nkeys + 1), rocprim::plus<uint32_t>(), s);
+      });
+      k_scatter_heads<<<grid_for(nkeys, kBlock), kBlock, 0, s>>>(sorted.p, nkeys, pos.p, uniq.p);
+      uint32_t cnt = 0;
+      TB_HIP(hipMemcpyAsync(&cnt, pos.p + nkeys, 4, hipMemcpyDeviceToHost, s));
+      TB_HIP(hipMemcpyAsync(&bad, flag.p, 4, hipMemcpyDeviceToHost, s));
+      ctx->sync();
+      m = cnt;
+    } else {
+      DevBuf<uint64_t> ucount(ctx, 1);
+      with_temp(ctx, [&](void* t, size_t& b) {
+        return rocprim::unique(t, b, sorted.p, uniq.p, ucount.p, (size_t)nkeys, rocprim::equal_to<uint64_t>(), s);
+      });
+      uint64_t h[2] = {0, 0};
+      TB_HIP(hipMemcpyAsync(&h[0], ucount.p, 8, hipMemcpyDeviceToHost, s));
+      ctx->sync();
+      if (h[0]) TB_HIP(hipMemcpyAsync(&h[1], uniq.p + h[0] - 1, 8, hipMemcpyDeviceToHost, s));
+      TB_HIP(hipMemcpyAsync(&bad, flag.p, 4, hipMemcpyDeviceToHost, s));
+      ctx->sync();
+      m = h[0] - ((h[0] && h[1] == kSentinel) ? 1 : 0);
+    }
     if (bad) fail(TRIBIT_ERR_RANGE, "edge key is not (min < max < n)");
-    m = h[0] - ((h[0] && h[1] == kSentinel) ? 1 : 0);
   } else {
     keys.reset();
   }
@@ -440,7 +488,14 @@ void preprocess(tribit_ctx* ctx, uint64_t* d_keys_in, uint64_t nkeys, uint64_t n
   // 3. degrees
   DevBuf<uint32_t> deg(ctx, n);
   k_fill_u32<<<grid_for(n, kBlock), kBlock, 0, s>>>(deg.p, n, 0u);
-  if (m) k_degree<<<grid_for(m, kBlock), kBlock, 0, s>>>(uniq.p, m, deg.p);
+  if (m) {
+    DevBuf<uint32_t> hi(ctx, m), hi_sorted(ctx, m);
+    k_degree_lo<<<grid_for(m, kBlock), kBlock, 0, s>>>(uniq.p, m, deg.p, hi.p);
+    with_temp(ctx, [&](void* t, size_t& b) {
+      return rocprim::radix_sort_keys(t, b, hi.p, hi_sorted.p, (size_t)m, 0u, bits_for(n), s);
+    });
+    k_add_runs<<<grid_for(m, kBlock), kBlock, 0, s>>>(hi_sorted.p, m, deg.p);
+  }
 
   // 3b. optional degree relabelling: internal id = rank of (deg, id), so the
   // orientation below becomes the id order and the out-neighbours of
