@@ -86,6 +86,8 @@ typedef enum {
 #define TRIBIT_COUNT_WIDE_IDS 32          /* test hook: wedge ranges whose ids share their high 16 bits are
                                              streamed as 32-bit ids too (by default block tasks stream them
                                              from the 16-bit low halves, two ids per load) */
+#define TRIBIT_COUNT_FORCE_MULTI 128      /* test hook: every launch of the 64-slot wave kernel runs as multi-task
+                                             waves, whatever its task count */
 #define TRIBIT_COUNT_NO_JUDGE 64          /* LEARNED: run the density-threshold heuristic's candidate (preferred
                                              layout + dense core) without timing the candidates first; for
                                              graphs counted once, where judging would cost several counts */

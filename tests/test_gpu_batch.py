@@ -47,9 +47,10 @@ def test_batch_from_host_matches_oracle(gpu_ctx, oracle_mod):
             assert list(n) == ns
             assert [int(x) for x in m] == [oracle_mod.dedup(k).size for k in keys]
             for sched in SCHEDS:
-                c, ci = b.count(sched=sched, repeats=2)
-                assert [int(x) for x in c] == want, sched
-                assert ci["repeats_agree"] and ci["triangles"] == sum(want)
+                for fl in (0, tribit_amd.FORCE_MULTI, tribit_amd.FORCE_MULTI | tribit_amd.FORCE_LINEAR_PROBE):
+                    c, ci = b.count(sched=sched, repeats=2, flags=fl)
+                    assert [int(x) for x in c] == want, (sched, fl)
+                    assert ci["repeats_agree"] and ci["triangles"] == sum(want)
     finally:
         gpu_ctx.set_batch_part_keys(1 << 30)
 
@@ -66,6 +67,8 @@ def test_batch_chung_lu_and_autotune(gpu_ctx, oracle_mod):
         c, ci = b.count(repeats=3)
         assert [int(x) for x in c] == want
         assert ci["launches"] >= 3 and ci["repeats_agree"]
+        c, _ = b.count(sched="wedge", repeats=2, flags=tribit_amd.FORCE_MULTI)  # multi-task waves, per-graph counts
+        assert [int(x) for x in c] == want
         best = b.autotune(repeats=2)
         assert best["triangles"] == sum(want)
         c2, ci2 = b.count(repeats=2)

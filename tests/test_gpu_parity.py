@@ -225,6 +225,19 @@ def test_narrow_ranges(gpu_ctx):
                     assert g.count(sched="wedge", flags=fl, group_width=gw).triangles == want, (n, fl, gw)
 
 
+def test_forced_multi_task_waves(gpu_ctx, oracle_mod):
+    """Every launch of the 64-slot wave kernel as multi-task waves (the kernel large launches switch to),
+    under every schedule, with cuckoo and with fallback sets."""
+    note("forced multi-task waves")
+    for spec in [(13, 12, 1), (16, 16, 4)]:
+        keys = oracle_mod.gen_rmat(spec[0], spec[1], seed=spec[2])
+        want = oracle_mod.count_keys(keys, 1 << spec[0])
+        with gpu_ctx.gen_rmat(spec[0], spec[1], seed=spec[2]) as g:
+            for sched in scheds_of(g):
+                for fl in (tribit_amd.FORCE_MULTI, tribit_amd.FORCE_MULTI | tribit_amd.FORCE_LINEAR_PROBE):
+                    assert g.count(sched=sched, flags=fl).triangles == want, (spec, sched, fl)
+
+
 def test_fallback_linear_probe(gpu_ctx, oracle_mod):
     """The fallback sets (used when every cuckoo build of a task fails) count the same."""
     note("forced linear-probe fallback")

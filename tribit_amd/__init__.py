@@ -59,6 +59,7 @@ FORCE_TILES = 8
 NO_BITMAP = 16
 WIDE_IDS = 32
 NO_JUDGE = 64
+FORCE_MULTI = 128
 OPT_RELABEL = 1
 OPT_BATCH_PART_KEYS = 2
 

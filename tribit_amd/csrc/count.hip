@@ -240,9 +240,14 @@ __device__ __forceinline__ ColSrc make_col_src(const uint32_t* col, uint64_t m, 
   return c;
 }
 
-template <int G, int U, bool SHARED, bool BUF, typename Probe>
+// ATTR (multi-task waves of a batch): the hits of a range go to s_acc[its
+// set] when the group leaves the range (an LDS atomic per lane with hits),
+// so that every task's count can be added to its own graph; the return value
+// is then 0.
+template <int G, int U, bool SHARED, bool BUF, bool ATTR = false, typename Probe>
 __device__ __forceinline__ uint32_t stream_ranges(const ColSrc& src, const uint2* R, uint32_t nk, uint32_t* s_next,
-                                                  uint32_t lane, const Probe& probe, const uint8_t* Rsel = nullptr) {
+                                                  uint32_t lane, const Probe& probe, const uint8_t* Rsel = nullptr,
+                                                  unsigned long long* s_acc = nullptr) {
   constexpr uint64_t kLeads = leader_mask<G>();
   const uint32_t g = lane / G, gl = lane % G;
   const uint64_t below = (g == 0) ? 0ull : ((1ull << (g * G)) - 1);
@@ -263,6 +268,10 @@ __device__ __forceinline__ uint32_t stream_ranges(const ColSrc& src, const uint2
       if (want == kLeads && kb >= nk) break;
     }
     if (need) {
+      if (ATTR && cnt) {  // the range just left
+        atomicAdd(&s_acc[sel], (unsigned long long)cnt);
+        cnt = 0;
+      }
       const uint32_t k = kb + (uint32_t)__popcll(want & below);
       if (k < nk) {
         const uint2 r = R[k];
@@ -294,6 +303,10 @@ __device__ __forceinline__ uint32_t stream_ranges(const ColSrc& src, const uint2
 #pragma unroll
     for (int j = 0; j < U; ++j) cnt += probe(x[j], sel);
     base += (uint32_t)(G * U);
+  }
+  if (ATTR) {
+    if (cnt) atomicAdd(&s_acc[sel], (unsigned long long)cnt);
+    return 0;
   }
   return cnt;
 }
@@ -624,8 +637,10 @@ __global__ __launch_bounds__(256) void k_count_wave(const uint32_t* __restrict__
 // builds their MT sets (2S slots each) side by side in its LDS slice, stages
 // the concatenation of their range lists with the set of each range, and
 // streams them together, so a wave keeps up to 64 short lists in flight
-// instead of the few of one small owner. Batches (per-graph counts) use the
-// one-task kernel; timing logs split a batch's time evenly over its tasks.
+// instead of the few of one small owner. With per-graph counts (batches)
+// every range's hits are added to its task's LDS counter and each task's
+// count to its owner's graph; timing logs split a group's time evenly over
+// its tasks.
 template <int LOGS, int KMAX, int MT, int G, int U, bool BUF>
 __global__ __launch_bounds__(256) void k_count_wave_multi(const uint32_t* __restrict__ row_ptr,
                                                           const uint32_t* __restrict__ col, uint64_t m,
@@ -633,7 +648,9 @@ __global__ __launch_bounds__(256) void k_count_wave_multi(const uint32_t* __rest
                                                           const uint4* __restrict__ tasks, uint32_t ntasks,
                                                           uint32_t* __restrict__ queue,
                                                           unsigned long long* __restrict__ result, int attempts,
-                                                          uint32_t* __restrict__ tdur, uint32_t wb) {
+                                                          uint32_t* __restrict__ tdur, uint32_t wb,
+                                                          const uint32_t* __restrict__ gid,
+                                                          unsigned long long* __restrict__ gcount) {
   constexpr uint32_t S = 1u << LOGS;
   static_assert(MT <= 16 && (MT & (MT - 1)) == 0, "MT: power of two up to 16");
   const ColSrc src = make_col_src(col, m);
@@ -641,6 +658,7 @@ __global__ __launch_bounds__(256) void k_count_wave_multi(const uint32_t* __rest
   __shared__ uint2 sR[4][KMAX];
   __shared__ uint8_t sSel[4][KMAX];
   __shared__ uint32_t sInfo[4][4][MT];  // per task: outdeg prefix, row start, range prefix, range base
+  __shared__ unsigned long long sAcc[4][MT];  // per task: its count (batches: each task's graph gets its own)
   const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   uint32_t* T = sT[wv];
   uint2* R = sR[wv];
@@ -649,6 +667,7 @@ __global__ __launch_bounds__(256) void k_count_wave_multi(const uint32_t* __rest
   uint32_t* R0 = sInfo[wv][1];
   uint32_t* KP = sInfo[wv][2];
   uint32_t* RB = sInfo[wv][3];
+  unsigned long long* A = sAcc[wv];
   uint64_t acc = 0;
   uint32_t fallbacks = 0;
   TaskFeed feed(queue, ntasks, MT, wb / MT);
@@ -683,6 +702,7 @@ __global__ __launch_bounds__(256) void k_count_wave_multi(const uint32_t* __rest
         R0[lane] = r0;
         KP[lane] = lane < nt ? kp : 0xFFFFFFFFu;
         RB[lane] = tk.y;
+        A[lane] = 0ull;
       }
       wave_lds_sync();
       uint32_t sa = 0;
@@ -724,11 +744,23 @@ __global__ __launch_bounds__(256) void k_count_wave_multi(const uint32_t* __rest
           Sel[k] = (uint8_t)j;
         }
         wave_lds_sync();
-        cnt += ok ? stream_ranges<G, U, false, BUF>(src, R, nk, nullptr, lane, MultiCuckooProbe<LOGS>{T, sa}, Sel)
-                  : stream_ranges<G, U, false, BUF>(src, R, nk, nullptr, lane, LinearProbe<LOGS + 1>{T}, Sel);
+        if (gcount) {  // hits of each range go to its task's counter
+          if (ok)
+            stream_ranges<G, U, false, BUF, true>(src, R, nk, nullptr, lane, MultiCuckooProbe<LOGS>{T, sa}, Sel, A);
+          else
+            stream_ranges<G, U, false, BUF, true>(src, R, nk, nullptr, lane, LinearProbe<LOGS + 1>{T}, Sel, A);
+        } else {
+          cnt += ok ? stream_ranges<G, U, false, BUF>(src, R, nk, nullptr, lane, MultiCuckooProbe<LOGS>{T, sa}, Sel)
+                    : stream_ranges<G, U, false, BUF>(src, R, nk, nullptr, lane, LinearProbe<LOGS + 1>{T}, Sel);
+        }
         wave_lds_sync();
       }
       acc += cnt;
+      if (gcount && lane < nt) {  // the owner's graph
+        const unsigned long long v = A[lane];
+        acc += v;
+        if (v) atomicAdd(&gcount[gid[tk.x]], v);
+      }
       if (tdur && lane < nt) tdur[ntasks - 1 - (t0 + lane)] = (uint32_t)((wall_clock64() - c0t) / nt);
       wave_lds_sync();  // the next batch rewrites the task table and sets
     }
@@ -974,7 +1006,7 @@ struct Plan {
   int kind[kSlots];
   int g[kSlots];
   uint8_t table[kBuckets];  // bucket -> slot
-  bool menu = false;        // the learned schedule's menu (its launches may change kernel by task count)
+  bool menu = false;        // the learned schedule's menu
 };
 
 struct BucketStats {
@@ -1586,17 +1618,13 @@ static void launch_multi_t(const LaunchArgs& a, int slot, uint32_t ntasks, int c
   const unsigned grid = (unsigned)std::min<uint64_t>(want, (uint64_t)cus * per_cu);
   const uint32_t wb = claim_size(ntasks, 4ull * grid, kMultiTasks);
   k<<<grid, kBlock, 0, s>>>(g->row_ptr, g->col, g->m, rng, tasks, ntasks, a.queue + (size_t)slot * kQueues * kQStride, a.result, a.att & 0xFF,
-                            a.tdur ? a.tdur + (tasks - a.task0) : nullptr, wb);
+                            a.tdur ? a.tdur + (tasks - a.task0) : nullptr, wb, a.gid, a.gcount);
   *blocks += grid;
 }
 
 static void launch_multi(const LaunchArgs& a, int slot, int gw, uint32_t ntasks, int cus, hipStream_t s,
                          tribit_graph* g, const uint2* rng, const uint4* tasks, uint64_t* blocks) {
   if (!ntasks) return;
-  if (a.gcount) {  // per-graph counts: the one-task kernel attributes every task
-    launch_wave<kC0Logs, kC0K>(a, slot, gw, ntasks, cus, s, g, rng, tasks, blocks);
-    return;
-  }
   const bool buf = g->m * 4 <= kBufMax && !(a.att & kForceGlobal);
   switch (gw) {
     case 2:
@@ -1760,16 +1788,16 @@ void make_schedule(tribit_ctx* ctx, tribit_graph* g, int layout, int flags, bool
   S.tasks = DevBuf<uint4>(ctx, S.ntask ? S.ntask : 1);
   if (nb) k_sched_fill<<<(unsigned)nb, kSchedBlock, 0, s>>>(sv, pd, n, off.p, S.base, S.tasks.p);
   TB_HIP(hipGetLastError());
-  // menu plans: a one-task wave-64 launch with thousands of tasks per CU runs
-  // as multi-task waves (same tasks, eight per wave at once); with fewer, the
-  // one-task kernel was faster (R-MAT s25: 1.40 -> 1.03 ms at 2.1M tasks;
-  // s22, 0.39M tasks: 0.37 -> 0.57 ms)
+  // every plan but the textbook one: a one-task wave-64 launch with thousands
+  // of tasks per CU runs as multi-task waves (same tasks, eight per wave at
+  // once); with fewer, the one-task kernel was faster (R-MAT s25: 1.40 -> 1.03
+  // ms at 2.1M tasks; s22, 0.39M tasks: 0.37 -> 0.57 ms)
   const int cus = ctx->cus > 0 ? ctx->cus : 256;
-  if (plan.menu)
-    for (int i = 0; i < plan.nslots; ++i)
-      if (plan.kind[i] == kKindWave64 && S.tot.v[i] >= (uint64_t)kMultiMinTasksPerCU * cus &&
-          (plan.g[i] == 1 || plan.g[i] == 2 || plan.g[i] == 4))
-        plan.kind[i] = kKindWaveMulti;
+  const bool adapt = plan.menu || layout != TRIBIT_SCHED_PLAIN;
+  for (int i = 0; i < plan.nslots; ++i)
+    if (plan.kind[i] == kKindWave64 && (plan.g[i] == 1 || plan.g[i] == 2 || plan.g[i] == 4) &&
+        ((flags & TRIBIT_COUNT_FORCE_MULTI) || (adapt && S.tot.v[i] >= (uint64_t)kMultiMinTasksPerCU * cus)))
+      plan.kind[i] = kKindWaveMulti;
 }
 
 // The counting launches of a schedule (capturable: no allocation, no sync).
