@@ -259,6 +259,9 @@ def batch_workload(args) -> int:
         log("batch counts disagree")
         return 1
     m = inf["m"]
+    oracle_check = None
+    if args.cpu_full:
+        oracle_check = batch_oracle_check(counts, inf["graphs"])
     print(json.dumps({
         "metric": METRIC, "workload": f"{inf['graphs']} Chung-Lu graphs, batch seed 3 (gamma ~ U[2.1,2.8], n "
                                       f"log-uniform in [1024, 262144], avg degree ~ U[8,64])",
@@ -269,10 +272,36 @@ def batch_workload(args) -> int:
         "ms_per_replay_before_autotune": round(first["ms_count"], 4),
         "autotuned": {"group_width": after["group_width"], "waves_per_cu": after["waves"]},
         "ms_preprocess_device": round(inf["ms_preprocess"], 2),
+        "cpu_check": oracle_check,
     }), flush=True)
     b.free()
     ctx.close()
+    if oracle_check is not None and not oracle_check["count_match"]:
+        log(f"per-graph counts differ from the CPU oracle's on {oracle_check['mismatches']} graphs")
+        return 1
     return 0
+
+
+def batch_oracle_check(counts, ngraphs: int) -> dict:
+    """Every graph of the batch regenerated and counted by the CPU oracle (same generator streams)."""
+    import oracle
+
+    if not os.path.exists(oracle.LIB_PATH):
+        oracle.build()
+    threads = max(1, min(int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1), 16))
+    t0 = time.perf_counter()
+    bad = 0
+    total = 0
+    for i in range(ngraphs):
+        gamma, n, avg = oracle.chung_lu_batch_params(3, i)
+        keys = oracle.gen_chung_lu(n, gamma, avg, (3 << 32) + i)
+        c = oracle.count_keys(keys, n, threads)
+        total += c
+        bad += int(c != int(counts[i]))
+        if (i + 1) % 256 == 0:
+            log(f"oracle check: {i + 1}/{ngraphs} graphs, {bad} mismatches, {time.perf_counter() - t0:.0f} s")
+    return {"graphs": ngraphs, "mismatches": bad, "count_match": bad == 0, "oracle_triangles": total,
+            "seconds": round(time.perf_counter() - t0, 1), "cores": threads}
 
 
 def main() -> int:
@@ -288,7 +317,7 @@ def main() -> int:
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-schedule comparison run")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-full", action="store_true",
-                    help="run the CPU oracle over every row and compare its count with the GPU's")
+                    help="run the CPU oracle over every row (batch: every graph) and compare its counts with the GPU's")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (reference kernel for HBM byte counters)")
     ap.add_argument("--batch-graphs", type=int, default=4096, help="graphs of the batch workload")
