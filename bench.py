@@ -223,6 +223,7 @@ def mixed_family(args) -> int:
         if sel is None or plain is None or sel["triangles"] != plain["triangles"]:
             return 1
         m = inf["m"]
+        check = cpu_baseline(g, inf["n"], m, args.cpu_seconds, sel["triangles"], True) if args.cpu_full else None
         print(json.dumps({
             "metric": METRIC, "workload": desc, "n": inf["n"], "m": m, "wplus": inf["wplus"],
             "triangles": sel["triangles"], "sched": args.sched,
@@ -231,8 +232,12 @@ def mixed_family(args) -> int:
             "plain_gteps": round(m / (plain["ms_per_step"] * 1e-3) / 1e9, 4),
             "plain_ms_count_device": round(plain["ms_dev"], 4),
             "ms_preprocess_device": round(inf["ms_preprocess"], 3),
+            "cpu_check": check,
         }), flush=True)
         g.free()
+        if check is not None and not check.get("count_match"):
+            log(f"{name}: GPU count differs from the CPU oracle's")
+            return 1
     ctx.close()
     return 0
 
