@@ -322,7 +322,8 @@ def main() -> int:
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-schedule comparison run")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-full", action="store_true",
-                    help="run the CPU oracle over every row (batch: every graph) and compare its counts with the GPU's")
+                    help="run the CPU oracle over every row (mixed: every graph whole; batch: every graph) and "
+                         "compare its counts with the GPU's")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (reference kernel for HBM byte counters)")
     ap.add_argument("--batch-graphs", type=int, default=4096, help="graphs of the batch workload")
@@ -331,16 +332,16 @@ def main() -> int:
                          "(R-MAT s20, Erdos-Renyi 2^20 avg 32, 2-D RGG 2^21 avg 24, SBM 1000x512), one line each; "
                          "batch: 4,096 Chung-Lu graphs counted as one batch")
     args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU path is a design, not a build.",
+              file=sys.stderr)
+        return 2
     if args.copy_calibration:
         return copy_calibration()
     if args.workload == "mixed":
         return mixed_family(args)
     if args.workload == "batch":
         return batch_workload(args)
-    if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU path is a design, not a build.",
-              file=sys.stderr)
-        return 2
 
     import tribit_amd
 
