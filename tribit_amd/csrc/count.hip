@@ -2323,15 +2323,17 @@ void batch_autotune(tribit_ctx* ctx, tribit_batch* b, const tribit_count_opts& o
     for (auto& p : b->parts) nt += p.S->tot.v[i];
     if (!nt) continue;
     // widths the slot's kernels have: multi-task waves (a part's large wave-64
-    // launch) 1, 2 or 4 lanes per list; other waves up to 32; blocks 8 to 64
+    // launch) 1, 2 or 4 lanes per list; other waves 1 to 32; blocks 8 to 64
     bool multi = false;
     for (auto& p : b->parts) multi |= p.S->plan.kind[i] == kKindWaveMulti;
     const bool wave = pl.kind[i] == kKindWave64 || pl.kind[i] == kKindWave512;
-    const int gws[4] = {multi ? 1 : wave ? 4 : 8, multi ? 2 : wave ? 8 : 16, multi ? 4 : wave ? 16 : 32,
-                        multi ? 0 : wave ? 32 : 64};
+    static const int kMultiW[6] = {1, 2, 4, 0, 0, 0}, kWaveW[6] = {1, 2, 4, 8, 16, 32},
+                     kBlockW[6] = {8, 16, 32, 64, 0, 0};
+    const int* gws = multi ? kMultiW : wave ? kWaveW : kBlockW;
     const int g0 = b->gw[i] ? b->gw[i] : b->parts[0].S->gdef[i];
     int bestg = g0;
-    for (int gw : gws) {
+    for (int q = 0; q < 6; ++q) {
+      const int gw = gws[q];
       if (gw == 0 || gw == g0) continue;
       b->gw[i] = gw;
       const double ms = trial();
