@@ -1324,9 +1324,9 @@ static const uint2* sched_ranges(const tribit_graph* g, int sched) {
 
 // ------------------------------------------------------------------ plans
 // The learned scheduler's menu: (kind, lane-group width) per slot.
-static const int kMenuKind[kSlots] = {kKindWave64,  kKindWaveMulti, kKindWave512, kKindWave512,
-                                      kKindWave512, kKindBlock,     kKindBlock,   kKindTile};
-static const int kMenuG[kSlots] = {1, 1, 2, 4, 8, 8, 16, 8};
+static const int kMenuKind[kSlots] = {kKindWave64, kKindWaveMulti, kKindWave512, kKindWave512,
+                                      kKindBlock,  kKindBlock,     kKindBlock,   kKindTile};
+static const int kMenuG[kSlots] = {1, 1, 2, 4, 4, 8, 16, 8};
 
 static int kind_of_dbin(int db) {
   const uint32_t top = d_bin_top(db);
@@ -1605,6 +1605,7 @@ static void launch_wave(const LaunchArgs& a, int slot, int gw, uint32_t ntasks, 
 
 constexpr int kMultiTasks = 8;  // tasks per multi-task wave batch
 constexpr uint32_t kMultiMinTasksPerCU = 4096;  // menu wave-64 launches switch to multi-task waves above this
+constexpr uint32_t kBlockG4MaxTasksPerCU = 1536;  // menu 8-lane block launches use 4 lanes below this
 
 template <int G, bool BUF>
 static void launch_multi_t(const LaunchArgs& a, int slot, uint32_t ntasks, int cus, hipStream_t s, tribit_graph* g,
@@ -1798,6 +1799,13 @@ void make_schedule(tribit_ctx* ctx, tribit_graph* g, int layout, int flags, bool
     if (plan.kind[i] == kKindWave64 && (plan.g[i] == 1 || plan.g[i] == 2 || plan.g[i] == 4) &&
         ((flags & TRIBIT_COUNT_FORCE_MULTI) || (adapt && S.tot.v[i] >= (uint64_t)kMultiMinTasksPerCU * cus)))
       plan.kind[i] = kKindWaveMulti;
+  // menu plans: an 8-lane block launch of fewer than 1,536 tasks per CU
+  // streams 4 lanes per list (R-MAT s20-s24, 8K-283K tasks: 3-12% faster;
+  // s25, 540K tasks: 4% slower)
+  if (plan.menu)
+    for (int i = 0; i < plan.nslots; ++i)
+      if (plan.kind[i] == kKindBlock && plan.g[i] == 8 && S.tot.v[i] < (uint64_t)kBlockG4MaxTasksPerCU * cus)
+        plan.g[i] = 4;
 }
 
 // The counting launches of a schedule (capturable: no allocation, no sync).
