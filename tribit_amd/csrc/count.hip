@@ -1740,7 +1740,9 @@ void make_schedule(tribit_ctx* ctx, tribit_graph* g, int layout, int flags, bool
   if (learned) {
     // the model's choice is part of the scheduler's time
     BucketStats st;
-    bucket_stats(ctx, g, layout, &st, core_d0, core_cut);
+    // statistics of the whole lists, as in the timing profiles the model is
+    // trained on (counted without a dense core); the plan then runs with the core
+    bucket_stats(ctx, g, layout, &st);
     learned_table(ctx, st, layout, flags, plan.table);
   }
   // test hooks / opt-outs: every block launch streams tiles, or none does
