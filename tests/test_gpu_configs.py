@@ -1,0 +1,93 @@
+"""Every benchmark configuration at its full size on the GPU, compared with the
+CPU oracle's results committed in tests/golden/configs.json (written by
+tests/golden/make_configs.py; the GPU host does not run the oracle here).
+
+Single graphs: m, W+ and the exact triangle count of the learned schedule
+(with the committed cost model and its judge), of the wedge schedule with
+forced multi-task waves, and of the plain (textbook) schedule for the
+smaller ones; repeat determinism. The batch: all 4,096 Chung-Lu graphs of
+seed 3 in one captured replay, every per-graph count equal to the oracle's.
+"""
+import hashlib
+import json
+import os
+
+import numpy as np
+import pytest
+
+import tribit_amd
+from tests.conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+with open(os.path.join(GOLDEN, "configs.json")) as _f:
+    CONFIGS = json.load(_f)
+
+SINGLE = ["c1_rmat13", "c2_rmat22", "c4_rmat20", "c4_er", "c4_rgg", "c4_sbm", "c5_rmat25"]
+
+
+def note(msg):
+    print(f"[gpu-case] {msg}", flush=True)
+
+
+def build(ctx, spec):
+    g = spec["gen"]
+    if g == "rmat":
+        return ctx.gen_rmat(spec["scale"], spec["ef"], 0.57, 0.19, 0.19, spec["seed"])
+    if g == "er":
+        return ctx.gen_er(spec["n"], spec["samples"], spec["seed"])
+    if g == "rgg":
+        return ctx.gen_rgg(spec["n"], spec["avg_deg"], spec["seed"])
+    if g == "sbm":
+        return ctx.gen_sbm(spec["blocks"], spec["size"], spec["p_in"], spec["p_out"], spec["seed"])
+    raise ValueError(g)
+
+
+@pytest.fixture(scope="module")
+def model_ctx(gpu_ctx):
+    assert gpu_ctx.load_cost_model(), "tribit_amd/cost_model.f32 is missing"
+    yield gpu_ctx
+    gpu_ctx.set_cost_model(None)
+
+
+@pytest.mark.parametrize("name", SINGLE)
+def test_config_exact(model_ctx, name):
+    want = CONFIGS[name]
+    note(f"{name}: {want['workload']}")
+    with build(model_ctx, want["spec"]) as g:
+        inf = g.info()
+        assert (inf["n"], inf["entries"], inf["m"], inf["wplus"]) == \
+            (want["n"], want["entries"], want["m"], want["wplus"])
+        learned = g.count(sched="learned")
+        assert learned.triangles == want["triangles"], (learned.triangles, want["triangles"])
+        again = g.count(sched="learned", repeats=2)
+        assert again.repeats_agree and again.triangles == want["triangles"]
+        multi = g.count(sched="wedge", flags=tribit_amd.FORCE_MULTI)
+        assert multi.triangles == want["triangles"]
+        if want["m"] <= 70_000_000:
+            assert g.count(sched="plain").triangles == want["triangles"]
+        note(f"{name}: {learned.triangles} triangles; learned {again.ms_count:.2f} ms "
+             f"(candidate {learned.candidate}, judged in {learned.ms_judge:.1f} ms)")
+
+
+def test_config_batch_4096(gpu_ctx):
+    """Configuration 3: 4,096 Chung-Lu graphs (seed 3) in one batch, counted by one captured replay,
+    then after the autotuner; every per-graph count equals the oracle's."""
+    want = CONFIGS["c3_batch"]
+    note(f"c3 batch: {want['workload']}")
+    b = gpu_ctx.batch_gen_chung_lu(3, 0, want["graphs"])
+    try:
+        inf = b.info()
+        n, m = b.sizes()
+        assert inf["graphs"] == want["graphs"] and inf["parts"] >= 2
+        assert [int(x) for x in n] == want["n"] and [int(x) for x in m] == want["m"]
+        counts, ci = b.count(sched="wedge", repeats=2)
+        counts = np.asarray(counts, dtype=np.uint64)
+        assert ci["repeats_agree"]
+        bad = np.nonzero(counts != np.asarray(want["counts"], dtype=np.uint64))[0]
+        assert bad.size == 0, f"{bad.size} graphs differ, first {bad[:8].tolist()}"
+        assert hashlib.sha256(counts.tobytes()).hexdigest() == want["counts_sha256"]
+        assert int(counts.sum()) == want["triangles"]
+        note(f"c3: {inf['parts']} parts, {ci['launches']} launches, {ci['ms_count']:.1f} ms per replay")
+    finally:
+        b.free()

@@ -52,6 +52,58 @@ def test_complete_graph(oracle_mod, n):
     assert oracle_mod.ref_pipeline(oracle_mod.dedup(keys), n)["triangles"] == t
 
 
+@pytest.mark.parametrize("n", [3, 4, 5, 10, 64, 300])
+def test_wplus_complete_graph(oracle_mod, n):
+    """W+ closed form: every vertex of K_n has degree n-1, so ties go to the lower id, vertex j has
+    out-degree n-1-j and j in-arcs, and W+ = sum_j j (n-1-j) = C(n,3)."""
+    keys, _ = graphs.complete(n)
+    assert oracle_mod.Graph.from_keys(keys, n).wplus() == comb(n, 3)
+
+
+@pytest.mark.parametrize("k", [1, 5, 40])
+def test_wplus_stars_and_windmills(oracle_mod, k):
+    """Closed forms: a star K_{1,k} orients every leaf to the hub, whose out-degree is 0: W+ = 0. In the
+    windmill F_k each blade pair (a<b, degree 2) has a->b, a->hub, b->hub and the hub (degree 2k) has
+    out-degree 0 for k>1: W+ = sum over arcs a->b of outdeg(b) = k * 1."""
+    star = graphs.pack(np.zeros(k, np.int64), np.arange(1, k + 1))
+    assert oracle_mod.Graph.from_keys(star, k + 1).wplus() == 0
+    if k > 1:
+        keys, _ = graphs.windmill(k)
+        assert oracle_mod.Graph.from_keys(keys, 2 * k + 1).wplus() == k
+
+
+def numpy_wplus(ukeys: np.ndarray, n: int) -> int:
+    """W+ computed independently of the oracle: degrees by bincount, each edge oriented from the lower
+    (degree, id) rank to the higher, out-degrees by bincount over arc tails, summed over arc heads."""
+    a = (ukeys >> np.uint64(32)).astype(np.int64)
+    b = (ukeys & np.uint64(0xFFFFFFFF)).astype(np.int64)
+    deg = np.bincount(np.concatenate([a, b]), minlength=n)
+    a_first = (deg[a] < deg[b]) | ((deg[a] == deg[b]) & (a < b))
+    tail = np.where(a_first, a, b)
+    head = np.where(a_first, b, a)
+    outdeg = np.bincount(tail, minlength=n)
+    return int(outdeg[head].sum())
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_wplus_numpy_recompute(oracle_mod, seed):
+    """or_wplus against the numpy recomputation on R-MAT, Chung-Lu and Erdos-Renyi graphs."""
+    if seed % 3 == 0:
+        keys, n = oracle_mod.gen_rmat(12, 8, seed=seed + 1), 1 << 12
+    elif seed % 3 == 1:
+        keys, n = oracle_mod.gen_chung_lu(5000, 2.3, 12.0, seed), 5000
+    else:
+        keys, n = oracle_mod.gen_er(3000, 20000, seed), 3000
+    u = oracle_mod.dedup(keys)
+    assert oracle_mod.Graph(u, n).wplus() == numpy_wplus(u, n)
+
+
+def test_wplus_numpy_wiki_vote(oracle_mod):
+    st, keys, n, _ = oracle_mod.read_mtx(WIKI)
+    u = oracle_mod.dedup(keys)
+    assert oracle_mod.Graph(u, n).wplus() == numpy_wplus(u, n)
+
+
 @pytest.mark.parametrize("k", [3, 4, 5, 17, 100])
 def test_wheel(oracle_mod, k):
     keys, t = graphs.wheel(k)
