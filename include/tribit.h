@@ -59,10 +59,13 @@ typedef struct {
   uint64_t rows_active;    /* rows with outdeg >= 2 */
   uint32_t max_outdeg;     /* max oriented out-degree */
   uint32_t reserved0;
-  uint64_t tiles32;        /* 32-column bit tiles (0 if not built) */
+  uint64_t tiles32;        /* 32-column bit tiles of the oriented rows */
   uint64_t tiles64;        /* 64-column bit tiles of the oriented rows */
-  double ms_preprocess;    /* device time: keys in HBM -> count-ready (hipEvents) */
+  double ms_preprocess;    /* device time: keys in HBM -> count-ready (hipEvents): CSR, W+ and the default
+                              range layout (wedge ranges of relabelled graphs, else the directed layout) */
   uint64_t device_bytes;   /* bytes held by the resident structures */
+  double ms_layout[4];     /* device time of each other range layout, indexed by tribit_sched (PLAIN,
+                              DIRECTED), built on its first count outside that count's timing; 0: not built */
 } tribit_graph_info;
 
 /* Scheduler choice for tribit_count. */
@@ -70,7 +73,9 @@ typedef enum {
   TRIBIT_SCHED_PLAIN = 0,    /* static degree thresholds; textbook direction: N+(u) is the set, N+(v) streamed */
   TRIBIT_SCHED_LEARNED = 1,  /* per-bucket kernel variant from the cost model, dense core by cost estimate;
                                 the first count of a graph times the model's plan against the heuristic plans
-                                and keeps the fastest (see tribit_count_info.candidate) */
+                                and keeps the fastest (see tribit_count_info.candidate); the context keeps
+                                the choice per input shape (n, m, W+), so a graph rebuilt from the same
+                                input runs it without judging again */
   TRIBIT_SCHED_DIRECTED = 2, /* static thresholds; every edge streams the shorter of its two lists */
   TRIBIT_SCHED_WEDGE = 3     /* relabelled graphs: every arc u->v streams the part of N+(u) after v
                                 against N+(v) (each out-out wedge of u checked once) */
@@ -88,6 +93,8 @@ typedef enum {
                                              from the 16-bit low halves, two ids per load) */
 #define TRIBIT_COUNT_FORCE_MULTI 128      /* test hook: every launch of the 64-slot wave kernel runs as multi-task
                                              waves, whatever its task count */
+#define TRIBIT_COUNT_TILES32 256          /* launches that stream bit tiles stream 32-column tiles (uint32
+                                             masks) instead of 64-column ones */
 #define TRIBIT_COUNT_NO_JUDGE 64          /* LEARNED: run the density-threshold heuristic's candidate (preferred
                                              layout + dense core) without timing the candidates first; for
                                              graphs counted once, where judging would cost several counts */
@@ -186,6 +193,11 @@ int tribit_gen_rmat(tribit_ctx* ctx, uint32_t scale, uint32_t ef, double a, doub
                     uint64_t** d_keys, uint64_t* nkeys);
 /* Erdos-Renyi G(n, M): M samples of uniform vertex pairs. */
 int tribit_gen_er(tribit_ctx* ctx, uint64_t n, uint64_t nsamples, uint64_t seed, uint64_t** d_keys, uint64_t* nkeys);
+/* Copies generated keys back to a host buffer of nkeys (e.g. to partition
+ * them by the reference's multi-GPU layout, main.rs:150-161), and releases a
+ * generated key buffer that is not handed to tribit_graph_from_device_keys. */
+int tribit_device_keys_to_host(tribit_ctx* ctx, const uint64_t* d_keys, uint64_t nkeys, uint64_t* host);
+int tribit_free_device_keys(tribit_ctx* ctx, uint64_t* d_keys);
 
 /* Chung-Lu by edge sampling: weights w_i = (i+1)^(-1/(gamma-1)), round(n*avg/2)
  * samples with both endpoints drawn from the weight distribution. */
@@ -218,6 +230,14 @@ int tribit_graph_download(const tribit_graph* g, uint32_t* row_ptr, uint32_t* co
  * graph was relabelled (TRIBIT_OPT_RELABEL). Without relabelling the ids are
  * the input ids. */
 int tribit_graph_labels(const tribit_graph* g, uint32_t* input_id, int* relabeled);
+/* Bit tiles of width 32 or 64 of the oriented rows (built on first use; the
+ * counting kernels stream them in tile launches): *ntiles = their number;
+ * with non-NULL buffers also copies tile_ptr[n+1] (each row's first tile),
+ * blk[ntiles] (column / width) and mask[ntiles] (uint32 for width 32, uint64
+ * for 64; bit column % width). Replaces the reference's 32-column U/L words
+ * (tribit.cu:588-602, 672-740) for the rows whose tiles are dense. */
+int tribit_graph_tiles(tribit_ctx* ctx, tribit_graph* g, int width, uint64_t* ntiles, uint32_t* tile_ptr,
+                       uint32_t* blk, void* mask);
 int tribit_graph_free(tribit_graph* g);
 
 /* ---- counting (tribit.cu:765-802) ------------------------------------------ */
@@ -268,7 +288,10 @@ int tribit_set_cost_model(tribit_ctx* ctx, tribit_mlp* m);
 int tribit_learned_plan(tribit_ctx* ctx, tribit_graph* g, uint8_t* table);
 
 /* One-shot path with the shape of count_triangles (tribit.cu:656): host keys
- * in, exact count out; preprocessing and counting device times reported. */
+ * in, exact count out; preprocessing and counting device times reported.
+ * LEARNED does not judge here (the graph is counted once): it runs the choice
+ * this context judged for the same input before (tribit_count on a graph of
+ * equal n, m and W+), else the heuristic candidate (TRIBIT_COUNT_NO_JUDGE). */
 int tribit_count_keys(tribit_ctx* ctx, const uint64_t* keys, uint64_t nkeys, uint64_t n,
                       const tribit_count_opts* opts, uint64_t* triangles, double* ms_preprocess,
                       tribit_count_info* info);
@@ -290,6 +313,7 @@ typedef struct {
   int32_t nslots;
   int32_t group_width[8];  /* lanes per streamed list per slot */
   int32_t waves[8];        /* workgroups per CU per slot (0: occupancy limit) */
+  int32_t tile_width;      /* 0: the plans' kernels; 32 or 64: block launches stream bit tiles of that width */
 } tribit_batch_count_info;
 /* Graph j has keys[key_offsets[j] .. key_offsets[j+1]) ((min<<32)|max of local
  * ids in [0, nverts[j]); duplicates and UINT64_MAX sentinels allowed). */
@@ -309,8 +333,10 @@ int tribit_batch_graph_sizes(const tribit_batch* b, uint64_t* n, uint64_t* m);
 int tribit_batch_count(tribit_ctx* ctx, tribit_batch* b, const tribit_count_opts* opts, uint64_t* counts,
                        tribit_batch_count_info* info);
 /* Autotuner: coordinate search over lane-group width and workgroups per CU
- * of every launch slot, timing replays of the captured graph; the best
- * configuration stays with the batch for later counts and is returned. */
+ * of every launch slot, then the tile width (block launches streaming ids, or
+ * 32- or 64-column bit tiles), timing replays of the captured graph; every
+ * configuration must give the same per-graph counts; the best stays with the
+ * batch for later counts and is returned (tile_width, group_width, waves). */
 int tribit_batch_autotune(tribit_ctx* ctx, tribit_batch* b, const tribit_count_opts* opts,
                           tribit_batch_count_info* best);
 int tribit_batch_free(tribit_batch* b);
@@ -323,9 +349,11 @@ int tribit_batch_free(tribit_batch* b);
  * larger id is in partition p; returns the triangles whose two larger
  * vertices are in p (the triangles of those edges).
  * Off-diagonal task (main6_multi.cu:595-751): cols hold the edges whose larger
- * id is in partition pc, idxs those whose larger id is in later partitions S;
- * returns the triangles a < b < c with b in pc and c in S (computed as
- * triangles(cols + idxs) - triangles(cols) - triangles(idxs)). */
+ * id is in partition pc, idxs those whose larger id is in later partitions S
+ * (and smaller id in partitions up to pc); returns the triangles a < b < c
+ * with b in pc and c in S, i.e. those with (a, b) in cols and (a, c), (b, c)
+ * in idxs: every cols edge streams one endpoint's idxs neighbours against
+ * the other's (one count, no union). */
 int tribit_count_partial_diagonal(tribit_ctx* ctx, const uint64_t* const* chunks, const uint64_t* nnz_ptr,
                                   uint32_t num_ptrs, uint64_t num_nodes, uint64_t* triangles);
 int tribit_count_partial_off_diagonal(tribit_ctx* ctx, const uint64_t* const* cols, const uint64_t* nnz_cols_ptr,
@@ -343,6 +371,20 @@ uint64_t count_partial_triangles_off_diagonal(uint64_t** cols, uint64_t* nnz_col
                                               uint64_t** idxs, uint64_t* nnz_idxs_ptr, uint32_t num_ptrs_idxs,
                                               uint64_t num_nodes);
 int tribit_partial_last_status(void);
+/* Sizes and device times of the calling thread's last successful partial
+ * count (either entry point, either naming). No reference counterpart: the
+ * reference times tasks on the host only (main.rs:357-361). */
+typedef struct {
+  uint64_t keys;       /* input keys of the task (cols + idxs for an off-diagonal task) */
+  uint64_t edges;      /* distinct edges it counts over (off-diagonal: idxs edges + cols edges) */
+  uint64_t lists;      /* streamed lists (diagonal: oriented edges; off-diagonal: distinct cols edges) */
+  uint64_t streamed;   /* streamed elements, i.e. membership checks (diagonal: W+) */
+  uint64_t triangles;
+  double ms_h2d;       /* host -> device copies of the chunks (wall clock) */
+  double ms_build;     /* device: keys in HBM -> count-ready structures */
+  double ms_count;     /* device: scheduler + counting kernels */
+} tribit_partial_info;
+int tribit_partial_last_info(tribit_partial_info* out);
 
 /* ---- cost model of the learned scheduler (no reference counterpart) -------
  * A 3-layer perceptron, F inputs -> 64 -> 64 -> 1 (ReLU), trained in FP32 by

@@ -47,19 +47,27 @@ def test_learned_exact_and_judged(model_ctx, oracle_mod, spec):
         assert again.triangles == want and again.candidate == first.candidate and again.ms_judge == 0
 
 
-def test_learned_without_judge(model_ctx, oracle_mod):
-    """NO_JUDGE runs the heuristic candidate at once (the partial-count entry points use it for graphs
-    counted once); a later plain LEARNED count of the same graph still judges."""
+def test_learned_without_judge(oracle_mod):
+    """NO_JUDGE runs the heuristic candidate at once (graphs counted once); a later plain LEARNED count
+    of the same graph still judges, and a graph rebuilt from the same input in that context runs the
+    judged choice without judging again (the context keeps choices per input shape). A fresh context:
+    the module's shared one has judged these inputs already."""
     spec = HELD_OUT[0]
     note(f"learned without judge {spec}")
     r = {"spec": spec, "n": n_of(spec)}
     want = oracle_mod.count_keys(gens.oracle_keys(oracle_mod, r), r["n"])
-    with gens.gpu_graph(model_ctx, r) as g:
-        quick = g.count(sched="learned", flags=tribit_amd.NO_JUDGE)
-        assert quick.triangles == want
-        assert quick.candidate == 1 and quick.ms_judge == 0 and not any(quick.ms_candidates)
-        judged = g.count(sched="learned")
-        assert judged.triangles == want and judged.ms_judge > 0
+    with tribit_amd.Context(0) as ctx:
+        assert ctx.load_cost_model()
+        with gens.gpu_graph(ctx, r) as g:
+            quick = g.count(sched="learned", flags=tribit_amd.NO_JUDGE)
+            assert quick.triangles == want
+            assert quick.candidate == 1 and quick.ms_judge == 0 and not any(quick.ms_candidates)
+            judged = g.count(sched="learned")
+            assert judged.triangles == want and judged.ms_judge > 0
+        with gens.gpu_graph(ctx, r) as g2:
+            again = g2.count(sched="learned")
+            assert again.triangles == want and again.ms_judge == 0 and again.candidate == judged.candidate
+            assert list(again.ms_candidates) == list(judged.ms_candidates)
 
 
 def test_trained_model_gain(model_ctx):

@@ -101,3 +101,61 @@ def test_partial_bad_arguments(gpu_ctx):
     with pytest.raises(tribit_amd.TribitError) as ei:
         tribit_amd.count_partial_diagonal(gpu_ctx, bad, 5)
     assert ei.value.status == 4
+
+
+def test_decomposition_rmat20_config(gpu_ctx):
+    """The whole decomposition at a configuration's size: R-MAT s20 ef16 seed 7 (the mixed family's
+    R-MAT member) in P = 16 partitions; every diagonal task and every off-diagonal group (nnz threshold
+    2M keys, so columns split into several groups as main.rs:186-220 splits at 1e9) through the
+    reference-named symbols. The partial counts sum to the oracle's whole-graph count committed in
+    tests/golden/configs.json."""
+    import json
+    import os
+
+    from tests.conftest import GOLDEN
+
+    with open(os.path.join(GOLDEN, "configs.json")) as f:
+        want = json.load(f)["c4_rmat20"]["triangles"]
+    note("decomposition R-MAT s20 ef16 seed 7, P=16")
+    lib = gpu_ctx.lib
+    lib.init_gpu_kernel5(0)
+    n = 1 << 20
+    keys = gpu_ctx.gen_rmat_keys(20, 16, seed=7)
+    keys = keys[keys != np.uint64(0xFFFFFFFFFFFFFFFF)]
+    P, threshold = 16, 2_000_000
+    bs = -(-(n // P) // 32) * 32
+    pmax = ((keys & np.uint64(0xFFFFFFFF)) // np.uint64(bs)).astype(np.int64)
+    pmin = ((keys >> np.uint64(32)) // np.uint64(bs)).astype(np.int64)
+    nparts = -(-n // bs)
+    total, groups = 0, 0
+    for pc in range(nparts):
+        cols = [keys[(pmax == pc) & (pmin == j)] for j in range(pc + 1)]
+        c = tribit_amd.Chunks(cols)
+        t = lib.count_partial_triangles_diagonal(ctypes.addressof(c.ptrs), c.cum.ctypes.data, c.num, n)
+        assert lib.tribit_partial_last_status() == 0
+        total += t
+        below = pmin <= pc
+        nnz = np.bincount(pmax[below], minlength=nparts)
+        group, acc = [], 0
+        for s in list(range(pc + 1, nparts)) + [None]:
+            if s is not None:
+                acc += int(nnz[s])
+            if s is None or (acc >= threshold and group):
+                if group:
+                    idxs = [keys[(pmax == q) & below & (pmin == j)] for q in group for j in range(pc + 1)]
+                    x = tribit_amd.Chunks(idxs)
+                    o = lib.count_partial_triangles_off_diagonal(ctypes.addressof(c.ptrs), c.cum.ctypes.data, c.num,
+                                                                 ctypes.addressof(x.ptrs), x.cum.ctypes.data, x.num,
+                                                                 n)
+                    assert lib.tribit_partial_last_status() == 0
+                    info = tribit_amd.partial_last_info(gpu_ctx)
+                    assert info["triangles"] == o and info["lists"] > 0
+                    total += o
+                    groups += 1
+                if s is None:
+                    break
+                group, acc = [s], int(nnz[s])
+            else:
+                group.append(s)
+    assert groups > nparts  # some columns split into several groups
+    assert total == want, (total, want)

@@ -44,6 +44,7 @@ class GraphInfo(ctypes.Structure):
         ("wplus", ctypes.c_uint64), ("rows_active", ctypes.c_uint64), ("max_outdeg", ctypes.c_uint32),
         ("reserved0", ctypes.c_uint32), ("tiles32", ctypes.c_uint64), ("tiles64", ctypes.c_uint64),
         ("ms_preprocess", ctypes.c_double), ("device_bytes", ctypes.c_uint64),
+        ("ms_layout", ctypes.c_double * 4),
     ]
 
 
@@ -56,6 +57,7 @@ FORCE_LINEAR_PROBE = 1
 FORCE_GLOBAL_LOADS = 2
 NO_TILES = 4
 FORCE_TILES = 8
+TILES32 = 256
 NO_BITMAP = 16
 WIDE_IDS = 32
 NO_JUDGE = 64
@@ -84,7 +86,7 @@ class BatchCountInfo(ctypes.Structure):
         ("ms_count", ctypes.c_double), ("ms_sched", ctypes.c_double), ("ms_capture", ctypes.c_double),
         ("triangles", ctypes.c_uint64), ("tasks", ctypes.c_uint64), ("parts", ctypes.c_uint32),
         ("launches", ctypes.c_uint32), ("repeats_agree", ctypes.c_int32), ("nslots", ctypes.c_int32),
-        ("group_width", ctypes.c_int32 * 8), ("waves", ctypes.c_int32 * 8),
+        ("group_width", ctypes.c_int32 * 8), ("waves", ctypes.c_int32 * 8), ("tile_width", ctypes.c_int32),
     ]
 
 
@@ -108,6 +110,12 @@ class VariantStats(ctypes.Structure):
     _fields_ = [("rows", ctypes.c_uint64 * 8), ("edges", ctypes.c_uint64 * 8), ("wplus", ctypes.c_uint64 * 8),
                 ("narrow", ctypes.c_uint64 * 8), ("narrow_ranges", ctypes.c_uint64 * 8),
                 ("narrow2_ranges", ctypes.c_uint64 * 8)]
+
+
+class PartialInfo(ctypes.Structure):
+    _fields_ = [("keys", ctypes.c_uint64), ("edges", ctypes.c_uint64), ("lists", ctypes.c_uint64),
+                ("streamed", ctypes.c_uint64), ("triangles", ctypes.c_uint64), ("ms_h2d", ctypes.c_double),
+                ("ms_build", ctypes.c_double), ("ms_count", ctypes.c_double)]
 
 
 _lib = None
@@ -148,6 +156,8 @@ def load_library(path: str | None = None) -> ctypes.CDLL:
         "tribit_gen_rmat": (ctypes.c_int, [_P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_double, ctypes.c_double,
                                            ctypes.c_double, _U64, ctypes.POINTER(_P), _U64P]),
         "tribit_gen_er": (ctypes.c_int, [_P, _U64, _U64, _U64, ctypes.POINTER(_P), _U64P]),
+        "tribit_device_keys_to_host": (ctypes.c_int, [_P, _P, _U64, _P]),
+        "tribit_free_device_keys": (ctypes.c_int, [_P, _P]),
         "tribit_gen_chung_lu": (ctypes.c_int, [_P, _U64, ctypes.c_double, ctypes.c_double, _U64, ctypes.POINTER(_P),
                                                _U64P]),
         "tribit_chung_lu_batch_params": (None, [_U64, _U64, ctypes.POINTER(ctypes.c_double), _U64P,
@@ -160,6 +170,7 @@ def load_library(path: str | None = None) -> ctypes.CDLL:
         "tribit_graph_get_info": (ctypes.c_int, [_P, ctypes.POINTER(GraphInfo)]),
         "tribit_graph_download": (ctypes.c_int, [_P, _P, _P, _P]),
         "tribit_graph_labels": (ctypes.c_int, [_P, _P, ctypes.POINTER(ctypes.c_int)]),
+        "tribit_graph_tiles": (ctypes.c_int, [_P, _P, ctypes.c_int, _U64P, _P, _P, _P]),
         "tribit_set_option": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int64]),
         "tribit_graph_free": (ctypes.c_int, [_P]),
         "tribit_count": (ctypes.c_int, [_P, _P, ctypes.POINTER(CountOpts), ctypes.POINTER(CountInfo)]),
@@ -188,6 +199,7 @@ def load_library(path: str | None = None) -> ctypes.CDLL:
         "count_partial_triangles_diagonal": (_U64, [_P, _P, ctypes.c_uint32, _U64]),
         "count_partial_triangles_off_diagonal": (_U64, [_P, _P, ctypes.c_uint32, _P, _P, ctypes.c_uint32, _U64]),
         "tribit_partial_last_status": (ctypes.c_int, []),
+        "tribit_partial_last_info": (ctypes.c_int, [ctypes.POINTER(PartialInfo)]),
         "tribit_mlp_create": (ctypes.c_int, [_P, ctypes.c_int, _U64, ctypes.POINTER(_P)]),
         "tribit_mlp_free": (ctypes.c_int, [_P]),
         "tribit_mlp_num_params": (_U64, [_P]),
@@ -248,6 +260,13 @@ def count_partial_off_diagonal(ctx: "Context", cols, idxs, num_nodes: int) -> in
                                                      ctypes.addressof(x.ptrs), x.cum.ctypes.data, x.num, num_nodes,
                                                      ctypes.byref(out)), "count_partial_off_diagonal")
     return out.value
+
+
+def partial_last_info(ctx: "Context") -> dict:
+    """Sizes and device times of this thread's last partial count (tribit_partial_last_info)."""
+    pi = PartialInfo()
+    _check(ctx.lib.tribit_partial_last_info(ctypes.byref(pi)), "partial_last_info")
+    return {f: getattr(pi, f) for f, _ in PartialInfo._fields_}
 
 
 def profile_features(p: Profile, bucket: int, slot: int) -> np.ndarray:
@@ -460,7 +479,9 @@ class Graph:
     def info(self) -> dict:
         gi = GraphInfo()
         _check(self.ctx.lib.tribit_graph_get_info(self.h, ctypes.byref(gi)), "graph_get_info")
-        return {f: getattr(gi, f) for f, _ in GraphInfo._fields_ if not f.startswith("reserved")}
+        d = {f: getattr(gi, f) for f, _ in GraphInfo._fields_ if not f.startswith("reserved")}
+        d["ms_layout"] = list(gi.ms_layout)
+        return d
 
     def download(self):
         """(row_ptr uint32[n+1], col uint32[m], deg uint32[n]) of the oriented CSR."""
@@ -489,6 +510,19 @@ class Graph:
         t = np.zeros(BUCKETS, np.uint8)
         _check(self.ctx.lib.tribit_learned_plan(self.ctx.h, self.h, t.ctypes.data), "learned_plan")
         return t
+
+    def tiles(self, width: int = 64):
+        """(tile_ptr uint32[n+1], blk uint32[T], mask uint32|uint64[T]) of the `width`-column bit tiles."""
+        nt = _U64()
+        lib = self.ctx.lib
+        _check(lib.tribit_graph_tiles(self.ctx.h, self.h, width, ctypes.byref(nt), None, None, None), "graph_tiles")
+        n = self.info()["n"]
+        tp = np.zeros(n + 1, np.uint32)
+        blk = np.zeros(max(nt.value, 1), np.uint32)
+        mask = np.zeros(max(nt.value, 1), np.uint32 if width == 32 else np.uint64)
+        _check(lib.tribit_graph_tiles(self.ctx.h, self.h, width, ctypes.byref(nt), tp.ctypes.data, blk.ctypes.data,
+                                      mask.ctypes.data), "graph_tiles")
+        return tp, blk[:nt.value], mask[:nt.value]
 
     def labels(self):
         """(input id of every internal id uint32[n], relabelled?)."""
@@ -643,6 +677,20 @@ class Context:
         _check(self.lib.tribit_graph_from_device_keys(self.h, d, nk.value, 1 << scale, ctypes.byref(g)),
                "graph_from_device_keys")
         return Graph(self, g.value)
+
+    def gen_rmat_keys(self, scale: int, ef: int, a=0.57, b=0.19, c=0.19, seed: int = 1) -> np.ndarray:
+        """The R-MAT generator's input keys ((min << 32) | max, self-loops as the all-ones sentinel),
+        generated on the GPU and copied to a host array."""
+        d = _P()
+        nk = _U64()
+        _check(self.lib.tribit_gen_rmat(self.h, scale, ef, a, b, c, seed, ctypes.byref(d), ctypes.byref(nk)),
+               "gen_rmat")
+        out = np.empty(nk.value, np.uint64)
+        try:
+            _check(self.lib.tribit_device_keys_to_host(self.h, d, nk.value, out.ctypes.data), "device_keys_to_host")
+        finally:
+            self.lib.tribit_free_device_keys(self.h, d)
+        return out
 
     def gen_er(self, n: int, nsamples: int, seed: int) -> Graph:
         d = _P()

@@ -1,6 +1,7 @@
 // api.hip -- extern "C" entry points of libtribit_hip.so (declared in
 // include/tribit.h). Every entry point catches internal errors and returns a
 // tribit_status; nothing throws across the ABI and nothing exits the process.
+#include <chrono>
 #include <cstdlib>
 #include <cstring>
 #include <vector>
@@ -193,6 +194,24 @@ int tribit_gen_rmat(tribit_ctx* ctx, uint32_t scale, uint32_t ef, double a, doub
   });
 }
 
+int tribit_device_keys_to_host(tribit_ctx* ctx, const uint64_t* d_keys, uint64_t nkeys, uint64_t* host) {
+  if (!ctx || (nkeys && (!d_keys || !host))) return TRIBIT_ERR_ARG;
+  return guarded(ctx, [&] {
+    TB_HIP(hipSetDevice(ctx->device));
+    if (nkeys) TB_HIP(hipMemcpyAsync(host, d_keys, 8 * nkeys, hipMemcpyDeviceToHost, ctx->stream));
+    ctx->sync();
+  });
+}
+
+int tribit_free_device_keys(tribit_ctx* ctx, uint64_t* d_keys) {
+  if (!ctx) return TRIBIT_ERR_ARG;
+  return guarded(ctx, [&] {
+    TB_HIP(hipSetDevice(ctx->device));
+    ctx->free(d_keys);
+    ctx->sync();
+  });
+}
+
 int tribit_gen_er(tribit_ctx* ctx, uint64_t n, uint64_t nsamples, uint64_t seed, uint64_t** d_keys,
                   uint64_t* nkeys) {
   if (!ctx || !d_keys || !nkeys || n < 1 || n > 0xFFFFFFFEull) return TRIBIT_ERR_ARG;
@@ -317,9 +336,43 @@ int tribit_graph_get_info(const tribit_graph* g, tribit_graph_info* info) {
   info->rows_active = g->rows_active;
   info->max_outdeg = g->max_outdeg;
   info->ms_preprocess = g->ms_prep;
+  for (int l = 0; l < 4; ++l) info->ms_layout[l] = g->ms_layout[l];
+  if (!g->tiles_counted && g->ctx) {  // tile counts on first request (untimed; not part of preprocessing)
+    tribit_graph* gm = const_cast<tribit_graph*>(g);
+    const int st = guarded(g->ctx, [&] {
+      TB_HIP(hipSetDevice(g->ctx->device));
+      gm->tiles32 = gm->m ? tb::count_tiles(g->ctx, gm, 32) : 0;
+      gm->tiles64 = gm->m ? tb::count_tiles(g->ctx, gm, 64) : 0;
+      gm->tiles_counted = true;
+    });
+    if (st != TRIBIT_OK) return st;
+  }
   info->device_bytes = g->bytes;
+  info->tiles32 = g->tiles32;
   info->tiles64 = g->tiles64;
   return TRIBIT_OK;
+}
+
+int tribit_graph_tiles(tribit_ctx* ctx, tribit_graph* g, int width, uint64_t* ntiles, uint32_t* tile_ptr,
+                       uint32_t* blk, void* mask) {
+  if (!ctx || !g || g->ctx != ctx || !ntiles || (width != 32 && width != 64)) return TRIBIT_ERR_ARG;
+  return guarded(ctx, [&] {
+    TB_HIP(hipSetDevice(ctx->device));
+    const int layout = g->rng_sfx ? TRIBIT_SCHED_WEDGE : TRIBIT_SCHED_DIRECTED;
+    if (g->m) tb::build_tiles(ctx, g, layout, width);
+    const tribit_graph::TileSet& T = g->tiles[width == 32 ? 0 : 1];
+    *ntiles = T.ntiles;
+    if (tile_ptr) {
+      if (T.ptr)
+        TB_HIP(hipMemcpyAsync(tile_ptr, T.ptr, 4 * (g->n + 1), hipMemcpyDeviceToHost, ctx->stream));
+      else
+        memset(tile_ptr, 0, 4 * (g->n + 1));
+    }
+    if (blk && T.ntiles) TB_HIP(hipMemcpyAsync(blk, T.blk, 4 * T.ntiles, hipMemcpyDeviceToHost, ctx->stream));
+    if (mask && T.ntiles)
+      TB_HIP(hipMemcpyAsync(mask, T.mask, (width / 8) * T.ntiles, hipMemcpyDeviceToHost, ctx->stream));
+    ctx->sync();
+  });
 }
 
 int tribit_graph_download(const tribit_graph* g, uint32_t* row_ptr, uint32_t* col, uint32_t* deg) {
@@ -408,7 +461,15 @@ int tribit_count_keys(tribit_ctx* ctx, const uint64_t* keys, uint64_t nkeys, uin
   if (st != TRIBIT_OK) return st;
   tribit_count_info local;
   memset(&local, 0, sizeof(local));
-  st = tribit_count(ctx, g, opts, &local);
+  tribit_count_opts o;
+  memset(&o, 0, sizeof(o));
+  if (opts) o = *opts;
+  // a graph counted once is not judged (judging runs every candidate twice):
+  // LEARNED runs the judged choice if this context has judged the same input
+  // before, else the heuristic candidate
+  if (o.sched == TRIBIT_SCHED_LEARNED && !(o.flags & TRIBIT_COUNT_NO_JUDGE) && !tb::judged_known(ctx, g, o))
+    o.flags |= TRIBIT_COUNT_NO_JUDGE;
+  st = tribit_count(ctx, g, &o, &local);
   if (st == TRIBIT_OK) {
     *triangles = local.triangles;
     if (ms_preprocess) *ms_preprocess = g->ms_prep;
@@ -614,32 +675,97 @@ int tribit_batch_free(tribit_batch* b) {
 // ---- one rank of the reference's multi-GPU decomposition ------------------
 namespace {
 
-// Host keys of `num` chunks (chunk i: ptr[i+1] - ptr[i] keys) in one vector.
-std::vector<uint64_t> gather_chunks(const uint64_t* const* chunks, const uint64_t* ptr, uint32_t num) {
-  std::vector<uint64_t> v;
-  v.reserve(ptr[num] - ptr[0]);
+thread_local tribit_partial_info t_partial_info;
+
+// The chunks' keys copied into one device buffer (chunk by chunk, no host
+// gather); the caller owns the result. Sets the key count and the copy time.
+uint64_t* upload_chunks(tribit_ctx* ctx, const uint64_t* const* chunks, const uint64_t* ptr, uint32_t num,
+                        uint64_t* nkeys, double* ms) {
+  const uint64_t total = ptr[num] - ptr[0];
+  const auto t0 = std::chrono::steady_clock::now();
+  uint64_t* d = ctx->alloc_n<uint64_t>(total ? total : 1);
+  uint64_t at = 0;
   for (uint32_t i = 0; i < num; ++i) {
     const uint64_t k = ptr[i + 1] - ptr[i];
-    if (k) v.insert(v.end(), chunks[i], chunks[i] + k);
+    if (k) TB_HIP(hipMemcpyAsync(d + at, chunks[i], 8 * k, hipMemcpyHostToDevice, ctx->stream));
+    at += k;
   }
-  return v;
+  ctx->sync();
+  *ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  *nkeys = total;
+  return d;
 }
 
-uint64_t count_vector(tribit_ctx* ctx, const std::vector<uint64_t>& keys, uint64_t n) {
+// Diagonal task: the triangles of its edges (preprocessing + the learned
+// schedule's heuristic candidate: a graph counted once is not judged).
+uint64_t count_diagonal(tribit_ctx* ctx, const uint64_t* const* chunks, const uint64_t* ptr, uint32_t num,
+                        uint64_t n) {
+  tribit_partial_info pi;
+  memset(&pi, 0, sizeof(pi));
   tribit_graph* g = new tribit_graph;
   try {
-    uint64_t* d = ctx->alloc_n<uint64_t>(keys.empty() ? 1 : keys.size());
-    if (!keys.empty())
-      TB_HIP(hipMemcpyAsync(d, keys.data(), 8 * keys.size(), hipMemcpyHostToDevice, ctx->stream));
-    tb::preprocess(ctx, d, keys.size(), n, g);
+    uint64_t nk = 0;
+    uint64_t* d = upload_chunks(ctx, chunks, ptr, num, &nk, &pi.ms_h2d);
+    tb::preprocess(ctx, d, nk, n, g);
     tribit_count_opts o;
     memset(&o, 0, sizeof(o));
     o.sched = TRIBIT_SCHED_LEARNED;
-    o.flags = TRIBIT_COUNT_NO_JUDGE;  // a graph counted once: no candidate timing
+    o.flags = TRIBIT_COUNT_NO_JUDGE;
     o.repeats = 1;
     tribit_count_info ci;
     tb::count(ctx, g, o, &ci);
+    pi.keys = nk;
+    pi.edges = g->m;
+    pi.lists = g->m;
+    pi.streamed = g->wplus;
+    pi.triangles = ci.triangles;
+    pi.ms_build = g->ms_prep + g->ms_layout[ci.layout & 3];
+    pi.ms_count = ci.ms_count;
     tb::free_graph(g);
+    t_partial_info = pi;
+    return ci.triangles;
+  } catch (...) {
+    tb::free_graph(g);
+    throw;
+  }
+}
+
+// Off-diagonal task: triangles a < b < c with (a, b) in cols and (a, c),
+// (b, c) in idxs, counted directly (build_offdiag) on the DIRECTED schedule.
+uint64_t count_off_diagonal(tribit_ctx* ctx, const uint64_t* const* cols, const uint64_t* cptr, uint32_t cnum,
+                            const uint64_t* const* idxs, const uint64_t* iptr, uint32_t inum, uint64_t n) {
+  tribit_partial_info pi;
+  memset(&pi, 0, sizeof(pi));
+  tribit_graph* g = new tribit_graph;
+  DevBuf<uint64_t> dc, dx;
+  try {
+    uint64_t nc = 0, nx = 0;
+    dc.ctx = dx.ctx = ctx;
+    dc.p = upload_chunks(ctx, cols, cptr, cnum, &nc, &pi.ms_h2d);
+    dx.p = upload_chunks(ctx, idxs, iptr, inum, &nx, &pi.ms_h2d);
+    TB_HIP(hipEventRecord(ctx->ev[0], ctx->stream));
+    tb::build_offdiag(ctx, dc.p, nc, dx.p, nx, n, g);
+    TB_HIP(hipEventRecord(ctx->ev[1], ctx->stream));
+    TB_HIP(hipEventSynchronize(ctx->ev[1]));
+    float ms = 0.f;
+    TB_HIP(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
+    dc.reset();
+    dx.reset();
+    tribit_count_opts o;
+    memset(&o, 0, sizeof(o));
+    o.sched = TRIBIT_SCHED_DIRECTED;
+    o.repeats = 1;
+    tribit_count_info ci;
+    tb::count(ctx, g, o, &ci);
+    pi.keys = nc + nx;
+    pi.edges = g->m + g->offdiag_lists;
+    pi.lists = g->offdiag_lists;
+    pi.streamed = g->wplus;
+    pi.triangles = ci.triangles;
+    pi.ms_build = ms;
+    pi.ms_count = ci.ms_count;
+    tb::free_graph(g);
+    t_partial_info = pi;
     return ci.triangles;
   } catch (...) {
     tb::free_graph(g);
@@ -666,7 +792,7 @@ int tribit_count_partial_diagonal(tribit_ctx* ctx, const uint64_t* const* chunks
   if (!ctx || !triangles || !chunks_ok(chunks, nnz_ptr, num_ptrs) || num_nodes == 0) return TRIBIT_ERR_ARG;
   return guarded(ctx, [&] {
     TB_HIP(hipSetDevice(ctx->device));
-    *triangles = count_vector(ctx, gather_chunks(chunks, nnz_ptr, num_ptrs), num_nodes);
+    *triangles = count_diagonal(ctx, chunks, nnz_ptr, num_ptrs, num_nodes);
   });
 }
 
@@ -679,19 +805,15 @@ int tribit_count_partial_off_diagonal(tribit_ctx* ctx, const uint64_t* const* co
     return TRIBIT_ERR_ARG;
   return guarded(ctx, [&] {
     TB_HIP(hipSetDevice(ctx->device));
-    // every triangle a < b < c of cols + idxs has (b, c) and (a, c) in one set:
-    // both in cols (c in the column partition), both in idxs with (a, b) in
-    // idxs (b beyond the column partition) or (a, b) in cols (the wanted ones)
-    const std::vector<uint64_t> c = gather_chunks(cols, nnz_cols_ptr, num_ptrs_cols);
-    const std::vector<uint64_t> x = gather_chunks(idxs, nnz_idxs_ptr, num_ptrs_idxs);
-    std::vector<uint64_t> u(c);
-    u.insert(u.end(), x.begin(), x.end());
-    const uint64_t tu = count_vector(ctx, u, num_nodes);
-    const uint64_t tc = count_vector(ctx, c, num_nodes);
-    const uint64_t tx = count_vector(ctx, x, num_nodes);
-    if (tc + tx > tu) tb::fail(TRIBIT_ERR_ARG, "column and index sets overlap");
-    *triangles = tu - tc - tx;
+    *triangles = count_off_diagonal(ctx, cols, nnz_cols_ptr, num_ptrs_cols, idxs, nnz_idxs_ptr, num_ptrs_idxs,
+                                    num_nodes);
   });
+}
+
+int tribit_partial_last_info(tribit_partial_info* out) {
+  if (!out) return TRIBIT_ERR_ARG;
+  *out = t_partial_info;
+  return TRIBIT_OK;
 }
 
 // Reference-named entry points (same signatures as main5_multi.cu:578-589 and

@@ -65,6 +65,17 @@ struct tribit_ctx {
   tribit_mlp* model = nullptr;  // cost model of the learned scheduler (not owned)
   uint64_t model_version = 1;   // bumped whenever the model is set (judged choices go stale)
   uint64_t part_keys = 1ull << 30;  // TRIBIT_OPT_BATCH_PART_KEYS: input keys per batch part
+  // Judged choices of the learned schedule, kept per graph shape so that a
+  // graph rebuilt from the same input (a warm-up pass, then the measured one)
+  // runs the choice without judging again. Key: (n, m, W+, relabelled, flags,
+  // core request, model version); a collision can only cost speed, never
+  // exactness (every candidate counts the same triangles).
+  struct Judged {
+    uint64_t n, m, wplus, model;
+    int relabeled, flags, core, choice;
+    double ms[4];
+  };
+  std::vector<Judged> judged;
   std::string last_error;
 
   void* alloc(uint64_t bytes) {
@@ -170,15 +181,17 @@ struct tribit_graph {
   uint8_t* sfx_dh = nullptr;    // per wedge range: its ids' high half minus its owner's (255: read col)
   uint16_t* sfx_split = nullptr; // per wedge range crossing one 64K-id boundary: offset of its first id past it
   // 64-column bit tiles (tiles.hip), built on first use
-  uint64_t tiles64 = 0;          // tile count (computed by preprocessing)
-  uint64_t ntiles = 0;           // tiles built (0 until built)
-  uint32_t* tile_ptr = nullptr;  // n+1
-  uint32_t* tile_blk = nullptr;  // ntiles, column / 64
-  uint64_t* tile_mask = nullptr; // ntiles, bit column % 64
-  uint32_t* tile_idx = nullptr;  // m, tile holding each element
-  uint2* trng_all = nullptr;     // m, tile ranges aligned with rng_all
-  uint2* trng_dir = nullptr;     // m, tile ranges aligned with rng_dir
-  uint2* trng_sfx = nullptr;     // m, tile ranges aligned with rng_sfx
+  // bit tiles of the oriented rows (tiles.hip), one set per width (index 0:
+  // 32 columns, uint32 masks; 1: 64 columns, uint64 masks), built on first use
+  struct TileSet {
+    uint64_t ntiles = 0;       // tiles built (0 until built)
+    uint32_t* ptr = nullptr;   // n+1, each row's first tile
+    uint32_t* blk = nullptr;   // ntiles, column / width
+    void* mask = nullptr;      // ntiles, bit (column % width), uint32 or uint64
+    uint32_t* idx = nullptr;   // m, tile holding each element
+    uint2* trng[4] = {nullptr, nullptr, nullptr, nullptr};  // m, tile ranges aligned with each layout's ranges
+  } tiles[2];
+  uint64_t tiles32 = 0, tiles64 = 0;  // tile counts (computed on the first tribit_graph_get_info)
   // dense core (dense.hip): int8 adjacency of the top core_c ids, padded to core_kp
   int8_t* core_A = nullptr;
   uint64_t core_c = 0, core_kp = 0;
@@ -192,6 +205,9 @@ struct tribit_graph {
   int judge_flags = 0, judge_core = 0;
   double judge_ms[4] = {0, 0, 0, 0};
   double ms_prep = 0.0;
+  double ms_layout[4] = {0, 0, 0, 0};  // device time of range layouts built after preprocessing, by schedule id
+  bool tiles_counted = false;          // tiles32 / tiles64 computed (on the first tribit_graph_get_info)
+  uint64_t offdiag_lists = 0;          // an off-diagonal task's graph: its streamed lists (cols edges)
   uint64_t bytes = 0;
 };
 
@@ -221,6 +237,8 @@ struct tribit_batch {
   int sched = -1, flags = 0;              // what the schedules / capture were built for
   int gw[8] = {0};                        // tuned lane-group width per slot (0: the plan's)
   int waves[8] = {0};                     // tuned workgroups per CU per slot (0: occupancy limit)
+  int tile_width = 0;                     // tuned: 0 the plans' kernels; 32 / 64 every block launch streams
+                                          // bit tiles of that width
   uint32_t launches = 0;                  // kernels in the captured graph
   uint64_t tasks = 0;
   double ms_prep = 0.0;
@@ -229,6 +247,11 @@ struct tribit_batch {
 namespace tb {
 // prep.hip
 void preprocess(tribit_ctx* ctx, uint64_t* d_keys, uint64_t nkeys, uint64_t n, tribit_graph* g);
+void build_layout(tribit_ctx* ctx, tribit_graph* g, int layout);
+void build_offdiag(tribit_ctx* ctx, const uint64_t* d_cols, uint64_t ncols, const uint64_t* d_idxs, uint64_t nidxs,
+                   uint64_t n, tribit_graph* g);
+bool layout_built(const tribit_graph* g, int layout);
+void ensure_layout(tribit_ctx* ctx, tribit_graph* g, int layout);
 void free_graph_memory(tribit_graph* g);
 void bucket_cache_free(tribit_graph* g);
 void free_graph(tribit_graph* g);
@@ -247,6 +270,7 @@ void gen_sbm(tribit_ctx* ctx, uint64_t blocks, uint64_t size, double p_in, doubl
              uint64_t* d_keys);
 // count.hip
 void count(tribit_ctx* ctx, tribit_graph* g, const tribit_count_opts& opts, tribit_count_info* info);
+bool judged_known(const tribit_ctx* ctx, const tribit_graph* g, const tribit_count_opts& opts);
 void variant_stats(tribit_ctx* ctx, tribit_graph* g, int sched, tribit_variant_work* out);
 
 // Mask of a range end: wedge range ends carry flags in bits 31 (narrow) and
@@ -288,8 +312,8 @@ void core_launch(tribit_ctx* ctx, const tribit_graph* g, unsigned long long* res
 uint64_t core_choose(tribit_ctx* ctx, const tribit_graph* g);
 double core_macs(uint64_t kp);
 // tiles.hip
-uint64_t count_tiles(tribit_ctx* ctx, tribit_graph* g);
-void build_tiles(tribit_ctx* ctx, tribit_graph* g, int layout);
+uint64_t count_tiles(tribit_ctx* ctx, tribit_graph* g, int width);
+void build_tiles(tribit_ctx* ctx, tribit_graph* g, int layout, int width);
 void free_tiles(tribit_graph* g);
 // count.hip
 // mtx.cpp

@@ -463,11 +463,13 @@ __device__ __forceinline__ uint32_t stream_narrow2(const ColSrc& src, const uint
 // 64 columns. Same range hand-out as stream_ranges (block-shared counter).
 struct TileSrc {
   const uint32_t* blk;
-  const unsigned long long* mask;
+  const unsigned long long* mask;   // 64-column tiles
   const uint2* trng;
+  const uint32_t* mask32 = nullptr;  // 32-column tiles (then mask is null)
 };
 
-template <int G, int U>
+// TW: tile width, 32 (uint32 masks against one 32-bit bitmap word) or 64.
+template <int G, int U, int TW>
 __device__ __forceinline__ uint32_t stream_tiles(const TileSrc& ts, const uint2* R, uint32_t nk, uint32_t* s_next,
                                                  uint32_t lane, const uint32_t* T, uint32_t base, uint32_t span) {
   constexpr uint64_t kLeads = leader_mask<G>();
@@ -501,13 +503,20 @@ __device__ __forceinline__ uint32_t stream_tiles(const TileSrc& ts, const uint2*
     for (int j = 0; j < U; ++j) {
       const bool ok = j * G < rem;
       b[j] = ok ? ts.blk[p0 + j * G] : 0xFFFFFFFFu;
-      mk[j] = ok ? ts.mask[p0 + j * G] : 0ull;
+      if constexpr (TW == 32)
+        mk[j] = ok ? ts.mask32[p0 + j * G] : 0u;
+      else
+        mk[j] = ok ? ts.mask[p0 + j * G] : 0ull;
     }
 #pragma unroll
     for (int j = 0; j < U; ++j) {
-      const uint32_t i = b[j] * 64u - base;  // words past the window: the zero word after it
-      const unsigned long long w = W[min(i >> 6, span >> 6)];
-      cnt += (uint32_t)__popcll(mk[j] & w);
+      const uint32_t i = b[j] * (uint32_t)TW - base;  // words past the window: the zero word after it
+      if constexpr (TW == 32) {
+        cnt += (uint32_t)__popc((uint32_t)mk[j] & T[min(i >> 5, span >> 5)]);
+      } else {
+        const unsigned long long w = W[min(i >> 6, span >> 6)];
+        cnt += (uint32_t)__popcll(mk[j] & w);
+      }
     }
     base_r += (uint32_t)(G * U);
   }
@@ -779,7 +788,7 @@ __global__ __launch_bounds__(256) void k_count_wave_multi(const uint32_t* __rest
 
 constexpr int kBlockThreads = 512;
 
-template <int LOGS, int G, int U, bool BUF, bool TILES, bool N16>
+template <int LOGS, int G, int U, bool BUF, int TW, bool N16>
 __global__ __launch_bounds__(kBlockThreads) void k_count_block(const uint32_t* __restrict__ row_ptr,
                                                                const uint32_t* __restrict__ col, uint64_t m,
                                                                const uint16_t* __restrict__ col16,
@@ -851,7 +860,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_count_block(const uint32_t* _
       for (uint32_t i = p0 + threadIdx.x; i < p1; i += kBlockThreads) linear_insert<LOGS + 1>(T, col[r0 + i]);
     }
     uint32_t cnt = 0;
-    const bool tiles = TILES && bitmap;  // block-uniform
+    const bool tiles = TW != 0 && bitmap;  // block-uniform
     const uint2* rs = tiles ? ts.trng : rng;
     for (uint32_t c0 = 0; c0 < rcount; c0 += kChunk) {
       const uint32_t nk = min((uint32_t)kChunk, rcount - c0);
@@ -922,7 +931,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_count_block(const uint32_t* _
       __syncthreads();
       const uint32_t nn = n1 + n2;
       if (tiles) {
-        cnt += stream_tiles<G, U>(ts, R, nk, &s_next, lane, T, base, kSpan);
+        if constexpr (TW != 0) cnt += stream_tiles<G, U, TW>(ts, R, nk, &s_next, lane, T, base, kSpan);
       } else if (bitmap) {
         const BitmapProbe<LOGS + 1> pr{T, base};
         if (N16 && n1) cnt += stream_narrow<G, U>(src, R, Rh, n1, &s_next16, lane, pr);
@@ -1286,6 +1295,14 @@ __global__ void k_zero_u64(unsigned long long* p, uint32_t n) {
   if (i < n) p[i] = 0ull;
 }
 
+// A plan's bucket -> slot table, passed by value as a kernel argument.
+struct PlanTable {
+  uint8_t t[kBuckets];
+};
+__global__ void k_put_table(PlanTable pt, uint8_t* out) {
+  for (uint32_t i = threadIdx.x; i < (uint32_t)kBuckets; i += blockDim.x) out[i] = pt.t[i];
+}
+
 static SchedView sched_view(const tribit_graph* g, int sched, uint32_t core_d0 = 0xFFFFFFFFu,
                             uint32_t core_cut = 0) {
   SchedView v;
@@ -1409,6 +1426,7 @@ void bucket_stats(tribit_ctx* ctx, tribit_graph* g, int sched, BucketStats* out,
     *out = g->bcache->st;
     return;
   }
+  ensure_layout(ctx, g, sched);
   memset(out, 0, sizeof(*out));
   DevBuf<unsigned long long> st(ctx, kStat * kBuckets);
   k_zero_u64<<<grid_for(kStat * kBuckets, kBlock), kBlock, 0, ctx->stream>>>(st.p, kStat * kBuckets);
@@ -1643,16 +1661,16 @@ static void launch_multi(const LaunchArgs& a, int slot, int gw, uint32_t ntasks,
   }
 }
 
-template <int LOGS, int G, int U, bool BUF, bool TILES, bool N16>
+template <int LOGS, int G, int U, bool BUF, int TW, bool N16>
 static void launch_block_t(const LaunchArgs& a, int slot, uint32_t ntasks, int cus, hipStream_t s, tribit_graph* g,
                            const uint2* rng, const TileSrc& ts, const uint4* tasks, uint64_t* blocks) {
   int per_cu = 0;
-  TB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_count_block<LOGS, G, U, BUF, TILES, N16>,
+  TB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_count_block<LOGS, G, U, BUF, TW, N16>,
                                                       kBlockThreads, 0));
   if (a.waves[slot] > 0 && per_cu > a.waves[slot]) per_cu = a.waves[slot];
   if (per_cu < 1) per_cu = 1;
   const unsigned grid = (unsigned)std::min<uint64_t>(ntasks, (uint64_t)cus * per_cu);
-  k_count_block<LOGS, G, U, BUF, TILES, N16><<<grid, kBlockThreads, 0, s>>>(
+  k_count_block<LOGS, G, U, BUF, TW, N16><<<grid, kBlockThreads, 0, s>>>(
       g->row_ptr, g->col, g->m, g->col16, g->sfx_dh, g->sfx_split, rng, ts, tasks, ntasks, a.queue + (size_t)slot * kQueues * kQStride, a.result,
       a.att & ~kForceGlobal,
       a.tdur ? a.tdur + (tasks - a.task0) : nullptr, a.gid, a.gcount);
@@ -1665,19 +1683,26 @@ static void launch_block_g(const LaunchArgs& a, int slot, uint32_t ntasks, int c
   const bool buf = g->m * 4 <= kBufMax && !(a.att & kForceGlobal);
   // narrow ranges exist only in the wedge layout (col16 is built with it)
   const bool n16 = buf && g->col16 && g->sfx_dh && g->sfx_split && rng == g->rng_sfx && !(a.att & kWideBit);
-  if (ts.trng) {
+  if (ts.trng && ts.mask32) {
     if (n16)
-      launch_block_t<LOGS, G, kU, true, true, true>(a, slot, ntasks, cus, s, g, rng, ts, tasks, blocks);
+      launch_block_t<LOGS, G, kU, true, 32, true>(a, slot, ntasks, cus, s, g, rng, ts, tasks, blocks);
     else if (buf)
-      launch_block_t<LOGS, G, kU, true, true, false>(a, slot, ntasks, cus, s, g, rng, ts, tasks, blocks);
+      launch_block_t<LOGS, G, kU, true, 32, false>(a, slot, ntasks, cus, s, g, rng, ts, tasks, blocks);
     else
-      launch_block_t<LOGS, G, kU, false, true, false>(a, slot, ntasks, cus, s, g, rng, ts, tasks, blocks);
+      launch_block_t<LOGS, G, kU, false, 32, false>(a, slot, ntasks, cus, s, g, rng, ts, tasks, blocks);
+  } else if (ts.trng) {
+    if (n16)
+      launch_block_t<LOGS, G, kU, true, 64, true>(a, slot, ntasks, cus, s, g, rng, ts, tasks, blocks);
+    else if (buf)
+      launch_block_t<LOGS, G, kU, true, 64, false>(a, slot, ntasks, cus, s, g, rng, ts, tasks, blocks);
+    else
+      launch_block_t<LOGS, G, kU, false, 64, false>(a, slot, ntasks, cus, s, g, rng, ts, tasks, blocks);
   } else if (n16) {
-    launch_block_t<LOGS, G, kU, true, false, true>(a, slot, ntasks, cus, s, g, rng, ts, tasks, blocks);
+    launch_block_t<LOGS, G, kU, true, 0, true>(a, slot, ntasks, cus, s, g, rng, ts, tasks, blocks);
   } else if (buf) {
-    launch_block_t<LOGS, G, kU, true, false, false>(a, slot, ntasks, cus, s, g, rng, ts, tasks, blocks);
+    launch_block_t<LOGS, G, kU, true, 0, false>(a, slot, ntasks, cus, s, g, rng, ts, tasks, blocks);
   } else {
-    launch_block_t<LOGS, G, kU, false, false, false>(a, slot, ntasks, cus, s, g, rng, ts, tasks, blocks);
+    launch_block_t<LOGS, G, kU, false, 0, false>(a, slot, ntasks, cus, s, g, rng, ts, tasks, blocks);
   }
 }
 
@@ -1710,6 +1735,7 @@ static int mode_word(int flags) {
 struct Schedule {
   Plan plan;
   int gdef[kSlots] = {0};  // the plan's own widths (a batch's tuned widths override them)
+  int kdef[kSlots] = {0};  // the plan's own kernel kinds (a batch's tuned tile width overrides block kinds)
   uint64_t core_c = 0;     // dense-core vertices counted on the matrix cores (0: none)
   int layout = 0;
   const uint2* rng = nullptr;
@@ -1719,14 +1745,45 @@ struct Schedule {
   uint64_t ntask = 0;
 };
 
+// Bump allocation from one caller-held device buffer (the no-allocation entry
+// point's scratch array): 256-byte aligned slices, released all at once by
+// resetting `used`. A request beyond `size` fails with TRIBIT_ERR_TOO_LARGE.
+struct Arena {
+  char* base = nullptr;
+  uint64_t size = 0, used = 0;
+  template <typename T>
+  T* take(uint64_t count) {
+    uint64_t bytes = (count ? count : 1) * sizeof(T);
+    bytes = (bytes + 255) & ~uint64_t(255);
+    if (used + bytes > size)
+      fail(TRIBIT_ERR_TOO_LARGE, "the schedule needs " + std::to_string((used + bytes) >> 20) +
+                                     " MiB of scratch, more than the no-allocation entry point's " +
+                                     std::to_string(size >> 20) + " MiB");
+    T* p = reinterpret_cast<T*>(base + used);
+    used += bytes;
+    return p;
+  }
+};
+
+// A device buffer from the arena when there is one, else from the context's pool.
+template <typename T>
+static T* scratch_buf(tribit_ctx* ctx, Arena* arena, DevBuf<T>& own, uint64_t count) {
+  if (arena) return arena->take<T>(count);
+  own = DevBuf<T>(ctx, count ? count : 1);
+  return own.p;
+}
+
 // Resolves the plan (learned table, tile opt-in/out) and builds the task
-// lists (one host sync for the per-slot totals).
+// lists (one host sync for the per-slot totals). With `arena`, every device
+// buffer comes from it (the caller rules out what would allocate: learned
+// tables, tiles, a dense core).
 void make_schedule(tribit_ctx* ctx, tribit_graph* g, int layout, int flags, bool learned, Schedule& S,
-                   uint64_t core_c) {
+                   uint64_t core_c, Arena* arena = nullptr) {
   hipStream_t s = ctx->stream;
   const uint64_t n = g->n;
   Plan& plan = S.plan;
   S.layout = layout;
+  ensure_layout(ctx, g, layout);  // no-op when built (count() builds it before its timing starts)
   uint32_t core_d0 = 0xFFFFFFFFu, core_cut = 0;
   if (core_c > g->n || !g->relabeled) core_c = 0;
   S.core_c = core_c;
@@ -1753,34 +1810,44 @@ void make_schedule(tribit_ctx* ctx, tribit_graph* g, int layout, int flags, bool
   bool any_tiles = false;
   for (int i = 0; i < plan.nslots; ++i) any_tiles |= plan.kind[i] == kKindTile;
   if (any_tiles && g->m) {
-    build_tiles(ctx, g, layout);  // once per graph and layout
-    S.ts.blk = g->tile_blk;
-    S.ts.mask = reinterpret_cast<const unsigned long long*>(g->tile_mask);
-    S.ts.trng = layout == TRIBIT_SCHED_PLAIN ? g->trng_all : layout == TRIBIT_SCHED_WEDGE ? g->trng_sfx
-                                                                                          : g->trng_dir;
+    const int width = (flags & TRIBIT_COUNT_TILES32) ? 32 : 64;
+    build_tiles(ctx, g, layout, width);  // once per graph, width and layout
+    const tribit_graph::TileSet& T = g->tiles[width == 32 ? 0 : 1];
+    S.ts.blk = T.blk;
+    S.ts.mask = width == 64 ? reinterpret_cast<const unsigned long long*>(T.mask) : nullptr;
+    S.ts.mask32 = width == 32 ? reinterpret_cast<const uint32_t*>(T.mask) : nullptr;
+    S.ts.trng = T.trng[layout & 3];
   }
-  DevBuf<uint8_t> dtable(ctx, kBuckets);
-  TB_HIP(hipMemcpyAsync(dtable.p, plan.table, kBuckets, hipMemcpyHostToDevice, s));
+  DevBuf<uint8_t> dtable_own;
+  uint8_t* dtable = scratch_buf(ctx, arena, dtable_own, kBuckets);
+  {
+    PlanTable pt;
+    memcpy(pt.t, plan.table, kBuckets);
+    k_put_table<<<1, kBlock, 0, s>>>(pt, dtable);  // a kernel argument: no host staging
+  }
   PlanDev pd;
   for (int i = 0; i < kSlots; ++i) pd.kind[i] = i < plan.nslots ? plan.kind[i] : kKindBlock;
-  pd.table = dtable.p;
+  pd.table = dtable;
 
   // per-block task counts (block nb covers no owner: its sum is zero), scanned
   const uint64_t nb = (n + kSchedBlock - 1) / kSchedBlock;
-  DevBuf<Cnt8> cnt(ctx, nb + 1), off(ctx, nb + 1);
+  DevBuf<Cnt8> cnt_own, off_own;
+  Cnt8* cnt = scratch_buf(ctx, arena, cnt_own, nb + 1);
+  Cnt8* off = scratch_buf(ctx, arena, off_own, nb + 1);
   const SchedView sv = sched_view(g, layout, core_d0, core_cut);
   S.rng = sched_ranges(g, layout);
-  k_sched_count<<<(unsigned)(nb + 1), kSchedBlock, 0, s>>>(sv, pd, n, cnt.p);
+  k_sched_count<<<(unsigned)(nb + 1), kSchedBlock, 0, s>>>(sv, pd, n, cnt);
   {
     Cnt8 zero;
     for (int i = 0; i < kSlots; ++i) zero.v[i] = 0;
     size_t bytes = 0;
-    TB_HIP(rocprim::exclusive_scan(nullptr, bytes, cnt.p, off.p, zero, (size_t)(nb + 1), Cnt8Plus(), s));
-    DevBuf<uint8_t> tmp(ctx, bytes ? bytes : 8);
-    TB_HIP(rocprim::exclusive_scan(tmp.p, bytes, cnt.p, off.p, zero, (size_t)(nb + 1), Cnt8Plus(), s));
+    TB_HIP(rocprim::exclusive_scan(nullptr, bytes, cnt, off, zero, (size_t)(nb + 1), Cnt8Plus(), s));
+    DevBuf<uint8_t> tmp_own;
+    uint8_t* tmp = scratch_buf(ctx, arena, tmp_own, bytes ? bytes : 8);
+    TB_HIP(rocprim::exclusive_scan(tmp, bytes, cnt, off, zero, (size_t)(nb + 1), Cnt8Plus(), s));
   }
   // off[nb] = totals
-  TB_HIP(hipMemcpyAsync(&S.tot, off.p + nb, sizeof(Cnt8), hipMemcpyDeviceToHost, s));
+  TB_HIP(hipMemcpyAsync(&S.tot, off + nb, sizeof(Cnt8), hipMemcpyDeviceToHost, s));
   ctx->sync();
   S.ntask = 0;
   for (int i = 0; i < kSlots; ++i) {
@@ -1788,8 +1855,11 @@ void make_schedule(tribit_ctx* ctx, tribit_graph* g, int layout, int flags, bool
     S.ntask += S.tot.v[i];
   }
   if (S.ntask > 0xFFFFFFFFull) fail(TRIBIT_ERR_TOO_LARGE, "more than 2^32-1 tasks");
-  S.tasks = DevBuf<uint4>(ctx, S.ntask ? S.ntask : 1);
-  if (nb) k_sched_fill<<<(unsigned)nb, kSchedBlock, 0, s>>>(sv, pd, n, off.p, S.base, S.tasks.p);
+  if (arena)
+    S.tasks.p = arena->take<uint4>(S.ntask);  // S.tasks.ctx stays null: the arena is never freed here
+  else
+    S.tasks = DevBuf<uint4>(ctx, S.ntask ? S.ntask : 1);
+  if (nb) k_sched_fill<<<(unsigned)nb, kSchedBlock, 0, s>>>(sv, pd, n, off, S.base, S.tasks.p);
   TB_HIP(hipGetLastError());
   // every plan but the textbook one: a one-task wave-64 launch with thousands
   // of tasks per CU runs as multi-task waves (same tasks, eight per wave at
@@ -1847,19 +1917,21 @@ void launch_schedule(tribit_ctx* ctx, tribit_graph* g, const Schedule& S, const 
 static uint64_t count_once(tribit_ctx* ctx, tribit_graph* g, int sched, int flags, Plan& plan, bool learned,
                            double* ms_total, double* ms_sched, double* ms_kernel, double* ms_var,
                            tribit_count_info* info, unsigned long long* tlog, uint64_t core_c = 0,
-                           double* ms_core = nullptr) {
+                           double* ms_core = nullptr, Arena* arena = nullptr) {
   hipStream_t s = ctx->stream;
   TB_HIP(hipEventRecord(ctx->ev[2], s));
   Schedule S;
   S.plan = plan;
-  make_schedule(ctx, g, sched, flags, learned, S, core_c);
+  DevBuf<unsigned long long> counters_own;
+  // counters first: an arena that is too small for the tasks fails before any launch
+  unsigned long long* counters = scratch_buf(ctx, arena, counters_own, kCountScratch);
+  make_schedule(ctx, g, sched, flags, learned, S, core_c, arena);
   plan = S.plan;
-  DevBuf<unsigned long long> scratch(ctx, kCountScratch);
-  k_zero_u64<<<grid_for(kCountScratch, kBlock), kBlock, 0, s>>>(scratch.p, kCountScratch);
+  k_zero_u64<<<grid_for(kCountScratch, kBlock), kBlock, 0, s>>>(counters, kCountScratch);
   LaunchArgs a;
   a.att = mode_word(flags);
-  a.result = scratch.p;
-  a.queue = reinterpret_cast<uint32_t*>(scratch.p + kQueueOff);
+  a.result = counters;
+  a.queue = reinterpret_cast<uint32_t*>(counters + kQueueOff);
   DevBuf<uint32_t> tdur;
   if (tlog) {
     tdur = DevBuf<uint32_t>(ctx, S.ntask ? S.ntask : 1);
@@ -1967,6 +2039,35 @@ static bool candidate(tribit_ctx* ctx, tribit_graph* g, const tribit_count_opts&
   }
 }
 
+static const tribit_ctx::Judged* judged_lookup(const tribit_ctx* ctx, const tribit_graph* g,
+                                               const tribit_count_opts& opts) {
+  for (const auto& j : ctx->judged)
+    if (j.n == g->n && j.m == g->m && j.wplus == g->wplus && j.relabeled == g->relabeled &&
+        j.flags == opts.flags && j.core == opts.dense_core && j.model == ctx->model_version)
+      return &j;
+  return nullptr;
+}
+
+static void judged_store(tribit_ctx* ctx, const tribit_graph* g, const tribit_count_opts& opts) {
+  constexpr size_t kKeep = 64;  // most recent shapes kept
+  if (ctx->judged.size() >= kKeep) ctx->judged.erase(ctx->judged.begin());
+  tribit_ctx::Judged j;
+  j.n = g->n;
+  j.m = g->m;
+  j.wplus = g->wplus;
+  j.model = ctx->model_version;
+  j.relabeled = g->relabeled;
+  j.flags = opts.flags;
+  j.core = opts.dense_core;
+  j.choice = g->judge_choice;
+  for (int c = 0; c < kCands; ++c) j.ms[c] = g->judge_ms[c];
+  ctx->judged.push_back(j);
+}
+
+bool judged_known(const tribit_ctx* ctx, const tribit_graph* g, const tribit_count_opts& opts) {
+  return judged_lookup(ctx, g, opts) != nullptr;
+}
+
 void count(tribit_ctx* ctx, tribit_graph* g, const tribit_count_opts& opts, tribit_count_info* info) {
   const int reps = opts.repeats > 0 ? opts.repeats : 1;
   std::vector<double> tt, ts, tk, tv[kSlots];
@@ -1982,8 +2083,20 @@ void count(tribit_ctx* ctx, tribit_graph* g, const tribit_count_opts& opts, trib
     chosen = kCandHeur;
     candidate(ctx, g, opts, chosen, &run);
   } else if (learned) {
-    const bool stale = g->judge_choice < 0 || g->judge_model != ctx->model_version ||
-                       g->judge_flags != opts.flags || g->judge_core != opts.dense_core;
+    bool stale = g->judge_choice < 0 || g->judge_model != ctx->model_version ||
+                 g->judge_flags != opts.flags || g->judge_core != opts.dense_core;
+    if (stale) {
+      // the same input judged before in this context (e.g. a warm-up pass)
+      const tribit_ctx::Judged* hit = judged_lookup(ctx, g, opts);
+      if (hit) {
+        g->judge_choice = hit->choice;
+        for (int c = 0; c < kCands; ++c) g->judge_ms[c] = hit->ms[c];
+        g->judge_model = ctx->model_version;
+        g->judge_flags = opts.flags;
+        g->judge_core = opts.dense_core;
+        stale = false;
+      }
+    }
     if (stale) {
       for (int c = 0; c < kCands; ++c) g->judge_ms[c] = 0.0;
       double best = INFINITY;
@@ -1992,6 +2105,7 @@ void count(tribit_ctx* ctx, tribit_graph* g, const tribit_count_opts& opts, trib
       for (int c = 0; c < kCands; ++c) {
         CandidateRun cr;
         if (!candidate(ctx, g, opts, c, &cr)) continue;
+        ensure_layout(ctx, g, cr.layout);  // a layout's first use: built before the timed runs
         // twice: the first run also builds what the candidate caches (core
         // matrix, bucket statistics, tiles); the second is the one compared
         double a = 0.0, b, k, v[kSlots] = {0};
@@ -2014,6 +2128,7 @@ void count(tribit_ctx* ctx, tribit_graph* g, const tribit_count_opts& opts, trib
       g->judge_model = ctx->model_version;
       g->judge_flags = opts.flags;
       g->judge_core = opts.dense_core;
+      judged_store(ctx, g, opts);
     }
     chosen = g->judge_choice;
     candidate(ctx, g, opts, chosen, &run);
@@ -2026,6 +2141,7 @@ void count(tribit_ctx* ctx, tribit_graph* g, const tribit_count_opts& opts, trib
     run.core_c = (opts.dense_core > 0 && g->relabeled) ? std::min<uint64_t>((uint64_t)opts.dense_core, g->n) : 0;
   }
   std::vector<double> tcore;
+  ensure_layout(ctx, g, run.layout);  // first use of a layout: built outside the timing
   for (int r = 0; r < reps; ++r) {
     double a, b, c, v[kSlots] = {0}, mc = 0.0;
     Plan plan = run.plan;
@@ -2190,7 +2306,10 @@ static double batch_schedules(tribit_ctx* ctx, tribit_batch* b, int sched, int f
     p.S = new Schedule;
     p.S->plan = plan;
     make_schedule(ctx, p.g, layout, flags, learned, *p.S, 0);  // per-graph attribution: no dense core
-    for (int i = 0; i < kSlots; ++i) p.S->gdef[i] = p.S->plan.g[i];
+    for (int i = 0; i < kSlots; ++i) {
+      p.S->gdef[i] = p.S->plan.g[i];
+      p.S->kdef[i] = p.S->plan.kind[i];
+    }
   }
   TB_HIP(hipEventRecord(ctx->ev[1], ctx->stream));
   TB_HIP(hipEventSynchronize(ctx->ev[1]));
@@ -2205,6 +2324,27 @@ static double batch_schedules(tribit_ctx* ctx, tribit_batch* b, int sched, int f
 static double batch_capture(tribit_ctx* ctx, tribit_batch* b, int att, uint32_t* launches, uint64_t* tasks) {
   batch_drop_exec(b);
   hipStream_t s = ctx->stream;
+  // the tuned tile width: block launches become tile launches over tiles of
+  // that width (built here, before the capture: building allocates and syncs)
+  for (auto& p : b->parts) {
+    Schedule& S = *p.S;
+    const int tw = b->tile_width;
+    bool tiles = false;
+    for (int k = 0; k < S.plan.nslots; ++k) {
+      const bool blockish = S.kdef[k] == kKindBlock || S.kdef[k] == kKindTile;
+      S.plan.kind[k] = (tw && blockish) ? kKindTile : S.kdef[k];
+      tiles |= S.plan.kind[k] == kKindTile;
+    }
+    if (tiles && p.g->m) {
+      const int width = tw ? tw : (b->flags & TRIBIT_COUNT_TILES32) ? 32 : 64;
+      build_tiles(ctx, p.g, S.layout, width);
+      const tribit_graph::TileSet& T = p.g->tiles[width == 32 ? 0 : 1];
+      S.ts.blk = T.blk;
+      S.ts.mask = width == 64 ? reinterpret_cast<const unsigned long long*>(T.mask) : nullptr;
+      S.ts.mask32 = width == 32 ? reinterpret_cast<const uint32_t*>(T.mask) : nullptr;
+      S.ts.trng = T.trng[S.layout & 3];
+    }
+  }
   TB_HIP(hipStreamSynchronize(s));
   const auto t0 = std::chrono::steady_clock::now();
   uint64_t blocks = 0;
@@ -2282,6 +2422,7 @@ static void batch_fill_info(tribit_batch* b, double ms, double ms_sched, double 
   info->parts = (uint32_t)b->parts.size();
   info->launches = launches;
   info->repeats_agree = agree ? 1 : 0;
+  info->tile_width = b->tile_width;
   if (!b->parts.empty() && b->parts[0].S) {
     const Plan& pl = b->parts[0].S->plan;
     info->nslots = pl.nslots;
@@ -2365,6 +2506,26 @@ void batch_autotune(tribit_ctx* ctx, tribit_batch* b, const tribit_count_opts& o
       }
     }
     b->waves[i] = bestw;
+  }
+  // tile width: the block launches' lists streamed as 32- or 64-column bit
+  // tiles instead of ids (tried when a part has block tasks)
+  bool blocks = false;
+  for (auto& p : b->parts)
+    for (int i = 0; i < p.S->plan.nslots; ++i)
+      blocks |= p.S->tot.v[i] && (p.S->kdef[i] == kKindBlock || p.S->kdef[i] == kKindTile);
+  if (blocks) {
+    const int t0 = b->tile_width;
+    int bestt = t0;
+    for (int tw : {32, 64, 0}) {
+      if (tw == t0) continue;
+      b->tile_width = tw;
+      const double ms = trial();
+      if (ms < best) {
+        best = ms;
+        bestt = tw;
+      }
+    }
+    b->tile_width = bestt;
   }
   const double ms_cap = batch_capture(ctx, b, att, &launches, &tasks);
   const double ms = batch_replay(ctx, b, reps, c, &agree);

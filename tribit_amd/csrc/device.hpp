@@ -64,4 +64,23 @@ __device__ __forceinline__ uint32_t wave_run_length(uint32_t key, bool valid) {
   return next - lane;
 }
 
+// Segmented sum inside a wave of lanes sorted by `key` (equal keys
+// contiguous, valid lanes first): returns true on the last valid lane of each
+// run of equal keys, with *sum = the run's total of `val` up to that lane.
+__device__ __forceinline__ bool wave_segment_tail(uint32_t key, bool valid, uint64_t val, uint64_t* sum) {
+  const uint32_t lane = lane_id();
+  uint64_t s = valid ? val : 0;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint64_t t = __shfl_up(s, o);
+    const uint32_t kv = __shfl_up(key, o);
+    if (lane >= (uint32_t)o && kv == key) s += t;
+  }
+  const uint32_t nk = __shfl_down(key, 1);
+  const uint64_t vmask = __ballot(valid);
+  const uint64_t above = (lane == 63) ? 0ull : (vmask >> (lane + 1));
+  *sum = s;
+  return valid && (!(above & 1ull) || nk != key);
+}
+
 }  // namespace tb

@@ -141,14 +141,21 @@ __global__ void k_add_runs(const uint32_t* __restrict__ ids, uint64_t m, uint32_
   if (run) atomicAdd(&deg[a], run);
 }
 
+// Oriented arcs u -> v are sorted as packed keys (u << kb) | v with kb =
+// bits_for(n): 2*kb bits (50 at 2^25 vertices) instead of 32 + kb, one radix
+// digit pass fewer than the input key layout.
+__device__ __forceinline__ uint64_t arc_key(uint32_t u, uint32_t v, uint32_t kb) {
+  return ((uint64_t)u << kb) | (uint64_t)v;
+}
+
 // Degree orientation: {a<b} becomes a->b iff deg(a) <= deg(b).
 __global__ void k_orient(const uint64_t* __restrict__ keys, uint64_t m, const uint32_t* __restrict__ deg,
-                         uint64_t* __restrict__ okeys) {
+                         uint32_t kb, uint64_t* __restrict__ okeys) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m) return;
   const uint64_t k = keys[i];
   const uint32_t a = (uint32_t)(k >> 32), b = (uint32_t)k;
-  okeys[i] = (deg[a] <= deg[b]) ? k : (((uint64_t)b << 32) | a);
+  okeys[i] = (deg[a] <= deg[b]) ? arc_key(a, b, kb) : arc_key(b, a, kb);
 }
 
 // col[i] = dst, odeg[src] += 1 (sorted by src: aggregated per run).
@@ -165,21 +172,21 @@ __global__ void k_scatter_labels(const uint32_t* __restrict__ old_of_new, uint64
 
 // With relabelled ids the degree order is the id order: a -> b iff a < b.
 __global__ void k_orient_relabel(const uint64_t* __restrict__ keys, uint64_t m, const uint32_t* __restrict__ lab,
-                                 uint64_t* __restrict__ okeys) {
+                                 uint32_t kb, uint64_t* __restrict__ okeys) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m) return;
   const uint64_t k = keys[i];
   const uint32_t a = lab[(uint32_t)(k >> 32)], b = lab[(uint32_t)k];
-  okeys[i] = a < b ? (((uint64_t)a << 32) | b) : (((uint64_t)b << 32) | a);
+  okeys[i] = a < b ? arc_key(a, b, kb) : arc_key(b, a, kb);
 }
 
-__global__ void k_csr_split(const uint64_t* __restrict__ okeys, uint64_t m, uint32_t* __restrict__ col,
+__global__ void k_csr_split(const uint64_t* __restrict__ okeys, uint64_t m, uint32_t kb, uint32_t* __restrict__ col,
                             uint32_t* __restrict__ odeg) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = i < m;
   const uint64_t k = valid ? okeys[i] : 0;
-  const uint32_t s = valid ? (uint32_t)(k >> 32) : 0xFFFFFFFFu;
-  if (valid) col[i] = (uint32_t)k;
+  const uint32_t s = valid ? (uint32_t)(k >> kb) : 0xFFFFFFFFu;
+  if (valid) col[i] = (uint32_t)(k & ((1ull << kb) - 1));
   const uint32_t run = wave_run_length(s, valid);
   if (run) atomicAdd(&odeg[s], run);
 }
@@ -191,29 +198,71 @@ struct RowStats {
   unsigned int pad;
 };
 
-// wrow[u] = sum of outdeg(v), v in N+(u); totals reduced per wave.
-__global__ void k_row_stats(const uint32_t* __restrict__ row_ptr, const uint32_t* __restrict__ col, uint64_t n,
-                            uint64_t* __restrict__ wrow, RowStats* __restrict__ st) {
+// Rows with outdeg >= 2 and the largest outdeg (coalesced over row_ptr).
+__global__ void k_row_flags(const uint32_t* __restrict__ row_ptr, uint64_t n, RowStats* __restrict__ st) {
   const uint64_t u = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  uint64_t w = 0;
-  uint32_t d = 0;
-  if (u < n) {
-    const uint32_t r0 = row_ptr[u], r1 = row_ptr[u + 1];
-    d = r1 - r0;
-    for (uint32_t e = r0; e < r1; ++e) {
-      const uint32_t v = col[e];
-      w += row_ptr[v + 1] - row_ptr[v];
-    }
-    wrow[u] = w;
-  }
-  const uint64_t ws = wave_sum_u64(w);
+  const uint32_t d = u < n ? row_ptr[u + 1] - row_ptr[u] : 0;
   const uint64_t act = (uint64_t)__popcll(__ballot(d >= 2));
   const uint32_t dm = wave_max_u32(d);
   if (lane_id() == 0) {
-    if (ws) atomicAdd(&st->wplus, (unsigned long long)ws);
     if (act) atomicAdd(&st->rows_active, (unsigned long long)act);
     if (dm) atomicMax(&st->max_outdeg, dm);
   }
+}
+
+// One pass over the arcs e = (u -> v) in CSR order (okeys sorted, packed
+// (u << kb) | v), every lane one arc:
+//   wrow[u] += outdeg(v): W+ per row, one atomic per run of u in a wave, and
+//     the wave's total to st->wplus (wrow zeroed by the caller);
+//   rend[e] = row_ptr[u + 1] (relabelled graphs): the end of arc e's wedge
+//     range [e + 1, rend[e]), the part of N+(u) after v;
+//   nar[e] (relabelled graphs, m < 2^30): that range's narrow record: bit 31
+//     its ids share their high 16 bits H; bit 30 (with 31) shorter than 2^16
+//     ids with high halves H then H + 1, bits 0..15 the offset of the first
+//     id with H + 1; bits 16..23 H - (v >> 16), 255 when larger. In CSR order
+//     the range's ids are the row's next ones, so the reads stay in cache.
+__global__ void k_arc_stats(const uint64_t* __restrict__ okeys, uint64_t m, uint32_t kb,
+                            const uint32_t* __restrict__ row_ptr, const uint32_t* __restrict__ col,
+                            unsigned long long* __restrict__ wrow, RowStats* __restrict__ st,
+                            uint32_t* __restrict__ rend, uint32_t* __restrict__ nar) {
+  const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = e < m;
+  const uint64_t k = valid ? okeys[e] : 0;
+  const uint32_t u = valid ? (uint32_t)(k >> kb) : 0xFFFFFFFFu;
+  const uint32_t v = (uint32_t)(k & ((1ull << kb) - 1));
+  const uint64_t w = valid ? (uint64_t)(row_ptr[v + 1] - row_ptr[v]) : 0;
+  uint64_t sum = 0;
+  if (wave_segment_tail(u, valid, w, &sum) && sum) atomicAdd(&wrow[u], (unsigned long long)sum);
+  const uint64_t ws = wave_sum_u64(w);
+  if (lane_id() == 0 && ws) atomicAdd(&st->wplus, (unsigned long long)ws);
+  if (!valid || !rend) return;
+  const uint32_t end = row_ptr[u + 1];
+  rend[e] = end;
+  if (!nar) return;
+  uint32_t rec = 0;
+  const uint32_t b = (uint32_t)e + 1;
+  if (b < end) {
+    const uint32_t f = col[b] >> 16, l = col[end - 1] >> 16;
+    const bool two = l == f + 1 && end - b < 65536u;
+    if (f == l || two) {
+      uint32_t sp = 0;
+      if (two) {  // first position with high half l: col[lo] has f, col[hi] has l
+        uint32_t lo = b, hi = end - 1;
+        while (lo + 1 < hi) {
+          const uint32_t mid = lo + ((hi - lo) >> 1);
+          if ((col[mid] >> 16) == f)
+            lo = mid;
+          else
+            hi = mid;
+        }
+        sp = hi - b;
+      }
+      uint32_t d = f - (v >> 16);  // ids after v: H >= v's high half
+      d = d < 255u ? d : 255u;
+      rec = 0x80000000u | (two ? 0x40000000u : 0u) | (d << 16) | sp;
+    }
+  }
+  nar[e] = rec;
 }
 
 // ------------------------------------------------------------ range lists
@@ -317,19 +366,18 @@ __global__ void k_rv_fill(const uint64_t* __restrict__ keys, uint64_t mr, const 
   if (tail && sum) atomicAdd(&w_rv[v], (unsigned long long)sum);
 }
 
-// Row end of every arc e = (u -> v): rend[e] = row_ptr[u + 1] (okeys sorted by (u, v)).
-__global__ void k_arc_row_end(const uint64_t* __restrict__ okeys, uint64_t m, const uint32_t* __restrict__ row_ptr,
-                              uint32_t* __restrict__ rend) {
-  const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e < m) rend[e] = row_ptr[(uint32_t)(okeys[e] >> 32) + 1];
-}
-
 // Arcs e = (u -> v) sorted by head v: the suffix of N+(u) after v,
 // [e + 1, rend[e]), is streamed against N+(v). With ids in degree-rank order
 // N+(v) holds only ids > v, so the elements of N+(u) up to v can never match.
 // sfx_cnt[v] += 1, w_sfx[v] += suffix length, aggregated per run of v in a wave.
+// With nar (the narrow records of k_arc_stats, by arc), wedge range j = [b, e)
+// is flagged narrow when its ids share their high 16 bits H (end bit 31), or,
+// shorter than 2^16 ids, when they have high half H up to position
+// b + split[j] and H + 1 from there (end bits 31 and 30); dh[j] = H - (v >> 16)
+// (255: not representable, the counting kernel reads col[b] instead).
 __global__ void k_sfx_fill(const uint32_t* __restrict__ head, const uint32_t* __restrict__ arc, uint64_t m,
-                           const uint32_t* __restrict__ rend, uint2* __restrict__ rng,
+                           const uint32_t* __restrict__ rend, const uint32_t* __restrict__ nar,
+                           uint2* __restrict__ rng, uint8_t* __restrict__ dh, uint16_t* __restrict__ split,
                            uint32_t* __restrict__ sfx_cnt, unsigned long long* __restrict__ w_sfx) {
   const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = j < m;
@@ -337,66 +385,27 @@ __global__ void k_sfx_fill(const uint32_t* __restrict__ head, const uint32_t* __
   uint64_t len = 0;
   if (valid) {
     const uint32_t e = arc[j], end = rend[e];
-    rng[j] = make_uint2(e + 1, end);
+    uint32_t y = end;
+    if (nar) {
+      const uint32_t rec = nar[e];
+      y |= rec & 0xC0000000u;
+      dh[j] = (uint8_t)(rec >> 16);
+      split[j] = (uint16_t)rec;
+    }
+    rng[j] = make_uint2(e + 1, y);
     len = end - e - 1;
   }
-  const uint32_t lane = lane_id();
-  uint64_t sum = len;
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const uint64_t t = __shfl_up(sum, o);
-    const uint32_t kv = __shfl_up(v, o);
-    if (lane >= (uint32_t)o && kv == v) sum += t;
-  }
-  const uint32_t nv = __shfl_down(v, 1);
-  const uint64_t vmask = __ballot(valid);
-  const uint64_t above = (lane == 63) ? 0ull : (vmask >> (lane + 1));
-  const bool tail = valid && (!(above & 1ull) || nv != v);
+  uint64_t sum = 0;
+  const bool tail = wave_segment_tail(v, valid, len, &sum);
   const uint32_t run = wave_run_length(v, valid);
   if (run) atomicAdd(&sfx_cnt[v], run);
   if (tail && sum) atomicAdd(&w_sfx[v], (unsigned long long)sum);
 }
 
-// Low 16 bits of every id, and the narrow flag (bit 31 of the end) on every
-// non-empty range whose ids share their high 16 bits.
+// Low 16 bits of every id (narrow wedge ranges are streamed from these).
 __global__ void k_col16(const uint32_t* __restrict__ col, uint64_t m, uint16_t* __restrict__ col16) {
   const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e < m) col16[e] = (uint16_t)(col[e] & 0xFFFFu);
-}
-
-// Wedge range k = [b, e) (the part of N+(u) after v, v = col[b - 1]) is
-// narrow when its ids share their high 16 bits H (end bit 31), or, shorter
-// than 2^16 ids, when they have high half H up to position b + split[k] and
-// H + 1 from there (end bits 31 and 30). dh[k] = H - (v >> 16) (255: not
-// representable, the kernel reads col[b] instead).
-__global__ void k_flag_narrow(uint2* __restrict__ rng, uint64_t m, const uint32_t* __restrict__ col,
-                              uint8_t* __restrict__ dh, uint16_t* __restrict__ split) {
-  const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= m) return;
-  const uint2 r = rng[k];
-  uint32_t d = 0, sp = 0;
-  if (r.x < r.y) {
-    const uint32_t f = col[r.x] >> 16, l = col[r.y - 1] >> 16;
-    const bool two = l == f + 1 && r.y - r.x < 65536u;
-    if (f == l || two) {
-      if (two) {  // first position with high half l: col[lo] has f, col[hi] has l
-        uint32_t lo = r.x, hi = r.y - 1;
-        while (lo + 1 < hi) {
-          const uint32_t mid = lo + ((hi - lo) >> 1);
-          if ((col[mid] >> 16) == f)
-            lo = mid;
-          else
-            hi = mid;
-        }
-        sp = hi - r.x;
-      }
-      rng[k].y = r.y | 0x80000000u | (two ? 0x40000000u : 0u);
-      d = f - (col[r.x - 1] >> 16);  // ids after v: H >= v's high half
-      d = d < 255u ? d : 255u;
-    }
-  }
-  dh[k] = (uint8_t)d;
-  split[k] = (uint16_t)sp;
 }
 
 __global__ void k_fill_u64(unsigned long long* __restrict__ p, uint64_t n) {
@@ -500,8 +509,11 @@ void preprocess(tribit_ctx* ctx, uint64_t* d_keys_in, uint64_t nkeys, uint64_t n
   // 3b. optional degree relabelling: internal id = rank of (deg, id), so the
   // orientation below becomes the id order and the out-neighbours of
   // high-degree vertices are packed into a narrow id range at the top.
+  const uint32_t kb = bits_for(n);
   DevBuf<uint32_t> labels;
   labels.ctx = ctx;
+  DevBuf<uint64_t> okeys;  // oriented arcs, packed (u << kb) | v
+  okeys.ctx = ctx;
   if (ctx->relabel) {
     DevBuf<uint32_t> ids(ctx, n), deg_sorted(ctx, n), lab(ctx, n);
     labels = DevBuf<uint32_t>(ctx, n);
@@ -513,57 +525,151 @@ void preprocess(tribit_ctx* ctx, uint64_t* d_keys_in, uint64_t nkeys, uint64_t n
     std::swap(deg.p, deg_sorted.p);  // deg now indexed by internal id
     k_scatter_labels<<<grid_for(n, kBlock), kBlock, 0, s>>>(labels.p, n, lab.p);
     if (m) {
-      DevBuf<uint64_t> okeys(ctx, m);
-      k_orient_relabel<<<grid_for(m, kBlock), kBlock, 0, s>>>(uniq.p, m, lab.p, okeys.p);
-      std::swap(uniq.p, okeys.p);
+      okeys = DevBuf<uint64_t>(ctx, m);
+      k_orient_relabel<<<grid_for(m, kBlock), kBlock, 0, s>>>(uniq.p, m, lab.p, kb, okeys.p);
     }
+  } else if (m) {
+    okeys = DevBuf<uint64_t>(ctx, m);
+    k_orient<<<grid_for(m, kBlock), kBlock, 0, s>>>(uniq.p, m, deg.p, kb, okeys.p);
   }
 
-  // 4-5. orient, sort by (src, dst)
+  // 4-6. sort the arcs by (u, v) into the CSR
   DevBuf<uint32_t> col(ctx, m ? m : 1);
   DevBuf<uint32_t> row_ptr(ctx, n + 1);
   {
     DevBuf<uint32_t> odeg(ctx, n + 1);
     k_fill_u32<<<grid_for(n + 1, kBlock), kBlock, 0, s>>>(odeg.p, n + 1, 0u);
     if (m) {
-      DevBuf<uint64_t> okeys(ctx, m);
-      if (!ctx->relabel) k_orient<<<grid_for(m, kBlock), kBlock, 0, s>>>(uniq.p, m, deg.p, okeys.p);
-      else std::swap(uniq.p, okeys.p);
-      sort_keys(ctx, okeys.p, uniq.p, m, end_bit);
-      // 6. CSR
-      k_csr_split<<<grid_for(m, kBlock), kBlock, 0, s>>>(uniq.p, m, col.p, odeg.p);
+      sort_keys(ctx, okeys.p, uniq.p, m, 2 * kb);
+      okeys.reset();
+      k_csr_split<<<grid_for(m, kBlock), kBlock, 0, s>>>(uniq.p, m, kb, col.p, odeg.p);
     }
     with_temp(ctx, [&](void* t, size_t& b) {
       return rocprim::exclusive_scan(t, b, odeg.p, row_ptr.p, 0u, (size_t)(n + 1), rocprim::plus<uint32_t>(), s);
     });
   }
-  DevBuf<uint32_t> rend;  // row end of every arc (wedge ranges of relabelled graphs)
-  rend.ctx = ctx;
-  if (ctx->relabel && m) {
-    rend = DevBuf<uint32_t>(ctx, m);
-    k_arc_row_end<<<grid_for(m, kBlock), kBlock, 0, s>>>(uniq.p, m, row_ptr.p, rend.p);
-  }
-  uniq.reset();
 
-  // 7. per-row wedge counts and totals
+  // 7. per-row wedge counts and totals, one pass over the arcs; relabelled
+  // graphs also get every arc's row end and the narrow record of its wedge range
   DevBuf<uint64_t> wrow(ctx, n);
   DevBuf<RowStats> st(ctx, 1);
   k_fill_u32<<<1, 64, 0, s>>>(reinterpret_cast<uint32_t*>(st.p), sizeof(RowStats) / 4, 0u);
-  k_row_stats<<<grid_for(n, kBlock), kBlock, 0, s>>>(row_ptr.p, col.p, n, wrow.p, st.p);
+  k_fill_u64<<<grid_for(n, kBlock), kBlock, 0, s>>>(reinterpret_cast<unsigned long long*>(wrow.p), n);
+  const bool narrow = ctx->relabel && m && m < (1ull << 30);
+  DevBuf<uint32_t> rend, nar;  // by arc: row end, narrow record (wedge ranges of relabelled graphs)
+  rend.ctx = nar.ctx = ctx;
+  if (ctx->relabel && m) rend = DevBuf<uint32_t>(ctx, m);
+  if (narrow) nar = DevBuf<uint32_t>(ctx, m);
+  if (m)
+    k_arc_stats<<<grid_for(m, kBlock), kBlock, 0, s>>>(uniq.p, m, kb, row_ptr.p, col.p,
+                                                       reinterpret_cast<unsigned long long*>(wrow.p), st.p, rend.p,
+                                                       nar.p);
+  k_row_flags<<<grid_for(n, kBlock), kBlock, 0, s>>>(row_ptr.p, n, st.p);
+  uniq.reset();
   RowStats hst;
   TB_HIP(hipMemcpyAsync(&hst, st.p, sizeof(hst), hipMemcpyDeviceToHost, s));
 
-  // 8. range lists (forward-only, and direction-split)
-  DevBuf<uint2> rng_all(ctx, m ? m : 1), rng_dir(ctx, m ? m : 1);
+  // 8. wedge ranges (relabelled graphs): in-arcs of every v, each with its
+  // suffix range; narrow ranges flagged, and col16 (the low 16 bits of every
+  // id, padded to whole dwords plus one) built for streaming them
+  DevBuf<uint2> rng_sfx;
+  DevBuf<uint32_t> sfx_ptr;
+  DevBuf<uint64_t> w_sfx;
+  DevBuf<uint32_t> col16;
+  DevBuf<uint8_t> sfx_dh;
+  DevBuf<uint16_t> sfx_split;
+  rng_sfx.ctx = sfx_ptr.ctx = w_sfx.ctx = ctx;
+  if (ctx->relabel) {
+    rng_sfx = DevBuf<uint2>(ctx, m ? m : 1);
+    sfx_ptr = DevBuf<uint32_t>(ctx, n + 1);
+    w_sfx = DevBuf<uint64_t>(ctx, n);
+    if (narrow) {
+      sfx_dh = DevBuf<uint8_t>(ctx, m);
+      sfx_split = DevBuf<uint16_t>(ctx, m);
+    }
+    k_fill_u64<<<grid_for(n, kBlock), kBlock, 0, s>>>(reinterpret_cast<unsigned long long*>(w_sfx.p), n);
+    DevBuf<uint32_t> cnt(ctx, n + 1);
+    k_fill_u32<<<grid_for(n + 1, kBlock), kBlock, 0, s>>>(cnt.p, n + 1, 0u);
+    if (m) {
+      DevBuf<uint32_t> arcs(ctx, m), head(ctx, m), arc_sorted(ctx, m);
+      k_iota_u32<<<grid_for(m, kBlock), kBlock, 0, s>>>(arcs.p, m);
+      with_temp(ctx, [&](void* t, size_t& b) {
+        return rocprim::radix_sort_pairs(t, b, col.p, head.p, arcs.p, arc_sorted.p, (size_t)m, 0u, bits_for(n), s);
+      });
+      arcs.reset();
+      k_sfx_fill<<<grid_for(m, kBlock), kBlock, 0, s>>>(head.p, arc_sorted.p, m, rend.p, nar.p, rng_sfx.p, sfx_dh.p,
+                                                        sfx_split.p, cnt.p,
+                                                        reinterpret_cast<unsigned long long*>(w_sfx.p));
+    }
+    with_temp(ctx, [&](void* t, size_t& b) {
+      return rocprim::exclusive_scan(t, b, cnt.p, sfx_ptr.p, 0u, (size_t)(n + 1), rocprim::plus<uint32_t>(), s);
+    });
+    if (narrow) {
+      const uint64_t words = (m + 1) / 2 + 1;
+      col16 = DevBuf<uint32_t>(ctx, words);
+      k_fill_u32<<<1, 64, 0, s>>>(col16.p + words - 2, 2, 0u);
+      k_col16<<<grid_for(m, kBlock), kBlock, 0, s>>>(col.p, m, reinterpret_cast<uint16_t*>(col16.p));
+    }
+  }
+  rend.reset();
+  nar.reset();
+
+  // the graph owns its buffers from here on (freed by tribit_graph_free, also on error)
+  g->row_ptr = row_ptr.release();
+  g->col = col.release();
+  g->deg = deg.release();
+  g->labels = labels.release();
+  g->relabeled = ctx->relabel ? 1 : 0;
+  g->wrow = wrow.release();
+  g->rng_sfx = rng_sfx.release();
+  g->col16 = reinterpret_cast<uint16_t*>(col16.release());
+  g->sfx_dh = sfx_dh.release();
+  g->sfx_split = sfx_split.release();
+  g->sfx_ptr = sfx_ptr.release();
+  g->w_sfx = w_sfx.release();
+  g->bytes = 4 * (n + 1) + 4 * (m ? m : 1) + 4 * n + 8 * n + (g->labels ? 4 * n : 0) +
+             (g->rng_sfx ? 8 * (m ? m : 1) + 4 * (n + 1) + 8 * n : 0) + (g->col16 ? 4 * ((m + 1) / 2 + 1) + 3 * m : 0);
+  // a graph without wedge ranges counts on the direction-split layout by default: built here
+  if (!g->rng_sfx) build_layout(ctx, g, TRIBIT_SCHED_DIRECTED);
+  TB_HIP(hipEventRecord(ctx->ev[1], s));
+  TB_HIP(hipEventSynchronize(ctx->ev[1]));
+  TB_HIP(hipGetLastError());
+  float ms = 0.f;
+  TB_HIP(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
+  g->ms_prep = ms;
+  g->wplus = hst.wplus;
+  g->rows_active = hst.rows_active;
+  g->max_outdeg = hst.max_outdeg;
+}
+
+// ------------------------------------------------------------ range layouts
+// The layouts a schedule streams beside the wedge ranges: rng_all (PLAIN,
+// every arc forward in CSR order) and rng_dir (DIRECTED). Preprocessing builds
+// the graph's default one (wedge ranges for relabelled graphs, else DIRECTED);
+// the others are built on their first use, outside the count's timing, and
+// their device time is kept per layout (tribit_graph_info.ms_layout).
+void build_layout(tribit_ctx* ctx, tribit_graph* g, int layout) {
+  hipStream_t s = ctx->stream;
+  const uint64_t n = g->n, m = g->m;
+  if (layout == TRIBIT_SCHED_PLAIN) {
+    DevBuf<uint2> rng_all(ctx, m ? m : 1);
+    if (m) k_ranges_all<<<grid_for(m, kBlock), kBlock, 0, s>>>(g->row_ptr, g->col, m, rng_all.p);
+    TB_HIP(hipGetLastError());
+    g->rng_all = rng_all.release();
+    g->bytes += 8 * (m ? m : 1);
+    return;
+  }
+  if (layout != TRIBIT_SCHED_DIRECTED) fail(TRIBIT_ERR_ARG, "no such range layout");
+  const uint32_t end_bit = 32 + bits_for(n);
+  DevBuf<uint2> rng_dir(ctx, m ? m : 1);
   DevBuf<uint32_t> pid_dir(ctx, m ? m : 1);
   DevBuf<uint32_t> fw_ptr(ctx, n + 1), rv_ptr(ctx, n + 1);
   DevBuf<uint64_t> w_fw(ctx, n), w_rv(ctx, n);
   uint32_t m_fwd = 0;
   k_fill_u64<<<grid_for(n, kBlock), kBlock, 0, s>>>(reinterpret_cast<unsigned long long*>(w_rv.p), n);
   if (m) {
-    k_ranges_all<<<grid_for(m, kBlock), kBlock, 0, s>>>(row_ptr.p, col.p, m, rng_all.p);
     DevBuf<uint32_t> fw_cnt(ctx, n + 1), rv_cnt(ctx, n + 1), rv_off(ctx, n + 1);
-    k_dir_count<<<grid_for(n + 1, kBlock), kBlock, 0, s>>>(row_ptr.p, col.p, n, fw_cnt.p, rv_cnt.p, w_fw.p);
+    k_dir_count<<<grid_for(n + 1, kBlock), kBlock, 0, s>>>(g->row_ptr, g->col, n, fw_cnt.p, rv_cnt.p, w_fw.p);
     with_temp(ctx, [&](void* t, size_t& b) {
       return rocprim::exclusive_scan(t, b, fw_cnt.p, fw_ptr.p, 0u, (size_t)(n + 1), rocprim::plus<uint32_t>(), s);
     });
@@ -574,13 +680,13 @@ void preprocess(tribit_ctx* ctx, uint64_t* d_keys_in, uint64_t nkeys, uint64_t n
     ctx->sync();
     const uint64_t m_rev = m - m_fwd;
     DevBuf<uint64_t> rv_keys(ctx, m_rev ? m_rev : 1);
-    k_dir_fill<<<grid_for(n, kBlock), kBlock, 0, s>>>(row_ptr.p, col.p, n, fw_ptr.p, rv_off.p, rng_dir.p,
+    k_dir_fill<<<grid_for(n, kBlock), kBlock, 0, s>>>(g->row_ptr, g->col, n, fw_ptr.p, rv_off.p, rng_dir.p,
                                                       pid_dir.p, rv_keys.p);
     k_fill_u32<<<grid_for(n + 1, kBlock), kBlock, 0, s>>>(rv_cnt.p, n + 1, 0u);
     if (m_rev) {
       DevBuf<uint64_t> rv_sorted(ctx, m_rev);
       sort_keys(ctx, rv_keys.p, rv_sorted.p, m_rev, end_bit);
-      k_rv_fill<<<grid_for(m_rev, kBlock), kBlock, 0, s>>>(rv_sorted.p, m_rev, row_ptr.p, rng_dir.p + m_fwd,
+      k_rv_fill<<<grid_for(m_rev, kBlock), kBlock, 0, s>>>(rv_sorted.p, m_rev, g->row_ptr, rng_dir.p + m_fwd,
                                                            pid_dir.p + m_fwd, rv_cnt.p,
                                                            reinterpret_cast<unsigned long long*>(w_rv.p));
     }
@@ -593,83 +699,194 @@ void preprocess(tribit_ctx* ctx, uint64_t* d_keys_in, uint64_t nkeys, uint64_t n
     k_fill_u32<<<grid_for(n + 1, kBlock), kBlock, 0, s>>>(rv_ptr.p, n + 1, 0u);
     k_fill_u64<<<grid_for(n, kBlock), kBlock, 0, s>>>(reinterpret_cast<unsigned long long*>(w_fw.p), n);
   }
-  // 9. wedge ranges (relabelled graphs): in-arcs of every v, each with its suffix range
-  DevBuf<uint2> rng_sfx;
-  DevBuf<uint32_t> sfx_ptr;
-  DevBuf<uint64_t> w_sfx;
-  rng_sfx.ctx = sfx_ptr.ctx = w_sfx.ctx = ctx;
-  if (ctx->relabel) {
-    rng_sfx = DevBuf<uint2>(ctx, m ? m : 1);
-    sfx_ptr = DevBuf<uint32_t>(ctx, n + 1);
-    w_sfx = DevBuf<uint64_t>(ctx, n);
-    k_fill_u64<<<grid_for(n, kBlock), kBlock, 0, s>>>(reinterpret_cast<unsigned long long*>(w_sfx.p), n);
-    DevBuf<uint32_t> cnt(ctx, n + 1);
-    k_fill_u32<<<grid_for(n + 1, kBlock), kBlock, 0, s>>>(cnt.p, n + 1, 0u);
-    if (m) {
-      DevBuf<uint32_t> arcs(ctx, m), head(ctx, m), arc_sorted(ctx, m);
-      k_iota_u32<<<grid_for(m, kBlock), kBlock, 0, s>>>(arcs.p, m);
-      with_temp(ctx, [&](void* t, size_t& b) {
-        return rocprim::radix_sort_pairs(t, b, col.p, head.p, arcs.p, arc_sorted.p, (size_t)m, 0u, bits_for(n), s);
-      });
-      arcs.reset();
-      k_sfx_fill<<<grid_for(m, kBlock), kBlock, 0, s>>>(head.p, arc_sorted.p, m, rend.p, rng_sfx.p, cnt.p,
-                                                        reinterpret_cast<unsigned long long*>(w_sfx.p));
-    }
-    with_temp(ctx, [&](void* t, size_t& b) {
-      return rocprim::exclusive_scan(t, b, cnt.p, sfx_ptr.p, 0u, (size_t)(n + 1), rocprim::plus<uint32_t>(), s);
-    });
-  }
-  rend.reset();
-  // 10. narrow wedge ranges (ids sharing their high 16 bits) are flagged and
-  // col16, the low 16 bits of every id, is built for streaming them (padded
-  // to whole dwords plus one)
-  DevBuf<uint32_t> col16;
-  DevBuf<uint8_t> sfx_dh;
-  DevBuf<uint16_t> sfx_split;
-  if (ctx->relabel && m && m < (1ull << 30)) {
-    sfx_dh = DevBuf<uint8_t>(ctx, m);
-    sfx_split = DevBuf<uint16_t>(ctx, m);
-    k_flag_narrow<<<grid_for(m, kBlock), kBlock, 0, s>>>(rng_sfx.p, m, col.p, sfx_dh.p, sfx_split.p);
-    const uint64_t words = (m + 1) / 2 + 1;
-    col16 = DevBuf<uint32_t>(ctx, words);
-    k_fill_u32<<<1, 64, 0, s>>>(col16.p + words - 2, 2, 0u);
-    k_col16<<<grid_for(m, kBlock), kBlock, 0, s>>>(col.p, m, reinterpret_cast<uint16_t*>(col16.p));
-  }
-
-  // the graph owns its buffers from here on (freed by tribit_graph_free, also on error)
-  g->row_ptr = row_ptr.release();
-  g->col = col.release();
-  g->deg = deg.release();
-  g->labels = labels.release();
-  g->relabeled = ctx->relabel ? 1 : 0;
-  g->wrow = wrow.release();
+  TB_HIP(hipGetLastError());
   g->m_fwd = m_fwd;
-  g->rng_all = rng_all.release();
   g->rng_dir = rng_dir.release();
   g->pid_dir = pid_dir.release();
   g->fw_ptr = fw_ptr.release();
   g->rv_ptr = rv_ptr.release();
   g->w_fw = w_fw.release();
   g->w_rv = w_rv.release();
-  g->rng_sfx = rng_sfx.release();
-  g->col16 = reinterpret_cast<uint16_t*>(col16.release());
-  g->sfx_dh = sfx_dh.release();
-  g->sfx_split = sfx_split.release();
+  g->bytes += 12 * (m ? m : 1) + 8 * (n + 1) + 16 * n;
+}
 
-  g->sfx_ptr = sfx_ptr.release();
-  g->w_sfx = w_sfx.release();
-  g->bytes = 4 * (n + 1) * 3 + 4 * (m ? m : 1) + 20 * (m ? m : 1) + 4 * n + 8 * n * 3 + (g->labels ? 4 * n : 0) +
-             (g->rng_sfx ? 8 * (m ? m : 1) + 4 * (n + 1) + 8 * n : 0) + (g->col16 ? 4 * ((m + 1) / 2 + 1) + 3 * m : 0);
-  g->tiles64 = m ? count_tiles(ctx, g) : 0;
-  TB_HIP(hipEventRecord(ctx->ev[1], s));
-  TB_HIP(hipEventSynchronize(ctx->ev[1]));
-  TB_HIP(hipGetLastError());
+bool layout_built(const tribit_graph* g, int layout) {
+  if (layout == TRIBIT_SCHED_PLAIN) return g->rng_all != nullptr;
+  if (layout == TRIBIT_SCHED_DIRECTED) return g->rng_dir != nullptr;
+  if (layout == TRIBIT_SCHED_WEDGE) return g->rng_sfx != nullptr;
+  return false;
+}
+
+void ensure_layout(tribit_ctx* ctx, tribit_graph* g, int layout) {
+  if (layout_built(g, layout)) return;
+  if (layout == TRIBIT_SCHED_WEDGE) fail(TRIBIT_ERR_ARG, "wedge ranges need a relabelled graph (TRIBIT_OPT_RELABEL)");
+  hipStream_t s = ctx->stream;
+  hipEvent_t a = nullptr, b = nullptr;
+  TB_HIP(hipEventCreate(&a));
+  TB_HIP(hipEventCreate(&b));
+  TB_HIP(hipEventRecord(a, s));
+  build_layout(ctx, g, layout);
+  TB_HIP(hipEventRecord(b, s));
+  TB_HIP(hipEventSynchronize(b));
   float ms = 0.f;
-  TB_HIP(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
-  g->ms_prep = ms;
-  g->wplus = hst.wplus;
+  TB_HIP(hipEventElapsedTime(&ms, a, b));
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  g->ms_layout[layout] = ms;
+}
+
+// ------------------------------------------- one rank's off-diagonal task
+// The reference's off-diagonal task (main6_multi.cu:595-751, task shapes
+// main.rs:186-220) counts the triangles a < b < c with (a, b) in `cols` and
+// (a, c), (b, c) in `idxs`. As a count of this library: the idxs edges x -> c
+// (x the smaller id) form the CSR H; every cols edge {a, b} is one streamed
+// list, H's row of one endpoint against the set of H's row of the other, the
+// endpoint with the longer row owning the set. The lists go into the reverse
+// part of a direction-split layout (no forward part), which the DIRECTED
+// schedule counts unchanged.
+
+// Owner-first keys (o << 32) | s of the cols edges: o the endpoint with the
+// longer H row (ties: the larger id); sentinels stay sentinels.
+__global__ void k_offdiag_keys(const uint64_t* __restrict__ keys, uint64_t nk, const uint32_t* __restrict__ hrp,
+                               uint64_t* __restrict__ out) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nk) return;
+  const uint64_t k = keys[i];
+  if (k == kSentinel) {
+    out[i] = kSentinel;
+    return;
+  }
+  const uint32_t a = (uint32_t)(k >> 32), b = (uint32_t)k;
+  const uint32_t la = hrp[a + 1] - hrp[a], lb = hrp[b + 1] - hrp[b];
+  out[i] = la > lb ? (((uint64_t)a << 32) | b) : (((uint64_t)b << 32) | a);
+}
+
+__global__ void k_sum_u64(const unsigned long long* __restrict__ v, uint64_t n, unsigned long long* __restrict__ out) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t s = wave_sum_u64(i < n ? v[i] : 0ull);
+  if (lane_id() == 0 && s) atomicAdd(out, (unsigned long long)s);
+}
+
+// Sorted distinct keys of in[0, nk) (the sentinel dropped) into out; returns
+// their number. Sets *bad when a key is not (min < max < n).
+static uint64_t sort_unique(tribit_ctx* ctx, const uint64_t* in, uint64_t nk, uint64_t n, uint64_t* out, bool* bad) {
+  hipStream_t s = ctx->stream;
+  if (!nk) return 0;
+  if (nk > 0xFFFFFFFEull) fail(TRIBIT_ERR_TOO_LARGE, "more than 2^32-2 keys in one task set");
+  DevBuf<uint32_t> flag(ctx, 1);
+  k_fill_u32<<<1, 64, 0, s>>>(flag.p, 1, 0u);
+  k_validate<<<grid_for(nk, kBlock), kBlock, 0, s>>>(in, nk, n, flag.p);
+  DevBuf<uint64_t> sorted(ctx, nk);
+  sort_keys(ctx, in, sorted.p, nk, 32 + bits_for(n));
+  DevBuf<uint32_t> head(ctx, nk + 1), pos(ctx, nk + 1);
+  k_head_flags<<<grid_for(nk, kBlock), kBlock, 0, s>>>(sorted.p, nk, head.p);
+  k_fill_u32<<<1, 64, 0, s>>>(head.p + nk, 1, 0u);
+  with_temp(ctx, [&](void* t, size_t& b) {
+    return rocprim::exclusive_scan(t, b, head.p, pos.p, 0u, (size_t)(nk + 1), rocprim::plus<uint32_t>(), s);
+  });
+  k_scatter_heads<<<grid_for(nk, kBlock), kBlock, 0, s>>>(sorted.p, nk, pos.p, out);
+  uint32_t cnt = 0, f = 0;
+  TB_HIP(hipMemcpyAsync(&cnt, pos.p + nk, 4, hipMemcpyDeviceToHost, s));
+  TB_HIP(hipMemcpyAsync(&f, flag.p, 4, hipMemcpyDeviceToHost, s));
+  ctx->sync();
+  *bad = *bad || f != 0;
+  return cnt;
+}
+
+void build_offdiag(tribit_ctx* ctx, const uint64_t* d_cols, uint64_t ncols, const uint64_t* d_idxs, uint64_t nidxs,
+                   uint64_t n, tribit_graph* g) {
+  if (n == 0 || n > 0xFFFFFFFEull) fail(TRIBIT_ERR_TOO_LARGE, "vertex count must be in [1, 2^32-2]");
+  hipStream_t s = ctx->stream;
+  g->ctx = ctx;
+  g->n = n;
+  g->entries = ncols + nidxs;
+  bool bad = false;
+  // H: the idxs edges as a CSR by the smaller id
+  DevBuf<uint64_t> hk(ctx, nidxs ? nidxs : 1);
+  const uint64_t mh = sort_unique(ctx, d_idxs, nidxs, n, hk.p, &bad);
+  if (mh > 0xFFFFFFFFull) fail(TRIBIT_ERR_TOO_LARGE, "more than 2^32-1 index edges in one task");
+  DevBuf<uint32_t> col(ctx, mh ? mh : 1), row_ptr(ctx, n + 1);
+  {
+    DevBuf<uint32_t> odeg(ctx, n + 1);
+    k_fill_u32<<<grid_for(n + 1, kBlock), kBlock, 0, s>>>(odeg.p, n + 1, 0u);
+    if (mh) k_csr_split<<<grid_for(mh, kBlock), kBlock, 0, s>>>(hk.p, mh, 32, col.p, odeg.p);
+    with_temp(ctx, [&](void* t, size_t& b) {
+      return rocprim::exclusive_scan(t, b, odeg.p, row_ptr.p, 0u, (size_t)(n + 1), rocprim::plus<uint32_t>(), s);
+    });
+  }
+  hk.reset();
+  // the cols edges, owner first, distinct, as streamed ranges of H rows grouped by owner
+  uint64_t mc = 0;
+  DevBuf<uint64_t> ck(ctx, ncols ? ncols : 1);
+  if (ncols) {
+    DevBuf<uint32_t> flag(ctx, 1);
+    k_fill_u32<<<1, 64, 0, s>>>(flag.p, 1, 0u);
+    k_validate<<<grid_for(ncols, kBlock), kBlock, 0, s>>>(d_cols, ncols, n, flag.p);
+    uint32_t f = 0;
+    TB_HIP(hipMemcpyAsync(&f, flag.p, 4, hipMemcpyDeviceToHost, s));
+    ctx->sync();
+    if (f) fail(TRIBIT_ERR_RANGE, "edge key is not (min < max < n)");
+    DevBuf<uint64_t> ok(ctx, ncols);
+    k_offdiag_keys<<<grid_for(ncols, kBlock), kBlock, 0, s>>>(d_cols, ncols, row_ptr.p, ok.p);
+    // owner-first keys are no longer (min < max): dedup without the range check
+    DevBuf<uint64_t> sorted(ctx, ncols);
+    sort_keys(ctx, ok.p, sorted.p, ncols, 32 + bits_for(n));
+    ok.reset();
+    DevBuf<uint32_t> head(ctx, ncols + 1), pos(ctx, ncols + 1);
+    k_head_flags<<<grid_for(ncols, kBlock), kBlock, 0, s>>>(sorted.p, ncols, head.p);
+    k_fill_u32<<<1, 64, 0, s>>>(head.p + ncols, 1, 0u);
+    with_temp(ctx, [&](void* t, size_t& b) {
+      return rocprim::exclusive_scan(t, b, head.p, pos.p, 0u, (size_t)(ncols + 1), rocprim::plus<uint32_t>(), s);
+    });
+    k_scatter_heads<<<grid_for(ncols, kBlock), kBlock, 0, s>>>(sorted.p, ncols, pos.p, ck.p);
+    uint32_t cnt = 0;
+    TB_HIP(hipMemcpyAsync(&cnt, pos.p + ncols, 4, hipMemcpyDeviceToHost, s));
+    ctx->sync();
+    mc = cnt;
+  }
+  if (bad) fail(TRIBIT_ERR_RANGE, "edge key is not (min < max < n)");
+  DevBuf<uint2> rng(ctx, mc ? mc : 1);
+  DevBuf<uint32_t> pid(ctx, mc ? mc : 1), fw_ptr(ctx, n + 1), rv_ptr(ctx, n + 1);
+  DevBuf<uint64_t> w_fw(ctx, n), w_rv(ctx, n);
+  {
+    DevBuf<uint32_t> rv_cnt(ctx, n + 1);
+    k_fill_u32<<<grid_for(n + 1, kBlock), kBlock, 0, s>>>(rv_cnt.p, n + 1, 0u);
+    k_fill_u32<<<grid_for(n + 1, kBlock), kBlock, 0, s>>>(fw_ptr.p, n + 1, 0u);
+    k_fill_u64<<<grid_for(n, kBlock), kBlock, 0, s>>>(reinterpret_cast<unsigned long long*>(w_fw.p), n);
+    k_fill_u64<<<grid_for(n, kBlock), kBlock, 0, s>>>(reinterpret_cast<unsigned long long*>(w_rv.p), n);
+    if (mc)
+      k_rv_fill<<<grid_for(mc, kBlock), kBlock, 0, s>>>(ck.p, mc, row_ptr.p, rng.p, pid.p, rv_cnt.p,
+                                                        reinterpret_cast<unsigned long long*>(w_rv.p));
+    with_temp(ctx, [&](void* t, size_t& b) {
+      return rocprim::exclusive_scan(t, b, rv_cnt.p, rv_ptr.p, 0u, (size_t)(n + 1), rocprim::plus<uint32_t>(), s);
+    });
+  }
+  ck.reset();
+  // totals: streamed elements, owner rows
+  DevBuf<RowStats> st(ctx, 1);
+  k_fill_u32<<<1, 64, 0, s>>>(reinterpret_cast<uint32_t*>(st.p), sizeof(RowStats) / 4, 0u);
+  k_sum_u64<<<grid_for(n, kBlock), kBlock, 0, s>>>(reinterpret_cast<const unsigned long long*>(w_rv.p), n,
+                                                   &st.p->wplus);
+  k_row_flags<<<grid_for(n, kBlock), kBlock, 0, s>>>(row_ptr.p, n, st.p);
+  RowStats hst;
+  TB_HIP(hipMemcpyAsync(&hst, st.p, sizeof(hst), hipMemcpyDeviceToHost, s));
+  ctx->sync();
+  g->m = mh;
+  g->row_ptr = row_ptr.release();
+  g->col = col.release();
+  g->relabeled = 0;
+  g->m_fwd = 0;
+  g->rng_dir = rng.release();
+  g->pid_dir = pid.release();
+  g->fw_ptr = fw_ptr.release();
+  g->rv_ptr = rv_ptr.release();
+  g->w_fw = w_fw.release();
+  g->w_rv = w_rv.release();
+  g->wplus = hst.wplus;  // streamed elements of the task
   g->rows_active = hst.rows_active;
   g->max_outdeg = hst.max_outdeg;
+  g->offdiag_lists = mc;
+  g->bytes = 4 * (n + 1) * 3 + 4 * (mh ? mh : 1) + 12 * (mc ? mc : 1) + 16 * n;
 }
 
 // Releases every device buffer of a graph (the struct stays).
