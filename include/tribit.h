@@ -82,6 +82,7 @@ typedef enum {
 } tribit_sched;
 
 /* tribit_count_opts.flags */
+#define TRIBIT_CANDIDATES 5               /* plans the learned schedule judges (tribit_count_info.candidate) */
 #define TRIBIT_COUNT_FORCE_LINEAR_PROBE 1 /* test hook: skip the cuckoo sets, use the fallback tables */
 #define TRIBIT_COUNT_FORCE_GLOBAL_LOADS 2 /* test hook: stream with global loads (the path for col >= 4 GiB) */
 #define TRIBIT_COUNT_NO_TILES 4           /* never stream bit tiles (block launches stream elements) */
@@ -93,6 +94,9 @@ typedef enum {
                                              from the 16-bit low halves, two ids per load) */
 #define TRIBIT_COUNT_FORCE_MULTI 128      /* test hook: every launch of the 64-slot wave kernel runs as multi-task
                                              waves, whatever its task count */
+#define TRIBIT_COUNT_DENSE_BLOCKS 512     /* WEDGE: dense diagonal blocks (id ranges whose induced subgraph is
+                                             dense, e.g. the blocks of a block model under TRIBIT_OPT_RELABEL 2)
+                                             are counted on the matrix cores, the rest streamed; no top core */
 #define TRIBIT_COUNT_TILES32 256          /* launches that stream bit tiles stream 32-column tiles (uint32
                                              masks) instead of 64-column ones */
 #define TRIBIT_COUNT_NO_JUDGE 64          /* LEARNED: run the density-threshold heuristic's candidate (preferred
@@ -125,12 +129,15 @@ typedef struct {
   uint64_t dense_core;     /* core vertices counted by the int8 MFMA kernel (0: none) */
   double ms_dense;         /* device time of the core launch (median) */
   int32_t candidate;       /* LEARNED: plan that ran (0 cost model, 1 heuristic, 2 heuristic on the
-                              directed layout, 3 textbook plain); -1 for the fixed schedules */
+                              directed layout, 3 textbook plain, 4 heuristic with dense diagonal blocks on the
+                              matrix cores); -1 for the fixed schedules */
   int32_t layout;          /* range layout that ran (TRIBIT_SCHED_PLAIN / DIRECTED / WEDGE) */
   double ms_judge;         /* LEARNED: device time of the candidate trials this call ran (first call) */
-  double ms_candidates[4]; /* LEARNED: judged time of each candidate (0 = not a candidate for this graph) */
+  double ms_candidates[TRIBIT_CANDIDATES]; /* LEARNED: judged time of each candidate (0 = not a candidate) */
   int32_t slot_kind[8];    /* kernel of each launch: 0 wave/64-slot set, 1 wave/512-slot set, 2 block, 3 block
                               streaming bit tiles, 4 multi-task wave (eight 64-slot sets) */
+  uint32_t dense_blocks;   /* dense diagonal blocks counted on the matrix cores (their vertices in dense_core) */
+  uint32_t reserved1;
 } tribit_count_info;
 
 /* Work assigned to each kernel variant by a scheduler, in the units of the
@@ -158,7 +165,10 @@ int tribit_shutdown(tribit_ctx* ctx);
 int tribit_set_alloc_cap(tribit_ctx* ctx, uint64_t bytes);
 /* Context options (tribit_set_option). */
 #define TRIBIT_OPT_RELABEL 1 /* 1 (default): preprocessing renumbers vertices by (degree, id)
-                                rank; 0: input ids kept. Counts are the same either way. */
+                                rank; 2: by (octave of the degree, id) rank, so runs of consecutive
+                                input ids of similar degree (the blocks of a block model) stay runs
+                                and can be counted as dense diagonal blocks; 0: input ids kept.
+                                Counts are the same either way. */
 #define TRIBIT_OPT_BATCH_PART_KEYS 2 /* input keys per batch part (default 2^30) */
 int tribit_set_option(tribit_ctx* ctx, int option, int64_t value);
 /* Peak bytes held from the memory pool since tribit_init. */

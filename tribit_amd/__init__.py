@@ -62,6 +62,7 @@ NO_BITMAP = 16
 WIDE_IDS = 32
 NO_JUDGE = 64
 FORCE_MULTI = 128
+DENSE_BLOCKS = 512
 OPT_RELABEL = 1
 OPT_BATCH_PART_KEYS = 2
 
@@ -74,7 +75,8 @@ class CountInfo(ctypes.Structure):
         ("ms_by_variant", ctypes.c_double * 8), ("fallback_tasks", ctypes.c_uint64),
         ("group_width", ctypes.c_int32 * 8), ("dense_core", ctypes.c_uint64), ("ms_dense", ctypes.c_double),
         ("candidate", ctypes.c_int32), ("layout", ctypes.c_int32), ("ms_judge", ctypes.c_double),
-        ("ms_candidates", ctypes.c_double * 4), ("slot_kind", ctypes.c_int32 * 8),
+        ("ms_candidates", ctypes.c_double * 5), ("slot_kind", ctypes.c_int32 * 8),
+        ("dense_blocks", ctypes.c_uint32), ("reserved1", ctypes.c_uint32),
     ]
 
 
@@ -342,6 +344,7 @@ class CountResult:
     ms_judge: float = 0.0
     ms_candidates: tuple = ()
     slot_kind: tuple = ()
+    dense_blocks: int = 0
 
 
 class Mlp:
@@ -548,7 +551,7 @@ class Graph:
                            list(ci.tasks_by_variant)[:max(nv, 4)], bool(ci.repeats_agree),
                            list(ci.ms_by_variant)[:nv], ci.fallback_tasks, tuple(ci.group_width)[:nv],
                            ci.dense_core, ci.ms_dense, ci.candidate, ci.layout, ci.ms_judge,
-                           tuple(ci.ms_candidates), tuple(ci.slot_kind)[:nv])
+                           tuple(ci.ms_candidates), tuple(ci.slot_kind)[:nv], ci.dense_blocks)
 
     def variant_stats(self, sched: str = "plain") -> dict:
         vs = VariantStats()
@@ -649,9 +652,11 @@ class Context:
     def set_batch_part_keys(self, nkeys: int) -> None:
         _check(self.lib.tribit_set_option(self.h, OPT_BATCH_PART_KEYS, nkeys), "set_option")
 
-    def set_relabel(self, on: bool) -> None:
-        """Degree-rank relabelling of graphs built from now on (default on)."""
-        _check(self.lib.tribit_set_option(self.h, OPT_RELABEL, 1 if on else 0), "set_option")
+    def set_relabel(self, on) -> None:
+        """Relabelling of graphs built from now on: True / 1 degree rank (default), 2 octave classes of the
+        degree (keeps runs of consecutive ids of similar degree together), False / 0 input ids."""
+        mode = 1 if on is True else 0 if on is False else int(on)
+        _check(self.lib.tribit_set_option(self.h, OPT_RELABEL, mode), "set_option")
 
     def set_alloc_cap(self, nbytes: int) -> None:
         _check(self.lib.tribit_set_alloc_cap(self.h, nbytes), "set_alloc_cap")

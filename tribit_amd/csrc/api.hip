@@ -120,7 +120,7 @@ int tribit_set_option(tribit_ctx* ctx, int option, int64_t value) {
   if (!ctx) return TRIBIT_ERR_ARG;
   switch (option) {
     case TRIBIT_OPT_RELABEL:
-      if (value != 0 && value != 1) return TRIBIT_ERR_ARG;
+      if (value != 0 && value != 1 && value != 2) return TRIBIT_ERR_ARG;
       ctx->relabel = (int)value;
       return TRIBIT_OK;
     case TRIBIT_OPT_BATCH_PART_KEYS:

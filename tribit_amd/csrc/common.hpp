@@ -73,7 +73,7 @@ struct tribit_ctx {
   struct Judged {
     uint64_t n, m, wplus, model;
     int relabeled, flags, core, choice;
-    double ms[4];
+    double ms[TRIBIT_CANDIDATES];
   };
   std::vector<Judged> judged;
   std::string last_error;
@@ -180,7 +180,6 @@ struct tribit_graph {
   uint16_t* col16 = nullptr;    // low 16 bits of every id, for narrow wedge ranges (relabelled, m < 2^30)
   uint8_t* sfx_dh = nullptr;    // per wedge range: its ids' high half minus its owner's (255: read col)
   uint16_t* sfx_split = nullptr; // per wedge range crossing one 64K-id boundary: offset of its first id past it
-  // 64-column bit tiles (tiles.hip), built on first use
   // bit tiles of the oriented rows (tiles.hip), one set per width (index 0:
   // 32 columns, uint32 masks; 1: 64 columns, uint64 masks), built on first use
   struct TileSet {
@@ -197,13 +196,27 @@ struct tribit_graph {
   uint64_t core_c = 0, core_kp = 0;
   uint32_t core_e0 = 0;    // row_ptr[n - core_c]: first arc of the core's rows
   int64_t core_pick = -1;  // the learned schedule's core size from core_choose (-1: not chosen yet)
+  // dense diagonal blocks (dense.hip): id ranges [s, e) whose induced subgraph
+  // is dense, counted on the matrix cores; found on first use
+  struct DenseBlocks {
+    bool found = false;           // detection ran
+    uint32_t nblocks = 0;
+    uint64_t vertices = 0;        // ids inside blocks
+    double macs = 0.0;            // int8 multiply-accumulates of one launch
+    int8_t* A = nullptr;          // the blocks' 0/1 upper-triangular matrices, each padded to kp x kp
+    uint4* work = nullptr;        // per workgroup: matrix offset (lo, hi), kp, tile pair I | J << 16
+    uint32_t nwork = 0;
+    uint2* rng = nullptr;         // m: wedge ranges with the in-block part of in-block arcs cut away
+    uint64_t* w = nullptr;        // n: streamed elements per owner over rng
+    uint64_t cut = 0;             // streamed elements the cut removed
+  } blocks;
   tb::BucketCache* bcache = nullptr;  // scheduler statistics (count.hip)
   // the learned schedule's judged candidate (count.hip), valid for the model
   // version, flags and core request it was judged under
   int judge_choice = -1;
   uint64_t judge_model = 0;
   int judge_flags = 0, judge_core = 0;
-  double judge_ms[4] = {0, 0, 0, 0};
+  double judge_ms[TRIBIT_CANDIDATES] = {};
   double ms_prep = 0.0;
   double ms_layout[4] = {0, 0, 0, 0};  // device time of range layouts built after preprocessing, by schedule id
   bool tiles_counted = false;          // tiles32 / tiles64 computed (on the first tribit_graph_get_info)
@@ -235,6 +248,7 @@ struct tribit_batch {
   hipGraph_t graph = nullptr;             // captured counting launches of all parts
   hipGraphExec_t exec = nullptr;
   int sched = -1, flags = 0;              // what the schedules / capture were built for
+  uint64_t model_version = 0;             // the context's cost model they were built with
   int gw[8] = {0};                        // tuned lane-group width per slot (0: the plan's)
   int waves[8] = {0};                     // tuned workgroups per CU per slot (0: occupancy limit)
   int tile_width = 0;                     // tuned: 0 the plans' kernels; 32 / 64 every block launch streams
@@ -311,6 +325,9 @@ void core_release(tribit_graph* g);
 void core_launch(tribit_ctx* ctx, const tribit_graph* g, unsigned long long* result);
 uint64_t core_choose(tribit_ctx* ctx, const tribit_graph* g);
 double core_macs(uint64_t kp);
+bool blocks_prepare(tribit_ctx* ctx, tribit_graph* g);  // false: no dense block found
+void blocks_release(tribit_graph* g);
+void blocks_launch(tribit_ctx* ctx, const tribit_graph* g, unsigned long long* result);
 // tiles.hip
 uint64_t count_tiles(tribit_ctx* ctx, tribit_graph* g, int width);
 void build_tiles(tribit_ctx* ctx, tribit_graph* g, int layout, int width);

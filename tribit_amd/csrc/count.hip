@@ -1016,6 +1016,12 @@ struct Plan {
   int g[kSlots];
   uint8_t table[kBuckets];  // bucket -> slot
   bool menu = false;        // the learned schedule's menu
+  // how make_schedule resolved it (what ran): layout, count flags, the top
+  // core's cut and the dense-block view
+  bool resolved = false;
+  int layout = 0, flags = 0;
+  uint32_t core_d0 = 0xFFFFFFFFu, core_cut = 0;
+  bool blocks = false;
 };
 
 struct BucketStats {
@@ -1303,8 +1309,10 @@ __global__ void k_put_table(PlanTable pt, uint8_t* out) {
   for (uint32_t i = threadIdx.x; i < (uint32_t)kBuckets; i += blockDim.x) out[i] = pt.t[i];
 }
 
+// With `blocks` (wedge layout, dense diagonal blocks on the matrix cores) the
+// wedge ranges are the blocks' copy with in-block arcs cut.
 static SchedView sched_view(const tribit_graph* g, int sched, uint32_t core_d0 = 0xFFFFFFFFu,
-                            uint32_t core_cut = 0) {
+                            uint32_t core_cut = 0, bool blocks = false) {
   SchedView v;
   v.row_ptr = g->row_ptr;
   v.min_d0 = 2;
@@ -1332,10 +1340,15 @@ static SchedView sched_view(const tribit_graph* g, int sched, uint32_t core_d0 =
     v.w_rv = g->w_rv;
     v.nlists = 2;
   }
+  if (blocks && sched == TRIBIT_SCHED_WEDGE && g->blocks.rng) {
+    v.rng = g->blocks.rng;
+    v.w_fw = v.w_rv = g->blocks.w;
+  }
   return v;
 }
 
-static const uint2* sched_ranges(const tribit_graph* g, int sched) {
+static const uint2* sched_ranges(const tribit_graph* g, int sched, bool blocks = false) {
+  if (blocks && sched == TRIBIT_SCHED_WEDGE && g->blocks.rng) return g->blocks.rng;
   return sched == TRIBIT_SCHED_PLAIN ? g->rng_all : sched == TRIBIT_SCHED_WEDGE ? g->rng_sfx : g->rng_dir;
 }
 
@@ -1407,22 +1420,32 @@ static void default_widths(int sched, int gw[4]) {
   }
 }
 
-// Bucket statistics depend only on the graph, the range layout and the core:
-// computed once per graph and kept (the last combination asked for).
+// Bucket statistics depend only on the graph, the range layout, the core and
+// the dense-block view: computed once per graph and kept (the last
+// combination the scheduler asked for). Also kept per graph: the resolved
+// plan of the last count of each schedule (what tribit_variant_stats reports).
 struct BucketCache {
-  int layout;
-  uint32_t core_d0;
+  int layout = -1;
+  uint32_t core_d0 = 0;
+  bool blocks = false;
   BucketStats st;
+  Plan last[4];  // by tribit_sched
 };
+
+static BucketCache* cache_of(tribit_graph* g) {
+  if (!g->bcache) g->bcache = new BucketCache;
+  return g->bcache;
+}
 
 void bucket_cache_free(tribit_graph* g) {
   delete g->bcache;
   g->bcache = nullptr;
 }
 
+// With keep = false the statistics are computed without replacing the kept ones.
 void bucket_stats(tribit_ctx* ctx, tribit_graph* g, int sched, BucketStats* out, uint32_t core_d0 = 0xFFFFFFFFu,
-                  uint32_t core_cut = 0) {
-  if (g->bcache && g->bcache->layout == sched && g->bcache->core_d0 == core_d0) {
+                  uint32_t core_cut = 0, bool blocks = false, bool keep = true) {
+  if (g->bcache && g->bcache->layout == sched && g->bcache->core_d0 == core_d0 && g->bcache->blocks == blocks) {
     *out = g->bcache->st;
     return;
   }
@@ -1431,8 +1454,8 @@ void bucket_stats(tribit_ctx* ctx, tribit_graph* g, int sched, BucketStats* out,
   DevBuf<unsigned long long> st(ctx, kStat * kBuckets);
   k_zero_u64<<<grid_for(kStat * kBuckets, kBlock), kBlock, 0, ctx->stream>>>(st.p, kStat * kBuckets);
   if (g->n && g->m)
-    k_bucket_stats<<<grid_for(g->n, 256), 256, 0, ctx->stream>>>(sched_view(g, sched, core_d0, core_cut), g->col,
-                                                                 g->n, st.p);
+    k_bucket_stats<<<grid_for(g->n, 256), 256, 0, ctx->stream>>>(sched_view(g, sched, core_d0, core_cut, blocks),
+                                                                 g->col, g->n, st.p);
   std::vector<unsigned long long> h(kStat * kBuckets);
   TB_HIP(hipMemcpyAsync(h.data(), st.p, 8 * h.size(), hipMemcpyDeviceToHost, ctx->stream));
   ctx->sync();
@@ -1444,10 +1467,12 @@ void bucket_stats(tribit_ctx* ctx, tribit_graph* g, int sched, BucketStats* out,
     out->eligible[b] = h[4 * kBuckets + b];
     out->sum_t[b] = h[5 * kBuckets + b];
   }
-  if (!g->bcache) g->bcache = new BucketCache;
-  g->bcache->layout = sched;
-  g->bcache->core_d0 = core_d0;
-  g->bcache->st = *out;
+  if (!keep) return;
+  BucketCache* c = cache_of(g);
+  c->layout = sched;
+  c->core_d0 = core_d0;
+  c->blocks = blocks;
+  c->st = *out;
 }
 
 // Feature vector of (bucket, menu slot); the same function builds training
@@ -1682,7 +1707,8 @@ static void launch_block_g(const LaunchArgs& a, int slot, uint32_t ntasks, int c
                            const uint2* rng, const TileSrc& ts, const uint4* tasks, uint64_t* blocks) {
   const bool buf = g->m * 4 <= kBufMax && !(a.att & kForceGlobal);
   // narrow ranges exist only in the wedge layout (col16 is built with it)
-  const bool n16 = buf && g->col16 && g->sfx_dh && g->sfx_split && rng == g->rng_sfx && !(a.att & kWideBit);
+  const bool n16 = buf && g->col16 && g->sfx_dh && g->sfx_split &&
+                   (rng == g->rng_sfx || (rng && rng == g->blocks.rng)) && !(a.att & kWideBit);
   if (ts.trng && ts.mask32) {
     if (n16)
       launch_block_t<LOGS, G, kU, true, 32, true>(a, slot, ntasks, cus, s, g, rng, ts, tasks, blocks);
@@ -1737,6 +1763,7 @@ struct Schedule {
   int gdef[kSlots] = {0};  // the plan's own widths (a batch's tuned widths override them)
   int kdef[kSlots] = {0};  // the plan's own kernel kinds (a batch's tuned tile width overrides block kinds)
   uint64_t core_c = 0;     // dense-core vertices counted on the matrix cores (0: none)
+  bool blocks = false;     // dense diagonal blocks counted on the matrix cores, ranges cut to match
   int layout = 0;
   const uint2* rng = nullptr;
   TileSrc ts{nullptr, nullptr, nullptr};
@@ -1786,6 +1813,10 @@ void make_schedule(tribit_ctx* ctx, tribit_graph* g, int layout, int flags, bool
   ensure_layout(ctx, g, layout);  // no-op when built (count() builds it before its timing starts)
   uint32_t core_d0 = 0xFFFFFFFFu, core_cut = 0;
   if (core_c > g->n || !g->relabeled) core_c = 0;
+  // dense diagonal blocks replace the top core (a block inside the core would
+  // count its triangles twice)
+  S.blocks = (flags & TRIBIT_COUNT_DENSE_BLOCKS) && layout == TRIBIT_SCHED_WEDGE && blocks_prepare(ctx, g);
+  if (S.blocks) core_c = 0;
   S.core_c = core_c;
   if (core_c) {
     core_prepare(ctx, g, core_c);
@@ -1794,6 +1825,12 @@ void make_schedule(tribit_ctx* ctx, tribit_graph* g, int layout, int flags, bool
     // wedge entries start one past their arc; reverse entries at their row start
     core_cut = layout == TRIBIT_SCHED_WEDGE ? e0 + 1 : e0;
   }
+  plan.resolved = true;
+  plan.layout = layout;
+  plan.flags = flags;
+  plan.core_d0 = core_d0;
+  plan.core_cut = core_cut;
+  plan.blocks = S.blocks;
   if (learned) {
     // the model's choice is part of the scheduler's time
     BucketStats st;
@@ -1802,10 +1839,11 @@ void make_schedule(tribit_ctx* ctx, tribit_graph* g, int layout, int flags, bool
     bucket_stats(ctx, g, layout, &st);
     learned_table(ctx, st, layout, flags, plan.table);
   }
-  // test hooks / opt-outs: every block launch streams tiles, or none does
+  // test hooks / opt-outs: every block launch streams tiles, or none does. With
+  // dense blocks none does: tile ranges follow the uncut wedge ranges.
   for (int i = 0; i < plan.nslots; ++i) {
     if ((flags & TRIBIT_COUNT_FORCE_TILES) && plan.kind[i] == kKindBlock) plan.kind[i] = kKindTile;
-    if ((flags & TRIBIT_COUNT_NO_TILES) && plan.kind[i] == kKindTile) plan.kind[i] = kKindBlock;
+    if (((flags & TRIBIT_COUNT_NO_TILES) || S.blocks) && plan.kind[i] == kKindTile) plan.kind[i] = kKindBlock;
   }
   bool any_tiles = false;
   for (int i = 0; i < plan.nslots; ++i) any_tiles |= plan.kind[i] == kKindTile;
@@ -1834,8 +1872,8 @@ void make_schedule(tribit_ctx* ctx, tribit_graph* g, int layout, int flags, bool
   DevBuf<Cnt8> cnt_own, off_own;
   Cnt8* cnt = scratch_buf(ctx, arena, cnt_own, nb + 1);
   Cnt8* off = scratch_buf(ctx, arena, off_own, nb + 1);
-  const SchedView sv = sched_view(g, layout, core_d0, core_cut);
-  S.rng = sched_ranges(g, layout);
+  const SchedView sv = sched_view(g, layout, core_d0, core_cut, S.blocks);
+  S.rng = sched_ranges(g, layout, S.blocks);
   k_sched_count<<<(unsigned)(nb + 1), kSchedBlock, 0, s>>>(sv, pd, n, cnt);
   {
     Cnt8 zero;
@@ -1907,6 +1945,10 @@ void launch_schedule(tribit_ctx* ctx, tribit_graph* g, const Schedule& S, const 
     core_launch(ctx, g, a.result);
     if (ev) TB_HIP(hipEventRecord(ev[plan.nslots], s));
   }
+  if (S.blocks) {
+    blocks_launch(ctx, g, a.result);
+    if (ev) TB_HIP(hipEventRecord(ev[plan.nslots + (S.core_c ? 1 : 0)], s));
+  }
   TB_HIP(hipGetLastError());
 }
 
@@ -1941,7 +1983,7 @@ static uint64_t count_once(tribit_ctx* ctx, tribit_graph* g, int sched, int flag
   TB_HIP(hipEventRecord(ctx->ev[3], s));
   uint64_t blocks = 0;
   launch_schedule(ctx, g, S, a, ctx->ev + 4, &blocks);
-  hipEvent_t last = ctx->ev[3 + plan.nslots + (S.core_c ? 1 : 0)];
+  hipEvent_t last = ctx->ev[3 + plan.nslots + (S.core_c ? 1 : 0) + (S.blocks ? 1 : 0)];
   if (tlog)
     for (int v = 0; v < plan.nslots; ++v)
       if (S.tot.v[v])
@@ -1955,7 +1997,7 @@ static uint64_t count_once(tribit_ctx* ctx, tribit_graph* g, int sched, int flag
   *ms_kernel = elapsed(ctx->ev[3], last);
   *ms_total = elapsed(ctx->ev[2], last);
   for (int v = 0; v < plan.nslots; ++v) ms_var[v] = elapsed(ctx->ev[3 + v], ctx->ev[4 + v]);
-  if (ms_core) *ms_core = S.core_c ? elapsed(ctx->ev[3 + plan.nslots], last) : 0.0;
+  if (ms_core) *ms_core = (S.core_c || S.blocks) ? elapsed(ctx->ev[3 + plan.nslots], last) : 0.0;
   if (info) {
     info->launch_blocks = blocks;
     info->tasks = S.ntask;
@@ -1964,7 +2006,8 @@ static uint64_t count_once(tribit_ctx* ctx, tribit_graph* g, int sched, int flag
     info->fallback_tasks = res[1];
     for (int i = 0; i < 8; ++i) info->group_width[i] = i < plan.nslots ? plan.g[i] : 0;
     for (int i = 0; i < 8; ++i) info->slot_kind[i] = i < plan.nslots ? plan.kind[i] : 0;
-    info->dense_core = S.core_c;
+    info->dense_core = S.core_c ? S.core_c : S.blocks ? g->blocks.vertices : 0;
+    info->dense_blocks = S.blocks ? g->blocks.nblocks : 0;
   }
   return res[0];
 }
@@ -2000,13 +2043,15 @@ static Plan plan_for(int sched, const int32_t* override_gw, int* layout) {
 //   2: the heuristic on the DIRECTED layout (+ dense core), for relabelled graphs
 //   3: the textbook PLAIN plan
 // (preferred layout: WEDGE for relabelled graphs, else DIRECTED).
-enum { kCandModel = 0, kCandHeur = 1, kCandDirected = 2, kCandPlain = 3, kCands = 4 };
+enum { kCandModel = 0, kCandHeur = 1, kCandDirected = 2, kCandPlain = 3, kCandBlocks = 4, kCands = 5 };
+static_assert(kCands <= TRIBIT_CANDIDATES, "candidate times");
 
 struct CandidateRun {
   int layout;
   Plan plan;
   bool model;
   uint64_t core_c;
+  int flags = 0;  // count flags the candidate adds (dense diagonal blocks)
 };
 
 static bool candidate(tribit_ctx* ctx, tribit_graph* g, const tribit_count_opts& opts, int c, CandidateRun* out) {
@@ -2032,6 +2077,13 @@ static bool candidate(tribit_ctx* ctx, tribit_graph* g, const tribit_count_opts&
       if (pref != TRIBIT_SCHED_WEDGE) return false;  // already candidate 1
       *out = {TRIBIT_SCHED_DIRECTED, plan_for(TRIBIT_SCHED_LEARNED, nullptr, &dummy), false, core};
       heuristic_table(out->plan.table, TRIBIT_SCHED_DIRECTED);
+      return true;
+    case kCandBlocks:
+      // the heuristic plan with dense diagonal blocks on the matrix cores, when
+      // the graph has any (detection runs once per graph, outside the timing)
+      if (pref != TRIBIT_SCHED_WEDGE || opts.dense_core > 0 || !blocks_prepare(ctx, g)) return false;
+      *out = {pref, plan_for(TRIBIT_SCHED_LEARNED, nullptr, &dummy), false, 0, TRIBIT_COUNT_DENSE_BLOCKS};
+      heuristic_table(out->plan.table, pref);
       return true;
     default:
       *out = {TRIBIT_SCHED_PLAIN, plan_for(TRIBIT_SCHED_PLAIN, nullptr, &dummy), false, 0};
@@ -2111,8 +2163,8 @@ void count(tribit_ctx* ctx, tribit_graph* g, const tribit_count_opts& opts, trib
         double a = 0.0, b, k, v[kSlots] = {0};
         for (int rep = 0; rep < 2; ++rep) {
           Plan pl = cr.plan;
-          const uint64_t tri = count_once(ctx, g, cr.layout, opts.flags, pl, cr.model, &a, &b, &k, v, nullptr,
-                                          nullptr, cr.core_c, nullptr);
+          const uint64_t tri = count_once(ctx, g, cr.layout, opts.flags | cr.flags, pl, cr.model, &a, &b, &k, v,
+                                          nullptr, nullptr, cr.core_c, nullptr);
           if (have && tri != ref) fail(TRIBIT_ERR_HIP, "learned candidates disagree");
           ref = tri;
           have = true;
@@ -2145,9 +2197,12 @@ void count(tribit_ctx* ctx, tribit_graph* g, const tribit_count_opts& opts, trib
   for (int r = 0; r < reps; ++r) {
     double a, b, c, v[kSlots] = {0}, mc = 0.0;
     Plan plan = run.plan;
-    const uint64_t tri = count_once(ctx, g, run.layout, opts.flags, plan, run.model, &a, &b, &c, v, &local, nullptr,
-                                    run.core_c, &mc);
-    if (r == reps - 1) run.plan = plan;
+    const uint64_t tri = count_once(ctx, g, run.layout, opts.flags | run.flags, plan, run.model, &a, &b, &c, v,
+                                    &local, nullptr, run.core_c, &mc);
+    if (r == reps - 1) {
+      run.plan = plan;
+      cache_of(g)->last[opts.sched & 3] = plan;  // what ran, for tribit_variant_stats
+    }
     tcore.push_back(mc);
     if (r == 0)
       first = tri;
@@ -2229,7 +2284,21 @@ void variant_stats(tribit_ctx* ctx, tribit_graph* g, int sched, tribit_variant_w
   Plan plan = plan_for(sched, nullptr, &layout);
   BucketStats st;
   uint32_t d0 = 0xFFFFFFFFu, cut = 0;
-  if (sched == TRIBIT_SCHED_LEARNED) {
+  bool blocks = false;
+  const Plan* ran = g->bcache ? &g->bcache->last[sched & 3] : nullptr;
+  if (ran && ran->resolved) {
+    // the plan the last count of this schedule ran, as make_schedule resolved
+    // it (table, kernel kinds and widths after its launch-size adjustments,
+    // core cut, dense-block view); statistics computed without evicting the
+    // scheduler's kept ones
+    plan = *ran;
+    layout = ran->layout;
+    flags = ran->flags;
+    d0 = ran->core_d0;
+    cut = ran->core_cut;
+    blocks = ran->blocks;
+    bucket_stats(ctx, g, layout, &st, d0, cut, blocks, false);
+  } else if (sched == TRIBIT_SCHED_LEARNED) {
     tribit_count_opts o;
     memset(&o, 0, sizeof(o));
     o.sched = TRIBIT_SCHED_LEARNED;
@@ -2246,7 +2315,7 @@ void variant_stats(tribit_ctx* ctx, tribit_graph* g, int sched, tribit_variant_w
       ctx->sync();
       cut = layout == TRIBIT_SCHED_WEDGE ? e0 + 1 : e0;
     }
-    bucket_stats(ctx, g, layout, &st, d0, cut);
+    bucket_stats(ctx, g, layout, &st, d0, cut, false, false);
     if (cr.model) learned_table(ctx, st, layout, 0, plan.table);
     flags = o.flags;
   } else {
@@ -2265,7 +2334,8 @@ void variant_stats(tribit_ctx* ctx, tribit_graph* g, int sched, tribit_variant_w
     DevBuf<unsigned long long> ns(ctx, 3 * kSlots);
     TB_HIP(hipMemcpyAsync(dt.p, plan.table, kBuckets, hipMemcpyHostToDevice, ctx->stream));
     k_zero_u64<<<1, 64, 0, ctx->stream>>>(ns.p, 3 * kSlots);
-    k_narrow_stats<<<grid_for(g->n, 256), 256, 0, ctx->stream>>>(sched_view(g, layout, d0, cut), g->n, dt.p, ns.p);
+    k_narrow_stats<<<grid_for(g->n, 256), 256, 0, ctx->stream>>>(sched_view(g, layout, d0, cut, blocks), g->n, dt.p,
+                                                                 ns.p);
     unsigned long long h[3 * kSlots];
     TB_HIP(hipMemcpyAsync(h, ns.p, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
     ctx->sync();
@@ -2290,7 +2360,9 @@ void batch_drop_exec(tribit_batch* b) {
 
 // Task lists of every part for (sched, flags); returns the device ms spent.
 static double batch_schedules(tribit_ctx* ctx, tribit_batch* b, int sched, int flags) {
-  bool ready = b->sched == sched && b->flags == flags;
+  // a learned batch's tables come from the cost model: a new model makes them stale
+  bool ready = b->sched == sched && b->flags == flags &&
+               (sched != TRIBIT_SCHED_LEARNED || b->model_version == ctx->model_version);
   for (auto& p : b->parts) ready = ready && p.S;
   if (ready) return 0.0;
   batch_drop_exec(b);
@@ -2305,7 +2377,9 @@ static double batch_schedules(tribit_ctx* ctx, tribit_batch* b, int sched, int f
     if (layout == TRIBIT_SCHED_WEDGE && !p.g->rng_sfx) fail(TRIBIT_ERR_ARG, "the wedge schedule needs relabelled graphs");
     p.S = new Schedule;
     p.S->plan = plan;
-    make_schedule(ctx, p.g, layout, flags, learned, *p.S, 0);  // per-graph attribution: no dense core
+    // per-graph attribution: no dense core and no dense blocks (their launches
+    // add to the batch total, not to each graph's count)
+    make_schedule(ctx, p.g, layout, flags & ~TRIBIT_COUNT_DENSE_BLOCKS, learned, *p.S, 0);
     for (int i = 0; i < kSlots; ++i) {
       p.S->gdef[i] = p.S->plan.g[i];
       p.S->kdef[i] = p.S->plan.kind[i];
@@ -2315,6 +2389,7 @@ static double batch_schedules(tribit_ctx* ctx, tribit_batch* b, int sched, int f
   TB_HIP(hipEventSynchronize(ctx->ev[1]));
   b->sched = sched;
   b->flags = flags;
+  b->model_version = ctx->model_version;
   return elapsed(ctx->ev[0], ctx->ev[1]);
 }
 

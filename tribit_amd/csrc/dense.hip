@@ -13,11 +13,15 @@
 // (A[c][k] = 0 for k <= c), 32 x 32 x 32 MFMA blocks fed from LDS. The
 // streaming kernels skip exactly these triangles (the scheduler drops every
 // list entry whose smallest vertex is in the core), so the two sums add up.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
 
+#include <rocprim/device/device_scan.hpp>
+
 #include "common.hpp"
+#include "device.hpp"
 
 namespace tb {
 
@@ -48,17 +52,14 @@ __global__ void k_core_fill(const uint32_t* __restrict__ row_ptr, const uint32_t
 // (column = l % 32): rows come in four groups of 8, each lane half holding 4.
 __device__ __forceinline__ int mfma_row(int v, int l) { return 8 * (v >> 2) + 4 * (l >> 5) + (v & 3); }
 
-__global__ __launch_bounds__(kCoreThreads) void k_core_count(const int8_t* __restrict__ A, uint32_t kp,
-                                                             unsigned long long* __restrict__ result) {
+// One 128 x 128 output tile (I <= J) of the masked product over a kp x kp 0/1
+// upper-triangular matrix A: adds sum over rows r of tile block I and columns
+// c of tile block J of A[r][c] * (A A^T)[r][c] to *result.
+__device__ __forceinline__ void core_tile(const int8_t* __restrict__ A, uint32_t kp, uint32_t I, uint32_t J,
+                                          unsigned long long* __restrict__ result) {
   __shared__ __attribute__((aligned(16))) int8_t sA[kCoreTile][kCoreK + kCorePad];
   __shared__ __attribute__((aligned(16))) int8_t sB[kCoreTile][kCoreK + kCorePad];
   __shared__ unsigned long long sPart[kCoreThreads / 64];
-  // tile pair (I <= J) from the linear index over the upper triangle, column by column
-  const uint64_t t = blockIdx.x;
-  uint32_t J = (uint32_t)((std::sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
-  while ((uint64_t)(J + 1) * (J + 2) / 2 <= t) ++J;
-  while ((uint64_t)J * (J + 1) / 2 > t) --J;
-  const uint32_t I = (uint32_t)(t - (uint64_t)J * (J + 1) / 2);
   const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const uint32_t wr = w >> 1, wc = w & 1;
   const uint64_t rowA = (uint64_t)I * kCoreTile, rowB = (uint64_t)J * kCoreTile;
@@ -131,6 +132,17 @@ __global__ __launch_bounds__(kCoreThreads) void k_core_count(const int8_t* __res
     for (int i = 0; i < kCoreThreads / 64; ++i) s += sPart[i];
     if (s) atomicAdd(result, s);
   }
+}
+
+__global__ __launch_bounds__(kCoreThreads) void k_core_count(const int8_t* __restrict__ A, uint32_t kp,
+                                                             unsigned long long* __restrict__ result) {
+  // tile pair (I <= J) from the linear index over the upper triangle, column by column
+  const uint64_t t = blockIdx.x;
+  uint32_t J = (uint32_t)((std::sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+  while ((uint64_t)(J + 1) * (J + 2) / 2 <= t) ++J;
+  while ((uint64_t)J * (J + 1) / 2 > t) --J;
+  const uint32_t I = (uint32_t)(t - (uint64_t)J * (J + 1) / 2);
+  core_tile(A, kp, I, J, result);
 }
 
 // Builds the core matrix of the top `c` ids (cached per graph and size).
@@ -213,6 +225,277 @@ uint64_t core_choose(tribit_ctx* ctx, const tribit_graph* g) {
     }
   }
   return best;
+}
+
+// ------------------------------------------------ dense diagonal blocks
+// The same contraction on many small dense id ranges [s, e) instead of one
+// top core (the blocks of a block model, once the octave-class relabelling
+// keeps each block a run of consecutive ids). A triangle a < b < c with all
+// three vertices in one block is counted by the block's masked product; the
+// streaming kernels count the rest over a copy of the wedge ranges in which
+// the range of every arc u -> v inside a block (the part of N+(u) after v,
+// streamed against N+(v)) starts at the first id >= e instead: the elements
+// it loses are exactly the third vertices inside the block. Arcs entering the
+// block from below keep their whole range, so triangles with the smallest
+// vertex outside the block are all streamed.
+//
+// Detection: cross[p] = arcs a -> b with b - a <= kSpanMax and a <= p < b
+// (short arcs over the gap between ids p and p + 1). A gap with fewer than
+// kCrossMin such arcs ends a segment; segments of kBlockMin..kBlockMax ids
+// whose induced density is at least kBlockDensity become blocks. The window is
+// short so that sparse background edges seldom span a gap (a block model with
+// p_out = 1e-4 puts ~0.2 background arcs over a block boundary) while inside
+// a block of density rho even the gap after its first id has rho * kSpanMax
+// of them (19 at rho = 0.3).
+constexpr uint32_t kSpanMax = 64;
+constexpr int32_t kCrossMin = 8;
+constexpr uint32_t kBlockMin = 64, kBlockMax = 4096;
+constexpr double kBlockDensity = 0.05;
+
+// diff[u] += short out-arcs of u, diff[v] -= 1 for each (rows ascending:
+// the short arcs are a prefix of the row).
+__global__ void k_span_diff(const uint32_t* __restrict__ row_ptr, const uint32_t* __restrict__ col, uint64_t n,
+                            int32_t* __restrict__ diff) {
+  const uint64_t u = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= n) return;
+  int32_t c = 0;
+  for (uint32_t e = row_ptr[u]; e < row_ptr[u + 1]; ++e) {
+    const uint32_t v = col[e];
+    if (v - (uint32_t)u > kSpanMax) break;
+    atomicSub(&diff[v], 1);
+    ++c;
+  }
+  if (c) atomicAdd(&diff[u], c);
+}
+
+// cut[p] = 1 when the gap after id p ends a segment (the last id always does).
+__global__ void k_cut_flags(const int32_t* __restrict__ cross, uint64_t n, uint32_t* __restrict__ cut) {
+  const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < n) cut[p] = (p + 1 == n || cross[p] < kCrossMin) ? 1u : 0u;
+}
+
+// Segment k = ids with seg[v] == k (seg = exclusive scan of cut): its first id
+// and length; intra[k] = arcs with both ends in it.
+__global__ void k_seg_stats(const uint32_t* __restrict__ row_ptr, const uint32_t* __restrict__ col,
+                            const uint32_t* __restrict__ seg, uint64_t n, uint32_t* __restrict__ start,
+                            uint32_t* __restrict__ len, unsigned long long* __restrict__ intra) {
+  const uint64_t u = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = u < n;
+  const uint32_t k = valid ? seg[u] : 0xFFFFFFFFu;
+  if (valid && (u == 0 || seg[u - 1] != k)) start[k] = (uint32_t)u;
+  const uint32_t run = wave_run_length(k, valid);
+  if (run) atomicAdd(&len[k], run);
+  if (!valid) return;
+  uint32_t c = 0;
+  for (uint32_t e = row_ptr[u]; e < row_ptr[u + 1]; ++e) {
+    const uint32_t v = col[e];
+    if (v - (uint32_t)u > kBlockMax || seg[v] != k) break;  // rows ascending: the segment's ids come first
+    ++c;
+  }
+  if (c) atomicAdd(&intra[k], (unsigned long long)c);
+}
+
+// Segments dense enough to be blocks: (start, length) appended at out[*count].
+__global__ void k_seg_select(const uint32_t* __restrict__ start, const uint32_t* __restrict__ len,
+                             const unsigned long long* __restrict__ intra, uint32_t nseg, uint2* __restrict__ out,
+                             uint32_t cap, uint32_t* __restrict__ count) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nseg) return;
+  const uint32_t L = len[k];
+  if (L < kBlockMin || L > kBlockMax) return;
+  const double pairs = 0.5 * (double)L * (double)(L - 1);
+  if ((double)intra[k] < kBlockDensity * pairs) return;
+  const uint32_t at = atomicAdd(count, 1u);
+  if (at < cap) out[at] = make_uint2(start[k], L);
+}
+
+__global__ void k_blk_mark(const uint2* __restrict__ blk, uint32_t nb, int32_t* __restrict__ blk_of) {
+  const uint32_t i = blockIdx.x;
+  if (i >= nb) return;
+  for (uint32_t v = threadIdx.x; v < blk[i].y; v += blockDim.x) blk_of[blk[i].x + v] = (int32_t)i;
+}
+
+// A_i[(u - s) * kp + (v - s)] = 1 for every arc u -> v inside block i.
+__global__ void k_blk_fill(const uint32_t* __restrict__ row_ptr, const uint32_t* __restrict__ col,
+                           const uint2* __restrict__ blk, const uint64_t* __restrict__ aoff,
+                           const uint32_t* __restrict__ kps, uint32_t nb, int8_t* __restrict__ A) {
+  const uint32_t i = blockIdx.x;
+  if (i >= nb) return;
+  const uint32_t s = blk[i].x, e = s + blk[i].y, kp = kps[i];
+  int8_t* M = A + aoff[i];
+  for (uint32_t u = s + threadIdx.x; u < e; u += blockDim.x) {
+    int8_t* row = M + (uint64_t)(u - s) * kp;
+    for (uint32_t q = row_ptr[u]; q < row_ptr[u + 1]; ++q) {
+      const uint32_t v = col[q];
+      if (v >= e) break;
+      row[v - s] = 1;
+    }
+  }
+}
+
+// The wedge ranges of owners inside blocks with in-block sources cut to ids
+// >= the block's end (range copies were made by the caller); w[v] reduced by
+// the elements cut. Ranges of one owner are ordered by source, so the
+// in-block sources are those whose arc index is >= row_ptr[s].
+__global__ void k_blk_trim(const uint32_t* __restrict__ row_ptr, const uint32_t* __restrict__ col,
+                           const uint32_t* __restrict__ sfx_ptr, const uint2* __restrict__ blk,
+                           const int32_t* __restrict__ blk_of, uint64_t n, uint32_t emask, uint2* __restrict__ rng,
+                           unsigned long long* __restrict__ w, unsigned long long* __restrict__ total_cut) {
+  const uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  const int32_t i = blk_of[v];
+  if (i < 0) return;
+  const uint32_t s = blk[i].x, e = s + blk[i].y, first = row_ptr[s];
+  uint64_t cut = 0;
+  for (uint32_t j = sfx_ptr[v]; j < sfx_ptr[v + 1]; ++j) {
+    const uint2 r = rng[j];
+    if (r.x <= first) continue;  // arc index r.x - 1 < row_ptr[s]: the source is below the block
+    const uint32_t end = r.y & emask;
+    uint32_t lo = r.x, hi = end;  // first position >= r.x with col >= e
+    while (lo < hi) {
+      const uint32_t mid = lo + ((hi - lo) >> 1);
+      if (col[mid] < e)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    cut += lo - r.x;
+    rng[j] = make_uint2(lo, end);  // a cut range streams 32-bit ids (no narrow flags)
+  }
+  if (cut) {
+    w[v] -= cut;
+    atomicAdd(total_cut, (unsigned long long)cut);
+  }
+}
+
+__global__ __launch_bounds__(kCoreThreads) void k_blocks_count(const int8_t* __restrict__ A,
+                                                               const uint4* __restrict__ work,
+                                                               unsigned long long* __restrict__ result) {
+  const uint4 w = work[blockIdx.x];
+  core_tile(A + (((uint64_t)w.y << 32) | w.x), w.z, w.w & 0xFFFFu, w.w >> 16, result);
+}
+
+void blocks_release(tribit_graph* g) {
+  tribit_graph::DenseBlocks& B = g->blocks;
+  if (g->ctx) {
+    g->ctx->free(B.A);
+    g->ctx->free(B.work);
+    g->ctx->free(B.rng);
+    g->ctx->free(B.w);
+  }
+  B = tribit_graph::DenseBlocks{};
+}
+
+bool blocks_prepare(tribit_ctx* ctx, tribit_graph* g) {
+  tribit_graph::DenseBlocks& B = g->blocks;
+  if (B.found) return B.nblocks > 0;
+  B.found = true;
+  const uint64_t n = g->n, m = g->m;
+  if (!g->relabeled || !g->rng_sfx || n < kBlockMin || !m) return false;
+  hipStream_t s = ctx->stream;
+  std::vector<uint2> hb;
+  {
+    DevBuf<int32_t> diff(ctx, n), cross(ctx, n);
+    DevBuf<uint32_t> cut(ctx, n), seg(ctx, n);
+    TB_HIP(hipMemsetAsync(diff.p, 0, 4 * n, s));
+    k_span_diff<<<grid_for(n, 256), 256, 0, s>>>(g->row_ptr, g->col, n, diff.p);
+    size_t tb = 0;
+    TB_HIP(rocprim::inclusive_scan(nullptr, tb, diff.p, cross.p, (size_t)n, rocprim::plus<int32_t>(), s));
+    DevBuf<uint8_t> tmp(ctx, tb ? tb : 8);
+    TB_HIP(rocprim::inclusive_scan(tmp.p, tb, diff.p, cross.p, (size_t)n, rocprim::plus<int32_t>(), s));
+    k_cut_flags<<<grid_for(n, 256), 256, 0, s>>>(cross.p, n, cut.p);
+    size_t tb2 = 0;
+    TB_HIP(rocprim::exclusive_scan(nullptr, tb2, cut.p, seg.p, 0u, (size_t)n, rocprim::plus<uint32_t>(), s));
+    DevBuf<uint8_t> tmp2(ctx, tb2 ? tb2 : 8);
+    TB_HIP(rocprim::exclusive_scan(tmp2.p, tb2, cut.p, seg.p, 0u, (size_t)n, rocprim::plus<uint32_t>(), s));
+    uint32_t nseg = 0;
+    TB_HIP(hipMemcpyAsync(&nseg, seg.p + n - 1, 4, hipMemcpyDeviceToHost, s));
+    ctx->sync();
+    nseg += 1;  // the last id's cut flag is 1: its segment is the last
+    DevBuf<uint32_t> start(ctx, nseg), len(ctx, nseg);
+    DevBuf<unsigned long long> intra(ctx, nseg);
+    TB_HIP(hipMemsetAsync(len.p, 0, 4ull * nseg, s));
+    TB_HIP(hipMemsetAsync(intra.p, 0, 8ull * nseg, s));
+    k_seg_stats<<<grid_for(n, 256), 256, 0, s>>>(g->row_ptr, g->col, seg.p, n, start.p, len.p, intra.p);
+    constexpr uint32_t kCap = 1u << 20;
+    const uint32_t cap = std::min<uint32_t>(nseg, kCap);
+    DevBuf<uint2> sel(ctx, cap);
+    DevBuf<uint32_t> cnt(ctx, 1);
+    TB_HIP(hipMemsetAsync(cnt.p, 0, 4, s));
+    k_seg_select<<<grid_for(nseg, 256), 256, 0, s>>>(start.p, len.p, intra.p, nseg, sel.p, cap, cnt.p);
+    uint32_t nb = 0;
+    TB_HIP(hipMemcpyAsync(&nb, cnt.p, 4, hipMemcpyDeviceToHost, s));
+    ctx->sync();
+    nb = std::min(nb, cap);
+    hb.resize(nb);
+    if (nb) TB_HIP(hipMemcpyAsync(hb.data(), sel.p, 8ull * nb, hipMemcpyDeviceToHost, s));
+    ctx->sync();
+  }
+  if (hb.empty()) return false;
+  std::sort(hb.begin(), hb.end(), [](const uint2& a, const uint2& b) { return a.x < b.x; });
+  const uint32_t nb = (uint32_t)hb.size();
+  // matrices, padded to multiples of the output tile, and the workgroups' tile pairs
+  std::vector<uint64_t> aoff(nb);
+  std::vector<uint32_t> kps(nb);
+  std::vector<uint4> work;
+  uint64_t total = 0, verts = 0;
+  double macs = 0.0;
+  for (uint32_t i = 0; i < nb; ++i) {
+    const uint32_t kp = (hb[i].y + kCoreTile - 1) / kCoreTile * kCoreTile;
+    aoff[i] = total;
+    kps[i] = kp;
+    total += (uint64_t)kp * kp;
+    verts += hb[i].y;
+    macs += core_macs(kp);
+    for (uint32_t J = 0; J < kp / kCoreTile; ++J)
+      for (uint32_t I = 0; I <= J; ++I)
+        work.push_back(make_uint4((uint32_t)aoff[i], (uint32_t)(aoff[i] >> 32), kp, I | (J << 16)));
+  }
+  DevBuf<int8_t> A(ctx, total);
+  DevBuf<uint4> dwork(ctx, work.size());
+  DevBuf<uint2> dblk(ctx, nb);
+  DevBuf<uint64_t> daoff(ctx, nb);
+  DevBuf<uint32_t> dkp(ctx, nb);
+  TB_HIP(hipMemcpyAsync(dwork.p, work.data(), 16ull * work.size(), hipMemcpyHostToDevice, s));
+  TB_HIP(hipMemcpyAsync(dblk.p, hb.data(), 8ull * nb, hipMemcpyHostToDevice, s));
+  TB_HIP(hipMemcpyAsync(daoff.p, aoff.data(), 8ull * nb, hipMemcpyHostToDevice, s));
+  TB_HIP(hipMemcpyAsync(dkp.p, kps.data(), 4ull * nb, hipMemcpyHostToDevice, s));
+  k_core_zero<<<2048, 256, 0, s>>>(reinterpret_cast<uint4*>(A.p), total / 16);
+  k_blk_fill<<<nb, 256, 0, s>>>(g->row_ptr, g->col, dblk.p, daoff.p, dkp.p, nb, A.p);
+  // the streamed ranges: a copy of the wedge ranges with in-block arcs cut
+  DevBuf<int32_t> blk_of(ctx, n);
+  TB_HIP(hipMemsetAsync(blk_of.p, 0xFF, 4 * n, s));
+  k_blk_mark<<<nb, 256, 0, s>>>(dblk.p, nb, blk_of.p);
+  DevBuf<uint2> rng(ctx, m);
+  DevBuf<uint64_t> w(ctx, n);
+  DevBuf<unsigned long long> cut(ctx, 1);
+  TB_HIP(hipMemcpyAsync(rng.p, g->rng_sfx, 8 * m, hipMemcpyDeviceToDevice, s));
+  TB_HIP(hipMemcpyAsync(w.p, g->w_sfx, 8 * n, hipMemcpyDeviceToDevice, s));
+  TB_HIP(hipMemsetAsync(cut.p, 0, 8, s));
+  k_blk_trim<<<grid_for(n, 256), 256, 0, s>>>(g->row_ptr, g->col, g->sfx_ptr, dblk.p, blk_of.p, n,
+                                              range_end_mask(m), rng.p,
+                                              reinterpret_cast<unsigned long long*>(w.p), cut.p);
+  TB_HIP(hipGetLastError());
+  unsigned long long hcut = 0;
+  TB_HIP(hipMemcpyAsync(&hcut, cut.p, 8, hipMemcpyDeviceToHost, s));
+  ctx->sync();
+  B.nblocks = nb;
+  B.vertices = verts;
+  B.macs = macs;
+  B.A = A.release();
+  B.work = dwork.release();
+  B.nwork = (uint32_t)work.size();
+  B.rng = rng.release();
+  B.w = w.release();
+  B.cut = hcut;
+  g->bytes += total + 16ull * work.size() + 8 * m + 8 * n;
+  return true;
+}
+
+void blocks_launch(tribit_ctx* ctx, const tribit_graph* g, unsigned long long* result) {
+  if (!g->blocks.A || !g->blocks.nwork) return;
+  k_blocks_count<<<g->blocks.nwork, kCoreThreads, 0, ctx->stream>>>(g->blocks.A, g->blocks.work, result);
+  TB_HIP(hipGetLastError());
 }
 
 }  // namespace tb

@@ -165,6 +165,19 @@ __global__ void k_iota_u32(uint32_t* __restrict__ p, uint64_t n) {
   if (i < n) p[i] = (uint32_t)i;
 }
 
+// cls[v] = floor(log2(deg[v])) + 1 (0 for an isolated vertex): octave classes.
+__global__ void k_octave(const uint32_t* __restrict__ deg, uint64_t n, uint32_t* __restrict__ cls) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) cls[i] = deg[i] ? 32u - (uint32_t)__clz(deg[i]) : 0u;
+}
+
+// out[i] = src[idx[i]].
+__global__ void k_gather_u32(const uint32_t* __restrict__ src, const uint32_t* __restrict__ idx, uint64_t n,
+                             uint32_t* __restrict__ out) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = src[idx[i]];
+}
+
 __global__ void k_scatter_labels(const uint32_t* __restrict__ old_of_new, uint64_t n, uint32_t* __restrict__ lab) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) lab[old_of_new[i]] = (uint32_t)i;
@@ -518,10 +531,21 @@ void preprocess(tribit_ctx* ctx, uint64_t* d_keys_in, uint64_t nkeys, uint64_t n
     DevBuf<uint32_t> ids(ctx, n), deg_sorted(ctx, n), lab(ctx, n);
     labels = DevBuf<uint32_t>(ctx, n);
     k_iota_u32<<<grid_for(n, kBlock), kBlock, 0, s>>>(ids.p, n);
-    with_temp(ctx, [&](void* t, size_t& b) {
-      return rocprim::radix_sort_pairs(t, b, deg.p, deg_sorted.p, ids.p, labels.p, (size_t)n, 0u,
-                                       bits_for(n), s);
-    });
+    if (ctx->relabel == 2) {
+      // octave classes: rank of (floor(log2(deg)), id); runs of consecutive
+      // input ids with similar degrees (a block of a block model) stay runs
+      DevBuf<uint32_t> cls(ctx, n);
+      k_octave<<<grid_for(n, kBlock), kBlock, 0, s>>>(deg.p, n, cls.p);
+      with_temp(ctx, [&](void* t, size_t& b) {
+        return rocprim::radix_sort_pairs(t, b, cls.p, deg_sorted.p, ids.p, labels.p, (size_t)n, 0u, 6u, s);
+      });
+      k_gather_u32<<<grid_for(n, kBlock), kBlock, 0, s>>>(deg.p, labels.p, n, deg_sorted.p);
+    } else {
+      with_temp(ctx, [&](void* t, size_t& b) {
+        return rocprim::radix_sort_pairs(t, b, deg.p, deg_sorted.p, ids.p, labels.p, (size_t)n, 0u,
+                                         bits_for(n), s);
+      });
+    }
     std::swap(deg.p, deg_sorted.p);  // deg now indexed by internal id
     k_scatter_labels<<<grid_for(n, kBlock), kBlock, 0, s>>>(labels.p, n, lab.p);
     if (m) {
@@ -894,6 +918,7 @@ void free_graph_memory(tribit_graph* g) {
   if (!g || !g->ctx) return;
   free_tiles(g);
   core_release(g);
+  blocks_release(g);
   bucket_cache_free(g);
   g->ctx->free(g->row_ptr);
   g->ctx->free(g->col);
