@@ -262,6 +262,7 @@ def measure(g, sched: str, steps: int, warmup: int):
         "layout": results[0].layout,
         "dense_core": results[0].dense_core,
         "ms_dense": statistics.median(r.ms_dense for r in results),
+        "dense_blocks": results[0].dense_blocks,
         "ms_judge": (first or results[0]).ms_judge,
         "ms_first_count": (first or results[0]).ms_count,
         "ms_candidates": list((first or results[0]).ms_candidates),
@@ -309,6 +310,7 @@ def mixed_family(args) -> int:
     if args.sched == "learned":
         log(f"cost model loaded: {ctx.load_cost_model()}")
     warm_up(ctx)
+    ctx.set_relabel(args.relabel)
     for name, desc, make in MIXED:
         g = make(ctx)
         inf = g.info()
@@ -327,6 +329,9 @@ def mixed_family(args) -> int:
             "plain_gteps": round(m / (plain["ms_per_step"] * 1e-3) / 1e9, 4),
             "plain_ms_count_device": round(plain["ms_dev"], 4),
             "ms_preprocess_device": round(inf["ms_preprocess"], 3),
+            "relabel": args.relabel, "candidate": sel["candidate"], "ms_judge": round(sel["ms_judge"], 3),
+            "ms_candidates": [round(x, 3) for x in sel["ms_candidates"]],
+            "dense_blocks": sel.get("dense_blocks", 0), "ms_dense": round(sel["ms_dense"], 4),
             "cpu_check": check,
         }), flush=True)
         g.free()
@@ -526,6 +531,8 @@ def main() -> int:
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (reference kernel for HBM byte counters)")
     ap.add_argument("--batch-graphs", type=int, default=4096, help="graphs of the batch workload")
+    ap.add_argument("--relabel", type=int, default=1, choices=[0, 1, 2],
+                    help="mixed workload: vertex ids (1 degree rank, 2 octave classes of the degree, 0 input ids)")
     ap.add_argument("--partitions", type=int, default=64, help="rank workload: partitions P")
     ap.add_argument("--partition", type=int, default=0, help="rank workload: the column partition")
     ap.add_argument("--workload", choices=["rmat", "mixed", "batch", "rank"], default="rmat",

@@ -78,6 +78,10 @@ __device__ __forceinline__ void core_tile(const int8_t* __restrict__ A, uint32_t
   const uint4* gb = reinterpret_cast<const uint4*>(A + (rowB + lr) * kp + k_begin + lh);
   uint4 pa0 = ga[0], pa1 = ga[1], pa2 = ga[2], pa3 = ga[3];
   uint4 pb0 = gb[0], pb1 = gb[1], pb2 = gb[2], pb3 = gb[3];
+  // the epilogue's mask A[r][c] over this output tile is the first K stage of
+  // the A operand (columns from J * 128): taken from LDS into one bit per
+  // accumulator element instead of 64 byte reads per lane from global memory
+  uint64_t msk = 0;
   for (uint32_t k0 = k_begin; k0 < kp; k0 += kCoreK) {
     *reinterpret_cast<uint4*>(&sA[lr][lh]) = pa0;
     *reinterpret_cast<uint4*>(&sA[lr][lh + 16]) = pa1;
@@ -88,6 +92,17 @@ __device__ __forceinline__ void core_tile(const int8_t* __restrict__ A, uint32_t
     *reinterpret_cast<uint4*>(&sB[lr][lh + 32]) = pb2;
     *reinterpret_cast<uint4*>(&sB[lr][lh + 48]) = pb3;
     __syncthreads();
+    if (k0 == k_begin) {
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+          for (int v = 0; v < 16; ++v) {
+            const int r = wr * 64 + i * 32 + mfma_row(v, lane), c = wc * 64 + j * 32 + (lane & 31);
+            msk |= (uint64_t)(sA[r][c] != 0) << ((i * 2 + j) * 16 + v);
+          }
+    }
     if (k0 + kCoreK < kp) {  // next stage in flight during the MFMAs
       ga += kCoreK / 16;
       gb += kCoreK / 16;
@@ -115,14 +130,10 @@ __device__ __forceinline__ void core_tile(const int8_t* __restrict__ A, uint32_t
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      const uint64_t c = rowB + wc * 64 + j * 32 + (lane & 31);
+    for (int j = 0; j < 2; ++j)
 #pragma unroll
-      for (int v = 0; v < 16; ++v) {
-        const uint64_t r = rowA + wr * 64 + i * 32 + mfma_row(v, lane);
-        if (A[r * kp + c]) sum += (uint32_t)acc[i][j][v];
-      }
-    }
+      for (int v = 0; v < 16; ++v)
+        if ((msk >> ((i * 2 + j) * 16 + v)) & 1) sum += (uint32_t)acc[i][j][v];
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o);
   if (lane == 0) sPart[w] = sum;
