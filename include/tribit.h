@@ -252,6 +252,20 @@ int tribit_graph_free(tribit_graph* g);
 
 /* ---- counting (tribit.cu:765-802) ------------------------------------------ */
 int tribit_count(tribit_ctx* ctx, tribit_graph* g, const tribit_count_opts* opts, tribit_count_info* info);
+/* No reference counterpart: for callers that cannot allocate device memory
+ * (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc). tribit_count with no allocation and no free: the schedule's
+ * task lists, its counters and the result live in one scratch array of
+ * tribit_noalloc_scratch_bytes() (128 MiB) per GPU, file-scope device memory
+ * held until the process ends. opts->sched must be PLAIN, DIRECTED or WEDGE
+ * (LEARNED judges candidates and builds tables: TRIBIT_ERR_ARG), with no dense
+ * core and no TRIBIT_COUNT_FORCE_TILES. A schedule needing more scratch than
+ * the array fails with TRIBIT_ERR_TOO_LARGE (R-MAT scale 25, 5.1M tasks, uses
+ * about 90 MiB). Every call on one GPU shares that GPU's array and clears
+ * what it needs at the start: calls that overlap on one GPU (two threads or
+ * contexts on the same device) race and must be serialised by the caller. */
+int tribit_count_noalloc(tribit_ctx* ctx, tribit_graph* g, const tribit_count_opts* opts, tribit_count_info* info);
+uint64_t tribit_noalloc_scratch_bytes(void);
 /* Per-variant work of the scheduler `sched` on this graph (not timed). */
 int tribit_variant_stats(tribit_ctx* ctx, tribit_graph* g, int sched, tribit_variant_work* out);
 

@@ -176,6 +176,8 @@ def load_library(path: str | None = None) -> ctypes.CDLL:
         "tribit_set_option": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int64]),
         "tribit_graph_free": (ctypes.c_int, [_P]),
         "tribit_count": (ctypes.c_int, [_P, _P, ctypes.POINTER(CountOpts), ctypes.POINTER(CountInfo)]),
+        "tribit_count_noalloc": (ctypes.c_int, [_P, _P, ctypes.POINTER(CountOpts), ctypes.POINTER(CountInfo)]),
+        "tribit_noalloc_scratch_bytes": (_U64, []),
         "tribit_count_profile": (ctypes.c_int, [_P, _P, ctypes.c_int, _P, ctypes.POINTER(Profile)]),
         "tribit_profile_features": (ctypes.c_int, [ctypes.POINTER(Profile), ctypes.c_int, ctypes.c_int, _P]),
         "tribit_menu_slot": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),

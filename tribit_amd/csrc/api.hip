@@ -90,6 +90,7 @@ int tribit_init(int device, tribit_ctx** out) {
     uint64_t thr = UINT64_MAX;  // keep freed blocks in the pool (tribit.cu:829-830)
     TB_HIP(hipMemPoolSetAttribute(ctx->pool, hipMemPoolAttrReleaseThreshold, &thr));
     for (auto& e : ctx->ev) TB_HIP(hipEventCreate(&e));
+    tb::noalloc_init(ctx);  // the device is current: its copy of the scratch array
   });
   if (st != TRIBIT_OK) {
     delete ctx;
@@ -441,6 +442,38 @@ int tribit_count(tribit_ctx* ctx, tribit_graph* g, const tribit_count_opts* opts
     if (info) *info = local;
   });
 }
+
+int tribit_count_noalloc(tribit_ctx* ctx, tribit_graph* g, const tribit_count_opts* opts, tribit_count_info* info) {
+  if (!ctx || !g || g->ctx != ctx || !opts) return TRIBIT_ERR_ARG;
+  const tribit_count_opts o = *opts;
+  if (o.sched != TRIBIT_SCHED_PLAIN && o.sched != TRIBIT_SCHED_DIRECTED && o.sched != TRIBIT_SCHED_WEDGE) {
+    ctx->last_error = "tribit_count_noalloc runs PLAIN, DIRECTED or WEDGE (LEARNED judges and allocates)";
+    return TRIBIT_ERR_ARG;
+  }
+  if (o.dense_core > 0 || (o.flags & TRIBIT_COUNT_FORCE_TILES)) {
+    ctx->last_error = "tribit_count_noalloc counts without a dense core and without tiles";
+    return TRIBIT_ERR_ARG;
+  }
+  if (o.sched == TRIBIT_SCHED_WEDGE && !g->rng_sfx) {
+    ctx->last_error = "the wedge schedule needs a relabelled graph (TRIBIT_OPT_RELABEL)";
+    return TRIBIT_ERR_ARG;
+  }
+  for (int i = 0; i < 4; ++i) {
+    const int w = o.group_width[i];
+    if (w != 0 && w != 1 && w != 2 && w != 4 && w != 8 && w != 16 && w != 32 && w != 64) return TRIBIT_ERR_ARG;
+    if (i <= 1 && w == 64) return TRIBIT_ERR_ARG;
+    if (i >= 2 && w != 0 && w < 4) return TRIBIT_ERR_ARG;
+  }
+  return guarded(ctx, [&] {
+    TB_HIP(hipSetDevice(ctx->device));
+    tribit_count_info local;
+    memset(&local, 0, sizeof(local));
+    tb::count(ctx, g, o, &local, /*noalloc=*/true);
+    if (info) *info = local;
+  });
+}
+
+uint64_t tribit_noalloc_scratch_bytes(void) { return tb::noalloc_scratch_bytes(); }
 
 int tribit_variant_stats(tribit_ctx* ctx, tribit_graph* g, int sched, tribit_variant_work* out) {
   if (!ctx || !g || !out || g->ctx != ctx) return TRIBIT_ERR_ARG;

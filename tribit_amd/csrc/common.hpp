@@ -76,6 +76,9 @@ struct tribit_ctx {
     double ms[TRIBIT_CANDIDATES];
   };
   std::vector<Judged> judged;
+  // device address of the no-allocation count's scratch array on this
+  // context's device (count.hip; looked up once at tribit_init)
+  void* noalloc_scratch = nullptr;
   std::string last_error;
 
   void* alloc(uint64_t bytes) {
@@ -283,7 +286,10 @@ uint64_t sbm_slots(uint64_t blocks, uint64_t size, double p_out);
 void gen_sbm(tribit_ctx* ctx, uint64_t blocks, uint64_t size, double p_in, double p_out, uint64_t seed,
              uint64_t* d_keys);
 // count.hip
-void count(tribit_ctx* ctx, tribit_graph* g, const tribit_count_opts& opts, tribit_count_info* info);
+void count(tribit_ctx* ctx, tribit_graph* g, const tribit_count_opts& opts, tribit_count_info* info,
+           bool noalloc = false);
+void noalloc_init(tribit_ctx* ctx);
+uint64_t noalloc_scratch_bytes();
 bool judged_known(const tribit_ctx* ctx, const tribit_graph* g, const tribit_count_opts& opts);
 void variant_stats(tribit_ctx* ctx, tribit_graph* g, int sched, tribit_variant_work* out);
 

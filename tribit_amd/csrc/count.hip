@@ -2120,7 +2120,24 @@ bool judged_known(const tribit_ctx* ctx, const tribit_graph* g, const tribit_cou
   return judged_lookup(ctx, g, opts) != nullptr;
 }
 
-void count(tribit_ctx* ctx, tribit_graph* g, const tribit_count_opts& opts, tribit_count_info* info) {
+// ---- the no-allocation count (tribit_count_noalloc) ------------------------
+// Its scratch: one file-scope device array per GPU, held until the process
+// ends. Host code never passes the symbol itself (on the host it names a
+// stand-in, not device memory): noalloc_init looks its device address up once
+// per context, with the context's device current, and kernels get that.
+constexpr uint64_t kNoAllocScratchBytes = 128ull << 20;
+static __device__ __align__(256) unsigned char noalloc_scratch[kNoAllocScratchBytes];
+
+void noalloc_init(tribit_ctx* ctx) {
+  void* p = nullptr;
+  TB_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(noalloc_scratch)));
+  if (!p) fail(TRIBIT_ERR_HIP, "hipGetSymbolAddress returned no address for the scratch array");
+  ctx->noalloc_scratch = p;
+}
+
+uint64_t noalloc_scratch_bytes() { return kNoAllocScratchBytes; }
+
+void count(tribit_ctx* ctx, tribit_graph* g, const tribit_count_opts& opts, tribit_count_info* info, bool noalloc) {
   const int reps = opts.repeats > 0 ? opts.repeats : 1;
   std::vector<double> tt, ts, tk, tv[kSlots];
   uint64_t first = 0;
@@ -2128,6 +2145,9 @@ void count(tribit_ctx* ctx, tribit_graph* g, const tribit_count_opts& opts, trib
   tribit_count_info local;
   memset(&local, 0, sizeof(local));
   const bool learned = opts.sched == TRIBIT_SCHED_LEARNED;
+  // what would allocate: judging, learned tables, tiles, a dense core matrix
+  if (noalloc && (learned || opts.dense_core > 0 || (opts.flags & TRIBIT_COUNT_FORCE_TILES)))
+    fail(TRIBIT_ERR_ARG, "the no-allocation count runs PLAIN, DIRECTED or WEDGE without a dense core or tiles");
   CandidateRun run;
   int chosen = -1;
   double ms_judge = 0.0;
@@ -2193,12 +2213,22 @@ void count(tribit_ctx* ctx, tribit_graph* g, const tribit_count_opts& opts, trib
     run.core_c = (opts.dense_core > 0 && g->relabeled) ? std::min<uint64_t>((uint64_t)opts.dense_core, g->n) : 0;
   }
   std::vector<double> tcore;
-  ensure_layout(ctx, g, run.layout);  // first use of a layout: built outside the timing
+  Arena arena;
+  if (noalloc) {
+    if (!layout_built(g, run.layout))
+      fail(TRIBIT_ERR_ARG, "the no-allocation count needs the schedule's range layout resident: count the graph "
+                           "once with tribit_count and this schedule first");
+    arena.base = static_cast<char*>(ctx->noalloc_scratch);
+    arena.size = kNoAllocScratchBytes;
+    if (!arena.base) fail(TRIBIT_ERR_HIP, "no scratch array address for this device");
+  }
+  if (!noalloc) ensure_layout(ctx, g, run.layout);  // first use of a layout: built outside the timing
   for (int r = 0; r < reps; ++r) {
     double a, b, c, v[kSlots] = {0}, mc = 0.0;
     Plan plan = run.plan;
+    arena.used = 0;  // every repeat reuses the whole array
     const uint64_t tri = count_once(ctx, g, run.layout, opts.flags | run.flags, plan, run.model, &a, &b, &c, v,
-                                    &local, nullptr, run.core_c, &mc);
+                                    &local, nullptr, run.core_c, &mc, noalloc ? &arena : nullptr);
     if (r == reps - 1) {
       run.plan = plan;
       cache_of(g)->last[opts.sched & 3] = plan;  // what ran, for tribit_variant_stats
