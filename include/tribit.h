@@ -271,14 +271,19 @@ int tribit_variant_stats(tribit_ctx* ctx, tribit_graph* g, int sched, tribit_var
 
 /* ---- learned scheduler: row-block buckets, menu, timing logs --------------
  * Owner lists are grouped into TRIBIT_BUCKETS buckets (outdeg bin x bin of the
- * mean streamed-range length). A plan maps each bucket to a menu slot
- * (kernel kind + lane-group width). LEARNED picks, per bucket, the feasible
- * slot of lowest cost predicted by the cost model (tribit_set_cost_model);
- * without a model, or for a bucket whose prediction is not finite, it keeps
- * the density-threshold heuristic (outdeg <= 32 / <= 256 / larger). */
+ * mean streamed-range length) and the buckets into TRIBIT_CLASSES classes by
+ * owner outdeg (<= 32, <= 256, larger: the density-threshold heuristic's
+ * split). A plan maps each bucket to a menu slot (kernel kind + lane-group
+ * width); LEARNED maps each class to one slot: the heuristic's, unless the
+ * cost model (tribit_set_cost_model) predicts a feasible slot at least about
+ * 3% faster for the whole count. The model's inputs are the class's features
+ * (tribit_class_features) and its target the log2 ratio of the count's device
+ * time with the class moved to that slot over the heuristic plan's
+ * (tribit_count_plan). */
 #define TRIBIT_BUCKETS 800
 #define TRIBIT_MENU_SLOTS 8
-#define TRIBIT_FEATURES 16
+#define TRIBIT_FEATURES 24
+#define TRIBIT_CLASSES 3
 typedef struct {
   uint64_t lists[TRIBIT_BUCKETS];     /* owner lists in the bucket */
   uint64_t sum_d[TRIBIT_BUCKETS];     /* sum of owner outdegrees */
@@ -297,9 +302,15 @@ typedef struct {
  * over the range layout `sched` (PLAIN, DIRECTED or WEDGE) and logs every
  * task's duration into out->ticks[slot][bucket]. */
 int tribit_count_profile(tribit_ctx* ctx, tribit_graph* g, int sched, const uint8_t* table, tribit_profile* out);
-/* Feature vector (TRIBIT_FEATURES floats) of (bucket, slot) from a profile:
- * the inputs the cost model is trained on and scored with. */
-int tribit_profile_features(const tribit_profile* p, int bucket, int slot, float* features);
+/* Device time of the counting launches (median over `repeats` >= 1 counts, no
+ * task logging) under the menu plan `table` (slot per bucket; NULL = the
+ * heuristic) over the range layout `sched`; the cost model's training target. */
+int tribit_count_plan(tribit_ctx* ctx, tribit_graph* g, int sched, const uint8_t* table, int repeats,
+                      double* ms_kernel, uint64_t* triangles);
+/* Feature vector (TRIBIT_FEATURES floats) of (outdeg class, slot) from the
+ * bucket statistics of a profile: the inputs the cost model is trained on and
+ * scored with. The slot must be feasible for the whole class. */
+int tribit_class_features(const tribit_profile* p, int cls, int slot, float* features);
 /* Menu slot description: kind (0 wave/64-slot set, 1 wave/512-slot set,
  * 2 block), lane-group width and the largest owner outdeg it accepts. */
 int tribit_menu_slot(int slot, int* kind, int* group_width, uint32_t* max_outdeg);

@@ -466,6 +466,14 @@ def test_plans_profiles_and_learned_tables(gpu_ctx, oracle_mod):
                     assert g.profile(layout, table).triangles == want, (r, layout)
             heur = g.learned_plan()
             assert g.count(sched="learned").triangles == want
+            ms, tri = g.count_plan("wedge", None, 2)
+            assert tri == want and ms > 0
+            for c in range(tribit_amd.CLASSES):
+                for s in tribit_amd.class_slots(c):
+                    a = tribit_amd.heuristic_slots("wedge")
+                    a[c] = s
+                    assert g.count_plan("wedge", tribit_amd.class_table(a), 1)[1] == want, (c, s)
+                    assert np.all(np.isfinite(tribit_amd.class_features(p, c, s)))
             with gpu_ctx.mlp(seed=3) as m:
                 gpu_ctx.set_cost_model(m)
                 try:

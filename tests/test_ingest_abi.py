@@ -177,3 +177,32 @@ def test_cli_bad_file_fails_before_device(cli_built, tmp_path):
 def test_chung_lu_batch_params_match_oracle(lib_built, oracle_mod):
     for i in range(32):
         assert tribit_amd.chung_lu_batch_params(3, i) == oracle_mod.chung_lu_batch_params(3, i)
+
+
+def test_class_features_host(lib_built):
+    """Cost-model features of (outdeg class, slot) from bucket statistics (host code, no device): the
+    class's size, outdeg min / median / max by outdeg bin, work share and the one-hot class, kind and
+    heuristic-slot markers; a slot the class does not fit is refused."""
+    p = tribit_amd.Profile()
+    # class 0: 10 lists of outdeg <= 1 (bin 0) and 30 of outdeg 25..32 (bin 9); class 2: 5 lists in bin 20
+    for b, lists, w, k in ((0 * 32 + 3, 10, 100, 10), (9 * 32 + 5, 30, 900, 300), (20 * 32 + 7, 5, 5000, 500)):
+        p.lists[b], p.sum_w[b], p.sum_k[b] = lists, w, k
+        p.sum_d[b] = lists * tribit_amd.bucket_max_outdeg(b)
+    f = tribit_amd.class_features(p, 0, 1)
+    assert f.shape == (tribit_amd.FEATURES,) and np.all(np.isfinite(f))
+    assert f[0] == pytest.approx(np.log2(41) / 8)
+    assert f[7] == pytest.approx(np.log2(2) / 4)  # smallest outdeg bin: top 1
+    assert f[8] == pytest.approx(np.log2(33) / 4)  # median list in the bin of top 32
+    assert f[9] == pytest.approx(np.log2(33) / 4)
+    assert f[11] == pytest.approx((1000 + 310) / (1000 + 310 + 5500))
+    assert list(f[12:15]) == [1, 0, 0] and list(f[15:20]) == [0, 1, 0, 0, 0] and f[21] == 0
+    assert tribit_amd.class_features(p, 0, 0)[21] == 1  # the heuristic's slot of the class
+    f2 = tribit_amd.class_features(p, 2, 5)
+    assert list(f2[12:15]) == [0, 0, 1] and f2[18] == 1 and f2[21] == 1
+    assert f2[7] == f2[8] == f2[9] == pytest.approx(np.log2(1537) / 4)
+    empty = tribit_amd.class_features(p, 1, 3)
+    assert empty[0] == 0 and empty[11] == 0
+    assert sorted(tribit_amd.class_slots(1)) == [2, 3, 4, 5, 6, 7] and tribit_amd.class_slots(2) == [4, 5, 6, 7]
+    for cls, slot in ((1, 0), (2, 3), (3, 4), (0, 8)):
+        with pytest.raises(tribit_amd.TribitError):
+            tribit_amd.class_features(p, cls, slot)

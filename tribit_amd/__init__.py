@@ -94,7 +94,8 @@ class BatchCountInfo(ctypes.Structure):
 
 BUCKETS = 800
 MENU_SLOTS = 8
-FEATURES = 16
+FEATURES = 24
+CLASSES = 3
 
 
 class Profile(ctypes.Structure):
@@ -179,7 +180,9 @@ def load_library(path: str | None = None) -> ctypes.CDLL:
         "tribit_count_noalloc": (ctypes.c_int, [_P, _P, ctypes.POINTER(CountOpts), ctypes.POINTER(CountInfo)]),
         "tribit_noalloc_scratch_bytes": (_U64, []),
         "tribit_count_profile": (ctypes.c_int, [_P, _P, ctypes.c_int, _P, ctypes.POINTER(Profile)]),
-        "tribit_profile_features": (ctypes.c_int, [ctypes.POINTER(Profile), ctypes.c_int, ctypes.c_int, _P]),
+        "tribit_class_features": (ctypes.c_int, [ctypes.POINTER(Profile), ctypes.c_int, ctypes.c_int, _P]),
+        "tribit_count_plan": (ctypes.c_int, [_P, _P, ctypes.c_int, _P, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                                             ctypes.POINTER(_U64)]),
         "tribit_menu_slot": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                             ctypes.POINTER(ctypes.c_uint32)]),
         "tribit_bucket_max_outdeg": (ctypes.c_uint32, [ctypes.c_int]),
@@ -273,10 +276,33 @@ def partial_last_info(ctx: "Context") -> dict:
     return {f: getattr(pi, f) for f, _ in PartialInfo._fields_}
 
 
-def profile_features(p: Profile, bucket: int, slot: int) -> np.ndarray:
+def class_features(p: Profile, cls: int, slot: int) -> np.ndarray:
+    """Cost-model features of (owner-outdeg class, menu slot) from a profile's bucket statistics."""
     f = np.zeros(FEATURES, np.float32)
-    _check(load_library().tribit_profile_features(ctypes.byref(p), bucket, slot, f.ctypes.data), "profile_features")
+    _check(load_library().tribit_class_features(ctypes.byref(p), cls, slot, f.ctypes.data), "class_features")
     return f
+
+
+def outdeg_class(bucket: int) -> int:
+    """The heuristic's class of a bucket: owner outdeg <= 32 (0), <= 256 (1), larger (2)."""
+    top = bucket_max_outdeg(bucket)
+    return 0 if top <= 32 else 1 if top <= 256 else 2
+
+
+def class_slots(cls: int) -> list:
+    """Menu slots that every bucket of the class fits."""
+    top = (32, 256, 0xFFFFFFFF)[cls]
+    return [s for s, (_, _, dmax) in enumerate(menu()) if top <= dmax]
+
+
+def heuristic_slots(sched: str = "wedge") -> list:
+    """The density-threshold heuristic's menu slot per outdeg class."""
+    return [0, 3, 5 if sched == "wedge" else 6]
+
+
+def class_table(assign) -> np.ndarray:
+    """Menu plan (slot per bucket) that runs class c on assign[c]."""
+    return np.array([assign[outdeg_class(b)] for b in range(BUCKETS)], np.uint8)
 
 
 def menu():
@@ -510,6 +536,18 @@ class Graph:
         _check(self.ctx.lib.tribit_count_profile(self.ctx.h, self.h, SCHED[sched], tp, ctypes.byref(p)),
                "count_profile")
         return p
+
+    def count_plan(self, sched: str = "wedge", table=None, repeats: int = 3):
+        """(median device ms of the counting launches, triangles) under a menu plan (None = heuristic)."""
+        tp = None
+        if table is not None:
+            t = np.ascontiguousarray(table, np.uint8)
+            assert t.size == BUCKETS
+            tp = t.ctypes.data
+        ms, tri = ctypes.c_double(), _U64()
+        _check(self.ctx.lib.tribit_count_plan(self.ctx.h, self.h, SCHED[sched], tp, repeats, ctypes.byref(ms),
+                                              ctypes.byref(tri)), "count_plan")
+        return ms.value, int(tri.value)
 
     def learned_plan(self) -> np.ndarray:
         t = np.zeros(BUCKETS, np.uint8)

@@ -585,10 +585,23 @@ int tribit_count_profile(tribit_ctx* ctx, tribit_graph* g, int sched, const uint
   });
 }
 
-int tribit_profile_features(const tribit_profile* p, int bucket, int slot, float* features) {
-  if (!p || !features || bucket < 0 || bucket >= TRIBIT_BUCKETS || slot < 0 || slot >= TRIBIT_MENU_SLOTS)
+int tribit_count_plan(tribit_ctx* ctx, tribit_graph* g, int sched, const uint8_t* table, int repeats,
+                      double* ms_kernel, uint64_t* triangles) {
+  if (!ctx || !g || g->ctx != ctx || !ms_kernel || !triangles || repeats < 1) return TRIBIT_ERR_ARG;
+  if (sched != TRIBIT_SCHED_PLAIN && sched != TRIBIT_SCHED_DIRECTED && sched != TRIBIT_SCHED_WEDGE)
     return TRIBIT_ERR_ARG;
-  tb::profile_features(p, bucket, slot, features);
+  if (sched == TRIBIT_SCHED_WEDGE && !g->rng_sfx) return TRIBIT_ERR_ARG;
+  return guarded(ctx, [&] {
+    TB_HIP(hipSetDevice(ctx->device));
+    tb::count_plan(ctx, g, sched, table, repeats, ms_kernel, triangles);
+  });
+}
+
+int tribit_class_features(const tribit_profile* p, int cls, int slot, float* features) {
+  if (!p || !features || cls < 0 || cls >= TRIBIT_CLASSES || slot < 0 || slot >= TRIBIT_MENU_SLOTS ||
+      !tb::class_slot_ok(cls, slot))
+    return TRIBIT_ERR_ARG;
+  tb::profile_class_features(p, cls, slot, features);
   return TRIBIT_OK;
 }
 

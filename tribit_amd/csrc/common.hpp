@@ -299,7 +299,10 @@ __host__ __device__ inline uint32_t range_end_mask(uint64_t m) {
   return m < (1ull << 30) ? 0x3FFFFFFFu : 0xFFFFFFFFu;
 }
 void count_profile(tribit_ctx* ctx, tribit_graph* g, int layout, const uint8_t* table, tribit_profile* out);
-void profile_features(const tribit_profile* p, int bucket, int slot, float* f);
+void profile_class_features(const tribit_profile* p, int cls, int slot, float* f);
+bool class_slot_ok(int cls, int slot);
+void count_plan(tribit_ctx* ctx, tribit_graph* g, int layout, const uint8_t* table, int repeats, double* ms_kernel,
+                uint64_t* triangles);
 void learned_plan(tribit_ctx* ctx, tribit_graph* g, uint8_t* table);
 void menu_slot(int slot, int* kind, int* g, uint32_t* max_d);
 uint32_t bucket_max_outdeg(int bucket);

@@ -1475,31 +1475,90 @@ void bucket_stats(tribit_ctx* ctx, tribit_graph* g, int sched, BucketStats* out,
   c->st = *out;
 }
 
-// Feature vector of (bucket, menu slot); the same function builds training
-// samples and scores candidates.
-void features(const BucketStats& st, int b, int slot, float* f) {
-  const double L = (double)(st.lists[b] ? st.lists[b] : 1);
-  const double K = (double)(st.sum_k[b] ? st.sum_k[b] : 1);
-  f[0] = (float)std::log2(1.0 + st.sum_d[b] / L);
-  f[1] = (float)std::log2(1.0 + st.sum_k[b] / L);
-  f[2] = (float)std::log2(1.0 + st.sum_w[b] / L);
-  f[3] = (float)std::log2(1.0 + st.sum_w[b] / K);
-  f[4] = (float)(st.eligible[b] / L);
-  f[5] = (float)std::log2((1.0 + st.sum_d[b]) / (1.0 + st.sum_t[b]));  // owner-row elements per 64-column tile
-  const int kind = kMenuKind[slot], g = kMenuG[slot];
-  f[6] = kind == kKindWave64;
-  f[7] = kind == kKindWave512;
-  f[8] = kind_is_block(kind);
-  f[9] = (float)(std::log2((double)g) / 6.0);
-  f[10] = g == 4;
-  f[11] = g == 8;
-  f[12] = g == 16;
-  f[13] = g == 32;
-  f[14] = (float)(std::log2(1.0 + (st.sum_w[b] + st.sum_k[b]) / L) / 32.0);
-  f[15] = kind == kKindTile;
+// The heuristic's owner-outdeg classes (<= 32, <= 256, larger). The learned
+// plan runs one menu slot per class (one launch per class).
+constexpr int kClasses = TRIBIT_CLASSES;
+static int class_of_bucket(int b) {
+  const uint32_t top = d_bin_top(b / kLBins);
+  return top <= 32 ? 0 : top <= 256 ? 1 : 2;
 }
 
-double bucket_work(const BucketStats& st, int b) { return (double)st.sum_w[b] + (double)st.sum_k[b] + 1.0; }
+static uint32_t class_top(int c) { return c == 0 ? 32u : c == 1 ? 256u : 0xFFFFFFFFu; }
+
+// A slot every bucket of the class fits (its sets hold the class's largest outdeg).
+static bool class_slot_feasible(int c, int slot) { return class_top(c) <= kind_max_d(kMenuKind[slot]); }
+
+static int heuristic_slot(int c, int layout) {
+  uint8_t t[kBuckets];
+  heuristic_table(t, layout);
+  for (int b = 0; b < kBuckets; ++b)
+    if (class_of_bucket(b) == c) return t[b];
+  return 0;
+}
+
+// Feature vector of (outdeg class, menu slot) over a graph's bucket
+// statistics; the same function builds training samples and scores plans.
+// The cost model's target: log2 of the count's device time with the class on
+// this slot and the other classes on the heuristic's slots, over the time of
+// the heuristic plan.
+void class_features(const BucketStats& st, int c, int slot, float* f) {
+  double L = 0, D = 0, K = 0, W = 0, E = 0, T = 0, Lg = 0, Wg = 0;
+  double hist[kDBins] = {};
+  for (int b = 0; b < kBuckets; ++b) {
+    if (!st.lists[b]) continue;
+    Lg += (double)st.lists[b];
+    Wg += (double)st.sum_w[b] + (double)st.sum_k[b];
+    if (class_of_bucket(b) != c) continue;
+    L += (double)st.lists[b];
+    D += (double)st.sum_d[b];
+    K += (double)st.sum_k[b];
+    W += (double)st.sum_w[b];
+    E += (double)st.eligible[b];
+    T += (double)st.sum_t[b];
+    hist[b / kLBins] += (double)st.lists[b];
+  }
+  // owner outdeg min / median / max over the class's lists, by outdeg bin
+  // (the bin's top; the open last bin counts as 16384)
+  auto top_of = [](int db) { return db < kDBins - 1 ? (double)d_bin_top(db) : 16384.0; };
+  double dmin = 0, dmed = 0, dmax = 0, acc = 0;
+  bool seen = false;
+  for (int db = 0; db < kDBins; ++db) {
+    if (hist[db] <= 0) continue;
+    if (!seen) dmin = top_of(db);
+    seen = true;
+    dmax = top_of(db);
+    if (acc < 0.5 * L && acc + hist[db] >= 0.5 * L) dmed = top_of(db);
+    acc += hist[db];
+  }
+  // log2 magnitudes scaled to a few units (the perceptron takes raw inputs)
+  const double Lc = L > 0 ? L : 1.0;
+  f[0] = (float)(std::log2(1.0 + L) / 8.0);
+  f[1] = (float)(std::log2(1.0 + D / Lc) / 4.0);                 // mean owner outdeg
+  f[2] = (float)(std::log2(1.0 + K / Lc) / 4.0);                 // ranges per list
+  f[3] = (float)(std::log2(1.0 + W / (K > 0 ? K : 1.0)) / 4.0);  // elements per range
+  f[4] = (float)(std::log2(1.0 + W + K) / 8.0);                  // the class's work
+  f[5] = (float)(E / Lc);                                        // bitmap-window eligible share
+  f[6] = (float)(std::log2((1.0 + D) / (1.0 + T)) / 4.0);        // owner-row elements per 64-column tile
+  f[7] = (float)(std::log2(1.0 + dmin) / 4.0);
+  f[8] = (float)(std::log2(1.0 + dmed) / 4.0);
+  f[9] = (float)(std::log2(1.0 + dmax) / 4.0);
+  f[10] = (float)(std::log2(1.0 + L / 256.0) / 8.0);             // lists per CU
+  f[11] = (float)((W + K) / (Wg > 0 ? Wg : 1.0));                // the class's share of the graph's work
+  f[12] = c == 0;
+  f[13] = c == 1;
+  f[14] = c == 2;
+  const int kind = kMenuKind[slot], g = kMenuG[slot];
+  f[15] = kind == kKindWave64;
+  f[16] = kind == kKindWaveMulti;
+  f[17] = kind == kKindWave512;
+  f[18] = kind == kKindBlock;
+  f[19] = kind == kKindTile;
+  f[20] = (float)(std::log2((double)g) / 4.0);
+  f[21] = slot == heuristic_slot(c, TRIBIT_SCHED_WEDGE);
+  f[22] = (float)(std::log2(1.0 + Lg) / 8.0);
+  f[23] = (float)(std::log2(1.0 + Wg) / 8.0);
+  static_assert(kFeatures == 24, "feature count");
+}
 
 static void stats_of_profile(const tribit_profile* p, BucketStats* st) {
   memcpy(st->lists, p->lists, sizeof(st->lists));
@@ -1510,11 +1569,13 @@ static void stats_of_profile(const tribit_profile* p, BucketStats* st) {
   memcpy(st->sum_t, p->sum_t, sizeof(st->sum_t));
 }
 
-void profile_features(const tribit_profile* p, int bucket, int slot, float* f) {
+void profile_class_features(const tribit_profile* p, int cls, int slot, float* f) {
   BucketStats st;
   stats_of_profile(p, &st);
-  features(st, bucket, slot, f);
+  class_features(st, cls, slot, f);
 }
+
+bool class_slot_ok(int cls, int slot) { return class_slot_feasible(cls, slot); }
 
 void menu_slot(int slot, int* kind, int* g, uint32_t* max_d) {
   if (kind) *kind = kMenuKind[slot];
@@ -1525,54 +1586,51 @@ void menu_slot(int slot, int* kind, int* g, uint32_t* max_d) {
 uint32_t bucket_max_outdeg(int bucket) { return d_bin_top(bucket / kLBins); }
 
 // LEARNED: one menu slot per outdeg class of the heuristic (<= 32, <= 256,
-// larger), the slot feasible for every bucket of the class with the lowest
-// predicted class cost: the sum over the class's buckets of
-// 2^model(features) * work. One launch per class keeps the per-bucket costs,
-// which were measured as shares of whole launches, additive: a plan that
-// scatters a class's buckets over several launches pays launch tails and
-// contention that no per-bucket figure holds. Without a model, or when a
-// class has no slot with finite predictions for all its buckets, the
-// heuristic's slot.
+// larger). The heuristic's slot stays unless the cost model predicts, for the
+// whole count, a feasible slot of the class at least kSwitch faster (predicted
+// log2 time ratios, each against the heuristic plan, compared with the
+// heuristic slot's own prediction). Classes are judged one at a time against
+// the heuristic plan and the switches combined. Without a model, a class
+// without lists or a prediction that is not finite: the heuristic's slot.
+constexpr double kSwitch = 0.04;  // log2 units: a predicted gain of about 3%
+
 static void learned_table(tribit_ctx* ctx, const BucketStats& st, int layout, int flags, uint8_t* table) {
   heuristic_table(table, layout);
   if (!ctx->model || layout != TRIBIT_SCHED_WEDGE) return;  // the model is trained on wedge-schedule timings
+  bool any[kClasses] = {};
+  for (int b = 0; b < kBuckets; ++b)
+    if (st.lists[b]) any[class_of_bucket(b)] = true;
   std::vector<float> X;
   std::vector<std::pair<int, int>> ids;
-  for (int b = 0; b < kBuckets; ++b) {
-    if (!st.lists[b]) continue;
+  for (int c = 0; c < kClasses; ++c) {
+    if (!any[c]) continue;
     for (int s = 0; s < kSlots; ++s) {
-      if (!slot_feasible(s, b) || ((flags & TRIBIT_COUNT_NO_TILES) && kMenuKind[s] == kKindTile)) continue;
+      if (!class_slot_feasible(c, s) || ((flags & TRIBIT_COUNT_NO_TILES) && kMenuKind[s] == kKindTile)) continue;
       float f[kFeatures];
-      features(st, b, s, f);
+      class_features(st, c, s, f);
       X.insert(X.end(), f, f + kFeatures);
-      ids.push_back({b, s});
+      ids.push_back({c, s});
     }
   }
   if (ids.empty()) return;
   std::vector<float> y(ids.size());
   mlp_infer(ctx->model, X.data(), ids.size(), y.data());
-  constexpr int kClasses = 3;
-  auto class_of = [](int b) {
-    const uint32_t top = d_bin_top(b / kLBins);
-    return top <= 32 ? 0 : top <= 256 ? 1 : 2;
-  };
-  double cost[kClasses][kSlots] = {};
-  int covered[kClasses][kSlots] = {}, need[kClasses] = {};
-  for (int b = 0; b < kBuckets; ++b)
-    if (st.lists[b]) ++need[class_of(b)];
-  for (size_t i = 0; i < ids.size(); ++i) {
-    const int b = ids[i].first, s = ids[i].second, c = class_of(b);
-    if (!std::isfinite(y[i])) continue;
-    cost[c][s] += std::exp2((double)y[i]) * bucket_work(st, b);
-    ++covered[c][s];
-  }
   for (int c = 0; c < kClasses; ++c) {
-    int best = -1;
-    for (int s = 0; s < kSlots; ++s)
-      if (need[c] && covered[c][s] == need[c] && (best < 0 || cost[c][s] < cost[c][best])) best = s;
-    if (best < 0) continue;
+    const int h = heuristic_slot(c, layout);
+    double yh = NAN;
+    for (size_t i = 0; i < ids.size(); ++i)
+      if (ids[i].first == c && ids[i].second == h) yh = y[i];
+    if (!std::isfinite(yh)) continue;
+    int best = h;
+    double yb = yh;
+    for (size_t i = 0; i < ids.size(); ++i)
+      if (ids[i].first == c && std::isfinite(y[i]) && y[i] < yb) {
+        yb = y[i];
+        best = ids[i].second;
+      }
+    if (best == h || yb > yh - kSwitch) continue;
     for (int b = 0; b < kBuckets; ++b)
-      if (class_of(b) == c) table[b] = (uint8_t)best;
+      if (class_of_bucket(b) == c) table[b] = (uint8_t)best;
   }
 }
 
@@ -2296,6 +2354,36 @@ void count_profile(tribit_ctx* ctx, tribit_graph* g, int layout, const uint8_t* 
   int khz = 0;
   TB_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, ctx->device));
   out->tick_ns = khz > 0 ? 1e6 / khz : 10.0;
+}
+
+// Device time of the counting launches under a menu plan given per bucket
+// (NULL: the heuristic), median over `repeats` counts, no task logging: the
+// cost model's training target.
+void count_plan(tribit_ctx* ctx, tribit_graph* g, int layout, const uint8_t* table, int repeats, double* ms_kernel,
+                uint64_t* triangles) {
+  Plan base = menu_plan();
+  if (table) {
+    for (int b = 0; b < kBuckets; ++b) {
+      if (table[b] >= kSlots || !slot_feasible(table[b], b)) fail(TRIBIT_ERR_ARG, "plan slot not feasible for bucket");
+      base.table[b] = table[b];
+    }
+  } else {
+    heuristic_table(base.table, layout);
+  }
+  ensure_layout(ctx, g, layout);
+  std::vector<double> tk;
+  uint64_t tri = 0;
+  for (int r = 0; r < std::max(repeats, 1); ++r) {
+    Plan plan = base;
+    double a, b, c, v[kSlots] = {0};
+    const uint64_t t = count_once(ctx, g, layout, 0, plan, false, &a, &b, &c, v, nullptr, nullptr);
+    if (r && t != tri) fail(TRIBIT_ERR_HIP, "repeated counts disagree");
+    tri = t;
+    tk.push_back(c);
+  }
+  std::sort(tk.begin(), tk.end());
+  *ms_kernel = tk[tk.size() / 2];
+  *triangles = tri;
 }
 
 void learned_plan(tribit_ctx* ctx, tribit_graph* g, uint8_t* table) {
