@@ -44,6 +44,9 @@ def test_learned_exact_and_judged(model_ctx, oracle_mod, spec):
         cands = [t for t in first.ms_candidates if t > 0]
         assert first.ms_judge > 0 and len(cands) >= 2
         assert first.ms_candidates[first.candidate] == min(cands)
+        # the model's candidate is timed only when its plan differs from the heuristic's
+        same = np.array_equal(g.learned_plan(), tribit_amd.class_table(tribit_amd.heuristic_slots("wedge")))
+        assert (first.ms_candidates[0] == 0) == same
         again = g.count(sched="learned", repeats=3)
         assert again.triangles == want and again.candidate == first.candidate and again.ms_judge == 0
 

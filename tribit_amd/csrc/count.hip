@@ -2236,10 +2236,22 @@ void count(tribit_ctx* ctx, tribit_graph* g, const tribit_count_opts& opts, trib
         CandidateRun cr;
         if (!candidate(ctx, g, opts, c, &cr)) continue;
         ensure_layout(ctx, g, cr.layout);  // a layout's first use: built before the timed runs
+        if (c == kCandModel) {
+          // the model keeps the heuristic's slot for every class it predicts
+          // no gain for: then it is the heuristic candidate, not timed twice
+          BucketStats st;
+          bucket_stats(ctx, g, cr.layout, &st);
+          uint8_t mt[kBuckets], ht[kBuckets];
+          learned_table(ctx, st, cr.layout, opts.flags | cr.flags, mt);
+          heuristic_table(ht, cr.layout);
+          if (!memcmp(mt, ht, sizeof(mt))) continue;
+        }
         // twice: the first run also builds what the candidate caches (core
-        // matrix, bucket statistics, tiles); the second is the one compared
+        // matrix, bucket statistics, tiles); the second is the one compared,
+        // unless the first already took more than twice the best time
         double a = 0.0, b, k, v[kSlots] = {0};
         for (int rep = 0; rep < 2; ++rep) {
+          if (rep == 1 && a > 2.0 * best) break;
           Plan pl = cr.plan;
           const uint64_t tri = count_once(ctx, g, cr.layout, opts.flags | cr.flags, pl, cr.model, &a, &b, &k, v,
                                           nullptr, nullptr, cr.core_c, nullptr);

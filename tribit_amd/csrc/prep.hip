@@ -8,7 +8,7 @@
 //   deg[v] (wave-aggregated atomics)
 //   orient {a<b}: a->b iff deg(a) <= deg(b)       (rank = (deg, id))
 //   --sort--> row_ptr[n+1] (u32), col[m] (u32, ascending within a row)
-//   wrow[u] = sum over v in N+(u) of outdeg(v)     (scheduler input)
+//   W+ = sum over v of indeg(v) * outdeg(v); wedge ranges (relabelled graphs)
 // Every launch is a 1-D grid of 256-thread blocks (4 waves).
 #include <cstring>
 #include <rocprim/device/device_radix_sort.hpp>
@@ -100,6 +100,21 @@ __global__ void k_validate(const uint64_t* __restrict__ keys, uint64_t nk, uint6
   if (!(a < b && b < n)) atomicOr(bad, 1u);
 }
 
+// Checks every key (min << 32) | max (min < max < n) and packs it in place to
+// (min << kb) | max, so the sort covers 2 kb bits instead of 32 + kb. The
+// sentinel stays all ones and still sorts last: a packed key has min < max,
+// so its low 2 kb bits are never all ones.
+__global__ void k_validate_pack(uint64_t* __restrict__ keys, uint64_t nk, uint64_t n, uint32_t kb,
+                                uint32_t* __restrict__ bad) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nk) return;
+  const uint64_t k = keys[i];
+  if (k == kSentinel) return;
+  const uint64_t a = k >> 32, b = k & 0xFFFFFFFFull;
+  if (!(a < b && b < n)) atomicOr(bad, 1u);
+  keys[i] = (a << kb) | b;
+}
+
 // pos[i] = 1 iff sorted key i is the first of its value and not the sentinel.
 __global__ void k_head_flags(const uint64_t* __restrict__ k, uint64_t nk, uint32_t* __restrict__ pos) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -121,13 +136,13 @@ __global__ void k_scatter_heads(const uint64_t* __restrict__ k, uint64_t nk, con
 // endpoints are written to `hi`, sorted, and added the same way by
 // k_add_runs (one atomic per key took 37 ms for the 523M keys of R-MAT
 // scale 25).
-__global__ void k_degree_lo(const uint64_t* __restrict__ keys, uint64_t m, uint32_t* __restrict__ deg,
+__global__ void k_degree_lo(const uint64_t* __restrict__ keys, uint64_t m, uint32_t kb, uint32_t* __restrict__ deg,
                             uint32_t* __restrict__ hi) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = i < m;
   const uint64_t k = valid ? keys[i] : 0;
-  const uint32_t a = valid ? (uint32_t)(k >> 32) : 0xFFFFFFFFu;
-  if (valid) hi[i] = (uint32_t)k;
+  const uint32_t a = valid ? (uint32_t)(k >> kb) : 0xFFFFFFFFu;
+  if (valid) hi[i] = (uint32_t)(k & ((1ull << kb) - 1));
   const uint32_t run = wave_run_length(a, valid);
   if (run) atomicAdd(&deg[a], run);
 }
@@ -154,7 +169,7 @@ __global__ void k_orient(const uint64_t* __restrict__ keys, uint64_t m, const ui
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m) return;
   const uint64_t k = keys[i];
-  const uint32_t a = (uint32_t)(k >> 32), b = (uint32_t)k;
+  const uint32_t a = (uint32_t)(k >> kb), b = (uint32_t)(k & ((1ull << kb) - 1));
   okeys[i] = (deg[a] <= deg[b]) ? arc_key(a, b, kb) : arc_key(b, a, kb);
 }
 
@@ -189,7 +204,7 @@ __global__ void k_orient_relabel(const uint64_t* __restrict__ keys, uint64_t m, 
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m) return;
   const uint64_t k = keys[i];
-  const uint32_t a = lab[(uint32_t)(k >> 32)], b = lab[(uint32_t)k];
+  const uint32_t a = lab[(uint32_t)(k >> kb)], b = lab[(uint32_t)(k & ((1ull << kb) - 1))];
   okeys[i] = a < b ? arc_key(a, b, kb) : arc_key(b, a, kb);
 }
 
@@ -211,71 +226,107 @@ struct RowStats {
   unsigned int pad;
 };
 
-// Rows with outdeg >= 2 and the largest outdeg (coalesced over row_ptr).
-__global__ void k_row_flags(const uint32_t* __restrict__ row_ptr, uint64_t n, RowStats* __restrict__ st) {
-  const uint64_t u = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t d = u < n ? row_ptr[u + 1] - row_ptr[u] : 0;
-  const uint64_t act = (uint64_t)__popcll(__ballot(d >= 2));
-  const uint32_t dm = wave_max_u32(d);
+// Totals over a bounded grid (grid-stride loops, one atomic per block and
+// field): one atomic per wave on a single address serialises at about 13 ns
+// each, 7 ms for 2^25 rows and 100 ms for 2^29 arcs.
+constexpr unsigned kTotalsBlocks = 1024;
+inline unsigned totals_grid(uint64_t n) {
+  return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(kTotalsBlocks, (n + kBlock - 1) / kBlock));
+}
+
+// Rows with outdeg >= 2, the largest outdeg and, with deg (undirected
+// degrees), W+ = sum over v of indeg(v) * outdeg(v): every arc u -> v streams
+// N+(v) once, and v has deg(v) - outdeg(v) in-arcs.
+__global__ void __launch_bounds__(kBlock) k_row_flags(const uint32_t* __restrict__ row_ptr,
+                                                      const uint32_t* __restrict__ deg, uint64_t n,
+                                                      RowStats* __restrict__ st) {
+  __shared__ unsigned long long s_act[kBlock / 64], s_w[kBlock / 64];
+  __shared__ uint32_t s_max[kBlock / 64];
+  uint64_t act = 0, w = 0;
+  uint32_t dm = 0;
+  for (uint64_t u = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; u < n; u += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t d = row_ptr[u + 1] - row_ptr[u];
+    act += d >= 2;
+    dm = d > dm ? d : dm;
+    if (deg) w += (uint64_t)(deg[u] - d) * d;
+  }
+  act = wave_sum_u64(act);
+  w = wave_sum_u64(w);
+  dm = wave_max_u32(dm);
   if (lane_id() == 0) {
+    s_act[threadIdx.x >> 6] = act;
+    s_w[threadIdx.x >> 6] = w;
+    s_max[threadIdx.x >> 6] = dm;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (unsigned i = 1; i < kBlock / 64; ++i) {
+      act += s_act[i];
+      w += s_w[i];
+      dm = s_max[i] > dm ? s_max[i] : dm;
+    }
     if (act) atomicAdd(&st->rows_active, (unsigned long long)act);
+    if (w) atomicAdd(&st->wplus, (unsigned long long)w);
     if (dm) atomicMax(&st->max_outdeg, dm);
   }
 }
 
-// One pass over the arcs e = (u -> v) in CSR order (okeys sorted, packed
-// (u << kb) | v), every lane one arc:
-//   wrow[u] += outdeg(v): W+ per row, one atomic per run of u in a wave, and
-//     the wave's total to st->wplus (wrow zeroed by the caller);
-//   rend[e] = row_ptr[u + 1] (relabelled graphs): the end of arc e's wedge
-//     range [e + 1, rend[e]), the part of N+(u) after v;
-//   nar[e] (relabelled graphs, m < 2^30): that range's narrow record: bit 31
-//     its ids share their high 16 bits H; bit 30 (with 31) shorter than 2^16
-//     ids with high halves H then H + 1, bits 0..15 the offset of the first
-//     id with H + 1; bits 16..23 H - (v >> 16), 255 when larger. In CSR order
-//     the range's ids are the row's next ones, so the reads stay in cache.
-__global__ void k_arc_stats(const uint64_t* __restrict__ okeys, uint64_t m, uint32_t kb,
-                            const uint32_t* __restrict__ row_ptr, const uint32_t* __restrict__ col,
-                            unsigned long long* __restrict__ wrow, RowStats* __restrict__ st,
-                            uint32_t* __restrict__ rend, uint32_t* __restrict__ nar) {
+// Per row u of a relabelled graph: the position where the row's last 64K-id
+// window begins (the first id whose high 16 bits equal the last id's), for
+// the narrow records of its wedge ranges. One lower-bound search per row.
+__global__ void k_row_split(const uint32_t* __restrict__ row_ptr, const uint32_t* __restrict__ col, uint64_t n,
+                            uint32_t* __restrict__ split) {
+  const uint64_t u = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= n) return;
+  const uint32_t b = row_ptr[u], e = row_ptr[u + 1];
+  uint32_t lo = b;
+  if (b < e) {
+    const uint32_t l = col[e - 1] >> 16;
+    uint32_t hi = e - 1;  // col[hi] is in window l
+    while (lo < hi) {
+      const uint32_t mid = lo + ((hi - lo) >> 1);
+      if ((col[mid] >> 16) >= l)
+        hi = mid;
+      else
+        lo = mid + 1;
+    }
+  }
+  split[u] = lo;
+}
+
+// Relabelled graphs, one pass over the arcs e = (u -> v) in CSR order (okeys
+// sorted, packed (u << kb) | v), every lane one arc, into arec[e]:
+//   .x = row_ptr[u + 1] (the end of arc e's wedge range [e + 1, end), the part
+//        of N+(u) after v), with bits 31 / 30 of the narrow record;
+//   .y (with nar, m < 2^30) = the rest of that range's narrow record: bit 31
+//        of .x: its ids share their high 16 bits H; bit 30 (with 31): shorter
+//        than 2^16 ids with high halves H then H + 1, .y bits 0..15 the offset
+//        of the first id with H + 1 (from the row's split position); .y bits
+//        16..23 H - (v >> 16), 255 when larger. In CSR order the range's ids
+//        are the row's next ones, so the reads stay in cache.
+// One 8-byte record per arc, so the wedge-range fill gathers once per range.
+__global__ void k_arc_rec(const uint64_t* __restrict__ okeys, uint64_t m, uint32_t kb,
+                          const uint32_t* __restrict__ row_ptr, const uint32_t* __restrict__ col,
+                          const uint32_t* __restrict__ rsplit, bool nar, uint2* __restrict__ arec) {
   const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool valid = e < m;
-  const uint64_t k = valid ? okeys[e] : 0;
-  const uint32_t u = valid ? (uint32_t)(k >> kb) : 0xFFFFFFFFu;
+  if (e >= m) return;
+  const uint64_t k = okeys[e];
+  const uint32_t u = (uint32_t)(k >> kb);
   const uint32_t v = (uint32_t)(k & ((1ull << kb) - 1));
-  const uint64_t w = valid ? (uint64_t)(row_ptr[v + 1] - row_ptr[v]) : 0;
-  uint64_t sum = 0;
-  if (wave_segment_tail(u, valid, w, &sum) && sum) atomicAdd(&wrow[u], (unsigned long long)sum);
-  const uint64_t ws = wave_sum_u64(w);
-  if (lane_id() == 0 && ws) atomicAdd(&st->wplus, (unsigned long long)ws);
-  if (!valid || !rend) return;
   const uint32_t end = row_ptr[u + 1];
-  rend[e] = end;
-  if (!nar) return;
-  uint32_t rec = 0;
+  uint32_t x = end, y = 0;
   const uint32_t b = (uint32_t)e + 1;
-  if (b < end) {
+  if (nar && b < end) {
     const uint32_t f = col[b] >> 16, l = col[end - 1] >> 16;
     const bool two = l == f + 1 && end - b < 65536u;
     if (f == l || two) {
-      uint32_t sp = 0;
-      if (two) {  // first position with high half l: col[lo] has f, col[hi] has l
-        uint32_t lo = b, hi = end - 1;
-        while (lo + 1 < hi) {
-          const uint32_t mid = lo + ((hi - lo) >> 1);
-          if ((col[mid] >> 16) == f)
-            lo = mid;
-          else
-            hi = mid;
-        }
-        sp = hi - b;
-      }
       uint32_t d = f - (v >> 16);  // ids after v: H >= v's high half
       d = d < 255u ? d : 255u;
-      rec = 0x80000000u | (two ? 0x40000000u : 0u) | (d << 16) | sp;
+      x |= 0x80000000u | (two ? 0x40000000u : 0u);
+      y = (d << 16) | (two ? rsplit[u] - b : 0u);
     }
   }
-  nar[e] = rec;
+  arec[e] = make_uint2(x, y);
 }
 
 // ------------------------------------------------------------ range lists
@@ -380,39 +431,54 @@ __global__ void k_rv_fill(const uint64_t* __restrict__ keys, uint64_t mr, const 
 }
 
 // Arcs e = (u -> v) sorted by head v: the suffix of N+(u) after v,
-// [e + 1, rend[e]), is streamed against N+(v). With ids in degree-rank order
+// [e + 1, end), is streamed against N+(v). With ids in degree-rank order
 // N+(v) holds only ids > v, so the elements of N+(u) up to v can never match.
-// sfx_cnt[v] += 1, w_sfx[v] += suffix length, aggregated per run of v in a wave.
-// With nar (the narrow records of k_arc_stats, by arc), wedge range j = [b, e)
-// is flagged narrow when its ids share their high 16 bits H (end bit 31), or,
-// shorter than 2^16 ids, when they have high half H up to position
-// b + split[j] and H + 1 from there (end bits 31 and 30); dh[j] = H - (v >> 16)
-// (255: not representable, the counting kernel reads col[b] instead).
+// w_sfx[v] += suffix length, aggregated per run of v in a wave.
+// With narrow records (arec of k_arc_rec), wedge range j = [b, e) is flagged
+// narrow when its ids share their high 16 bits H (end bit 31), or, shorter
+// than 2^16 ids, when they have high half H up to position b + split[j] and
+// H + 1 from there (end bits 31 and 30); dh[j] = H - (v >> 16) (255: not
+// representable, the counting kernel reads col[b] instead).
 __global__ void k_sfx_fill(const uint32_t* __restrict__ head, const uint32_t* __restrict__ arc, uint64_t m,
-                           const uint32_t* __restrict__ rend, const uint32_t* __restrict__ nar,
-                           uint2* __restrict__ rng, uint8_t* __restrict__ dh, uint16_t* __restrict__ split,
-                           uint32_t* __restrict__ sfx_cnt, unsigned long long* __restrict__ w_sfx) {
+                           const uint2* __restrict__ arec, uint2* __restrict__ rng, uint8_t* __restrict__ dh,
+                           uint16_t* __restrict__ split, unsigned long long* __restrict__ w_sfx) {
   const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = j < m;
   const uint32_t v = valid ? head[j] : 0xFFFFFFFFu;
   uint64_t len = 0;
   if (valid) {
-    const uint32_t e = arc[j], end = rend[e];
-    uint32_t y = end;
-    if (nar) {
-      const uint32_t rec = nar[e];
-      y |= rec & 0xC0000000u;
-      dh[j] = (uint8_t)(rec >> 16);
-      split[j] = (uint16_t)rec;
+    const uint32_t e = arc[j];
+    const uint2 r = arec[e];
+    if (dh) {
+      dh[j] = (uint8_t)(r.y >> 16);
+      split[j] = (uint16_t)r.y;
     }
-    rng[j] = make_uint2(e + 1, y);
-    len = end - e - 1;
+    rng[j] = make_uint2(e + 1, r.x);
+    len = (r.x & 0x3FFFFFFFu) - e - 1;
   }
   uint64_t sum = 0;
-  const bool tail = wave_segment_tail(v, valid, len, &sum);
-  const uint32_t run = wave_run_length(v, valid);
-  if (run) atomicAdd(&sfx_cnt[v], run);
-  if (tail && sum) atomicAdd(&w_sfx[v], (unsigned long long)sum);
+  if (wave_segment_tail(v, valid, len, &sum) && sum) atomicAdd(&w_sfx[v], (unsigned long long)sum);
+}
+
+// In-arcs per vertex: deg(v) - outdeg(v) (cnt[n] = 0 for the exclusive scan).
+__global__ void k_in_count(const uint32_t* __restrict__ row_ptr, const uint32_t* __restrict__ deg, uint64_t n,
+                           uint32_t* __restrict__ cnt) {
+  const uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v < n) cnt[v] = deg[v] - (row_ptr[v + 1] - row_ptr[v]);
+  if (v == n) cnt[v] = 0;
+}
+
+// W+(u) = sum of outdeg(v) over v in N+(u), per row (the PLAIN layout's work).
+__global__ void k_row_wplus(const uint32_t* __restrict__ row_ptr, const uint32_t* __restrict__ col, uint64_t n,
+                            uint64_t* __restrict__ wrow) {
+  const uint64_t u = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= n) return;
+  uint64_t w = 0;
+  for (uint32_t e = row_ptr[u]; e < row_ptr[u + 1]; ++e) {
+    const uint32_t v = col[e];
+    w += row_ptr[v + 1] - row_ptr[v];
+  }
+  wrow[u] = w;
 }
 
 // Low 16 bits of every id (narrow wedge ranges are streamed from these).
@@ -457,12 +523,14 @@ void preprocess(tribit_ctx* ctx, uint64_t* d_keys_in, uint64_t nkeys, uint64_t n
   g->entries = nkeys;
   TB_HIP(hipEventRecord(ctx->ev[0], s));
 
+  // keys checked and packed to (min << kb) | max for the sort
+  const uint32_t kb = bits_for(n);
   DevBuf<uint32_t> flag(ctx, 4);
   k_fill_u32<<<1, 64, 0, s>>>(flag.p, 4, 0u);
-  if (nkeys) k_validate<<<grid_for(nkeys, kBlock), kBlock, 0, s>>>(keys.p, nkeys, n, flag.p);
+  if (nkeys) k_validate_pack<<<grid_for(nkeys, kBlock), kBlock, 0, s>>>(keys.p, nkeys, n, kb, flag.p);
 
   // 1-2. sort + unique (tribit.cu:666-667). The sentinel sorts last.
-  const uint32_t end_bit = 32 + bits_for(n);
+  const uint32_t end_bit = 2 * kb;
   uint64_t m = 0;
   DevBuf<uint64_t> uniq(ctx, nkeys ? nkeys : 1);
   if (nkeys) {
@@ -512,7 +580,7 @@ void preprocess(tribit_ctx* ctx, uint64_t* d_keys_in, uint64_t nkeys, uint64_t n
   k_fill_u32<<<grid_for(n, kBlock), kBlock, 0, s>>>(deg.p, n, 0u);
   if (m) {
     DevBuf<uint32_t> hi(ctx, m), hi_sorted(ctx, m);
-    k_degree_lo<<<grid_for(m, kBlock), kBlock, 0, s>>>(uniq.p, m, deg.p, hi.p);
+    k_degree_lo<<<grid_for(m, kBlock), kBlock, 0, s>>>(uniq.p, m, kb, deg.p, hi.p);
     with_temp(ctx, [&](void* t, size_t& b) {
       return rocprim::radix_sort_keys(t, b, hi.p, hi_sorted.p, (size_t)m, 0u, bits_for(n), s);
     });
@@ -522,7 +590,6 @@ void preprocess(tribit_ctx* ctx, uint64_t* d_keys_in, uint64_t nkeys, uint64_t n
   // 3b. optional degree relabelling: internal id = rank of (deg, id), so the
   // orientation below becomes the id order and the out-neighbours of
   // high-degree vertices are packed into a narrow id range at the top.
-  const uint32_t kb = bits_for(n);
   DevBuf<uint32_t> labels;
   labels.ctx = ctx;
   DevBuf<uint64_t> okeys;  // oriented arcs, packed (u << kb) | v
@@ -573,22 +640,24 @@ void preprocess(tribit_ctx* ctx, uint64_t* d_keys_in, uint64_t nkeys, uint64_t n
     });
   }
 
-  // 7. per-row wedge counts and totals, one pass over the arcs; relabelled
-  // graphs also get every arc's row end and the narrow record of its wedge range
-  DevBuf<uint64_t> wrow(ctx, n);
+  // 7. totals (W+, active rows, largest outdeg) from the rows; relabelled
+  // graphs also get every arc's record (row end, narrow record of its wedge range)
   DevBuf<RowStats> st(ctx, 1);
   k_fill_u32<<<1, 64, 0, s>>>(reinterpret_cast<uint32_t*>(st.p), sizeof(RowStats) / 4, 0u);
-  k_fill_u64<<<grid_for(n, kBlock), kBlock, 0, s>>>(reinterpret_cast<unsigned long long*>(wrow.p), n);
+  k_row_flags<<<totals_grid(n), kBlock, 0, s>>>(row_ptr.p, deg.p, n, st.p);
   const bool narrow = ctx->relabel && m && m < (1ull << 30);
-  DevBuf<uint32_t> rend, nar;  // by arc: row end, narrow record (wedge ranges of relabelled graphs)
-  rend.ctx = nar.ctx = ctx;
-  if (ctx->relabel && m) rend = DevBuf<uint32_t>(ctx, m);
-  if (narrow) nar = DevBuf<uint32_t>(ctx, m);
-  if (m)
-    k_arc_stats<<<grid_for(m, kBlock), kBlock, 0, s>>>(uniq.p, m, kb, row_ptr.p, col.p,
-                                                       reinterpret_cast<unsigned long long*>(wrow.p), st.p, rend.p,
-                                                       nar.p);
-  k_row_flags<<<grid_for(n, kBlock), kBlock, 0, s>>>(row_ptr.p, n, st.p);
+  DevBuf<uint2> arec;
+  arec.ctx = ctx;
+  if (ctx->relabel && m) {
+    arec = DevBuf<uint2>(ctx, m);
+    DevBuf<uint32_t> rsplit;
+    rsplit.ctx = ctx;
+    if (narrow) {
+      rsplit = DevBuf<uint32_t>(ctx, n);
+      k_row_split<<<grid_for(n, kBlock), kBlock, 0, s>>>(row_ptr.p, col.p, n, rsplit.p);
+    }
+    k_arc_rec<<<grid_for(m, kBlock), kBlock, 0, s>>>(uniq.p, m, kb, row_ptr.p, col.p, rsplit.p, narrow, arec.p);
+  }
   uniq.reset();
   RowStats hst;
   TB_HIP(hipMemcpyAsync(&hst, st.p, sizeof(hst), hipMemcpyDeviceToHost, s));
@@ -613,7 +682,7 @@ void preprocess(tribit_ctx* ctx, uint64_t* d_keys_in, uint64_t nkeys, uint64_t n
     }
     k_fill_u64<<<grid_for(n, kBlock), kBlock, 0, s>>>(reinterpret_cast<unsigned long long*>(w_sfx.p), n);
     DevBuf<uint32_t> cnt(ctx, n + 1);
-    k_fill_u32<<<grid_for(n + 1, kBlock), kBlock, 0, s>>>(cnt.p, n + 1, 0u);
+    k_in_count<<<grid_for(n + 1, kBlock), kBlock, 0, s>>>(row_ptr.p, deg.p, n, cnt.p);
     if (m) {
       DevBuf<uint32_t> arcs(ctx, m), head(ctx, m), arc_sorted(ctx, m);
       k_iota_u32<<<grid_for(m, kBlock), kBlock, 0, s>>>(arcs.p, m);
@@ -621,9 +690,8 @@ void preprocess(tribit_ctx* ctx, uint64_t* d_keys_in, uint64_t nkeys, uint64_t n
         return rocprim::radix_sort_pairs(t, b, col.p, head.p, arcs.p, arc_sorted.p, (size_t)m, 0u, bits_for(n), s);
       });
       arcs.reset();
-      k_sfx_fill<<<grid_for(m, kBlock), kBlock, 0, s>>>(head.p, arc_sorted.p, m, rend.p, nar.p, rng_sfx.p, sfx_dh.p,
-                                                        sfx_split.p, cnt.p,
-                                                        reinterpret_cast<unsigned long long*>(w_sfx.p));
+      k_sfx_fill<<<grid_for(m, kBlock), kBlock, 0, s>>>(head.p, arc_sorted.p, m, arec.p, rng_sfx.p, sfx_dh.p,
+                                                        sfx_split.p, reinterpret_cast<unsigned long long*>(w_sfx.p));
     }
     with_temp(ctx, [&](void* t, size_t& b) {
       return rocprim::exclusive_scan(t, b, cnt.p, sfx_ptr.p, 0u, (size_t)(n + 1), rocprim::plus<uint32_t>(), s);
@@ -635,8 +703,7 @@ void preprocess(tribit_ctx* ctx, uint64_t* d_keys_in, uint64_t nkeys, uint64_t n
       k_col16<<<grid_for(m, kBlock), kBlock, 0, s>>>(col.p, m, reinterpret_cast<uint16_t*>(col16.p));
     }
   }
-  rend.reset();
-  nar.reset();
+  arec.reset();
 
   // the graph owns its buffers from here on (freed by tribit_graph_free, also on error)
   g->row_ptr = row_ptr.release();
@@ -644,14 +711,13 @@ void preprocess(tribit_ctx* ctx, uint64_t* d_keys_in, uint64_t nkeys, uint64_t n
   g->deg = deg.release();
   g->labels = labels.release();
   g->relabeled = ctx->relabel ? 1 : 0;
-  g->wrow = wrow.release();
   g->rng_sfx = rng_sfx.release();
   g->col16 = reinterpret_cast<uint16_t*>(col16.release());
   g->sfx_dh = sfx_dh.release();
   g->sfx_split = sfx_split.release();
   g->sfx_ptr = sfx_ptr.release();
   g->w_sfx = w_sfx.release();
-  g->bytes = 4 * (n + 1) + 4 * (m ? m : 1) + 4 * n + 8 * n + (g->labels ? 4 * n : 0) +
+  g->bytes = 4 * (n + 1) + 4 * (m ? m : 1) + 4 * n + (g->labels ? 4 * n : 0) +
              (g->rng_sfx ? 8 * (m ? m : 1) + 4 * (n + 1) + 8 * n : 0) + (g->col16 ? 4 * ((m + 1) / 2 + 1) + 3 * m : 0);
   // a graph without wedge ranges counts on the direction-split layout by default: built here
   if (!g->rng_sfx) build_layout(ctx, g, TRIBIT_SCHED_DIRECTED);
@@ -677,10 +743,13 @@ void build_layout(tribit_ctx* ctx, tribit_graph* g, int layout) {
   const uint64_t n = g->n, m = g->m;
   if (layout == TRIBIT_SCHED_PLAIN) {
     DevBuf<uint2> rng_all(ctx, m ? m : 1);
+    DevBuf<uint64_t> wrow(ctx, n ? n : 1);
     if (m) k_ranges_all<<<grid_for(m, kBlock), kBlock, 0, s>>>(g->row_ptr, g->col, m, rng_all.p);
+    if (n) k_row_wplus<<<grid_for(n, kBlock), kBlock, 0, s>>>(g->row_ptr, g->col, n, wrow.p);
     TB_HIP(hipGetLastError());
     g->rng_all = rng_all.release();
-    g->bytes += 8 * (m ? m : 1);
+    g->wrow = wrow.release();
+    g->bytes += 8 * (m ? m : 1) + 8 * (n ? n : 1);
     return;
   }
   if (layout != TRIBIT_SCHED_DIRECTED) fail(TRIBIT_ERR_ARG, "no such range layout");
@@ -785,10 +854,20 @@ __global__ void k_offdiag_keys(const uint64_t* __restrict__ keys, uint64_t nk, c
   out[i] = la > lb ? (((uint64_t)a << 32) | b) : (((uint64_t)b << 32) | a);
 }
 
-__global__ void k_sum_u64(const unsigned long long* __restrict__ v, uint64_t n, unsigned long long* __restrict__ out) {
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const uint64_t s = wave_sum_u64(i < n ? v[i] : 0ull);
-  if (lane_id() == 0 && s) atomicAdd(out, (unsigned long long)s);
+// Sum of v[0, n) into *out (launched on totals_grid(n) blocks of kBlock).
+__global__ void __launch_bounds__(kBlock) k_sum_u64(const unsigned long long* __restrict__ v, uint64_t n,
+                                                    unsigned long long* __restrict__ out) {
+  __shared__ unsigned long long s_part[kBlock / 64];
+  uint64_t acc = 0;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    acc += v[i];
+  acc = wave_sum_u64(acc);
+  if (lane_id() == 0) s_part[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (unsigned i = 1; i < kBlock / 64; ++i) acc += s_part[i];
+    if (acc) atomicAdd(out, (unsigned long long)acc);
+  }
 }
 
 // Sorted distinct keys of in[0, nk) (the sentinel dropped) into out; returns
@@ -889,9 +968,9 @@ void build_offdiag(tribit_ctx* ctx, const uint64_t* d_cols, uint64_t ncols, cons
   // totals: streamed elements, owner rows
   DevBuf<RowStats> st(ctx, 1);
   k_fill_u32<<<1, 64, 0, s>>>(reinterpret_cast<uint32_t*>(st.p), sizeof(RowStats) / 4, 0u);
-  k_sum_u64<<<grid_for(n, kBlock), kBlock, 0, s>>>(reinterpret_cast<const unsigned long long*>(w_rv.p), n,
-                                                   &st.p->wplus);
-  k_row_flags<<<grid_for(n, kBlock), kBlock, 0, s>>>(row_ptr.p, n, st.p);
+  k_sum_u64<<<totals_grid(n), kBlock, 0, s>>>(reinterpret_cast<const unsigned long long*>(w_rv.p), n,
+                                              &st.p->wplus);
+  k_row_flags<<<totals_grid(n), kBlock, 0, s>>>(row_ptr.p, nullptr, n, st.p);
   RowStats hst;
   TB_HIP(hipMemcpyAsync(&hst, st.p, sizeof(hst), hipMemcpyDeviceToHost, s));
   ctx->sync();
