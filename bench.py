@@ -48,11 +48,13 @@ sys.path.insert(0, ROOT)
 
 METRIC = "exact-TC GTEPS on 1 MI355X: learned-sched vs plain; % HBM roofline"
 HBM_PEAK_GBS = 8000.0
-def variant_name(kind: int, width: int, narrow: bool = False) -> str:
-    """Kernel instantiation of a launch of `kind` at lane-group width `width` (buffer-load path)."""
+def variant_name(kind: int, width: int, narrow: bool = False, tile_width: int = 64) -> str:
+    """Kernel instantiation of a launch of `kind` at lane-group width `width` (buffer-load path; block
+    template <log2 set slots, lanes per list, loads per lane, buffer loads, tile width (0: none), 16-bit
+    narrow ranges>)."""
     n16 = "true" if narrow else "false"
     return [f"k_count_wave<6,64,{width},8,true>", f"k_count_wave<9,256,{width},8,true>",
-            f"k_count_block<12,{width},8,true,false,{n16}>", f"k_count_block<12,{width},8,true,true,{n16}>",
+            f"k_count_block<12,{width},8,true,0,{n16}>", f"k_count_block<12,{width},8,true,{tile_width},{n16}>",
             f"k_count_wave_multi<6,64,8,{width},8,true>"][kind]
 
 
@@ -364,6 +366,12 @@ def batch_workload(args) -> int:
         log("batch counts disagree")
         return 1
     m = inf["m"]
+    # roofline of the whole replay: the wedge byte model of every part's lists over the replay time
+    work = b.variant_stats(sched)
+    nar = sum(work["narrow"])
+    b_all = (4.0 * sum(work["rows"]) + 4.0 * sum(work["edges"]) + 4.0 * (sum(work["wplus"]) - nar) + 2.0 * nar +
+             1.0 * sum(work["narrow_ranges"]) + 2.0 * sum(work["narrow2_ranges"]))
+    achieved = b_all / (after["ms_count"] * 1e-3) / 1e9
     oracle_check = None
     if args.cpu_full:
         oracle_check = batch_oracle_check(counts, inf["graphs"])
@@ -375,7 +383,15 @@ def batch_workload(args) -> int:
         "gteps": round(m / (after["ms_count"] * 1e-3) / 1e9, 4), "ms_per_replay": round(after["ms_count"], 4),
         "gteps_before_autotune": round(m / (first["ms_count"] * 1e-3) / 1e9, 4),
         "ms_per_replay_before_autotune": round(first["ms_count"], 4),
-        "autotuned": {"group_width": after["group_width"], "waves_per_cu": after["waves"]},
+        "autotuned": {"group_width": after["group_width"], "waves_per_cu": after["waves"],
+                      "tile_width": after.get("tile_width", 0)},
+        "roofline": {"bound": "hbm", "kernel": f"whole replay ({after['launches']} launches)",
+                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "algorithmic_bytes": b_all,
+                     "byte_model": "every part's wedge lists: 4 B per owner list and per streamed range, per "
+                                   "streamed element 4 B (32-bit id) or 2 B (16-bit half of a narrow range), 1 B "
+                                   "per narrow range, 2 B more per range across a 64K-id boundary; block launches "
+                                   "that stream bit tiles (tile_width > 0) read 12 B per tile instead"},
         "ms_preprocess_device": round(inf["ms_preprocess"], 2),
         "cpu_check": oracle_check,
     }), flush=True)
@@ -628,7 +644,7 @@ def main() -> int:
         "tasks_by_variant": sel["tasks_by_variant"],
         "learned": None if args.sched != "learned" else {
             "candidate": ["cost-model plan", "heuristic plan", "heuristic plan, directed layout",
-                          "textbook plain plan"][sel["candidate"]],
+                          "textbook plain plan", "heuristic plan with dense diagonal blocks"][sel["candidate"]],
             "layout": {0: "plain", 2: "directed", 3: "wedge"}[sel["layout"]],
             "dense_core_vertices": sel["dense_core"], "ms_dense_core": round(sel["ms_dense"], 4),
             "ms_judge": round(sel["ms_judge"], 3),

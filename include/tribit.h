@@ -374,6 +374,10 @@ int tribit_batch_count(tribit_ctx* ctx, tribit_batch* b, const tribit_count_opts
  * batch for later counts and is returned (tile_width, group_width, waves). */
 int tribit_batch_autotune(tribit_ctx* ctx, tribit_batch* b, const tribit_count_opts* opts,
                           tribit_batch_count_info* best);
+/* Work of every part's schedule `sched` summed over the batch (the units of
+ * tribit_variant_stats; no reference counterpart: the algorithmic bytes of a
+ * whole replay). */
+int tribit_batch_variant_stats(tribit_ctx* ctx, tribit_batch* b, int sched, tribit_variant_work* out);
 int tribit_batch_free(tribit_batch* b);
 
 /* ---- one rank of the reference's multi-GPU decomposition (main.rs:97-220) ---

@@ -217,6 +217,7 @@ def load_library(path: str | None = None) -> ctypes.CDLL:
                                             ctypes.POINTER(ctypes.c_double)]),
         "tribit_mlp_infer": (ctypes.c_int, [_P, _P, _U64, _P]),
         "tribit_variant_stats": (ctypes.c_int, [_P, _P, ctypes.c_int, ctypes.POINTER(VariantStats)]),
+        "tribit_batch_variant_stats": (ctypes.c_int, [_P, _P, ctypes.c_int, ctypes.POINTER(VariantStats)]),
         "tribit_count_keys": (ctypes.c_int, [_P, _P, _U64, _U64, ctypes.POINTER(CountOpts), _U64P,
                                              ctypes.POINTER(ctypes.c_double), ctypes.POINTER(CountInfo)]),
     }
@@ -480,6 +481,15 @@ class Batch:
         _check(self.ctx.lib.tribit_batch_count(self.ctx.h, self.h, ctypes.byref(o), out.ctypes.data,
                                                ctypes.byref(ci)), "batch_count")
         return out, _info_dict(ci)
+
+    def variant_stats(self, sched: str = "wedge") -> dict:
+        """Work of every part's schedule summed over the batch (units of Graph.variant_stats)."""
+        vs = VariantStats()
+        _check(self.ctx.lib.tribit_batch_variant_stats(self.ctx.h, self.h, SCHED[sched], ctypes.byref(vs)),
+               "batch_variant_stats")
+        return {"rows": list(vs.rows), "edges": list(vs.edges), "wplus": list(vs.wplus),
+                "narrow": list(vs.narrow), "narrow_ranges": list(vs.narrow_ranges),
+                "narrow2_ranges": list(vs.narrow2_ranges)}
 
     def autotune(self, sched: str = "learned", repeats: int = 3, flags: int = 0) -> dict:
         o = self._opts(sched, repeats, flags)

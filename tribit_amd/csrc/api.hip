@@ -676,6 +676,29 @@ int tribit_batch_graph_sizes(const tribit_batch* b, uint64_t* n, uint64_t* m) {
   return TRIBIT_OK;
 }
 
+int tribit_batch_variant_stats(tribit_ctx* ctx, tribit_batch* b, int sched, tribit_variant_work* out) {
+  if (!ctx || !b || !out || b->ctx != ctx) return TRIBIT_ERR_ARG;
+  if (sched < TRIBIT_SCHED_PLAIN || sched > TRIBIT_SCHED_WEDGE) return TRIBIT_ERR_ARG;
+  for (const auto& part : b->parts)
+    if (sched == TRIBIT_SCHED_WEDGE && !part.g->rng_sfx) return TRIBIT_ERR_ARG;
+  return guarded(ctx, [&] {
+    TB_HIP(hipSetDevice(ctx->device));
+    memset(out, 0, sizeof(*out));
+    for (const auto& part : b->parts) {
+      tribit_variant_work w;
+      tb::variant_stats(ctx, part.g, sched, &w);
+      for (int i = 0; i < 8; ++i) {
+        out->rows[i] += w.rows[i];
+        out->edges[i] += w.edges[i];
+        out->wplus[i] += w.wplus[i];
+        out->narrow[i] += w.narrow[i];
+        out->narrow_ranges[i] += w.narrow_ranges[i];
+        out->narrow2_ranges[i] += w.narrow2_ranges[i];
+      }
+    }
+  });
+}
+
 static int batch_opts(const tribit_count_opts* opts, const tribit_batch* b, tribit_count_opts* o) {
   memset(o, 0, sizeof(*o));
   o->sched = TRIBIT_SCHED_LEARNED;

@@ -1248,37 +1248,92 @@ __global__ __launch_bounds__(256) void k_bucket_stats(SchedView sv, const uint32
 
 // Per slot of a wedge plan: elements, ranges and two-window ranges of narrow
 // wedge ranges (the lists' kept ranges whose end carries the narrow flag).
+// A thread per owner walks lists of up to kNarrowLight ranges; longer lists
+// (hubs have up to millions of in-arcs) go to `heavy` (at most m / kNarrowLight
+// of them) and k_narrow_heavy walks each with a whole block.
+constexpr uint32_t kNarrowLight = 256;
+
+__device__ __forceinline__ void narrow_add(const uint2& r, unsigned long long& ne, unsigned long long& nr,
+                                           unsigned long long& n2) {
+  if (r.y & kNarrowFlag) {
+    ne += (r.y & 0x3FFFFFFFu) - r.x;  // flags exist only when m < 2^30
+    ++nr;
+    n2 += (r.y & kTwoWinFlag) ? 1u : 0u;
+  }
+}
+
 __global__ __launch_bounds__(256) void k_narrow_stats(SchedView sv, uint64_t n, const uint8_t* __restrict__ table,
-                                                      unsigned long long* __restrict__ out) {
+                                                      unsigned long long* __restrict__ out,
+                                                      uint4* __restrict__ heavy, uint32_t* __restrict__ nheavy) {
   __shared__ unsigned long long acc[3 * kSlots];
   if (threadIdx.x < 3 * kSlots) acc[threadIdx.x] = 0ull;
   __syncthreads();
   const uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  bool big = false;
+  uint32_t b = 0, k = 0;
+  int slot = 0;
   if (x < n) {
     const uint32_t d = sv.row_ptr[x + 1] - sv.row_ptr[x];
-    uint32_t b, k;
     uint64_t w;
     owner_list(sv, x, 0, &b, &k, &w);
     if (list_useful(sv, 0, d, k)) {
-      const int slot = table[bucket_of(d, w, k)];
-      unsigned long long ne = 0, nr = 0, n2 = 0;
-      for (uint32_t i = b; i < b + k; ++i) {
-        const uint2 r = sv.rng[i];
-        if (r.y & kNarrowFlag) {
-          ne += (r.y & 0x3FFFFFFFu) - r.x;  // flags exist only when m < 2^30
-          ++nr;
-          n2 += (r.y & kTwoWinFlag) ? 1u : 0u;
+      slot = table[bucket_of(d, w, k)];
+      big = k > kNarrowLight;
+      if (!big) {
+        unsigned long long ne = 0, nr = 0, n2 = 0;
+        for (uint32_t i = b; i < b + k; ++i) narrow_add(sv.rng[i], ne, nr, n2);
+        if (nr) {
+          atomicAdd(&acc[slot], ne);
+          atomicAdd(&acc[kSlots + slot], nr);
+          if (n2) atomicAdd(&acc[2 * kSlots + slot], n2);
         }
-      }
-      if (nr) {
-        atomicAdd(&acc[slot], ne);
-        atomicAdd(&acc[kSlots + slot], nr);
-        if (n2) atomicAdd(&acc[2 * kSlots + slot], n2);
       }
     }
   }
+  // heavy owners: one append per wave
+  const uint64_t hm = __ballot(big);
+  if (hm) {
+    uint32_t base = 0;
+    if (lane_id() == 0) base = atomicAdd(nheavy, (uint32_t)__popcll(hm));
+    base = __shfl(base, 0);
+    if (big) heavy[base + __popcll(hm & ((1ull << lane_id()) - 1))] = make_uint4(b, k, (uint32_t)slot, 0u);
+  }
   __syncthreads();
   if (threadIdx.x < 3 * kSlots && acc[threadIdx.x]) atomicAdd(&out[threadIdx.x], acc[threadIdx.x]);
+}
+
+__global__ __launch_bounds__(256) void k_narrow_heavy(SchedView sv, const uint4* __restrict__ heavy,
+                                                      const uint32_t* __restrict__ nheavy,
+                                                      unsigned long long* __restrict__ out) {
+  __shared__ unsigned long long part[3][256 / 64];
+  const uint32_t nh = *nheavy;
+  for (uint32_t h = blockIdx.x; h < nh; h += gridDim.x) {
+    const uint4 t = heavy[h];
+    unsigned long long ne = 0, nr = 0, n2 = 0;
+    for (uint32_t i = t.x + threadIdx.x; i < t.x + t.y; i += blockDim.x) narrow_add(sv.rng[i], ne, nr, n2);
+    ne = wave_sum_u64(ne);
+    nr = wave_sum_u64(nr);
+    n2 = wave_sum_u64(n2);
+    if (lane_id() == 0) {
+      part[0][threadIdx.x >> 6] = ne;
+      part[1][threadIdx.x >> 6] = nr;
+      part[2][threadIdx.x >> 6] = n2;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int i = 1; i < 256 / 64; ++i) {
+        ne += part[0][i];
+        nr += part[1][i];
+        n2 += part[2][i];
+      }
+      if (nr) {
+        atomicAdd(&out[t.z], ne);
+        atomicAdd(&out[kSlots + t.z], nr);
+        if (n2) atomicAdd(&out[2 * kSlots + t.z], n2);
+      }
+    }
+    __syncthreads();
+  }
 }
 
 // Timing logs: per (bucket) sum of the durations of a slot's tasks, from the
@@ -2461,11 +2516,14 @@ void variant_stats(tribit_ctx* ctx, tribit_graph* g, int sched, tribit_variant_w
   if (layout == TRIBIT_SCHED_WEDGE && g->col16 && g->m * 4 <= kBufMax && !(flags & TRIBIT_COUNT_WIDE_IDS) &&
       !(flags & TRIBIT_COUNT_FORCE_GLOBAL_LOADS) && g->n) {
     DevBuf<uint8_t> dt(ctx, kBuckets);
-    DevBuf<unsigned long long> ns(ctx, 3 * kSlots);
+    DevBuf<unsigned long long> ns(ctx, 3 * kSlots + 1);  // the last word: heavy-owner count
+    DevBuf<uint4> heavy(ctx, g->m / kNarrowLight + 1);
     TB_HIP(hipMemcpyAsync(dt.p, plan.table, kBuckets, hipMemcpyHostToDevice, ctx->stream));
-    k_zero_u64<<<1, 64, 0, ctx->stream>>>(ns.p, 3 * kSlots);
-    k_narrow_stats<<<grid_for(g->n, 256), 256, 0, ctx->stream>>>(sched_view(g, layout, d0, cut, blocks), g->n, dt.p,
-                                                                 ns.p);
+    k_zero_u64<<<1, 64, 0, ctx->stream>>>(ns.p, 3 * kSlots + 1);
+    uint32_t* nheavy = reinterpret_cast<uint32_t*>(ns.p + 3 * kSlots);
+    const SchedView sv = sched_view(g, layout, d0, cut, blocks);
+    k_narrow_stats<<<grid_for(g->n, 256), 256, 0, ctx->stream>>>(sv, g->n, dt.p, ns.p, heavy.p, nheavy);
+    k_narrow_heavy<<<2048, 256, 0, ctx->stream>>>(sv, heavy.p, nheavy, ns.p);
     unsigned long long h[3 * kSlots];
     TB_HIP(hipMemcpyAsync(h, ns.p, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
     ctx->sync();

@@ -57,8 +57,10 @@ __device__ __forceinline__ int mfma_row(int v, int l) { return 8 * (v >> 2) + 4 
 // c of tile block J of A[r][c] * (A A^T)[r][c] to *result.
 __device__ __forceinline__ void core_tile(const int8_t* __restrict__ A, uint32_t kp, uint32_t I, uint32_t J,
                                           unsigned long long* __restrict__ result) {
-  __shared__ __attribute__((aligned(16))) int8_t sA[kCoreTile][kCoreK + kCorePad];
-  __shared__ __attribute__((aligned(16))) int8_t sB[kCoreTile][kCoreK + kCorePad];
+  // two LDS stages: stage s is written while the MFMAs of stage s - 1 read
+  // the other buffer, so one barrier per stage orders both
+  __shared__ __attribute__((aligned(16))) int8_t sA[2][kCoreTile][kCoreK + kCorePad];
+  __shared__ __attribute__((aligned(16))) int8_t sB[2][kCoreTile][kCoreK + kCorePad];
   __shared__ unsigned long long sPart[kCoreThreads / 64];
   const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const uint32_t wr = w >> 1, wc = w & 1;
@@ -82,15 +84,18 @@ __device__ __forceinline__ void core_tile(const int8_t* __restrict__ A, uint32_t
   // the A operand (columns from J * 128): taken from LDS into one bit per
   // accumulator element instead of 64 byte reads per lane from global memory
   uint64_t msk = 0;
-  for (uint32_t k0 = k_begin; k0 < kp; k0 += kCoreK) {
-    *reinterpret_cast<uint4*>(&sA[lr][lh]) = pa0;
-    *reinterpret_cast<uint4*>(&sA[lr][lh + 16]) = pa1;
-    *reinterpret_cast<uint4*>(&sA[lr][lh + 32]) = pa2;
-    *reinterpret_cast<uint4*>(&sA[lr][lh + 48]) = pa3;
-    *reinterpret_cast<uint4*>(&sB[lr][lh]) = pb0;
-    *reinterpret_cast<uint4*>(&sB[lr][lh + 16]) = pb1;
-    *reinterpret_cast<uint4*>(&sB[lr][lh + 32]) = pb2;
-    *reinterpret_cast<uint4*>(&sB[lr][lh + 48]) = pb3;
+  uint32_t buf = 0;
+  for (uint32_t k0 = k_begin; k0 < kp; k0 += kCoreK, buf ^= 1u) {
+    // a wave stores stage s only after every wave passed the barrier of stage
+    // s - 1, that is after the MFMAs of stage s - 2 read this buffer
+    *reinterpret_cast<uint4*>(&sA[buf][lr][lh]) = pa0;
+    *reinterpret_cast<uint4*>(&sA[buf][lr][lh + 16]) = pa1;
+    *reinterpret_cast<uint4*>(&sA[buf][lr][lh + 32]) = pa2;
+    *reinterpret_cast<uint4*>(&sA[buf][lr][lh + 48]) = pa3;
+    *reinterpret_cast<uint4*>(&sB[buf][lr][lh]) = pb0;
+    *reinterpret_cast<uint4*>(&sB[buf][lr][lh + 16]) = pb1;
+    *reinterpret_cast<uint4*>(&sB[buf][lr][lh + 32]) = pb2;
+    *reinterpret_cast<uint4*>(&sB[buf][lr][lh + 48]) = pb3;
     __syncthreads();
     if (k0 == k_begin) {
 #pragma unroll
@@ -100,7 +105,7 @@ __device__ __forceinline__ void core_tile(const int8_t* __restrict__ A, uint32_t
 #pragma unroll
           for (int v = 0; v < 16; ++v) {
             const int r = wr * 64 + i * 32 + mfma_row(v, lane), c = wc * 64 + j * 32 + (lane & 31);
-            msk |= (uint64_t)(sA[r][c] != 0) << ((i * 2 + j) * 16 + v);
+            msk |= (uint64_t)(sA[0][r][c] != 0) << ((i * 2 + j) * 16 + v);
           }
     }
     if (k0 + kCoreK < kp) {  // next stage in flight during the MFMAs
@@ -115,15 +120,14 @@ __device__ __forceinline__ void core_tile(const int8_t* __restrict__ A, uint32_t
       v4i fa[2], fb[2];
 #pragma unroll
       for (int i = 0; i < 2; ++i) {
-        fa[i] = *reinterpret_cast<const v4i*>(&sA[wr * 64 + i * 32 + (lane & 31)][ko]);
-        fb[i] = *reinterpret_cast<const v4i*>(&sB[wc * 64 + i * 32 + (lane & 31)][ko]);
+        fa[i] = *reinterpret_cast<const v4i*>(&sA[buf][wr * 64 + i * 32 + (lane & 31)][ko]);
+        fb[i] = *reinterpret_cast<const v4i*>(&sB[buf][wc * 64 + i * 32 + (lane & 31)][ko]);
       }
 #pragma unroll
       for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[i], fb[j], acc[i][j], 0, 0, 0);
     }
-    __syncthreads();
   }
   // masked sum: (A A^T)[r][c] counts where A[r][c] = 1
   uint64_t sum = 0;

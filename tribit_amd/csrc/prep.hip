@@ -505,6 +505,16 @@ static void with_temp(tribit_ctx* ctx, F&& f) {
   TB_HIP(f(tmp.p, bytes));
 }
 
+// 9-bit digits for the sorts of 32-bit ids (three passes instead of four for
+// ids below 2^27). Measured on MI355X over 2^29 random 25-bit keys: pairs
+// with 32-bit values 14.7 -> 11.5 ms, keys alone 9.6 -> 8.0 ms. The 64-bit
+// key sorts keep rocPRIM's 8-bit digits (9 bits: 22.1 -> 23.4 ms; 10 bits
+// slower still).
+using IdSortConfig = rocprim::radix_sort_config<
+    rocprim::default_config, rocprim::default_config,
+    rocprim::radix_sort_onesweep_config<rocprim::kernel_config<1024, 16>, rocprim::kernel_config<1024, 16>, 9,
+                                        rocprim::block_radix_rank_algorithm::match>>;
+
 static void sort_keys(tribit_ctx* ctx, const uint64_t* in, uint64_t* out, uint64_t nk, uint32_t end_bit) {
   with_temp(ctx, [&](void* t, size_t& b) {
     return rocprim::radix_sort_keys(t, b, in, out, (size_t)nk, 0u, end_bit, ctx->stream);
@@ -582,7 +592,7 @@ void preprocess(tribit_ctx* ctx, uint64_t* d_keys_in, uint64_t nkeys, uint64_t n
     DevBuf<uint32_t> hi(ctx, m), hi_sorted(ctx, m);
     k_degree_lo<<<grid_for(m, kBlock), kBlock, 0, s>>>(uniq.p, m, kb, deg.p, hi.p);
     with_temp(ctx, [&](void* t, size_t& b) {
-      return rocprim::radix_sort_keys(t, b, hi.p, hi_sorted.p, (size_t)m, 0u, bits_for(n), s);
+      return rocprim::radix_sort_keys<IdSortConfig>(t, b, hi.p, hi_sorted.p, (size_t)m, 0u, bits_for(n), s);
     });
     k_add_runs<<<grid_for(m, kBlock), kBlock, 0, s>>>(hi_sorted.p, m, deg.p);
   }
@@ -609,7 +619,7 @@ void preprocess(tribit_ctx* ctx, uint64_t* d_keys_in, uint64_t nkeys, uint64_t n
       k_gather_u32<<<grid_for(n, kBlock), kBlock, 0, s>>>(deg.p, labels.p, n, deg_sorted.p);
     } else {
       with_temp(ctx, [&](void* t, size_t& b) {
-        return rocprim::radix_sort_pairs(t, b, deg.p, deg_sorted.p, ids.p, labels.p, (size_t)n, 0u,
+        return rocprim::radix_sort_pairs<IdSortConfig>(t, b, deg.p, deg_sorted.p, ids.p, labels.p, (size_t)n, 0u,
                                          bits_for(n), s);
       });
     }
@@ -687,7 +697,8 @@ void preprocess(tribit_ctx* ctx, uint64_t* d_keys_in, uint64_t nkeys, uint64_t n
       DevBuf<uint32_t> arcs(ctx, m), head(ctx, m), arc_sorted(ctx, m);
       k_iota_u32<<<grid_for(m, kBlock), kBlock, 0, s>>>(arcs.p, m);
       with_temp(ctx, [&](void* t, size_t& b) {
-        return rocprim::radix_sort_pairs(t, b, col.p, head.p, arcs.p, arc_sorted.p, (size_t)m, 0u, bits_for(n), s);
+        return rocprim::radix_sort_pairs<IdSortConfig>(t, b, col.p, head.p, arcs.p, arc_sorted.p, (size_t)m, 0u,
+                                                       bits_for(n), s);
       });
       arcs.reset();
       k_sfx_fill<<<grid_for(m, kBlock), kBlock, 0, s>>>(head.p, arc_sorted.p, m, arec.p, rng_sfx.p, sfx_dh.p,
