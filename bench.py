@@ -606,10 +606,24 @@ def main() -> int:
     ms_var = sel["ms_var"]
     work = g.variant_stats(args.sched)
     dom = int(np.argmax(ms_var))
+
+    def launch_bytes(i):
+        nar = work["narrow"][i]
+        return (4.0 * work["rows"][i] + 4.0 * work["edges"][i] + 4.0 * (work["wplus"][i] - nar) + 2.0 * nar +
+                1.0 * work["narrow_ranges"][i] + 2.0 * work["narrow2_ranges"][i])
+
     nar = work["narrow"][dom]
-    b_dom = (4.0 * work["rows"][dom] + 4.0 * work["edges"][dom] + 4.0 * (work["wplus"][dom] - nar) + 2.0 * nar +
-             1.0 * work["narrow_ranges"][dom] + 2.0 * work["narrow2_ranges"][dom])
+    b_dom = launch_bytes(dom)
     achieved = b_dom / (ms_var[dom] * 1e-3) / 1e9 if ms_var[dom] > 0 else 0.0
+    # every launch of the step with work, by the same byte model
+    by_launch = []
+    for i, ms in enumerate(ms_var):
+        if i < len(sel["tasks_by_variant"]) and sel["tasks_by_variant"][i] and ms > 0:
+            b = launch_bytes(i)
+            by_launch.append({"kernel": variant_name(sel["slot_kind"][i], sel["group_width"][i], work["narrow"][i] > 0),
+                              "ms": round(ms, 4), "algorithmic_bytes": b,
+                              "achieved": round(b / (ms * 1e-3) / 1e9, 1),
+                              "frac": round(b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)})
     b_alg = 4.0 * (n + 1) + 4.0 * m + 4.0 * inf["wplus"]
 
     out = {
@@ -673,6 +687,7 @@ def main() -> int:
             "ms_per_launch": round(ms_var[dom], 4),
             "textbook_b_alg": b_alg,
             "textbook_frac": round(b_alg / (ms_dev * 1e-3) / 1e9 / HBM_PEAK_GBS, 3),
+            "by_launch": by_launch,
         },
         "cpu_baseline": None,
         "device": dev["name"],
