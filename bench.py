@@ -356,7 +356,7 @@ def batch_workload(args) -> int:
     inf = b.info()
     log(f"batch of {inf['graphs']} Chung-Lu graphs in {inf['parts']} parts: n={inf['n']} m={inf['m']}, "
         f"built in {time.perf_counter() - t0:.1f} s (device preprocessing {inf['ms_preprocess']:.1f} ms)")
-    sched = "wedge"
+    sched = args.batch_sched
     counts, first = b.count(sched=sched, repeats=args.steps)
     log(f"count: {first['ms_count']:.2f} ms per replay ({first['launches']} launches, capture "
         f"{first['ms_capture']:.1f} ms)")
@@ -547,6 +547,8 @@ def main() -> int:
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (reference kernel for HBM byte counters)")
     ap.add_argument("--batch-graphs", type=int, default=4096, help="graphs of the batch workload")
+    ap.add_argument("--batch-sched", choices=["plain", "directed", "wedge", "learned"], default="wedge",
+                    help="batch workload: the schedule of every part")
     ap.add_argument("--relabel", type=int, default=1, choices=[0, 1, 2],
                     help="mixed workload: vertex ids (1 degree rank, 2 octave classes of the degree, 0 input ids)")
     ap.add_argument("--partitions", type=int, default=64, help="rank workload: partitions P")
