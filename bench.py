@@ -88,8 +88,22 @@ def host_cpus() -> dict:
                 break
     except Exception:
         model = cpu_model()
+    # a virtual machine may report only the brand: the family / model / stepping numbers identify the part
+    ident = {}
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                k, _, v = line.partition(":")
+                k = k.strip()
+                if k in ("cpu family", "model", "stepping", "cpu MHz", "cache size") and k not in ident:
+                    ident[k] = v.strip()
+                if not line.strip():
+                    break
+    except OSError:
+        pass
     omp = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
-    return {"logical": len(cpus), "physical": len(phys), "omp_num_threads": omp or None, "model": model}
+    return {"logical": len(cpus), "physical": len(phys), "omp_num_threads": omp or None, "model": model,
+            "cpuinfo": ident}
 
 
 def _median_runs(fn, runs: int = 5, warm: bool = True):

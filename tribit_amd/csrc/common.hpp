@@ -194,8 +194,8 @@ struct tribit_graph {
     uint2* trng[4] = {nullptr, nullptr, nullptr, nullptr};  // m, tile ranges aligned with each layout's ranges
   } tiles[2];
   uint64_t tiles32 = 0, tiles64 = 0;  // tile counts (computed on the first tribit_graph_get_info)
-  // dense core (dense.hip): int8 adjacency of the top core_c ids, padded to core_kp
-  int8_t* core_A = nullptr;
+  // dense core (dense.hip): bit adjacency of the top core_c ids, padded to core_kp
+  uint32_t* core_bits = nullptr;  // kp rows of kp / 32 words
   uint64_t core_c = 0, core_kp = 0;
   uint32_t core_e0 = 0;    // row_ptr[n - core_c]: first arc of the core's rows
   int64_t core_pick = -1;  // the learned schedule's core size from core_choose (-1: not chosen yet)
@@ -333,7 +333,7 @@ void core_prepare(tribit_ctx* ctx, tribit_graph* g, uint64_t c);
 void core_release(tribit_graph* g);
 void core_launch(tribit_ctx* ctx, const tribit_graph* g, unsigned long long* result);
 uint64_t core_choose(tribit_ctx* ctx, const tribit_graph* g);
-double core_macs(uint64_t kp);
+double core_macs(uint64_t kp, uint64_t tile);
 bool blocks_prepare(tribit_ctx* ctx, tribit_graph* g);  // false: no dense block found
 void blocks_release(tribit_graph* g);
 void blocks_launch(tribit_ctx* ctx, const tribit_graph* g, unsigned long long* result);

@@ -2155,6 +2155,7 @@ static Plan plan_for(int sched, const int32_t* override_gw, int* layout) {
 //   1: the density-threshold heuristic on the preferred layout (+ dense core)
 //   2: the heuristic on the DIRECTED layout (+ dense core), for relabelled graphs
 //   3: the textbook PLAIN plan
+//   4: the heuristic with dense diagonal blocks on the matrix cores
 // (preferred layout: WEDGE for relabelled graphs, else DIRECTED).
 enum { kCandModel = 0, kCandHeur = 1, kCandDirected = 2, kCandPlain = 3, kCandBlocks = 4, kCands = 5 };
 static_assert(kCands <= TRIBIT_CANDIDATES, "candidate times");

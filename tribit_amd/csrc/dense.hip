@@ -39,13 +39,36 @@ __global__ void k_core_zero(uint4* __restrict__ p, uint64_t n16) {
     p[i] = make_uint4(0u, 0u, 0u, 0u);
 }
 
-// A[(u - d0) * kp + (v - d0)] = 1 for every arc u -> v with u >= d0 (then v > u >= d0).
-__global__ void k_core_fill(const uint32_t* __restrict__ row_ptr, const uint32_t* __restrict__ col, uint64_t n,
-                            uint64_t d0, uint32_t kp, int8_t* __restrict__ A) {
+// The top core as bits: row u - d0 of kp bits (kp / 32 words) holds bit v - d0
+// for every arc u -> v with u >= d0. A thread owns its row's words, so no
+// atomics are needed.
+__global__ void k_core_fill_bits(const uint32_t* __restrict__ row_ptr, const uint32_t* __restrict__ col, uint64_t n,
+                                 uint64_t d0, uint32_t kp, uint32_t* __restrict__ bits) {
   const uint64_t u = d0 + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (u >= n) return;
-  int8_t* row = A + (u - d0) * (uint64_t)kp;
-  for (uint32_t e = row_ptr[u]; e < row_ptr[u + 1]; ++e) row[col[e] - d0] = 1;
+  uint32_t* row = bits + (u - d0) * (uint64_t)(kp / 32);
+  for (uint32_t e = row_ptr[u]; e < row_ptr[u + 1]; ++e) {
+    const uint32_t c = col[e] - (uint32_t)d0;
+    row[c >> 5] |= 1u << (c & 31);
+  }
+}
+
+// Four bits to four bytes 0/1: the nibble's copies shifted by 0, 7, 14 and 21
+// do not overlap, and bit k of the nibble lands on bit 8k.
+__device__ __forceinline__ uint32_t nibble_bytes(uint32_t nib) { return __umul24(nib, 0x204081u) & 0x01010101u; }
+
+// 64 bits to 64 int8 0/1 values, bit k to byte k.
+__device__ __forceinline__ void expand_bits64(uint2 b, int8_t* __restrict__ dst) {
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const uint32_t h = ((q < 2) ? b.x : b.y) >> ((q & 1) * 16);
+    uint4 o;
+    o.x = nibble_bytes(h & 0xFu);
+    o.y = nibble_bytes((h >> 4) & 0xFu);
+    o.z = nibble_bytes((h >> 8) & 0xFu);
+    o.w = nibble_bytes((h >> 12) & 0xFu);
+    *reinterpret_cast<uint4*>(dst + 16 * q) = o;
+  }
 }
 
 // Output row of accumulator element v of lane l in a 32 x 32 MFMA block
@@ -149,74 +172,184 @@ __device__ __forceinline__ void core_tile(const int8_t* __restrict__ A, uint32_t
   }
 }
 
-__global__ __launch_bounds__(kCoreThreads) void k_core_count(const int8_t* __restrict__ A, uint32_t kp,
-                                                             unsigned long long* __restrict__ result) {
-  // tile pair (I <= J) from the linear index over the upper triangle, column by column
-  const uint64_t t = blockIdx.x;
-  uint32_t J = (uint32_t)((std::sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
-  while ((uint64_t)(J + 1) * (J + 2) / 2 <= t) ++J;
-  while ((uint64_t)J * (J + 1) / 2 > t) --J;
-  const uint32_t I = (uint32_t)(t - (uint64_t)J * (J + 1) / 2);
-  core_tile(A, kp, I, J, result);
+// The top core's tile: 256 x 256 outputs per 512-thread block, each of its 8
+// waves a 128 x 64 sub-tile of 4 x 2 MFMA blocks (128 accumulator registers
+// per lane, two waves per SIMD). The core is kept as bits (kp^2 / 8 bytes:
+// 32 MiB for 16,384 ids, resident in the last-level cache, where the int8
+// matrix took 256 MiB and came from HBM); each 128-wide K stage reads 8 KiB
+// of bits for 8.4M MACs and expands them to int8 0/1 in LDS (a multiply and
+// a mask per four values). Two LDS stages (147 KiB: one block per CU). kp is
+// a multiple of 256, so the output tile's mask, A over the tile's own 256
+// columns, is the first two K stages of the A operand.
+constexpr int kCoreTile2 = 256;
+constexpr int kCoreThreads2 = 512;
+
+__device__ __forceinline__ void core_tile256(const uint32_t* __restrict__ bits, uint32_t kp, uint32_t I, uint32_t J,
+                                             unsigned long long* __restrict__ result) {
+  constexpr int NW = kCoreThreads2 / 64;
+  constexpr int WN = 64;                  // columns per wave
+  constexpr int NJ = WN / 32;             // MFMA blocks per wave along the columns
+  constexpr int WC = kCoreTile2 / WN;     // waves along the columns (two rows of waves)
+  static_assert(WC * 2 == NW, "waves tile the 256 x 256 output");
+  __shared__ __attribute__((aligned(16))) int8_t sA[2][kCoreTile2][kCoreK + kCorePad];
+  __shared__ __attribute__((aligned(16))) int8_t sB[2][kCoreTile2][kCoreK + kCorePad];
+  __shared__ unsigned long long sPart[NW];
+  const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const uint32_t wr = w / WC, wc = w % WC;  // rows wr * 128, columns wc * WN of the tile
+  const uint64_t rowA = (uint64_t)I * kCoreTile2, rowB = (uint64_t)J * kCoreTile2;
+  const uint64_t kpw = kp / 32;  // words per bit row
+  v16i acc[4][NJ];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < NJ; ++j)
+#pragma unroll
+      for (int v = 0; v < 16; ++v) acc[i][j][v] = 0;
+  // staging: two threads per row, 64 bits (= 64 int8 values) of each operand per stage
+  const uint32_t lr = tid >> 1, lh = (tid & 1) * 64;
+  const uint32_t k_begin = J * kCoreTile2;
+  const uint2* ga = reinterpret_cast<const uint2*>(bits + (rowA + lr) * kpw + ((k_begin + lh) >> 5));
+  const uint2* gb = reinterpret_cast<const uint2*>(bits + (rowB + lr) * kpw + ((k_begin + lh) >> 5));
+  uint2 pa = *ga, pb = *gb;
+  // this wave's columns lie in one K stage (the tile's columns 0..127, then 128..255)
+  const uint32_t k_mask = k_begin + (wc * WN / kCoreK) * kCoreK, c_off = (wc * WN) % kCoreK;
+  uint64_t msk[4] = {0ull, 0ull, 0ull, 0ull};  // bit j * 16 + v of msk[i]: A[r][c] of accumulator (i, j, v)
+  uint32_t buf = 0;
+  for (uint32_t k0 = k_begin; k0 < kp; k0 += kCoreK, buf ^= 1u) {
+    // a wave stores stage s only after every wave passed the barrier of stage
+    // s - 1, that is after the MFMAs of stage s - 2 read this buffer
+    expand_bits64(pa, &sA[buf][lr][lh]);
+    expand_bits64(pb, &sB[buf][lr][lh]);
+    __syncthreads();
+    if (k0 == k_mask) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < NJ; ++j)
+#pragma unroll
+          for (int v = 0; v < 16; ++v) {
+            const int r = wr * 128 + i * 32 + mfma_row(v, lane), c = c_off + j * 32 + (lane & 31);
+            msk[i] |= (uint64_t)(sA[buf][r][c] != 0) << (j * 16 + v);
+          }
+    }
+    if (k0 + kCoreK < kp) {  // next stage in flight during the MFMAs
+      ga += kCoreK / 64;
+      gb += kCoreK / 64;
+      pa = *ga;
+      pb = *gb;
+    }
+#pragma unroll
+    for (int kk = 0; kk < kCoreK; kk += 32) {
+      const int ko = kk + (lane >> 5) * 16;  // the same k slice for both operands
+      v4i fa[4], fb[NJ];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        fa[i] = *reinterpret_cast<const v4i*>(&sA[buf][wr * 128 + i * 32 + (lane & 31)][ko]);
+#pragma unroll
+      for (int j = 0; j < NJ; ++j)
+        fb[j] = *reinterpret_cast<const v4i*>(&sB[buf][wc * WN + j * 32 + (lane & 31)][ko]);
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < NJ; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[i], fb[j], acc[i][j], 0, 0, 0);
+    }
+  }
+  // masked sum: (A A^T)[r][c] counts where A[r][c] = 1
+  uint64_t sum = 0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < NJ; ++j)
+#pragma unroll
+      for (int v = 0; v < 16; ++v)
+        if ((msk[i] >> (j * 16 + v)) & 1ull) sum += (uint32_t)acc[i][j][v];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o);
+  if (lane == 0) sPart[w] = sum;
+  __syncthreads();
+  if (tid == 0) {
+    unsigned long long s = 0;
+    for (int i = 0; i < NW; ++i) s += sPart[i];
+    if (s) atomicAdd(result, s);
+  }
+}
+
+// tile pair (I <= J) of block t, the upper triangle enumerated column by column
+__device__ __forceinline__ void tile_pair(uint64_t t, uint32_t* I, uint32_t* J) {
+  uint32_t j = (uint32_t)((std::sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+  while ((uint64_t)(j + 1) * (j + 2) / 2 <= t) ++j;
+  while ((uint64_t)j * (j + 1) / 2 > t) --j;
+  *J = j;
+  *I = (uint32_t)(t - (uint64_t)j * (j + 1) / 2);
+}
+
+__global__ __launch_bounds__(kCoreThreads2) void k_core_count(const uint32_t* __restrict__ bits, uint32_t kp,
+                                                              unsigned long long* __restrict__ result) {
+  uint32_t I, J;
+  tile_pair(blockIdx.x, &I, &J);
+  core_tile256(bits, kp, I, J, result);
 }
 
 // Builds the core matrix of the top `c` ids (cached per graph and size).
 void core_prepare(tribit_ctx* ctx, tribit_graph* g, uint64_t c) {
-  if (g->core_A && g->core_c == c) return;
+  if (g->core_bits && g->core_c == c) return;
   core_release(g);
   if (c == 0) return;
-  const uint64_t kp = (c + kCoreTile - 1) / kCoreTile * kCoreTile;
+  const uint64_t kp = (c + kCoreTile2 - 1) / kCoreTile2 * kCoreTile2;
   const uint64_t d0 = g->n - c;
-  DevBuf<int8_t> A(ctx, kp * kp);
+  DevBuf<uint32_t> bits(ctx, kp * kp / 32);
   hipStream_t s = ctx->stream;
-  k_core_zero<<<2048, 256, 0, s>>>(reinterpret_cast<uint4*>(A.p), kp * kp / 16);
-  k_core_fill<<<grid_for(c, 256), 256, 0, s>>>(g->row_ptr, g->col, g->n, d0, (uint32_t)kp, A.p);
+  k_core_zero<<<2048, 256, 0, s>>>(reinterpret_cast<uint4*>(bits.p), kp * kp / 128);
+  k_core_fill_bits<<<grid_for(c, 256), 256, 0, s>>>(g->row_ptr, g->col, g->n, d0, (uint32_t)kp, bits.p);
   TB_HIP(hipGetLastError());
   uint32_t e0 = 0;
   TB_HIP(hipMemcpyAsync(&e0, g->row_ptr + d0, 4, hipMemcpyDeviceToHost, s));
   ctx->sync();
-  g->core_A = A.release();
+  g->core_bits = bits.release();
   g->core_c = c;
   g->core_kp = kp;
   g->core_e0 = e0;
 }
 
 void core_release(tribit_graph* g) {
-  if (g->core_A && g->ctx) g->ctx->free(g->core_A);
-  g->core_A = nullptr;
+  if (g->core_bits && g->ctx) g->ctx->free(g->core_bits);
+  g->core_bits = nullptr;
   g->core_c = g->core_kp = 0;
 }
 
-// Tiles of the upper triangle.
-uint64_t core_tiles(uint64_t kp) {
-  const uint64_t nb = kp / kCoreTile;
+// Tiles of the upper triangle, side `tile`.
+uint64_t core_tiles(uint64_t kp, uint64_t tile) {
+  const uint64_t nb = kp / tile;
   return nb * (nb + 1) / 2;
 }
 
 void core_launch(tribit_ctx* ctx, const tribit_graph* g, unsigned long long* result) {
-  if (!g->core_A) return;
-  const uint64_t nt = core_tiles(g->core_kp);
-  k_core_count<<<(unsigned)nt, kCoreThreads, 0, ctx->stream>>>(g->core_A, (uint32_t)g->core_kp, result);
+  if (!g->core_bits) return;
+  const uint64_t nt = core_tiles(g->core_kp, kCoreTile2);
+  k_core_count<<<(unsigned)nt, kCoreThreads2, 0, ctx->stream>>>(g->core_bits, (uint32_t)g->core_kp, result);
   TB_HIP(hipGetLastError());
 }
 
-// int8 multiply-accumulates of the core launch (the upper triangle, K from
-// each tile's first column): sum over J of (J+1) * 128^2 * (kp - 128 J).
-double core_macs(uint64_t kp) {
-  const uint64_t nb = kp / kCoreTile;
+// int8 multiply-accumulates of a launch over the upper triangle in tiles of
+// side `tile`, K from each tile's first column:
+// sum over J of (J+1) * tile^2 * (kp - tile J).
+double core_macs(uint64_t kp, uint64_t tile) {
+  const uint64_t nb = kp / tile;
   double macs = 0.0;
-  for (uint64_t J = 0; J < nb; ++J) macs += (double)(J + 1) * kCoreTile * kCoreTile * (double)(kp - kCoreTile * J);
+  for (uint64_t J = 0; J < nb; ++J) macs += (double)(J + 1) * tile * tile * (double)(kp - tile * J);
   return macs;
 }
 
-static double kCoreMacPerMs = 5.5e11;
+static double kCoreMacPerMs = 1.3e12;
 
 // Core size chosen for a relabelled graph: the candidate (4096 .. 65536
 // ids, at most n) with the largest estimated saving
 //     streamed wedges of the core / wedge rate  -  MACs / MAC rate
 // if that saving exceeds 0.5 ms; 0 = no core. The rates are the kernels'
 // measured figures on MI355X (streaming kernels ~2e12 wedge checks/s, the
-// core kernel ~5.5e14 int8 MAC/s including its staging).
+// core kernel ~1.3e15 int8 MAC/s including its staging: 7.7e11 MACs in
+// 0.57 ms for 16,384 ids).
 uint64_t core_choose(tribit_ctx* ctx, const tribit_graph* g) {
   if (!g->relabeled || g->n < 4096 || !g->m) return 0;
   const uint64_t top = std::min<uint64_t>(g->n, 65536);
@@ -231,9 +364,9 @@ uint64_t core_choose(tribit_ctx* ctx, const tribit_graph* g) {
       const double d = (double)(rp[i + 1] - rp[i]);
       wedges += d * (d - 1.0) * 0.5;
     }
-    const uint64_t kp = (c + kCoreTile - 1) / kCoreTile * kCoreTile;
+    const uint64_t kp = (c + kCoreTile2 - 1) / kCoreTile2 * kCoreTile2;
     const double sparse_ms = wedges / 2.0e9;
-    const double dense_ms = core_macs(kp) / kCoreMacPerMs;
+    const double dense_ms = core_macs(kp, kCoreTile2) / kCoreMacPerMs;
     if (sparse_ms - dense_ms > best_gain) {
       best_gain = sparse_ms - dense_ms;
       best = c;
@@ -461,7 +594,7 @@ bool blocks_prepare(tribit_ctx* ctx, tribit_graph* g) {
     kps[i] = kp;
     total += (uint64_t)kp * kp;
     verts += hb[i].y;
-    macs += core_macs(kp);
+    macs += core_macs(kp, kCoreTile);
     for (uint32_t J = 0; J < kp / kCoreTile; ++J)
       for (uint32_t I = 0; I <= J; ++I)
         work.push_back(make_uint4((uint32_t)aoff[i], (uint32_t)(aoff[i] >> 32), kp, I | (J << 16)));
