@@ -336,6 +336,7 @@ def mixed_family(args) -> int:
         if sel is None or plain is None or sel["triangles"] != plain["triangles"]:
             return 1
         m = inf["m"]
+        after = g.info()
         check = cpu_baseline(g, inf["n"], m, args.cpu_seconds, sel["triangles"], True) if args.cpu_full else None
         print(json.dumps({
             "metric": METRIC, "workload": desc, "n": inf["n"], "m": m, "wplus": inf["wplus"],
@@ -345,6 +346,11 @@ def mixed_family(args) -> int:
             "plain_gteps": round(m / (plain["ms_per_step"] * 1e-3) / 1e9, 4),
             "plain_ms_count_device": round(plain["ms_dev"], 4),
             "ms_preprocess_device": round(inf["ms_preprocess"], 3),
+            # preprocessing of the path that ran: the graph's build, the range layout its plan streams when
+            # that one is built on first use, and the dense blocks' structures when it counts them
+            "ms_preprocess_selected": round(inf["ms_preprocess"] + after["ms_layout"][sel["layout"]] +
+                                            (after["ms_blocks"] if sel.get("dense_blocks", 0) else 0.0), 3),
+            "ms_dense_blocks_prepare": round(after["ms_blocks"], 3),
             "relabel": args.relabel, "candidate": sel["candidate"], "ms_judge": round(sel["ms_judge"], 3),
             "ms_candidates": [round(x, 3) for x in sel["ms_candidates"]],
             "dense_blocks": sel.get("dense_blocks", 0), "ms_dense": round(sel["ms_dense"], 4),

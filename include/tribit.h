@@ -66,6 +66,9 @@ typedef struct {
   uint64_t device_bytes;   /* bytes held by the resident structures */
   double ms_layout[4];     /* device time of each other range layout, indexed by tribit_sched (PLAIN,
                               DIRECTED), built on its first count outside that count's timing; 0: not built */
+  double ms_blocks;        /* device time of the dense diagonal blocks' detection and structures, built on the
+                              first count that considers them (relabelled graphs, learned or DENSE_BLOCKS),
+                              outside that count's timing; 0: not built */
 } tribit_graph_info;
 
 /* Scheduler choice for tribit_count. */

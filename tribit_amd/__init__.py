@@ -44,7 +44,7 @@ class GraphInfo(ctypes.Structure):
         ("wplus", ctypes.c_uint64), ("rows_active", ctypes.c_uint64), ("max_outdeg", ctypes.c_uint32),
         ("reserved0", ctypes.c_uint32), ("tiles32", ctypes.c_uint64), ("tiles64", ctypes.c_uint64),
         ("ms_preprocess", ctypes.c_double), ("device_bytes", ctypes.c_uint64),
-        ("ms_layout", ctypes.c_double * 4),
+        ("ms_layout", ctypes.c_double * 4), ("ms_blocks", ctypes.c_double),
     ]
 
 
@@ -522,6 +522,7 @@ class Graph:
         _check(self.ctx.lib.tribit_graph_get_info(self.h, ctypes.byref(gi)), "graph_get_info")
         d = {f: getattr(gi, f) for f, _ in GraphInfo._fields_ if not f.startswith("reserved")}
         d["ms_layout"] = list(gi.ms_layout)
+        d["ms_blocks"] = gi.ms_blocks
         return d
 
     def download(self):

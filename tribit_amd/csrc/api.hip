@@ -338,6 +338,7 @@ int tribit_graph_get_info(const tribit_graph* g, tribit_graph_info* info) {
   info->max_outdeg = g->max_outdeg;
   info->ms_preprocess = g->ms_prep;
   for (int l = 0; l < 4; ++l) info->ms_layout[l] = g->ms_layout[l];
+  info->ms_blocks = g->blocks.ms_prepare;
   if (!g->tiles_counted && g->ctx) {  // tile counts on first request (untimed; not part of preprocessing)
     tribit_graph* gm = const_cast<tribit_graph*>(g);
     const int st = guarded(g->ctx, [&] {

@@ -209,9 +209,24 @@ struct tribit_graph {
     int8_t* A = nullptr;          // the blocks' 0/1 upper-triangular matrices, each padded to kp x kp
     uint4* work = nullptr;        // per workgroup: matrix offset (lo, hi), kp, tile pair I | J << 16
     uint32_t nwork = 0;
-    uint2* rng = nullptr;         // m: wedge ranges with the in-block part of in-block arcs cut away
+    uint2* rng = nullptr;         // the owners' wedge ranges without those of in-block arcs (32-bit ids)
+    uint32_t* ptr = nullptr;      // n + 1: each owner's ranges in rng
     uint64_t* w = nullptr;        // n: streamed elements per owner over rng
     uint64_t cut = 0;             // streamed elements the cut removed
+    // the pair kernel's input: arcs u -> w from a block to a later id, sorted
+    // as (w << 32) | u, and the runs (first key, keys) of one w with two or
+    // more sources in one block, short and long; the blocks' extents, offsets and sides
+    uint64_t* key = nullptr;
+    uint64_t nkeys = 0;
+    uint2* runs = nullptr;
+    uint32_t nruns = 0;
+    uint2* runs_long = nullptr;
+    uint32_t nruns_long = 0;
+    uint2* blk = nullptr;
+    int32_t* blk_of = nullptr;    // n: block of each id, -1 outside blocks
+    uint64_t* aoff = nullptr;
+    uint32_t* kps = nullptr;
+    double ms_prepare = 0.0;      // device time of detection and these structures
   } blocks;
   tb::BucketCache* bcache = nullptr;  // scheduler statistics (count.hip)
   // the learned schedule's judged candidate (count.hip), valid for the model

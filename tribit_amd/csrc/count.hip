@@ -1397,6 +1397,7 @@ static SchedView sched_view(const tribit_graph* g, int sched, uint32_t core_d0 =
   }
   if (blocks && sched == TRIBIT_SCHED_WEDGE && g->blocks.rng) {
     v.rng = g->blocks.rng;
+    v.fw_ptr = v.rv_ptr = g->blocks.ptr;
     v.w_fw = v.w_rv = g->blocks.w;
   }
   return v;
@@ -1819,9 +1820,9 @@ template <int LOGS, int G>
 static void launch_block_g(const LaunchArgs& a, int slot, uint32_t ntasks, int cus, hipStream_t s, tribit_graph* g,
                            const uint2* rng, const TileSrc& ts, const uint4* tasks, uint64_t* blocks) {
   const bool buf = g->m * 4 <= kBufMax && !(a.att & kForceGlobal);
-  // narrow ranges exist only in the wedge layout (col16 is built with it)
-  const bool n16 = buf && g->col16 && g->sfx_dh && g->sfx_split &&
-                   (rng == g->rng_sfx || (rng && rng == g->blocks.rng)) && !(a.att & kWideBit);
+  // narrow ranges exist only in the wedge layout (col16 is built with it); the
+  // dense blocks' compacted copy carries no narrow flags
+  const bool n16 = buf && g->col16 && g->sfx_dh && g->sfx_split && rng == g->rng_sfx && !(a.att & kWideBit);
   if (ts.trng && ts.mask32) {
     if (n16)
       launch_block_t<LOGS, G, kU, true, 32, true>(a, slot, ntasks, cus, s, g, rng, ts, tasks, blocks);

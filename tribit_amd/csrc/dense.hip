@@ -18,6 +18,7 @@
 #include <cstring>
 #include <vector>
 
+#include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
 
 #include "common.hpp"
@@ -481,25 +482,54 @@ __global__ void k_blk_fill(const uint32_t* __restrict__ row_ptr, const uint32_t*
   }
 }
 
-// The wedge ranges of owners inside blocks with in-block sources cut to ids
-// >= the block's end (range copies were made by the caller); w[v] reduced by
-// the elements cut. Ranges of one owner are ordered by source, so the
-// in-block sources are those whose arc index is >= row_ptr[s].
-__global__ void k_blk_trim(const uint32_t* __restrict__ row_ptr, const uint32_t* __restrict__ col,
-                           const uint32_t* __restrict__ sfx_ptr, const uint2* __restrict__ blk,
-                           const int32_t* __restrict__ blk_of, uint64_t n, uint32_t emask, uint2* __restrict__ rng,
-                           unsigned long long* __restrict__ w, unsigned long long* __restrict__ total_cut) {
+// A triangle u < v < w of a graph with dense diagonal blocks (contiguous id
+// ranges) is counted once, by the first of these that applies:
+//   all three in one block: the block's matrix on the matrix cores;
+//   u and v in one block B, w after it: the pair kernel below, from the
+//     sources u < v in B of w's in-arcs (a pair per w and block, rare unless
+//     w has many neighbours in B) and B's matrix;
+//   otherwise (u -> v crosses blocks): the streamed wedge of arc u -> v.
+// (u and w in one block puts v in it too.) So the wedge ranges of in-block
+// arcs are cut entirely (range copies were made by the caller) and w[v]
+// reduced by their elements. Ranges of one owner are ordered by source, so
+// the in-block sources are those whose arc index is >= row_ptr[s].
+__global__ void k_blk_trim(const uint32_t* __restrict__ row_ptr, const uint32_t* __restrict__ sfx_ptr,
+                           const uint2* __restrict__ blk, const int32_t* __restrict__ blk_of, uint64_t n,
+                           uint32_t emask, uint2* __restrict__ rng, unsigned long long* __restrict__ w,
+                           unsigned long long* __restrict__ total_cut) {
   const uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= n) return;
   const int32_t i = blk_of[v];
   if (i < 0) return;
-  const uint32_t s = blk[i].x, e = s + blk[i].y, first = row_ptr[s];
+  const uint32_t first = row_ptr[blk[i].x];
   uint64_t cut = 0;
   for (uint32_t j = sfx_ptr[v]; j < sfx_ptr[v + 1]; ++j) {
     const uint2 r = rng[j];
     if (r.x <= first) continue;  // arc index r.x - 1 < row_ptr[s]: the source is below the block
     const uint32_t end = r.y & emask;
-    uint32_t lo = r.x, hi = end;  // first position >= r.x with col >= e
+    cut += end - r.x;
+    rng[j] = make_uint2(end, end);
+  }
+  if (cut) {
+    w[v] -= cut;
+    atomicAdd(total_cut, (unsigned long long)cut);
+  }
+}
+
+// Arcs u -> w with u in a block and w after it, as (w << 32) | u, for the pair
+// kernel; counted per source row first (pass 1: out == nullptr).
+__global__ void k_blk_out_arcs(const uint32_t* __restrict__ row_ptr, const uint32_t* __restrict__ col,
+                               const uint2* __restrict__ blk, const int32_t* __restrict__ blk_of, uint64_t n,
+                               const uint64_t* __restrict__ off, uint32_t* __restrict__ cnt,
+                               uint64_t* __restrict__ out) {
+  const uint64_t u = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= n) return;
+  const int32_t i = blk_of[u];
+  uint32_t k = 0;
+  if (i >= 0) {
+    const uint32_t e = blk[i].x + blk[i].y;
+    // rows are sorted: the arcs leaving the block are a suffix of the row
+    uint32_t lo = row_ptr[u], hi = row_ptr[u + 1];
     while (lo < hi) {
       const uint32_t mid = lo + ((hi - lo) >> 1);
       if (col[mid] < e)
@@ -507,13 +537,105 @@ __global__ void k_blk_trim(const uint32_t* __restrict__ row_ptr, const uint32_t*
       else
         hi = mid;
     }
-    cut += lo - r.x;
-    rng[j] = make_uint2(lo, end);  // a cut range streams 32-bit ids (no narrow flags)
+    k = row_ptr[u + 1] - lo;
+    if (out) {
+      uint64_t o = off[u];
+      for (uint32_t q = lo; q < row_ptr[u + 1]; ++q) out[o++] = ((uint64_t)col[q] << 32) | u;
+    }
   }
-  if (cut) {
-    w[v] -= cut;
-    atomicAdd(total_cut, (unsigned long long)cut);
+  if (!out) cnt[u] = k;
+}
+
+// Runs of the sorted (w, u) keys with one w and two or more sources in one
+// block: a run starts where the previous key has another w or another block.
+__global__ void k_blk_run_heads(const uint64_t* __restrict__ key, uint64_t nk, const int32_t* __restrict__ blk_of,
+                                uint32_t* __restrict__ head) {
+  const uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nk) return;
+  const uint64_t k = key[q];
+  const int32_t b = blk_of[(uint32_t)k];
+  const bool first = q == 0 || (key[q - 1] >> 32) != (k >> 32) || blk_of[(uint32_t)key[q - 1]] != b;
+  const bool more = q + 1 < nk && (key[q + 1] >> 32) == (k >> 32) && blk_of[(uint32_t)key[q + 1]] == b;
+  head[q] = (first && more) ? 1u : 0u;
+}
+
+// Length of every run (prepare time): keys from its first with one w and one block.
+__global__ void k_blk_run_len(const uint64_t* __restrict__ key, uint64_t nk, const int32_t* __restrict__ blk_of,
+                              const uint32_t* __restrict__ first, uint32_t nruns, uint2* __restrict__ runs) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nruns) return;
+  const uint32_t q0 = first[t];
+  const uint64_t w = key[q0] >> 32;
+  const int32_t b = blk_of[(uint32_t)key[q0]];
+  uint32_t q1 = q0 + 1;
+  while (q1 < nk && (key[q1] >> 32) == w && blk_of[(uint32_t)key[q1]] == b) ++q1;
+  runs[t] = make_uint2(q0, q1 - q0);
+}
+
+// Triangles u < v < w with u, v in one block and w after it: for every run
+// (w, block i, sources u_1 < ... < u_k), the edges among its sources, read
+// from block i's matrix. Short runs (at most kPairShort sources; almost all,
+// most of them pairs): one thread per run.
+constexpr uint32_t kPairShort = 32;
+
+__device__ __forceinline__ uint64_t run_pairs(const uint64_t* __restrict__ key, uint2 r, uint32_t x0, uint32_t dx,
+                                              const int32_t* __restrict__ blk_of, const uint2* __restrict__ blk,
+                                              const uint64_t* __restrict__ aoff, const uint32_t* __restrict__ kps,
+                                              const int8_t* __restrict__ A) {
+  const int32_t b = blk_of[(uint32_t)key[r.x]];
+  const uint32_t s = blk[b].x, kp = kps[b];
+  const int8_t* M = A + aoff[b];
+  uint64_t c = 0;
+  for (uint32_t x = r.x + x0; x < r.x + r.y; x += dx) {
+    const uint32_t ux = (uint32_t)key[x] - s;
+    for (uint32_t y = x + 1; y < r.x + r.y; ++y) c += M[(uint64_t)ux * kp + ((uint32_t)key[y] - s)] != 0;
   }
+  return c;
+}
+
+__global__ void k_blk_pairs(const uint64_t* __restrict__ key, const uint2* __restrict__ runs, uint32_t nruns,
+                            const int32_t* __restrict__ blk_of, const uint2* __restrict__ blk,
+                            const uint64_t* __restrict__ aoff, const uint32_t* __restrict__ kps,
+                            const int8_t* __restrict__ A, unsigned long long* __restrict__ result) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t c = t < nruns ? run_pairs(key, runs[t], 0, 1, blk_of, blk, aoff, kps, A) : 0;
+  c = wave_sum_u64(c);
+  if (lane_id() == 0 && c) atomicAdd(result, (unsigned long long)c);
+}
+
+// Long runs (a vertex with many neighbours in one earlier block, e.g. a
+// member of the block whose degree put its id above the block): one 256-thread
+// workgroup per run, its threads striding over the first source of a pair.
+__global__ void __launch_bounds__(256) k_blk_pairs_long(const uint64_t* __restrict__ key,
+                                                        const uint2* __restrict__ runs,
+                                                        const int32_t* __restrict__ blk_of,
+                                                        const uint2* __restrict__ blk,
+                                                        const uint64_t* __restrict__ aoff,
+                                                        const uint32_t* __restrict__ kps,
+                                                        const int8_t* __restrict__ A,
+                                                        unsigned long long* __restrict__ result) {
+  uint64_t c = run_pairs(key, runs[blockIdx.x], threadIdx.x, blockDim.x, blk_of, blk, aoff, kps, A);
+  c = wave_sum_u64(c);
+  if (lane_id() == 0 && c) atomicAdd(result, (unsigned long long)c);
+}
+
+// Owners' nonempty ranges of the cut copy, compacted per owner (pass 1:
+// counts), flags cleared: the cut copy streams 32-bit ids.
+__global__ void k_blk_compact(const uint32_t* __restrict__ sfx_ptr, const uint2* __restrict__ rng, uint64_t n,
+                              uint32_t emask, const uint32_t* __restrict__ ptr, uint32_t* __restrict__ cnt,
+                              uint2* __restrict__ out) {
+  const uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  uint32_t k = 0, o = out ? ptr[v] : 0;
+  for (uint32_t j = sfx_ptr[v]; j < sfx_ptr[v + 1]; ++j) {
+    const uint2 r = rng[j];
+    const uint32_t end = r.y & emask;
+    if (r.x < end) {
+      if (out) out[o + k] = make_uint2(r.x, end);
+      ++k;
+    }
+  }
+  if (!out) cnt[v] = k;
 }
 
 __global__ __launch_bounds__(kCoreThreads) void k_blocks_count(const int8_t* __restrict__ A,
@@ -529,17 +651,61 @@ void blocks_release(tribit_graph* g) {
     g->ctx->free(B.A);
     g->ctx->free(B.work);
     g->ctx->free(B.rng);
+    g->ctx->free(B.ptr);
     g->ctx->free(B.w);
+    g->ctx->free(B.key);
+    g->ctx->free(B.runs);
+    g->ctx->free(B.runs_long);
+    g->ctx->free(B.blk);
+    g->ctx->free(B.blk_of);
+    g->ctx->free(B.aoff);
+    g->ctx->free(B.kps);
   }
   B = tribit_graph::DenseBlocks{};
 }
 
+// Positions of the set flags: out[pos[q]] = q where flag[q] (pos = exclusive scan of flag).
+__global__ void k_flag_positions(const uint32_t* __restrict__ flag, const uint32_t* __restrict__ pos, uint64_t n,
+                                 uint32_t* __restrict__ out) {
+  const uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n && flag[q]) out[pos[q]] = (uint32_t)q;
+}
+
+static bool blocks_build(tribit_ctx* ctx, tribit_graph* g);
+
+// Detection and the blocks' structures, once per graph; their device time is
+// kept (tribit_graph_info.ms_blocks), like a range layout's.
 bool blocks_prepare(tribit_ctx* ctx, tribit_graph* g) {
   tribit_graph::DenseBlocks& B = g->blocks;
   if (B.found) return B.nblocks > 0;
   B.found = true;
   const uint64_t n = g->n, m = g->m;
   if (!g->relabeled || !g->rng_sfx || n < kBlockMin || !m) return false;
+  hipEvent_t a = nullptr, b = nullptr;
+  TB_HIP(hipEventCreate(&a));
+  TB_HIP(hipEventCreate(&b));
+  bool ok = false;
+  try {
+    TB_HIP(hipEventRecord(a, ctx->stream));
+    ok = blocks_build(ctx, g);
+    TB_HIP(hipEventRecord(b, ctx->stream));
+    TB_HIP(hipEventSynchronize(b));
+    float ms = 0.f;
+    TB_HIP(hipEventElapsedTime(&ms, a, b));
+    B.ms_prepare = ms;
+  } catch (...) {
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    throw;
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  return ok;
+}
+
+static bool blocks_build(tribit_ctx* ctx, tribit_graph* g) {
+  tribit_graph::DenseBlocks& B = g->blocks;
+  const uint64_t n = g->n, m = g->m;
   hipStream_t s = ctx->stream;
   std::vector<uint2> hb;
   {
@@ -620,29 +786,135 @@ bool blocks_prepare(tribit_ctx* ctx, tribit_graph* g) {
   TB_HIP(hipMemcpyAsync(rng.p, g->rng_sfx, 8 * m, hipMemcpyDeviceToDevice, s));
   TB_HIP(hipMemcpyAsync(w.p, g->w_sfx, 8 * n, hipMemcpyDeviceToDevice, s));
   TB_HIP(hipMemsetAsync(cut.p, 0, 8, s));
-  k_blk_trim<<<grid_for(n, 256), 256, 0, s>>>(g->row_ptr, g->col, g->sfx_ptr, dblk.p, blk_of.p, n,
-                                              range_end_mask(m), rng.p,
-                                              reinterpret_cast<unsigned long long*>(w.p), cut.p);
+  k_blk_trim<<<grid_for(n, 256), 256, 0, s>>>(g->row_ptr, g->sfx_ptr, dblk.p, blk_of.p, n, range_end_mask(m),
+                                              rng.p, reinterpret_cast<unsigned long long*>(w.p), cut.p);
   TB_HIP(hipGetLastError());
   unsigned long long hcut = 0;
   TB_HIP(hipMemcpyAsync(&hcut, cut.p, 8, hipMemcpyDeviceToHost, s));
-  ctx->sync();
+  // the owners' lists without the empty ranges of in-block arcs
+  DevBuf<uint32_t> bptr(ctx, n + 1);
+  DevBuf<uint2> crng;
+  crng.ctx = ctx;
+  {
+    DevBuf<uint32_t> cnt(ctx, n + 1);
+    TB_HIP(hipMemsetAsync(cnt.p + n, 0, 4, s));
+    k_blk_compact<<<grid_for(n, 256), 256, 0, s>>>(g->sfx_ptr, rng.p, n, range_end_mask(m), nullptr, cnt.p, nullptr);
+    size_t tb = 0;
+    TB_HIP(rocprim::exclusive_scan(nullptr, tb, cnt.p, bptr.p, 0u, (size_t)(n + 1), rocprim::plus<uint32_t>(), s));
+    DevBuf<uint8_t> tmp(ctx, tb ? tb : 8);
+    TB_HIP(rocprim::exclusive_scan(tmp.p, tb, cnt.p, bptr.p, 0u, (size_t)(n + 1), rocprim::plus<uint32_t>(), s));
+    uint32_t kept = 0;
+    TB_HIP(hipMemcpyAsync(&kept, bptr.p + n, 4, hipMemcpyDeviceToHost, s));
+    ctx->sync();
+    crng = DevBuf<uint2>(ctx, kept ? kept : 1);
+    k_blk_compact<<<grid_for(n, 256), 256, 0, s>>>(g->sfx_ptr, rng.p, n, range_end_mask(m), bptr.p, nullptr,
+                                                   crng.p);
+    TB_HIP(hipGetLastError());
+    ctx->sync();
+    rng.reset();
+  }
+  // the pair kernel's runs: arcs from a block to a later id, sorted by (w, u)
+  DevBuf<uint64_t> key;
+  key.ctx = ctx;
+  DevBuf<uint2> runs, runs_long;
+  runs.ctx = runs_long.ctx = ctx;
+  uint64_t nkeys = 0;
+  uint32_t nruns = 0, nruns_long = 0;
+  {
+    DevBuf<uint32_t> cnt(ctx, n);
+    DevBuf<uint64_t> off(ctx, n);
+    k_blk_out_arcs<<<grid_for(n, 256), 256, 0, s>>>(g->row_ptr, g->col, dblk.p, blk_of.p, n, nullptr, cnt.p,
+                                                    nullptr);
+    size_t tb = 0;
+    TB_HIP(rocprim::exclusive_scan(nullptr, tb, cnt.p, off.p, (uint64_t)0, (size_t)n, rocprim::plus<uint64_t>(), s));
+    DevBuf<uint8_t> tmp(ctx, tb ? tb : 8);
+    TB_HIP(rocprim::exclusive_scan(tmp.p, tb, cnt.p, off.p, (uint64_t)0, (size_t)n, rocprim::plus<uint64_t>(), s));
+    uint64_t last_off = 0;
+    uint32_t last_cnt = 0;
+    TB_HIP(hipMemcpyAsync(&last_off, off.p + n - 1, 8, hipMemcpyDeviceToHost, s));
+    TB_HIP(hipMemcpyAsync(&last_cnt, cnt.p + n - 1, 4, hipMemcpyDeviceToHost, s));
+    ctx->sync();
+    nkeys = last_off + last_cnt;
+    if (nkeys) {
+      DevBuf<uint64_t> raw(ctx, nkeys);
+      key = DevBuf<uint64_t>(ctx, nkeys);
+      k_blk_out_arcs<<<grid_for(n, 256), 256, 0, s>>>(g->row_ptr, g->col, dblk.p, blk_of.p, n, off.p, nullptr,
+                                                      raw.p);
+      size_t tb2 = 0;
+      TB_HIP(rocprim::radix_sort_keys(nullptr, tb2, raw.p, key.p, (size_t)nkeys, 0u, 32u + bits_for(n), s));
+      DevBuf<uint8_t> tmp2(ctx, tb2 ? tb2 : 8);
+      TB_HIP(rocprim::radix_sort_keys(tmp2.p, tb2, raw.p, key.p, (size_t)nkeys, 0u, 32u + bits_for(n), s));
+      DevBuf<uint32_t> head(ctx, nkeys + 1), pos(ctx, nkeys + 1);
+      k_blk_run_heads<<<grid_for(nkeys, 256), 256, 0, s>>>(key.p, nkeys, blk_of.p, head.p);
+      TB_HIP(hipMemsetAsync(head.p + nkeys, 0, 4, s));
+      size_t tb3 = 0;
+      TB_HIP(rocprim::exclusive_scan(nullptr, tb3, head.p, pos.p, 0u, (size_t)(nkeys + 1), rocprim::plus<uint32_t>(),
+                                     s));
+      DevBuf<uint8_t> tmp3(ctx, tb3 ? tb3 : 8);
+      TB_HIP(rocprim::exclusive_scan(tmp3.p, tb3, head.p, pos.p, 0u, (size_t)(nkeys + 1), rocprim::plus<uint32_t>(),
+                                     s));
+      TB_HIP(hipMemcpyAsync(&nruns, pos.p + nkeys, 4, hipMemcpyDeviceToHost, s));
+      ctx->sync();
+      if (nruns) {
+        // every run's first key and length; short and long runs apart
+        DevBuf<uint32_t> first(ctx, nruns);
+        DevBuf<uint2> all(ctx, nruns);
+        k_flag_positions<<<grid_for(nkeys, 256), 256, 0, s>>>(head.p, pos.p, nkeys, first.p);
+        k_blk_run_len<<<grid_for(nruns, 256), 256, 0, s>>>(key.p, nkeys, blk_of.p, first.p, nruns, all.p);
+        std::vector<uint2> h(nruns), hs, hl;
+        TB_HIP(hipMemcpyAsync(h.data(), all.p, 8ull * nruns, hipMemcpyDeviceToHost, s));
+        ctx->sync();
+        for (const uint2& r : h) (r.y <= kPairShort ? hs : hl).push_back(r);
+        nruns = (uint32_t)hs.size();
+        nruns_long = (uint32_t)hl.size();
+        if (nruns) {
+          runs = DevBuf<uint2>(ctx, nruns);
+          TB_HIP(hipMemcpyAsync(runs.p, hs.data(), 8ull * nruns, hipMemcpyHostToDevice, s));
+        }
+        if (nruns_long) {
+          runs_long = DevBuf<uint2>(ctx, nruns_long);
+          TB_HIP(hipMemcpyAsync(runs_long.p, hl.data(), 8ull * nruns_long, hipMemcpyHostToDevice, s));
+        }
+      }
+    }
+    TB_HIP(hipGetLastError());
+    ctx->sync();
+  }
   B.nblocks = nb;
   B.vertices = verts;
   B.macs = macs;
   B.A = A.release();
   B.work = dwork.release();
   B.nwork = (uint32_t)work.size();
-  B.rng = rng.release();
+  const uint64_t kept = crng.n;
+  B.rng = crng.release();
+  B.ptr = bptr.release();
   B.w = w.release();
   B.cut = hcut;
-  g->bytes += total + 16ull * work.size() + 8 * m + 8 * n;
+  B.key = key.release();
+  B.nkeys = nkeys;
+  B.runs = runs.release();
+  B.nruns = nruns;
+  B.runs_long = runs_long.release();
+  B.nruns_long = nruns_long;
+  B.blk = dblk.release();
+  B.blk_of = blk_of.release();
+  B.aoff = daoff.release();
+  B.kps = dkp.release();
+  g->bytes += total + 16ull * work.size() + 8 * kept + 16 * n + 8 * nkeys + 8ull * (nruns + nruns_long) + 16ull * nb;
   return true;
 }
 
 void blocks_launch(tribit_ctx* ctx, const tribit_graph* g, unsigned long long* result) {
-  if (!g->blocks.A || !g->blocks.nwork) return;
-  k_blocks_count<<<g->blocks.nwork, kCoreThreads, 0, ctx->stream>>>(g->blocks.A, g->blocks.work, result);
+  const tribit_graph::DenseBlocks& B = g->blocks;
+  if (!B.A || !B.nwork) return;
+  k_blocks_count<<<B.nwork, kCoreThreads, 0, ctx->stream>>>(B.A, B.work, result);
+  if (B.nruns)
+    k_blk_pairs<<<(B.nruns + 255) / 256, 256, 0, ctx->stream>>>(B.key, B.runs, B.nruns, B.blk_of, B.blk, B.aoff,
+                                                                B.kps, B.A, result);
+  if (B.nruns_long)
+    k_blk_pairs_long<<<B.nruns_long, 256, 0, ctx->stream>>>(B.key, B.runs_long, B.blk_of, B.blk, B.aoff, B.kps, B.A,
+                                                              result);
   TB_HIP(hipGetLastError());
 }
 

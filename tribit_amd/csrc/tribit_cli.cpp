@@ -240,7 +240,8 @@ int main(int argc, char** argv) {
   tribit_graph_get_info(g, &gi);
   // preprocessing = the graph's build + the range layout the count ran on, when
   // that layout was built on the count's first use (outside its Kernel time)
-  const double ms_prep = gi.ms_preprocess + (ci.layout >= 0 && ci.layout < 4 ? gi.ms_layout[ci.layout] : 0.0);
+  const double ms_prep = gi.ms_preprocess + (ci.layout >= 0 && ci.layout < 4 ? gi.ms_layout[ci.layout] : 0.0) +
+                        (ci.dense_blocks ? gi.ms_blocks : 0.0);
   gi.ms_preprocess = ms_prep;
 
   printf(
