@@ -344,8 +344,8 @@ double core_macs(uint64_t kp, uint64_t tile) {
 
 static double kCoreMacPerMs = 1.3e12;
 
-// Core size chosen for a relabelled graph: the candidate (4096 .. 65536
-// ids, at most n) with the largest estimated saving
+// Core size chosen for a relabelled graph: the candidate (multiples of 4096
+// up to 65536 ids, at most n) with the largest estimated saving
 //     streamed wedges of the core / wedge rate  -  MACs / MAC rate
 // if that saving exceeds 0.5 ms; 0 = no core. The rates are the kernels'
 // measured figures on MI355X (streaming kernels ~2e12 wedge checks/s, the
@@ -359,9 +359,9 @@ uint64_t core_choose(tribit_ctx* ctx, const tribit_graph* g) {
   ctx->sync();
   uint64_t best = 0;
   double best_gain = 0.5;
-  for (uint64_t c = 4096; c <= top; c *= 2) {
-    double wedges = 0.0;
-    for (uint64_t i = top - c; i < top; ++i) {
+  double wedges = 0.0;  // of the top c ids, grown 4,096 ids at a time
+  for (uint64_t c = 4096; c <= top; c += 4096) {
+    for (uint64_t i = top - c; i < top - c + 4096; ++i) {
       const double d = (double)(rp[i + 1] - rp[i]);
       wedges += d * (d - 1.0) * 0.5;
     }
@@ -380,13 +380,10 @@ uint64_t core_choose(tribit_ctx* ctx, const tribit_graph* g) {
 // The same contraction on many small dense id ranges [s, e) instead of one
 // top core (the blocks of a block model, once the octave-class relabelling
 // keeps each block a run of consecutive ids). A triangle a < b < c with all
-// three vertices in one block is counted by the block's masked product; the
-// streaming kernels count the rest over a copy of the wedge ranges in which
-// the range of every arc u -> v inside a block (the part of N+(u) after v,
-// streamed against N+(v)) starts at the first id >= e instead: the elements
-// it loses are exactly the third vertices inside the block. Arcs entering the
-// block from below keep their whole range, so triangles with the smallest
-// vertex outside the block are all streamed.
+// three vertices in one block is counted by the block's masked product, one
+// with a and b in a block and c after it by the pair kernel (k_blk_pairs),
+// and the rest, whose arc a -> b crosses blocks, by the streaming kernels over
+// a copy of the wedge ranges without those of in-block arcs (k_blk_trim).
 //
 // Detection: cross[p] = arcs a -> b with b - a <= kSpanMax and a <= p < b
 // (short arcs over the gap between ids p and p + 1). A gap with fewer than
