@@ -318,6 +318,9 @@ def warm_up(ctx) -> None:
         w.count(sched="plain")
 
 
+ID_ORDER = {0: "input ids", 1: "degree-rank ids", 2: "octave-class ids"}
+
+
 def mixed_family(args) -> int:
     """One JSON line per member of the mixed family (plain vs selected schedule)."""
     import tribit_amd
@@ -351,7 +354,8 @@ def mixed_family(args) -> int:
             "ms_preprocess_selected": round(inf["ms_preprocess"] + after["ms_layout"][sel["layout"]] +
                                             (after["ms_blocks"] if sel.get("dense_blocks", 0) else 0.0), 3),
             "ms_dense_blocks_prepare": round(after["ms_blocks"], 3),
-            "relabel": args.relabel, "candidate": sel["candidate"], "ms_judge": round(sel["ms_judge"], 3),
+            "relabel": args.relabel, "ids": ID_ORDER[g.id_order()], "candidate": sel["candidate"],
+            "ms_judge": round(sel["ms_judge"], 3),
             "ms_candidates": [round(x, 3) for x in sel["ms_candidates"]],
             "dense_blocks": sel.get("dense_blocks", 0), "ms_dense": round(sel["ms_dense"], 4),
             "cpu_check": check,
@@ -569,8 +573,9 @@ def main() -> int:
     ap.add_argument("--batch-graphs", type=int, default=4096, help="graphs of the batch workload")
     ap.add_argument("--batch-sched", choices=["plain", "directed", "wedge", "learned"], default="wedge",
                     help="batch workload: the schedule of every part")
-    ap.add_argument("--relabel", type=int, default=1, choices=[0, 1, 2],
-                    help="mixed workload: vertex ids (1 degree rank, 2 octave classes of the degree, 0 input ids)")
+    ap.add_argument("--relabel", type=int, default=3, choices=[0, 1, 2, 3],
+                    help="mixed workload: vertex ids (1 degree rank, 2 octave classes of the degree, 0 input ids, "
+                         "3 the library's automatic choice between 1 and 2)")
     ap.add_argument("--partitions", type=int, default=64, help="rank workload: partitions P")
     ap.add_argument("--partition", type=int, default=0, help="rank workload: the column partition")
     ap.add_argument("--workload", choices=["rmat", "mixed", "batch", "rank"], default="rmat",
@@ -663,7 +668,7 @@ def main() -> int:
         "data": "synthetic (R-MAT generated in HBM by the library's counter-based generator)",
         "config": {
             "workload": f"R-MAT scale {args.scale}, edge factor {args.ef}, (a,b,c,d)=(0.57,0.19,0.19,0.05), "
-                        f"seed {args.seed}; symmetrised, deduplicated, self-loops removed; degree-rank ids; "
+                        f"seed {args.seed}; symmetrised, deduplicated, self-loops removed; {ID_ORDER[g.id_order()]}; "
                         f"degree-oriented int32 CSR",
             "n": n, "m": m, "wplus": inf["wplus"], "sched": args.sched, "parallelism": "single GPU",
             "cost_model": "tribit_amd/cost_model.f32" if model else None,

@@ -167,11 +167,13 @@ int tribit_shutdown(tribit_ctx* ctx);
 /* Test hook: fail device allocations beyond `bytes` in total (0 = no cap). */
 int tribit_set_alloc_cap(tribit_ctx* ctx, uint64_t bytes);
 /* Context options (tribit_set_option). */
-#define TRIBIT_OPT_RELABEL 1 /* 1 (default): preprocessing renumbers vertices by (degree, id)
-                                rank; 2: by (octave of the degree, id) rank, so runs of consecutive
-                                input ids of similar degree (the blocks of a block model) stay runs
-                                and can be counted as dense diagonal blocks; 0: input ids kept.
-                                Counts are the same either way. */
+#define TRIBIT_OPT_RELABEL 1 /* 1: preprocessing renumbers vertices by (degree, id) rank; 2: by
+                                (octave of the degree, id) rank, so runs of consecutive input ids
+                                of similar degree (the blocks of a block model) stay runs and can be
+                                counted as dense diagonal blocks; 3 (default): 2 when the edges
+                                joining input ids less than 512 apart fill at least 1/16 of such
+                                id pairs (and n >= 4096), else 1; 0: input ids kept. Counts are the
+                                same either way. */
 #define TRIBIT_OPT_BATCH_PART_KEYS 2 /* input keys per batch part (default 2^30) */
 int tribit_set_option(tribit_ctx* ctx, int option, int64_t value);
 /* Peak bytes held from the memory pool since tribit_init. */
@@ -239,9 +241,9 @@ int tribit_graph_get_info(const tribit_graph* g, tribit_graph_info* info);
 /* Copies the oriented CSR to host buffers (row_ptr[n+1], col[m], deg[n]);
  * any pointer may be NULL. Ids are internal ids (see tribit_graph_labels). */
 int tribit_graph_download(const tribit_graph* g, uint32_t* row_ptr, uint32_t* col, uint32_t* deg);
-/* Input id of every internal id (input_id[n]; may be NULL) and whether the
- * graph was relabelled (TRIBIT_OPT_RELABEL). Without relabelling the ids are
- * the input ids. */
+/* Input id of every internal id (input_id[n]; may be NULL) and the ids the
+ * graph got (*relabeled: 0 input ids, 1 degree rank, 2 octave classes; see
+ * TRIBIT_OPT_RELABEL, whose mode 3 resolves to 1 or 2 per graph). */
 int tribit_graph_labels(const tribit_graph* g, uint32_t* input_id, int* relabeled);
 /* Bit tiles of width 32 or 64 of the oriented rows (built on first use; the
  * counting kernels stream them in tile launches): *ntiles = their number;

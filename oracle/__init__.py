@@ -61,9 +61,11 @@ def lib() -> ctypes.CDLL:
         L.or_dedup.argtypes = [_P, _U64]
         L.or_dedup.restype = _U64
         L.or_orient_csr.argtypes = [_P, _U64, _U64, ctypes.POINTER(_Graph)]
+        L.or_id_csr.argtypes = [_P, _U64, _U64, ctypes.POINTER(_Graph)]
         L.or_sort_u64.argtypes = [_P, _U64]
         L.or_sort_u64.restype = None
         L.or_degree_labels.argtypes = [_P, _U64, _U64, _P, _P]
+        L.or_octave_labels.argtypes = [_P, _U64, _U64, _P, _P]
         L.or_relabel_keys.argtypes = [_P, _U64, _P, _P]
         L.or_relabel_keys.restype = None
         L.or_graph_free.argtypes = [ctypes.POINTER(_Graph)]
@@ -164,10 +166,23 @@ def degree_labels(ukeys: np.ndarray, n: int):
     return o2[:n], n2[:n]
 
 
-def relabel(ukeys: np.ndarray, n: int):
-    """Deduplicated keys renumbered by degree rank (sorted) and old_of_new."""
+def octave_labels(ukeys: np.ndarray, n: int):
+    """(old_of_new, new_of_old): new id = rank of (bit length of the degree, id), the id order of
+    TRIBIT_OPT_RELABEL 2 (no counterpart in the reference)."""
     k = np.ascontiguousarray(ukeys, dtype=np.uint64)
-    o2, n2 = degree_labels(k, n)
+    o2 = np.zeros(max(n, 1), np.uint32)
+    n2 = np.zeros(max(n, 1), np.uint32)
+    st = lib().or_octave_labels(_ptr(k), k.size, n, _ptr(o2), _ptr(n2))
+    if st != 0:
+        raise ValueError(f"or_octave_labels status {st}")
+    return o2[:n], n2[:n]
+
+
+def relabel(ukeys: np.ndarray, n: int, mode: int = 1):
+    """Deduplicated keys renumbered by degree rank (mode 1) or octave class (mode 2), sorted, and
+    old_of_new."""
+    k = np.ascontiguousarray(ukeys, dtype=np.uint64)
+    o2, n2 = octave_labels(k, n) if int(mode) == 2 else degree_labels(k, n)
     out = np.zeros(max(k.size, 1), np.uint64)
     lib().or_relabel_keys(_ptr(k), k.size, _ptr(n2), _ptr(out))
     out = out[:k.size].copy()
@@ -176,22 +191,25 @@ def relabel(ukeys: np.ndarray, n: int):
 
 
 class Graph:
-    """Degree-oriented CSR built by the oracle from deduplicated keys."""
+    """Degree-oriented CSR built by the oracle from deduplicated keys (by_id: {a < b} oriented a -> b)."""
 
-    def __init__(self, ukeys: np.ndarray, n: int):
+    def __init__(self, ukeys: np.ndarray, n: int, by_id: bool = False):
         self._k = np.ascontiguousarray(ukeys, dtype=np.uint64)
         self.g = _Graph()
-        st = lib().or_orient_csr(_ptr(self._k), self._k.size, n, ctypes.byref(self.g))
+        fn = lib().or_id_csr if by_id else lib().or_orient_csr
+        st = fn(_ptr(self._k), self._k.size, n, ctypes.byref(self.g))
         if st != 0:
             raise ValueError(f"or_orient_csr status {st}")
         self.n, self.m = self.g.n, self.g.m
 
     @classmethod
-    def from_keys(cls, keys: np.ndarray, n: int, relabel: bool = False) -> "Graph":
+    def from_keys(cls, keys: np.ndarray, n: int, relabel=False) -> "Graph":
+        """relabel: False / 0 input ids, True / 1 degree rank, 2 octave classes."""
         u = dedup(keys)
         if relabel:
-            u, old_of_new = globals()["relabel"](u, n)
-            self = cls(u, n)
+            mode = 2 if relabel == 2 else 1
+            u, old_of_new = globals()["relabel"](u, n, mode)
+            self = cls(u, n, by_id=mode == 2)
             self.labels = old_of_new
             return self
         return cls(u, n)

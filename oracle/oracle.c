@@ -488,7 +488,7 @@ void or_graph_free(or_graph* g) {
   memset(g, 0, sizeof(*g));
 }
 
-int or_orient_csr(const uint64_t* ukeys, uint64_t m, uint64_t n, or_graph* g) {
+static int orient_csr(const uint64_t* ukeys, uint64_t m, uint64_t n, or_graph* g, int by_id) {
   memset(g, 0, sizeof(*g));
   g->n = n;
   g->m = m;
@@ -514,8 +514,8 @@ int or_orient_csr(const uint64_t* ukeys, uint64_t m, uint64_t n, or_graph* g) {
 #pragma omp parallel for schedule(static)
   for (int64_t i = 0; i < (int64_t)m; ++i) {
     uint64_t a = ukeys[i] >> 32, b = ukeys[i] & 0xFFFFFFFFull;
-    /* a < b: a->b iff (deg a, a) < (deg b, b) iff deg a <= deg b */
-    okeys[i] = (g->deg[a] <= g->deg[b]) ? ((a << 32) | b) : ((b << 32) | a);
+    /* a < b: a->b iff (deg a, a) < (deg b, b) iff deg a <= deg b; by_id: a->b */
+    okeys[i] = (by_id || g->deg[a] <= g->deg[b]) ? ((a << 32) | b) : ((b << 32) | a);
   }
   or_sort_u64(okeys, m);
   for (uint64_t i = 0; i < m; ++i) {
@@ -526,6 +526,10 @@ int or_orient_csr(const uint64_t* ukeys, uint64_t m, uint64_t n, or_graph* g) {
   free(okeys);
   return OR_OK;
 }
+
+int or_orient_csr(const uint64_t* ukeys, uint64_t m, uint64_t n, or_graph* g) { return orient_csr(ukeys, m, n, g, 0); }
+
+int or_id_csr(const uint64_t* ukeys, uint64_t m, uint64_t n, or_graph* g) { return orient_csr(ukeys, m, n, g, 1); }
 
 int or_degree_labels(const uint64_t* ukeys, uint64_t m, uint64_t n, uint32_t* old_of_new, uint32_t* new_of_old) {
   /* counting sort of the vertices by degree, stable in id: new id = rank of (deg, id) */
@@ -555,6 +559,37 @@ int or_degree_labels(const uint64_t* ukeys, uint64_t m, uint64_t n, uint32_t* ol
   }
   free(deg);
   free(start);
+  return OR_OK;
+}
+
+int or_octave_labels(const uint64_t* ukeys, uint64_t m, uint64_t n, uint32_t* old_of_new, uint32_t* new_of_old) {
+  /* counting sort of the vertices by octave class, stable in id: class = number
+   * of bits of deg (0 for an isolated vertex), new id = rank of (class, id) */
+  uint32_t* deg = (uint32_t*)calloc(n ? n : 1, sizeof(uint32_t));
+  uint64_t start[34] = {0};
+  if (!deg) return OR_ERR_NOMEM;
+  for (uint64_t i = 0; i < m; ++i) {
+    uint64_t a = ukeys[i] >> 32, b = ukeys[i] & 0xFFFFFFFFull;
+    if (a >= n || b >= n || a >= b) {
+      free(deg);
+      return OR_ERR_RANGE;
+    }
+    deg[a]++;
+    deg[b]++;
+  }
+  for (uint64_t v = 0; v < n; ++v) {
+    uint32_t c = 0;
+    while (c < 32 && (deg[v] >> c)) ++c;
+    deg[v] = c;
+    start[c + 1]++;
+  }
+  for (int c = 0; c < 33; ++c) start[c + 1] += start[c];
+  for (uint64_t v = 0; v < n; ++v) {
+    uint64_t r = start[deg[v]]++;
+    old_of_new[r] = (uint32_t)v;
+    if (new_of_old) new_of_old[v] = (uint32_t)r;
+  }
+  free(deg);
   return OR_OK;
 }
 

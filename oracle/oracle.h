@@ -28,6 +28,8 @@
  *                        its count with or_count.
  *   or_degree_labels     a permutation; degrees non-decreasing along new ids,
  *                        ties in input-id order; counts unchanged.
+ *   or_octave_labels     equals a sort of (int.bit_length(deg), id) computed
+ *                        in Python; counts unchanged.
  *   or_tiles             popcount(tile masks) == row lengths; decode of the
  *                        tiles == CSR rows.
  */
@@ -116,12 +118,19 @@ typedef struct {
 /* Degree orientation of deduplicated keys: edge {a<b} is a->b iff
  * deg(a) <= deg(b) (rank = (deg, id), ties by lower id), else b->a. */
 int or_orient_csr(const uint64_t* ukeys, uint64_t m, uint64_t n, or_graph* g);
+/* Id orientation: edge {a<b} is a->b (the orientation of octave-class ids,
+ * whose order is not the degree order inside a class). */
+int or_id_csr(const uint64_t* ukeys, uint64_t m, uint64_t n, or_graph* g);
 void or_graph_free(or_graph* g);
 
 /* Degree relabelling: new id = rank of (deg, id) (a counting sort by degree,
  * stable in id). old_of_new[n] and, if not NULL, new_of_old[n]. With these ids
  * the degree orientation above is the id order (a -> b iff a < b). */
 int or_degree_labels(const uint64_t* ukeys, uint64_t m, uint64_t n, uint32_t* old_of_new, uint32_t* new_of_old);
+/* Octave-class relabelling: new id = rank of (bit length of deg, id), a
+ * counting sort over 33 classes, stable in id. No counterpart in the
+ * reference: the library's block-preserving id order (TRIBIT_OPT_RELABEL 2). */
+int or_octave_labels(const uint64_t* ukeys, uint64_t m, uint64_t n, uint32_t* old_of_new, uint32_t* new_of_old);
 /* out[i] = (min<<32)|max of the relabelled pair of ukeys[i] (not re-sorted). */
 void or_relabel_keys(const uint64_t* ukeys, uint64_t m, const uint32_t* new_of_old, uint64_t* out);
 

@@ -7,7 +7,11 @@ the HIP path with them without running the oracle on the GPU host.
 Per single graph: n, input keys, m (undirected simple edges), W+ (sum over
 degree-oriented arcs u->v of outdeg(v), ties by lower id), triangles and the
 sha256 of the deduplicated key set (the GPU generator must produce the same
-set). For the batch of 4,096 Chung-Lu graphs (seed 3): every graph's count,
+set); the edges joining ids less than 512 apart, the id order the library's
+default (TRIBIT_OPT_RELABEL 3) gives the graph by its rule (octave classes, 2,
+when those edges fill at least 1/16 of such id pairs and n >= 4096, else
+degree rank, 1) and W+ under those ids (octave-class ids orient {a < b} as
+a -> b). `--ids-only` adds the last three to existing records without counting. For the batch of 4,096 Chung-Lu graphs (seed 3): every graph's count,
 vertices and edges, the sha256 of the uint64 count array and the total.
 
 Generators, dedup, orientation and counting are the oracle's (oracle/oracle.c);
@@ -67,6 +71,18 @@ def keys_of(spec: dict):
     raise ValueError(g)
 
 
+def id_fields(u: np.ndarray, n: int, wplus: int) -> dict:
+    """Band edges, the default id order's choice and W+ under it."""
+    gap = (u & np.uint64(0xFFFFFFFF)) - (u >> np.uint64(32))
+    band = int(np.count_nonzero(gap < np.uint64(512)))
+    w = min(512, n)
+    ids = 2 if n >= 4096 and band * 16 >= (w - 1) * n - w * (w - 1) / 2 else 1
+    if ids == 2:
+        r, _ = oracle.relabel(u, n, 2)
+        wplus = int(oracle.Graph(r, n, by_id=True).wplus())
+    return {"band_edges": band, "ids": ids, "wplus_ids": wplus}
+
+
 def single(name: str, desc: str, spec: dict) -> dict:
     t0 = time.perf_counter()
     keys, n = keys_of(spec)
@@ -76,11 +92,12 @@ def single(name: str, desc: str, spec: dict) -> dict:
     sha = hashlib.sha256(u.tobytes()).hexdigest()
     g = oracle.Graph(u, n)
     m = int(g.m)
-    del u
     wplus = int(g.wplus())
+    ids = id_fields(u, n, wplus)
+    del u
     tri = int(g.count())
     rec = {"name": name, "workload": desc, "spec": spec, "n": n, "entries": entries, "m": m, "wplus": wplus,
-           "triangles": tri, "keys_sha256": sha}
+           "triangles": tri, "keys_sha256": sha, **ids}
     print(f"{name}: n={n} m={m} W+={wplus} triangles={tri} ({time.perf_counter() - t0:.1f} s)", flush=True)
     return rec
 
@@ -111,6 +128,7 @@ def batch(name: str, desc: str, spec: dict) -> dict:
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--only", nargs="*", help="configuration names to (re)compute; others are kept")
+    ap.add_argument("--ids-only", action="store_true", help="only add the id-order fields to single graphs")
     args = ap.parse_args()
     if not os.path.exists(oracle.LIB_PATH):
         oracle.build()
@@ -120,10 +138,18 @@ def main() -> None:
             out = json.load(f)
     todo = set(args.only) if args.only else set(SINGLE) | set(BATCH)
     for name, (desc, spec) in SINGLE.items():
-        if name in todo:
+        if name in todo and args.ids_only:
+            keys, n = keys_of(spec)
+            u = oracle.dedup(keys)
+            del keys
+            assert hashlib.sha256(u.tobytes()).hexdigest() == out[name]["keys_sha256"], name
+            out[name].update(id_fields(u, n, out[name]["wplus"]))
+            print(f"{name}: {out[name]['band_edges']} band edges, ids {out[name]['ids']}, "
+                  f"W+ {out[name]['wplus_ids']}", flush=True)
+        elif name in todo:
             out[name] = single(name, desc, spec)
     for name, (desc, spec) in BATCH.items():
-        if name in todo:
+        if name in todo and not args.ids_only:
             out[name] = batch(name, desc, spec)
     with open(OUT, "w") as f:
         json.dump(out, f, indent=1)

@@ -2,7 +2,8 @@
 CPU oracle's results committed in tests/golden/configs.json (written by
 tests/golden/make_configs.py; the GPU host does not run the oracle here).
 
-Single graphs: m, W+ and the exact triangle count of the learned schedule
+Single graphs, under the default id order: the ids it picks, m, W+ under those
+ids, and the exact triangle count of the learned schedule
 (with the committed cost model and its judge), of the wedge schedule with
 forced multi-task waves, and of the plain (textbook) schedule for the
 smaller ones; repeat determinism. The batch: all 4,096 Chung-Lu graphs of
@@ -54,10 +55,12 @@ def model_ctx(gpu_ctx):
 def test_config_exact(model_ctx, name):
     want = CONFIGS[name]
     note(f"{name}: {want['workload']}")
+    model_ctx.set_relabel("auto")
     with build(model_ctx, want["spec"]) as g:
         inf = g.info()
+        assert g.id_order() == want["ids"]
         assert (inf["n"], inf["entries"], inf["m"], inf["wplus"]) == \
-            (want["n"], want["entries"], want["m"], want["wplus"])
+            (want["n"], want["entries"], want["m"], want["wplus_ids"])
         learned = g.count(sched="learned")
         assert learned.triangles == want["triangles"], (learned.triangles, want["triangles"])
         again = g.count(sched="learned", repeats=2)

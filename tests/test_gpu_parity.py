@@ -79,12 +79,13 @@ def test_wiki_vote(gpu_ctx, oracle_mod):
 
 
 def test_relabel_option(gpu_ctx, oracle_mod):
-    """Input ids kept (relabelling off) and degree-rank ids give the same counts; both CSRs equal the oracle's."""
-    note("relabel on/off, rmat s14 ef8 seed 3")
+    """Input ids kept (relabelling off), degree-rank ids and octave-class ids give the same counts; every
+    CSR equals the oracle's under the same ids."""
+    note("relabel off / degree rank / octave classes, rmat s14 ef8 seed 3")
     keys = oracle_mod.gen_rmat(14, 8, seed=3)
     n = 1 << 14
     want = oracle_mod.count_keys(keys, n)
-    for on in (False, True):
+    for on in (False, True, 2):
         gpu_ctx.set_relabel(on)
         try:
             with gpu_ctx.graph_from_keys(keys, n) as g:
@@ -667,3 +668,47 @@ def test_dense_diagonal_blocks(gpu_ctx, oracle_mod):
                 assert r.triangles == oracle_mod.count_bruteforce(nn, kk), seed
         finally:
             gpu_ctx.set_relabel(1)
+
+
+def auto_mode(oracle_mod, keys, n):
+    """The automatic id order's rule, restated on the host: octave classes (2) when the edges joining
+    ids less than 512 apart fill at least 1/16 of such id pairs and n >= 4096, else degree rank (1)."""
+    if n < 4096:
+        return 1
+    u = oracle_mod.dedup(keys)
+    gap = (u & np.uint64(0xFFFFFFFF)).astype(np.int64) - (u >> np.uint64(32)).astype(np.int64)
+    w = min(512, n)
+    pairs = (w - 1) * n - w * (w - 1) / 2
+    return 2 if np.count_nonzero(gap < 512) * 16 >= pairs else 1
+
+
+def test_auto_ids(gpu_ctx, oracle_mod):
+    """The default id order (TRIBIT_OPT_RELABEL 3) picks octave classes for a block model with dense
+    512-id blocks and degree rank for sparse blocks, R-MAT, a cell-ordered geometric graph and graphs
+    under 4,096 ids; CSR, degrees, labels and W+ equal the oracle's under the chosen ids, counts are
+    exact, and the learned judge times the dense-block candidate on the block model."""
+    cliques = np.concatenate([graphs.complete(k, offset=o)[0]
+                              for k, o in ((70, 0), (130, 70), (256, 200), (300, 456), (90, 756))])
+    cases = [
+        ("sbm 40x512 p_in 0.3", oracle_mod.gen_sbm(40, 512, 0.3, 1e-4, 7), 40 * 512, 2),
+        ("sbm 40x512 p_in 0.05", oracle_mod.gen_sbm(40, 512, 0.05, 1e-4, 7), 40 * 512, 1),
+        ("rmat s16 ef16", oracle_mod.gen_rmat(16, 16, seed=4), 1 << 16, 1),
+        ("rgg 2^16 avg 24", oracle_mod.gen_rgg(1 << 16, 24.0, 3), 1 << 16, 1),
+        ("cliques on 846 ids", cliques, 846, 1),
+    ]
+    gpu_ctx.set_relabel("auto")
+    try:
+        for name, keys, n, want_mode in cases:
+            note(f"auto ids: {name}")
+            assert auto_mode(oracle_mod, keys, n) == want_mode, name
+            want = oracle_mod.count_keys(keys, n)
+            with gpu_ctx.graph_from_keys(keys, n) as g:
+                assert g.id_order() == want_mode, name
+                assert_same_structure(g, keys, n, oracle_mod)
+                assert g.count(sched="wedge").triangles == want
+                r = g.count(sched="learned")
+                assert r.triangles == want
+                if want_mode == 2:  # the dense-block candidate is judged
+                    assert r.ms_candidates[4] > 0, r.ms_candidates
+    finally:
+        gpu_ctx.set_relabel(1)

@@ -267,6 +267,36 @@ def test_degree_labels(oracle_mod, seed):
     assert np.array_equal(np.sort(arcs), np.sort(arcs0))
 
 
+@pytest.mark.parametrize("seed", [2, 7])
+def test_octave_labels(oracle_mod, seed):
+    """Octave-class labels equal a Python sort of (int.bit_length(degree), id); the id-oriented CSR
+    under them is a -> b iff a < b, with the same m and count, and its arcs map back to the input
+    edges."""
+    keys = np.concatenate([oracle_mod.gen_sbm(6, 200, 0.3, 0.01, seed), oracle_mod.gen_rmat(10, 4, seed=seed)])
+    n = 1200
+    keys = keys[(keys >> np.uint64(32)) < np.uint64(n)]
+    keys = keys[(keys & np.uint64(0xFFFFFFFF)) < np.uint64(n)]
+    u = oracle_mod.dedup(keys)
+    a = (u >> np.uint64(32)).astype(np.int64)
+    b = (u & np.uint64(0xFFFFFFFF)).astype(np.int64)
+    deg = np.bincount(np.concatenate([a, b]), minlength=n)
+    want = sorted(range(n), key=lambda v: (int(deg[v]).bit_length(), v))
+    old_of_new, new_of_old = oracle_mod.octave_labels(u, n)
+    assert old_of_new.tolist() == want
+    assert np.array_equal(new_of_old[old_of_new], np.arange(n, dtype=np.uint32))
+    g0 = oracle_mod.Graph(u, n)
+    g2 = oracle_mod.Graph.from_keys(keys, n, relabel=2)
+    assert np.array_equal(g2.labels, old_of_new)
+    rp, col, d2 = g2.arrays()
+    src = np.repeat(np.arange(n, dtype=np.int64), np.diff(rp).astype(np.int64))
+    assert np.all(src < col.astype(np.int64))
+    assert np.array_equal(d2, deg[old_of_new].astype(np.uint32))
+    assert (g2.m, g2.count()) == (g0.m, g0.count()) == (g0.m, oracle_mod.count_bruteforce(n, u))
+    lo = np.minimum(old_of_new[src], old_of_new[col]).astype(np.uint64)
+    hi = np.maximum(old_of_new[src], old_of_new[col]).astype(np.uint64)
+    assert np.array_equal(np.sort((lo << np.uint64(32)) | hi), u)
+
+
 @pytest.mark.parametrize("width", [32, 64])
 def test_tiles_decode(oracle_mod, width):
     keys = oracle_mod.gen_rmat(11, 12, seed=3)

@@ -578,13 +578,19 @@ class Graph:
                                       mask.ctypes.data), "graph_tiles")
         return tp, blk[:nt.value], mask[:nt.value]
 
+    def id_order(self) -> int:
+        """The ids the graph got: 0 input ids, 1 degree rank, 2 octave classes."""
+        rl = ctypes.c_int()
+        _check(self.ctx.lib.tribit_graph_labels(self.h, None, ctypes.byref(rl)), "graph_labels")
+        return int(rl.value)
+
     def labels(self):
-        """(input id of every internal id uint32[n], relabelled?)."""
+        """(input id of every internal id uint32[n], id order: 0 input ids, 1 degree rank, 2 octave classes)."""
         n = self.info()["n"]
         ids = np.zeros(max(n, 1), np.uint32)
         rl = ctypes.c_int()
         _check(self.ctx.lib.tribit_graph_labels(self.h, ids.ctypes.data, ctypes.byref(rl)), "graph_labels")
-        return ids[:n], bool(rl.value)
+        return ids[:n], int(rl.value)
 
     def count(self, sched: str = "plain", repeats: int = 1, flags: int = 0, group_width=None,
               dense_core: int = 0) -> CountResult:
@@ -704,8 +710,11 @@ class Context:
         _check(self.lib.tribit_set_option(self.h, OPT_BATCH_PART_KEYS, nkeys), "set_option")
 
     def set_relabel(self, on) -> None:
-        """Relabelling of graphs built from now on: True / 1 degree rank (default), 2 octave classes of the
-        degree (keeps runs of consecutive ids of similar degree together), False / 0 input ids."""
+        """Relabelling of graphs built from now on: True / 1 degree rank, 2 octave classes of the degree
+        (keeps runs of consecutive ids of similar degree together), 3 / "auto" (default) 2 when the edges
+        joining ids less than 512 apart fill at least 1/16 of such id pairs, else 1; False / 0 input ids."""
+        if on == "auto":
+            on = 3
         mode = 1 if on is True else 0 if on is False else int(on)
         _check(self.lib.tribit_set_option(self.h, OPT_RELABEL, mode), "set_option")
 

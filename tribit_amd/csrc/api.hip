@@ -121,7 +121,7 @@ int tribit_set_option(tribit_ctx* ctx, int option, int64_t value) {
   if (!ctx) return TRIBIT_ERR_ARG;
   switch (option) {
     case TRIBIT_OPT_RELABEL:
-      if (value != 0 && value != 1 && value != 2) return TRIBIT_ERR_ARG;
+      if (value < 0 || value > 3) return TRIBIT_ERR_ARG;
       ctx->relabel = (int)value;
       return TRIBIT_OK;
     case TRIBIT_OPT_BATCH_PART_KEYS:
@@ -776,7 +776,9 @@ uint64_t count_diagonal(tribit_ctx* ctx, const uint64_t* const* chunks, const ui
   try {
     uint64_t nk = 0;
     uint64_t* d = upload_chunks(ctx, chunks, ptr, num, &nk, &pi.ms_h2d);
-    tb::preprocess(ctx, d, nk, n, g);
+    // counted once without judging (no dense diagonal blocks): the automatic id
+    // order resolves to degree rank, which streams faster
+    tb::preprocess(ctx, d, nk, n, g, ctx->relabel == 3 ? 1 : ctx->relabel);
     tribit_count_opts o;
     memset(&o, 0, sizeof(o));
     o.sched = TRIBIT_SCHED_LEARNED;

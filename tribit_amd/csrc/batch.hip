@@ -123,7 +123,9 @@ void build_part(tribit_ctx* ctx, tribit_batch* b, uint32_t first, uint32_t count
   part.count = count;
   part.g = new tribit_graph;
   try {
-    preprocess(ctx, keys.release(), nk, n, part.g);
+    // a part's replay never counts dense diagonal blocks, so the automatic id
+    // order resolves to degree rank, which streams faster
+    preprocess(ctx, keys.release(), nk, n, part.g, ctx->relabel == 3 ? 1 : ctx->relabel);
     DevBuf<uint32_t> gid(ctx, n);
     k_vertex_gid<<<grid_for(n, 256), 256, 0, s>>>(part.g->labels, n, voff.p, count, first, gid.p);
     DevBuf<unsigned long long> m(ctx, b->ngraphs);

@@ -61,7 +61,7 @@ struct tribit_ctx {
   uint64_t peak = 0;
   std::unordered_map<void*, uint64_t> sizes;
   hipEvent_t ev[16] = {};
-  int relabel = 1;     // TRIBIT_OPT_RELABEL: degree-rank relabelling in preprocessing
+  int relabel = 3;     // TRIBIT_OPT_RELABEL: 0 input ids, 1 degree rank, 2 octave classes, 3 auto
   tribit_mlp* model = nullptr;  // cost model of the learned scheduler (not owned)
   uint64_t model_version = 1;   // bumped whenever the model is set (judged choices go stale)
   uint64_t part_keys = 1ull << 30;  // TRIBIT_OPT_BATCH_PART_KEYS: input keys per batch part
@@ -164,7 +164,7 @@ struct tribit_graph {
   uint32_t* col = nullptr;      // m, ascending ids within a row
   uint32_t* deg = nullptr;      // n, undirected degree (by internal id)
   uint32_t* labels = nullptr;   // n, input id of each internal id (null: ids kept)
-  int relabeled = 0;
+  int relabeled = 0;  // ids: 0 input, 1 degree rank, 2 octave classes
   uint64_t* wrow = nullptr;     // n, W+(u) = sum of outdeg(v) over v in N+(u)
   // streamed-range lists (see prep.hip): forward-only and direction-split
   uint2* rng_all = nullptr;     // m, in CSR order; owner u's ranges at row_ptr[u]
@@ -278,7 +278,8 @@ struct tribit_batch {
 
 namespace tb {
 // prep.hip
-void preprocess(tribit_ctx* ctx, uint64_t* d_keys, uint64_t nkeys, uint64_t n, tribit_graph* g);
+// relabel: a TRIBIT_OPT_RELABEL mode, or -1 for the context's option.
+void preprocess(tribit_ctx* ctx, uint64_t* d_keys, uint64_t nkeys, uint64_t n, tribit_graph* g, int relabel = -1);
 void build_layout(tribit_ctx* ctx, tribit_graph* g, int layout);
 void build_offdiag(tribit_ctx* ctx, const uint64_t* d_cols, uint64_t ncols, const uint64_t* d_idxs, uint64_t nidxs,
                    uint64_t n, tribit_graph* g);
