@@ -20,6 +20,7 @@
 #include <cstring>
 #include <algorithm>
 #include <chrono>
+#include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
 #include <vector>
 
@@ -576,6 +577,26 @@ struct TaskFeed {
   }
 };
 
+// A wave's per-graph counts (batches), summed while consecutive tasks belong
+// to one graph and added to that graph's counter when the graph changes: one
+// atomic per run of tasks instead of one per task, so that tasks grouped by
+// graph do not all add to the same address at once. Wave-uniform.
+struct GraphSum {
+  uint32_t g = 0xFFFFFFFFu;
+  uint64_t sum = 0;
+  __device__ void flush(unsigned long long* gcount, uint32_t lane) {
+    if (lane == 0 && sum) atomicAdd(&gcount[g], (unsigned long long)sum);
+    sum = 0;
+  }
+  __device__ void add(unsigned long long* gcount, uint32_t graph, uint64_t v, uint32_t lane) {
+    if (graph != g) {
+      flush(gcount, lane);
+      g = graph;
+    }
+    sum += v;
+  }
+};
+
 template <int LOGS, int KMAX, int G, int U, bool BUF>
 __global__ __launch_bounds__(256) void k_count_wave(const uint32_t* __restrict__ row_ptr,
                                                     const uint32_t* __restrict__ col, uint64_t m,
@@ -595,6 +616,7 @@ __global__ __launch_bounds__(256) void k_count_wave(const uint32_t* __restrict__
   uint2* R = sR[wv];
   uint64_t acc = 0;
   uint32_t fallbacks = 0;
+  GraphSum gsum;
   TaskFeed feed(queue, ntasks, 1, wb);
   uint32_t tb, tend;
   while (feed.get(lane, tb, tend)) {
@@ -630,13 +652,11 @@ __global__ __launch_bounds__(256) void k_count_wave(const uint32_t* __restrict__
         wave_lds_sync();
       }
       acc += cnt;
-      if (gcount) {  // per-graph counts of a batch: the owner's graph
-        const uint64_t tc = wave_sum_u64(cnt);
-        if (lane == 0 && tc) atomicAdd(&gcount[gid[x]], (unsigned long long)tc);
-      }
+      if (gcount) gsum.add(gcount, gid[x], wave_sum_u64(cnt), lane);  // per-graph counts of a batch
       if (tdur && lane == 0) tdur[ntasks - 1 - t] = (uint32_t)(wall_clock64() - t0);
     }
   }
+  if (gcount) gsum.flush(gcount, lane);
   acc = wave_sum_u64(acc);
   if (lane == 0 && acc) atomicAdd(result, (unsigned long long)acc);
   if (lane == 0 && fallbacks) atomicAdd(result + 1, (unsigned long long)fallbacks);
@@ -679,6 +699,7 @@ __global__ __launch_bounds__(256) void k_count_wave_multi(const uint32_t* __rest
   unsigned long long* A = sAcc[wv];
   uint64_t acc = 0;
   uint32_t fallbacks = 0;
+  GraphSum gsum;
   TaskFeed feed(queue, ntasks, MT, wb / MT);
   uint32_t tb, tend;
   while (feed.get(lane, tb, tend)) {
@@ -765,15 +786,18 @@ __global__ __launch_bounds__(256) void k_count_wave_multi(const uint32_t* __rest
         wave_lds_sync();
       }
       acc += cnt;
-      if (gcount && lane < nt) {  // the owner's graph
-        const unsigned long long v = A[lane];
+      if (gcount) {  // each task's count to its owner's graph, in task order
+        const unsigned long long v = lane < nt ? A[lane] : 0ull;
+        const uint32_t g = lane < nt ? gid[tk.x] : 0u;
         acc += v;
-        if (v) atomicAdd(&gcount[gid[tk.x]], v);
+        for (uint32_t j = 0; j < nt; ++j)
+          gsum.add(gcount, (uint32_t)__shfl((int)g, (int)j), (uint64_t)__shfl((long long)v, (int)j), lane);
       }
       if (tdur && lane < nt) tdur[ntasks - 1 - (t0 + lane)] = (uint32_t)((wall_clock64() - c0t) / nt);
       wave_lds_sync();  // the next batch rewrites the task table and sets
     }
   }
+  if (gcount) gsum.flush(gcount, lane);
   acc = wave_sum_u64(acc);
   if (lane == 0 && acc) atomicAdd(result, (unsigned long long)acc);
   if (lane == 0 && fallbacks) atomicAdd(result + 1, (unsigned long long)fallbacks);
@@ -2548,6 +2572,49 @@ void batch_drop_exec(tribit_batch* b) {
   b->graph = nullptr;
 }
 
+// Graph-major task order for a batch part's wave launches: key = the owner's
+// graph, value = the task's position; a stable sort keeps owner order inside a
+// graph (hubs still first within it, the kernels taking tasks from the end).
+__global__ void k_task_graph(const uint4* __restrict__ t, uint32_t nt, const uint32_t* __restrict__ gid,
+                             uint32_t* __restrict__ key, uint32_t* __restrict__ idx) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nt) return;
+  key[j] = gid[t[j].x];
+  idx[j] = j;
+}
+
+__global__ void k_task_gather(const uint4* __restrict__ t, const uint32_t* __restrict__ idx, uint32_t nt,
+                              uint4* __restrict__ out) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < nt) out[j] = t[idx[j]];
+}
+
+// Wave launches of a batch part stream short ranges of rows of the owner's
+// own graph: grouped by graph, the tasks in flight at once touch the rows of
+// a few graphs, which then stay in the caches, instead of the rows of every
+// graph of the part at once (tasks in owner order interleave all graphs of a
+// degree). Block launches keep hub-first order.
+static void order_tasks_by_graph(tribit_ctx* ctx, const tribit_batch::Part& p, Schedule& S) {
+  hipStream_t s = ctx->stream;
+  const uint32_t kbits = bits_for((uint64_t)p.first + p.count);
+  for (int i = 0; i < S.plan.nslots; ++i) {
+    const int kind = S.plan.kind[i];
+    const uint32_t nt = S.tot.v[i];
+    if (nt < 2 || kind_is_block(kind)) continue;
+    uint4* t = S.tasks.p + S.base.v[i];
+    DevBuf<uint32_t> key(ctx, nt), idx(ctx, nt), key2(ctx, nt), idx2(ctx, nt);
+    DevBuf<uint4> out(ctx, nt);
+    k_task_graph<<<grid_for(nt, kBlock), kBlock, 0, s>>>(t, nt, p.gid, key.p, idx.p);
+    size_t bytes = 0;
+    TB_HIP(rocprim::radix_sort_pairs(nullptr, bytes, key.p, key2.p, idx.p, idx2.p, (size_t)nt, 0u, kbits, s));
+    DevBuf<uint8_t> tmp(ctx, bytes ? bytes : 8);
+    TB_HIP(rocprim::radix_sort_pairs(tmp.p, bytes, key.p, key2.p, idx.p, idx2.p, (size_t)nt, 0u, kbits, s));
+    k_task_gather<<<grid_for(nt, kBlock), kBlock, 0, s>>>(t, idx2.p, nt, out.p);
+    TB_HIP(hipMemcpyAsync(t, out.p, (size_t)nt * sizeof(uint4), hipMemcpyDeviceToDevice, s));
+    ctx->sync();  // the temporaries are freed on return
+  }
+}
+
 // Task lists of every part for (sched, flags); returns the device ms spent.
 static double batch_schedules(tribit_ctx* ctx, tribit_batch* b, int sched, int flags) {
   // a learned batch's tables come from the cost model: a new model makes them stale
@@ -2570,6 +2637,7 @@ static double batch_schedules(tribit_ctx* ctx, tribit_batch* b, int sched, int f
     // per-graph attribution: no dense core and no dense blocks (their launches
     // add to the batch total, not to each graph's count)
     make_schedule(ctx, p.g, layout, flags & ~TRIBIT_COUNT_DENSE_BLOCKS, learned, *p.S, 0);
+    order_tasks_by_graph(ctx, p, *p.S);
     for (int i = 0; i < kSlots; ++i) {
       p.S->gdef[i] = p.S->plan.g[i];
       p.S->kdef[i] = p.S->plan.kind[i];
