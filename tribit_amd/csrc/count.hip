@@ -537,18 +537,31 @@ __device__ __forceinline__ uint32_t stream_tiles(const TileSrc& ts, const uint2*
 // Claims on one address serialise (about 13 ns each on MI355X): with one
 // queue, a launch of ~100K tasks spent most of its time claiming. Every value
 // is wave-uniform.
+//
+// Contiguous mode (batches, whose tasks are grouped by graph): sub-queue q
+// holds the q-th eighth of the units instead. Workgroup b runs on XCD b % 8
+// and starts on sub-queue b % kQueues, so each XCD streams the rows of its own
+// graphs and its L2 is not filled with the rows the other seven stream.
 struct TaskFeed {
   uint32_t* queue;
   uint32_t ntasks, unit, claim;
+  bool contig;
   uint32_t q, left;      // current sub-queue, sub-queues not yet found empty
   uint32_t c, cend, nq;  // claimed units [c, cend) of sub-queue q, its unit count
+  uint32_t q0;           // contiguous mode: the first unit of sub-queue q
 
-  __device__ TaskFeed(uint32_t* qs, uint32_t nt, uint32_t unit_, uint32_t claim_)
-      : queue(qs), ntasks(nt), unit(unit_), claim(claim_), q(blockIdx.x % kQueues), left(kQueues), c(0),
-        cend(0), nq(units_of(blockIdx.x % kQueues)) {}
+  __device__ TaskFeed(uint32_t* qs, uint32_t nt, uint32_t unit_, uint32_t claim_, bool contig_ = false)
+      : queue(qs), ntasks(nt), unit(unit_), claim(claim_), contig(contig_), q(blockIdx.x % kQueues),
+        left(kQueues), c(0), cend(0), nq(units_of(blockIdx.x % kQueues)), q0(first_unit(blockIdx.x % kQueues)) {}
+
+  __device__ uint32_t first_unit(uint32_t sq) const {
+    const uint64_t nu = (ntasks + unit - 1) / unit;
+    return contig ? (uint32_t)(nu * sq / kQueues) : 0u;
+  }
 
   __device__ uint32_t units_of(uint32_t sq) const {
     const uint32_t nu = (ntasks + unit - 1) / unit;
+    if (contig) return first_unit(sq + 1) - first_unit(sq);
     return nu > sq ? (nu - sq + kQueues - 1) / kQueues : 0u;
   }
 
@@ -556,7 +569,7 @@ struct TaskFeed {
   __device__ bool get(uint32_t lane, uint32_t& tb, uint32_t& tend) {
     for (;;) {
       if (c < cend) {
-        tb = (c * kQueues + q) * unit;
+        tb = (contig ? q0 + c : c * kQueues + q) * unit;
         tend = min(tb + unit, ntasks);
         ++c;
         return true;
@@ -572,6 +585,7 @@ struct TaskFeed {
         --left;
         q = (q + 1) % kQueues;
         nq = units_of(q);
+        q0 = first_unit(q);
       }
     }
   }
@@ -617,7 +631,7 @@ __global__ __launch_bounds__(256) void k_count_wave(const uint32_t* __restrict__
   uint64_t acc = 0;
   uint32_t fallbacks = 0;
   GraphSum gsum;
-  TaskFeed feed(queue, ntasks, 1, wb);
+  TaskFeed feed(queue, ntasks, 1, wb, gcount != nullptr);
   uint32_t tb, tend;
   while (feed.get(lane, tb, tend)) {
     for (uint32_t t = tb; t < tend; ++t) {
@@ -700,7 +714,7 @@ __global__ __launch_bounds__(256) void k_count_wave_multi(const uint32_t* __rest
   uint64_t acc = 0;
   uint32_t fallbacks = 0;
   GraphSum gsum;
-  TaskFeed feed(queue, ntasks, MT, wb / MT);
+  TaskFeed feed(queue, ntasks, MT, wb / MT, gcount != nullptr);
   uint32_t tb, tend;
   while (feed.get(lane, tb, tend)) {
     for (uint32_t t0 = tb; t0 < tend; t0 += MT) {
@@ -2225,7 +2239,10 @@ static bool candidate(tribit_ctx* ctx, tribit_graph* g, const tribit_count_opts&
       heuristic_table(out->plan.table, pref);
       return true;
     default:
-      *out = {TRIBIT_SCHED_PLAIN, plan_for(TRIBIT_SCHED_PLAIN, nullptr, &dummy), false, 0};
+      // the textbook plan has no core of its own; a core the caller asks for
+      // (dense_core > 0) is kept, as in every other candidate
+      *out = {TRIBIT_SCHED_PLAIN, plan_for(TRIBIT_SCHED_PLAIN, nullptr, &dummy), false,
+              opts.dense_core > 0 ? core : 0};
       return true;
   }
 }
