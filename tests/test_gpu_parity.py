@@ -166,6 +166,46 @@ def test_block_classes_and_chunks(gpu_ctx, oracle_mod):
         assert count_all(g) == oracle_mod.count_keys(keys, 5000)
 
 
+def test_cursor_tasks(gpu_ctx, oracle_mod):
+    """Owners of more than 2048 ids outside the wedge layout count each 2048-id chunk of their row in
+    turn, every streamed list keeping a cursor between chunks: K_2600 (two chunks per owner) with input
+    ids spread so that the chunks are LDS bitmaps (one 2^18-id window) or hashed sets (wider), under
+    the plain and directed schedules, every set kind and several lane-group widths; a 2,400-vertex
+    block of density 0.95 on random ids among sparse noise against the oracle."""
+    note("cursor tasks: K_2600 with spread ids, hubs")
+    k = 2600
+    a, b = np.triu_indices(k, 1)
+    want = comb(k, 3)
+    gpu_ctx.set_relabel(0)
+    try:
+        for spread in (1, 97, 1000):  # bitmap chunks; bitmap and hashed; hashed
+            ids = np.arange(k, dtype=np.uint64) * np.uint64(spread) + np.uint64(1)
+            keys = graphs.pack(ids[a], ids[b])
+            n = int(ids[-1]) + 3
+            with gpu_ctx.graph_from_keys(keys, n) as g:
+                for sched in ("plain", "directed"):
+                    for fl in (0, tribit_amd.NO_BITMAP, tribit_amd.FORCE_LINEAR_PROBE):
+                        for gw in (None, (1, 2, 4, 4), (2, 8, 64, 32)):
+                            r = g.count(sched=sched, flags=fl, group_width=gw)
+                            assert r.triangles == want, (spread, sched, fl, gw)
+        rng = np.random.default_rng(21)
+        n = 60000
+        blk = np.sort(rng.choice(n, 2400, replace=False)).astype(np.uint64)  # outdeg up to ~2,280
+        i, j = np.triu_indices(blk.size, 1)
+        sel = rng.random(i.size) < 0.95
+        u = rng.integers(0, n, 200000, dtype=np.uint64)
+        v = rng.integers(0, n, 200000, dtype=np.uint64)
+        keys = np.concatenate([graphs.pack(blk[i[sel]], blk[j[sel]]),
+                               graphs.pack(np.minimum(u, v)[u != v], np.maximum(u, v)[u != v])])
+        want = oracle_mod.count_keys(keys, n)
+        with gpu_ctx.graph_from_keys(keys, n) as g:
+            for sched in ("plain", "directed"):
+                for fl in (0, tribit_amd.NO_BITMAP):
+                    assert g.count(sched=sched, flags=fl).triangles == want, (sched, fl)
+    finally:
+        gpu_ctx.set_relabel(1)
+
+
 def test_bitmap_and_hashed_sets(gpu_ctx, oracle_mod):
     """Block tasks whose owner ids fit a 2^18-id window use LDS bitmaps, wider ones cuckoo sets:
     both, and the hashed sets forced everywhere, give the same counts."""

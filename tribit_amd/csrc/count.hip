@@ -42,6 +42,12 @@ constexpr uint64_t kBlockCap = 1u << 18; // target streamed elements per block t
 constexpr uint32_t kWaveListCap = 1024;   // most lists per wave task
 constexpr uint32_t kBucketShift = 21;   // task.w = chunk | bucket << 21
 constexpr uint32_t kChunkMask = (1u << kBucketShift) - 1;
+// Chunk field of a cursor task: one block task takes every chunk of its
+// owner's row in turn, each of its lists keeping a cursor (at most kChunk
+// lists per task, staged once), instead of one task per chunk whose lists are
+// each cut to the chunk by two binary searches.
+constexpr uint32_t kAllChunks = kChunkMask;
+constexpr uint64_t kCursorCap = 1u << 22;  // target streamed elements per cursor task
 
 // ------------------------------------------------------------ LDS sets
 // N+(u) (or one id-range chunk of it) is held in LDS as a two-table cuckoo
@@ -308,6 +314,76 @@ __device__ __forceinline__ uint32_t stream_ranges(const ColSrc& src, const uint2
   if (ATTR) {
     if (cnt) atomicAdd(&s_acc[sel], (unsigned long long)cnt);
     return 0;
+  }
+  return cnt;
+}
+
+// Cursor streaming (block tasks, SHARED hand-out) for an owner whose row is
+// counted one chunk at a time: range k runs from its cursor R[k].x up to its
+// first element >= bound (the next chunk's first id), and the cursor is left
+// there for the next chunk. The lists are sorted, so the elements below the
+// bound in a step are a prefix of it; elements at or past the bound are not in
+// this chunk's set, so probing them adds nothing.
+template <int G, int U, bool BUF, typename Probe>
+__device__ __forceinline__ uint32_t stream_bounded(const ColSrc& src, uint2* R, uint32_t nk, uint32_t* s_next,
+                                                   uint32_t lane, const Probe& probe, uint32_t bound) {
+  constexpr uint64_t kLeads = leader_mask<G>();
+  const uint32_t g = lane / G, gl = lane % G;
+  const uint64_t below = (g == 0) ? 0ull : ((1ull << (g * G)) - 1);
+  uint32_t base = 0, end = 0, cnt = 0, k = 0;
+  for (;;) {
+    const bool need = base >= end;  // uniform within a group
+    const uint64_t want = __ballot(need && gl == 0);
+    const uint32_t nw = (uint32_t)__popcll(want);
+    uint32_t kb = 0;
+    if (nw) {
+      if (lane == 0) kb = atomicAdd(s_next, nw);
+      kb = __shfl(kb, 0);
+      if (want == kLeads && kb >= nk) break;
+    }
+    if (need) {
+      k = kb + (uint32_t)__popcll(want & below);
+      base = end = 0;
+      if (k < nk) {
+        const uint2 r = R[k];
+        base = r.x;
+        end = r.y;
+      }
+    }
+    const uint32_t p0 = base + gl;
+    const int rem = (int)(end - p0);
+    uint32_t x[U];
+    if (BUF) {
+      const uint32_t off = p0 * 4u;
+#pragma unroll
+      for (int j = 0; j < U; ++j) {
+        const bool ok = j * G < rem;
+        x[j] = __builtin_amdgcn_raw_buffer_load_b32(src.rsrc, (int)(ok ? off : kBufOob), j * G * 4, 0);
+        if (!Probe::kZeroSafe) x[j] = ok ? x[j] : kNoKey;
+      }
+      __builtin_amdgcn_sched_barrier(0);
+    } else {
+#pragma unroll
+      for (int j = 0; j < U; ++j) x[j] = (j * G < rem) ? src.col[p0 + j * G] : kNoKey;
+    }
+    uint32_t under = 0;  // this lane's valid elements below the bound
+#pragma unroll
+    for (int j = 0; j < U; ++j) {
+      cnt += probe(x[j], 0);
+      under += (j * G < rem && x[j] < bound) ? 1u : 0u;
+    }
+#pragma unroll
+    for (int o = 1; o < G; o <<= 1) under += (uint32_t)__shfl_xor((int)under, o);  // group sum
+    if (base < end) {
+      const uint32_t step = min((uint32_t)(G * U), end - base);
+      if (under < step) {  // the bound lies inside this step
+        if (gl == 0) R[k].x = base + under;
+        base = end;
+      } else {
+        base += (uint32_t)(G * U);
+        if (base >= end && gl == 0) R[k].x = end;
+      }
+    }
   }
   return cnt;
 }
@@ -861,46 +937,78 @@ __global__ __launch_bounds__(kBlockThreads) void k_count_block(const uint32_t* _
     const uint32_t x = tk.x, rbase = tk.y, rcount = tk.z, c = tk.w & kChunkMask;
     const uint64_t t0 = tdur ? (uint64_t)wall_clock64() : 0;
     const uint32_t r0 = row_ptr[x], d = row_ptr[x + 1] - r0;
+    // The set of N+(x)[q0, q1) in T: a bitmap over [base, base + kSpan) when
+    // the ids fit, else a cuckoo set (a linear-probing table when every seed fails).
+    auto build_set = [&](uint32_t q0, uint32_t q1, bool& bm, bool& ck, uint32_t& sa, uint32_t& base) {
+      const uint32_t lo = col[r0 + q0], hi = col[r0 + q1 - 1];
+      base = lo & ~63u;
+      bm = bitmaps && (hi - base) < kSpan && hi < 0xFFF00000u && lo != 0;
+      sa = 0;
+      ck = bm;
+      if (bm) {
+        for (uint32_t i = threadIdx.x; i < 2 * S + 2; i += kBlockThreads) T[i] = 0u;
+        __syncthreads();
+        for (uint32_t i = q0 + threadIdx.x; i < q1; i += kBlockThreads) {
+          const uint32_t y = col[r0 + i] - base;
+          atomicOr(&T[y >> 5], 1u << (y & 31));
+        }
+        __syncthreads();
+      }
+      for (int attempt = 0; attempt < attempts && !ck; ++attempt) {
+        sa = kSeedA[attempt];
+        for (uint32_t i = threadIdx.x; i < 2 * S; i += kBlockThreads) T[i] = kEmpty;
+        if (threadIdx.x == 0) s_fail = 0;
+        __syncthreads();
+        bool fail = false;
+        for (uint32_t i = q0 + threadIdx.x; i < q1; i += kBlockThreads)
+          fail |= !cuckoo_insert<LOGS>(T, col[r0 + i], sa);
+        if (fail) s_fail = 1;
+        __syncthreads();
+        ck = (s_fail == 0);
+        __syncthreads();
+      }
+      if (!ck) {
+        ++fallbacks;
+        for (uint32_t i = threadIdx.x; i < 2 * S; i += kBlockThreads) T[i] = kEmpty;
+        __syncthreads();
+        for (uint32_t i = q0 + threadIdx.x; i < q1; i += kBlockThreads) linear_insert<LOGS + 1>(T, col[r0 + i]);
+      }
+    };
+    uint32_t cnt = 0;
+    if (c == kAllChunks) {
+      // cursor task: every chunk of N+(x) in turn against the same (at most
+      // kChunk) lists, staged once with their cursors
+      for (uint32_t k = threadIdx.x; k < rcount; k += kBlockThreads) {
+        const uint2 r = rng[rbase + k];
+        R[k] = make_uint2(r.x, r.y & src.emask);
+      }
+      const uint32_t nch = (d + kChunk - 1) / kChunk;
+      for (uint32_t ch = 0; ch < nch; ++ch) {
+        const uint32_t q0 = ch * kChunk, q1 = min(q0 + kChunk, d);
+        const uint32_t bound = q1 < d ? col[r0 + q1] : 0xFFFFFFFFu;
+        bool bm, ck;
+        uint32_t sa, sbase;
+        build_set(q0, q1, bm, ck, sa, sbase);
+        if (threadIdx.x == 0) s_next = 0;
+        __syncthreads();
+        if (bm)
+          cnt += stream_bounded<G, U, BUF>(src, R, rcount, &s_next, lane, BitmapProbe<LOGS + 1>{T, sbase}, bound);
+        else if (ck)
+          cnt += stream_bounded<G, U, BUF>(src, R, rcount, &s_next, lane, CuckooProbe<LOGS>{T, sa}, bound);
+        else
+          cnt += stream_bounded<G, U, BUF>(src, R, rcount, &s_next, lane, LinearProbe<LOGS + 1>{T}, bound);
+        __syncthreads();
+      }
+    }
     const uint32_t p0 = c * kChunk, p1 = min(p0 + kChunk, d);
     const bool ranged = (p0 > 0) || (p1 < d);
-    const uint32_t lo = col[r0 + p0], hi = col[r0 + p1 - 1];
-    const uint32_t base = lo & ~63u;
-    const bool bitmap = bitmaps && (hi - base) < kSpan && hi < 0xFFF00000u && lo != 0;
-    uint32_t sa = 0;
-    bool ok = false;
-    if (bitmap) {
-      for (uint32_t i = threadIdx.x; i < 2 * S + 2; i += kBlockThreads) T[i] = 0u;
-      __syncthreads();
-      for (uint32_t i = p0 + threadIdx.x; i < p1; i += kBlockThreads) {
-        const uint32_t y = col[r0 + i] - base;
-        atomicOr(&T[y >> 5], 1u << (y & 31));
-      }
-      __syncthreads();
-      ok = true;
-    }
-    for (int attempt = 0; attempt < attempts && !ok; ++attempt) {
-      sa = kSeedA[attempt];
-      for (uint32_t i = threadIdx.x; i < 2 * S; i += kBlockThreads) T[i] = kEmpty;
-      if (threadIdx.x == 0) s_fail = 0;
-      __syncthreads();
-      bool fail = false;
-      for (uint32_t i = p0 + threadIdx.x; i < p1; i += kBlockThreads)
-        fail |= !cuckoo_insert<LOGS>(T, col[r0 + i], sa);
-      if (fail) s_fail = 1;
-      __syncthreads();
-      ok = (s_fail == 0);
-      __syncthreads();
-    }
-    if (!ok) {
-      ++fallbacks;
-      for (uint32_t i = threadIdx.x; i < 2 * S; i += kBlockThreads) T[i] = kEmpty;
-      __syncthreads();
-      for (uint32_t i = p0 + threadIdx.x; i < p1; i += kBlockThreads) linear_insert<LOGS + 1>(T, col[r0 + i]);
-    }
-    uint32_t cnt = 0;
+    const uint32_t lo = c == kAllChunks ? 0u : col[r0 + p0], hi = c == kAllChunks ? 0u : col[r0 + p1 - 1];
+    bool bitmap = false, ok = false;
+    uint32_t sa = 0, base = 0;
+    if (c != kAllChunks) build_set(p0, p1, bitmap, ok, sa, base);
     const bool tiles = TW != 0 && bitmap;  // block-uniform
     const uint2* rs = tiles ? ts.trng : rng;
-    for (uint32_t c0 = 0; c0 < rcount; c0 += kChunk) {
+    for (uint32_t c0 = 0; c0 < (c == kAllChunks ? 0u : rcount); c0 += kChunk) {
       const uint32_t nk = min((uint32_t)kChunk, rcount - c0);
       uint32_t n1 = 0, n2 = 0;  // with N16: R[0, n1) narrow, R[n1, n1 + n2) narrow across two windows, then wide
       if (N16 && !tiles) {
@@ -1157,7 +1265,21 @@ __device__ __forceinline__ bool list_useful(const SchedView& v, int l, uint32_t 
   return count > 0 && d >= (l == 0 ? v.min_d0 : 1u);
 }
 
-__device__ __forceinline__ uint32_t list_splits(int kind, uint32_t count, uint64_t w) {
+// Cursor tasks: block lists (not tiles) of owners with more than kChunk ids,
+// outside the wedge layout (whose narrow ranges go through the 16-bit path).
+__device__ __forceinline__ bool cursor_list(int kind, uint32_t d, int layout) {
+  return kind == kKindBlock && d > kChunk && layout != TRIBIT_SCHED_WEDGE;
+}
+
+__device__ __forceinline__ uint32_t list_splits(int kind, uint32_t count, uint64_t w, bool cursor = false) {
+  if (cursor) {  // at most kChunk lists per task, about kCursorCap elements
+    uint64_t s = (w + kCursorCap - 1) / kCursorCap;
+    const uint64_t sl = (count + kChunk - 1) / kChunk;
+    if (sl > s) s = sl;
+    if (s < 1) s = 1;
+    if (s > count) s = count;
+    return (uint32_t)s;
+  }
   const uint64_t cap = kind_is_block(kind) ? kBlockCap : kWaveCap;
   uint64_t s = (w + cap - 1) / cap;
   // wave tasks stage 64 lists per round trip: hubs with many short lists
@@ -1171,8 +1293,8 @@ __device__ __forceinline__ uint32_t list_splits(int kind, uint32_t count, uint64
   return (uint32_t)s;
 }
 
-__device__ __forceinline__ uint32_t list_chunks(int kind, uint32_t d) {
-  return (kind_is_block(kind) && d > kChunk) ? (d + kChunk - 1) / kChunk : 1u;
+__device__ __forceinline__ uint32_t list_chunks(int kind, uint32_t d, bool cursor = false) {
+  return (!cursor && kind_is_block(kind) && d > kChunk) ? (d + kChunk - 1) / kChunk : 1u;
 }
 
 // Per owner: tasks per slot.
@@ -1188,7 +1310,8 @@ __device__ __forceinline__ Cnt8 owner_counts(const SchedView& sv, const PlanDev&
     if (!list_useful(sv, l, d, k)) continue;
     const int s = pd.table[bucket_of(d, w, k)];
     const int kind = pd.kind[s];
-    c.v[s] += list_splits(kind, k, w) * list_chunks(kind, d);
+    const bool cur = cursor_list(kind, d, sv.layout);
+    c.v[s] += list_splits(kind, k, w, cur) * list_chunks(kind, d, cur);
   }
   return c;
 }
@@ -1234,14 +1357,16 @@ __global__ __launch_bounds__(kSchedBlock) void k_sched_fill(SchedView sv, PlanDe
     const uint32_t bucket = (uint32_t)bucket_of(d, w, k);
     const int s = pd.table[bucket];
     const int kind = pd.kind[s];
-    const uint32_t sp = list_splits(kind, k, w), nch = list_chunks(kind, d);
+    const bool cur = cursor_list(kind, d, sv.layout);
+    const uint32_t sp = list_splits(kind, k, w, cur), nch = list_chunks(kind, d, cur);
     const uint32_t q = k / sp, r = k % sp;  // split i: q lists, one more for i < r
     uint4* out = tasks + base.v[s] + blockoff[blockIdx.x].v[s] + atomicAdd(&sc[s], sp * nch);
     uint32_t j = 0;
     for (uint32_t c = 0; c < nch; ++c)
       for (uint32_t i = 0; i < sp; ++i) {
         const uint32_t k0 = i * q + min(i, r);
-        out[j++] = make_uint4((uint32_t)x, b + k0, q + (i < r ? 1u : 0u), c | (bucket << kBucketShift));
+        out[j++] = make_uint4((uint32_t)x, b + k0, q + (i < r ? 1u : 0u),
+                              (cur ? kAllChunks : c) | (bucket << kBucketShift));
       }
   }
 }
