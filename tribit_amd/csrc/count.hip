@@ -47,7 +47,8 @@ constexpr uint32_t kChunkMask = (1u << kBucketShift) - 1;
 // lists per task, staged once), instead of one task per chunk whose lists are
 // each cut to the chunk by two binary searches.
 constexpr uint32_t kAllChunks = kChunkMask;
-constexpr uint64_t kCursorCap = 1u << 22;  // target streamed elements per cursor task
+constexpr uint64_t kCursorCap = 1u << 20;  // target streamed elements per cursor task (2^20: 62.8 ms for the s25 rank
+                                            // off-diagonal task, 2^22: 66.4, 2^24: 64.0)
 
 // ------------------------------------------------------------ LDS sets
 // N+(u) (or one id-range chunk of it) is held in LDS as a two-table cuckoo
